@@ -1,0 +1,189 @@
+/*
+ * pivtools_amd.h -- C ABI of the MI355X correlation hot path.
+ *
+ * Two layers, both plain C (no HIP or torch types in any signature):
+ *
+ *  1. Drop-in host-pointer entry points with the exact names, arguments,
+ *     ownership and error codes of the reference's ctypes-bound libraries
+ *     libbulkxcorr2d / libfusedwarp (built here as libbulkxcorr2d.so and
+ *     libfusedwarp.so).  Each call is synchronous: host -> HBM, kernels,
+ *     HBM -> host.  Reference declarations cited per function.
+ *
+ *  2. The device-pointer API (pivamd_*) underneath, exported by
+ *     libpivtools_amd.so: every pointer is device memory, work is queued on
+ *     the given stream (a hipStream_t passed as void*), nothing is copied.
+ *     The Python host layer and the benchmark drive this layer.
+ *
+ * Error codes of layer 1 are the reference's (pivtools_cli/lib/common.h:31-36):
+ *   0 none, 1 NOMEM, 2 NOPLAN_FWD, 4 NOPLAN_BWD (unsupported window size),
+ *   8 NOPLAN (selector misuse / device failure), 9 OUT_OF_BOUNDS.
+ * Layer 2 returns 0 on success, 1 bad argument, 2 unsupported size,
+ *   3 launch failure, 4 out of device memory, 5 runtime error.
+ */
+#ifndef PIVTOOLS_AMD_H
+#define PIVTOOLS_AMD_H
+
+#include <stdbool.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ======================================================================= */
+/* Layer 1: libbulkxcorr2d drop-in                                          */
+/* ======================================================================= */
+
+/* Replaces bulkxcorr2d, pivtools_cli/lib/PIV_2d_cross_correlate.h:18-21
+ * (ctypes binding pivtools_cli/piv/piv_backend/cpu_instantaneous.py:118-165).
+ * Outputs (N, nPeaks, nWy, nWx) at n*nPeaks*nWin + p*nWin + w; fPkRatio
+ * (N, nWy, nWx); masked or out-of-bounds windows are left untouched;
+ * fCorrelPlane_Out (nullable) receives the raw planes (N*nWin*h*w). */
+unsigned char bulkxcorr2d(const float *fImageA, const float *fImageB, const float *fMask,
+                          const int *nImageSize, int N_images, const float *fWinCtrsX,
+                          const float *fWinCtrsY, const int *nWindows, float *fWindowWeightA,
+                          bool bEnsemble, const float *fWindowWeightB, const int *nWindowSize,
+                          int nPeaks, int iPeakFinder, float *fPkLocX, float *fPkLocY,
+                          float *fPkHeight, float *fSx, float *fSy, float *fSxy, float *fPkRatio,
+                          float *fCorrelPlane_Out);
+
+/* Replaces bulkxcorr2d_accumulate, PIV_2d_cross_correlate.h:37-51.
+ * Adds (+=) the central nFitWindowSize crop of every pair's AB plane into
+ * fCorrelPlane_Sum (nWin * fh * fw); the caller zeroes it. */
+unsigned char bulkxcorr2d_accumulate(const float *fImageA_stack, const float *fImageB_stack,
+                                     const float *fMask, const int *nImageSize, int N_images,
+                                     const float *fWinCtrsX, const float *fWinCtrsY,
+                                     const int *nWindows, const float *fWindowWeightA,
+                                     const float *fWindowWeightB, const int *nWindowSize,
+                                     const int *nFitWindowSize, float *fCorrelPlane_Sum);
+
+/* Replaces bulkxcorr2d_accumulate_triple, PIV_2d_cross_correlate.h:66-86
+ * (ctypes binding unit-tests/test_ensemble_options.py:43-65). */
+unsigned char bulkxcorr2d_accumulate_triple(
+    const float *fImageA_stack, const float *fImageB_stack, const float *fMask,
+    const int *nImageSize, int N_images, const float *fWinCtrsX, const float *fWinCtrsY,
+    const int *nWindows, const float *fWindowWeightA_AB, const float *fWindowWeightB_AB,
+    const float *fAutoWeightA, const float *fAutoWeightB, const int *nWindowSize,
+    const int *nFitWindowSize, int bMeanSubtract, int bPerPairNorm, float *fCorrAB_Sum,
+    float *fCorrAA_Sum, float *fCorrBB_Sum);
+
+/* PIV_2d_cross_correlate.h:88-89 */
+float fminvec(const float *fVec, int n);
+float fmaxvec(const float *fVec, int n);
+
+/* PIV_2d_cross_correlate.h:97-99.  fft_s accumulates device seconds of the
+ * fused correlate-and-fit kernel; fit_s stays 0 (the fit is fused in). */
+void bulkxcorr2d_set_timing_enabled(int on);
+void bulkxcorr2d_reset_timing(void);
+void bulkxcorr2d_get_timing(double *fft_s, double *fit_s);
+
+/* PIV_2d_cross_correlate.h:107-109.  0 = scalar fitter flavour (libm
+ * exponential), 1 = batch flavour (Cephes exponential); both run on the
+ * device, so batch is always available. */
+int bulkxcorr2d_set_peakfit_impl(int impl);
+int bulkxcorr2d_get_peakfit_impl(void);
+int bulkxcorr2d_peakfit_batch_available(void);
+
+/* peak_locate_lm.h:39-51.  One plane (any N[0] x N[1] >= 8 x 8). */
+void lsqpeaklocate_lm(const float *xcorr, const int *N, float *peak_loc, int nPeaks,
+                      int iFitType, float *std_dev, const float *fPlaneWeight);
+float peak_detectability(const float *xcorr, const int *N);
+/* Returns 1 when a HIP device is present (the loader's guard,
+ * pivtools_cli/piv/piv_backend/lib_guard.py:6-26). */
+int pivtools_cpu_supported(void);
+
+/* peak_locate_lm_batch.h:39-51.  Lane width W = peakfit_batch_lanes(). */
+void lsqpeaklocate_lm_batch(const float *planes, int L_real, const int *N, int iFitType,
+                            float *peak_loc, float *std_dev, const float *fPlaneWeight);
+int peakfit_batch_available(void);
+int peakfit_batch_lanes(void);
+
+/* xcorr.h:26-27.  convolve: centre crop of the 2N zero-padded correlation
+ * (any size).  xcorr: shift(IFFT(FFT(a) .* conj(FFT(b))))/numel for built
+ * sizes up to 128 (192/256 return 4). */
+unsigned convolve(const float *w1, const float *w2, float *conv, const int *N);
+unsigned xcorr(const float *w1, const float *w2, float *corr, const int *N);
+/* BUILD-SIMD.md verify step: windows per batched call (here, per workgroup
+ * for a 32x32 window). */
+int codelet_lanes(void);
+
+/* ======================================================================= */
+/* Layer 1: libfusedwarp drop-in                                            */
+/* ======================================================================= */
+
+/* fused_warp.h:21-22.  Both values run the same device kernel. */
+void fused_warp_set_impl(int impl);
+int fused_warp_get_impl(void);
+
+/* fused_warp.h:55-66 */
+int fused_symmetric_warp(const float *img_a, const float *img_b, float *out_a, float *out_b,
+                         const float *pred_dy, const float *pred_dx, int H, int W, int nPY,
+                         int nPX, const float *ctrs_y, const float *ctrs_x, int interp_mode,
+                         int round_shifts);
+
+/* fused_warp.h:78-93 */
+int fused_symmetric_warp_batch(const float *imgs_a, const float *imgs_b, float *outs_a,
+                               float *outs_b, const float *pred_dy, const float *pred_dx, int N,
+                               int H, int W, int nPY, int nPX, const float *ctrs_y,
+                               const float *ctrs_x, int interp_mode, int shared_predictor,
+                               int round_shifts);
+
+/* ======================================================================= */
+/* Layer 2: device-pointer API (libpivtools_amd.so)                          */
+/* ======================================================================= */
+
+/* 1 if an NY x NX window has a correlation kernel. */
+int pivamd_size_supported(int ny, int nx);
+
+/* d_out = centre crop of the 2N zero-padded correlation of a and b (FP64). */
+int pivamd_convolve(const float *d_a, const float *d_b, int ny, int nx, float *d_out, void *stream);
+
+/* d_out[ny*nx] = numel / (wB (*) wB), the plane weight of bulkxcorr2d. */
+int pivamd_plane_weight(const float *d_wB, int ny, int nx, float *d_out, void *stream);
+
+/* One bulkxcorr2d over device buffers.  d_mask may be NULL (no mask);
+ * d_planes may be NULL.  impl: 0 scalar, 1 batch fitter flavour.  d_work is
+ * caller-owned device scratch of pivamd_xcorr_workspace() bytes. */
+int pivamd_xcorr(const float *d_imgA, const float *d_imgB, const float *d_mask, int H, int W,
+                 int n_images, const float *d_ctrX, const float *d_ctrY, int nWy, int nWx,
+                 const float *d_wA, const float *d_wB, const float *d_planeWeight, int ensemble,
+                 int winY, int winX, int nPeaks, int finder, int impl, float *d_pkX, float *d_pkY,
+                 float *d_pkH, float *d_sx, float *d_sy, float *d_sxy, float *d_ratio,
+                 float *d_planes, void *d_work, unsigned long work_bytes, void *stream);
+
+/* Device workspace bytes pivamd_xcorr needs (0 in ensemble mode). */
+unsigned long pivamd_xcorr_workspace(int n_images, int nWy, int nWx, int winY, int winX, int nPeaks,
+                                     int ensemble);
+
+/* One fused_symmetric_warp_batch over device buffers. */
+int pivamd_warp(const float *d_imgA, const float *d_imgB, float *d_outA, float *d_outB,
+                const float *d_predDy, const float *d_predDx, int N, int H, int W, int nPY,
+                int nPX, const float *d_ctrY, const float *d_ctrX, int interp_mode,
+                int shared_predictor, int round_shifts, void *stream);
+
+/* Ensemble sums over device buffers.  triple = 1: bulkxcorr2d_accumulate_triple
+ * (AB, AA, BB with optional mean removal and per-pair normalisation; sumW*
+ * are the weight sums of the mean removal); triple = 0:
+ * bulkxcorr2d_accumulate (AB only, weights d_wAab / d_wBab).  Sums are
+ * added (+=) to the fitY x fitX central crops in d_out*; d_work is
+ * pivamd_ensemble_workspace() bytes of device scratch (FP64 spectra). */
+unsigned long pivamd_ensemble_workspace(int nWy, int nWx, int winY, int winX);
+int pivamd_accumulate(const float *d_A, const float *d_B, const float *d_mask, int H, int W,
+                      int n_images, const float *d_ctrX, const float *d_ctrY, int nWy, int nWx,
+                      const float *d_wAab, const float *d_wBab, const float *d_wAA,
+                      const float *d_wBB, int winY, int winX, int fitY, int fitX, int meanSub,
+                      int perPairNorm, int triple, float sumWBab, float sumWAA, float sumWBB,
+                      float *d_outAB, float *d_outAA, float *d_outBB, void *d_work,
+                      unsigned long work_bytes, void *stream);
+
+/* Peak search + fit + detectability on n planes of ny x nx in device
+ * memory (window-major); outputs peak_loc[3*n] (row, col, height) and
+ * std_dev[3*n] per plane, component-major with stride n, and ratio[n]. */
+int pivamd_peakfit(const float *d_planes, int n, int ny, int nx, int nPeaks, int finder, int impl,
+                   const float *d_planeWeight, float *d_loc, float *d_std, float *d_ratio,
+                   void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PIVTOOLS_AMD_H */

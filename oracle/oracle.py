@@ -1,0 +1,243 @@
+"""CPU oracle for the PIVtools correlation hot path -- TEST INFRASTRUCTURE ONLY.
+
+This module is the checker used by ``tests/``, ``__graft_entry__.smoke()``
+and the CPU-baseline leg of ``bench.py``.  The shipped library never imports
+it.  Two flavours live here:
+
+* ``liboracle.so`` (``piv_oracle.c``): a C restatement of the reference C
+  path, same arguments and outputs as the reference's ctypes bindings
+  (``pivtools_cli/piv/piv_backend/cpu_instantaneous.py:68-165``).
+* FP64 NumPy functions (``*_fp64``): the same mathematics in double precision,
+  used to state the error of every result against an exact-ish answer.
+
+Pins (which check fixes each function) are listed in ``piv_oracle.c``'s
+header and in the tests that import this module.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+def build() -> None:
+    """Compile liboracle.so (and oracle/_ref when the reference is mounted)."""
+    import subprocess
+
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib() -> ctypes.CDLL:
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(HERE, "liboracle.so")
+        if not os.path.isfile(path):
+            build()
+        L = ctypes.CDLL(path)
+        L.orc_bulkxcorr2d.restype = ctypes.c_ubyte
+        L.orc_bulkxcorr2d.argtypes = [
+            _f32p, _f32p, _f32p, _i32p, ctypes.c_int, _f32p, _f32p, _i32p, _f32p,
+            ctypes.c_int, _f32p, _i32p, ctypes.c_int, ctypes.c_int,
+            _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, ctypes.c_void_p,
+        ]
+        L.orc_lsqpeaklocate_lm.restype = None
+        L.orc_lsqpeaklocate_lm.argtypes = [
+            _f32p, _i32p, _f32p, ctypes.c_int, ctypes.c_int, _f32p, ctypes.c_void_p,
+        ]
+        L.orc_peak_detectability.restype = ctypes.c_float
+        L.orc_peak_detectability.argtypes = [_f32p, _i32p]
+        L.orc_convolve.restype = ctypes.c_uint
+        L.orc_convolve.argtypes = [_f32p, _f32p, _f32p, _i32p]
+        L.orc_fused_symmetric_warp_batch.restype = ctypes.c_int
+        L.orc_fused_symmetric_warp_batch.argtypes = [
+            _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, ctypes.c_int, ctypes.c_int,
+            ctypes.c_int, ctypes.c_int, ctypes.c_int, _f32p, _f32p, ctypes.c_int,
+            ctypes.c_int, ctypes.c_int,
+        ]
+        L.orc_bulkxcorr2d_accumulate_triple.restype = ctypes.c_ubyte
+        L.orc_bulkxcorr2d_accumulate_triple.argtypes = [
+            _f32p, _f32p, _f32p, _i32p, ctypes.c_int, _f32p, _f32p, _i32p,
+            _f32p, _f32p, _f32p, _f32p, _i32p, ctypes.c_void_p, ctypes.c_int,
+            ctypes.c_int, _f32p, _f32p, _f32p,
+        ]
+        L.orc_bulkxcorr2d_accumulate.restype = ctypes.c_ubyte
+        L.orc_bulkxcorr2d_accumulate.argtypes = [
+            _f32p, _f32p, _f32p, _i32p, ctypes.c_int, _f32p, _f32p, _i32p,
+            _f32p, _f32p, _i32p, ctypes.c_void_p, _f32p,
+        ]
+        L.orc_max_threads.restype = ctypes.c_int
+        _LIB = L
+    return _LIB
+
+
+# ---------------------------------------------------------------------------
+# C restatement wrappers (same argument meaning as the reference bindings)
+# ---------------------------------------------------------------------------
+
+
+def bulkxcorr2d(img_a, img_b, mask, ctrs_x, ctrs_y, win, weight_a, weight_b,
+                n_peaks=1, peak_finder=3, ensemble=False, want_planes=False):
+    """Run the oracle ``bulkxcorr2d``; returns a dict of the output arrays."""
+    img_a = np.ascontiguousarray(img_a, np.float32)
+    img_b = np.ascontiguousarray(img_b, np.float32)
+    if img_a.ndim == 2:
+        img_a, img_b = img_a[None], img_b[None]
+    N, H, W = img_a.shape
+    ny, nx = len(ctrs_y), len(ctrs_x)
+    out_shape = (N, n_peaks, ny, nx)
+    o = {k: np.zeros(out_shape, np.float32) for k in ("pk_x", "pk_y", "pk_h", "sx", "sy", "sxy")}
+    o["ratio"] = np.zeros((N, ny, nx), np.float32)
+    planes = np.zeros(N * ny * nx * win[0] * win[1], np.float32) if want_planes else None
+    err = lib().orc_bulkxcorr2d(
+        img_a, img_b, np.ascontiguousarray(mask, np.float32), np.array([H, W], np.int32), N,
+        np.ascontiguousarray(ctrs_x, np.float32), np.ascontiguousarray(ctrs_y, np.float32),
+        np.array([ny, nx], np.int32), np.ascontiguousarray(weight_a, np.float32), int(ensemble),
+        np.ascontiguousarray(weight_b, np.float32), np.array(win, np.int32), n_peaks, peak_finder,
+        o["pk_x"], o["pk_y"], o["pk_h"], o["sx"], o["sy"], o["sxy"], o["ratio"],
+        planes.ctypes.data if planes is not None else None,
+    )
+    o["err"] = int(err)
+    if planes is not None:
+        o["planes"] = planes.reshape(N, ny, nx, win[0], win[1])
+    return o
+
+
+def lsqpeaklocate_lm(plane, n_peaks=1, fit_type=3, weight=None):
+    plane = np.ascontiguousarray(plane, np.float32)
+    N = np.array(plane.shape, np.int32)
+    loc = np.zeros(3 * n_peaks, np.float32)
+    std = np.zeros(3 * n_peaks, np.float32)
+    w = None if weight is None else np.ascontiguousarray(weight, np.float32)
+    lib().orc_lsqpeaklocate_lm(plane, N, loc, n_peaks, fit_type, std,
+                               None if w is None else w.ctypes.data)
+    return loc, std
+
+
+def peak_detectability(plane):
+    plane = np.ascontiguousarray(plane, np.float32)
+    return float(lib().orc_peak_detectability(plane, np.array(plane.shape, np.int32)))
+
+
+def plane_weight(weight_b):
+    """numel / (wB (*) wB) on the central crop (xcorr.c:22-89)."""
+    wb = np.ascontiguousarray(weight_b, np.float32)
+    conv = np.zeros_like(wb)
+    err = lib().orc_convolve(wb, wb, conv, np.array(wb.shape, np.int32))
+    if err:
+        raise ValueError(f"convolve error {err}")
+    return (np.float32(wb.size) / conv).astype(np.float32)
+
+
+def fused_warp_batch(imgs_a, imgs_b, pred_dy, pred_dx, ctrs_y, ctrs_x, interp_mode=0,
+                     shared_predictor=False, round_shifts=False):
+    imgs_a = np.ascontiguousarray(imgs_a, np.float32)
+    imgs_b = np.ascontiguousarray(imgs_b, np.float32)
+    N, H, W = imgs_a.shape
+    pred_dy = np.ascontiguousarray(pred_dy, np.float32)
+    pred_dx = np.ascontiguousarray(pred_dx, np.float32)
+    nPY, nPX = pred_dy.shape[-2:]
+    oa = np.zeros_like(imgs_a)
+    ob = np.zeros_like(imgs_b)
+    r = lib().orc_fused_symmetric_warp_batch(
+        imgs_a, imgs_b, oa, ob, pred_dy, pred_dx, N, H, W, nPY, nPX,
+        np.ascontiguousarray(ctrs_y, np.float32), np.ascontiguousarray(ctrs_x, np.float32),
+        interp_mode, int(shared_predictor), int(round_shifts))
+    if r != 0:
+        raise RuntimeError(f"oracle warp failed ({r})")
+    return oa, ob
+
+
+def accumulate_triple(imgs_a, imgs_b, mask, ctrs_x, ctrs_y, win, w_a_ab, w_b_ab, w_auto_a,
+                      w_auto_b, fit_win=None, mean_subtract=False, per_pair_norm=False):
+    imgs_a = np.ascontiguousarray(imgs_a, np.float32)
+    imgs_b = np.ascontiguousarray(imgs_b, np.float32)
+    N, H, W = imgs_a.shape
+    ny, nx = len(ctrs_y), len(ctrs_x)
+    oh, ow = fit_win if fit_win is not None else win
+    outs = [np.zeros(ny * nx * oh * ow, np.float32) for _ in range(3)]
+    fw = None if fit_win is None else np.array(fit_win, np.int32)
+    err = lib().orc_bulkxcorr2d_accumulate_triple(
+        imgs_a, imgs_b, np.ascontiguousarray(mask, np.float32), np.array([H, W], np.int32), N,
+        np.ascontiguousarray(ctrs_x, np.float32), np.ascontiguousarray(ctrs_y, np.float32),
+        np.array([ny, nx], np.int32),
+        *[np.ascontiguousarray(x, np.float32) for x in (w_a_ab, w_b_ab, w_auto_a, w_auto_b)],
+        np.array(win, np.int32), None if fw is None else fw.ctypes.data,
+        int(mean_subtract), int(per_pair_norm), *outs)
+    if err:
+        raise RuntimeError(f"oracle accumulate_triple error {err}")
+    return [o.reshape(ny, nx, oh, ow) for o in outs]
+
+
+# ---------------------------------------------------------------------------
+# FP64 NumPy oracle
+# ---------------------------------------------------------------------------
+
+
+def window_origin(ctr, n):
+    """row_min/col_min = floor(ctr - (n-1)/2 + 0.5) (PIV_2d_cross_correlate.c:299-300)."""
+    return int(np.floor(np.float64(np.float32(ctr)) - (n - 1.0) / 2.0 + 0.5))
+
+
+def xcorr_plane_fp64(win_a, win_b):
+    """shift(ifft2(fft2(B) * conj(fft2(A)))) in FP64: zero lag at [H/2][W/2]."""
+    fa = np.fft.fft2(np.asarray(win_a, np.float64))
+    fb = np.fft.fft2(np.asarray(win_b, np.float64))
+    return np.fft.fftshift(np.real(np.fft.ifft2(fb * np.conj(fa))))
+
+
+def xcorr_plane_bruteforce(win_a, win_b):
+    """Direct circular correlation ref[m,n] = sum B[(i+m)%H,(j+n)%W] A[i,j], shifted.
+
+    The brute-force FP64 definition the reference's own gate uses
+    (pivtools_cli/lib/test_codelet_gate.c:35-51).  O(N^4): small windows only.
+    """
+    a = np.asarray(win_a, np.float64)
+    b = np.asarray(win_b, np.float64)
+    H, W = a.shape
+    out = np.zeros((H, W))
+    for m in range(H):
+        for n in range(W):
+            out[m, n] = np.sum(np.roll(np.roll(b, -m, 0), -n, 1) * a)
+    return np.fft.fftshift(out)
+
+
+def windows_fp64(img_a, img_b, ctrs_x, ctrs_y, win, wa, wb, ensemble=False):
+    """Extract, weight and mean-subtract every window in FP64 (instantaneous rule)."""
+    H, W = img_a.shape
+    h, w = win
+    out_a, out_b, ok = [], [], []
+    for jj, cy in enumerate(ctrs_y):
+        r0 = window_origin(cy, h)
+        for ii, cx in enumerate(ctrs_x):
+            c0 = window_origin(cx, w)
+            valid = r0 >= 0 and c0 >= 0 and r0 + h <= H and c0 + w <= W
+            ok.append(valid)
+            if not valid:
+                out_a.append(np.zeros((h, w)))
+                out_b.append(np.zeros((h, w)))
+                continue
+            a = img_a[r0:r0 + h, c0:c0 + w].astype(np.float64) * wa
+            b = img_b[r0:r0 + h, c0:c0 + w].astype(np.float64) * wb
+            if not ensemble:
+                a = a - a.mean()
+                b = b - b.mean()
+            out_a.append(a)
+            out_b.append(b)
+    return np.array(out_a), np.array(out_b), np.array(ok)
+
+
+def planes_fp64(img_a, img_b, ctrs_x, ctrs_y, win, wa, wb, ensemble=False):
+    """All window planes of one pair in FP64, shape (ny*nx, h, w)."""
+    a, b, ok = windows_fp64(img_a, img_b, ctrs_x, ctrs_y, win, wa, wb, ensemble)
+    fa = np.fft.fft2(a)
+    fb = np.fft.fft2(b)
+    planes = np.fft.fftshift(np.real(np.fft.ifft2(fb * np.conj(fa))), axes=(-2, -1))
+    return planes, ok

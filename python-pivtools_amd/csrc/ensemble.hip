@@ -1,0 +1,338 @@
+// ensemble.hip -- ensemble correlation accumulation (gfx950).
+//
+// Replaces bulkxcorr2d_accumulate_triple and bulkxcorr2d_accumulate
+// (pivtools_cli/lib/PIV_2d_cross_correlate.c:432-484, 502-833).  For every
+// window and every image pair it forms
+//     AB = xcorr(slot0 = (B - mB) wB_AB, slot1 = (A - mA) wA_AB)
+//     AA = power((A - mA) wAutoA),   BB = power((B - mB) wAutoB)
+// (optional weighted means, optional per-pair normalisation by the zero-lag
+// auto energies, pairs with a non-positive energy skipped) and adds the
+// central crop of each plane to the caller's sums.
+//
+// By linearity of the inverse FFT and of the crop, the sums are taken in
+// the frequency domain: each window keeps the half-spectrum sums of AB, AA
+// and BB in FP64 (device workspace, owned lane by lane, no atomics), and is
+// inverse-transformed once after the last pair.  Per pair and window the
+// work is two complex 2D FFTs: z1 = pAB_B + i pAB_A gives F(pAB_B) and
+// F(pAB_A), z2 = pAA + i pBB gives F(pAA) and F(pBB).
+#include <hip/hip_runtime.h>
+
+#include "fft_reg.h"
+#include "pivamd_dev.h"
+
+namespace pivamd {
+
+struct EnsArgs {
+    const float *imgA, *imgB, *mask, *ctrX, *ctrY, *wAab, *wBab, *wAA, *wBB;
+    float *outAB, *outAA, *outBB;
+    double *acc;            // nWin * 4 * NY * (NX/2+1)
+    int H, W, nImages, nWy, nWx, fitY, fitX, meanSub, perPairNorm, triple, nblocks;
+    float sumWBab, sumWAA, sumWBB;
+};
+
+constexpr int kEnsMaxLds = 150 * 1024;
+
+template <int N>
+struct EnsCfg {
+    static constexpr int T = N;
+    static constexpr int PX = N + 1;
+    static constexpr int tile = N * PX;
+    static constexpr int HW = N / 2 + 1;
+    static constexpr int lds_win = 2 * tile * 4 + 6 * T * 4;
+    static constexpr int wpb_thr = (256 / T) > 0 ? (256 / T) : 1;
+    static constexpr int wpb_lds = kEnsMaxLds / lds_win;
+    static constexpr int WPB = wpb_thr < wpb_lds ? wpb_thr : wpb_lds;
+    static constexpr int BLOCK = WPB * T;
+};
+
+// Sum over the T lanes of one window via LDS, every lane summing in lane
+// order (identical bits in every lane).
+__device__ __forceinline__ void ens_sum3(float &a, float &b, float &c, float *ra, float *rb, float *rc,
+                                         int t, int T)
+{
+    ra[t] = a;
+    rb[t] = b;
+    rc[t] = c;
+    __syncthreads();
+    float sa = 0.0f, sb = 0.0f, sc = 0.0f;
+    for (int k = 0; k < T; ++k) {
+        sa += ra[k];
+        sb += rb[k];
+        sc += rc[k];
+    }
+    __syncthreads();
+    a = sa;
+    b = sb;
+    c = sc;
+}
+
+// Forward 2D FFT of the packed tile (column pass from registers, row pass
+// through LDS).  On return the tile holds Z[ky][kx].
+template <int N>
+__device__ __forceinline__ void fwd2d(float (&zr)[N], float (&zi)[N], float *Sr, float *Si, int t, bool on)
+{
+    constexpr int PX = N + 1;
+    if (on) {
+        fft_reg<N, false>(zr, zi);
+#pragma unroll
+        for (int y = 0; y < N; ++y) {
+            Sr[y * PX + t] = zr[y];
+            Si[y * PX + t] = zi[y];
+        }
+    }
+    __syncthreads();
+    if (on) {
+        float xr[N], xi[N];
+#pragma unroll
+        for (int x = 0; x < N; ++x) {
+            xr[x] = Sr[t * PX + x];
+            xi[x] = Si[t * PX + x];
+        }
+        fft_reg<N, false>(xr, xi);
+#pragma unroll
+        for (int x = 0; x < N; ++x) {
+            Sr[t * PX + x] = xr[x];
+            Si[t * PX + x] = xi[x];
+        }
+    }
+    __syncthreads();
+}
+
+template <int N>
+__global__ void __launch_bounds__((EnsCfg<N>::BLOCK)) ens_kernel(EnsArgs g)
+{
+    using C = EnsCfg<N>;
+    constexpr int T = C::T, PX = C::PX, WPB = C::WPB, HW = C::HW;
+    constexpr int numel = N * N;
+    __shared__ float s_re[WPB * C::tile];
+    __shared__ float s_im[WPB * C::tile];
+    __shared__ float s_red[6][WPB * T];
+
+    const int per = (int)gridDim.x >> 3;
+    const int bb = ((int)blockIdx.x & 7) * per + ((int)blockIdx.x >> 3);
+    if (bb >= g.nblocks) return;
+    const int wloc = (int)threadIdx.x / T;
+    const int t = (int)threadIdx.x - wloc * T;
+    float *Sr = s_re + wloc * C::tile;
+    float *Si = s_im + wloc * C::tile;
+    float *r0 = &s_red[0][wloc * T], *r1 = &s_red[1][wloc * T], *r2 = &s_red[2][wloc * T];
+
+    const int nWin = g.nWy * g.nWx;
+    const int w = bb * WPB + wloc;
+    bool valid = w < nWin;
+    int row0 = 0, col0 = 0;
+    if (valid) {
+        const int ii = w % g.nWx, jj = w / g.nWx;
+        if (g.mask && g.mask[w] == 1.0f) valid = false;
+        // float arithmetic, as PIV_2d_cross_correlate.c:555-556 / 743-744
+        row0 = (int)floorf(g.ctrY[jj] - ((float)N - 1.0f) / 2.0f + 0.5f);
+        col0 = (int)floorf(g.ctrX[ii] - ((float)N - 1.0f) / 2.0f + 0.5f);
+        if (row0 < 0 || col0 < 0 || row0 + N > g.H || col0 + N > g.W) valid = false;
+    }
+    double *acc = g.acc + (size_t)(valid ? w : 0) * 4 * N * HW;
+    const bool specLane = valid && t < HW;
+    if (specLane) {
+        for (int k = 0; k < 4 * N; ++k) acc[(size_t)k * HW + t] = 0.0;
+    }
+
+    const size_t plane = (size_t)g.H * g.W;
+    for (int n = 0; n < g.nImages; ++n) {
+        float ra[N], rb[N];
+        float sAB_B = 0.0f, sAA = 0.0f, sBB = 0.0f;
+        if (valid) {
+            const float *pa = g.imgA + (size_t)n * plane + (size_t)row0 * g.W + col0 + t;
+            const float *pb = g.imgB + (size_t)n * plane + (size_t)row0 * g.W + col0 + t;
+#pragma unroll
+            for (int y = 0; y < N; ++y) {
+                ra[y] = pa[(size_t)y * g.W];
+                rb[y] = pb[(size_t)y * g.W];
+                if (g.meanSub) {
+                    sAB_B += rb[y] * g.wBab[y * N + t];
+                    sAA += ra[y] * g.wAA[y * N + t];
+                    sBB += rb[y] * g.wBB[y * N + t];
+                }
+            }
+        } else {
+#pragma unroll
+            for (int y = 0; y < N; ++y) { ra[y] = 0.0f; rb[y] = 0.0f; }
+        }
+        float mB = 0.0f, mAA = 0.0f, mBB = 0.0f;
+        if (g.meanSub) {
+            ens_sum3(sAB_B, sAA, sBB, r0, r1, r2, t, T);
+            mB = (g.sumWBab > 0.0f) ? sAB_B / g.sumWBab : 0.0f;
+            mAA = (g.sumWAA > 0.0f) ? sAA / g.sumWAA : 0.0f;
+            mBB = (g.sumWBB > 0.0f) ? sBB / g.sumWBB : 0.0f;
+        }
+        float z1r[N], z1i[N], z2r[N], z2i[N];
+        float eAA = 0.0f, eBB = 0.0f, dummy = 0.0f;
+#pragma unroll
+        for (int y = 0; y < N; ++y) {
+            z1r[y] = (rb[y] - mB) * g.wBab[y * N + t];
+            z1i[y] = (ra[y] - mAA) * g.wAab[y * N + t];
+            z2r[y] = (ra[y] - mAA) * g.wAA[y * N + t];
+            z2i[y] = (rb[y] - mBB) * g.wBB[y * N + t];
+            eAA += z2r[y] * z2r[y];
+            eBB += z2i[y] * z2i[y];
+        }
+        double sAB = 1.0, sA = 1.0, sB = 1.0;
+        bool use = valid;
+        if (g.perPairNorm) {
+            ens_sum3(eAA, eBB, dummy, r0, r1, r2, t, T);
+            if (eAA <= 0.0f || eBB <= 0.0f) use = false;
+            sAB = (double)(1.0f / sqrtf(eAA * eBB));
+            sA = (double)(1.0f / eAA);
+            sB = (double)(1.0f / eBB);
+        }
+        // ---- AB: z1 = pAB_B + i pAB_A, C = F(pAB_B) conj(F(pAB_A)) ----
+        fwd2d<N>(z1r, z1i, Sr, Si, t, use);
+        if (use && t < HW) {
+            const int mx = (N - t) % N;
+#pragma unroll 4
+            for (int ky = 0; ky < N; ++ky) {
+                const int my = (N - ky) % N;
+                const float zR = Sr[ky * PX + t], zI = Si[ky * PX + t];
+                const float mR = Sr[my * PX + mx], mI = Si[my * PX + mx];
+                const float xr_ = 0.5f * (zR + mR), xi_ = 0.5f * (zI - mI);      // F(pAB_B)
+                const float yr_ = 0.5f * (zI + mI), yi_ = -0.5f * (zR - mR);     // F(pAB_A)
+                const double cr = (double)(xr_ * yr_ + xi_ * yi_);
+                const double ci = (double)(xi_ * yr_ - xr_ * yi_);
+                acc[(size_t)(0 * N + ky) * HW + t] += sAB * cr;
+                acc[(size_t)(1 * N + ky) * HW + t] += sAB * ci;
+            }
+        }
+        if (g.triple) {
+            __syncthreads();
+            // ---- AA, BB: z2 = pAA + i pBB -------------------------------
+            fwd2d<N>(z2r, z2i, Sr, Si, t, use);
+            if (use && t < HW) {
+                const int mx = (N - t) % N;
+#pragma unroll 4
+                for (int ky = 0; ky < N; ++ky) {
+                    const int my = (N - ky) % N;
+                    const float zR = Sr[ky * PX + t], zI = Si[ky * PX + t];
+                    const float mR = Sr[my * PX + mx], mI = Si[my * PX + mx];
+                    const float ar = 0.5f * (zR + mR), ai = 0.5f * (zI - mI);
+                    const float br = 0.5f * (zI + mI), bi = -0.5f * (zR - mR);
+                    acc[(size_t)(2 * N + ky) * HW + t] += sA * (double)(ar * ar + ai * ai);
+                    acc[(size_t)(3 * N + ky) * HW + t] += sB * (double)(br * br + bi * bi);
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    // ---- readout: one c2r per accumulated spectrum, shift, crop, += -------
+    const int oh = g.fitY, ow = g.fitX;
+    const int sy = (N - oh) / 2, sx = (N - ow) / 2;
+    for (int which = 0; which < (g.triple ? 3 : 1); ++which) {
+        if (valid && t < HW) {
+            float cr[N], ci[N];
+#pragma unroll
+            for (int ky = 0; ky < N; ++ky) {
+                if (which == 0) {
+                    cr[ky] = (float)acc[(size_t)(0 * N + ky) * HW + t];
+                    ci[ky] = (float)acc[(size_t)(1 * N + ky) * HW + t];
+                } else {
+                    cr[ky] = (float)acc[(size_t)((which + 1) * N + ky) * HW + t];
+                    ci[ky] = 0.0f;
+                }
+            }
+            fft_reg<N, true>(cr, ci);
+#pragma unroll
+            for (int y = 0; y < N; ++y) {
+                Sr[y * PX + t] = cr[y];
+                Si[y * PX + t] = ci[y];
+            }
+        }
+        __syncthreads();
+        if (valid && t < N / 2) {
+            const int ya = t, yb = t + N / 2;
+            float xr[N], xi[N];
+#pragma unroll
+            for (int k = 0; k <= N / 2; ++k) {
+                float g1r = Sr[ya * PX + k], g1i = Si[ya * PX + k];
+                float g2r = Sr[yb * PX + k], g2i = Si[yb * PX + k];
+                if (k == 0 || k == N / 2) { g1i = 0.0f; g2i = 0.0f; }
+                xr[k] = g1r - g2i;
+                xi[k] = g1i + g2r;
+                if (k != 0 && k != N / 2) {
+                    xr[N - k] = g1r + g2i;
+                    xi[N - k] = -g1i + g2r;
+                }
+            }
+            fft_reg<N, true>(xr, xi);
+            const float mul = 1.0f / (float)numel;
+            float *out = which == 0 ? g.outAB : (which == 1 ? g.outAA : g.outBB);
+            out += (size_t)w * oh * ow;
+            // plane row (y + N/2) % N holds c[y]; crop rows sy..sy+oh-1
+#pragma unroll
+            for (int x = 0; x < N; ++x) {
+                const int xs = (x + N / 2) % N;
+                if (xs >= sx && xs < sx + ow) {
+                    if (yb >= sy && yb < sy + oh) out[(yb - sy) * ow + (xs - sx)] += xr[x] * mul;
+                    if (ya >= sy && ya < sy + oh) out[(ya - sy) * ow + (xs - sx)] += xi[x] * mul;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <int N>
+static int launch_ens(EnsArgs g, hipStream_t st)
+{
+    using C = EnsCfg<N>;
+    const int nWin = g.nWy * g.nWx;
+    const int nb = (nWin + C::WPB - 1) / C::WPB;
+    if (nb <= 0) return 0;
+    g.nblocks = nb;
+    const int grid = (nb + 7) / 8 * 8;
+    hipLaunchKernelGGL((ens_kernel<N>), dim3(grid), dim3(C::BLOCK), 0, st, g);
+    return hipGetLastError() == hipSuccess ? 0 : PIVAMD_ERR_LAUNCH;
+}
+
+}  // namespace pivamd
+
+using namespace pivamd;
+
+extern "C" size_t pivamd_ensemble_workspace(int nWy, int nWx, int winY, int winX)
+{
+    return (size_t)nWy * nWx * 4 * winY * (winX / 2 + 1) * sizeof(double);
+}
+
+// triple = 1: AB, AA, BB (accumulate_triple); triple = 0: AB only with
+// weights (wA, wB) and no mean removal (accumulate).  Sums are added to the
+// caller's output planes; sumW* are the weight sums of the mean removal.
+extern "C" int pivamd_accumulate(const float *d_A, const float *d_B, const float *d_mask, int H, int W,
+                                 int n_images, const float *d_ctrX, const float *d_ctrY, int nWy, int nWx,
+                                 const float *d_wAab, const float *d_wBab, const float *d_wAA,
+                                 const float *d_wBB, int winY, int winX, int fitY, int fitX, int meanSub,
+                                 int perPairNorm, int triple, float sumWBab, float sumWAA, float sumWBB,
+                                 float *d_outAB, float *d_outAA, float *d_outBB, void *d_work,
+                                 size_t work_bytes, hipStream_t st)
+{
+    if (winY != winX) return PIVAMD_ERR_SIZE;
+    if (n_images < 0 || nWy < 0 || nWx < 0 || H <= 0 || W <= 0) return PIVAMD_ERR_ARG;
+    if (fitY <= 0 || fitX <= 0 || fitY > winY || fitX > winX) return PIVAMD_ERR_ARG;
+    if (work_bytes < pivamd_ensemble_workspace(nWy, nWx, winY, winX) || !d_work) return PIVAMD_ERR_ARG;
+    if (!d_outAB || (triple && (!d_outAA || !d_outBB))) return PIVAMD_ERR_ARG;
+    EnsArgs g;
+    g.imgA = d_A; g.imgB = d_B; g.mask = d_mask; g.ctrX = d_ctrX; g.ctrY = d_ctrY;
+    g.wAab = d_wAab; g.wBab = d_wBab; g.wAA = triple ? d_wAA : d_wAab; g.wBB = triple ? d_wBB : d_wBab;
+    g.outAB = d_outAB; g.outAA = d_outAA; g.outBB = d_outBB; g.acc = (double *)d_work;
+    g.H = H; g.W = W; g.nImages = n_images; g.nWy = nWy; g.nWx = nWx; g.fitY = fitY; g.fitX = fitX;
+    g.meanSub = triple ? meanSub : 0; g.perPairNorm = triple ? perPairNorm : 0; g.triple = triple;
+    g.sumWBab = sumWBab; g.sumWAA = sumWAA; g.sumWBB = sumWBB; g.nblocks = 0;
+    switch (winY) {
+    case 8: return launch_ens<8>(g, st);
+    case 12: return launch_ens<12>(g, st);
+    case 16: return launch_ens<16>(g, st);
+    case 24: return launch_ens<24>(g, st);
+    case 32: return launch_ens<32>(g, st);
+    case 48: return launch_ens<48>(g, st);
+    case 64: return launch_ens<64>(g, st);
+    case 96: return launch_ens<96>(g, st);
+    case 128: return launch_ens<128>(g, st);
+    default: return PIVAMD_ERR_SIZE;
+    }
+}

@@ -1,0 +1,43 @@
+// pivamd_dev.h -- internal declarations shared by the HIP translation units.
+// The public C ABI (host-pointer drop-ins and the device-pointer API) is
+// declared in include/pivtools_amd.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+// Error codes of the device API (the host-pointer drop-ins translate them to
+// the reference's codes, pivtools_cli/lib/common.h:31-36).
+#define PIVAMD_OK 0
+#define PIVAMD_ERR_ARG 1
+#define PIVAMD_ERR_SIZE 2
+#define PIVAMD_ERR_LAUNCH 3
+#define PIVAMD_ERR_NOMEM 4
+#define PIVAMD_ERR_RUNTIME 5
+
+extern "C" {
+int pivamd_size_supported(int ny, int nx);
+int pivamd_convolve(const float *d_a, const float *d_b, int ny, int nx, float *d_out, hipStream_t st);
+int pivamd_plane_weight(const float *d_wB, int ny, int nx, float *d_out, hipStream_t st);
+int pivamd_xcorr(const float *d_imgA, const float *d_imgB, const float *d_mask, int H, int W,
+                 int n_images, const float *d_ctrX, const float *d_ctrY, int nWy, int nWx,
+                 const float *d_wA, const float *d_wB, const float *d_planeWeight, int ensemble,
+                 int winY, int winX, int nPeaks, int finder, int impl, float *d_pkX, float *d_pkY,
+                 float *d_pkH, float *d_sx, float *d_sy, float *d_sxy, float *d_ratio,
+                 float *d_planes, void *d_work, size_t work_bytes, hipStream_t st);
+size_t pivamd_xcorr_workspace(int n_images, int nWy, int nWx, int winY, int winX, int nPeaks,
+                              int ensemble);
+int pivamd_warp(const float *d_imgA, const float *d_imgB, float *d_outA, float *d_outB,
+                const float *d_predDy, const float *d_predDx, int N, int H, int W, int nPY, int nPX,
+                const float *d_ctrY, const float *d_ctrX, int interp_mode, int shared_predictor,
+                int round_shifts, hipStream_t st);
+size_t pivamd_ensemble_workspace(int nWy, int nWx, int winY, int winX);
+int pivamd_accumulate(const float *d_A, const float *d_B, const float *d_mask, int H, int W, int n_images,
+                      const float *d_ctrX, const float *d_ctrY, int nWy, int nWx, const float *d_wAab,
+                      const float *d_wBab, const float *d_wAA, const float *d_wBB, int winY, int winX,
+                      int fitY, int fitX, int meanSub, int perPairNorm, int triple, float sumWBab,
+                      float sumWAA, float sumWBB, float *d_outAB, float *d_outAA, float *d_outBB,
+                      void *d_work, size_t work_bytes, hipStream_t st);
+int pivamd_peakfit(const float *d_planes, int n, int ny, int nx, int nPeaks, int finder, int impl,
+                   const float *d_planeWeight, float *d_loc, float *d_std, float *d_ratio,
+                   hipStream_t st);
+}

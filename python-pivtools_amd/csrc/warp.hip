@@ -1,0 +1,255 @@
+// warp.hip -- fused symmetric image warp (gfx950).
+//
+// Replaces fused_symmetric_warp(_batch) of pivtools_cli/lib/fused_warp.c:
+//   phase A  the coarse predictor (nPY x nPX, per image or shared) is
+//            upsampled at each pixel by bicubic Keys a=-0.75 with replicate
+//            border, its fractional grid index being np.interp of the pixel
+//            coordinate over the previous pass's padded centres (clamped);
+//   phase B  half = 0.5*d (optionally rounded to whole pixels);
+//   phase C  A is sampled at (i - hy, j - hx) and B at (i + hy, j + hx) by
+//            bicubic (4x4, floor-1) or Lanczos-3 (6x6, floor-2, 4097-entry
+//            LUT with lerp), zero outside the image.
+// The arithmetic is the reference's scalar path (impl 0) term for term with
+// contraction off, so every output pixel is bit-identical to it.
+//
+// Mapping: a 256-lane workgroup covers a 64-column x 16-row tile of one
+// image (grid.z = image).  Each lane owns one column and four rows; the
+// tile's predictor indices are computed once into LDS; every tap is bounds
+// checked, so a non-finite predictor can never produce an out-of-range read.
+#include <hip/hip_runtime.h>
+
+#include "pivamd_dev.h"
+
+namespace pivamd {
+
+#pragma clang fp contract(off)
+
+constexpr int kLanczosLut = 4096;
+constexpr int kTileX = 64, kTileY = 16, kLanesY = 4;
+
+__device__ __forceinline__ void keys4(float d, float w[4])
+{
+    const float s0 = 1.0f + d, q2 = 1.0f - d, s3 = 2.0f - d;
+    w[1] = (1.25f * d - 2.25f) * d * d + 1.0f;
+    w[2] = (1.25f * q2 - 2.25f) * q2 * q2 + 1.0f;
+    w[0] = ((-0.75f * s0 + 3.75f) * s0 - 6.0f) * s0 + 3.0f;
+    w[3] = ((-0.75f * s3 + 3.75f) * s3 - 6.0f) * s3 + 3.0f;
+}
+
+// np.interp(p, ctrs, arange(nC)) with clamp, as the sequential segment walk
+// of fused_warp.c:259-307 resolves it for ascending centres.
+__device__ float pred_index(float coord, const float *ctrs, int nC)
+{
+    if (nC == 1) return 0.0f;
+    if (coord <= ctrs[0]) return 0.0f;
+    if (coord >= ctrs[nC - 1]) return (float)(nC - 1);
+    // smallest seg in [0, nC-2] with ctrs[seg+1] >= coord
+    int lo = 0, hi = nC - 2;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (ctrs[mid + 1] < coord) lo = mid + 1;
+        else hi = mid;
+    }
+    const int seg = lo;
+    const float den = ctrs[seg + 1] - ctrs[seg];
+    if (den < 1e-12f) return (float)seg;
+    const float t = (coord - ctrs[seg]) / den;
+    return (float)seg + t;
+}
+
+__device__ __forceinline__ float sample_bicubic(const float *img, float fy, float fx, int H, int W)
+{
+    const float fyf = floorf(fy), fxf = floorf(fx);
+    const int iy = (int)fyf - 1, ix = (int)fxf - 1;
+    float wy[4], wx[4];
+    keys4(fy - fyf, wy);
+    keys4(fx - fxf, wx);
+    float val = 0.0f;
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+        const int row = iy + m;
+        if (row < 0 || row >= H) continue;
+#pragma unroll
+        for (int n = 0; n < 4; n++) {
+            const int col = ix + n;
+            if (col < 0 || col >= W) continue;
+            val += wy[m] * wx[n] * img[(size_t)row * W + col];
+        }
+    }
+    return val;
+}
+
+__device__ __forceinline__ void lanczos_w(float d, const float *lut, float w[6])
+{
+    const float pos = d * (float)kLanczosLut;
+    int idx = (int)pos;
+    float fr = pos - (float)idx;
+    if (idx >= kLanczosLut) { idx = kLanczosLut - 1; fr = 0.0f; }
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        const float a = lut[idx * 6 + k], b = lut[(idx + 1) * 6 + k];
+        w[k] = a + fr * (b - a);
+    }
+}
+
+__device__ __forceinline__ float sample_lanczos(const float *img, float fy, float fx, int H, int W,
+                                                const float *lut)
+{
+    const float fyf = floorf(fy), fxf = floorf(fx);
+    const int iy = (int)fyf - 2, ix = (int)fxf - 2;
+    float wy[6], wx[6];
+    lanczos_w(fy - fyf, lut, wy);
+    lanczos_w(fx - fxf, lut, wx);
+    float val = 0.0f;
+#pragma unroll
+    for (int m = 0; m < 6; m++) {
+        const int row = iy + m;
+        if (row < 0 || row >= H) continue;
+#pragma unroll
+        for (int n = 0; n < 6; n++) {
+            const int col = ix + n;
+            if (col < 0 || col >= W) continue;
+            val += wy[m] * wx[n] * img[(size_t)row * W + col];
+        }
+    }
+    return val;
+}
+
+struct WarpArgs {
+    const float *imgA, *imgB, *pdy, *pdx, *cy, *cx, *lut;
+    float *outA, *outB;
+    int H, W, nPY, nPX, predStride, mode, roundShifts;
+};
+
+__global__ void __launch_bounds__(256) warp_kernel(WarpArgs g)
+{
+    __shared__ float s_ly[kTileY];
+    __shared__ float s_lx[kTileX];
+    const int tx = threadIdx.x & (kTileX - 1);
+    const int ty = threadIdx.x / kTileX;
+    const int x0 = blockIdx.x * kTileX, y0 = blockIdx.y * kTileY;
+    const int img = blockIdx.z;
+    if (threadIdx.x < kTileX) {
+        const int x = x0 + threadIdx.x;
+        s_lx[threadIdx.x] = x < g.W ? pred_index((float)x, g.cx, g.nPX) : 0.0f;
+    } else if (threadIdx.x < kTileX + kTileY) {
+        const int y = y0 + threadIdx.x - kTileX;
+        s_ly[threadIdx.x - kTileX] = y < g.H ? pred_index((float)y, g.cy, g.nPY) : 0.0f;
+    }
+    __syncthreads();
+
+    const int j = x0 + tx;
+    if (j >= g.W) return;
+    const size_t plane = (size_t)g.H * g.W;
+    const float *A = g.imgA + (size_t)img * plane;
+    const float *B = g.imgB + (size_t)img * plane;
+    const float *dyp = g.pdy + (size_t)img * g.predStride;
+    const float *dxp = g.pdx + (size_t)img * g.predStride;
+
+    const float fx = s_lx[tx], fxf = floorf(fx);
+    const int ix = (int)fxf - 1;
+    float wx[4];
+    keys4(fx - fxf, wx);
+    int cols[4];
+#pragma unroll
+    for (int n = 0; n < 4; n++) cols[n] = min(max(ix + n, 0), g.nPX - 1);
+
+#pragma unroll
+    for (int r = 0; r < kTileY / kLanesY; ++r) {
+        const int ly = ty + r * kLanesY;
+        const int i = y0 + ly;
+        if (i >= g.H) break;
+        const float fiy = s_ly[ly], fiyf = floorf(fiy);
+        const int iyb = (int)fiyf - 1;
+        float wy[4];
+        keys4(fiy - fiyf, wy);
+        float va = 0.0f, vb = 0.0f;
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            const int row = min(max(iyb + m, 0), g.nPY - 1);
+#pragma unroll
+            for (int n = 0; n < 4; n++) {
+                const float w = wy[m] * wx[n];
+                const int k = row * g.nPX + cols[n];
+                va += w * dyp[k];
+                vb += w * dxp[k];
+            }
+        }
+        float hy = 0.5f * va, hx = 0.5f * vb;
+        if (g.roundShifts) { hy = roundf(hy); hx = roundf(hx); }
+        const size_t o = (size_t)img * plane + (size_t)i * g.W + j;
+        if (g.mode == 0) {
+            g.outA[o] = sample_bicubic(A, (float)i - hy, (float)j - hx, g.H, g.W);
+            g.outB[o] = sample_bicubic(B, (float)i + hy, (float)j + hx, g.H, g.W);
+        } else {
+            g.outA[o] = sample_lanczos(A, (float)i - hy, (float)j - hx, g.H, g.W, g.lut);
+            g.outB[o] = sample_lanczos(B, (float)i + hy, (float)j + hx, g.H, g.W, g.lut);
+        }
+    }
+}
+
+#pragma clang fp contract(on)
+
+// Lanczos-3 weight table built on the host with the host libm (the same
+// table fused_warp.c:148-169 builds), uploaded once per device.
+static float *g_lut_dev[64];
+
+static float lanczos_weight_host(float t)
+{
+    const float at = fabsf(t);
+    if (at < 1e-6f) return 1.0f;
+    if (at >= 3.0f) return 0.0f;
+    const float pt = (float)M_PI * at;
+    const float pta = pt / 3.0f;
+    return (sinf(pt) / pt) * (sinf(pta) / pta);
+}
+
+static const float *lanczos_lut_device()
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+    if (g_lut_dev[dev]) return g_lut_dev[dev];
+    static float host[(kLanczosLut + 1) * 6];
+    for (int i = 0; i <= kLanczosLut; i++) {
+        const float d = (float)i / (float)kLanczosLut;
+        for (int k = 0; k < 6; k++) host[i * 6 + k] = lanczos_weight_host(d - (float)(k - 2));
+    }
+    float *p = nullptr;
+    if (hipMalloc(&p, sizeof(host)) != hipSuccess) return nullptr;
+    if (hipMemcpy(p, host, sizeof(host), hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(p);
+        return nullptr;
+    }
+    g_lut_dev[dev] = p;
+    return p;
+}
+
+}  // namespace pivamd
+
+using namespace pivamd;
+
+extern "C" int pivamd_warp(const float *d_imgA, const float *d_imgB, float *d_outA, float *d_outB,
+                           const float *d_predDy, const float *d_predDx, int N, int H, int W,
+                           int nPY, int nPX, const float *d_ctrY, const float *d_ctrX,
+                           int interp_mode, int shared_predictor, int round_shifts, hipStream_t st)
+{
+    if (N <= 0 || H <= 0 || W <= 0 || nPY <= 0 || nPX <= 0) return PIVAMD_ERR_ARG;
+    if (!d_imgA || !d_imgB || !d_outA || !d_outB || !d_predDy || !d_predDx || !d_ctrY || !d_ctrX)
+        return PIVAMD_ERR_ARG;
+    if (N > 65535) return PIVAMD_ERR_ARG;
+    WarpArgs g;
+    g.imgA = d_imgA; g.imgB = d_imgB; g.pdy = d_predDy; g.pdx = d_predDx;
+    g.cy = d_ctrY; g.cx = d_ctrX; g.outA = d_outA; g.outB = d_outB;
+    g.H = H; g.W = W; g.nPY = nPY; g.nPX = nPX;
+    g.predStride = shared_predictor ? 0 : nPY * nPX;
+    g.mode = interp_mode == 0 ? 0 : 1;
+    g.roundShifts = round_shifts ? 1 : 0;
+    g.lut = nullptr;
+    if (g.mode == 1) {
+        g.lut = lanczos_lut_device();
+        if (!g.lut) return PIVAMD_ERR_NOMEM;
+    }
+    dim3 grid((W + kTileX - 1) / kTileX, (H + kTileY - 1) / kTileY, N);
+    hipLaunchKernelGGL(warp_kernel, grid, dim3(256), 0, st, g);
+    return hipGetLastError() == hipSuccess ? 0 : PIVAMD_ERR_LAUNCH;
+}

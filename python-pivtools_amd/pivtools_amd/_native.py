@@ -1,0 +1,150 @@
+"""ctypes bindings of the three in-tree libraries.
+
+* ``libpivtools_amd.so``: the device-pointer API (``pivamd_*``), driven by
+  the host layer of this package with device buffers it owns.
+* ``libbulkxcorr2d.so`` / ``libfusedwarp.so``: host-pointer drop-ins with
+  the reference's exported names and arguments; bound here with the same
+  ``argtypes`` the reference's correlators declare
+  (``pivtools_cli/piv/piv_backend/cpu_instantaneous.py:68-165``).
+
+Loading never falls back: a missing library raises ``FileNotFoundError``.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+LIB_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib")
+
+_f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+
+_CORE = None
+_XC = None
+_FW = None
+
+
+def lib_path(name: str) -> str:
+    path = os.path.join(LIB_DIR, name)
+    if not os.path.isfile(path):
+        raise FileNotFoundError(
+            f"{path} is not built; run python python-pivtools_amd/build.py "
+            "(or __graft_entry__.build()) first")
+    return path
+
+
+def core() -> ctypes.CDLL:
+    """libpivtools_amd.so with argtypes of the device-pointer API."""
+    global _CORE
+    if _CORE is None:
+        L = ctypes.CDLL(lib_path("libpivtools_amd.so"))
+        L.pivamd_size_supported.argtypes = [_i, _i]
+        L.pivamd_size_supported.restype = _i
+        L.pivamd_plane_weight.argtypes = [_vp, _i, _i, _vp, _vp]
+        L.pivamd_plane_weight.restype = _i
+        L.pivamd_convolve.argtypes = [_vp, _vp, _i, _i, _vp, _vp]
+        L.pivamd_convolve.restype = _i
+        L.pivamd_xcorr_workspace.argtypes = [_i, _i, _i, _i, _i, _i, _i]
+        L.pivamd_xcorr_workspace.restype = ctypes.c_size_t
+        L.pivamd_xcorr.argtypes = [
+            _vp, _vp, _vp, _i, _i, _i, _vp, _vp, _i, _i, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i,
+            _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_size_t, _vp,
+        ]
+        L.pivamd_xcorr.restype = _i
+        L.pivamd_warp.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp, _vp, _i, _i, _i, _vp]
+        L.pivamd_warp.restype = _i
+        L.pivamd_peakfit.argtypes = [_vp, _i, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp]
+        L.pivamd_peakfit.restype = _i
+        for name, argt in GLUE_SIGNATURES.items():
+            if hasattr(L, name):
+                f = getattr(L, name)
+                f.argtypes = argt
+                f.restype = _i
+        if hasattr(L, "pivamd_accumulate_triple"):
+            L.pivamd_accumulate_triple.argtypes = [
+                _vp, _vp, _vp, _i, _i, _i, _vp, _vp, _i, _i, _vp, _vp, _vp, _vp, _i, _i, _i, _i,
+                _i, _i, _i, _vp, _vp, _vp, _vp,
+            ]
+            L.pivamd_accumulate_triple.restype = _i
+        _CORE = L
+    return _CORE
+
+
+# device-pointer glue entry points (multipass loop on the device)
+GLUE_SIGNATURES: dict = {}
+
+
+def bulkxcorr2d_lib() -> ctypes.CDLL:
+    """libbulkxcorr2d.so bound exactly as the reference's correlators bind it."""
+    global _XC
+    if _XC is None:
+        L = ctypes.CDLL(lib_path("libbulkxcorr2d.so"))
+        L.bulkxcorr2d.restype = ctypes.c_ubyte
+        L.bulkxcorr2d.argtypes = [
+            _f32p, _f32p, _f32p, _i32p, _i, _f32p, _f32p, _i32p, _f32p, ctypes.c_bool, _f32p, _i32p,
+            _i, _i, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _vp,
+        ]
+        if hasattr(L, "bulkxcorr2d_accumulate"):
+            L.bulkxcorr2d_accumulate.restype = ctypes.c_ubyte
+            L.bulkxcorr2d_accumulate.argtypes = [
+                _f32p, _f32p, _f32p, _i32p, _i, _f32p, _f32p, _i32p, _f32p, _f32p, _i32p, _vp, _f32p,
+            ]
+            L.bulkxcorr2d_accumulate_triple.restype = ctypes.c_ubyte
+            L.bulkxcorr2d_accumulate_triple.argtypes = [
+                _f32p, _f32p, _f32p, _i32p, _i, _f32p, _f32p, _i32p, _f32p, _f32p, _f32p, _f32p, _i32p,
+                _vp, _i, _i, _f32p, _f32p, _f32p,
+            ]
+        L.lsqpeaklocate_lm.restype = None
+        L.lsqpeaklocate_lm.argtypes = [_f32p, _i32p, _f32p, _i, _i, _f32p, _vp]
+        L.lsqpeaklocate_lm_batch.restype = None
+        L.lsqpeaklocate_lm_batch.argtypes = [_f32p, _i, _i32p, _i, _f32p, _f32p, _vp]
+        L.peak_detectability.restype = ctypes.c_float
+        L.peak_detectability.argtypes = [_f32p, _i32p]
+        L.bulkxcorr2d_set_peakfit_impl.argtypes = [_i]
+        L.bulkxcorr2d_set_peakfit_impl.restype = _i
+        L.bulkxcorr2d_get_peakfit_impl.restype = _i
+        L.peakfit_batch_lanes.restype = _i
+        L.pivtools_cpu_supported.restype = _i
+        L.convolve.restype = ctypes.c_uint
+        L.convolve.argtypes = [_f32p, _f32p, _f32p, _i32p]
+        L.xcorr.restype = ctypes.c_uint
+        L.xcorr.argtypes = [_f32p, _f32p, _f32p, _i32p]
+        L.fminvec.restype = ctypes.c_float
+        L.fminvec.argtypes = [_f32p, _i]
+        L.fmaxvec.restype = ctypes.c_float
+        L.fmaxvec.argtypes = [_f32p, _i]
+        L.bulkxcorr2d_get_timing.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+        L.bulkxcorr2d_get_timing.restype = None
+        _XC = L
+    return _XC
+
+
+def fusedwarp_lib() -> ctypes.CDLL:
+    """libfusedwarp.so bound as cpu_instantaneous.py:81-104 binds the reference."""
+    global _FW
+    if _FW is None:
+        L = ctypes.CDLL(lib_path("libfusedwarp.so"))
+        L.fused_symmetric_warp_batch.restype = ctypes.c_int
+        L.fused_symmetric_warp_batch.argtypes = [
+            _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _i, _i, _i, _i, _i, _f32p, _f32p, _i, _i, _i,
+        ]
+        L.fused_symmetric_warp.restype = ctypes.c_int
+        L.fused_symmetric_warp.argtypes = [
+            _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _i, _i, _i, _i, _f32p, _f32p, _i, _i,
+        ]
+        L.fused_warp_set_impl.argtypes = [_i]
+        L.fused_warp_get_impl.restype = _i
+        _FW = L
+    return _FW
+
+
+def check(rc: int, what: str) -> None:
+    if rc != 0:
+        names = {1: "bad argument", 2: "unsupported window size", 3: "kernel launch failed",
+                 4: "out of device memory", 5: "runtime error"}
+        raise RuntimeError(f"{what} failed: {names.get(rc, rc)}")

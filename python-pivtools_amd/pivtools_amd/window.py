@@ -1,0 +1,155 @@
+"""Window grid, window weights and the multipass grid caches.
+
+Host-side restatements of the reference's grid helpers; they produce the
+arrays that cross the C ABI (window centres, weights, padded predictor grid)
+so the numbers handed to the kernels are the reference's numbers:
+
+* ``compute_window_centers``  pivtools_core/window_utils.py:142-273
+* ``window_weight``           pivtools_cli/piv/piv_backend/base.py:52-135
+* ``PassGrid`` / ``build_pass_grids``
+                              pivtools_cli/piv/piv_backend/base.py:137-300
+                              (instantaneous: first-pass kernel (0, 0), no BCs)
+                              and base.py:410-483 (predictor interpolation maps)
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+BUILT_FFT_SIZES = (8, 12, 16, 24, 32, 48, 64, 96, 128)   # pivtools_core/fft_sizes.py
+
+
+@dataclass
+class WindowCenters:
+    win_ctrs_x: np.ndarray
+    win_ctrs_y: np.ndarray
+    n_win_x: int
+    n_win_y: int
+    win_spacing_x: int
+    win_spacing_y: int
+
+
+def compute_window_centers(image_shape, window_size, overlap, validate=True) -> WindowCenters:
+    """Window centres: first at (w-1)/2, last at W-(w+1)/2, y anchored to the bottom."""
+    H, W = image_shape
+    wh, ww = window_size
+    if validate:
+        if wh > H or ww > W:
+            raise ValueError(f"Window size {window_size} exceeds image dimensions {image_shape}")
+        if overlap < 0 or overlap >= 100:
+            raise ValueError(f"Overlap must be in range [0, 100), got {overlap}")
+        if wh < 1 or ww < 1:
+            raise ValueError(f"Window size must be positive, got {window_size}")
+    sx = max(1, round((1 - overlap / 100) * ww))
+    sy = max(1, round((1 - overlap / 100) * wh))
+    fx, lx = (ww - 1) / 2.0, W - (ww + 1) / 2.0
+    fy, ly = (wh - 1) / 2.0, H - (wh + 1) / 2.0
+    nx = max(1, int(np.floor((lx - fx) / sx)) + 1)
+    ny = max(1, int(np.floor((ly - fy) / sy)) + 1)
+    cx = np.linspace(fx, fx + sx * (nx - 1), nx, dtype=np.float32)
+    cy = np.linspace(ly - sy * (ny - 1), ly, ny, dtype=np.float32)
+    return WindowCenters(cx, cy, nx, ny, sx, sy)
+
+
+def window_weight(wsize, window_type: str) -> np.ndarray:
+    """2D window weight (float32), the reference's _window_weight_fun for the
+    types an instantaneous or standard ensemble pass uses."""
+    m, n = wsize
+    t = window_type.lower()
+    if t == "square":
+        return np.ones((m, n), dtype=np.float32)
+    if t in ("gaussian", "gaussiantop"):
+        wxv = np.arange(-m / 2 + 0.5, m / 2, dtype=np.float32)
+        wyv = np.arange(-n / 2 + 0.5, n / 2, dtype=np.float32)
+        xm, ym = np.meshgrid(wxv, wyv, indexing="ij")
+        w = np.exp(-0.5 * ((1.0 * xm / (m / 2)) ** 2 + (1.0 * ym / (n / 2)) ** 2))
+        if t == "gaussiantop":
+            if m > 6 and n > 6:
+                cm = np.tile(np.array([[1, 0], [0, 0]], dtype=np.float32), (m // 2, n // 2))
+                cm = np.roll(np.roll(cm, int(np.ceil(m / 4)), axis=0), int(np.ceil(n / 4)), axis=1)
+            else:
+                cm = np.zeros((m, n), dtype=np.float32)
+                r0, c0 = m // 2 - 1, n // 2 - 1
+                cm[r0:r0 + 4, c0:c0 + 4] = 1
+            w = w * cm
+        return np.ascontiguousarray(w.astype(np.float32))
+    if t == "blackman":
+        a0, a1, a2 = 0.42659, 0.49656, 0.076849
+        dmax = np.sqrt(((m + 1) / 2) ** 2 + ((n + 1) / 2) ** 2)
+        w = np.zeros((m, n), dtype=np.float32)
+        for ii in range(m):
+            for jj in range(n):
+                d = np.sqrt((ii - (m - 1) / 2) ** 2 + (jj - (n - 1) / 2) ** 2)
+                th = d / dmax * np.pi + np.pi
+                w[ii, jj] = a0 - a1 * np.cos(th) + a2 * np.cos(2 * th)
+        w[w <= 0] = 0.01
+        return w
+    raise ValueError(f"Unrecognized window type '{window_type}'")
+
+
+@dataclass
+class PassGrid:
+    """Everything one pass of the instantaneous multipass loop needs."""
+
+    win: tuple
+    ctrs_x: np.ndarray          # measured centres (n_win_x,)
+    ctrs_y: np.ndarray
+    spacing: tuple              # (spacing_y, spacing_x)
+    ctrs_x_all: np.ndarray      # padded centres (predictor grid)
+    ctrs_y_all: np.ndarray
+    n_pre: tuple                # (pre_y, pre_x)
+    n_post: tuple
+    ksize: tuple                # predictor smoothing kernel size
+    sd: float                   # predictor smoothing sigma
+    pred_map_x: np.ndarray | None = None   # fractional index into previous padded grid
+    pred_map_y: np.ndarray | None = None
+
+    @property
+    def n_win(self):
+        return (len(self.ctrs_y), len(self.ctrs_x))
+
+
+def _pad_axis(ctrs, spacing, size):
+    pre = np.arange(1, ctrs[0] - spacing / 2, spacing)
+    if pre.size == 0:
+        pre = np.array([1])
+    pre = pre - 1
+    while len(pre) < 2:
+        pre = np.concatenate([[max(0, pre[0] - spacing)], pre])
+    post = np.arange(size, ctrs[-1] + spacing / 2, -spacing)
+    if post.size == 0:
+        post = np.array([size])
+    post = post - 1
+    while len(post) < 2:
+        post = np.concatenate([post, [min(size - 1, post[-1] + spacing)]])
+    return np.concatenate([pre, ctrs, post[::-1]]), len(pre), len(post)
+
+
+def build_pass_grids(image_shape, window_sizes, overlaps) -> list[PassGrid]:
+    H, W = image_shape
+    grids: list[PassGrid] = []
+    for p, win in enumerate(window_sizes):
+        wc = compute_window_centers(image_shape, win, overlaps[p])
+        xa, prex, postx = _pad_axis(wc.win_ctrs_x, wc.win_spacing_x, W)
+        ya, prey, posty = _pad_axis(wc.win_ctrs_y, wc.win_spacing_y, H)
+        if p == 0:
+            ksize, sd = (0, 0), 0.0
+        else:
+            pw, ps = window_sizes[p - 1], grids[p - 1].spacing
+            k = np.round(np.array(pw) / np.array(ps)).astype(int) + 1
+            ksize = (int(k[0]) + (int(k[0]) % 2 == 0), int(k[1]) + (int(k[1]) % 2 == 0))
+            sd = float(np.sqrt(np.prod(ksize)) / 3 * 0.65)
+        g = PassGrid(tuple(win), wc.win_ctrs_x.astype(np.float32), wc.win_ctrs_y.astype(np.float32),
+                     (wc.win_spacing_y, wc.win_spacing_x), xa.astype(np.float32), ya.astype(np.float32),
+                     (prey, prex), (posty, postx), ksize, sd)
+        if p > 0:
+            prev = grids[p - 1]
+            wy, wx = np.meshgrid(g.ctrs_y, g.ctrs_x, indexing="ij")
+            ix = np.interp(wx.ravel(), prev.ctrs_x_all, np.arange(len(prev.ctrs_x_all)))
+            iy = np.interp(wy.ravel(), prev.ctrs_y_all, np.arange(len(prev.ctrs_y_all)))
+            g.pred_map_x = ix.reshape(wx.shape).astype(np.float32)
+            g.pred_map_y = iy.reshape(wy.shape).astype(np.float32)
+        grids.append(g)
+    return grids
