@@ -1,0 +1,292 @@
+"""CPU oracle of the instantaneous multipass loop -- TEST INFRASTRUCTURE ONLY.
+
+NumPy/SciPy restatement of the host glue of
+pivtools_cli/piv/piv_backend/cpu_instantaneous.py:381-1147 around the two C
+calls, with the C calls taken from oracle.py (liboracle.so) or, for the
+FP64 flavour, from FP64 NumPy planes.
+
+Pins:
+  gate / reason codes   restated from cpu_instantaneous.py:576-757 (unpinned
+                        beyond the reference's own invariants; tested for
+                        (nan_reason != 0) == nan_mask)
+  median test           outlier_detection.py:87-243 restated; checked against
+                        a brute-force per-node loop in the tests
+  nearest infill        scipy.ndimage.distance_transform_edt, as the reference
+                        calls it (infilling.py:114-141)
+  smoothing             scipy.ndimage.gaussian_filter, as the reference calls
+                        it (cpu_instantaneous.py:1135-1147)
+  predictor remap       cv2.remap(INTER_CUBIC, BORDER_REPLICATE) is not
+                        importable here: restated from OpenCV's published
+                        algorithm (float map -> 1/32 fixed point, Keys a=-0.75
+                        per axis, replicate border).  Parity unpinned: only the
+                        closed forms (constant, linear and integer-shift
+                        fields) check it.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+from numpy.lib.stride_tricks import sliding_window_view
+from scipy import ndimage as ndi
+
+import oracle as O
+
+
+def gate(pkx, pky, pkh, pred, mask, tx, ty):
+    """cpu_instantaneous.py:576-635 for one peak.  pk* (N, ny, nx); pred
+    (2, N, ny, nx) planar [dy, dx] or None; returns u planar, nan_mask,
+    nan_reason, peak height with NaN at gated vectors."""
+    pkx = pkx.astype(np.float32).copy()
+    pky = pky.astype(np.float32).copy()
+    pkh = pkh.astype(np.float32).copy()
+    N = pkx.shape[0]
+    m = np.zeros(pkx.shape[1:], bool) if mask is None else mask.astype(bool)
+    mb = np.broadcast_to(m, pkx.shape)
+    large = (np.abs(pkx) > tx) | (np.abs(pky) > ty)
+    fit_fail = np.isnan(pkx) | np.isnan(pky)
+    inv = mb | large
+    pkx[inv] = np.nan
+    pky[inv] = np.nan
+    pkh[inv] = np.nan
+    if pred is not None:
+        pkx += pred[1]
+        pky += pred[0]
+    nm = np.isnan(pkx) | np.isnan(pky) | mb
+    ux = pkx.copy()
+    uy = pky.copy()
+    ux[nm] = 0.0
+    uy[nm] = 0.0
+    reason = np.zeros(pkx.shape, np.int8)
+    reason[fit_fail] = 1
+    reason[large] = 6
+    del N
+    return np.stack([uy, ux]).astype(np.float32), nm, reason, pkh
+
+
+def median_outliers(ux, uy, eps=0.1, thr=2.0, size=5, masked=None):
+    """outlier_detection.py:87-243 (combine='norm'), nanmedian via NumPy."""
+    ux = ux.astype(np.float32, copy=True)
+    uy = uy.astype(np.float32, copy=True)
+    if masked is not None:
+        ux[masked] = np.nan
+        uy[masked] = np.nan
+
+    def nbrs(U):
+        pad = size // 2
+        Up = np.pad(U, pad, mode="constant", constant_values=np.nan)
+        win = sliding_window_view(Up, (size, size)).reshape(U.shape[0], U.shape[1], size * size)
+        c = (size * size) // 2
+        return np.concatenate([win[..., :c], win[..., c + 1:]], axis=-1)
+
+    import warnings
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore", RuntimeWarning)
+        uxn, uyn = nbrs(ux), nbrs(uy)
+        mu = np.nanmedian(uxn, axis=-1).astype(np.float32)
+        mv = np.nanmedian(uyn, axis=-1).astype(np.float32)
+        r0 = np.sqrt((ux - mu) ** 2 + (uy - mv) ** 2)
+        ri = np.sqrt((uxn - mu[..., None]) ** 2 + (uyn - mv[..., None]) ** 2)
+        rm = np.nanmedian(ri, axis=-1).astype(np.float32)
+        nr = r0 / (rm + np.float32(eps))
+    exceeded = (nr > thr) | ~np.isfinite(nr)
+    present = np.isfinite(ux) & np.isfinite(uy)
+    if masked is not None:
+        present |= masked
+    npres = ndi.correlate(present.astype(np.float32), np.ones((3, 3), np.float32), mode="constant", cval=0.0)
+    out = exceeded | (npres < 6)
+    if masked is not None:
+        out &= ~masked
+    return out
+
+
+def median_stage(u, pkh, nanm, reason, mask=None, eps=0.1, thr=2.0, size=5):
+    """Outlier stage plus the bookkeeping of cpu_instantaneous.py:639-757."""
+    N = u.shape[1]
+    nanm = nanm.copy()
+    reason = reason.copy()
+    m = np.zeros(u.shape[2:], bool) if mask is None else mask.astype(bool)
+    for i in range(N):
+        om = median_outliers(u[1, i], u[0, i], eps, thr, size, masked=m if mask is not None else None)
+        reason[i][om & (reason[i] == 0)] = 10
+        nanm[i] |= om
+    hn = np.isnan(pkh)
+    nanm |= hn
+    reason[hn & (reason == 0)] = 13
+    reason[nanm & (reason == 0)] = 9
+    mb = np.broadcast_to(m, nanm.shape)
+    reason[mb] = -1
+    uo = u.copy()
+    uo[:, nanm] = np.nan
+    uo[:, mb] = 0.0
+    pk = pkh.copy()
+    pk[nanm] = np.nan
+    return uo, nanm, reason, pk
+
+
+def infill_nearest(u, spacing, exclude=None):
+    """Nearest-donor fill per image (infilling.py:114-141, 290-333)."""
+    out = u.astype(np.float64).copy()
+    for i in range(u.shape[1]):
+        ux, uy = u[1, i], u[0, i]
+        fill = np.isnan(ux) | np.isnan(uy)
+        if not fill.any():
+            continue
+        donor = ~fill & np.isfinite(ux) & np.isfinite(uy)
+        if exclude is not None:
+            donor &= ~exclude
+        if not donor.any():
+            raise ValueError("no donor")
+        iy, ix = ndi.distance_transform_edt(~donor, sampling=spacing, return_distances=False, return_indices=True)
+        for c in range(2):
+            f = out[c, i]
+            f[fill] = u[c, i][iy, ix][fill]
+    return out.astype(np.float32)
+
+
+def pad_edge(u, pre, post):
+    return np.pad(u, ((0, 0), (0, 0), (pre[0], post[0]), (pre[1], post[1])), mode="edge")
+
+
+def smooth(field, sd, ksize):
+    """gaussian_filter per component and image (cpu_instantaneous.py:1135-1147)."""
+    trunc = (ksize[0] - 1) / (2 * sd)
+    out = np.empty_like(field)
+    for c in range(field.shape[0]):
+        for i in range(field.shape[1]):
+            out[c, i] = ndi.gaussian_filter(field[c, i], sigma=sd, truncate=trunc, mode="nearest")
+    return out
+
+
+def _cubic(x):
+    A = np.float32(-0.75)
+    x = np.float32(x)
+    one = np.float32(1)
+    w0 = ((A * (x + one) - 5 * A) * (x + one) + 8 * A) * (x + one) - 4 * A
+    w1 = ((A + 2) * x - (A + 3)) * x * x + one
+    w2 = ((A + 2) * (one - x) - (A + 3)) * (one - x) * (one - x) + one
+    w3 = one - w0 - w1 - w2
+    return np.array([w0, w1, w2, w3], np.float32)
+
+
+def remap_cubic(field, map_x, map_y):
+    """Bicubic remap on the 1/32 lattice with replicate border (cv2.remap
+    INTER_CUBIC as published); field (2, N, nyo, nxo) -> (2, N, ny, nx)."""
+    C, N, nyo, nxo = field.shape
+    ny, nx = map_x.shape
+    X = np.rint(map_x.astype(np.float32) * np.float32(32)).astype(np.int64)
+    Y = np.rint(map_y.astype(np.float32) * np.float32(32)).astype(np.int64)
+    out = np.zeros((C, N, ny, nx), np.float32)
+    for r in range(ny):
+        for c in range(nx):
+            sx, sy = (X[r, c] >> 5) - 1, (Y[r, c] >> 5) - 1
+            wx = _cubic(np.float32(X[r, c] & 31) * np.float32(1 / 32))
+            wy = _cubic(np.float32(Y[r, c] & 31) * np.float32(1 / 32))
+            rows = np.clip(sy + np.arange(4), 0, nyo - 1)
+            cols = np.clip(sx + np.arange(4), 0, nxo - 1)
+            w2 = (wy[:, None] * wx[None, :]).astype(np.float32)
+            patch = field[:, :, rows][:, :, :, cols]
+            out[:, :, r, c] = np.sum(np.sum(patch * w2, axis=-1, dtype=np.float32), axis=-1, dtype=np.float32)
+    return out
+
+
+def peaks_fp64(planes, pw):
+    """Peak search + 3-point fit of FP64 planes (n, h, w), vectorised:
+    the quarter-region 3x3 local-max rule of peak_locate_lm.c:796-813 and
+    threept_estimate (:35-66) on the 5x5 patch times the plane weight."""
+    n, h, w = planes.shape
+    P = planes
+    i_lo, i_hi, j_lo, j_hi = h // 4, h * 3 // 4, w // 4, w * 3 // 4
+    core = P[:, i_lo:i_hi, j_lo:j_hi]
+    ismax = np.ones_like(core, bool)
+    for di in (-1, 0, 1):
+        for dj in (-1, 0, 1):
+            nb = P[:, i_lo + di:i_hi + di, j_lo + dj:j_hi + dj]
+            ismax &= ~(nb > core)
+    cand = np.where(ismax & (core > 0), core, -np.inf)
+    flat = cand.reshape(n, -1)
+    k = np.argmax(flat, axis=1)                      # first maximum in row-major order
+    h0 = flat[np.arange(n), k]
+    ok = np.isfinite(h0)
+    cw = j_hi - j_lo
+    i0 = i_lo + k // cw
+    j0 = j_lo + k % cw
+    rows = np.clip(i0[:, None] + np.arange(-1, 2)[None], 0, h - 1)
+    cols = np.clip(j0[:, None] + np.arange(-1, 2)[None], 0, w - 1)
+    xv = P[np.arange(n)[:, None], rows, j0[:, None]] * pw[rows, j0[:, None]]
+    yv = P[np.arange(n)[:, None], i0[:, None], cols] * pw[i0[:, None], cols]
+    eps = np.finfo(np.float32).eps
+    lx = np.log(np.maximum(xv, eps))
+    ly = np.log(np.maximum(yv, eps))
+    dx = 2 * lx[:, 0] - 4 * lx[:, 1] + 2 * lx[:, 2]
+    dy = 2 * ly[:, 0] - 4 * ly[:, 1] + 2 * ly[:, 2]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        oi = np.where(np.abs(dx) > eps, (lx[:, 0] - lx[:, 2]) / dx, 0.0)
+        oj = np.where(np.abs(dy) > eps, (ly[:, 0] - ly[:, 2]) / dy, 0.0)
+    row = np.where(ok, i0 + oi, np.nan)
+    col = np.where(ok, j0 + oj, np.nan)
+    return row, col, np.where(ok, h0, 0.0)
+
+
+def xcorr_pass(A, B, g, w, finder, flavour):
+    """One bulkxcorr2d over all pairs: flavour 'oracle' (float32 C
+    restatement) or 'fp64' (FP64 planes, FP64 3-point fit)."""
+    ny, nx = g.n_win
+    win = g.win
+    if flavour == "oracle":
+        o = O.bulkxcorr2d(A, B, np.zeros((ny, nx), np.float32), g.ctrs_x, g.ctrs_y, win, w, w, 1, finder)
+        return o["pk_x"][:, 0], o["pk_y"][:, 0], o["pk_h"][:, 0]
+    if finder != 3:
+        raise ValueError("the FP64 flavour fits with the 3-point estimator")
+    pw = O.plane_weight(w).astype(np.float64)
+    N = A.shape[0]
+    pkx = np.zeros((N, ny, nx))
+    pky = np.zeros((N, ny, nx))
+    pkh = np.zeros((N, ny, nx))
+    for i in range(N):
+        a, b, ok = O.windows_fp64(A[i], B[i], g.ctrs_x, g.ctrs_y, win, w.astype(np.float64), w.astype(np.float64))
+        fa, fb = np.fft.fft2(a), np.fft.fft2(b)
+        planes = np.fft.fftshift(np.real(np.fft.ifft2(fb * np.conj(fa))), axes=(-2, -1)) / 1.0
+        row, col, h = peaks_fp64(planes, pw)
+        en = 1.0 / np.sqrt(np.sum(a * a, axis=(1, 2)) * np.sum(b * b, axis=(1, 2)))
+        pkx[i] = (col - win[1] / 2.0).reshape(ny, nx)
+        pky[i] = (row - win[0] / 2.0).reshape(ny, nx)
+        pkh[i] = (h * en).reshape(ny, nx)
+    return pkx, pky, pkh
+
+
+def run_multipass(A, B, grids, window_type="gaussian", finder=3, smoothing=True, median=(0.1, 2.0, 5),
+                  final_infill="nearest", flavour="oracle", interp=0, keep_passes=False):
+    """The whole pass loop on the CPU; A, B (N, H, W) float32."""
+    from pivtools_amd.window import window_weight
+
+    A = np.ascontiguousarray(A, np.float32)
+    B = np.ascontiguousarray(B, np.float32)
+    prev_pad = None
+    prev_grid = None
+    out = []
+    for p, g in enumerate(grids):
+        w = window_weight(g.win, window_type)
+        pred = None
+        if p == 0:
+            a_in, b_in = A, B
+        else:
+            field = prev_pad
+            if smoothing and g.sd > 0:
+                field = smooth(field, g.sd, g.ksize)
+            pred = remap_cubic(field, g.pred_map_x, g.pred_map_y)
+            a_in, b_in = O.fused_warp_batch(A, B, field[0], field[1], prev_grid.ctrs_y_all, prev_grid.ctrs_x_all,
+                                            interp, False, False)
+        pkx, pky, pkh = xcorr_pass(a_in, b_in, g, w, finder, flavour)
+        div = 2.0 if p == 0 else 4.0
+        u, nm, reason, pkg = gate(pkx, pky, pkh, pred, None, g.win[1] / div, g.win[0] / div)
+        u, nm, reason, pko = median_stage(u, pkg, nm, reason, None, *median)
+        final = p == len(grids) - 1
+        if not final or final_infill == "nearest":
+            u = infill_nearest(u, (float(g.spacing[0]), float(g.spacing[1])))
+        if not final:
+            prev_pad = pad_edge(u, g.n_pre, g.n_post)
+        if keep_passes or final:
+            out.append({"ux": u[1], "uy": u[0], "nan_mask": nm, "nan_reason": reason, "peak_mag": pko,
+                        "predictor": pred})
+        prev_grid = g
+    return out

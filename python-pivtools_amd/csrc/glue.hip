@@ -1,0 +1,385 @@
+// glue.hip -- device-resident multipass glue between correlation passes.
+//
+// The reference runs these steps in host NumPy/cv2/scipy between its two C
+// calls (pivtools_cli/piv/piv_backend/cpu_instantaneous.py:555-865); here
+// they are small kernels so a whole pass stays on the device:
+//   gate_kernel       displacement gate N/2 (first pass) or N/4, NaN sentinel,
+//                     predictor add-back, nan_mask / nan_reason (:576-635)
+//   median_kernel     normalised-median test, 5x5, vector norm, 3x3 guard
+//                     (outlier_detection.py:87-243) and the reason codes (:726-757)
+//   infill_kernel     nearest-donor fill in physical distance, ties to the
+//                     smallest column then row, as scipy's EDT resolves them
+//                     (infilling.py:114-141)
+//   pad_kernel        edge padding into the predictor grid (:836-851)
+//   smooth_kernel     scipy gaussian_filter, mode 'nearest', axis 0 then 1 with
+//                     float32 rounding between axes (:1135-1147)
+//   remap_kernel      predictor onto the new window grid, bicubic a=-0.75 on a
+//                     1/32-pixel fixed-point grid with replicate border, the
+//                     interpolation cv2.remap(INTER_CUBIC) performs (:1050-1071)
+//   u16_to_f32_kernel 12-bit ingest
+// Fields are planar: [component][image][row][col] with component 0 = dy
+// (uy) and 1 = dx (ux), which is what the warp kernel consumes.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pivamd_dev.h"
+
+namespace pivamd {
+
+__global__ void u16_to_f32_kernel(const uint16_t *in, float *out, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t k = i; k < n; k += stride) out[k] = (float)in[k];
+}
+
+// pk fields (N, ny, nx) of a single-peak bulkxcorr2d; pred planar (2, N, ny, nx)
+// or null; u planar (2, N, ny, nx) out; nan_mask u8, reason i8 out.
+__global__ void gate_kernel(const float *pkx, const float *pky, const float *pkh, const float *pred,
+                            const float *mask, int N, int nWin, float tx, float ty, float *u,
+                            uint8_t *nanm, int8_t *reason, float *pkh_out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int total = N * nWin;
+    if (i >= total) return;
+    const int w = i % nWin;
+    float x = pkx[i], y = pky[i], h = pkh[i];
+    const bool masked = mask && mask[w] != 0.0f;
+    const bool fit_fail = (x != x) || (y != y);
+    const bool large = (fabsf(x) > tx) || (fabsf(y) > ty);
+    if (masked || large) {
+        x = __builtin_nanf("");
+        y = __builtin_nanf("");
+        h = __builtin_nanf("");
+    }
+    if (pred) {
+        y += pred[i];
+        x += pred[(size_t)total + i];
+    }
+    const bool nm = (x != x) || (y != y) || masked;
+    u[i] = nm ? 0.0f : y;
+    u[(size_t)total + i] = nm ? 0.0f : x;
+    nanm[i] = nm ? 1 : 0;
+    int8_t r = 0;
+    if (fit_fail) r = 1;
+    if (large) r = 6;
+    reason[i] = r;
+    pkh_out[i] = h;
+}
+
+// insertion sort of n <= 24 floats in registers, then the median
+// (bottleneck nanmedian: NaNs dropped, mean of the middle pair)
+__device__ __forceinline__ float median_of(float *v, int n)
+{
+    if (n == 0) return __builtin_nanf("");
+    for (int a = 1; a < n; ++a) {
+        const float key = v[a];
+        int b = a - 1;
+        while (b >= 0 && v[b] > key) {
+            v[b + 1] = v[b];
+            --b;
+        }
+        v[b + 1] = key;
+    }
+    if (n & 1) return v[n / 2];
+    return 0.5f * (v[n / 2 - 1] + v[n / 2]);
+}
+
+// Normalised-median outlier test on u (planar, masked cells read as NaN),
+// then the end-of-stage bookkeeping of cpu_instantaneous.py:726-757.
+__global__ void median_kernel(const float *u, const float *pkh, const float *mask, int N, int ny, int nx,
+                              float eps, float thr, int size, uint8_t *nanm, int8_t *reason, float *uo,
+                              float *pkh_out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nWin = ny * nx, total = N * nWin;
+    if (i >= total) return;
+    const int n = i / nWin, w = i % nWin, r = w / nx, c = w % nx;
+    const float *uy = u + (size_t)n * nWin;
+    const float *ux = u + (size_t)total + (size_t)n * nWin;
+    auto is_masked = [&](int k) { return mask && mask[k] != 0.0f; };
+    auto vy = [&](int k) { return is_masked(k) ? __builtin_nanf("") : uy[k]; };
+    auto vx = [&](int k) { return is_masked(k) ? __builtin_nanf("") : ux[k]; };
+    const bool self_masked = is_masked(w);
+    bool outlier = false;
+    if (!self_masked) {
+        const int h = size / 2;
+        float nnx[24], nny[24], tx[24], ty[24];
+        int cnt = 0, cx = 0, cy = 0;
+        for (int dr = -h; dr <= h; ++dr)
+            for (int dc = -h; dc <= h; ++dc) {
+                if (dr == 0 && dc == 0) continue;
+                const int rr = r + dr, cc = c + dc;
+                float a = __builtin_nanf(""), b = __builtin_nanf("");
+                if (rr >= 0 && rr < ny && cc >= 0 && cc < nx) {
+                    a = vx(rr * nx + cc);
+                    b = vy(rr * nx + cc);
+                }
+                nnx[cnt] = a;
+                nny[cnt] = b;
+                ++cnt;
+                if (a == a) tx[cx++] = a;
+                if (b == b) ty[cy++] = b;
+            }
+        const float mu = median_of(tx, cx), mv = median_of(ty, cy);
+        const float x0 = vx(w), y0 = vy(w);
+        const float ddx = x0 - mu, ddy = y0 - mv;
+        const float r0 = sqrtf(ddx * ddx + ddy * ddy);
+        float ri[24];
+        int cr = 0;
+        for (int k = 0; k < cnt; ++k) {
+            const float ex = nnx[k] - mu, ey = nny[k] - mv;
+            const float v = sqrtf(ex * ex + ey * ey);
+            if (v == v) ri[cr++] = v;
+        }
+        const float rm = median_of(ri, cr);
+        const float nr = r0 / (rm + eps);
+        const bool exceeded = (nr > thr) || !isfinite(nr);
+        int present = 0;
+        for (int dr = -1; dr <= 1; ++dr)
+            for (int dc = -1; dc <= 1; ++dc) {
+                const int rr = r + dr, cc = c + dc;
+                if (rr < 0 || rr >= ny || cc < 0 || cc >= nx) continue;
+                const int k = rr * nx + cc;
+                const float a = vx(k), b = vy(k);
+                if ((isfinite(a) && isfinite(b)) || is_masked(k)) ++present;
+            }
+        outlier = exceeded || present < 6;
+    }
+    bool nm = nanm[i] != 0;
+    int8_t rs = reason[i];
+    if (outlier) {
+        nm = true;
+        if (rs == 0) rs = 10;
+    }
+    const float hv = pkh[i];
+    if (hv != hv) {
+        nm = true;
+        if (rs == 0) rs = 13;
+    }
+    if (nm && rs == 0) rs = 9;
+    if (self_masked) rs = -1;
+    nanm[i] = nm ? 1 : 0;
+    reason[i] = rs;
+    const float nanv = __builtin_nanf("");
+    uo[i] = self_masked ? 0.0f : (nm ? nanv : uy[w]);
+    uo[(size_t)total + i] = self_masked ? 0.0f : (nm ? nanv : ux[w]);
+    pkh_out[i] = nm ? nanv : hv;
+}
+
+// Nearest-donor fill of NaN cells (both components from the same donor).
+// Donors: finite in both components and not excluded (masked).  Distance
+// (dy*py)^2 + (dx*px)^2, ties to the smallest column, then the smallest row.
+// One lane per cell; a ring search stops once the ring is farther than the
+// best distance found.  A cell with no donor anywhere sets *err.
+__global__ void infill_kernel(const float *u, const float *mask, int N, int ny, int nx, float py, float px,
+                              float *uo, int *err)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nWin = ny * nx, total = N * nWin;
+    if (i >= total) return;
+    const int n = i / nWin, w = i % nWin, r = w / nx, c = w % nx;
+    const float *uy = u + (size_t)n * nWin;
+    const float *ux = u + (size_t)total + (size_t)n * nWin;
+    const float y0 = uy[w], x0 = ux[w];
+    uo[i] = y0;
+    uo[(size_t)total + i] = x0;
+    const bool fill = (y0 != y0) || (x0 != x0);
+    if (!fill) return;
+    double best = 1e300;
+    int br = -1, bc = -1;
+    const int maxring = ny > nx ? ny : nx;
+    for (int ring = 1; ring <= maxring; ++ring) {
+        // every cell of this ring is at least ring*min(py,px) away
+        const double lo = (double)ring * (double)(py < px ? py : px);
+        if (br >= 0 && lo * lo > best) break;
+        for (int dr = -ring; dr <= ring; ++dr) {
+            const int rr = r + dr;
+            if (rr < 0 || rr >= ny) continue;
+            const bool edge_row = (dr == -ring || dr == ring);
+            for (int dc = -ring; dc <= ring; dc += (edge_row ? 1 : 2 * ring)) {
+                const int cc = c + dc;
+                if (cc < 0 || cc >= nx) continue;
+                const int k = rr * nx + cc;
+                if (mask && mask[k] != 0.0f) continue;
+                const float a = uy[k], b = ux[k];
+                if (!(isfinite(a) && isfinite(b))) continue;
+                const double d = ((double)dr * py) * ((double)dr * py) + ((double)dc * px) * ((double)dc * px);
+                if (d < best || (d == best && (cc < bc || (cc == bc && rr < br)))) {
+                    best = d;
+                    br = rr;
+                    bc = cc;
+                }
+            }
+        }
+    }
+    if (br < 0) {
+        atomicOr(err, 1);
+        return;
+    }
+    uo[i] = uy[br * nx + bc];
+    uo[(size_t)total + i] = ux[br * nx + bc];
+}
+
+// np.pad(mode='edge') of the planar field (2, N, ny, nx) into (2, N, nyp, nxp).
+__global__ void pad_kernel(const float *u, int N, int ny, int nx, int pre_y, int pre_x, int nyp, int nxp,
+                           float *out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int per = nyp * nxp, total = 2 * N * per;
+    if (i >= total) return;
+    const int plane = i / per, k = i % per;
+    const int r = min(max(k / nxp - pre_y, 0), ny - 1);
+    const int c = min(max(k % nxp - pre_x, 0), nx - 1);
+    out[i] = u[(size_t)plane * ny * nx + r * nx + c];
+}
+
+// One axis of scipy.ndimage.gaussian_filter1d (mode 'nearest'): double
+// accumulation of the symmetric kernel in the order of scipy's symmetric
+// correlate1d (centre tap, then the paired taps from the outermost pair
+// inward), rounded to float32.
+__global__ void smooth_kernel(const float *in, int planes, int ny, int nx, int axis, const double *wts,
+                              int radius, float *out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int per = ny * nx, total = planes * per;
+    if (i >= total) return;
+    const int p = i / per, k = i % per, r = k / nx, c = k % nx;
+    const float *src = in + (size_t)p * per;
+    double s;
+    if (axis == 0) {
+        s = (double)src[r * nx + c] * wts[0];
+        for (int j = radius; j >= 1; --j) {
+            const int ra = min(max(r + j, 0), ny - 1), rb = min(max(r - j, 0), ny - 1);
+            s += ((double)src[ra * nx + c] + (double)src[rb * nx + c]) * wts[j];
+        }
+    } else {
+        s = (double)src[r * nx + c] * wts[0];
+        for (int j = radius; j >= 1; --j) {
+            const int ca = min(max(c + j, 0), nx - 1), cb = min(max(c - j, 0), nx - 1);
+            s += ((double)src[r * nx + ca] + (double)src[r * nx + cb]) * wts[j];
+        }
+    }
+    out[i] = (float)s;
+}
+
+__device__ __forceinline__ void cubic_coeffs(float x, float *w)
+{
+    const float A = -0.75f;
+    w[0] = ((A * (x + 1) - 5 * A) * (x + 1) + 8 * A) * (x + 1) - 4 * A;
+    w[1] = ((A + 2) * x - (A + 3)) * x * x + 1;
+    w[2] = ((A + 2) * (1 - x) - (A + 3)) * (1 - x) * (1 - x) + 1;
+    w[3] = 1.f - w[0] - w[1] - w[2];
+}
+
+// Bicubic remap of the planar field (2, N, nyo, nxo) at fractional indices
+// (map_y, map_x) of the new grid (ny, nx): coordinates rounded to 1/32 px
+// (round half to even), per-axis cubic weights on that lattice, replicate
+// border.
+__global__ void remap_kernel(const float *in, int N, int nyo, int nxo, const float *map_x, const float *map_y,
+                             int ny, int nx, float *out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int per = ny * nx, total = 2 * N * per;
+    if (i >= total) return;
+    const int plane = i / per, k = i % per;
+    const float *src = in + (size_t)plane * nyo * nxo;
+    const int X = __float2int_rn(map_x[k] * 32.0f);
+    const int Y = __float2int_rn(map_y[k] * 32.0f);
+    const int sx = (X >> 5) - 1, sy = (Y >> 5) - 1;
+    float wx[4], wy[4];
+    cubic_coeffs((float)(X & 31) * (1.0f / 32.0f), wx);
+    cubic_coeffs((float)(Y & 31) * (1.0f / 32.0f), wy);
+    float sum = 0.0f;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        const int rr = min(max(sy + a, 0), nyo - 1);
+        const float *row = src + (size_t)rr * nxo;
+        float rs = 0.0f;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int cc = min(max(sx + b, 0), nxo - 1);
+            rs += row[cc] * (wy[a] * wx[b]);
+        }
+        sum += rs;
+    }
+    out[i] = sum;
+}
+
+static inline unsigned nblk(size_t n) { return (unsigned)((n + 255) / 256); }
+
+}  // namespace pivamd
+
+using namespace pivamd;
+
+#define PIVAMD_LAUNCHED() (hipGetLastError() == hipSuccess ? 0 : PIVAMD_ERR_LAUNCH)
+
+extern "C" int pivamd_u16_to_f32(const void *d_in, float *d_out, size_t n, hipStream_t st)
+{
+    if (n == 0) return 0;
+    unsigned g = nblk(n);
+    if (g > 65536u * 8) g = 65536u * 8;
+    hipLaunchKernelGGL(u16_to_f32_kernel, dim3(g), dim3(256), 0, st, (const uint16_t *)d_in, d_out, n);
+    return PIVAMD_LAUNCHED();
+}
+
+extern "C" int pivamd_gate(const float *pkx, const float *pky, const float *pkh, const float *pred,
+                           const float *mask, int N, int nWin, float tx, float ty, float *u, uint8_t *nanm,
+                           int8_t *reason, float *pkh_out, hipStream_t st)
+{
+    const size_t n = (size_t)N * nWin;
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(gate_kernel, dim3(nblk(n)), dim3(256), 0, st, pkx, pky, pkh, pred, mask, N, nWin, tx, ty,
+                       u, nanm, reason, pkh_out);
+    return PIVAMD_LAUNCHED();
+}
+
+extern "C" int pivamd_median(const float *u, const float *pkh, const float *mask, int N, int ny, int nx,
+                             float eps, float thr, int size, uint8_t *nanm, int8_t *reason, float *uo,
+                             float *pkh_out, hipStream_t st)
+{
+    if (size != 3 && size != 5) return PIVAMD_ERR_ARG;
+    const size_t n = (size_t)N * ny * nx;
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(median_kernel, dim3(nblk(n)), dim3(256), 0, st, u, pkh, mask, N, ny, nx, eps, thr, size,
+                       nanm, reason, uo, pkh_out);
+    return PIVAMD_LAUNCHED();
+}
+
+extern "C" int pivamd_infill_nearest(const float *u, const float *mask, int N, int ny, int nx, float py,
+                                     float px, float *uo, int *d_err, hipStream_t st)
+{
+    const size_t n = (size_t)N * ny * nx;
+    if (n == 0) return 0;
+    if (!(py > 0.0f) || !(px > 0.0f)) return PIVAMD_ERR_ARG;
+    hipLaunchKernelGGL(infill_kernel, dim3(nblk(n)), dim3(256), 0, st, u, mask, N, ny, nx, py, px, uo, d_err);
+    return PIVAMD_LAUNCHED();
+}
+
+extern "C" int pivamd_pad_edge(const float *u, int N, int ny, int nx, int pre_y, int pre_x, int nyp, int nxp,
+                               float *out, hipStream_t st)
+{
+    const size_t n = (size_t)2 * N * nyp * nxp;
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(pad_kernel, dim3(nblk(n)), dim3(256), 0, st, u, N, ny, nx, pre_y, pre_x, nyp, nxp, out);
+    return PIVAMD_LAUNCHED();
+}
+
+extern "C" int pivamd_smooth_axis(const float *in, int planes, int ny, int nx, int axis, const double *d_wts,
+                                  int radius, float *out, hipStream_t st)
+{
+    const size_t n = (size_t)planes * ny * nx;
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(smooth_kernel, dim3(nblk(n)), dim3(256), 0, st, in, planes, ny, nx, axis, d_wts, radius,
+                       out);
+    return PIVAMD_LAUNCHED();
+}
+
+extern "C" int pivamd_remap_cubic(const float *in, int N, int nyo, int nxo, const float *map_x,
+                                  const float *map_y, int ny, int nx, float *out, hipStream_t st)
+{
+    const size_t n = (size_t)2 * N * ny * nx;
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(remap_kernel, dim3(nblk(n)), dim3(256), 0, st, in, N, nyo, nxo, map_x, map_y, ny, nx, out);
+    return PIVAMD_LAUNCHED();
+}

@@ -1,0 +1,265 @@
+"""Device-resident instantaneous multipass PIV.
+
+The pass loop of ``InstantaneousCorrelatorCPU.correlate_batch``
+(pivtools_cli/piv/piv_backend/cpu_instantaneous.py:381-927) with every step
+on the device: for pass p > 0 the previous pass's padded field is smoothed
+(scipy gaussian_filter, :1135-1147), remapped onto the new window grid as the
+predictor (cv2 bicubic remap, :1050-1071) and used to warp both frames
+(fused_symmetric_warp_batch, :1112-1131); the frames are correlated
+(bulkxcorr2d, :505-528); the result is gated (N/2 on the first pass, N/4
+after), the predictor added back, outliers flagged by the normalised median
+test, holes infilled by nearest donor and the field edge-padded into the next
+predictor (:555-865).
+
+Torch provides device memory and the stream; every computation is a HIP
+kernel of libpivtools_amd.so launched through its C ABI on torch's current
+stream.  Buffers are allocated once per (batch size, schedule) and reused.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _native
+from .window import PassGrid, build_pass_grids, window_weight
+
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+_f = ctypes.c_float
+_sz = ctypes.c_size_t
+
+_GLUE = {
+    "pivamd_u16_to_f32": [_vp, _vp, _sz, _vp],
+    "pivamd_gate": [_vp, _vp, _vp, _vp, _vp, _i, _i, _f, _f, _vp, _vp, _vp, _vp, _vp],
+    "pivamd_median": [_vp, _vp, _vp, _i, _i, _i, _f, _f, _i, _vp, _vp, _vp, _vp, _vp],
+    "pivamd_infill_nearest": [_vp, _vp, _i, _i, _i, _f, _f, _vp, _vp, _vp],
+    "pivamd_pad_edge": [_vp, _i, _i, _i, _i, _i, _i, _i, _vp, _vp],
+    "pivamd_smooth_axis": [_vp, _i, _i, _i, _i, _vp, _i, _vp, _vp],
+    "pivamd_remap_cubic": [_vp, _i, _i, _i, _vp, _vp, _i, _i, _vp, _vp],
+}
+_native.GLUE_SIGNATURES.update(_GLUE)
+
+PEAK_FINDERS = {"gauss3": 3, "gauss4": 4, "gauss5": 5, "gauss6": 6}
+
+
+def _ptr(t: torch.Tensor | None) -> int | None:
+    return None if t is None else t.data_ptr()
+
+
+def gaussian_weights_1d(sigma: float, truncate: float) -> tuple[np.ndarray, int]:
+    """Half kernel w[0..radius] of scipy.ndimage.gaussian_filter1d."""
+    radius = int(truncate * float(sigma) + 0.5)
+    x = np.arange(-radius, radius + 1)
+    phi = np.exp(-0.5 / (sigma * sigma) * x ** 2)
+    phi = phi / phi.sum()
+    return np.ascontiguousarray(phi[radius:], dtype=np.float64), radius
+
+
+@dataclass
+class PassResult:
+    """Per-pass device outputs (cpu_instantaneous.py:871-906 fields)."""
+
+    ux: torch.Tensor
+    uy: torch.Tensor
+    nan_mask: torch.Tensor
+    nan_reason: torch.Tensor
+    peak_mag: torch.Tensor
+    peak_ratio: torch.Tensor
+    predictor: torch.Tensor | None
+    window_size: tuple
+    n_windows: tuple
+
+
+@dataclass
+class _PassBuf:
+    grid: PassGrid
+    ctrs: dict = field(default_factory=dict)
+    w: torch.Tensor | None = None
+    pw: torch.Tensor | None = None
+    mask: torch.Tensor | None = None
+    out: dict = field(default_factory=dict)
+    work: torch.Tensor | None = None
+    maps: tuple | None = None
+    smooth: tuple | None = None
+
+
+class MultipassPIV:
+    """Multipass instantaneous PIV for batches of pairs of one image shape."""
+
+    def __init__(self, image_shape, window_sizes, overlaps, *, window_type="gaussian", peak_finder="gauss3",
+                 peak_fit_impl="scalar", warp_interpolation="cubic", predictor_smoothing=True,
+                 median=(0.1, 2.0, 5), final_infill="nearest", device="cuda"):
+        self.H, self.W = image_shape
+        self.window_sizes = [tuple(w) if np.ndim(w) else (w, w) for w in window_sizes]
+        if len(overlaps) != len(self.window_sizes):
+            raise ValueError("one overlap per pass")
+        self.overlaps = list(overlaps)
+        self.finder = PEAK_FINDERS[peak_finder] if isinstance(peak_finder, str) else int(peak_finder)
+        self.impl = 1 if peak_fit_impl == "batch" else 0
+        self.interp = 0 if warp_interpolation == "cubic" else 1
+        self.smoothing = bool(predictor_smoothing)
+        self.median = median
+        if final_infill not in ("nearest", "none"):
+            raise NotImplementedError(f"final-pass infill '{final_infill}' is not on the device path")
+        self.final_infill = final_infill
+        self.device = torch.device(device)
+        self.lib = _native.core()
+        for win in self.window_sizes:
+            if not self.lib.pivamd_size_supported(win[0], win[1]):
+                raise ValueError(f"window {win} has no device kernel")
+        self.grids = build_pass_grids((self.H, self.W), self.window_sizes, self.overlaps)
+        self.window_type = window_type
+        self._bufs: list[_PassBuf] | None = None
+        self._n = 0
+
+    # -- buffers ---------------------------------------------------------------
+    def _stream(self):
+        return torch.cuda.current_stream(self.device).cuda_stream
+
+    def _alloc(self, n):
+        dev, f32 = self.device, torch.float32
+        st = self._stream()
+        bufs = []
+        for p, g in enumerate(self.grids):
+            b = _PassBuf(g)
+            ny, nx = g.n_win
+            b.ctrs = {k: torch.as_tensor(v, device=dev) for k, v in
+                      (("x", g.ctrs_x), ("y", g.ctrs_y), ("xa", g.ctrs_x_all), ("ya", g.ctrs_y_all))}
+            w = window_weight(g.win, self.window_type)
+            b.w = torch.as_tensor(w, device=dev)
+            b.pw = torch.empty(g.win, dtype=f32, device=dev)
+            _native.check(self.lib.pivamd_plane_weight(_ptr(b.w), g.win[0], g.win[1], _ptr(b.pw), st),
+                          "plane weight")
+            shape = (n, ny, nx)
+            b.out = {k: torch.empty(shape, dtype=f32, device=dev)
+                     for k in ("pkx", "pky", "pkh", "sx", "sy", "sxy", "ratio", "pkh_g", "pkh_o")}
+            b.out["u"] = torch.empty((2,) + shape, dtype=f32, device=dev)
+            b.out["u2"] = torch.empty((2,) + shape, dtype=f32, device=dev)
+            b.out["u3"] = torch.empty((2,) + shape, dtype=f32, device=dev)
+            b.out["nanm"] = torch.empty(shape, dtype=torch.uint8, device=dev)
+            b.out["reason"] = torch.empty(shape, dtype=torch.int8, device=dev)
+            b.out["err"] = torch.zeros(1, dtype=torch.int32, device=dev)
+            nyp, nxp = len(g.ctrs_y_all), len(g.ctrs_x_all)
+            b.out["pad"] = torch.empty((2, n, nyp, nxp), dtype=f32, device=dev)
+            b.out["pad_s"] = torch.empty((2, n, nyp, nxp), dtype=f32, device=dev)
+            b.out["pad_t"] = torch.empty((2, n, nyp, nxp), dtype=f32, device=dev)
+            ws = self.lib.pivamd_xcorr_workspace(n, ny, nx, g.win[0], g.win[1], 1, 0)
+            b.work = torch.empty(max(ws, 4), dtype=torch.uint8, device=dev)
+            if p > 0:
+                b.maps = (torch.as_tensor(g.pred_map_x, device=dev), torch.as_tensor(g.pred_map_y, device=dev))
+                b.out["pred"] = torch.empty((2,) + shape, dtype=f32, device=dev)
+                if self.smoothing and g.sd > 0:
+                    trunc = (g.ksize[0] - 1) / (2 * g.sd)
+                    wts, rad = gaussian_weights_1d(g.sd, trunc)
+                    b.smooth = (torch.as_tensor(wts, device=dev), rad)
+            bufs.append(b)
+        self.imgf = torch.empty((2, n, self.H, self.W), dtype=f32, device=dev)
+        self.warped = torch.empty((2, n, self.H, self.W), dtype=f32, device=dev)
+        self._bufs = bufs
+        self._n = n
+
+    # -- one batch ---------------------------------------------------------------
+    def run(self, img_a: torch.Tensor, img_b: torch.Tensor, keep_passes: bool = False, events=None):
+        """Correlate a batch of pairs (N, H, W) already on the device.
+
+        ``img_a``/``img_b`` are float32, or 16-bit counts (uint16 data in an
+        int16 or uint16 tensor).  Returns the list of ``PassResult`` (all
+        passes if ``keep_passes``, else the final one) as device tensors.
+        ``events`` (optional list) receives, per pass, a pair of recorded
+        torch.cuda.Events bracketing the correlation launch on this stream.
+        """
+        if img_a.shape != img_b.shape or img_a.dim() != 3 or tuple(img_a.shape[1:]) != (self.H, self.W):
+            raise ValueError(f"expected two (N, {self.H}, {self.W}) stacks, got {tuple(img_a.shape)}")
+        n = int(img_a.shape[0])
+        if self._bufs is None or self._n != n:
+            self._alloc(n)
+        st = self._stream()
+        L = self.lib
+        npix = n * self.H * self.W
+        if img_a.dtype == torch.float32:
+            A, B = img_a.contiguous(), img_b.contiguous()
+        elif img_a.dtype in (torch.int16, torch.uint16):
+            _native.check(L.pivamd_u16_to_f32(_ptr(img_a.contiguous()), _ptr(self.imgf[0]), npix, st), "ingest")
+            _native.check(L.pivamd_u16_to_f32(_ptr(img_b.contiguous()), _ptr(self.imgf[1]), npix, st), "ingest")
+            A, B = self.imgf[0], self.imgf[1]
+        else:
+            raise TypeError(f"unsupported image dtype {img_a.dtype}")
+
+        results = []
+        prev = None
+        for p, b in enumerate(self._bufs):
+            g = b.grid
+            ny, nx = g.n_win
+            o = b.out
+            final = p == len(self._bufs) - 1
+            pred = None
+            if p == 0:
+                a_in, b_in = A, B
+            else:
+                field = prev.out["pad"]
+                nyp, nxp = field.shape[2], field.shape[3]
+                if b.smooth is not None:
+                    wts, rad = b.smooth
+                    _native.check(L.pivamd_smooth_axis(_ptr(field), 2 * n, nyp, nxp, 0, _ptr(wts), rad,
+                                                       _ptr(prev.out["pad_t"]), st), "smooth")
+                    _native.check(L.pivamd_smooth_axis(_ptr(prev.out["pad_t"]), 2 * n, nyp, nxp, 1, _ptr(wts), rad,
+                                                       _ptr(prev.out["pad_s"]), st), "smooth")
+                    field = prev.out["pad_s"]
+                pred = o["pred"]
+                _native.check(L.pivamd_remap_cubic(_ptr(field), n, nyp, nxp, _ptr(b.maps[0]), _ptr(b.maps[1]),
+                                                   ny, nx, _ptr(pred), st), "predictor remap")
+                _native.check(L.pivamd_warp(_ptr(A), _ptr(B), _ptr(self.warped[0]), _ptr(self.warped[1]),
+                                            _ptr(field[0]), _ptr(field[1]), n, self.H, self.W, nyp, nxp,
+                                            _ptr(prev.ctrs["ya"]), _ptr(prev.ctrs["xa"]), self.interp, 0, 0, st),
+                              "warp")
+                a_in, b_in = self.warped[0], self.warped[1]
+            if events is not None:
+                ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+                ev[0].record()
+            _native.check(L.pivamd_xcorr(_ptr(a_in), _ptr(b_in), None, self.H, self.W, n, _ptr(b.ctrs["x"]),
+                                         _ptr(b.ctrs["y"]), ny, nx, _ptr(b.w), _ptr(b.w), _ptr(b.pw), 0,
+                                         g.win[0], g.win[1], 1, self.finder, self.impl, _ptr(o["pkx"]),
+                                         _ptr(o["pky"]), _ptr(o["pkh"]), _ptr(o["sx"]), _ptr(o["sy"]),
+                                         _ptr(o["sxy"]), _ptr(o["ratio"]), None, _ptr(b.work), b.work.numel(), st),
+                          "bulkxcorr2d")
+            if events is not None:
+                ev[1].record()
+                events.append((p, ev))
+            div = 2.0 if p == 0 else 4.0
+            _native.check(L.pivamd_gate(_ptr(o["pkx"]), _ptr(o["pky"]), _ptr(o["pkh"]), _ptr(pred), None, n,
+                                        ny * nx, g.win[1] / div, g.win[0] / div, _ptr(o["u"]), _ptr(o["nanm"]),
+                                        _ptr(o["reason"]), _ptr(o["pkh_g"]), st), "gate")
+            if self.median is not None:
+                eps, thr, size = self.median
+                _native.check(L.pivamd_median(_ptr(o["u"]), _ptr(o["pkh_g"]), None, n, ny, nx, eps, thr, size,
+                                              _ptr(o["nanm"]), _ptr(o["reason"]), _ptr(o["u2"]), _ptr(o["pkh_o"]),
+                                              st), "median test")
+                u = o["u2"]
+            else:
+                raise NotImplementedError("the device pass loop runs with the median test")
+            if not final or self.final_infill == "nearest":
+                o["err"].zero_()
+                _native.check(L.pivamd_infill_nearest(_ptr(u), None, n, ny, nx, float(g.spacing[0]),
+                                                      float(g.spacing[1]), _ptr(o["u3"]), _ptr(o["err"]), st),
+                              "infill")
+                u = o["u3"]
+            if not final:
+                nyp, nxp = len(g.ctrs_y_all), len(g.ctrs_x_all)
+                _native.check(L.pivamd_pad_edge(_ptr(u), n, ny, nx, g.n_pre[0], g.n_pre[1], nyp, nxp,
+                                                _ptr(o["pad"]), st), "pad")
+            if keep_passes or final:
+                results.append(PassResult(ux=u[1], uy=u[0], nan_mask=o["nanm"], nan_reason=o["reason"],
+                                          peak_mag=o["pkh_o"], peak_ratio=o["ratio"], predictor=pred,
+                                          window_size=g.win, n_windows=(ny, nx)))
+            prev = b
+        return results
+
+    def check_infill(self) -> None:
+        """Raise if any infill found no donor (a fully invalid field)."""
+        for b in self._bufs or []:
+            if int(b.out["err"].item()) != 0:
+                raise ValueError("infilling has no donor points: 100% of vectors invalid")
