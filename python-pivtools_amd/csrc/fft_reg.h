@@ -177,3 +177,90 @@ __device__ __forceinline__ void fft_reg(float (&xr)[N], float (&xi)[N])
 }
 
 }  // namespace pivamd
+
+namespace pivamd {
+
+// ---------------------------------------------------------------------------
+// One transform of length N shared by R lanes (R = 1, 2 or 4, the R lanes of
+// a line are consecutive and quad-aligned).
+//   in : lane q holds x[n1*R + q], n1 = 0..N/R-1
+//   out: lane q holds X[k1 + (N/R) * out_block(q)], k1 = 0..N/R-1
+// X[k1 + N1 k2] = sum_n2 W_R^(n2 k2) W_N^(n2 k1) FFT_N1(x[. R + n2])[k1]:
+// each lane does the N1-point sub-transform in registers, applies its
+// twiddle W_N^(q k1), then the radix-R combination runs across the R lanes
+// as R = 2: one DPP exchange; R = 4: two exchange stages (DIF, so the
+// blocks come out bit-reversed).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float dpp_swap1(float v)
+{
+    // quad_perm [1,0,3,2]
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, false));
+}
+
+__device__ __forceinline__ float dpp_swap2(float v)
+{
+    // quad_perm [2,3,0,1]
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4E, 0xF, 0xF, false));
+}
+
+template <int R>
+__device__ __forceinline__ int out_block(int q)
+{
+    if constexpr (R == 4) return ((q & 1) << 1) | (q >> 1);
+    else return q;
+}
+
+template <int N, int R, bool INV>
+__device__ __forceinline__ void fft_split(float (&xr)[N / R], float (&xi)[N / R], int q)
+{
+    constexpr int N1 = N / R;
+    fft_reg<N1, INV>(xr, xi);
+    if constexpr (R > 1) {
+        static_for<1, N1>([&](auto K) {
+            constexpr int k1 = decltype(K)::value;
+            constexpr float c1 = (float)ct_cos2pi(k1, N), s1 = (float)(INV ? ct_sin2pi(k1, N) : -ct_sin2pi(k1, N));
+            float c, s;
+            if constexpr (R == 2) {
+                c = q ? c1 : 1.0f;
+                s = q ? s1 : 0.0f;
+            } else {
+                constexpr float c2 = (float)ct_cos2pi(2 * k1, N);
+                constexpr float s2 = (float)(INV ? ct_sin2pi(2 * k1, N) : -ct_sin2pi(2 * k1, N));
+                constexpr float c3 = (float)ct_cos2pi(3 * k1, N);
+                constexpr float s3 = (float)(INV ? ct_sin2pi(3 * k1, N) : -ct_sin2pi(3 * k1, N));
+                c = q == 0 ? 1.0f : (q == 1 ? c1 : (q == 2 ? c2 : c3));
+                s = q == 0 ? 0.0f : (q == 1 ? s1 : (q == 2 ? s2 : s3));
+            }
+            const float r = xr[k1], i = xi[k1];
+            xr[k1] = r * c - i * s;
+            xi[k1] = r * s + i * c;
+        });
+        if constexpr (R == 2) {
+            const float sg = (q & 1) ? -1.0f : 1.0f;
+#pragma unroll
+            for (int k = 0; k < N1; ++k) {
+                const float orr = dpp_swap1(xr[k]), oi = dpp_swap1(xi[k]);
+                xr[k] = orr + sg * xr[k];
+                xi[k] = oi + sg * xi[k];
+            }
+        } else {
+            const float sa = (q & 2) ? -1.0f : 1.0f;
+            const float sb = (q & 1) ? -1.0f : 1.0f;
+            const bool rot = (q == 3);
+#pragma unroll
+            for (int k = 0; k < N1; ++k) {
+                const float orr = dpp_swap2(xr[k]), oi = dpp_swap2(xi[k]);
+                float tr = orr + sa * xr[k], ti = oi + sa * xi[k];
+                // lane 3 multiplies by W_4 = -i (forward) or +i (inverse)
+                const float rr = INV ? -ti : ti, ri = INV ? tr : -tr;
+                tr = rot ? rr : tr;
+                ti = rot ? ri : ti;
+                const float pr = dpp_swap1(tr), pi = dpp_swap1(ti);
+                xr[k] = pr + sb * tr;
+                xi[k] = pi + sb * ti;
+            }
+        }
+    }
+}
+
+}  // namespace pivamd

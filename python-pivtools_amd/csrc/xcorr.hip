@@ -45,12 +45,17 @@ struct XcorrArgs {
 
 constexpr int kMaxLds = 150 * 1024;
 
-template <int NY, int NX>
+// Launch geometry of one square window size N: R lanes share each line
+// (row or column) transform, T = N*R lanes per window, WPB windows per
+// workgroup as LDS allows (one complex N x (N+1) tile per window).
+template <int N>
 struct XcCfg {
-    static constexpr int T = NY > NX ? NY : NX;
-    static constexpr int PX = NX + 1;
-    static constexpr int tile = NY * PX;
-    static constexpr int lds_win = 2 * tile * 4 + 6 * T * 4;
+    static constexpr int R = N >= 96 ? 4 : (N >= 48 ? 2 : 1);
+    static constexpr int N1 = N / R;
+    static constexpr int T = N * R;
+    static constexpr int PX = N + 1;
+    static constexpr int tile = N * PX;
+    static constexpr int lds_win = 2 * tile * 4 + 5 * T * 4;
     static constexpr int wpb_thr = (256 / T) > 0 ? (256 / T) : 1;
     static constexpr int wpb_lds = kMaxLds / lds_win;
     static constexpr int WPB = wpb_thr < wpb_lds ? wpb_thr : wpb_lds;
@@ -58,8 +63,9 @@ struct XcCfg {
 };
 
 // Sum of (a, b) over the T lanes of one window; every lane gets the same
-// bit pattern.  T | 64: xor butterfly inside the wave.  Otherwise through
-// LDS, each lane summing the T partials in lane order.
+// bits.  T | 64: xor butterfly inside the wave.  64 | T: butterfly per wave,
+// then the per-wave partials through LDS in wave order.  Otherwise every
+// lane sums the T partials from LDS in lane order.
 template <int T>
 __device__ __forceinline__ void win_sum2(float &a, float &b, float *ra, float *rb, int t)
 {
@@ -69,6 +75,26 @@ __device__ __forceinline__ void win_sum2(float &a, float &b, float *ra, float *r
             a += __shfl_xor(a, o, T);
             b += __shfl_xor(b, o, T);
         }
+    } else if constexpr (T % 64 == 0) {
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            a += __shfl_xor(a, o, 64);
+            b += __shfl_xor(b, o, 64);
+        }
+        if ((t & 63) == 0) {
+            ra[t >> 6] = a;
+            rb[t >> 6] = b;
+        }
+        __syncthreads();
+        float sa = 0.0f, sb = 0.0f;
+#pragma unroll
+        for (int k = 0; k < T / 64; ++k) {
+            sa += ra[k];
+            sb += rb[k];
+        }
+        __syncthreads();
+        a = sa;
+        b = sb;
     } else {
         ra[t] = a;
         rb[t] = b;
@@ -83,6 +109,37 @@ __device__ __forceinline__ void win_sum2(float &a, float &b, float *ra, float *r
         b = sb;
     }
 }
+
+// Candidate summary of a set of plane pixels: the largest 3x3 local maximum
+// (first in row-major candidate order on ties), the two largest candidate
+// values (detectability) and whether a NaN was seen.  merge() is
+// commutative and associative and only selects, so any reduction order
+// gives the same bits.
+struct Cand {
+    float v;
+    int c;
+    float p1, p2;
+    int nan;
+    __device__ void merge(const Cand &o)
+    {
+        if (o.v > v || (o.v == v && o.v > 0.0f && o.c < c)) { v = o.v; c = o.c; }
+        const float n1 = fmaxf(p1, o.p1);
+        const float n2 = fmaxf(fminf(p1, o.p1), fmaxf(p2, o.p2));
+        p1 = n1;
+        p2 = n2;
+        nan |= o.nan;
+    }
+    __device__ Cand shfl(int m, int w) const
+    {
+        Cand o;
+        o.v = __shfl_xor(v, m, w);
+        o.c = __shfl_xor(c, m, w);
+        o.p1 = __shfl_xor(p1, m, w);
+        o.p2 = __shfl_xor(p2, m, w);
+        o.nan = __shfl_xor(nan, m, w);
+        return o;
+    }
+};
 
 __device__ __forceinline__ int remap_block(int b, int nb)
 {
@@ -343,27 +400,30 @@ __device__ void reduce_candidates(const float *r0, const float *r1, const float 
     *h_out = h; *i_out = i0; *j_out = j0; *det_out = det;
 }
 
-template <int NY, int NX>
-__global__ void __launch_bounds__((XcCfg<NY, NX>::BLOCK))
+template <int N>
+__global__ void __launch_bounds__((XcCfg<N>::BLOCK))
 xcorr_kernel(XcorrArgs g)
 {
-    using C = XcCfg<NY, NX>;
-    constexpr int T = C::T, PX = C::PX, WPB = C::WPB;
-    constexpr int numel = NY * NX;
+    using C = XcCfg<N>;
+    constexpr int R = C::R, N1 = C::N1, T = C::T, PX = C::PX, WPB = C::WPB;
+    constexpr int numel = N * N;
     __shared__ float s_re[WPB * C::tile];
     __shared__ float s_im[WPB * C::tile];
-    __shared__ float s_red[6][WPB * T];
+    __shared__ float s_red[5][WPB * T];
 
     const int b = remap_block((int)blockIdx.x, g.nblocks);
     if (b < 0) return;    // whole block exits together
 
     const int wloc = (int)threadIdx.x / T;
     const int t = (int)threadIdx.x - wloc * T;
+    const int l = t / R;              // line (row or column) of this lane
+    const int q = t - l * R;          // lane within the line
+    const int ob = out_block<R>(q);   // output block of fft_split
     float *Sr = s_re + wloc * C::tile;
     float *Si = s_im + wloc * C::tile;
     float *r0 = &s_red[0][wloc * T], *r1 = &s_red[1][wloc * T];
     float *r2 = &s_red[2][wloc * T], *r3 = &s_red[3][wloc * T];
-    float *r4 = &s_red[4][wloc * T], *r5 = &s_red[5][wloc * T];
+    float *r4 = &s_red[4][wloc * T];
 
     const int nWin = g.nWy * g.nWx;
     const long long gw = (long long)b * WPB + wloc;
@@ -374,133 +434,132 @@ xcorr_kernel(XcorrArgs g)
         w = (int)(gw - (long long)n * nWin);
         const int ii = w % g.nWx, jj = w / g.nWx;
         if (g.mask && g.mask[jj * g.nWx + ii] == 1.0f) valid = false;
-        row0 = (int)floor((double)g.ctrY[jj] - ((double)(float)NY - 1.0) / 2.0 + 0.5);
-        col0 = (int)floor((double)g.ctrX[ii] - ((double)(float)NX - 1.0) / 2.0 + 0.5);
-        if (row0 < 0 || col0 < 0 || row0 + NY > g.H || col0 + NX > g.W) valid = false;
+        row0 = (int)floor((double)g.ctrY[jj] - ((double)(float)N - 1.0) / 2.0 + 0.5);
+        col0 = (int)floor((double)g.ctrX[ii] - ((double)(float)N - 1.0) / 2.0 + 0.5);
+        if (row0 < 0 || col0 < 0 || row0 + N > g.H || col0 + N > g.W) valid = false;
     }
 
-    // ---- 1. column load, weight, mean, energy, forward column FFT --------
-    float zr[NY], zi[NY];
+    // ---- 1. column x = l: rows n1*R + q, weight, mean, energy, FFT --------
+    float zr[N1], zi[N1];
     float sa = 0.0f, sb = 0.0f;
-    const bool colLane = valid && t < NX;
-    if (colLane) {
+    if (valid) {
         const size_t img = (size_t)n * g.H * g.W;
-        const float *pa = g.imgA + img + (size_t)row0 * g.W + col0 + t;
-        const float *pb = g.imgB + img + (size_t)row0 * g.W + col0 + t;
+        const float *pa = g.imgA + img + (size_t)(row0 + q) * g.W + col0 + l;
+        const float *pb = g.imgB + img + (size_t)(row0 + q) * g.W + col0 + l;
+        const float *wa = g.wA + q * N + l, *wb = g.wB + q * N + l;
 #pragma unroll
-        for (int y = 0; y < NY; ++y) {
-            const float a = pa[(size_t)y * g.W] * g.wA[y * NX + t];
-            const float bb = pb[(size_t)y * g.W] * g.wB[y * NX + t];
-            zr[y] = a;
-            zi[y] = bb;
+        for (int k = 0; k < N1; ++k) {
+            const float a = pa[(size_t)k * R * g.W] * wa[k * R * N];
+            const float bb = pb[(size_t)k * R * g.W] * wb[k * R * N];
+            zr[k] = a;
+            zi[k] = bb;
             sa += a;
             sb += bb;
         }
     } else {
 #pragma unroll
-        for (int y = 0; y < NY; ++y) { zr[y] = 0.0f; zi[y] = 0.0f; }
+        for (int k = 0; k < N1; ++k) { zr[k] = 0.0f; zi[k] = 0.0f; }
     }
     win_sum2<T>(sa, sb, r0, r1, t);
     const float meanA = sa / (float)numel;
     const float meanB = sb / (float)numel;
     float ea = 0.0f, eb = 0.0f;
-    if (!g.ensemble) {
-        if (colLane) {
+    if (valid) {
+        if (!g.ensemble) {
 #pragma unroll
-            for (int y = 0; y < NY; ++y) {
-                zr[y] -= meanA;
-                zi[y] -= meanB;
-                ea += zr[y] * zr[y];
-                eb += zi[y] * zi[y];
+            for (int k = 0; k < N1; ++k) {
+                zr[k] -= meanA;
+                zi[k] -= meanB;
+                ea += zr[k] * zr[k];
+                eb += zi[k] * zi[k];
             }
-        }
-    } else if (colLane) {
+        } else {
 #pragma unroll
-        for (int y = 0; y < NY; ++y) {
-            ea += zr[y] * zr[y];
-            eb += zi[y] * zi[y];
+            for (int k = 0; k < N1; ++k) {
+                ea += zr[k] * zr[k];
+                eb += zi[k] * zi[k];
+            }
         }
     }
     win_sum2<T>(ea, eb, r0, r1, t);
     const float enorm = 1.0f / sqrtf(ea * eb);
 
-    if (colLane) {
-        fft_reg<NY, false>(zr, zi);
+    if (valid) {
+        fft_split<N, R, false>(zr, zi, q);
 #pragma unroll
-        for (int y = 0; y < NY; ++y) {
-            Sr[y * PX + t] = zr[y];
-            Si[y * PX + t] = zi[y];
+        for (int k = 0; k < N1; ++k) {
+            Sr[(k + N1 * ob) * PX + l] = zr[k];
+            Si[(k + N1 * ob) * PX + l] = zi[k];
         }
     }
     __syncthreads();
 
-    // ---- 2. forward row FFT ----------------------------------------------
-    if (valid && t < NY) {
-        float xr[NX], xi[NX];
+    // ---- 2. row ky = l --------------------------------------------------------
+    if (valid) {
+        float xr[N1], xi[N1];
 #pragma unroll
-        for (int x = 0; x < NX; ++x) {
-            xr[x] = Sr[t * PX + x];
-            xi[x] = Si[t * PX + x];
+        for (int k = 0; k < N1; ++k) {
+            xr[k] = Sr[l * PX + k * R + q];
+            xi[k] = Si[l * PX + k * R + q];
         }
-        fft_reg<NX, false>(xr, xi);
+        fft_split<N, R, false>(xr, xi, q);
 #pragma unroll
-        for (int x = 0; x < NX; ++x) {
-            Sr[t * PX + x] = xr[x];
-            Si[t * PX + x] = xi[x];
+        for (int k = 0; k < N1; ++k) {
+            Sr[l * PX + k + N1 * ob] = xr[k];
+            Si[l * PX + k + N1 * ob] = xi[k];
         }
     }
     __syncthreads();
 
-    // ---- 3. cross spectrum of column kx, inverse column FFT --------------
-    if (valid && t <= NX / 2) {
-        const int mx = (NX - t) % NX;
-        float cr[NY], ci[NY];
+    // ---- 3. column kx = l <= N/2: C = FB conj(FA), inverse column FFT ------
+    if (valid && l <= N / 2) {
+        const int mx = (N - l) % N;
+        float cr[N1], ci[N1];
 #pragma unroll
-        for (int ky = 0; ky < NY; ++ky) {
-            const int my = (NY - ky) % NY;
-            const float zR = Sr[ky * PX + t], zI = Si[ky * PX + t];
+        for (int k = 0; k < N1; ++k) {
+            const int ky = k * R + q;
+            const int my = (N - ky) % N;
+            const float zR = Sr[ky * PX + l], zI = Si[ky * PX + l];
             const float mR = Sr[my * PX + mx], mI = Si[my * PX + mx];
             // FA = (Z + conj(Zm)) / 2 ; FB = (Z - conj(Zm)) / (2i)
             const float ar = 0.5f * (zR + mR), ai = 0.5f * (zI - mI);
             const float br = 0.5f * (zI + mI), bi = -0.5f * (zR - mR);
-            // C = FB * conj(FA)
-            cr[ky] = br * ar + bi * ai;
-            ci[ky] = bi * ar - br * ai;
+            cr[k] = br * ar + bi * ai;
+            ci[k] = bi * ar - br * ai;
         }
-        fft_reg<NY, true>(cr, ci);
+        fft_split<N, R, true>(cr, ci, q);
 #pragma unroll
-        for (int y = 0; y < NY; ++y) {
-            Sr[y * PX + t] = cr[y];
-            Si[y * PX + t] = ci[y];
+        for (int k = 0; k < N1; ++k) {
+            Sr[(k + N1 * ob) * PX + l] = cr[k];
+            Si[(k + N1 * ob) * PX + l] = ci[k];
         }
     }
     __syncthreads();
 
-    // ---- 4. Hermitian rows -> real rows, scale, fftshift into the plane ---
-    if (valid && t < NY / 2) {
-        const int ya = t, yb = t + NY / 2;
-        float xr[NX], xi[NX];
+    // ---- 4. rows j and j + N/2 (j = l < N/2): Hermitian rows -> real rows -----
+    if (valid && l < N / 2) {
+        const int ya = l, yb = l + N / 2;
+        float xr[N1], xi[N1];
 #pragma unroll
-        for (int k = 0; k <= NX / 2; ++k) {
-            float g1r = Sr[ya * PX + k], g1i = Si[ya * PX + k];
-            float g2r = Sr[yb * PX + k], g2i = Si[yb * PX + k];
-            if (k == 0 || k == NX / 2) { g1i = 0.0f; g2i = 0.0f; }
+        for (int k = 0; k < N1; ++k) {
+            const int kx = k * R + q;
+            const bool up = kx > N / 2;
+            const int kk = up ? N - kx : kx;
+            float g1r = Sr[ya * PX + kk], g1i = Si[ya * PX + kk];
+            float g2r = Sr[yb * PX + kk], g2i = Si[yb * PX + kk];
+            if (kk == 0 || kk == N / 2) { g1i = 0.0f; g2i = 0.0f; }
+            if (up) { g1i = -g1i; g2i = -g2i; }     // G(N-k) = conj(G(k)) per real row
             xr[k] = g1r - g2i;
             xi[k] = g1i + g2r;
-            if (k != 0 && k != NX / 2) {
-                // G(NX-k) = conj(G(k)) for each real row
-                xr[NX - k] = g1r + g2i;
-                xi[NX - k] = -g1i + g2r;
-            }
         }
-        fft_reg<NX, true>(xr, xi);
+        fft_split<N, R, true>(xr, xi, q);
         const float mul = 1.0f / (float)numel;
-        // plane[(y + NY/2) % NY][(x + NX/2) % NX] = c[y][x] / numel
+        // plane[(y + N/2) % N][(x + N/2) % N] = c[y][x] / numel
 #pragma unroll
-        for (int x = 0; x < NX; ++x) {
-            const int xs = (x + NX / 2) % NX;
-            Sr[yb * PX + xs] = xr[x] * mul;
-            Sr[ya * PX + xs] = xi[x] * mul;
+        for (int k = 0; k < N1; ++k) {
+            const int xs = (k + N1 * ob + N / 2) % N;
+            Sr[yb * PX + xs] = xr[k] * mul;
+            Sr[ya * PX + xs] = xi[k] * mul;
         }
     }
     __syncthreads();
@@ -508,7 +567,7 @@ xcorr_kernel(XcorrArgs g)
     const float *P = Sr;
     if (g.planes && valid) {
         float *dst = g.planes + (size_t)gw * numel;
-        for (int e = t; e < numel; e += T) dst[e] = P[(e / NX) * PX + (e % NX)];
+        for (int e = t; e < numel; e += T) dst[e] = P[(e / N) * PX + (e % N)];
     }
     if (g.ensemble) return;    // uniform: the flag is a kernel argument
 
@@ -517,7 +576,7 @@ xcorr_kernel(XcorrArgs g)
         // multi-peak: the plane goes to the workspace for multipeak_kernel
         if (valid) {
             float *dst = g.work_planes + (size_t)gw * numel;
-            for (int e = t; e < numel; e += T) dst[e] = P[(e / NX) * PX + (e % NX)];
+            for (int e = t; e < numel; e += T) dst[e] = P[(e / N) * PX + (e % N)];
         }
         if (t == 0 && gw < g.total) {
             float *rec = g.rec + (size_t)gw * kRec;
@@ -526,14 +585,13 @@ xcorr_kernel(XcorrArgs g)
         }
         return;
     }
-    constexpr int CW = NX / 2, NC = (NY / 2) * (NX / 2);
-    float bv = 0.0f, p1 = 0.0f, p2 = 0.0f, nan_seen = 0.0f;
-    int bc = 0x7fffffff;
+    constexpr int CW = N / 2, NC = (N / 2) * (N / 2);
+    Cand cd = {0.0f, 0x7fffffff, 0.0f, 0.0f, 0};
     if (valid) {
         for (int c = t; c < NC; c += T) {
-            const int i = NY / 4 + c / CW, j = NX / 4 + c % CW;
+            const int i = N / 4 + c / CW, j = N / 4 + c % CW;
             const float v = P[i * PX + j];
-            if (v != v) { nan_seen = 1.0f; continue; }
+            if (v != v) { cd.nan = 1; continue; }
             if (!(v > 0.0f)) continue;
             bool is_max = true;
 #pragma unroll
@@ -542,31 +600,63 @@ xcorr_kernel(XcorrArgs g)
                 for (int dj = -1; dj <= 1; ++dj)
                     if (P[(i + di) * PX + (j + dj)] > v) is_max = false;
             if (!is_max) continue;
-            if (v > bv) { bv = v; bc = c; }
-            if (v > p1) { p2 = p1; p1 = v; }
-            else if (v > p2) p2 = v;
+            if (v > cd.v) { cd.v = v; cd.c = c; }
+            if (v > cd.p1) { cd.p2 = cd.p1; cd.p1 = v; }
+            else if (v > cd.p2) cd.p2 = v;
         }
     }
-    r0[t] = bv;
-    r1[t] = __int_as_float(bc);
-    r2[t] = p1;
-    r3[t] = p2;
-    r4[t] = nan_seen;
-    __syncthreads();
+    if constexpr (64 % T == 0) {
+#pragma unroll
+        for (int o = 1; o < T; o <<= 1) cd.merge(cd.shfl(o, T));
+    } else if constexpr (T % 64 == 0) {
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) cd.merge(cd.shfl(o, 64));
+        if ((t & 63) == 0) {
+            r0[t >> 6] = cd.v;
+            r1[t >> 6] = __int_as_float(cd.c);
+            r2[t >> 6] = cd.p1;
+            r3[t >> 6] = cd.p2;
+            r4[t >> 6] = (float)cd.nan;
+        }
+        __syncthreads();
+        if (t == 0) {
+            for (int k = 1; k < T / 64; ++k) {
+                Cand o = {r0[k], __float_as_int(r1[k]), r2[k], r3[k], r4[k] != 0.0f};
+                cd.merge(o);
+            }
+        }
+    } else {
+        r0[t] = cd.v;
+        r1[t] = __int_as_float(cd.c);
+        r2[t] = cd.p1;
+        r3[t] = cd.p2;
+        r4[t] = (float)cd.nan;
+        __syncthreads();
+        if (t == 0) {
+            for (int k = 1; k < T; ++k) {
+                Cand o = {r0[k], __float_as_int(r1[k]), r2[k], r3[k], r4[k] != 0.0f};
+                cd.merge(o);
+            }
+        }
+    }
     if (t == 0 && gw < g.total) {
         float *rec = g.rec + (size_t)gw * kRec;
         if (!valid) {
             rec[kRecFlag] = 0.0f;
         } else {
-            float h, det;
-            int i0, j0;
-            reduce_candidates(r0, r1, r2, r3, r4, T, NY, NX, P, PX, &h, &i0, &j0, &det);
+            float h = cd.v, det = (cd.p2 > 0.0f) ? cd.p1 / cd.p2 : 0.0f;
+            int i0 = 0, j0 = 0;
+            if (cd.c != 0x7fffffff) { i0 = N / 4 + cd.c / CW; j0 = N / 4 + cd.c % CW; }
+            if (cd.nan) {
+                seq_search(P, N, N, PX, &h, &i0, &j0);
+                det = seq_detect(P, N, N, PX);
+            }
             g.ratio[(size_t)n * nWin + w] = det;
-            const bool gate = (h <= 0.0f) || i0 < 2 || i0 >= NY - 2 || j0 < 2 || j0 >= NX - 2;
+            const bool gate = (h <= 0.0f) || i0 < 2 || i0 >= N - 2 || j0 < 2 || j0 >= N - 2;
             if (gate) {
                 const size_t o = (size_t)n * nWin + w;
-                g.pkX[o] = __builtin_nanf("") - NX / 2.0f;
-                g.pkY[o] = __builtin_nanf("") - NY / 2.0f;
+                g.pkX[o] = __builtin_nanf("") - N / 2.0f;
+                g.pkY[o] = __builtin_nanf("") - N / 2.0f;
                 g.sx[o] = 0.0f; g.sy[o] = 0.0f; g.sxy[o] = 0.0f;
                 g.pkH[o] = 0.0f * enorm;
                 rec[kRecFlag] = 0.0f;
@@ -577,7 +667,7 @@ xcorr_kernel(XcorrArgs g)
                     for (int j = 0; j < PK; ++j) {
                         const int r = i0 + i - 2, c = j0 + j - 2;
                         const float v = P[r * PX + c];
-                        rec[i * PK + j] = g.pw ? v * g.pw[r * NX + c] : v;
+                        rec[i * PK + j] = g.pw ? v * g.pw[r * N + c] : v;
                     }
                 rec[kRecH] = h;
                 rec[kRecI0] = (float)i0;
@@ -743,13 +833,13 @@ template <>
 int launch_main<PIVAMD_XCORR_SIZE>(XcorrArgs g, hipStream_t st)
 {
     constexpr int N = PIVAMD_XCORR_SIZE;
-    using C = XcCfg<N, N>;
+    using C = XcCfg<N>;
     const long long nb = (g.total + C::WPB - 1) / C::WPB;
     if (nb <= 0) return 0;
     if (nb > (1LL << 30)) return PIVAMD_ERR_ARG;
     g.nblocks = (int)nb;
     const int grid = (int)((nb + 7) / 8 * 8);
-    hipLaunchKernelGGL((xcorr_kernel<N, N>), dim3(grid), dim3(C::BLOCK), 0, st, g);
+    hipLaunchKernelGGL((xcorr_kernel<N>), dim3(grid), dim3(C::BLOCK), 0, st, g);
     return hipGetLastError() == hipSuccess ? 0 : PIVAMD_ERR_LAUNCH;
 }
 #endif
