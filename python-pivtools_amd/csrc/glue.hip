@@ -67,22 +67,87 @@ __global__ void gate_kernel(const float *pkx, const float *pky, const float *pkh
     pkh_out[i] = h;
 }
 
-// insertion sort of n <= 24 floats in registers, then the median
-// (bottleneck nanmedian: NaNs dropped, mean of the middle pair)
-__device__ __forceinline__ float median_of(float *v, int n)
+// nanmedian of K values held in registers (bottleneck semantics: NaNs
+// dropped, the middle element for an odd count, the mean of the middle pair
+// for an even one, NaN when nothing is left).  Fully unrolled rank
+// selection: every index is a compile-time constant, nothing spills.
+template <int K>
+__device__ __forceinline__ float nanmedian_reg(const float (&v)[K])
 {
+    int n = 0;
+#pragma unroll
+    for (int i = 0; i < K; ++i) n += (v[i] == v[i]) ? 1 : 0;
     if (n == 0) return __builtin_nanf("");
-    for (int a = 1; a < n; ++a) {
-        const float key = v[a];
-        int b = a - 1;
-        while (b >= 0 && v[b] > key) {
-            v[b + 1] = v[b];
-            --b;
+    const int klo = (n - 1) >> 1, khi = n >> 1;
+    float lo = 0.0f, hi = 0.0f;
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+        int rank = 0;
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            if (j == i) continue;
+            // stable rank among the non-NaN values (NaN compares false)
+            rank += (v[j] < v[i] || (j < i && v[j] == v[i])) ? 1 : 0;
         }
-        v[b + 1] = key;
+        const bool ok = v[i] == v[i];
+        lo = (ok && rank == klo) ? v[i] : lo;
+        hi = (ok && rank == khi) ? v[i] : hi;
     }
-    if (n & 1) return v[n / 2];
-    return 0.5f * (v[n / 2 - 1] + v[n / 2]);
+    return (n & 1) ? lo : 0.5f * (lo + hi);
+}
+
+// Normalised-median test of one node with a SIZE x SIZE neighbourhood.
+template <int SIZE>
+__device__ __forceinline__ bool median_outlier(const float *uy, const float *ux, const float *mask, int r,
+                                               int c, int ny, int nx, float eps, float thr)
+{
+    constexpr int K = SIZE * SIZE - 1, h = SIZE / 2;
+    float vx[K], vy[K];
+    int k = 0;
+#pragma unroll
+    for (int dr = -h; dr <= h; ++dr)
+#pragma unroll
+        for (int dc = -h; dc <= h; ++dc) {
+            if (dr == 0 && dc == 0) continue;
+            const int rr = r + dr, cc = c + dc;
+            float a = __builtin_nanf(""), b = __builtin_nanf("");
+            if (rr >= 0 && rr < ny && cc >= 0 && cc < nx) {
+                const int q = rr * nx + cc;
+                if (!(mask && mask[q] != 0.0f)) {
+                    a = ux[q];
+                    b = uy[q];
+                }
+            }
+            vx[k] = a;
+            vy[k] = b;
+            ++k;
+        }
+    const float mu = nanmedian_reg<K>(vx), mv = nanmedian_reg<K>(vy);
+    const float x0 = ux[r * nx + c], y0 = uy[r * nx + c];
+    const float ddx = x0 - mu, ddy = y0 - mv;
+    const float r0 = sqrtf(ddx * ddx + ddy * ddy);
+    float ri[K];
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+        const float ex = vx[i] - mu, ey = vy[i] - mv;
+        ri[i] = sqrtf(ex * ex + ey * ey);
+    }
+    const float rm = nanmedian_reg<K>(ri);
+    const float nr = r0 / (rm + eps);
+    const bool exceeded = (nr > thr) || !isfinite(nr);
+    int present = 0;
+#pragma unroll
+    for (int dr = -1; dr <= 1; ++dr)
+#pragma unroll
+        for (int dc = -1; dc <= 1; ++dc) {
+            const int rr = r + dr, cc = c + dc;
+            if (rr < 0 || rr >= ny || cc < 0 || cc >= nx) continue;
+            const int q = rr * nx + cc;
+            const bool m = mask && mask[q] != 0.0f;
+            const float a = m ? __builtin_nanf("") : ux[q], b = m ? __builtin_nanf("") : uy[q];
+            if ((isfinite(a) && isfinite(b)) || m) ++present;
+        }
+    return exceeded || present < 6;
 }
 
 // Normalised-median outlier test on u (planar, masked cells read as NaN),
@@ -97,54 +162,11 @@ __global__ void median_kernel(const float *u, const float *pkh, const float *mas
     const int n = i / nWin, w = i % nWin, r = w / nx, c = w % nx;
     const float *uy = u + (size_t)n * nWin;
     const float *ux = u + (size_t)total + (size_t)n * nWin;
-    auto is_masked = [&](int k) { return mask && mask[k] != 0.0f; };
-    auto vy = [&](int k) { return is_masked(k) ? __builtin_nanf("") : uy[k]; };
-    auto vx = [&](int k) { return is_masked(k) ? __builtin_nanf("") : ux[k]; };
-    const bool self_masked = is_masked(w);
+    const bool self_masked = mask && mask[w] != 0.0f;
     bool outlier = false;
     if (!self_masked) {
-        const int h = size / 2;
-        float nnx[24], nny[24], tx[24], ty[24];
-        int cnt = 0, cx = 0, cy = 0;
-        for (int dr = -h; dr <= h; ++dr)
-            for (int dc = -h; dc <= h; ++dc) {
-                if (dr == 0 && dc == 0) continue;
-                const int rr = r + dr, cc = c + dc;
-                float a = __builtin_nanf(""), b = __builtin_nanf("");
-                if (rr >= 0 && rr < ny && cc >= 0 && cc < nx) {
-                    a = vx(rr * nx + cc);
-                    b = vy(rr * nx + cc);
-                }
-                nnx[cnt] = a;
-                nny[cnt] = b;
-                ++cnt;
-                if (a == a) tx[cx++] = a;
-                if (b == b) ty[cy++] = b;
-            }
-        const float mu = median_of(tx, cx), mv = median_of(ty, cy);
-        const float x0 = vx(w), y0 = vy(w);
-        const float ddx = x0 - mu, ddy = y0 - mv;
-        const float r0 = sqrtf(ddx * ddx + ddy * ddy);
-        float ri[24];
-        int cr = 0;
-        for (int k = 0; k < cnt; ++k) {
-            const float ex = nnx[k] - mu, ey = nny[k] - mv;
-            const float v = sqrtf(ex * ex + ey * ey);
-            if (v == v) ri[cr++] = v;
-        }
-        const float rm = median_of(ri, cr);
-        const float nr = r0 / (rm + eps);
-        const bool exceeded = (nr > thr) || !isfinite(nr);
-        int present = 0;
-        for (int dr = -1; dr <= 1; ++dr)
-            for (int dc = -1; dc <= 1; ++dc) {
-                const int rr = r + dr, cc = c + dc;
-                if (rr < 0 || rr >= ny || cc < 0 || cc >= nx) continue;
-                const int k = rr * nx + cc;
-                const float a = vx(k), b = vy(k);
-                if ((isfinite(a) && isfinite(b)) || is_masked(k)) ++present;
-            }
-        outlier = exceeded || present < 6;
+        outlier = size == 3 ? median_outlier<3>(uy, ux, mask, r, c, ny, nx, eps, thr)
+                            : median_outlier<5>(uy, ux, mask, r, c, ny, nx, eps, thr);
     }
     bool nm = nanm[i] != 0;
     int8_t rs = reason[i];

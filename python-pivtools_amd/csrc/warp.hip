@@ -13,9 +13,11 @@
 // contraction off, so every output pixel is bit-identical to it.
 //
 // Mapping: a 256-lane workgroup covers a 64-column x 16-row tile of one
-// image (grid.z = image).  Each lane owns one column and four rows; the
-// tile's predictor indices are computed once into LDS; every tap is bounds
-// checked, so a non-finite predictor can never produce an out-of-range read.
+// image (grid.z = image).  Each lane owns one column and four rows.  The
+// tile's predictor indices and the predictor cells its stencils touch are
+// staged in LDS once; image taps take one bounds test per stencil (interior)
+// or one per tap (border), so a non-finite predictor can never produce an
+// out-of-range read.
 #include <hip/hip_runtime.h>
 
 #include "pivamd_dev.h"
@@ -57,6 +59,9 @@ __device__ float pred_index(float coord, const float *ctrs, int nC)
     return (float)seg + t;
 }
 
+// Bicubic tap sum (Keys a=-0.75, 4x4 at floor-1, zero outside the image),
+// term order m outer / n inner as fused_warp.c:70-105.  The whole stencil
+// is tested once; only border stencils test each tap.
 __device__ __forceinline__ float sample_bicubic(const float *img, float fy, float fx, int H, int W)
 {
     const float fyf = floorf(fy), fxf = floorf(fx);
@@ -65,6 +70,16 @@ __device__ __forceinline__ float sample_bicubic(const float *img, float fy, floa
     keys4(fy - fyf, wy);
     keys4(fx - fxf, wx);
     float val = 0.0f;
+    if (iy >= 0 && iy + 3 < H && ix >= 0 && ix + 3 < W) {
+        const float *p = img + (size_t)iy * W + ix;
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+#pragma unroll
+            for (int n = 0; n < 4; n++) val += wy[m] * wx[n] * p[n];
+            p += W;
+        }
+        return val;
+    }
 #pragma unroll
     for (int m = 0; m < 4; m++) {
         const int row = iy + m;
@@ -101,6 +116,16 @@ __device__ __forceinline__ float sample_lanczos(const float *img, float fy, floa
     lanczos_w(fy - fyf, lut, wy);
     lanczos_w(fx - fxf, lut, wx);
     float val = 0.0f;
+    if (iy >= 0 && iy + 5 < H && ix >= 0 && ix + 5 < W) {
+        const float *p = img + (size_t)iy * W + ix;
+#pragma unroll
+        for (int m = 0; m < 6; m++) {
+#pragma unroll
+            for (int n = 0; n < 6; n++) val += wy[m] * wx[n] * p[n];
+            p += W;
+        }
+        return val;
+    }
 #pragma unroll
     for (int m = 0; m < 6; m++) {
         const int row = iy + m;
@@ -121,20 +146,57 @@ struct WarpArgs {
     int H, W, nPY, nPX, predStride, mode, roundShifts;
 };
 
+constexpr int kPredMax = 1024;   // predictor cells a tile may stage in LDS
+
 __global__ void __launch_bounds__(256) warp_kernel(WarpArgs g)
 {
     __shared__ float s_ly[kTileY];
     __shared__ float s_lx[kTileX];
+    __shared__ float s_pdy[kPredMax];
+    __shared__ float s_pdx[kPredMax];
     const int tx = threadIdx.x & (kTileX - 1);
     const int ty = threadIdx.x / kTileX;
     const int x0 = blockIdx.x * kTileX, y0 = blockIdx.y * kTileY;
     const int img = blockIdx.z;
+    const int nx_t = min(kTileX, g.W - x0), ny_t = min(kTileY, g.H - y0);
     if (threadIdx.x < kTileX) {
-        const int x = x0 + threadIdx.x;
-        s_lx[threadIdx.x] = x < g.W ? pred_index((float)x, g.cx, g.nPX) : 0.0f;
+        const int x = x0 + min((int)threadIdx.x, nx_t - 1);
+        s_lx[threadIdx.x] = pred_index((float)x, g.cx, g.nPX);
     } else if (threadIdx.x < kTileX + kTileY) {
-        const int y = y0 + threadIdx.x - kTileX;
-        s_ly[threadIdx.x - kTileX] = y < g.H ? pred_index((float)y, g.cy, g.nPY) : 0.0f;
+        const int y = y0 + min((int)threadIdx.x - kTileX, ny_t - 1);
+        s_ly[threadIdx.x - kTileX] = pred_index((float)y, g.cy, g.nPY);
+    }
+    __syncthreads();
+    // predictor cells the tile's stencils touch: integer min/max over the
+    // tile's own indices (no assumption on the order of the centres), so a
+    // staged index is always inside the staged rectangle
+    int fy0 = 0x7fffffff, fy1 = -0x7fffffff, fx0 = 0x7fffffff, fx1 = -0x7fffffff;
+#pragma unroll
+    for (int k = 0; k < kTileY; ++k) {
+        const int v = (int)floorf(s_ly[k]);
+        fy0 = min(fy0, v);
+        fy1 = max(fy1, v);
+    }
+#pragma unroll 8
+    for (int k = 0; k < kTileX; ++k) {
+        const int v = (int)floorf(s_lx[k]);
+        fx0 = min(fx0, v);
+        fx1 = max(fx1, v);
+    }
+    const int pr0 = min(max(fy0 - 1, 0), g.nPY - 1);
+    const int pr1 = min(max(fy1 + 2, 0), g.nPY - 1);
+    const int pc0 = min(max(fx0 - 1, 0), g.nPX - 1);
+    const int pc1 = min(max(fx1 + 2, 0), g.nPX - 1);
+    const int pcw = pc1 - pc0 + 1, prh = pr1 - pr0 + 1;
+    const bool staged = (pcw * prh <= kPredMax) && pr1 >= pr0 && pc1 >= pc0;
+    const float *dyp = g.pdy + (size_t)img * g.predStride;
+    const float *dxp = g.pdx + (size_t)img * g.predStride;
+    if (staged) {
+        for (int e = threadIdx.x; e < pcw * prh; e += 256) {
+            const int r = e / pcw, c = e - r * pcw;
+            s_pdy[e] = dyp[(pr0 + r) * g.nPX + pc0 + c];
+            s_pdx[e] = dxp[(pr0 + r) * g.nPX + pc0 + c];
+        }
     }
     __syncthreads();
 
@@ -143,8 +205,6 @@ __global__ void __launch_bounds__(256) warp_kernel(WarpArgs g)
     const size_t plane = (size_t)g.H * g.W;
     const float *A = g.imgA + (size_t)img * plane;
     const float *B = g.imgB + (size_t)img * plane;
-    const float *dyp = g.pdy + (size_t)img * g.predStride;
-    const float *dxp = g.pdx + (size_t)img * g.predStride;
 
     const float fx = s_lx[tx], fxf = floorf(fx);
     const int ix = (int)fxf - 1;
@@ -164,15 +224,27 @@ __global__ void __launch_bounds__(256) warp_kernel(WarpArgs g)
         float wy[4];
         keys4(fiy - fiyf, wy);
         float va = 0.0f, vb = 0.0f;
+        if (staged) {
 #pragma unroll
-        for (int m = 0; m < 4; m++) {
-            const int row = min(max(iyb + m, 0), g.nPY - 1);
+            for (int m = 0; m < 4; m++) {
+                const int row = (min(max(iyb + m, 0), g.nPY - 1) - pr0) * pcw - pc0;
 #pragma unroll
-            for (int n = 0; n < 4; n++) {
-                const float w = wy[m] * wx[n];
-                const int k = row * g.nPX + cols[n];
-                va += w * dyp[k];
-                vb += w * dxp[k];
+                for (int n = 0; n < 4; n++) {
+                    const float w = wy[m] * wx[n];
+                    va += w * s_pdy[row + cols[n]];
+                    vb += w * s_pdx[row + cols[n]];
+                }
+            }
+        } else {
+#pragma unroll
+            for (int m = 0; m < 4; m++) {
+                const int row = min(max(iyb + m, 0), g.nPY - 1) * g.nPX;
+#pragma unroll
+                for (int n = 0; n < 4; n++) {
+                    const float w = wy[m] * wx[n];
+                    va += w * dyp[row + cols[n]];
+                    vb += w * dxp[row + cols[n]];
+                }
             }
         }
         float hy = 0.5f * va, hx = 0.5f * vb;
