@@ -30,6 +30,14 @@ SCHEDULE = [(128, 128), (64, 64), (32, 32), (16, 16)]
 OVERLAP = [50, 50, 50, 50]
 PEAK_FP32_TFLOPS = 157.3
 HBM_TBS = 8.0
+# HBM bytes per correlation launch (xcorr_kernel + fit_kernel) of the 64-pair
+# 2160x2560 workload, from rocprofv3 FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0)
+# counters calibrated on a copy kernel of known size; recorded in
+# profiles/hbm_counters_multipass_2160x2560_u16.md.
+MEASURED_TRAFFIC_64_PAIRS = {128: 2.796e9 + 0.011e9 + 0.010e9 + 0.002e9,
+                             64: 2.811e9 + 0.044e9 + 0.043e9 + 0.008e9,
+                             32: 2.832e9 + 0.180e9 + 0.175e9 + 0.033e9,
+                             16: 2.832e9 + 0.725e9 + 0.703e9 + 0.132e9}
 
 
 def log(*a):
@@ -101,6 +109,8 @@ def main():
     ap.add_argument("--finder", default="gauss3")
     ap.add_argument("--impl", default="scalar")
     ap.add_argument("--cpu-budget", type=float, default=60.0, help="0 disables the CPU baseline leg")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run one plain 256 MiB device copy (byte-counter calibration for profiles)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -124,6 +134,14 @@ def main():
     torch.cuda.synchronize()
     mp.check_infill()
 
+    if args.calibrate_copy:
+        from pivtools_amd import _native
+        src = torch.ones(64 * 1024 * 1024, device=dev)
+        dst = torch.empty_like(src)
+        _native.core().pivamd_copy_f32(src.data_ptr(), dst.data_ptr(), src.numel(),
+                                       torch.cuda.current_stream().cuda_stream)
+        torch.cuda.synchronize()
+        del src, dst
     log(f"[bench] timing {args.steps} steps")
     step_ms = []
     events = []
@@ -155,6 +173,9 @@ def main():
     dom = max(passes, key=lambda d: d["ms"])
     dom_flops = f_corr(dom["window"]) * dom["windows"]
     achieved = dom_flops / (dom["ms"] * 1e-3) / 1e12
+    traffic = None
+    if args.pairs == 64 and dom["window"] in MEASURED_TRAFFIC_64_PAIRS:
+        traffic = MEASURED_TRAFFIC_64_PAIRS[dom["window"]]
 
     log(f"[bench] {value:.2f} pairs/s, {wall * 1e3 / args.steps:.1f} ms/step, passes {passes}")
     res_final = res[-1]
@@ -189,7 +210,7 @@ def main():
         "vectors_per_s": round(value * mp.grids[-1].n_win[0] * mp.grids[-1].n_win[1], 1),
         "passes": passes,
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
-                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": traffic,
                      "kernel": f"xcorr_kernel<{dom['window']},{dom['window']}> + fit_kernel",
                      "note": "FP32 VALU FFT; peak = MI355X FP32 157.3 TFLOP/s"},
         "cpu_baseline": cpu,

@@ -175,6 +175,27 @@ int pivamd_accumulate(const float *d_A, const float *d_B, const float *d_mask, i
                       float *d_outAB, float *d_outAA, float *d_outBB, void *d_work,
                       unsigned long work_bytes, void *stream);
 
+/* Multipass glue (device buffers, planar fields [dy plane, dx plane]):
+ * uint16 ingest, displacement gate + predictor add-back, normalised-median
+ * test, nearest infill, edge padding, one axis of the Gaussian predictor
+ * smoothing, bicubic predictor remap, and a plain copy (counter calibration). */
+int pivamd_u16_to_f32(const void *d_in, float *d_out, unsigned long n, void *stream);
+int pivamd_copy_f32(const float *d_in, float *d_out, unsigned long n, void *stream);
+int pivamd_gate(const float *pkx, const float *pky, const float *pkh, const float *pred,
+                const float *mask, int N, int nWin, float tx, float ty, float *u,
+                unsigned char *nanm, signed char *reason, float *pkh_out, void *stream);
+int pivamd_median(const float *u, const float *pkh, const float *mask, int N, int ny, int nx,
+                  float eps, float thr, int size, unsigned char *nanm, signed char *reason,
+                  float *uo, float *pkh_out, void *stream);
+int pivamd_infill_nearest(const float *u, const float *mask, int N, int ny, int nx, float py,
+                          float px, float *uo, int *d_err, void *stream);
+int pivamd_pad_edge(const float *u, int N, int ny, int nx, int pre_y, int pre_x, int nyp, int nxp,
+                    float *out, void *stream);
+int pivamd_smooth_axis(const float *in, int planes, int ny, int nx, int axis, const double *d_wts,
+                       int radius, float *out, void *stream);
+int pivamd_remap_cubic(const float *in, int N, int nyo, int nxo, const float *map_x,
+                       const float *map_y, int ny, int nx, float *out, void *stream);
+
 /* Peak search + fit + detectability on n planes of ny x nx in device
  * memory (window-major); outputs peak_loc[3*n] (row, col, height) and
  * std_dev[3*n] per plane, component-major with stride n, and ratio[n]. */

@@ -328,6 +328,14 @@ __global__ void remap_kernel(const float *in, int N, int nyo, int nxo, const flo
     out[i] = sum;
 }
 
+// Plain copy of n floats, the calibration reference for HBM byte counters.
+__global__ void copy_kernel(const float *in, float *out, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t k = i; k < n; k += stride) out[k] = in[k];
+}
+
 static inline unsigned nblk(size_t n) { return (unsigned)((n + 255) / 256); }
 
 }  // namespace pivamd
@@ -342,6 +350,13 @@ extern "C" int pivamd_u16_to_f32(const void *d_in, float *d_out, size_t n, hipSt
     unsigned g = nblk(n);
     if (g > 65536u * 8) g = 65536u * 8;
     hipLaunchKernelGGL(u16_to_f32_kernel, dim3(g), dim3(256), 0, st, (const uint16_t *)d_in, d_out, n);
+    return PIVAMD_LAUNCHED();
+}
+
+extern "C" int pivamd_copy_f32(const float *d_in, float *d_out, size_t n, hipStream_t st)
+{
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(copy_kernel, dim3(2048), dim3(256), 0, st, d_in, d_out, n);
     return PIVAMD_LAUNCHED();
 }
 

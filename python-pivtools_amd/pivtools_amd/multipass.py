@@ -34,6 +34,7 @@ _sz = ctypes.c_size_t
 
 _GLUE = {
     "pivamd_u16_to_f32": [_vp, _vp, _sz, _vp],
+    "pivamd_copy_f32": [_vp, _vp, _sz, _vp],
     "pivamd_gate": [_vp, _vp, _vp, _vp, _vp, _i, _i, _f, _f, _vp, _vp, _vp, _vp, _vp],
     "pivamd_median": [_vp, _vp, _vp, _i, _i, _i, _f, _f, _i, _vp, _vp, _vp, _vp, _vp],
     "pivamd_infill_nearest": [_vp, _vp, _i, _i, _i, _f, _f, _vp, _vp, _vp],
