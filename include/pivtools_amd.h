@@ -128,6 +128,30 @@ int fused_symmetric_warp_batch(const float *imgs_a, const float *imgs_b, float *
                                int round_shifts);
 
 /* ======================================================================= */
+/* Layer 1: libkspacefit drop-in                                            */
+/* ======================================================================= */
+
+/* Replaces kspace_lm_fit_batch, pivtools_cli/lib/kspace_lm_fit.h:54-61
+ * (caller pivtools_cli/piv/piv_backend/kspace_lm_fitting.py:877-900).
+ * Same layouts and contract: planes (n_windows, corr_h, corr_w) FP64,
+ * P_win centred or NULL, every output written.  n_threads is accepted and
+ * ignored (one workgroup per window).  Returns 0, -1 (axis length outside
+ * 8/12/16/24/32/48/64/96/128) or -2 (device failure). */
+int kspace_lm_fit_batch(const double *R_AA, const double *R_BB, const double *R_AB,
+                        const unsigned char *mask_flat, const double *P_win, int n_windows, int corr_h,
+                        int corr_w, int use_kx4, int use_ky4, int n_threads, double *gauss_flat,
+                        int *status_flat, double *initial_guess_flat, double *diag_gain, double *diag_N0,
+                        double *diag_b4x, double *diag_b4y, double *diag_cost_per_pt, int *diag_n_valid,
+                        int *diag_iter, unsigned char *diag_conv);
+/* kspace_lm_fit.h:65.  Loader probe: compute units of the current device. */
+int kspace_lm_fit_max_threads(void);
+/* kspace_lm_fit.h:79-82.  D in natural order and the coloured N0 seed. */
+int kspace_debug_prep(const double *R_AA, const double *R_BB, const double *R_AB, const double *P_centred,
+                      int H, int W, double *out_Dv, double *out_n0_seed);
+/* kspace_lm_fit.h:86-87.  fftshift(fft2(ifftshift(plane))) in natural order. */
+int kspace_fft2_centered(const double *plane, int H, int W, double *out_re, double *out_im);
+
+/* ======================================================================= */
 /* Layer 2: device-pointer API (libpivtools_amd.so)                          */
 /* ======================================================================= */
 
@@ -195,6 +219,22 @@ int pivamd_smooth_axis(const float *in, int planes, int ny, int nx, int axis, co
                        int radius, float *out, void *stream);
 int pivamd_remap_cubic(const float *in, int N, int nyo, int nxo, const float *map_x,
                        const float *map_y, int ny, int nx, float *out, void *stream);
+
+/* k-space fit over device buffers (same arguments as kspace_lm_fit_batch,
+ * plus a workspace of pivamd_kspace_workspace bytes and a stream). */
+unsigned long pivamd_kspace_workspace(int n_windows, int corr_h, int corr_w);
+int pivamd_kspace_fit(const double *d_aa, const double *d_bb, const double *d_ab,
+                      const unsigned char *d_mask, const double *d_pwin, int n_windows, int corr_h,
+                      int corr_w, int use_kx4, int use_ky4, double *d_gauss, int *d_status, double *d_init,
+                      double *d_gain, double *d_n0, double *d_b4x, double *d_b4y, double *d_cpp,
+                      int *d_nvalid, int *d_iter, unsigned char *d_conv, void *d_work,
+                      unsigned long work_bytes, void *stream);
+int pivamd_kspace_fft2_centered(const double *d_plane, int corr_h, int corr_w, double *d_re, double *d_im,
+                                void *d_work, unsigned long work_bytes, void *stream);
+/* d_work: pivamd_kspace_workspace(1, H, W) + 512 bytes. */
+int pivamd_kspace_debug_prep(const double *d_aa, const double *d_bb, const double *d_ab,
+                             const double *d_pcentred, int corr_h, int corr_w, double *d_dv,
+                             double *d_n0seed, void *d_work, unsigned long work_bytes, void *stream);
 
 /* Peak search + fit + detectability on n planes of ny x nx in device
  * memory (window-major); outputs peak_loc[3*n] (row, col, height) and

@@ -74,6 +74,19 @@ def lib() -> ctypes.CDLL:
             _f32p, _f32p, _i32p, ctypes.c_void_p, _f32p,
         ]
         L.orc_max_threads.restype = ctypes.c_int
+        _f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+        _u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+        L.orc_kspace_lm_fit_batch.restype = ctypes.c_int
+        L.orc_kspace_lm_fit_batch.argtypes = [
+            _f64p, _f64p, _f64p, _u8p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+            ctypes.c_int, ctypes.c_int, ctypes.c_int, _f64p, _i32p, _f64p, _f64p, _f64p, _f64p,
+            _f64p, _f64p, _i32p, _i32p, _u8p,
+        ]
+        L.orc_kspace_fft2_centered.restype = ctypes.c_int
+        L.orc_kspace_fft2_centered.argtypes = [_f64p, ctypes.c_int, ctypes.c_int, _f64p, _f64p]
+        L.orc_kspace_debug_prep.restype = ctypes.c_int
+        L.orc_kspace_debug_prep.argtypes = [_f64p, _f64p, _f64p, ctypes.c_void_p, ctypes.c_int,
+                                            ctypes.c_int, _f64p, _f64p]
         _LIB = L
     return _LIB
 
@@ -241,3 +254,58 @@ def planes_fp64(img_a, img_b, ctrs_x, ctrs_y, win, wa, wb, ensemble=False):
     fb = np.fft.fft2(b)
     planes = np.fft.fftshift(np.real(np.fft.ifft2(fb * np.conj(fa))), axes=(-2, -1))
     return planes, ok
+
+
+# ---------------------------------------------------------------------------
+# k-space transfer-function fit (kspace_oracle.c)
+# ---------------------------------------------------------------------------
+KSPACE_DIAG = ("gain", "N0", "b4x", "b4y", "cost_per_pt", "n_valid", "iter", "conv")
+
+
+def kspace_alloc(n):
+    """Output arrays of kspace_lm_fit_batch, initialised as the reference
+    caller does (kspace_lm_fitting.py:862-875)."""
+    return {
+        "gauss": np.zeros((n, 16)), "status": np.full(n, -1, np.int32), "init": np.zeros((n, 16)),
+        "gain": np.full(n, np.nan), "N0": np.full(n, np.nan), "b4x": np.full(n, np.nan),
+        "b4y": np.full(n, np.nan), "cost_per_pt": np.full(n, np.nan),
+        "n_valid": np.zeros(n, np.int32), "iter": np.zeros(n, np.int32), "conv": np.zeros(n, np.uint8),
+    }
+
+
+def kspace_fit(r_aa, r_bb, r_ab, mask, corr, p_win=None, kx4=False, ky4=False, threads=0):
+    """Oracle kspace_lm_fit_batch on (n, h, w) FP64 planes; returns (ret, outputs)."""
+    h, w = corr
+    aa = np.ascontiguousarray(r_aa, np.float64).reshape(-1)
+    bb = np.ascontiguousarray(r_bb, np.float64).reshape(-1)
+    ab = np.ascontiguousarray(r_ab, np.float64).reshape(-1)
+    n = aa.size // (h * w)
+    m = np.ascontiguousarray(mask, np.uint8).reshape(-1)
+    pw = None if p_win is None else np.ascontiguousarray(p_win, np.float64).reshape(-1)
+    o = kspace_alloc(n)
+    ret = lib().orc_kspace_lm_fit_batch(
+        aa, bb, ab, m, None if pw is None else pw.ctypes.data, n, h, w, int(kx4), int(ky4), threads,
+        o["gauss"], o["status"], o["init"], o["gain"], o["N0"], o["b4x"], o["b4y"], o["cost_per_pt"],
+        o["n_valid"], o["iter"], o["conv"])
+    return ret, o
+
+
+def kspace_fft2_centered(plane):
+    plane = np.ascontiguousarray(plane, np.float64)
+    h, w = plane.shape
+    re = np.empty((h, w))
+    im = np.empty((h, w))
+    ret = lib().orc_kspace_fft2_centered(plane, h, w, re, im)
+    return ret, re + 1j * im
+
+
+def kspace_debug_prep(r_aa, r_bb, r_ab, p_centred=None):
+    h, w = r_aa.shape
+    dv = np.empty(h * w)
+    n0 = np.zeros(1)
+    pc = None if p_centred is None else np.ascontiguousarray(p_centred, np.float64)
+    ret = lib().orc_kspace_debug_prep(np.ascontiguousarray(r_aa, np.float64),
+                                      np.ascontiguousarray(r_bb, np.float64),
+                                      np.ascontiguousarray(r_ab, np.float64),
+                                      None if pc is None else pc.ctypes.data, h, w, dv, n0)
+    return ret, dv.reshape(h, w), float(n0[0])

@@ -3,6 +3,7 @@
     libpivtools_amd.so   device kernels + the device-pointer C ABI (pivamd_*)
     libbulkxcorr2d.so    host-pointer drop-in for the reference's libbulkxcorr2d
     libfusedwarp.so      host-pointer drop-in for the reference's libfusedwarp
+    libkspacefit.so      host-pointer drop-in for the reference's libkspacefit
 
 Run ``python python-pivtools_amd/build.py`` (or ``__graft_entry__.build()``).
 Translation units compile in parallel; every HIP line carries the
@@ -35,7 +36,7 @@ XCORR_SIZES = (8, 12, 16, 24, 32, 48, 64, 96, 128)
 KERNEL_TUS = [("warp", "warp.hip", [])]
 KERNEL_TUS += [("xcorr_common", "xcorr.hip", ["-DPIVAMD_XCORR_COMMON"])]
 KERNEL_TUS += [(f"xcorr_{n}", "xcorr.hip", [f"-DPIVAMD_XCORR_SIZE={n}"]) for n in XCORR_SIZES]
-EXTRA_TUS = ["ensemble.hip", "glue.hip"]
+EXTRA_TUS = ["ensemble.hip", "glue.hip", "kspace.hip"]
 
 
 def _run_parallel(jobs):
@@ -69,7 +70,7 @@ def build(verbose: bool = True) -> None:
         obj = os.path.join(OBJ, name + ".o")
         objs.append(obj)
         jobs.append((name, [HIPCC, *COMMON, *defs, "-c", os.path.join(CSRC, src), "-o", obj]))
-    for flavour in ("XCORR", "WARP"):
+    for flavour in ("XCORR", "WARP", "KSPACE"):
         obj = os.path.join(OBJ, f"capi_{flavour.lower()}.o")
         jobs.append((f"capi_{flavour.lower()}",
                      [HIPCC, *COMMON, f"-DPIVAMD_CAPI_{flavour}", "-c", os.path.join(CSRC, "capi.cpp"), "-o", obj]))
@@ -80,7 +81,8 @@ def build(verbose: bool = True) -> None:
     link = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", core + ".tmp", *objs]
     subprocess.run(link, check=True)
     os.replace(core + ".tmp", core)
-    for flavour, lib in (("xcorr", "libbulkxcorr2d.so"), ("warp", "libfusedwarp.so")):
+    for flavour, lib in (("xcorr", "libbulkxcorr2d.so"), ("warp", "libfusedwarp.so"),
+                         ("kspace", "libkspacefit.so")):
         path = os.path.join(OUT, lib)
         cmd = [HIPCC, "-shared", "-fPIC", "-o", path + ".tmp", os.path.join(OBJ, f"capi_{flavour}.o"),
                f"-L{OUT}", "-lpivtools_amd", "-Wl,-rpath,$ORIGIN"]

@@ -1,9 +1,10 @@
 // capi.cpp -- host-pointer drop-ins for the reference's ctypes libraries.
 //
-// Built twice: with PIVAMD_CAPI_XCORR into libbulkxcorr2d.so (the symbols of
-// pivtools_cli/lib/PIV_2d_cross_correlate.h, peak_locate_lm.h,
-// peak_locate_lm_batch.h and xcorr.h) and with PIVAMD_CAPI_WARP into
-// libfusedwarp.so (pivtools_cli/lib/fused_warp.h).  Every call is
+// Built three times: with PIVAMD_CAPI_XCORR into libbulkxcorr2d.so (the
+// symbols of pivtools_cli/lib/PIV_2d_cross_correlate.h, peak_locate_lm.h,
+// peak_locate_lm_batch.h and xcorr.h), with PIVAMD_CAPI_WARP into
+// libfusedwarp.so (pivtools_cli/lib/fused_warp.h) and with
+// PIVAMD_CAPI_KSPACE into libkspacefit.so (pivtools_cli/lib/kspace_lm_fit.h).  Every call is
 // synchronous: the caller's host arrays go to HBM, the device kernels of
 // libpivtools_amd.so run on this library's stream, results come back.
 // Ownership is the reference's: the caller allocates every input and
@@ -456,3 +457,132 @@ int fused_symmetric_warp(const float *img_a, const float *img_b, float *out_a, f
 }  // extern "C"
 
 #endif  // PIVAMD_CAPI_WARP
+
+#ifdef PIVAMD_CAPI_KSPACE
+
+namespace {
+
+// kspace_lm_fit.h:48-53 return codes
+int ks_code(int e) { return e == PIVAMD_OK ? 0 : (e == PIVAMD_ERR_SIZE ? -1 : -2); }
+
+bool ks_axis(int n)
+{
+    switch (n) {
+    case 8: case 12: case 16: case 24: case 32: case 48: case 64: case 96: case 128: return true;
+    default: return false;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+// kspace_lm_fit.h:54-61.  n_threads is the reference's OpenMP width and has
+// no meaning on the device; every window is fitted by its own workgroup.
+int kspace_lm_fit_batch(const double *R_AA, const double *R_BB, const double *R_AB,
+                        const unsigned char *mask_flat, const double *P_win, int n_windows, int corr_h,
+                        int corr_w, int use_kx4, int use_ky4, int n_threads, double *gauss_flat,
+                        int *status_flat, double *initial_guess_flat, double *diag_gain, double *diag_N0,
+                        double *diag_b4x, double *diag_b4y, double *diag_cost_per_pt, int *diag_n_valid,
+                        int *diag_iter, unsigned char *diag_conv)
+{
+    (void)n_threads;
+    if (!ks_axis(corr_h) || !ks_axis(corr_w)) return -1;
+    if (n_windows <= 0) return 0;
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!ctx_ready()) return -2;
+    const size_t n = (size_t)n_windows, P = (size_t)corr_h * corr_w, pb = n * P * sizeof(double);
+    double *aa = (double *)dbuf(0, pb), *bb = (double *)dbuf(1, pb), *ab = (double *)dbuf(2, pb);
+    unsigned char *mask = (unsigned char *)dbuf(3, n);
+    double *pw = P_win ? (double *)dbuf(4, pb) : nullptr;
+    // outputs: gauss, init (16 n each), five diagnostics (n each) in doubles, then 3 int arrays, conv
+    const size_t nd = 32 * n + 5 * n;
+    char *out = (char *)dbuf(5, nd * sizeof(double) + 3 * n * sizeof(int) + n);
+    const size_t wb = pivamd_kspace_workspace(n_windows, corr_h, corr_w);
+    void *work = dbuf(6, wb);
+    if (!aa || !bb || !ab || !mask || (P_win && !pw) || !out || !work) return -2;
+    double *g = (double *)out, *ig = g + 16 * n, *dgn = ig + 16 * n, *dn0 = dgn + n, *dbx = dn0 + n,
+           *dby = dbx + n, *dcp = dby + n;
+    int *st = (int *)(dcp + n), *nv = st + n, *itr = nv + n;
+    unsigned char *cv = (unsigned char *)(itr + n);
+    if (!h2d(aa, R_AA, pb) || !h2d(bb, R_BB, pb) || !h2d(ab, R_AB, pb) || !h2d(mask, mask_flat, n) ||
+        (P_win && !h2d(pw, P_win, pb)))
+        return -2;
+    const int e = pivamd_kspace_fit(aa, bb, ab, mask, pw, n_windows, corr_h, corr_w, use_kx4, use_ky4, g, st, ig,
+                                    dgn, dn0, dbx, dby, dcp, nv, itr, cv, work, wb, g_ctx.st);
+    if (e != PIVAMD_OK) {
+        sync();
+        return ks_code(e);
+    }
+    const size_t d16 = 16 * n * sizeof(double), d1 = n * sizeof(double), i1 = n * sizeof(int);
+    const bool ok = d2h(gauss_flat, g, d16) && d2h(initial_guess_flat, ig, d16) && d2h(diag_gain, dgn, d1) &&
+                    d2h(diag_N0, dn0, d1) && d2h(diag_b4x, dbx, d1) && d2h(diag_b4y, dby, d1) &&
+                    d2h(diag_cost_per_pt, dcp, d1) && d2h(status_flat, st, i1) && d2h(diag_n_valid, nv, i1) &&
+                    d2h(diag_iter, itr, i1) && d2h(diag_conv, cv, n) && sync();
+    return ok ? 0 : -2;
+}
+
+// kspace_lm_fit.h:65.  A liveness probe for the loader: the number of
+// compute units of the current device (the device-side width), 0 without one.
+int kspace_lm_fit_max_threads(void)
+{
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) return 0;
+    return prop.multiProcessorCount;
+}
+
+// kspace_lm_fit.h:79-82
+int kspace_debug_prep(const double *R_AA, const double *R_BB, const double *R_AB, const double *P_centred, int H,
+                      int W, double *out_Dv, double *out_n0_seed)
+{
+    if (!ks_axis(H) || !ks_axis(W)) return -1;
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!ctx_ready()) return -2;
+    const size_t pb = (size_t)H * W * sizeof(double);
+    double *aa = (double *)dbuf(0, pb), *bb = (double *)dbuf(1, pb), *ab = (double *)dbuf(2, pb);
+    double *pc = P_centred ? (double *)dbuf(4, pb) : nullptr;
+    double *dv = (double *)dbuf(5, pb + 64);
+    const size_t wb = pivamd_kspace_workspace(1, H, W) + 512;
+    void *work = dbuf(6, wb);
+    if (!aa || !bb || !ab || (P_centred && !pc) || !dv || !work) return -2;
+    if (!h2d(aa, R_AA, pb) || !h2d(bb, R_BB, pb) || !h2d(ab, R_AB, pb) || (P_centred && !h2d(pc, P_centred, pb)))
+        return -2;
+    const int e = pivamd_kspace_debug_prep(aa, bb, ab, pc, H, W, dv, dv + (size_t)H * W, work, wb, g_ctx.st);
+    if (e != PIVAMD_OK) {
+        sync();
+        return ks_code(e);
+    }
+    return d2h(out_Dv, dv, pb) && d2h(out_n0_seed, dv + (size_t)H * W, sizeof(double)) && sync() ? 0 : -2;
+}
+
+// kspace_lm_fit.h:86-87
+int kspace_fft2_centered(const double *plane, int H, int W, double *out_re, double *out_im)
+{
+    if (!ks_axis(H) || !ks_axis(W)) return -1;
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!ctx_ready()) return -2;
+    const size_t pb = (size_t)H * W * sizeof(double);
+    double *pl = (double *)dbuf(0, pb), *re = (double *)dbuf(1, pb), *im = (double *)dbuf(2, pb);
+    void *work = dbuf(6, 5 * pb);
+    if (!pl || !re || !im || !work) return -2;
+    if (!h2d(pl, plane, pb)) return -2;
+    const int e = pivamd_kspace_fft2_centered(pl, H, W, re, im, work, 5 * pb, g_ctx.st);
+    if (e != PIVAMD_OK) {
+        sync();
+        return ks_code(e);
+    }
+    return d2h(out_re, re, pb) && d2h(out_im, im, pb) && sync() ? 0 : -2;
+}
+
+// The reference's loader also requires the CPU-support probe in every library
+// (lib_guard.py:6-26); here it reports whether a HIP device is present.
+int pivtools_cpu_supported(void)
+{
+    int n = 0;
+    return hipGetDeviceCount(&n) == hipSuccess && n > 0 ? 1 : 0;
+}
+
+}  // extern "C"
+
+#endif  // PIVAMD_CAPI_KSPACE

@@ -37,6 +37,17 @@ int pivamd_accumulate(const float *d_A, const float *d_B, const float *d_mask, i
                       int fitY, int fitX, int meanSub, int perPairNorm, int triple, float sumWBab,
                       float sumWAA, float sumWBB, float *d_outAB, float *d_outAA, float *d_outBB,
                       void *d_work, size_t work_bytes, hipStream_t st);
+size_t pivamd_kspace_workspace(int n_windows, int corr_h, int corr_w);
+int pivamd_kspace_fit(const double *d_aa, const double *d_bb, const double *d_ab, const unsigned char *d_mask,
+                      const double *d_pwin, int n_windows, int corr_h, int corr_w, int use_kx4, int use_ky4,
+                      double *d_gauss, int *d_status, double *d_init, double *d_gain, double *d_n0,
+                      double *d_b4x, double *d_b4y, double *d_cpp, int *d_nvalid, int *d_iter,
+                      unsigned char *d_conv, void *d_work, size_t work_bytes, hipStream_t st);
+int pivamd_kspace_fft2_centered(const double *d_plane, int corr_h, int corr_w, double *d_re, double *d_im,
+                                void *d_work, size_t work_bytes, hipStream_t st);
+int pivamd_kspace_debug_prep(const double *d_aa, const double *d_bb, const double *d_ab,
+                             const double *d_pcentred, int corr_h, int corr_w, double *d_dv, double *d_n0seed,
+                             void *d_work, size_t work_bytes, hipStream_t st);
 int pivamd_peakfit(const float *d_planes, int n, int ny, int nx, int nPeaks, int finder, int impl,
                    const float *d_planeWeight, float *d_loc, float *d_std, float *d_ratio,
                    hipStream_t st);

@@ -2,10 +2,11 @@
 
 * ``libpivtools_amd.so``: the device-pointer API (``pivamd_*``), driven by
   the host layer of this package with device buffers it owns.
-* ``libbulkxcorr2d.so`` / ``libfusedwarp.so``: host-pointer drop-ins with
-  the reference's exported names and arguments; bound here with the same
-  ``argtypes`` the reference's correlators declare
-  (``pivtools_cli/piv/piv_backend/cpu_instantaneous.py:68-165``).
+* ``libbulkxcorr2d.so`` / ``libfusedwarp.so`` / ``libkspacefit.so``:
+  host-pointer drop-ins with the reference's exported names and arguments;
+  bound here with the same ``argtypes`` the reference's callers declare
+  (``pivtools_cli/piv/piv_backend/cpu_instantaneous.py:68-165``,
+  ``kspace_lm_fitting.py:230-262``).
 
 Loading never falls back: a missing library raises ``FileNotFoundError``.
 """
@@ -27,6 +28,7 @@ _i = ctypes.c_int
 _CORE = None
 _XC = None
 _FW = None
+_KS = None
 
 
 def lib_path(name: str) -> str:
@@ -65,6 +67,11 @@ def core() -> ctypes.CDLL:
                 f = getattr(L, name)
                 f.argtypes = argt
                 f.restype = _i
+        L.pivamd_kspace_workspace.argtypes = [_i, _i, _i]
+        L.pivamd_kspace_workspace.restype = ctypes.c_size_t
+        L.pivamd_kspace_fit.argtypes = [_vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i] + [_vp] * 12 + [
+            ctypes.c_size_t, _vp]
+        L.pivamd_kspace_fit.restype = _i
         if hasattr(L, "pivamd_accumulate_triple"):
             L.pivamd_accumulate_triple.argtypes = [
                 _vp, _vp, _vp, _i, _i, _i, _vp, _vp, _i, _i, _vp, _vp, _vp, _vp, _i, _i, _i, _i,
@@ -141,6 +148,25 @@ def fusedwarp_lib() -> ctypes.CDLL:
         L.fused_warp_get_impl.restype = _i
         _FW = L
     return _FW
+
+
+def kspacefit_lib() -> ctypes.CDLL:
+    """libkspacefit.so bound as kspace_lm_fitting.py:230-262 binds the reference."""
+    global _KS
+    if _KS is None:
+        L = ctypes.CDLL(lib_path("libkspacefit.so"))
+        f64 = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+        u8 = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+        L.kspace_lm_fit_batch.restype = _i
+        L.kspace_lm_fit_batch.argtypes = [f64, f64, f64, u8, _vp, _i, _i, _i, _i, _i, _i, f64, _i32p, f64,
+                                          f64, f64, f64, f64, f64, _i32p, _i32p, u8]
+        L.kspace_lm_fit_max_threads.restype = _i
+        L.kspace_fft2_centered.restype = _i
+        L.kspace_fft2_centered.argtypes = [f64, _i, _i, f64, f64]
+        L.kspace_debug_prep.restype = _i
+        L.kspace_debug_prep.argtypes = [f64, f64, f64, _vp, _i, _i, f64, f64]
+        _KS = L
+    return _KS
 
 
 def check(rc: int, what: str) -> None:

@@ -15,6 +15,8 @@ HEADER = os.path.join(ROOT, "include", "pivtools_amd.h")
 
 WARP_NAMES = {"fused_warp_set_impl", "fused_warp_get_impl", "fused_symmetric_warp",
               "fused_symmetric_warp_batch"}
+KSPACE_NAMES = {"kspace_lm_fit_batch", "kspace_lm_fit_max_threads", "kspace_debug_prep",
+                "kspace_fft2_centered"}
 
 
 def _declared():
@@ -29,7 +31,8 @@ def test_header_declares_reference_symbols():
     d = set(_declared())
     for name in ("bulkxcorr2d", "bulkxcorr2d_accumulate", "bulkxcorr2d_accumulate_triple",
                  "lsqpeaklocate_lm", "lsqpeaklocate_lm_batch", "peak_detectability",
-                 "pivtools_cpu_supported", "fused_symmetric_warp_batch", "convolve", "xcorr"):
+                 "pivtools_cpu_supported", "fused_symmetric_warp_batch", "convolve", "xcorr",
+                 "kspace_lm_fit_batch", "kspace_fft2_centered", "kspace_debug_prep"):
         assert name in d
 
 
@@ -39,6 +42,8 @@ def test_symbol_exported(name):
         lib = ctypes.CDLL(_native.lib_path("libpivtools_amd.so"))
     elif name in WARP_NAMES:
         lib = ctypes.CDLL(_native.lib_path("libfusedwarp.so"))
+    elif name in KSPACE_NAMES:
+        lib = ctypes.CDLL(_native.lib_path("libkspacefit.so"))
     else:
         lib = ctypes.CDLL(_native.lib_path("libbulkxcorr2d.so"))
     assert hasattr(lib, name), name
@@ -48,5 +53,6 @@ def test_reference_style_bindings_load():
     L = _native.bulkxcorr2d_lib()
     assert L.bulkxcorr2d.restype is ctypes.c_ubyte
     assert _native.fusedwarp_lib().fused_symmetric_warp_batch.restype is ctypes.c_int
+    assert _native.kspacefit_lib().kspace_lm_fit_batch.restype is ctypes.c_int
     assert _native.core().pivamd_size_supported(32, 32) == 1
     assert _native.core().pivamd_size_supported(20, 20) == 0
