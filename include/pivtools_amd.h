@@ -221,13 +221,14 @@ int pivamd_remap_cubic(const float *in, int N, int nyo, int nxo, const float *ma
                        const float *map_y, int ny, int nx, float *out, void *stream);
 
 /* k-space fit over device buffers (same arguments as kspace_lm_fit_batch,
- * plus a workspace of pivamd_kspace_workspace bytes and a stream). */
+ * plus pwin_shared = 1 when d_pwin holds one shape for every window, a
+ * workspace of pivamd_kspace_workspace bytes and a stream). */
 unsigned long pivamd_kspace_workspace(int n_windows, int corr_h, int corr_w);
 int pivamd_kspace_fit(const double *d_aa, const double *d_bb, const double *d_ab,
                       const unsigned char *d_mask, const double *d_pwin, int n_windows, int corr_h,
                       int corr_w, int use_kx4, int use_ky4, double *d_gauss, int *d_status, double *d_init,
                       double *d_gain, double *d_n0, double *d_b4x, double *d_b4y, double *d_cpp,
-                      int *d_nvalid, int *d_iter, unsigned char *d_conv, void *d_work,
+                      int *d_nvalid, int *d_iter, unsigned char *d_conv, int pwin_shared, void *d_work,
                       unsigned long work_bytes, void *stream);
 int pivamd_kspace_fft2_centered(const double *d_plane, int corr_h, int corr_w, double *d_re, double *d_im,
                                 void *d_work, unsigned long work_bytes, void *stream);
@@ -235,6 +236,23 @@ int pivamd_kspace_fft2_centered(const double *d_plane, int corr_h, int corr_w, d
 int pivamd_kspace_debug_prep(const double *d_aa, const double *d_bb, const double *d_ab,
                              const double *d_pcentred, int corr_h, int corr_w, double *d_dv,
                              double *d_n0seed, void *d_work, unsigned long work_bytes, void *stream);
+
+/* Ensemble pass on the device (single_pass_accumulator.py:555-795,
+ * 1067-1290): FP64 image sums, mean images, plane finalize (sums / N minus
+ * the background planes (NULL: none), optional plane-mean removal and
+ * envelope divide (NULL: none), AB x ab_scale, division by
+ * sqrt(max(AA(0) BB(0), 1e-12)), FP64 out), and displacement / stress
+ * extraction from the k-space rows. */
+int pivamd_sum_images(const float *d_imgs, int n, unsigned long hw, double *d_sum, void *stream);
+int pivamd_mean_image(const double *d_sum, double n, unsigned long hw, float *d_out, void *stream);
+int pivamd_ensemble_finalize(const float *d_sAA, const float *d_sBB, const float *d_sAB, const float *d_bAA,
+                             const float *d_bBB, const float *d_bAB, const float *d_envAuto,
+                             const float *d_envAB, int n_windows, int corr_h, int corr_w, float n_images,
+                             float ab_scale, int dc_zero, double *d_oAA, double *d_oBB, double *d_oAB,
+                             void *stream);
+int pivamd_ensemble_extract(const double *d_gauss, const int *d_status, const unsigned char *d_mask,
+                            int n_windows, int corr_h, int corr_w, float *d_ux, float *d_uy, float *d_uu,
+                            float *d_vv, float *d_uv, float *d_peak, int *d_reason, void *stream);
 
 /* Peak search + fit + detectability on n planes of ny x nx in device
  * memory (window-major); outputs peak_loc[3*n] (row, col, height) and

@@ -509,7 +509,7 @@ int kspace_lm_fit_batch(const double *R_AA, const double *R_BB, const double *R_
         (P_win && !h2d(pw, P_win, pb)))
         return -2;
     const int e = pivamd_kspace_fit(aa, bb, ab, mask, pw, n_windows, corr_h, corr_w, use_kx4, use_ky4, g, st, ig,
-                                    dgn, dn0, dbx, dby, dcp, nv, itr, cv, work, wb, g_ctx.st);
+                                    dgn, dn0, dbx, dby, dcp, nv, itr, cv, 0, work, wb, g_ctx.st);
     if (e != PIVAMD_OK) {
         sync();
         return ks_code(e);

@@ -336,3 +336,184 @@ extern "C" int pivamd_accumulate(const float *d_A, const float *d_B, const float
     default: return PIVAMD_ERR_SIZE;
     }
 }
+
+// ---------------------------------------------------------------------------
+// Ensemble finalize (pivtools_cli/piv/piv_backend/single_pass_accumulator.py
+// :555-795, 1067-1290), on the device.  The arithmetic is float32 in the
+// reference's order (sums / N, minus the background correlation of the mean
+// images, optional plane-mean removal, optional envelope divide, division
+// by sqrt(max(AA(0) BB(0), 1e-12))); the planes go out in FP64 for the
+// k-space fit.
+// ---------------------------------------------------------------------------
+namespace pivamd {
+
+__global__ void sum_images_kernel(const float *img, int n, size_t hw, double *sum)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < hw; i += (size_t)gridDim.x * blockDim.x) {
+        double s = sum[i];
+        for (int k = 0; k < n; ++k) s += (double)img[(size_t)k * hw + i];
+        sum[i] = s;
+    }
+}
+
+__global__ void mean_image_kernel(const double *sum, double n, size_t hw, float *out)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < hw; i += (size_t)gridDim.x * blockDim.x)
+        out[i] = (float)(sum[i] / n);
+}
+
+struct FinArgs {
+    const float *sAA, *sBB, *sAB, *bAA, *bBB, *bAB, *envAuto, *envAB;
+    double *oAA, *oBB, *oAB;
+    float nImages, abScale;
+    int P, centre, dcZero;
+};
+
+__device__ __forceinline__ float fin_block_sum(float v, float *red)
+{
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    float s = 0.0f;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += red[w];
+    return s;
+}
+
+// One workgroup per window.
+__global__ void __launch_bounds__(256) ens_finalize_kernel(FinArgs a)
+{
+    __shared__ float red[4];
+    const size_t off = (size_t)blockIdx.x * a.P;
+    const int t = threadIdx.x, P = a.P;
+    const float *s3[3] = {a.sAA + off, a.sBB + off, a.sAB + off};
+    const float *b3[3] = {a.bAA ? a.bAA + off : nullptr, a.bBB ? a.bBB + off : nullptr,
+                          a.bAB ? a.bAB + off : nullptr};
+    float mean[3] = {0.0f, 0.0f, 0.0f};
+    if (a.dcZero) {
+        for (int p = 0; p < 3; ++p) {
+            float s = 0.0f;
+            for (int i = t; i < P; i += blockDim.x) s += s3[p][i] / a.nImages - (b3[p] ? b3[p][i] : 0.0f);
+            mean[p] = fin_block_sum(s, red) / (float)P;
+        }
+    }
+    // centre values of AA and BB after the same steps
+    float cv[2];
+    for (int p = 0; p < 2; ++p) {
+        const int c = a.centre;
+        float r = s3[p][c] / a.nImages;
+        if (b3[p]) r = r - b3[p][c];
+        if (a.dcZero) r = r - mean[p];
+        if (a.envAuto) r = r / a.envAuto[c];
+        cv[p] = r;
+    }
+    float prod = cv[0] * cv[1];
+    prod = prod > 1e-12f ? prod : 1e-12f;
+    const float nf = sqrtf(prod);
+    double *o3[3] = {a.oAA + off, a.oBB + off, a.oAB + off};
+    for (int p = 0; p < 3; ++p) {
+        const float *env = p < 2 ? a.envAuto : a.envAB;
+        for (int i = t; i < P; i += blockDim.x) {
+            float r = s3[p][i] / a.nImages;
+            if (b3[p]) r = r - b3[p][i];
+            if (a.dcZero) r = r - mean[p];
+            if (env) r = r / env[i];
+            if (p == 2) r = r * a.abScale;
+            o3[p][i] = (double)(r / nf);
+        }
+    }
+}
+
+// Displacements and stresses from the k-space fit rows
+// (single_pass_accumulator.py:1067-1290 without the optional outlier
+// methods): status != 0 or the 3/4-window rule gives NaN; masked windows 0.
+__global__ void ens_extract_kernel(const double *gauss, const int *status, const unsigned char *mask, int nWin,
+                                   float cx1, float cy1, float maxdx, float maxdy, float *ux, float *uy,
+                                   float *uu, float *vv, float *uv, float *peak, int *reason)
+{
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= nWin) return;
+    const double *g = gauss + (size_t)w * 16;
+    auto safe = [](double v, float lim) {
+        float f = (float)(v < -lim ? -lim : (v > lim ? lim : v));
+        return isfinite(f) ? f : 0.0f;
+    };
+    float x = (float)g[14] - cx1, y = (float)g[15] - cy1;
+    int st = status[w];
+    if (!isfinite(x) || !isfinite(y) || fabsf(x) > maxdx || fabsf(y) > maxdy) {
+        x = NAN;
+        y = NAN;
+        st = 6;
+    }
+    const float aA = safe(g[0], 1e10f), aB = safe(g[1], 1e10f), aAB = safe(g[2], 1e10f);
+    float gm = aA * aB;
+    gm = sqrtf(gm > 1e-12f ? gm : 1e-12f);
+    float ph = aAB / gm;
+    ph = ph < 0.0f ? 0.0f : (ph > 1.0f ? 1.0f : ph);
+    const float sAx = safe(g[6], 1e6f), sAy = safe(g[7], 1e6f), sAxy = safe(g[8], 1e6f);
+    float U = safe(g[9], 1e6f) - sAx, V = safe(g[10], 1e6f) - sAy, C = safe(g[11], 1e6f) - sAxy;
+    U = U > 0.0f ? U : 0.0f;
+    V = V > 0.0f ? V : 0.0f;
+    const bool m = mask && mask[w];
+    if (m) {
+        x = y = U = V = C = ph = 0.0f;
+        st = -1;
+    } else if (st != 0) {
+        x = y = U = V = C = ph = NAN;
+    }
+    ux[w] = x;
+    uy[w] = y;
+    uu[w] = U;
+    vv[w] = V;
+    uv[w] = C;
+    peak[w] = ph;
+    reason[w] = st;
+}
+
+}  // namespace pivamd
+
+extern "C" int pivamd_sum_images(const float *d_imgs, int n, unsigned long hw, double *d_sum, hipStream_t st)
+{
+    if (n <= 0 || hw == 0) return PIVAMD_OK;
+    const unsigned long blocks = (hw + 255) / 256;
+    hipLaunchKernelGGL(sum_images_kernel, dim3(blocks < 4096 ? blocks : 4096), dim3(256), 0, st, d_imgs, n,
+                       (size_t)hw, d_sum);
+    return hipGetLastError() == hipSuccess ? PIVAMD_OK : PIVAMD_ERR_LAUNCH;
+}
+
+extern "C" int pivamd_mean_image(const double *d_sum, double n, unsigned long hw, float *d_out, hipStream_t st)
+{
+    if (hw == 0) return PIVAMD_OK;
+    const unsigned long blocks = (hw + 255) / 256;
+    hipLaunchKernelGGL(mean_image_kernel, dim3(blocks < 4096 ? blocks : 4096), dim3(256), 0, st, d_sum, n,
+                       (size_t)hw, d_out);
+    return hipGetLastError() == hipSuccess ? PIVAMD_OK : PIVAMD_ERR_LAUNCH;
+}
+
+extern "C" int pivamd_ensemble_finalize(const float *d_sAA, const float *d_sBB, const float *d_sAB,
+                                        const float *d_bAA, const float *d_bBB, const float *d_bAB,
+                                        const float *d_envAuto, const float *d_envAB, int n_windows,
+                                        int corr_h, int corr_w, float n_images, float ab_scale, int dc_zero,
+                                        double *d_oAA, double *d_oBB, double *d_oAB, hipStream_t st)
+{
+    if (n_windows <= 0) return PIVAMD_OK;
+    if (corr_h <= 0 || corr_w <= 0 || !(n_images > 0.0f)) return PIVAMD_ERR_ARG;
+    FinArgs a{d_sAA, d_sBB, d_sAB, d_bAA, d_bBB, d_bAB, d_envAuto, d_envAB, d_oAA, d_oBB, d_oAB,
+              n_images, ab_scale, corr_h * corr_w, (corr_h / 2) * corr_w + corr_w / 2, dc_zero};
+    hipLaunchKernelGGL(ens_finalize_kernel, dim3(n_windows), dim3(256), 0, st, a);
+    return hipGetLastError() == hipSuccess ? PIVAMD_OK : PIVAMD_ERR_LAUNCH;
+}
+
+extern "C" int pivamd_ensemble_extract(const double *d_gauss, const int *d_status, const unsigned char *d_mask,
+                                       int n_windows, int corr_h, int corr_w, float *d_ux, float *d_uy,
+                                       float *d_uu, float *d_vv, float *d_uv, float *d_peak, int *d_reason,
+                                       hipStream_t st)
+{
+    if (n_windows <= 0) return PIVAMD_OK;
+    const float cx1 = corr_w / 2.0f + 1.0f, cy1 = corr_h / 2.0f + 1.0f;
+    hipLaunchKernelGGL(ens_extract_kernel, dim3((n_windows + 255) / 256), dim3(256), 0, st, d_gauss, d_status,
+                       d_mask, n_windows, cx1, cy1, 0.75f * corr_w, 0.75f * corr_h, d_ux, d_uy, d_uu, d_vv, d_uv,
+                       d_peak, d_reason);
+    return hipGetLastError() == hipSuccess ? PIVAMD_OK : PIVAMD_ERR_LAUNCH;
+}

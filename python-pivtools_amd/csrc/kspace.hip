@@ -58,6 +58,7 @@ struct KsArgs {
     const double *aa, *bb, *ab;
     const unsigned char *mask;
     const double *pwin;
+    size_t pwinStride;   // H * W (one shape per window) or 0 (one shared shape)
     int nWin, H, W;
     const int *nproc;
     double *gauss, *init;
@@ -597,7 +598,7 @@ __global__ __launch_bounds__(TPB) void kspace_kernel(KsArgs a)
             const double f = s.fr[b];
             const double fc = f > 1e-30 ? f : 1e-30;
             if (coloured) {
-                const double p = a.pwin[off + (size_t)((r + H / 2) % H) * W + (c + W / 2) % W];
+                const double p = a.pwin[(size_t)w * a.pwinStride + (size_t)((r + H / 2) % H) * W + (c + W / 2) % W];
                 s.fr[b] = p / fc;
                 const double kr2 = sh.ky[r] * sh.ky[r] + sh.kx[c] * sh.kx[c];
                 const bool tail = kr2 >= kTailKr * kTailKr;
@@ -825,19 +826,19 @@ static int kspace_run(const double *d_aa, const double *d_bb, const double *d_ab
                       int corr_w, int use_kx4, int use_ky4, double *d_gauss, int *d_status, double *d_init,
                       double *d_gain, double *d_n0, double *d_b4x, double *d_b4y, double *d_cpp,
                       int *d_nvalid, int *d_iter, unsigned char *d_conv, double *dbg_dv, double *dbg_n0,
-                      void *d_work, size_t work_bytes, hipStream_t st);
+                      int pwin_shared, void *d_work, size_t work_bytes, hipStream_t st);
 
 extern "C" int pivamd_kspace_fit(const double *d_aa, const double *d_bb, const double *d_ab,
                                  const unsigned char *d_mask, const double *d_pwin, int n_windows,
                                  int corr_h, int corr_w, int use_kx4, int use_ky4, double *d_gauss,
                                  int *d_status, double *d_init, double *d_gain, double *d_n0,
                                  double *d_b4x, double *d_b4y, double *d_cpp, int *d_nvalid,
-                                 int *d_iter, unsigned char *d_conv, void *d_work, size_t work_bytes,
-                                 hipStream_t st)
+                                 int *d_iter, unsigned char *d_conv, int pwin_shared, void *d_work,
+                                 size_t work_bytes, hipStream_t st)
 {
     return kspace_run(d_aa, d_bb, d_ab, d_mask, d_pwin, n_windows, corr_h, corr_w, use_kx4, use_ky4, d_gauss,
                       d_status, d_init, d_gain, d_n0, d_b4x, d_b4y, d_cpp, d_nvalid, d_iter, d_conv, nullptr,
-                      nullptr, d_work, work_bytes, st);
+                      nullptr, pwin_shared, d_work, work_bytes, st);
 }
 
 // Prep-only pass over one window: D in natural order and the coloured N0
@@ -855,7 +856,7 @@ extern "C" int pivamd_kspace_debug_prep(const double *d_aa, const double *d_bb, 
     unsigned char *msk = (unsigned char *)(ist + 4);
     if (hipMemsetAsync(msk, 0, 1, st) != hipSuccess) return PIVAMD_ERR_LAUNCH;
     return kspace_run(d_aa, d_bb, d_ab, msk, d_pcentred, 1, corr_h, corr_w, 0, 0, g, ist, ig, dg, dg + 1,
-                      dg + 2, dg + 3, dg + 4, ist + 1, ist + 2, msk + 1, d_dv, d_n0seed, d_work, need, st);
+                      dg + 2, dg + 3, dg + 4, ist + 1, ist + 2, msk + 1, d_dv, d_n0seed, 0, d_work, need, st);
 }
 
 static int kspace_run(const double *d_aa, const double *d_bb, const double *d_ab,
@@ -863,7 +864,7 @@ static int kspace_run(const double *d_aa, const double *d_bb, const double *d_ab
                       int corr_w, int use_kx4, int use_ky4, double *d_gauss, int *d_status, double *d_init,
                       double *d_gain, double *d_n0, double *d_b4x, double *d_b4y, double *d_cpp,
                       int *d_nvalid, int *d_iter, unsigned char *d_conv, double *dbg_dv, double *dbg_n0,
-                      void *d_work, size_t work_bytes, hipStream_t st)
+                      int pwin_shared, void *d_work, size_t work_bytes, hipStream_t st)
 {
     if (!axis_ok(corr_h) || !axis_ok(corr_w)) return PIVAMD_ERR_SIZE;
     if (n_windows < 0 || (n_windows > 0 && (!d_aa || !d_bb || !d_ab || !d_mask))) return PIVAMD_ERR_ARG;
@@ -875,6 +876,7 @@ static int kspace_run(const double *d_aa, const double *d_bb, const double *d_ab
     a.ab = d_ab;
     a.mask = d_mask;
     a.pwin = d_pwin;
+    a.pwinStride = pwin_shared ? 0 : (size_t)corr_h * corr_w;
     a.nWin = n_windows;
     a.H = corr_h;
     a.W = corr_w;

@@ -42,7 +42,16 @@ int pivamd_kspace_fit(const double *d_aa, const double *d_bb, const double *d_ab
                       const double *d_pwin, int n_windows, int corr_h, int corr_w, int use_kx4, int use_ky4,
                       double *d_gauss, int *d_status, double *d_init, double *d_gain, double *d_n0,
                       double *d_b4x, double *d_b4y, double *d_cpp, int *d_nvalid, int *d_iter,
-                      unsigned char *d_conv, void *d_work, size_t work_bytes, hipStream_t st);
+                      unsigned char *d_conv, int pwin_shared, void *d_work, size_t work_bytes, hipStream_t st);
+int pivamd_sum_images(const float *d_imgs, int n, unsigned long hw, double *d_sum, hipStream_t st);
+int pivamd_mean_image(const double *d_sum, double n, unsigned long hw, float *d_out, hipStream_t st);
+int pivamd_ensemble_finalize(const float *d_sAA, const float *d_sBB, const float *d_sAB, const float *d_bAA,
+                             const float *d_bBB, const float *d_bAB, const float *d_envAuto, const float *d_envAB,
+                             int n_windows, int corr_h, int corr_w, float n_images, float ab_scale, int dc_zero,
+                             double *d_oAA, double *d_oBB, double *d_oAB, hipStream_t st);
+int pivamd_ensemble_extract(const double *d_gauss, const int *d_status, const unsigned char *d_mask, int n_windows,
+                            int corr_h, int corr_w, float *d_ux, float *d_uy, float *d_uu, float *d_vv, float *d_uv,
+                            float *d_peak, int *d_reason, hipStream_t st);
 int pivamd_kspace_fft2_centered(const double *d_plane, int corr_h, int corr_w, double *d_re, double *d_im,
                                 void *d_work, size_t work_bytes, hipStream_t st);
 int pivamd_kspace_debug_prep(const double *d_aa, const double *d_bb, const double *d_ab,

@@ -63,15 +63,31 @@ def core() -> ctypes.CDLL:
         L.pivamd_peakfit.argtypes = [_vp, _i, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp]
         L.pivamd_peakfit.restype = _i
         for name, argt in GLUE_SIGNATURES.items():
-            if hasattr(L, name):
-                f = getattr(L, name)
-                f.argtypes = argt
-                f.restype = _i
+            f = getattr(L, name)
+            f.argtypes = argt
+            f.restype = _i
         L.pivamd_kspace_workspace.argtypes = [_i, _i, _i]
         L.pivamd_kspace_workspace.restype = ctypes.c_size_t
-        L.pivamd_kspace_fit.argtypes = [_vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i] + [_vp] * 12 + [
-            ctypes.c_size_t, _vp]
+        L.pivamd_kspace_fit.argtypes = [_vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i] + [_vp] * 11 + [
+            _i, _vp, ctypes.c_size_t, _vp]
+        L.pivamd_sum_images.argtypes = [_vp, _i, ctypes.c_ulong, _vp, _vp]
+        L.pivamd_mean_image.argtypes = [_vp, ctypes.c_double, ctypes.c_ulong, _vp, _vp]
+        L.pivamd_ensemble_finalize.argtypes = [_vp] * 8 + [_i, _i, _i, ctypes.c_float, ctypes.c_float, _i,
+                                                           _vp, _vp, _vp, _vp]
+        L.pivamd_ensemble_extract.argtypes = [_vp, _vp, _vp, _i, _i, _i] + [_vp] * 8
+        L.pivamd_ensemble_workspace.argtypes = [_i, _i, _i, _i]
+        L.pivamd_ensemble_workspace.restype = ctypes.c_size_t
+        L.pivamd_accumulate.argtypes = [_vp, _vp, _vp, _i, _i, _i, _vp, _vp, _i, _i, _vp, _vp, _vp, _vp, _i, _i,
+                                        _i, _i, _i, _i, _i, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                        _vp, _vp, _vp, _vp, ctypes.c_size_t, _vp]
+        for f in ("pivamd_sum_images", "pivamd_mean_image", "pivamd_ensemble_finalize", "pivamd_ensemble_extract",
+                  "pivamd_accumulate"):
+            getattr(L, f).restype = _i
         L.pivamd_kspace_fit.restype = _i
+        L.pivamd_kspace_fft2_centered.argtypes = [_vp, _i, _i, _vp, _vp, _vp, ctypes.c_size_t, _vp]
+        L.pivamd_kspace_fft2_centered.restype = _i
+        L.pivamd_kspace_debug_prep.argtypes = [_vp, _vp, _vp, _vp, _i, _i, _vp, _vp, _vp, ctypes.c_size_t, _vp]
+        L.pivamd_kspace_debug_prep.restype = _i
         if hasattr(L, "pivamd_accumulate_triple"):
             L.pivamd_accumulate_triple.argtypes = [
                 _vp, _vp, _vp, _i, _i, _i, _vp, _vp, _i, _i, _vp, _vp, _vp, _vp, _i, _i, _i, _i,
@@ -82,8 +98,20 @@ def core() -> ctypes.CDLL:
     return _CORE
 
 
-# device-pointer glue entry points (multipass loop on the device)
-GLUE_SIGNATURES: dict = {}
+# device-pointer glue entry points (multipass loop on the device); bound
+# with the library itself so that no caller can reach one without argtypes
+_sz = ctypes.c_size_t
+_f = ctypes.c_float
+GLUE_SIGNATURES: dict = {
+    "pivamd_u16_to_f32": [_vp, _vp, _sz, _vp],
+    "pivamd_copy_f32": [_vp, _vp, _sz, _vp],
+    "pivamd_gate": [_vp, _vp, _vp, _vp, _vp, _i, _i, _f, _f, _vp, _vp, _vp, _vp, _vp],
+    "pivamd_median": [_vp, _vp, _vp, _i, _i, _i, _f, _f, _i, _vp, _vp, _vp, _vp, _vp],
+    "pivamd_infill_nearest": [_vp, _vp, _i, _i, _i, _f, _f, _vp, _vp, _vp],
+    "pivamd_pad_edge": [_vp, _i, _i, _i, _i, _i, _i, _i, _vp, _vp],
+    "pivamd_smooth_axis": [_vp, _i, _i, _i, _i, _vp, _i, _vp, _vp],
+    "pivamd_remap_cubic": [_vp, _i, _i, _i, _vp, _vp, _i, _i, _vp, _vp],
+}
 
 
 def bulkxcorr2d_lib() -> ctypes.CDLL:

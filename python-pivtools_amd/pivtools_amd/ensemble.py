@@ -1,0 +1,214 @@
+"""Device-resident ensemble PIV pass (single pass, standard windows).
+
+Follows the reference's streaming ensemble pass:
+
+* per batch (``cpu_ensemble.py:826-1084``): image sums for the mean images
+  and ``bulkxcorr2d_accumulate_triple`` of every pair into the AB, AA, BB
+  plane sums (weights wA / wB for AB, wB for both autos);
+* finalize (``single_pass_accumulator.py:508-1290``): averages, background
+  correlation of the mean images (``correlation`` base method), optional
+  plane-mean removal (``+dc_zero``) and envelope divide, normalisation by
+  the auto peaks, the coloured noise-floor shape P (flat for ``flat``), the
+  k-space LM fit, and displacement / Reynolds-stress extraction.
+
+Every step is a HIP kernel of libpivtools_amd.so on torch's current stream;
+only the floor shape P is host work (one small analytic model per pass).
+The plane sums stay in HBM for the whole ensemble; the fitter inputs are
+FP64.  Multi-pass ensembles (predictor warp between passes) reuse the
+instantaneous warp and are not wired here.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _native, floor_psd
+from .window import compute_window_centers, window_weight
+
+BG_METHODS = {"correlation", "window_mean", "correlation+window_mean", "correlation+dc_zero", "none"}
+
+
+@dataclass
+class EnsembleResult:
+    """Per-window outputs of one ensemble pass (device tensors, (ny, nx))."""
+
+    ux: torch.Tensor
+    uy: torch.Tensor
+    UU: torch.Tensor
+    VV: torch.Tensor
+    UV: torch.Tensor
+    peakheight: torch.Tensor
+    nan_reason: torch.Tensor
+    gauss: torch.Tensor          # (n_windows, 16) FP64 rows of the k-space fit
+    status: torch.Tensor
+    init: torch.Tensor
+    diag: dict
+    planes: tuple                # FP64 (AA, BB, AB) normalised planes
+    n_images: int
+
+
+class EnsemblePIV:
+    def __init__(self, image_shape, window_size=(32, 32), overlap=75, window_type="square",
+                 bg_method="correlation", kspace_floor="coloured", kspace_shape="gaussian",
+                 envelope_divide=False, per_pair_normalization=False, warp_kernel="cubic",
+                 mask=None, device=None):
+        if bg_method not in BG_METHODS:
+            raise ValueError(f"background method {bg_method!r} not in {sorted(BG_METHODS)}")
+        if kspace_floor not in ("coloured", "flat"):
+            raise ValueError(f"kspace_floor {kspace_floor!r}")
+        if per_pair_normalization and bg_method != "window_mean":
+            raise ValueError("per-pair normalisation requires the window_mean background method")
+        L = _native.core()
+        if not L.pivamd_size_supported(int(window_size[0]), int(window_size[1])):
+            raise ValueError(f"window {window_size} has no device kernel")
+        self.L = L
+        self.dev = torch.device(device) if device is not None else torch.device("cuda")
+        self.H, self.W = (int(image_shape[0]), int(image_shape[1]))
+        self.win = (int(window_size[0]), int(window_size[1]))
+        self.grid = compute_window_centers((self.H, self.W), self.win, overlap)
+        self.ny, self.nx = int(self.grid.n_win_y), int(self.grid.n_win_x)
+        self.nwin = self.ny * self.nx
+        self.bg_method = bg_method
+        self.mean_sub = bg_method in ("window_mean", "correlation+window_mean")
+        self.bg_correlation = bg_method.startswith("correlation")
+        self.dc_zero = bg_method.endswith("dc_zero")
+        self.per_pair_norm = per_pair_normalization
+        self.floor = kspace_floor
+        self.kx4 = int("kx4" in kspace_shape)
+        self.ky4 = int("ky4" in kspace_shape)
+        self.envelope_divide = envelope_divide
+        self.warp_kernel = warp_kernel
+        d, f32 = self.dev, torch.float32
+        w = window_weight(self.win, window_type)
+        self.w_host = w
+        self.w = torch.as_tensor(w, device=d)
+        self.sum_w = float(np.float32(w.sum(dtype=np.float32)))
+        self.ctr_x = torch.as_tensor(self.grid.win_ctrs_x.astype(np.float32), device=d)
+        self.ctr_y = torch.as_tensor(self.grid.win_ctrs_y.astype(np.float32), device=d)
+        self.mask_host = (np.zeros(self.nwin, bool) if mask is None
+                          else np.asarray(mask, bool).reshape(-1))
+        self.mask_f = torch.as_tensor(self.mask_host.astype(np.float32), device=d)
+        self.mask_u8 = torch.as_tensor(self.mask_host.astype(np.uint8), device=d)
+        P = self.win[0] * self.win[1]
+        self.sums = [torch.zeros(self.nwin * P, dtype=f32, device=d) for _ in range(3)]   # AB, AA, BB
+        self.sum_a = torch.zeros(self.H * self.W, dtype=torch.float64, device=d)
+        self.sum_b = torch.zeros(self.H * self.W, dtype=torch.float64, device=d)
+        wb = int(L.pivamd_ensemble_workspace(self.ny, self.nx, self.win[0], self.win[1]))
+        self.work = torch.empty(max(wb, 8), dtype=torch.uint8, device=d)
+        self._f32 = [None, None]
+        self.n_images = 0
+
+    # ------------------------------------------------------------------
+    def _stream(self):
+        return torch.cuda.current_stream(self.dev).cuda_stream
+
+    def _frames(self, x: torch.Tensor, slot: int) -> torch.Tensor:
+        if x.dtype == torch.float32:
+            return x.contiguous()
+        if x.dtype in (torch.int16, torch.uint16):
+            buf = self._f32[slot]
+            if buf is None or buf.numel() < x.numel():
+                buf = self._f32[slot] = torch.empty(x.numel(), dtype=torch.float32, device=self.dev)
+            out = buf[:x.numel()].view(x.shape)
+            _native.check(self.L.pivamd_u16_to_f32(x.data_ptr(), out.data_ptr(), x.numel(), self._stream()),
+                          "u16_to_f32")
+            return out
+        raise TypeError(f"unsupported image dtype {x.dtype}")
+
+    def _accumulate(self, a, b, n, outs, per_pair_norm):
+        st = self._stream()
+        P = self.win
+        _native.check(self.L.pivamd_accumulate(
+            a.data_ptr(), b.data_ptr(), self.mask_f.data_ptr(), self.H, self.W, n, self.ctr_x.data_ptr(),
+            self.ctr_y.data_ptr(), self.ny, self.nx, self.w.data_ptr(), self.w.data_ptr(), self.w.data_ptr(),
+            self.w.data_ptr(), P[0], P[1], P[0], P[1], int(self.mean_sub), int(per_pair_norm),
+            1, self.sum_w, self.sum_w, self.sum_w, outs[0].data_ptr(), outs[1].data_ptr(), outs[2].data_ptr(),
+            self.work.data_ptr(), self.work.numel(), st), "accumulate_triple")
+
+    def accumulate(self, img_a: torch.Tensor, img_b: torch.Tensor) -> None:
+        """Add a batch of pairs (n, H, W) (float32 or 12-bit counts in uint16)."""
+        if img_a.shape != img_b.shape or img_a.dim() != 3 or tuple(img_a.shape[1:]) != (self.H, self.W):
+            raise ValueError(f"expected (n, {self.H}, {self.W}) frames, got {tuple(img_a.shape)}")
+        n = int(img_a.shape[0])
+        a = self._frames(img_a, 0)
+        b = self._frames(img_b, 1)
+        st = self._stream()
+        hw = self.H * self.W
+        _native.check(self.L.pivamd_sum_images(a.data_ptr(), n, hw, self.sum_a.data_ptr(), st), "sum_images")
+        _native.check(self.L.pivamd_sum_images(b.data_ptr(), n, hw, self.sum_b.data_ptr(), st), "sum_images")
+        self._accumulate(a, b, n, self.sums, self.per_pair_norm)
+        self.n_images += n
+
+    # ------------------------------------------------------------------
+    def floor_shape(self):
+        """Per-pass floor shape P in the centred layout (one shared shape:
+        a single pass has no predictor, so every window sits at f = 0)."""
+        corr = self.win
+        env_auto = floor_psd.pair_envelope(self.w_host, self.w_host, corr)
+        env_for_floor = env_auto if self.envelope_divide else np.ones_like(env_auto)
+        fs, grid = floor_psd.build_p_grid(env_for_floor, self.warp_kernel, "window_mean" in self.bg_method,
+                                          self.w_host, n_fracs=2)
+        p = floor_psd.interp_p(fs, grid, [0.0], [0.0])[0].astype(np.float32)
+        return p.astype(np.float64)
+
+    def finalize(self) -> EnsembleResult:
+        if self.n_images == 0:
+            raise RuntimeError("no pairs accumulated")
+        L, d, st = self.L, self.dev, self._stream()
+        h, w = self.win
+        P = h * w
+        hw = self.H * self.W
+        f32, f64 = torch.float32, torch.float64
+        bg = None
+        if self.bg_correlation:
+            ma = torch.empty(hw, dtype=f32, device=d)
+            mb = torch.empty(hw, dtype=f32, device=d)
+            _native.check(L.pivamd_mean_image(self.sum_a.data_ptr(), float(self.n_images), hw, ma.data_ptr(), st),
+                          "mean_image")
+            _native.check(L.pivamd_mean_image(self.sum_b.data_ptr(), float(self.n_images), hw, mb.data_ptr(), st),
+                          "mean_image")
+            bg = [torch.zeros(self.nwin * P, dtype=f32, device=d) for _ in range(3)]
+            self._accumulate(ma, mb, 1, bg, False)
+        env = None
+        if self.envelope_divide:
+            e = floor_psd.pair_envelope(self.w_host, self.w_host, self.win).astype(np.float32)
+            env = torch.as_tensor(e, device=d)
+        planes = [torch.empty(self.nwin * P, dtype=f64, device=d) for _ in range(3)]   # AA, BB, AB
+        _native.check(L.pivamd_ensemble_finalize(
+            self.sums[1].data_ptr(), self.sums[2].data_ptr(), self.sums[0].data_ptr(),
+            None if bg is None else bg[1].data_ptr(), None if bg is None else bg[2].data_ptr(),
+            None if bg is None else bg[0].data_ptr(), None if env is None else env.data_ptr(),
+            None if env is None else env.data_ptr(), self.nwin, h, w, float(np.float32(self.n_images)), 1.0,
+            int(self.dc_zero), planes[0].data_ptr(), planes[1].data_ptr(), planes[2].data_ptr(), st),
+            "ensemble_finalize")
+        pw = None
+        if self.floor == "coloured":
+            pw = torch.as_tensor(self.floor_shape(), device=d)
+        n = self.nwin
+        gauss = torch.empty((n, 16), dtype=f64, device=d)
+        init = torch.empty((n, 16), dtype=f64, device=d)
+        status = torch.empty(n, dtype=torch.int32, device=d)
+        diag = {k: torch.empty(n, dtype=f64, device=d) for k in ("gain", "N0", "b4x", "b4y", "cost_per_pt")}
+        diag.update({k: torch.empty(n, dtype=torch.int32, device=d) for k in ("n_valid", "iter")})
+        diag["conv"] = torch.empty(n, dtype=torch.uint8, device=d)
+        wb = int(L.pivamd_kspace_workspace(n, h, w))
+        work = torch.empty(max(wb, 256), dtype=torch.uint8, device=d)
+        _native.check(L.pivamd_kspace_fit(
+            planes[0].data_ptr(), planes[1].data_ptr(), planes[2].data_ptr(), self.mask_u8.data_ptr(),
+            None if pw is None else pw.data_ptr(), n, h, w, self.kx4, self.ky4, gauss.data_ptr(),
+            status.data_ptr(), init.data_ptr(), diag["gain"].data_ptr(), diag["N0"].data_ptr(),
+            diag["b4x"].data_ptr(), diag["b4y"].data_ptr(), diag["cost_per_pt"].data_ptr(),
+            diag["n_valid"].data_ptr(), diag["iter"].data_ptr(), diag["conv"].data_ptr(), 1,
+            work.data_ptr(), work.numel(), st), "kspace_fit")
+        out = [torch.empty(n, dtype=f32, device=d) for _ in range(6)]
+        reason = torch.empty(n, dtype=torch.int32, device=d)
+        _native.check(L.pivamd_ensemble_extract(
+            gauss.data_ptr(), status.data_ptr(), self.mask_u8.data_ptr(), n, h, w, *[o.data_ptr() for o in out],
+            reason.data_ptr(), st), "ensemble_extract")
+        shp = (self.ny, self.nx)
+        return EnsembleResult(out[0].view(shp), out[1].view(shp), out[2].view(shp), out[3].view(shp),
+                              out[4].view(shp), out[5].view(shp), reason.view(shp), gauss, status, init, diag,
+                              tuple(p.view(n, h, w) for p in planes), self.n_images)

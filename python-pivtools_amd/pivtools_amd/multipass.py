@@ -32,17 +32,6 @@ _i = ctypes.c_int
 _f = ctypes.c_float
 _sz = ctypes.c_size_t
 
-_GLUE = {
-    "pivamd_u16_to_f32": [_vp, _vp, _sz, _vp],
-    "pivamd_copy_f32": [_vp, _vp, _sz, _vp],
-    "pivamd_gate": [_vp, _vp, _vp, _vp, _vp, _i, _i, _f, _f, _vp, _vp, _vp, _vp, _vp],
-    "pivamd_median": [_vp, _vp, _vp, _i, _i, _i, _f, _f, _i, _vp, _vp, _vp, _vp, _vp],
-    "pivamd_infill_nearest": [_vp, _vp, _i, _i, _i, _f, _f, _vp, _vp, _vp],
-    "pivamd_pad_edge": [_vp, _i, _i, _i, _i, _i, _i, _i, _vp, _vp],
-    "pivamd_smooth_axis": [_vp, _i, _i, _i, _i, _vp, _i, _vp, _vp],
-    "pivamd_remap_cubic": [_vp, _i, _i, _i, _vp, _vp, _i, _i, _vp, _vp],
-}
-_native.GLUE_SIGNATURES.update(_GLUE)
 
 PEAK_FINDERS = {"gauss3": 3, "gauss4": 4, "gauss5": 5, "gauss6": 6}
 

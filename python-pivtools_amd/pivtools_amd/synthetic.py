@@ -14,6 +14,11 @@ Workloads (NumPy PCG64 seeds = 1000 * workload + pair index):
                     peak 8 px) + uniform 2 px in x; 12-bit counts with
                     Poisson shot noise (gain 3000 counts per unit
                     intensity), clipped to 4095, stored uint16
+  ensemble_1024     1024 x 1024 f32, particles as single_pass_1024; mean
+                    profile u(y) = 6 ln(1 + y/20) px plus a zero-mean field
+                    of 256 random Fourier modes (wavelengths 16-512 px,
+                    rms 0.6 px in u, 0.4 px in v, u-v correlation -0.3),
+                    redrawn for every pair
 """
 
 from __future__ import annotations
@@ -110,3 +115,41 @@ def multipass_pair(seed=1000, shape=(2160, 2560), ppp=0.03, device="cpu", gain=G
         out.append(torch.clamp(counts, 0, 4095).to(torch.int16).view(torch.int16))
     # int16 storage of 0..4095 is bit-identical to uint16
     return out[0], out[1]
+
+
+def fourier_fluctuation(rng, n_modes=256, lam=(16.0, 512.0), rms=(0.6, 0.4), corr=-0.3):
+    """A random-Fourier-mode fluctuation field (u', v') as a callable of (x, y).
+
+    Two independent unit-rms fields f1, f2 (equal-amplitude modes, random
+    direction and phase, log-uniform wavelength); u' = rms_u f1 and
+    v' = rms_v (corr f1 + sqrt(1 - corr^2) f2)."""
+    def field():
+        k = 2 * np.pi / np.exp(rng.uniform(np.log(lam[0]), np.log(lam[1]), n_modes))
+        th = rng.uniform(0, 2 * np.pi, n_modes)
+        ph = rng.uniform(0, 2 * np.pi, n_modes)
+        return k * np.cos(th), k * np.sin(th), ph
+    m1, m2 = field(), field()
+    amp = math.sqrt(2.0 / n_modes)
+
+    def ev(m, x, y):
+        kx, ky, ph = m
+        return amp * np.cos(np.outer(x, kx) + np.outer(y, ky) + ph).sum(axis=1)
+
+    def uv(x, y):
+        f1, f2 = ev(m1, x, y), ev(m2, x, y)
+        return rms[0] * f1, rms[1] * (corr * f1 + math.sqrt(1 - corr * corr) * f2)
+    return uv
+
+
+def ensemble_pair(seed=2000, shape=(1024, 1024), ppp=0.02, noise=0.01, log_scale=6.0, device="cpu"):
+    """One f32 pair of the ensemble workload (log-law mean + random modes)."""
+    rng = np.random.default_rng(seed)
+    x, y, inten = _positions(rng, shape, ppp)
+    up, vp = fourier_fluctuation(rng)(x, y)
+    dx = log_scale * np.log1p(np.maximum(y, 0.0) / 20.0) + up
+    dy = vp
+    a = render(shape, x, y, inten, device=device)
+    b = render(shape, x + dx, y + dy, inten, device=device)
+    na = torch.as_tensor(rng.normal(0, noise, shape), dtype=torch.float32, device=device)
+    nb = torch.as_tensor(rng.normal(0, noise, shape), dtype=torch.float32, device=device)
+    return a + na, b + nb

@@ -56,3 +56,12 @@ def test_reference_style_bindings_load():
     assert _native.kspacefit_lib().kspace_lm_fit_batch.restype is ctypes.c_int
     assert _native.core().pivamd_size_supported(32, 32) == 1
     assert _native.core().pivamd_size_supported(20, 20) == 0
+
+
+def test_every_device_entry_point_has_argtypes():
+    """ctypes passes untyped Python ints as 32-bit C int, which truncates
+    device pointers: every pivamd_* entry point must be bound with argtypes
+    as soon as the library is loaded, whichever module loads it first."""
+    L = _native.core()
+    missing = [n for n in _declared() if n.startswith("pivamd_") and getattr(L, n).argtypes is None]
+    assert not missing, missing

@@ -1,0 +1,130 @@
+"""Whole ensemble pass on the device (accumulate -> finalize -> k-space fit
+-> extraction) against the oracle chain on the same seeded pairs.
+
+Parity is stated in two parts, as for the LM peak fit:
+* the finalized FP64 planes agree with the oracle's to 1e-5 relative L2
+  (the device keeps FP64 spectral sums where the reference sums float32
+  planes, so the planes differ at the float32 rounding level);
+* the device fit of the device planes equals the oracle fit of those same
+  planes to the reference's own C-vs-oracle gate (identical statuses,
+  |dSigma|, |dmu| < 1e-8);
+and end to end the statuses and the stresses agree within the tolerance the
+plane differences allow.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+import ensemble_oracle as EO
+import oracle as O
+from pivtools_amd import synthetic as S
+from pivtools_amd.ensemble import EnsemblePIV
+from pivtools_amd.window import compute_window_centers, window_weight
+
+pytestmark = pytest.mark.gpu
+dev = torch.device("cuda")
+
+
+def _pairs(n, shape, seed0=500, log_scale=1.5):
+    A, B = [], []
+    for k in range(n):
+        a, b = S.ensemble_pair(seed0 + k, shape=shape, log_scale=log_scale)
+        A.append(a.numpy())
+        B.append(b.numpy())
+    return np.stack(A), np.stack(B)
+
+
+def _rel(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-30))
+
+
+@pytest.mark.parametrize("bg,floor", [("correlation", "coloured"), ("correlation", "flat"),
+                                      ("window_mean", "coloured"), ("correlation+dc_zero", "flat")])
+def test_ensemble_pass_matches_oracle(bg, floor):
+    shape, win = (192, 192), (32, 32)
+    A, B = _pairs(12, shape)
+    print(f"ensemble pass bg={bg} floor={floor}", flush=True)
+    ens = EnsemblePIV(shape, win, 75, "square", bg_method=bg, kspace_floor=floor)
+    for i in range(0, 12, 5):        # ragged batches: 5, 5, 2
+        ens.accumulate(torch.as_tensor(A[i:i + 5], device=dev), torch.as_tensor(B[i:i + 5], device=dev))
+    res = ens.finalize()
+    torch.cuda.synchronize()
+    wc = compute_window_centers(shape, win, 75)
+    w = window_weight(win, "square")
+    pc = ens.floor_shape() if floor == "coloured" else None
+    ref = EO.ensemble_pass(A, B, wc.win_ctrs_x, wc.win_ctrs_y, win, w, bg_method=bg, p_centred=pc)
+    dplanes = [p.cpu().numpy() for p in res.planes]
+    for dp, rp in zip(dplanes, ref["planes"]):
+        assert _rel(dp, rp) < 1e-5
+    # the fitter on identical inputs
+    pw = None if pc is None else np.tile(pc.ravel(), ens.nwin)
+    ret, o = O.kspace_fit(dplanes[0], dplanes[1], dplanes[2], np.zeros(ens.nwin, bool), win, p_win=pw)
+    st = res.status.cpu().numpy()
+    g = res.gauss.cpu().numpy()
+    np.testing.assert_array_equal(st, o["status"])
+    ok = st == 0
+    assert ok.sum() > 0.5 * ens.nwin
+    assert np.abs(g[ok, 9:12] - o["gauss"][ok, 9:12]).max() < 1e-8
+    assert np.abs(g[ok, 14:16] - o["gauss"][ok, 14:16]).max() < 1e-8
+    # end to end
+    same = (st == 0) & (ref["fit"]["status"] == 0)
+    assert np.mean(st != ref["fit"]["status"]) < 0.02
+    for k in ("ux", "uy", "UU", "VV", "UV"):
+        d = res.__dict__[k].cpu().numpy().ravel()[same] - ref[k].ravel()[same]
+        assert np.abs(d).max() < 1e-3, (k, np.abs(d).max())
+    agree = st == ref["fit"]["status"]
+    np.testing.assert_array_equal(np.isnan(res.ux.cpu().numpy().ravel()[agree]), np.isnan(ref["ux"].ravel()[agree]))
+    np.testing.assert_array_equal(res.nan_reason.cpu().numpy().ravel()[agree], ref["nan_reason"].ravel()[agree])
+
+
+def test_ensemble_uint16_frames_and_mask():
+    shape, win = (160, 160), (32, 32)
+    A, B = _pairs(6, shape, seed0=900)
+    Au = np.clip(np.rint(A * 3000), 0, 4095).astype(np.int16)
+    Bu = np.clip(np.rint(B * 3000), 0, 4095).astype(np.int16)
+    wc = compute_window_centers(shape, win, 75)
+    mask = np.zeros(wc.n_win_y * wc.n_win_x, bool)
+    mask[::5] = True
+    ens = EnsemblePIV(shape, win, 75, kspace_floor="flat", mask=mask)
+    ens.accumulate(torch.as_tensor(Au, device=dev), torch.as_tensor(Bu, device=dev))
+    res = ens.finalize()
+    ref = EO.ensemble_pass(Au.astype(np.float32), Bu.astype(np.float32), wc.win_ctrs_x, wc.win_ctrs_y, win,
+                           window_weight(win, "square"), mask=mask)
+    reason = res.nan_reason.cpu().numpy().ravel()
+    assert np.all(reason[mask] == -1)
+    assert np.all(res.ux.cpu().numpy().ravel()[mask] == 0.0)
+    for dp, rp in zip(res.planes, ref["planes"]):
+        dp = dp.cpu().numpy()[~mask]
+        assert _rel(dp, rp[~mask]) < 1e-5
+    np.testing.assert_array_equal(reason[mask], ref["nan_reason"].ravel()[mask])
+
+
+def test_stress_recovery_uniform_shear():
+    """Physical check: the Reynolds stresses of a flow with a known
+    displacement PDF.  Every pair translates by a random constant (normal,
+    std 0.5 px in x, 0.3 px in y), so the ensemble transfer ratio is the
+    characteristic function of that PDF: UU = 0.25, VV = 0.09, UV = 0."""
+    shape, win = (160, 160), (32, 32)
+    rng = np.random.default_rng(77)
+    A, B, dxs, dys = [], [], [], []
+    for k in range(48):
+        r = np.random.default_rng(3000 + k)
+        x, y, inten = S._positions(r, shape, 0.02)
+        dx, dy = rng.normal(1.0, 0.5), rng.normal(-0.5, 0.3)
+        dxs.append(dx)
+        dys.append(dy)
+        A.append(S.render(shape, x, y, inten).numpy())
+        B.append(S.render(shape, x + dx, y + dy, inten).numpy())
+    ens = EnsemblePIV(shape, win, 75, kspace_floor="flat")
+    ens.accumulate(torch.as_tensor(np.stack(A), device=dev), torch.as_tensor(np.stack(B), device=dev))
+    res = ens.finalize()
+    ok = res.nan_reason.cpu().numpy() == 0
+    assert ok.mean() > 0.8
+    uu = np.median(res.UU.cpu().numpy()[ok])
+    vv = np.median(res.VV.cpu().numpy()[ok])
+    ux = np.median(res.ux.cpu().numpy()[ok])
+    # the sample moments of the 48 drawn shifts are what the fit sees
+    assert abs(ux - np.mean(dxs)) < 0.05, (ux, np.mean(dxs))
+    assert abs(uu - np.var(dxs)) < 0.05, (uu, np.var(dxs))
+    assert abs(vv - np.var(dys)) < 0.05, (vv, np.var(dys))
