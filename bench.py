@@ -100,6 +100,128 @@ def cpu_baseline(A_dev, B_dev, gpu_final, finder, budget_s):
     return out, acc
 
 
+ENS_SHAPE = (1024, 1024)
+ENS_WIN = (32, 32)
+
+
+def f_ens(n):
+    """Algorithmic flops per window and pair of the ensemble accumulation:
+    2 r2c (5 N^2 log2 N each), weighting/mean (12 N^2) and the three spectral
+    products AB, AA, BB (18 N (N/2+1))."""
+    import math
+    return 10 * n * n * math.log2(n) + 12 * n * n + 18 * n * (n // 2 + 1)
+
+
+def run_ensemble(args):
+    """Ensemble workload: one 2000-pair pass at 1024 x 1024, 32 x 32 windows
+    at 75 % overlap, correlation background, coloured floor, k-space fit."""
+    import numpy as np
+    import torch
+    from pivtools_amd import synthetic as S
+    from pivtools_amd.ensemble import EnsemblePIV
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    nd = args.ens_distinct
+    log(f"[bench] generating {nd} distinct ensemble pairs of {ENS_SHAPE} f32 on the device")
+    pool_a = torch.empty((nd,) + ENS_SHAPE, dtype=torch.float32, device=dev)
+    pool_b = torch.empty_like(pool_a)
+    for k in range(nd):
+        a, b = S.ensemble_pair(seed=2000 + k, shape=ENS_SHAPE, device=dev)
+        pool_a[k].copy_(a)
+        pool_b[k].copy_(b)
+    torch.cuda.synchronize()
+    batch = args.pairs
+
+    def one_pass(n_pairs, timed):
+        ens = EnsemblePIV(ENS_SHAPE, ENS_WIN, 75, "square", bg_method="correlation", kspace_floor="coloured")
+        ev = []
+        done = 0
+        while done < n_pairs:
+            n = min(batch, n_pairs - done)
+            i0 = done % nd
+            n = min(n, nd - i0)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            ens.accumulate(pool_a[i0:i0 + n], pool_b[i0:i0 + n])
+            e1.record()
+            ev.append((e0, e1, n))
+            done += n
+        f0, f1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        f0.record()
+        res = ens.finalize()
+        f1.record()
+        return ens, res, ev, (f0, f1)
+
+    log("[bench] ensemble warmup")
+    one_pass(min(2 * batch, args.ens_pairs), False)
+    torch.cuda.synchronize()
+    log(f"[bench] timing one {args.ens_pairs}-pair ensemble pass")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ens, res, ev, fin = one_pass(args.ens_pairs, True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    acc_ms = sum(a.elapsed_time(b) for a, b, _ in ev)
+    fin_ms = fin[0].elapsed_time(fin[1])
+    value = args.ens_pairs / wall
+    nwin = ens.nwin
+    flops = f_ens(ENS_WIN[0]) * nwin * args.ens_pairs
+    achieved = flops / (acc_ms * 1e-3) / 1e12
+    ok = res.nan_reason.cpu().numpy() == 0
+    stats = {"valid_frac": round(float(ok.mean()), 4)}
+    if ok.any():
+        for k in ("UU", "VV", "UV"):
+            stats[f"{k}_median_px2"] = round(float(np.median(getattr(res, k).cpu().numpy()[ok])), 4)
+    cpu = None
+    if args.cpu_budget > 0:
+        cpu = ensemble_cpu_baseline(pool_a, pool_b, ens, res, args.ens_pairs)
+    line = {
+        "metric": "ensemble pairs/sec (one 2000-pair pass incl. finalize and k-space fit), 1 GPU",
+        "value": round(value, 3), "unit": "pairs/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": round(wall * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 accumulate in FP64 spectra, FP64 fit",
+        "data": f"synthetic ({nd} distinct seeded pairs cycled to {args.ens_pairs})",
+        "config": {"workload": "ensemble_1024_w32_o75", "pairs": args.ens_pairs, "window": list(ENS_WIN),
+                   "overlap": 75, "windows": nwin, "bg_method": "correlation", "kspace_floor": "coloured",
+                   "batch": batch},
+        "accumulate_ms": round(acc_ms, 3), "finalize_ms": round(fin_ms, 3),
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                     "kernel": "ens_kernel<32> (accumulate)",
+                     "note": "algorithmic flops 10N^2log2N + 12N^2 + 18N(N/2+1) per window and pair"},
+        "cpu_baseline": cpu, "fit": stats,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def ensemble_cpu_baseline(pool_a, pool_b, ens, res, n_pairs):
+    """Oracle accumulate_triple timed on 4 pairs and the oracle finalize +
+    k-space fit timed once, extrapolated to the n_pairs pass."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle as O
+
+    a = pool_a[:4].cpu().numpy()
+    b = pool_b[:4].cpu().numpy()
+    g = ens.grid
+    w = ens.w_host
+    mask = np.zeros(ens.nwin, np.float32)
+    t0 = time.perf_counter()
+    O.accumulate_triple(a, b, mask, g.win_ctrs_x, g.win_ctrs_y, ENS_WIN, w, w, w, w)
+    t_pair = (time.perf_counter() - t0) / 4
+    planes = [p.cpu().numpy() for p in res.planes]
+    pc = ens.floor_shape()
+    t0 = time.perf_counter()
+    O.kspace_fit(planes[0], planes[1], planes[2], np.zeros(ens.nwin, bool), ENS_WIN,
+                 p_win=np.tile(pc.ravel(), ens.nwin))
+    t_fit = time.perf_counter() - t0
+    total = n_pairs * t_pair + t_fit
+    return {"value": round(n_pairs / total, 4), "unit": "pairs/s", "cores": int(O.lib().orc_max_threads()),
+            "kind": "port", "sample": f"accumulate_triple on 4 pairs ({t_pair:.3f} s/pair) + one k-space fit "
+            f"of {ens.nwin} windows ({t_fit:.2f} s), extrapolated to {n_pairs} pairs"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -109,6 +231,9 @@ def main():
     ap.add_argument("--finder", default="gauss3")
     ap.add_argument("--impl", default="scalar")
     ap.add_argument("--cpu-budget", type=float, default=60.0, help="0 disables the CPU baseline leg")
+    ap.add_argument("--workload", choices=["multipass", "ensemble"], default="multipass")
+    ap.add_argument("--ens-pairs", type=int, default=2000)
+    ap.add_argument("--ens-distinct", type=int, default=64)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one plain 256 MiB device copy (byte-counter calibration for profiles)")
     args = ap.parse_args()
@@ -117,6 +242,9 @@ def main():
               "(pairs are independent: N GPUs would each run this batch on their own stream of pairs)",
               file=sys.stderr)
         sys.exit(2)
+    if args.workload == "ensemble":
+        run_ensemble(args)
+        return
 
     import numpy as np
     import torch

@@ -251,10 +251,22 @@ static unsigned char run_accumulate(const float *A, const float *B, const float 
     const size_t imgn = (size_t)N_images * H * W * sizeof(float);
     bool ok = h2d(dA, A, imgn) && h2d(dB, B, imgn) && h2d(dM, mask, nWin * sizeof(float)) &&
               h2d(dCx, ctrX, nWx * sizeof(float)) && h2d(dCy, ctrY, nWy * sizeof(float));
-    for (int k = 0; k < 4; ++k) ok = ok && h2d(dw[k], hw[k], numel * sizeof(float));
+    // identical weight contents share one device buffer, which lets the
+    // kernel take the auto spectra from the AB transform (wAab == wAA and
+    // wBab == wBB, the reference's default call)
+    const float *dwk[4];
+    for (int k = 0; k < 4; ++k) {
+        dwk[k] = dw[k];
+        for (int j = 0; j < k; ++j)
+            if (std::memcmp(hw[j], hw[k], numel * sizeof(float)) == 0) {
+                dwk[k] = dwk[j];
+                break;
+            }
+        if (dwk[k] == dw[k]) ok = ok && h2d(dw[k], hw[k], numel * sizeof(float));
+    }
     for (int k = 0; k < (triple ? 3 : 1); ++k) ok = ok && h2d(dO[k], ho[k], outb);
     if (!ok) return kErrNoplan;
-    const int rc = pivamd_accumulate(dA, dB, dM, H, W, N_images, dCx, dCy, nWy, nWx, dw[0], dw[1], dw[2], dw[3],
+    const int rc = pivamd_accumulate(dA, dB, dM, H, W, N_images, dCx, dCy, nWy, nWx, dwk[0], dwk[1], dwk[2], dwk[3],
                                      wy, wx, fy, fx, meanSub, perPairNorm, triple, sWB, sWAA, sWBB, dO[0],
                                      triple ? dO[1] : nullptr, triple ? dO[2] : nullptr, dWk, work, g_ctx.st);
     if (rc != 0) return rc == PIVAMD_ERR_SIZE ? kErrNomem : kErrNoplan;

@@ -27,8 +27,11 @@ struct EnsArgs {
     float *outAB, *outAA, *outBB;
     double *acc;            // nWin * 4 * NY * (NX/2+1)
     int H, W, nImages, nWy, nWx, fitY, fitX, meanSub, perPairNorm, triple, nblocks;
+    int sharedW;            // wAab == wAA and wBab == wBB: AA, BB come from the AB transform
     float sumWBab, sumWAA, sumWBB;
 };
+
+constexpr int kEnsFlush = 64;   // images per FP32 register partial before the FP64 flush
 
 constexpr int kEnsMaxLds = 150 * 1024;
 
@@ -278,6 +281,220 @@ __global__ void __launch_bounds__((EnsCfg<N>::BLOCK)) ens_kernel(EnsArgs g)
     }
 }
 
+// Register-accumulating variant for N <= 32.  Lane t of a window owns the
+// spectral row ky = t (all N lanes busy) and keeps the AB, AA and BB sums of
+// that row in FP32 registers across the call's images, flushing them into the
+// FP64 workspace every kEnsFlush images; the workspace is touched once per
+// flush instead of once per image.  With shared weights (wAab == wAA,
+// wBab == wBB) pAA = pAB_A and pBB = pAB_B, so one complex transform per pair
+// gives all three spectra.
+template <int N, bool SHARED>
+__global__ void __launch_bounds__((EnsCfg<N>::BLOCK)) ens_kernel_reg(EnsArgs g)
+{
+    using C = EnsCfg<N>;
+    constexpr int T = C::T, PX = C::PX, WPB = C::WPB, HW = C::HW;
+    constexpr int numel = N * N;
+    __shared__ float s_re[WPB * C::tile];
+    __shared__ float s_im[WPB * C::tile];
+    __shared__ float s_red[6][WPB * T];
+
+    const int per = (int)gridDim.x >> 3;
+    const int bb = ((int)blockIdx.x & 7) * per + ((int)blockIdx.x >> 3);
+    if (bb >= g.nblocks) return;
+    const int wloc = (int)threadIdx.x / T;
+    const int t = (int)threadIdx.x - wloc * T;
+    float *Sr = s_re + wloc * C::tile;
+    float *Si = s_im + wloc * C::tile;
+    float *r0 = &s_red[0][wloc * T], *r1 = &s_red[1][wloc * T], *r2 = &s_red[2][wloc * T];
+
+    const int nWin = g.nWy * g.nWx;
+    const int w = bb * WPB + wloc;
+    bool valid = w < nWin;
+    int row0 = 0, col0 = 0;
+    if (valid) {
+        const int ii = w % g.nWx, jj = w / g.nWx;
+        if (g.mask && g.mask[w] == 1.0f) valid = false;
+        // float arithmetic, as PIV_2d_cross_correlate.c:555-556 / 743-744
+        row0 = (int)floorf(g.ctrY[jj] - ((float)N - 1.0f) / 2.0f + 0.5f);
+        col0 = (int)floorf(g.ctrX[ii] - ((float)N - 1.0f) / 2.0f + 0.5f);
+        if (row0 < 0 || col0 < 0 || row0 + N > g.H || col0 + N > g.W) valid = false;
+    }
+    double *acc = g.acc + (size_t)(valid ? w : 0) * 4 * N * HW;
+    const int ky = t, my = (N - ky) % N;
+
+    float aRe[HW], aIm[HW], aAA[HW], aBB[HW];
+#pragma unroll
+    for (int k = 0; k < HW; ++k) { aRe[k] = 0.0f; aIm[k] = 0.0f; aAA[k] = 0.0f; aBB[k] = 0.0f; }
+    bool first = true;
+    auto flush = [&]() {
+        if (valid) {
+            double *row = acc + (size_t)ky * HW;
+#pragma unroll
+            for (int k = 0; k < HW; ++k) {
+                double *p0 = row + k, *p1 = row + N * HW + k, *p2 = row + 2 * N * HW + k, *p3 = row + 3 * N * HW + k;
+                *p0 = (first ? 0.0 : *p0) + (double)aRe[k];
+                *p1 = (first ? 0.0 : *p1) + (double)aIm[k];
+                *p2 = (first ? 0.0 : *p2) + (double)aAA[k];
+                *p3 = (first ? 0.0 : *p3) + (double)aBB[k];
+                aRe[k] = 0.0f; aIm[k] = 0.0f; aAA[k] = 0.0f; aBB[k] = 0.0f;
+            }
+        }
+        first = false;
+    };
+
+    const size_t plane = (size_t)g.H * g.W;
+    for (int n = 0; n < g.nImages; ++n) {
+        // keep the weight columns out of the loop-carried register set: they
+        // are re-read from the cache each image instead of pinned in VGPRs
+        asm volatile("" ::: "memory");
+        float z1r[N], z1i[N];
+        float sAB_B = 0.0f, sAA = 0.0f, sBB = 0.0f;
+        const float *pa = g.imgA + (size_t)n * plane + (size_t)row0 * g.W + col0 + t;
+        const float *pb = g.imgB + (size_t)n * plane + (size_t)row0 * g.W + col0 + t;
+        if (valid) {
+#pragma unroll
+            for (int y = 0; y < N; ++y) {
+                z1r[y] = pb[(size_t)y * g.W];
+                z1i[y] = pa[(size_t)y * g.W];
+                if (g.meanSub) {
+                    sAB_B += z1r[y] * g.wBab[y * N + t];
+                    sAA += z1i[y] * g.wAA[y * N + t];
+                    sBB += z1r[y] * g.wBB[y * N + t];
+                }
+            }
+        } else {
+#pragma unroll
+            for (int y = 0; y < N; ++y) { z1r[y] = 0.0f; z1i[y] = 0.0f; }
+        }
+        float mB = 0.0f, mAA = 0.0f, mBB = 0.0f;
+        if (g.meanSub) {
+            ens_sum3(sAB_B, sAA, sBB, r0, r1, r2, t, T);
+            mB = (g.sumWBab > 0.0f) ? sAB_B / g.sumWBab : 0.0f;
+            mAA = (g.sumWAA > 0.0f) ? sAA / g.sumWAA : 0.0f;
+            mBB = (g.sumWBB > 0.0f) ? sBB / g.sumWBB : 0.0f;
+        }
+        float eAA = 0.0f, eBB = 0.0f, dummy = 0.0f;
+#pragma unroll
+        for (int y = 0; y < N; ++y) {
+            const float b = z1r[y], a = z1i[y];
+            z1r[y] = (b - mB) * g.wBab[y * N + t];
+            z1i[y] = (a - mAA) * g.wAab[y * N + t];
+            const float pA = (a - mAA) * g.wAA[y * N + t], pB = (b - mBB) * g.wBB[y * N + t];
+            eAA += pA * pA;
+            eBB += pB * pB;
+        }
+        float sAB = 1.0f, sA = 1.0f, sB = 1.0f;
+        bool use = valid;
+        if (g.perPairNorm) {
+            ens_sum3(eAA, eBB, dummy, r0, r1, r2, t, T);
+            if (eAA <= 0.0f || eBB <= 0.0f) use = false;
+            sAB = 1.0f / sqrtf(eAA * eBB);
+            sA = 1.0f / eAA;
+            sB = 1.0f / eBB;
+        }
+        fwd2d<N>(z1r, z1i, Sr, Si, t, use);
+        if (use) {
+#pragma unroll
+            for (int kx = 0; kx < HW; ++kx) {
+                const int mx = (N - kx) % N;
+                const float zR = Sr[ky * PX + kx], zI = Si[ky * PX + kx];
+                const float mR = Sr[my * PX + mx], mI = Si[my * PX + mx];
+                const float xr_ = 0.5f * (zR + mR), xi_ = 0.5f * (zI - mI);      // F(pAB_B)
+                const float yr_ = 0.5f * (zI + mI), yi_ = -0.5f * (zR - mR);     // F(pAB_A)
+                aRe[kx] += sAB * (xr_ * yr_ + xi_ * yi_);
+                aIm[kx] += sAB * (xi_ * yr_ - xr_ * yi_);
+                if constexpr (SHARED) {
+                    aAA[kx] += sA * (yr_ * yr_ + yi_ * yi_);
+                    aBB[kx] += sB * (xr_ * xr_ + xi_ * xi_);
+                }
+            }
+        }
+        if (!SHARED && g.triple) {
+            // z2 = pAA + i pBB from a second read of the window
+            float z2r[N], z2i[N];
+#pragma unroll
+            for (int y = 0; y < N; ++y) {
+                const float a = valid ? pa[(size_t)y * g.W] : 0.0f, b = valid ? pb[(size_t)y * g.W] : 0.0f;
+                z2r[y] = (a - mAA) * g.wAA[y * N + t];
+                z2i[y] = (b - mBB) * g.wBB[y * N + t];
+            }
+            __syncthreads();
+            fwd2d<N>(z2r, z2i, Sr, Si, t, use);
+            if (use) {
+#pragma unroll
+                for (int kx = 0; kx < HW; ++kx) {
+                    const int mx = (N - kx) % N;
+                    const float zR = Sr[ky * PX + kx], zI = Si[ky * PX + kx];
+                    const float mR = Sr[my * PX + mx], mI = Si[my * PX + mx];
+                    const float ar = 0.5f * (zR + mR), ai = 0.5f * (zI - mI);
+                    const float br = 0.5f * (zI + mI), bi = -0.5f * (zR - mR);
+                    aAA[kx] += sA * (ar * ar + ai * ai);
+                    aBB[kx] += sB * (br * br + bi * bi);
+                }
+            }
+        }
+        __syncthreads();
+        if ((n + 1) % kEnsFlush == 0) flush();
+    }
+    if (first || g.nImages % kEnsFlush != 0) flush();
+    __syncthreads();   // rows written by every lane before the column readout
+
+    // ---- readout: one c2r per accumulated spectrum, shift, crop, += -------
+    const int oh = g.fitY, ow = g.fitX;
+    const int sy = (N - oh) / 2, sx = (N - ow) / 2;
+    for (int which = 0; which < (g.triple ? 3 : 1); ++which) {
+        if (valid && t < HW) {
+            float cr[N], ci[N];
+#pragma unroll
+            for (int k = 0; k < N; ++k) {
+                if (which == 0) {
+                    cr[k] = (float)acc[(size_t)(0 * N + k) * HW + t];
+                    ci[k] = (float)acc[(size_t)(1 * N + k) * HW + t];
+                } else {
+                    cr[k] = (float)acc[(size_t)((which + 1) * N + k) * HW + t];
+                    ci[k] = 0.0f;
+                }
+            }
+            fft_reg<N, true>(cr, ci);
+#pragma unroll
+            for (int y = 0; y < N; ++y) {
+                Sr[y * PX + t] = cr[y];
+                Si[y * PX + t] = ci[y];
+            }
+        }
+        __syncthreads();
+        if (valid && t < N / 2) {
+            const int ya = t, yb = t + N / 2;
+            float xr[N], xi[N];
+#pragma unroll
+            for (int k = 0; k <= N / 2; ++k) {
+                float g1r = Sr[ya * PX + k], g1i = Si[ya * PX + k];
+                float g2r = Sr[yb * PX + k], g2i = Si[yb * PX + k];
+                if (k == 0 || k == N / 2) { g1i = 0.0f; g2i = 0.0f; }
+                xr[k] = g1r - g2i;
+                xi[k] = g1i + g2r;
+                if (k != 0 && k != N / 2) {
+                    xr[N - k] = g1r + g2i;
+                    xi[N - k] = -g1i + g2r;
+                }
+            }
+            fft_reg<N, true>(xr, xi);
+            const float mul = 1.0f / (float)numel;
+            float *out = which == 0 ? g.outAB : (which == 1 ? g.outAA : g.outBB);
+            out += (size_t)w * oh * ow;
+#pragma unroll
+            for (int x = 0; x < N; ++x) {
+                const int xs = (x + N / 2) % N;
+                if (xs >= sx && xs < sx + ow) {
+                    if (yb >= sy && yb < sy + oh) out[(yb - sy) * ow + (xs - sx)] += xr[x] * mul;
+                    if (ya >= sy && ya < sy + oh) out[(ya - sy) * ow + (xs - sx)] += xi[x] * mul;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
 template <int N>
 static int launch_ens(EnsArgs g, hipStream_t st)
 {
@@ -287,7 +504,13 @@ static int launch_ens(EnsArgs g, hipStream_t st)
     if (nb <= 0) return 0;
     g.nblocks = nb;
     const int grid = (nb + 7) / 8 * 8;
-    hipLaunchKernelGGL((ens_kernel<N>), dim3(grid), dim3(C::BLOCK), 0, st, g);
+    if constexpr (N <= 32) {
+        if (g.sharedW)
+            hipLaunchKernelGGL((ens_kernel_reg<N, true>), dim3(grid), dim3(C::BLOCK), 0, st, g);
+        else
+            hipLaunchKernelGGL((ens_kernel_reg<N, false>), dim3(grid), dim3(C::BLOCK), 0, st, g);
+    } else
+        hipLaunchKernelGGL((ens_kernel<N>), dim3(grid), dim3(C::BLOCK), 0, st, g);
     return hipGetLastError() == hipSuccess ? 0 : PIVAMD_ERR_LAUNCH;
 }
 
@@ -323,6 +546,7 @@ extern "C" int pivamd_accumulate(const float *d_A, const float *d_B, const float
     g.H = H; g.W = W; g.nImages = n_images; g.nWy = nWy; g.nWx = nWx; g.fitY = fitY; g.fitX = fitX;
     g.meanSub = triple ? meanSub : 0; g.perPairNorm = triple ? perPairNorm : 0; g.triple = triple;
     g.sumWBab = sumWBab; g.sumWAA = sumWAA; g.sumWBB = sumWBB; g.nblocks = 0;
+    g.sharedW = triple && d_wAab == d_wAA && d_wBab == d_wBB;
     switch (winY) {
     case 8: return launch_ens<8>(g, st);
     case 12: return launch_ens<12>(g, st);

@@ -128,3 +128,20 @@ def test_accumulate_ab_only_matches_oracle():
                                        np.array([wc.n_win_y, wc.n_win_x], np.int32), wa, wb,
                                        np.array([n, n], np.int32), None, ref)
     assert _rel(out, ref) < 1e-5
+
+
+@pytest.mark.parametrize("shared", [True, False])
+def test_triple_many_pairs_crosses_flush(shared):
+    """130 pairs in one call: the per-lane FP32 partials are flushed into the
+    FP64 sums every 64 images (two full flushes and a ragged one)."""
+    n = 16
+    shape = (64, 80)
+    A, B = _stack(130, shape, seed=7, pedestal=0.1)
+    wc = compute_window_centers(shape, (n, n), 50)
+    mask = np.zeros((wc.n_win_y, wc.n_win_x), np.float32)
+    wsq = window_weight((n, n), "square")
+    wab = wsq if shared else window_weight((n, n), "gaussian")     # AB weight of frame A
+    got = _triple(A, B, mask, wc, (n, n), wab, wsq, wsq, wsq, None, 1, 0)
+    ref = O.accumulate_triple(A, B, mask, wc.win_ctrs_x, wc.win_ctrs_y, (n, n), wab, wsq, wsq, wsq, None, 1, 0)
+    for g, r in zip(got, ref):
+        assert _rel(g, r.reshape(g.shape)) < 1e-5
