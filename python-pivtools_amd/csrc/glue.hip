@@ -236,7 +236,11 @@ __global__ void infill_kernel(const float *u, const float *mask, int N, int ny, 
         }
     }
     if (br < 0) {
+        // no donor anywhere: flag it (the caller raises, as the reference
+        // does) and leave a zero displacement so later passes stay defined
         atomicOr(err, 1);
+        uo[i] = 0.0f;
+        uo[(size_t)total + i] = 0.0f;
         return;
     }
     uo[i] = uy[br * nx + bc];

@@ -62,8 +62,18 @@ __device__ float pred_index(float coord, const float *ctrs, int nC)
 // Bicubic tap sum (Keys a=-0.75, 4x4 at floor-1, zero outside the image),
 // term order m outer / n inner as fused_warp.c:70-105.  The whole stencil
 // is tested once; only border stencils test each tap.
+// Sample positions further than the stencil reach outside the image give 0;
+// clamping them to [-16, size + 16] keeps that value and keeps every index
+// computation in range for any input (non-finite or huge displacements).
+__device__ __forceinline__ float clamp_pos(float v, int size)
+{
+    return fminf(fmaxf(v, -16.0f), (float)size + 16.0f);
+}
+
 __device__ __forceinline__ float sample_bicubic(const float *img, float fy, float fx, int H, int W)
 {
+    fy = clamp_pos(fy, H);
+    fx = clamp_pos(fx, W);
     const float fyf = floorf(fy), fxf = floorf(fx);
     const int iy = (int)fyf - 1, ix = (int)fxf - 1;
     float wy[4], wx[4];
@@ -110,6 +120,8 @@ __device__ __forceinline__ void lanczos_w(float d, const float *lut, float w[6])
 __device__ __forceinline__ float sample_lanczos(const float *img, float fy, float fx, int H, int W,
                                                 const float *lut)
 {
+    fy = clamp_pos(fy, H);
+    fx = clamp_pos(fx, W);
     const float fyf = floorf(fy), fxf = floorf(fx);
     const int iy = (int)fyf - 2, ix = (int)fxf - 2;
     float wy[6], wx[6];

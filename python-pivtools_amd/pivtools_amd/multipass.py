@@ -105,6 +105,9 @@ class MultipassPIV:
         self.window_type = window_type
         self._bufs: list[_PassBuf] | None = None
         self._n = 0
+        # buffer sets per batch size, kept for the life of the object: a HIP
+        # graph captured for one batch size keeps pointing at its set
+        self._sets: dict = {}
 
     # -- buffers ---------------------------------------------------------------
     def _stream(self):
@@ -151,6 +154,7 @@ class MultipassPIV:
         self.warped = torch.empty((2, n, self.H, self.W), dtype=f32, device=dev)
         self._bufs = bufs
         self._n = n
+        self._sets[n] = (bufs, self.imgf, self.warped)
 
     # -- one batch ---------------------------------------------------------------
     def run(self, img_a: torch.Tensor, img_b: torch.Tensor, keep_passes: bool = False, events=None):
@@ -166,7 +170,11 @@ class MultipassPIV:
             raise ValueError(f"expected two (N, {self.H}, {self.W}) stacks, got {tuple(img_a.shape)}")
         n = int(img_a.shape[0])
         if self._bufs is None or self._n != n:
-            self._alloc(n)
+            if n in self._sets:
+                self._bufs, self.imgf, self.warped = self._sets[n]
+                self._n = n
+            else:
+                self._alloc(n)
         st = self._stream()
         L = self.lib
         npix = n * self.H * self.W
@@ -249,7 +257,8 @@ class MultipassPIV:
         return results
 
     def check_infill(self) -> None:
-        """Raise if any infill found no donor (a fully invalid field)."""
+        """Raise if any infill of the last batch size found no donor (a fully
+        invalid field; that field is then carried on as zero displacement)."""
         for b in self._bufs or []:
             if int(b.out["err"].item()) != 0:
                 raise ValueError("infilling has no donor points: 100% of vectors invalid")

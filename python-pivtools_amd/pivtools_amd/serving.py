@@ -1,0 +1,257 @@
+"""Continuous-batching PIV server on one GPU.
+
+Requests (one image pair of one camera stream each) arrive at any time.
+Each stream owns a small pool of *slots*.  A slot is a resident
+``MultipassPIV`` with its own batch buffers, fixed input staging buffers, a
+HIP stream and captured HIP graphs of the whole pass loop.  The scheduler is
+a polling loop:
+
+* a slot whose graph has finished hands its pairs' final fields back (copied
+  out of the slot, so the slot can be reused at once) and becomes free;
+* a free slot takes up to ``max_batch`` waiting pairs of its stream, copies
+  them into its staging buffers on its stream and replays the graph captured
+  for that batch size (capturing it the first time, after one eager run);
+* new slots are created while the HBM budget allows, so a stream with a
+  backlog gets more batches in flight; slots of different streams and of the
+  same stream run concurrently on their own HIP streams.
+
+A request that arrives while others are on the GPU is admitted as soon as a
+slot of its stream is free.  Batches never mix streams: a stream's image size
+and window schedule fix its buffers and graphs.  Streams flagged
+``ensemble`` also add every pair to that stream's ensemble accumulators
+(after the pass loop, on the stream's ensemble queue).
+
+Every pair is correlated independently of the others in its batch, so a
+pair's result does not depend on what it was batched with.  The serving tests
+check that a pair's result is bit-identical alone and in the mix.
+"""
+
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+
+import torch
+
+from .ensemble import EnsemblePIV
+from .multipass import MultipassPIV
+
+
+@dataclass
+class StreamSpec:
+    name: str
+    shape: tuple
+    window_sizes: list
+    overlaps: list | None = None
+    peak_finder: str = "gauss3"
+    ensemble: bool = False
+    ensemble_window: tuple = (32, 32)
+    ensemble_overlap: int = 75
+
+
+@dataclass
+class Request:
+    rid: int
+    stream: int
+    a: torch.Tensor
+    b: torch.Tensor
+    t_arrival: float
+    t_start: float = 0.0
+    t_done: float = 0.0
+    ux: torch.Tensor | None = None
+    uy: torch.Tensor | None = None
+    nan_mask: torch.Tensor | None = None
+    ready: torch.cuda.Event | None = None     # frames written (producer's stream)
+    done: torch.cuda.Event | None = None      # fields written (slot's stream)
+
+    @property
+    def latency(self) -> float:
+        return self.t_done - self.t_arrival
+
+    def result(self):
+        """(ux, uy, nan_mask), ordered after the batch on the caller's stream."""
+        torch.cuda.current_stream(self.ux.device).wait_event(self.done)
+        return self.ux, self.uy, self.nan_mask
+
+
+@dataclass
+class _Slot:
+    mp: MultipassPIV
+    stream: torch.cuda.Stream
+    stage_a: torch.Tensor
+    stage_b: torch.Tensor
+    graphs: dict = field(default_factory=dict)
+    outs: dict = field(default_factory=dict)
+    warm: set = field(default_factory=set)
+    batch: list = field(default_factory=list)
+    done_event: torch.cuda.Event | None = None
+    ens_event: torch.cuda.Event | None = None
+    nbytes: int = 0
+
+
+class PIVServer:
+    def __init__(self, streams: list[StreamSpec], max_batch: int = 8, max_slots_per_stream: int = 3,
+                 hbm_budget_bytes: int = 64 << 30, use_graphs: bool = True, device=None):
+        self.dev = torch.device(device) if device is not None else torch.device("cuda")
+        self.specs = list(streams)
+        self.max_batch = int(max_batch)
+        self.max_slots = int(max_slots_per_stream)
+        self.budget = int(hbm_budget_bytes)
+        self.use_graphs = use_graphs
+        self.slots: list[list[_Slot]] = [[] for _ in self.specs]
+        self.pending: list[list[Request]] = [[] for _ in self.specs]
+        self.ens: list[EnsemblePIV | None] = []
+        self.ens_stream: list[torch.cuda.Stream | None] = []
+        for s in self.specs:
+            if s.ensemble:
+                self.ens.append(EnsemblePIV(s.shape, s.ensemble_window, s.ensemble_overlap, "square",
+                                            device=self.dev))
+                self.ens_stream.append(torch.cuda.Stream(device=self.dev))
+            else:
+                self.ens.append(None)
+                self.ens_stream.append(None)
+        self.bytes_in_use = 0
+        self._rid = 0
+        self.completed: list[Request] = []
+
+    # -- admission ---------------------------------------------------------
+    def submit(self, stream: int, a: torch.Tensor, b: torch.Tensor, t_arrival: float | None = None) -> Request:
+        spec = self.specs[stream]
+        if tuple(a.shape) != tuple(spec.shape) or a.shape != b.shape:
+            raise ValueError(f"stream {spec.name} expects {spec.shape} frames, got {tuple(a.shape)}")
+        if a.dtype not in (torch.int16, torch.uint16) or b.dtype != a.dtype or not a.is_cuda:
+            raise TypeError("requests carry 16-bit camera counts (uint16 data) in device tensors")
+        r = Request(self._rid, stream, a, b, time.perf_counter() if t_arrival is None else t_arrival)
+        r.ready = torch.cuda.Event()
+        r.ready.record(torch.cuda.current_stream(a.device))
+        self._rid += 1
+        self.pending[stream].append(r)
+        return r
+
+    def _new_slot(self, i: int) -> _Slot | None:
+        spec = self.specs[i]
+        if len(self.slots[i]) >= self.max_slots:
+            return None
+        H, W = spec.shape
+        stage = 2 * self.max_batch * H * W * 2
+        est = stage + 2 * (2 * self.max_batch * H * W * 4) * 2   # staging + float frames + warped (+ fields)
+        if self.bytes_in_use + est > self.budget and self.slots[i]:
+            return None
+        st = torch.cuda.Stream(device=self.dev)
+        with torch.cuda.stream(st):
+            mp = MultipassPIV(spec.shape, spec.window_sizes, spec.overlaps, peak_finder=spec.peak_finder,
+                              device=self.dev)
+            sa = torch.zeros((self.max_batch, H, W), dtype=torch.int16, device=self.dev)
+            sb = torch.zeros_like(sa)
+        slot = _Slot(mp, st, sa, sb)
+        slot.nbytes = est
+        self.bytes_in_use += est
+        self.slots[i].append(slot)
+        return slot
+
+    # -- execution -----------------------------------------------------------
+    def _bucket(self, n: int) -> int:
+        b = 1
+        while b < n:
+            b *= 2
+        return min(b, self.max_batch)
+
+    def _launch(self, i: int, slot: _Slot, batch: list[Request]) -> None:
+        m = len(batch)
+        n = self._bucket(m)            # graphs exist per power-of-two batch size
+        now = time.perf_counter()
+        with torch.cuda.stream(slot.stream):
+            if slot.ens_event is not None:
+                slot.stream.wait_event(slot.ens_event)   # previous ensemble reads of the staging
+            for r in batch:
+                slot.stream.wait_event(r.ready)          # frames produced on the caller's stream
+            for k in range(n):
+                r = batch[min(k, m - 1)]      # padding positions repeat the last real pair
+                slot.stage_a[k].copy_(r.a, non_blocking=True)
+                slot.stage_b[k].copy_(r.b, non_blocking=True)
+            for r in batch:
+                r.t_start = now
+            a, b = slot.stage_a[:n], slot.stage_b[:n]
+            if not self.use_graphs:
+                res = slot.mp.run(a, b)[-1]
+                out = (res.ux, res.uy, res.nan_mask)
+            elif n in slot.graphs:
+                slot.graphs[n].replay()
+                out = slot.outs[n]
+            elif n in slot.warm:
+                g = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(g, stream=slot.stream):
+                    res = slot.mp.run(a, b)[-1]
+                slot.graphs[n] = g
+                slot.outs[n] = (res.ux, res.uy, res.nan_mask)
+                g.replay()
+                out = slot.outs[n]
+            else:
+                res = slot.mp.run(a, b)[-1]      # eager run sizes the buffers for this batch
+                slot.warm.add(n)
+                out = (res.ux, res.uy, res.nan_mask)
+            for k, r in enumerate(batch):
+                r.ux = out[0][k].clone()
+                r.uy = out[1][k].clone()
+                r.nan_mask = out[2][k].clone()
+            ev = torch.cuda.Event()
+            ev.record(slot.stream)
+            slot.done_event = ev
+            for r in batch:
+                r.done = ev
+        slot.ens_event = None
+        ens = self.ens[i]
+        if ens is not None:
+            es = self.ens_stream[i]
+            with torch.cuda.stream(es):
+                es.wait_event(ev)
+                ens.accumulate(slot.stage_a[:m], slot.stage_b[:m])
+                e2 = torch.cuda.Event()
+                e2.record(es)
+                slot.ens_event = e2
+        slot.batch = batch
+
+    def poll(self) -> list[Request]:
+        """Retire finished batches and dispatch waiting pairs; returns the
+        requests completed by this call."""
+        done = []
+        now = time.perf_counter()
+        for i, slots in enumerate(self.slots):
+            for s in slots:
+                if s.batch and s.done_event.query():
+                    for r in s.batch:
+                        r.t_done = now
+                    done.extend(s.batch)
+                    s.batch = []
+        for i in range(len(self.specs)):
+            while self.pending[i]:
+                slot = next((s for s in self.slots[i] if not s.batch and
+                             (s.ens_event is None or s.ens_event.query())), None)
+                if slot is None:
+                    slot = self._new_slot(i)
+                if slot is None:
+                    break
+                batch = self.pending[i][:self.max_batch]
+                del self.pending[i][:len(batch)]
+                self._launch(i, slot, batch)
+        self.completed.extend(done)
+        return done
+
+    def busy(self) -> bool:
+        return any(self.pending) or any(s.batch for slots in self.slots for s in slots)
+
+    def drain(self, timeout_s: float = 120.0) -> None:
+        t0 = time.perf_counter()
+        while self.busy():
+            self.poll()
+            if time.perf_counter() - t0 > timeout_s:
+                raise TimeoutError("server did not drain")
+
+    def finalize_ensembles(self) -> dict:
+        """Finish every ensemble accumulator (after draining)."""
+        out = {}
+        for i, e in enumerate(self.ens):
+            if e is not None and e.n_images:
+                torch.cuda.current_stream(self.dev).wait_stream(self.ens_stream[i])
+                out[self.specs[i].name] = e.finalize()
+        return out

@@ -1,0 +1,101 @@
+"""Continuous-batching server: every result is the same alone and in the mix.
+
+Requests of three camera streams (different sizes and schedules, one also
+feeding an ensemble) are submitted while earlier batches are on the GPU; the
+server batches them per stream into power-of-two buckets, replays captured
+HIP graphs (many replays, with eager work and synchronisations between) and
+hands results back as batches finish.  Each pair is then run alone through a
+fresh eager MultipassPIV: the fields must be bit-identical.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from pivtools_amd import synthetic as S
+from pivtools_amd.ensemble import EnsemblePIV
+from pivtools_amd.multipass import MultipassPIV
+from pivtools_amd.serving import PIVServer, StreamSpec
+
+pytestmark = pytest.mark.gpu
+dev = torch.device("cuda")
+
+SPECS = [
+    StreamSpec("cam0", (256, 256), [(64, 64), (32, 32), (16, 16)], [50, 50, 50]),
+    StreamSpec("cam1", (192, 320), [(32, 32), (16, 16)], [50, 50], ensemble=True),
+    StreamSpec("cam2", (256, 256), [(128, 128), (64, 64), (32, 32), (16, 16)], [50, 50, 50, 50]),
+]
+
+
+def _pair(spec, seed):
+    a, b = S.multipass_pair(seed=seed, shape=spec.shape, device=dev)
+    return a, b
+
+
+def _same(x, y):
+    return np.array_equal(x.cpu().numpy(), y.cpu().numpy(), equal_nan=True)
+
+
+@pytest.mark.parametrize("use_graphs", [True, False])
+def test_results_alone_equal_results_in_mix(use_graphs):
+    rng = np.random.default_rng(5)
+    srv = PIVServer(SPECS, max_batch=4, max_slots_per_stream=2, use_graphs=use_graphs)
+    reqs = []
+    black = None
+    for k in range(30):
+        i = int(rng.integers(0, 3))
+        a, b = _pair(SPECS[i], 7000 + k)
+        if k == 11:                         # a black frame pair in the mix
+            a.zero_()
+            b.zero_()
+        r = srv.submit(i, a, b)
+        if k == 11:
+            black = r
+        reqs.append(r)
+        if k % 3 == 0:
+            srv.poll()                      # earlier batches launched / retired
+    print("serving: drain", flush=True)
+    srv.drain()
+    torch.cuda.synchronize()
+    assert len(srv.completed) == 30
+    n_graphs = sum(len(s.graphs) for slots in srv.slots for s in slots)
+    if use_graphs:
+        assert n_graphs >= 3
+    alone = [MultipassPIV(s.shape, s.window_sizes, s.overlaps, peak_finder=s.peak_finder) for s in SPECS]
+    for r in reqs:
+        ux, uy, nm = r.result()
+        res = alone[r.stream].run(r.a[None], r.b[None])[-1]
+        torch.cuda.synchronize()
+        assert _same(ux, res.ux[0]) and _same(uy, res.uy[0]), (r.rid, r.stream)
+        assert _same(nm, res.nan_mask[0]), (r.rid, r.stream)
+    assert bool(black.nan_mask.bool().all())
+    assert all(r.latency > 0 for r in reqs)
+    # the ensemble stream accumulated exactly its own pairs
+    ens = srv.finalize_ensembles()["cam1"]
+    mine = [r for r in reqs if r.stream == 1]
+    ref = EnsemblePIV(SPECS[1].shape, (32, 32), 75)
+    ref.accumulate(torch.stack([r.a for r in mine]), torch.stack([r.b for r in mine]))
+    rres = ref.finalize()
+    assert ens.n_images == len(mine)
+    for p, q in zip(ens.planes, rres.planes):
+        p, q = p.cpu().numpy(), q.cpu().numpy()
+        assert np.linalg.norm(p - q) <= 1e-6 * np.linalg.norm(q)
+
+
+def test_graph_replays_match_eager_repeatedly():
+    """One slot's graph replayed for several different inputs, with eager
+    runs of other inputs in between, against eager results of the same input."""
+    spec = SPECS[0]
+    srv = PIVServer([spec], max_batch=2, max_slots_per_stream=1)
+    eager = MultipassPIV(spec.shape, spec.window_sizes, spec.overlaps)
+    for k in range(6):
+        a, b = _pair(spec, 8100 + k)
+        r = srv.submit(0, a, b)
+        srv.drain()
+        ea, eb = _pair(spec, 8200 + k)
+        eager.run(ea[None], eb[None])        # unrelated eager work between replays
+        res = eager.run(a[None], b[None])[-1]
+        ux, uy, _ = r.result()
+        torch.cuda.synchronize()
+        assert _same(ux, res.ux[0]) and _same(uy, res.uy[0])
+    assert 1 in srv.slots[0][0].graphs
