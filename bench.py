@@ -306,6 +306,8 @@ def main():
     ap.add_argument("--pairs", type=int, default=64)
     ap.add_argument("--finder", default="gauss3")
     ap.add_argument("--impl", default="scalar")
+    ap.add_argument("--dft", choices=["fft", "split"], default="fft",
+                    help="split: split-fp16 MFMA DFT for the 16/32/48/64 passes")
     ap.add_argument("--cpu-budget", type=float, default=60.0, help="0 disables the CPU baseline leg")
     ap.add_argument("--workload", choices=["multipass", "ensemble", "serving"], default="multipass")
     ap.add_argument("--serve-rates", type=float, nargs="+", default=[50, 200, 500, 1000, 2000])
@@ -338,7 +340,7 @@ def main():
     log(f"[bench] generating {args.pairs} pairs of {SHAPE} uint16 on the device")
     A, B = make_batch(args.pairs, dev)
     torch.cuda.synchronize()
-    mp = MultipassPIV(SHAPE, SCHEDULE, OVERLAP, peak_finder=args.finder, peak_fit_impl=args.impl)
+    mp = MultipassPIV(SHAPE, SCHEDULE, OVERLAP, peak_finder=args.finder, peak_fit_impl=args.impl, dft=args.dft)
     log("[bench] warmup")
     for _ in range(args.warmup):
         mp.run(A, B)
@@ -414,7 +416,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "multipass_2160x2560_u16", "pairs_per_step": args.pairs,
                    "schedule": [w[0] for w in SCHEDULE], "overlap": 50, "peak_finder": args.finder,
-                   "peak_fit_impl": args.impl, "warp": "bicubic", "outlier": "median 5x5 eps 0.1 thr 2",
+                   "peak_fit_impl": args.impl, "dft": args.dft, "warp": "bicubic", "outlier": "median 5x5 eps 0.1 thr 2",
                    "global_batch": args.pairs, "parallelism": "single-gpu"},
         "latency_ms_p50": round(float(np.percentile(steps_ms, 50)), 3),
         "latency_ms_p99": round(float(np.percentile(steps_ms, 99)), 3),

@@ -158,6 +158,11 @@ int kspace_fft2_centered(const double *plane, int H, int W, double *out_re, doub
 /* 1 if an NY x NX window has a correlation kernel. */
 int pivamd_size_supported(int ny, int nx);
 
+/* Transform of pivamd_xcorr: 0 = FP32 FFT (default), 1 = split-precision
+ * MFMA DFT (fp16 hi/lo operands, FP32 accumulation) for 16/32/48/64-point
+ * windows.  Returns the previous mode. */
+int pivamd_set_dft_mode(int mode);
+
 /* d_out = centre crop of the 2N zero-padded correlation of a and b (FP64). */
 int pivamd_convolve(const float *d_a, const float *d_b, int ny, int nx, float *d_out, void *stream);
 

@@ -16,6 +16,7 @@
 
 extern "C" {
 int pivamd_size_supported(int ny, int nx);
+int pivamd_set_dft_mode(int mode);
 int pivamd_convolve(const float *d_a, const float *d_b, int ny, int nx, float *d_out, hipStream_t st);
 int pivamd_plane_weight(const float *d_wB, int ny, int nx, float *d_out, hipStream_t st);
 int pivamd_xcorr(const float *d_imgA, const float *d_imgB, const float *d_mask, int H, int W,

@@ -28,6 +28,7 @@
 // contiguous run of windows, which keeps overlapping windows in one L2.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <type_traits>
 
 #include "fft_reg.h"
 #include "peakfit.h"
@@ -41,6 +42,7 @@ struct XcorrArgs {
     float *rec, *work_planes;   // workspace: per-window peak records, multi-peak planes
     long long total;        // N_images * nWin
     int H, W, nWy, nWx, nPeaks, finder, impl, ensemble, nblocks;
+    int dft;                // 1: split-fp16 MFMA DFT (N = 16, 32, 48, 64), 0: FP32 FFT
 };
 
 constexpr int kMaxLds = 150 * 1024;
@@ -400,6 +402,131 @@ __device__ void reduce_candidates(const float *r0, const float *r1, const float 
     *h_out = h; *i_out = i0; *j_out = j0; *det_out = det;
 }
 
+// Everything after the plane is in LDS (P, row stride PX, fftshifted, the
+// reference's raw plane): optional plane dump, then either the multi-peak
+// hand-off or the candidate search and the per-window peak record for
+// fit_kernel.  Called by all T lanes of the window.
+template <int N, int T>
+__device__ __forceinline__ void plane_epilogue(const XcorrArgs &g, const float *P, int PX, bool valid, long long gw,
+                                               int n, int w, float enorm, int t, float *r0, float *r1, float *r2,
+                                               float *r3, float *r4)
+{
+    constexpr int numel = N * N;
+    const int nWin = g.nWy * g.nWx;
+    if (g.planes && valid) {
+        float *dst = g.planes + (size_t)gw * numel;
+        for (int e = t; e < numel; e += T) dst[e] = P[(e / N) * PX + (e % N)];
+    }
+    if (g.ensemble) return;    // uniform: the flag is a kernel argument
+
+    // ---- 5. peak search; the fit runs in fit_kernel -----------------------
+    if (g.nPeaks > 1) {
+        // multi-peak: the plane goes to the workspace for multipeak_kernel
+        if (valid) {
+            float *dst = g.work_planes + (size_t)gw * numel;
+            for (int e = t; e < numel; e += T) dst[e] = P[(e / N) * PX + (e % N)];
+        }
+        if (t == 0 && gw < g.total) {
+            float *rec = g.rec + (size_t)gw * kRec;
+            rec[kRecFlag] = valid ? 1.0f : 0.0f;
+            rec[kRecEnorm] = enorm;
+        }
+        return;
+    }
+    constexpr int CW = N / 2, NC = (N / 2) * (N / 2);
+    Cand cd = {0.0f, 0x7fffffff, 0.0f, 0.0f, 0};
+    if (valid) {
+        for (int c = t; c < NC; c += T) {
+            const int i = N / 4 + c / CW, j = N / 4 + c % CW;
+            const float v = P[i * PX + j];
+            if (v != v) { cd.nan = 1; continue; }
+            if (!(v > 0.0f)) continue;
+            bool is_max = true;
+#pragma unroll
+            for (int di = -1; di <= 1; ++di)
+#pragma unroll
+                for (int dj = -1; dj <= 1; ++dj)
+                    if (P[(i + di) * PX + (j + dj)] > v) is_max = false;
+            if (!is_max) continue;
+            if (v > cd.v) { cd.v = v; cd.c = c; }
+            if (v > cd.p1) { cd.p2 = cd.p1; cd.p1 = v; }
+            else if (v > cd.p2) cd.p2 = v;
+        }
+    }
+    if constexpr (64 % T == 0) {
+#pragma unroll
+        for (int o = 1; o < T; o <<= 1) cd.merge(cd.shfl(o, T));
+    } else if constexpr (T % 64 == 0) {
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) cd.merge(cd.shfl(o, 64));
+        if ((t & 63) == 0) {
+            r0[t >> 6] = cd.v;
+            r1[t >> 6] = __int_as_float(cd.c);
+            r2[t >> 6] = cd.p1;
+            r3[t >> 6] = cd.p2;
+            r4[t >> 6] = (float)cd.nan;
+        }
+        __syncthreads();
+        if (t == 0) {
+            for (int k = 1; k < T / 64; ++k) {
+                Cand o = {r0[k], __float_as_int(r1[k]), r2[k], r3[k], r4[k] != 0.0f};
+                cd.merge(o);
+            }
+        }
+    } else {
+        r0[t] = cd.v;
+        r1[t] = __int_as_float(cd.c);
+        r2[t] = cd.p1;
+        r3[t] = cd.p2;
+        r4[t] = (float)cd.nan;
+        __syncthreads();
+        if (t == 0) {
+            for (int k = 1; k < T; ++k) {
+                Cand o = {r0[k], __float_as_int(r1[k]), r2[k], r3[k], r4[k] != 0.0f};
+                cd.merge(o);
+            }
+        }
+    }
+    if (t == 0 && gw < g.total) {
+        float *rec = g.rec + (size_t)gw * kRec;
+        if (!valid) {
+            rec[kRecFlag] = 0.0f;
+        } else {
+            float h = cd.v, det = (cd.p2 > 0.0f) ? cd.p1 / cd.p2 : 0.0f;
+            int i0 = 0, j0 = 0;
+            if (cd.c != 0x7fffffff) { i0 = N / 4 + cd.c / CW; j0 = N / 4 + cd.c % CW; }
+            if (cd.nan) {
+                seq_search(P, N, N, PX, &h, &i0, &j0);
+                det = seq_detect(P, N, N, PX);
+            }
+            g.ratio[(size_t)n * nWin + w] = det;
+            const bool gate = (h <= 0.0f) || i0 < 2 || i0 >= N - 2 || j0 < 2 || j0 >= N - 2;
+            if (gate) {
+                const size_t o = (size_t)n * nWin + w;
+                g.pkX[o] = __builtin_nanf("") - N / 2.0f;
+                g.pkY[o] = __builtin_nanf("") - N / 2.0f;
+                g.sx[o] = 0.0f; g.sy[o] = 0.0f; g.sxy[o] = 0.0f;
+                g.pkH[o] = 0.0f * enorm;
+                rec[kRecFlag] = 0.0f;
+            } else {
+#pragma unroll
+                for (int i = 0; i < PK; ++i)
+#pragma unroll
+                    for (int j = 0; j < PK; ++j) {
+                        const int r = i0 + i - 2, c = j0 + j - 2;
+                        const float v = P[r * PX + c];
+                        rec[i * PK + j] = g.pw ? v * g.pw[r * N + c] : v;
+                    }
+                rec[kRecH] = h;
+                rec[kRecI0] = (float)i0;
+                rec[kRecJ0] = (float)j0;
+                rec[kRecEnorm] = enorm;
+                rec[kRecFlag] = 1.0f;
+            }
+        }
+    }
+}
+
 template <int N>
 __global__ void __launch_bounds__((XcCfg<N>::BLOCK))
 xcorr_kernel(XcorrArgs g)
@@ -564,119 +691,7 @@ xcorr_kernel(XcorrArgs g)
     }
     __syncthreads();
 
-    const float *P = Sr;
-    if (g.planes && valid) {
-        float *dst = g.planes + (size_t)gw * numel;
-        for (int e = t; e < numel; e += T) dst[e] = P[(e / N) * PX + (e % N)];
-    }
-    if (g.ensemble) return;    // uniform: the flag is a kernel argument
-
-    // ---- 5. peak search; the fit runs in fit_kernel -----------------------
-    if (g.nPeaks > 1) {
-        // multi-peak: the plane goes to the workspace for multipeak_kernel
-        if (valid) {
-            float *dst = g.work_planes + (size_t)gw * numel;
-            for (int e = t; e < numel; e += T) dst[e] = P[(e / N) * PX + (e % N)];
-        }
-        if (t == 0 && gw < g.total) {
-            float *rec = g.rec + (size_t)gw * kRec;
-            rec[kRecFlag] = valid ? 1.0f : 0.0f;
-            rec[kRecEnorm] = enorm;
-        }
-        return;
-    }
-    constexpr int CW = N / 2, NC = (N / 2) * (N / 2);
-    Cand cd = {0.0f, 0x7fffffff, 0.0f, 0.0f, 0};
-    if (valid) {
-        for (int c = t; c < NC; c += T) {
-            const int i = N / 4 + c / CW, j = N / 4 + c % CW;
-            const float v = P[i * PX + j];
-            if (v != v) { cd.nan = 1; continue; }
-            if (!(v > 0.0f)) continue;
-            bool is_max = true;
-#pragma unroll
-            for (int di = -1; di <= 1; ++di)
-#pragma unroll
-                for (int dj = -1; dj <= 1; ++dj)
-                    if (P[(i + di) * PX + (j + dj)] > v) is_max = false;
-            if (!is_max) continue;
-            if (v > cd.v) { cd.v = v; cd.c = c; }
-            if (v > cd.p1) { cd.p2 = cd.p1; cd.p1 = v; }
-            else if (v > cd.p2) cd.p2 = v;
-        }
-    }
-    if constexpr (64 % T == 0) {
-#pragma unroll
-        for (int o = 1; o < T; o <<= 1) cd.merge(cd.shfl(o, T));
-    } else if constexpr (T % 64 == 0) {
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) cd.merge(cd.shfl(o, 64));
-        if ((t & 63) == 0) {
-            r0[t >> 6] = cd.v;
-            r1[t >> 6] = __int_as_float(cd.c);
-            r2[t >> 6] = cd.p1;
-            r3[t >> 6] = cd.p2;
-            r4[t >> 6] = (float)cd.nan;
-        }
-        __syncthreads();
-        if (t == 0) {
-            for (int k = 1; k < T / 64; ++k) {
-                Cand o = {r0[k], __float_as_int(r1[k]), r2[k], r3[k], r4[k] != 0.0f};
-                cd.merge(o);
-            }
-        }
-    } else {
-        r0[t] = cd.v;
-        r1[t] = __int_as_float(cd.c);
-        r2[t] = cd.p1;
-        r3[t] = cd.p2;
-        r4[t] = (float)cd.nan;
-        __syncthreads();
-        if (t == 0) {
-            for (int k = 1; k < T; ++k) {
-                Cand o = {r0[k], __float_as_int(r1[k]), r2[k], r3[k], r4[k] != 0.0f};
-                cd.merge(o);
-            }
-        }
-    }
-    if (t == 0 && gw < g.total) {
-        float *rec = g.rec + (size_t)gw * kRec;
-        if (!valid) {
-            rec[kRecFlag] = 0.0f;
-        } else {
-            float h = cd.v, det = (cd.p2 > 0.0f) ? cd.p1 / cd.p2 : 0.0f;
-            int i0 = 0, j0 = 0;
-            if (cd.c != 0x7fffffff) { i0 = N / 4 + cd.c / CW; j0 = N / 4 + cd.c % CW; }
-            if (cd.nan) {
-                seq_search(P, N, N, PX, &h, &i0, &j0);
-                det = seq_detect(P, N, N, PX);
-            }
-            g.ratio[(size_t)n * nWin + w] = det;
-            const bool gate = (h <= 0.0f) || i0 < 2 || i0 >= N - 2 || j0 < 2 || j0 >= N - 2;
-            if (gate) {
-                const size_t o = (size_t)n * nWin + w;
-                g.pkX[o] = __builtin_nanf("") - N / 2.0f;
-                g.pkY[o] = __builtin_nanf("") - N / 2.0f;
-                g.sx[o] = 0.0f; g.sy[o] = 0.0f; g.sxy[o] = 0.0f;
-                g.pkH[o] = 0.0f * enorm;
-                rec[kRecFlag] = 0.0f;
-            } else {
-#pragma unroll
-                for (int i = 0; i < PK; ++i)
-#pragma unroll
-                    for (int j = 0; j < PK; ++j) {
-                        const int r = i0 + i - 2, c = j0 + j - 2;
-                        const float v = P[r * PX + c];
-                        rec[i * PK + j] = g.pw ? v * g.pw[r * N + c] : v;
-                    }
-                rec[kRecH] = h;
-                rec[kRecI0] = (float)i0;
-                rec[kRecJ0] = (float)j0;
-                rec[kRecEnorm] = enorm;
-                rec[kRecFlag] = 1.0f;
-            }
-        }
-    }
+    plane_epilogue<N, T>(g, Sr, PX, valid, gw, n, w, enorm, t, r0, r1, r2, r3, r4);
 }
 
 #ifdef PIVAMD_XCORR_COMMON
@@ -823,6 +838,332 @@ __global__ void conv_kernel(const float *a, const float *b, int NY, int NX, floa
 
 #endif  // PIVAMD_XCORR_COMMON
 
+
+#if defined(PIVAMD_XCORR_SIZE) && (PIVAMD_XCORR_SIZE % 16 == 0) && (PIVAMD_XCORR_SIZE <= 64)
+// ---------------------------------------------------------------------------
+// Split-precision MFMA form of the same correlation (N = 16, 32, 48, 64).
+//
+// The 2D transforms are dense products with the DFT matrix F (F[m][k] =
+// exp(-2 pi i m k / N), symmetric), computed on the matrix cores in fp16
+// with a hi/lo split of every operand (x = hi + lo, both fp16) and FP32
+// accumulation of hi*hi + hi*lo + lo*hi, which keeps about 22 significant
+// bits per product:
+//   Z = A + i B                  (weighted, mean-removed windows)
+//   P   = F Z                    (columns)          4 real GEMMs
+//   Q^T = F P^T                  (rows)             4
+//   FA, FB from Q(k), Q(-k);  C = FB conj(FA)
+//   W   = conj(F) C                                  4
+//   plane^T = Re[conj(F) W^T] / N^2                  2
+// Every stage multiplies a constant by data, so F is always the A operand
+// (each wave keeps its 16-row tile of Fr, Fi and -Fi, hi and lo, in
+// registers for the whole kernel) and the data always enters as the B
+// operand from LDS, stored split and k-contiguous by the stage before.
+// Power-of-two scales keep every fp16 operand within 32768 (window values
+// <= 512, cross spectrum <= 512) and are removed exactly at the end.
+// One wave per 16-row output tile, N/16 waves per window; the plane lands
+// in the same fftshifted LDS tile as the FFT form and shares its epilogue.
+// ---------------------------------------------------------------------------
+typedef _Float16 mf_h4 __attribute__((ext_vector_type(4)));
+typedef _Float16 mf_h8 __attribute__((ext_vector_type(8)));
+typedef float mf_f4 __attribute__((ext_vector_type(4)));
+
+template <int N>
+struct MfCfg {
+    static constexpr int KV = (N % 32 == 0) ? 8 : 4;        // halfs per lane per operand
+    static constexpr int KM = 4 * KV;                       // K of one MFMA (32 or 16)
+    static constexpr int KCH = N / KM;                      // k-chunks per product
+    static constexpr int TL = N / 16;                       // 16x16 tiles per dimension
+    static constexpr int WPW = TL;                          // waves per window
+    static constexpr int T = 64 * WPW;                      // lanes per window
+    static constexpr int PX = N + 1;
+    static constexpr int NN = N * N;
+    // LDS per window (bytes): data operands (2 complex split matrices of
+    // fp16: 8 NN each) + the FP32 Q / plane region (8 NN)
+    static constexpr int lds_win = 8 * NN + 8 * NN + 8 * NN;
+    static constexpr int wpb_lds = (64 * 1024) / lds_win;
+    static constexpr int wpb_thr = 512 / T;
+    static constexpr int WPB0 = wpb_lds < wpb_thr ? wpb_lds : wpb_thr;
+    static constexpr int WPB = WPB0 > 0 ? WPB0 : 1;
+    static constexpr int BLOCK = WPB * T;
+};
+
+template <int KV>
+using mf_hv = typename std::conditional<KV == 8, mf_h8, mf_h4>::type;
+
+template <int KV>
+__device__ __forceinline__ mf_f4 mf_mma(const mf_hv<KV> &a, const mf_hv<KV> &b, mf_f4 c)
+{
+    if constexpr (KV == 8) return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+    else return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
+}
+
+// hi*hi + hi*lo + lo*hi
+template <int KV>
+__device__ __forceinline__ mf_f4 mf_mma3(const mf_hv<KV> &ah, const mf_hv<KV> &al, const mf_hv<KV> &bh,
+                                         const mf_hv<KV> &bl, mf_f4 c)
+{
+    c = mf_mma<KV>(ah, bh, c);
+    c = mf_mma<KV>(ah, bl, c);
+    c = mf_mma<KV>(al, bh, c);
+    return c;
+}
+
+__device__ __forceinline__ void mf_split(float x, _Float16 &hi, _Float16 &lo)
+{
+    hi = (_Float16)x;
+    lo = (_Float16)(x - (float)hi);
+}
+
+// One split complex matrix in LDS: four N x N fp16 planes (re hi, re lo,
+// im hi, im lo), each stored [j][k] (k contiguous) for the B operand.
+template <int N>
+struct MfMat {
+    _Float16 *rh, *rl, *ih, *il;
+    __device__ MfMat(_Float16 *base) : rh(base), rl(base + N * N), ih(base + 2 * N * N), il(base + 3 * N * N) {}
+};
+
+// Constant fragments of this wave's row tile: F (forward), hi/lo parts.
+template <int N>
+struct MfConst {
+    using C = MfCfg<N>;
+    mf_hv<C::KV> frh[C::KCH], frl[C::KCH], fih[C::KCH], fil[C::KCH], nih[C::KCH], nil[C::KCH];
+    __device__ void init(int rowtile, int lane)
+    {
+        const int m = 16 * rowtile + (lane & 15);
+#pragma unroll
+        for (int c = 0; c < C::KCH; ++c) {
+#pragma unroll
+            for (int v = 0; v < C::KV; ++v) {
+                const int k = c * C::KM + C::KV * (lane >> 4) + v;
+                const int t = (m * k) % N;
+                double sn, cs;
+                sincospi(2.0 * (double)t / (double)N, &sn, &cs);
+                const double re = cs, im = -sn;
+                const _Float16 rh = (_Float16)re, ih = (_Float16)im;
+                frh[c][v] = rh;
+                frl[c][v] = (_Float16)(re - (double)rh);
+                fih[c][v] = ih;
+                fil[c][v] = (_Float16)(im - (double)ih);
+                nih[c][v] = -fih[c][v];
+                nil[c][v] = -fil[c][v];
+            }
+        }
+    }
+};
+
+// Out = K1 X1 + K2 X2 for one output tile (row tile of this wave, column
+// tile J): X1, X2 are B operands from split planes (x1h/x1l, x2h/x2l), K1, K2
+// constant fragments (hi/lo).
+template <int N>
+__device__ __forceinline__ mf_f4 mf_pair(const mf_hv<MfCfg<N>::KV> *k1h, const mf_hv<MfCfg<N>::KV> *k1l,
+                                         const _Float16 *x1h, const _Float16 *x1l,
+                                         const mf_hv<MfCfg<N>::KV> *k2h, const mf_hv<MfCfg<N>::KV> *k2l,
+                                         const _Float16 *x2h, const _Float16 *x2l, int J, int lane)
+{
+    using C = MfCfg<N>;
+    using hv = mf_hv<C::KV>;
+    mf_f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    const int j = 16 * J + (lane & 15);
+#pragma unroll
+    for (int c = 0; c < C::KCH; ++c) {
+        const int k0 = c * C::KM + C::KV * (lane >> 4);
+        const hv b1h = *(const hv *)(x1h + j * N + k0);
+        const hv b1l = *(const hv *)(x1l + j * N + k0);
+        const hv b2h = *(const hv *)(x2h + j * N + k0);
+        const hv b2l = *(const hv *)(x2l + j * N + k0);
+        acc = mf_mma3<C::KV>(k1h[c], k1l[c], b1h, b1l, acc);
+        acc = mf_mma3<C::KV>(k2h[c], k2l[c], b2h, b2l, acc);
+    }
+    return acc;
+}
+
+template <int N>
+__global__ void __launch_bounds__((MfCfg<N>::BLOCK)) xcorr_mfma_kernel(XcorrArgs g)
+{
+    using C = MfCfg<N>;
+    constexpr int T = C::T, WPB = C::WPB, NN = C::NN, PX = C::PX;
+    constexpr int lds_half = 4 * NN;                  // halfs in one split complex matrix
+    __shared__ __attribute__((aligned(16))) _Float16 s_mat[WPB][2][lds_half];   // Z, then C | P, then W
+    __shared__ float s_q[WPB][2 * NN];                // Q (re, im), then the plane tile
+    __shared__ float s_red[5][WPB * T];
+
+    const int b = remap_block((int)blockIdx.x, g.nblocks);
+    if (b < 0) return;
+    const int wloc = (int)threadIdx.x / T;
+    const int t = (int)threadIdx.x - wloc * T;
+    const int lane = t & 63, wave = t >> 6;           // wave = this wave's row tile
+    float *r0 = &s_red[0][wloc * T], *r1 = &s_red[1][wloc * T], *r2 = &s_red[2][wloc * T];
+    float *r3 = &s_red[3][wloc * T], *r4 = &s_red[4][wloc * T];
+    MfMat<N> M0(s_mat[wloc][0]), M1(s_mat[wloc][1]);
+    float *Qr = s_q[wloc], *Qi = s_q[wloc] + NN;
+
+    MfConst<N> K;
+    K.init(wave, lane);
+
+    const int nWin = g.nWy * g.nWx;
+    const long long gw = (long long)b * WPB + wloc;
+    int n = 0, w = 0, row0 = 0, col0 = 0;
+    bool valid = gw < g.total;
+    if (valid) {
+        n = (int)(gw / nWin);
+        w = (int)(gw - (long long)n * nWin);
+        const int ii = w % g.nWx, jj = w / g.nWx;
+        if (g.mask && g.mask[jj * g.nWx + ii] == 1.0f) valid = false;
+        row0 = (int)floor((double)g.ctrY[jj] - ((double)(float)N - 1.0) / 2.0 + 0.5);
+        col0 = (int)floor((double)g.ctrX[ii] - ((double)(float)N - 1.0) / 2.0 + 0.5);
+        if (row0 < 0 || col0 < 0 || row0 + N > g.H || col0 + N > g.W) valid = false;
+    }
+
+    // ---- window: weight, mean, energy (element e = t + T m, row-major) -----
+    constexpr int EPL = NN / T;                       // elements per lane
+    float za[EPL], zb[EPL];
+    float sa = 0.0f, sb = 0.0f;
+    if (valid) {
+        const size_t img = (size_t)n * g.H * g.W;
+#pragma unroll
+        for (int m = 0; m < EPL; ++m) {
+            const int e = t + T * m, y = e / N, x = e - (e / N) * N;
+            const size_t o = img + (size_t)(row0 + y) * g.W + col0 + x;
+            za[m] = g.imgA[o] * g.wA[e];
+            zb[m] = g.imgB[o] * g.wB[e];
+            sa += za[m];
+            sb += zb[m];
+        }
+    } else {
+#pragma unroll
+        for (int m = 0; m < EPL; ++m) { za[m] = 0.0f; zb[m] = 0.0f; }
+    }
+    win_sum2<T>(sa, sb, r0, r1, t);
+    const float meanA = sa / (float)NN, meanB = sb / (float)NN;
+    float ea = 0.0f, eb = 0.0f;
+#pragma unroll
+    for (int m = 0; m < EPL; ++m) {
+        if (valid && !g.ensemble) {
+            za[m] -= meanA;
+            zb[m] -= meanB;
+        }
+        ea += za[m] * za[m];
+        eb += zb[m] * zb[m];
+    }
+    win_sum2<T>(ea, eb, r0, r1, t);
+    const float enorm = 1.0f / sqrtf(ea * eb);
+    // scales: sqrt(max energy) -> <= 512 for the window, N^2 sqrt(eA eB) -> <= 512 for C
+    // log2 bounds: sqrt(e) < 2^(ilogb + 1); |C| <= N^2 sqrt(eA eB) after the
+    // window scale 2^ez on both frames
+    const float emax = fmaxf(ea, eb);
+    const bool scalable = emax > 0.0f && isfinite(emax);
+    const int ez = scalable ? 8 - ilogbf(sqrtf(emax)) : 0;
+    const bool cross = scalable && ea > 0.0f && eb > 0.0f;
+    const int lc = cross ? (ilogbf(sqrtf(ea)) + 1) + (ilogbf(sqrtf(eb)) + 1) + 2 * ez + 2 * (ilogbf((float)N) + 1)
+                         : 0;
+    const int ec = cross ? 9 - lc : 0;
+#pragma unroll
+    for (int m = 0; m < EPL; ++m) {
+        const int e = t + T * m, y = e / N, x = e - (e / N) * N;
+        // Z stored [j = column x][k = row y]
+        _Float16 h, l;
+        mf_split(ldexpf(za[m], ez), h, l);
+        M0.rh[x * N + y] = h;
+        M0.rl[x * N + y] = l;
+        mf_split(ldexpf(zb[m], ez), h, l);
+        M0.ih[x * N + y] = h;
+        M0.il[x * N + y] = l;
+    }
+    __syncthreads();
+
+    // ---- P = F Z ; stored row-major P[r][c] into M1 ---------------------------
+    const int rb = 16 * wave + 4 * (lane >> 4), cl = lane & 15;
+#pragma unroll
+    for (int J = 0; J < C::TL; ++J) {
+        const mf_f4 pr = mf_pair<N>(K.frh, K.frl, M0.rh, M0.rl, K.nih, K.nil, M0.ih, M0.il, J, lane);
+        const mf_f4 pi = mf_pair<N>(K.fih, K.fil, M0.rh, M0.rl, K.frh, K.frl, M0.ih, M0.il, J, lane);
+        const int c = 16 * J + cl;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int r = rb + v;
+            mf_split(pr[v], M1.rh[r * N + c], M1.rl[r * N + c]);
+            mf_split(pi[v], M1.ih[r * N + c], M1.il[r * N + c]);
+        }
+    }
+    __syncthreads();
+
+    // ---- Q^T = F P^T ; Q[ky][kx] = Qt[kx][ky] in FP32 --------------------------
+#pragma unroll
+    for (int J = 0; J < C::TL; ++J) {
+        const mf_f4 qr = mf_pair<N>(K.frh, K.frl, M1.rh, M1.rl, K.nih, K.nil, M1.ih, M1.il, J, lane);
+        const mf_f4 qi = mf_pair<N>(K.fih, K.fil, M1.rh, M1.rl, K.frh, K.frl, M1.ih, M1.il, J, lane);
+        const int ky = 16 * J + cl;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            Qr[ky * N + rb + v] = qr[v];
+            Qi[ky * N + rb + v] = qi[v];
+        }
+    }
+    __syncthreads();
+
+    // ---- C = FB conj(FA), scaled by 2^ec; stored [kx][ky] into M0 ------------
+#pragma unroll
+    for (int m = 0; m < EPL; ++m) {
+        const int e = t + T * m, ky = e / N, kx = e - (e / N) * N;
+        const int my = (N - ky) % N, mx = (N - kx) % N;
+        const float zR = Qr[ky * N + kx], zI = Qi[ky * N + kx];
+        const float mR = Qr[my * N + mx], mI = Qi[my * N + mx];
+        const float ar = 0.5f * (zR + mR), ai = 0.5f * (zI - mI);
+        const float br = 0.5f * (zI + mI), bi = -0.5f * (zR - mR);
+        const float cr = ldexpf(br * ar + bi * ai, ec), ci = ldexpf(bi * ar - br * ai, ec);
+        mf_split(cr, M0.rh[kx * N + ky], M0.rl[kx * N + ky]);
+        mf_split(ci, M0.ih[kx * N + ky], M0.il[kx * N + ky]);
+    }
+    __syncthreads();
+
+    // ---- W = conj(F) C = (Fr Cr + Fi Ci) + i (Fr Ci - Fi Cr); row-major into M1 --
+#pragma unroll
+    for (int J = 0; J < C::TL; ++J) {
+        const mf_f4 wr = mf_pair<N>(K.frh, K.frl, M0.rh, M0.rl, K.fih, K.fil, M0.ih, M0.il, J, lane);
+        const mf_f4 wi = mf_pair<N>(K.nih, K.nil, M0.rh, M0.rl, K.frh, K.frl, M0.ih, M0.il, J, lane);
+        const int c = 16 * J + cl;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int r = rb + v;
+            mf_split(wr[v], M1.rh[r * N + c], M1.rl[r * N + c]);
+            mf_split(wi[v], M1.ih[r * N + c], M1.il[r * N + c]);
+        }
+    }
+    __syncthreads();
+
+    // ---- plane^T = Fr W^T_r + Fi W^T_i ; plane[y][x] = planeT[x][y] -----------
+    float *P = s_q[wloc];                              // plane tile, row stride PX
+    const int eback = -(2 * ez + ec);               // then 1 / numel, as the FFT form scales
+    const float mul = 1.0f / (float)NN;
+#pragma unroll
+    for (int J = 0; J < C::TL; ++J) {
+        const mf_f4 pt = mf_pair<N>(K.frh, K.frl, M1.rh, M1.rl, K.fih, K.fil, M1.ih, M1.il, J, lane);
+        const int y = 16 * J + cl;                    // planeT column = plane row
+        const int ys = (y + N / 2) % N;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int x = rb + v;
+            P[ys * PX + (x + N / 2) % N] = ldexpf(pt[v], eback) * mul;
+        }
+    }
+    __syncthreads();
+    plane_epilogue<N, T>(g, P, PX, valid, gw, n, w, enorm, t, r0, r1, r2, r3, r4);
+}
+
+static int launch_mfma(XcorrArgs g, hipStream_t st)
+{
+    constexpr int N = PIVAMD_XCORR_SIZE;
+    using C = MfCfg<N>;
+    const long long nb = (g.total + C::WPB - 1) / C::WPB;
+    if (nb <= 0) return 0;
+    if (nb > (1LL << 30)) return PIVAMD_ERR_ARG;
+    g.nblocks = (int)nb;
+    const int grid = (int)((nb + 7) / 8 * 8);
+    hipLaunchKernelGGL((xcorr_mfma_kernel<N>), dim3(grid), dim3(C::BLOCK), 0, st, g);
+    return hipGetLastError() == hipSuccess ? 0 : PIVAMD_ERR_LAUNCH;
+}
+#endif
+
 // Launch of the correlation kernel for one square size; defined in the
 // translation unit compiled with PIVAMD_XCORR_SIZE.
 template <int N>
@@ -833,6 +1174,9 @@ template <>
 int launch_main<PIVAMD_XCORR_SIZE>(XcorrArgs g, hipStream_t st)
 {
     constexpr int N = PIVAMD_XCORR_SIZE;
+#if (PIVAMD_XCORR_SIZE % 16 == 0) && (PIVAMD_XCORR_SIZE <= 64)
+    if (g.dft == 1) return launch_mfma(g, st);
+#endif
     using C = XcCfg<N>;
     const long long nb = (g.total + C::WPB - 1) / C::WPB;
     if (nb <= 0) return 0;
@@ -927,6 +1271,18 @@ extern "C" size_t pivamd_xcorr_workspace(int n_images, int nWy, int nWx, int win
     return bytes;
 }
 
+// Transform used by pivamd_xcorr: 0 = FP32 FFT (default, the reference's
+// arithmetic class), 1 = split-fp16 MFMA DFT for 16/32/48/64-point windows
+// (other sizes keep the FFT).  Read when a launch is issued.
+static int g_dft_mode = 0;
+
+extern "C" int pivamd_set_dft_mode(int mode)
+{
+    const int prev = g_dft_mode;
+    if (mode == 0 || mode == 1) g_dft_mode = mode;
+    return prev;
+}
+
 extern "C" int pivamd_xcorr(const float *d_imgA, const float *d_imgB, const float *d_mask, int H, int W,
                             int n_images, const float *d_ctrX, const float *d_ctrY, int nWy, int nWx,
                             const float *d_wA, const float *d_wB, const float *d_planeWeight, int ensemble,
@@ -947,6 +1303,7 @@ extern "C" int pivamd_xcorr(const float *d_imgA, const float *d_imgB, const floa
     g.total = (long long)n_images * nWy * nWx;
     g.H = H; g.W = W; g.nWy = nWy; g.nWx = nWx; g.nPeaks = nPeaks; g.finder = finder;
     g.impl = impl; g.ensemble = ensemble; g.nblocks = 0;
+    g.dft = (g_dft_mode == 1 && winY == winX && winY % 16 == 0 && winY <= 64) ? 1 : 0;
     if (!ensemble && (!d_pkX || !d_pkY || !d_pkH || !d_sx || !d_sy || !d_sxy || !d_ratio || !d_work))
         return PIVAMD_ERR_ARG;
 if (winY != winX || !pivamd_size_supported(winY, winX)) return PIVAMD_ERR_SIZE;

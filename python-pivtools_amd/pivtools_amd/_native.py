@@ -47,6 +47,8 @@ def core() -> ctypes.CDLL:
         L = ctypes.CDLL(lib_path("libpivtools_amd.so"))
         L.pivamd_size_supported.argtypes = [_i, _i]
         L.pivamd_size_supported.restype = _i
+        L.pivamd_set_dft_mode.argtypes = [_i]
+        L.pivamd_set_dft_mode.restype = _i
         L.pivamd_plane_weight.argtypes = [_vp, _i, _i, _vp, _vp]
         L.pivamd_plane_weight.restype = _i
         L.pivamd_convolve.argtypes = [_vp, _vp, _i, _i, _vp, _vp]

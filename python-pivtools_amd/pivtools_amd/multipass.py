@@ -82,8 +82,11 @@ class MultipassPIV:
 
     def __init__(self, image_shape, window_sizes, overlaps, *, window_type="gaussian", peak_finder="gauss3",
                  peak_fit_impl="scalar", warp_interpolation="cubic", predictor_smoothing=True,
-                 median=(0.1, 2.0, 5), final_infill="nearest", device="cuda"):
+                 median=(0.1, 2.0, 5), final_infill="nearest", dft="fft", device="cuda"):
         self.H, self.W = image_shape
+        if dft not in ("fft", "split"):
+            raise ValueError(f"dft {dft!r}: 'fft' (FP32 FFT) or 'split' (split-fp16 MFMA DFT, N <= 64)")
+        self.dft = 1 if dft == "split" else 0
         self.window_sizes = [tuple(w) if np.ndim(w) else (w, w) for w in window_sizes]
         if len(overlaps) != len(self.window_sizes):
             raise ValueError("one overlap per pass")
@@ -218,12 +221,14 @@ class MultipassPIV:
             if events is not None:
                 ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
                 ev[0].record()
-            _native.check(L.pivamd_xcorr(_ptr(a_in), _ptr(b_in), None, self.H, self.W, n, _ptr(b.ctrs["x"]),
+            prev_mode = L.pivamd_set_dft_mode(self.dft)
+            rc = L.pivamd_xcorr(_ptr(a_in), _ptr(b_in), None, self.H, self.W, n, _ptr(b.ctrs["x"]),
                                          _ptr(b.ctrs["y"]), ny, nx, _ptr(b.w), _ptr(b.w), _ptr(b.pw), 0,
                                          g.win[0], g.win[1], 1, self.finder, self.impl, _ptr(o["pkx"]),
                                          _ptr(o["pky"]), _ptr(o["pkh"]), _ptr(o["sx"]), _ptr(o["sy"]),
-                                         _ptr(o["sxy"]), _ptr(o["ratio"]), None, _ptr(b.work), b.work.numel(), st),
-                          "bulkxcorr2d")
+                                         _ptr(o["sxy"]), _ptr(o["ratio"]), None, _ptr(b.work), b.work.numel(), st)
+            L.pivamd_set_dft_mode(prev_mode)
+            _native.check(rc, "bulkxcorr2d")
             if events is not None:
                 ev[1].record()
                 events.append((p, ev))
