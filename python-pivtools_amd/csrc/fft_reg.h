@@ -190,17 +190,49 @@ namespace pivamd {
 // twiddle W_N^(q k1), then the radix-R combination runs across the R lanes
 // as R = 2: one DPP exchange; R = 4: two exchange stages (DIF, so the
 // blocks come out bit-reversed).
+//
+// The twiddles come from a table (SplitTw: entry q*N1 + k1 = W_N^(q k1),
+// forward sign) that the kernel copies to LDS once; lane q reads its row
+// with immediate offsets.  A per-lane select among R compile-time
+// constants costs 2(R-1) instructions per value, the table one LDS read per
+// two twiddles.  Each exchange step is own*sign + partner: the product is
+// exact (sign = +-1), and keeping it a separate multiply lets the DPP move
+// fold into the add (v_add_f32_dpp) instead of feeding an FMA.
 // ---------------------------------------------------------------------------
+template <int N, int R>
+struct SplitTw {
+    float v[2 * N];
+    constexpr SplitTw() : v{}
+    {
+        for (int q = 0; q < R; ++q)
+            for (int k = 0; k < N / R; ++k) {
+                v[2 * (q * (N / R) + k)] = (float)ct_cos2pi((long)q * k, N);
+                v[2 * (q * (N / R) + k) + 1] = (float)-ct_sin2pi((long)q * k, N);
+            }
+    }
+};
+
+template <int N, int R>
+__constant__ SplitTw<N, R> kSplitTw{};
+
 __device__ __forceinline__ float dpp_swap1(float v)
 {
     // quad_perm [1,0,3,2]
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, false));
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, true));
 }
 
 __device__ __forceinline__ float dpp_swap2(float v)
 {
     // quad_perm [2,3,0,1]
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4E, 0xF, 0xF, false));
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, true));
+}
+
+// partner + sign * own, rounded once (the product is exact)
+__device__ __forceinline__ float xadd(float partner, float sign, float own)
+{
+#pragma clang fp contract(off)
+    const float t = sign * own;
+    return partner + t;
 }
 
 template <int R>
@@ -210,27 +242,18 @@ __device__ __forceinline__ int out_block(int q)
     else return q;
 }
 
+// tw: this lane's table row, kSplitTw<N, R> entries q*N1 .. q*N1 + N1-1 (in
+// LDS); unused for R = 1.
 template <int N, int R, bool INV>
-__device__ __forceinline__ void fft_split(float (&xr)[N / R], float (&xi)[N / R], int q)
+__device__ __forceinline__ void fft_split(float (&xr)[N / R], float (&xi)[N / R], int q, const float2 *tw)
 {
     constexpr int N1 = N / R;
     fft_reg<N1, INV>(xr, xi);
     if constexpr (R > 1) {
         static_for<1, N1>([&](auto K) {
             constexpr int k1 = decltype(K)::value;
-            constexpr float c1 = (float)ct_cos2pi(k1, N), s1 = (float)(INV ? ct_sin2pi(k1, N) : -ct_sin2pi(k1, N));
-            float c, s;
-            if constexpr (R == 2) {
-                c = q ? c1 : 1.0f;
-                s = q ? s1 : 0.0f;
-            } else {
-                constexpr float c2 = (float)ct_cos2pi(2 * k1, N);
-                constexpr float s2 = (float)(INV ? ct_sin2pi(2 * k1, N) : -ct_sin2pi(2 * k1, N));
-                constexpr float c3 = (float)ct_cos2pi(3 * k1, N);
-                constexpr float s3 = (float)(INV ? ct_sin2pi(3 * k1, N) : -ct_sin2pi(3 * k1, N));
-                c = q == 0 ? 1.0f : (q == 1 ? c1 : (q == 2 ? c2 : c3));
-                s = q == 0 ? 0.0f : (q == 1 ? s1 : (q == 2 ? s2 : s3));
-            }
+            const float2 w = tw[k1];
+            const float c = w.x, s = INV ? -w.y : w.y;
             const float r = xr[k1], i = xi[k1];
             xr[k1] = r * c - i * s;
             xi[k1] = r * s + i * c;
@@ -239,9 +262,8 @@ __device__ __forceinline__ void fft_split(float (&xr)[N / R], float (&xi)[N / R]
             const float sg = (q & 1) ? -1.0f : 1.0f;
 #pragma unroll
             for (int k = 0; k < N1; ++k) {
-                const float orr = dpp_swap1(xr[k]), oi = dpp_swap1(xi[k]);
-                xr[k] = orr + sg * xr[k];
-                xi[k] = oi + sg * xi[k];
+                xr[k] = xadd(dpp_swap1(xr[k]), sg, xr[k]);
+                xi[k] = xadd(dpp_swap1(xi[k]), sg, xi[k]);
             }
         } else {
             const float sa = (q & 2) ? -1.0f : 1.0f;
@@ -249,15 +271,14 @@ __device__ __forceinline__ void fft_split(float (&xr)[N / R], float (&xi)[N / R]
             const bool rot = (q == 3);
 #pragma unroll
             for (int k = 0; k < N1; ++k) {
-                const float orr = dpp_swap2(xr[k]), oi = dpp_swap2(xi[k]);
-                float tr = orr + sa * xr[k], ti = oi + sa * xi[k];
+                float tr = xadd(dpp_swap2(xr[k]), sa, xr[k]);
+                float ti = xadd(dpp_swap2(xi[k]), sa, xi[k]);
                 // lane 3 multiplies by W_4 = -i (forward) or +i (inverse)
                 const float rr = INV ? -ti : ti, ri = INV ? tr : -tr;
                 tr = rot ? rr : tr;
                 ti = rot ? ri : ti;
-                const float pr = dpp_swap1(tr), pi = dpp_swap1(ti);
-                xr[k] = pr + sb * tr;
-                xi[k] = pi + sb * ti;
+                xr[k] = xadd(dpp_swap1(tr), sb, tr);
+                xi[k] = xadd(dpp_swap1(ti), sb, ti);
             }
         }
     }

@@ -45,24 +45,82 @@ struct XcorrArgs {
     int dft;                // 1: split-fp16 MFMA DFT (N = 16, 32, 48, 64), 0: FP32 FFT
 };
 
-constexpr int kMaxLds = 150 * 1024;
+constexpr int kMaxLds = 158 * 1024;
+
+// Lanes per line transform for window size N.
+constexpr int xc_radix(int N) { return N >= 64 ? 4 : (N >= 48 ? 2 : 1); }
+
+// Tile layout of the complex stages.  R = 1: element (r, c) at r*(N+1) + c.
+// R > 1: at r*TPX + SKEW*(r/N1) + c + PADC*(c/N1), i.e. rows padded to TPX,
+// PADC words after every block of N1 columns and the row blocks skewed by
+// SKEW words.  The R lanes of a line touch elements N1 apart (their output
+// blocks) or adjacent (their interleaved inputs); with the plain N+1 pitch
+// the former fall on one bank (2-4 way conflicts on every stage).  The
+// constants minimise the modelled conflicts of all six tile access patterns
+// (64 banks per wave and 32 banks per half wave); the block indices r/N1 and
+// c/N1 are compile-time or per-lane constants at every access, so the
+// addresses stay a lane base plus an immediate.  The final real plane is
+// written with the plain N+1 pitch (after a barrier) for the epilogue.
+struct XcLayout {
+    int tpx, padc, skew;
+};
+
+constexpr XcLayout xc_layout(int N, int R)
+{
+    return R == 1 ? XcLayout{N + 1, 0, 0}
+         : (N == 128 && R == 4) ? XcLayout{140, 3, 8}
+         : (N == 96 && R == 4) ? XcLayout{100, 1, 8}
+         : (N == 32 && R == 4) ? XcLayout{44, 3, 40}
+         : (N == 64 && R == 4) ? XcLayout{76, 3, 8}
+         : (N == 64 && R == 2) ? XcLayout{66, 1, 16}
+         : (N == 48 && R == 2) ? XcLayout{50, 1, 0}
+         : (N == 32 && R == 2) ? XcLayout{34, 1, 0}
+         : (N == 16 && R == 2) ? XcLayout{17, 0, 8}
+         : XcLayout{N + 1, 0, 0};
+}
 
 // Launch geometry of one square window size N: R lanes share each line
 // (row or column) transform, T = N*R lanes per window, WPB windows per
-// workgroup as LDS allows (one complex N x (N+1) tile per window).
+// workgroup as LDS allows (one complex tile and RED reduction slots per
+// window).
 template <int N>
 struct XcCfg {
-    static constexpr int R = N >= 96 ? 4 : (N >= 48 ? 2 : 1);
+    static constexpr int R = xc_radix(N);
     static constexpr int N1 = N / R;
     static constexpr int T = N * R;
-    static constexpr int PX = N + 1;
-    static constexpr int tile = N * PX;
-    static constexpr int lds_win = 2 * tile * 4 + 5 * T * 4;
+    static constexpr int PX = N + 1;      // plane pitch
+    static constexpr int TPX = xc_layout(N, R).tpx;
+    static constexpr int PADC = xc_layout(N, R).padc;
+    static constexpr int SKEW = xc_layout(N, R).skew;
+    static constexpr int tile_words = (N - 1) * TPX + SKEW * (R - 1) + (N - 1) + PADC * (R - 1) + 1;
+    static constexpr int tile = tile_words > N * PX ? tile_words : N * PX;
+    // reduction slots: shuffles only (T | 64), one per wave (64 | T), one per lane
+    static constexpr int RED = 64 % T == 0 ? 1 : (T % 64 == 0 ? T / 64 : T);
+    static constexpr int TWP = N1 + 2;    // twiddle row pitch (float2): rows q start 4 banks apart
+    static constexpr int lds_win = 2 * tile * 4 + 5 * RED * 4;
+    static constexpr int lds_blk = R > 1 ? R * TWP * 8 : 0;
     static constexpr int wpb_thr = (256 / T) > 0 ? (256 / T) : 1;
-    static constexpr int wpb_lds = kMaxLds / lds_win;
+    static constexpr int wpb_lds = (kMaxLds - lds_blk) / lds_win;
     static constexpr int WPB = wpb_thr < wpb_lds ? wpb_thr : wpb_lds;
     static constexpr int BLOCK = WPB * T;
 };
+
+// Tile word of element (r, c); rb = r / N1 and cb = c / N1 are passed in
+// by the caller, which knows them as compile-time or per-lane constants.
+template <int N>
+__device__ __forceinline__ int tix(int r, int rb, int c, int cb)
+{
+    using C = XcCfg<N>;
+    return r * C::TPX + C::SKEW * rb + c + C::PADC * cb;
+}
+
+// Block (of N1) of index jR - q, 0 <= q < R, j >= 1: jR/N1 unless jR starts
+// a block and q > 0.
+template <int N1, int R>
+__device__ __forceinline__ int mirror_block(int j, int q)
+{
+    return ((j * R) % N1 == 0 && q > 0) ? (j * R) / N1 - 1 : (j * R) / N1;
+}
 
 // Sum of (a, b) over the T lanes of one window; every lane gets the same
 // bits.  T | 64: xor butterfly inside the wave.  64 | T: butterfly per wave,
@@ -536,21 +594,29 @@ xcorr_kernel(XcorrArgs g)
     constexpr int numel = N * N;
     __shared__ float s_re[WPB * C::tile];
     __shared__ float s_im[WPB * C::tile];
-    __shared__ float s_red[5][WPB * T];
+    __shared__ float s_red[5][WPB * C::RED];
+    __shared__ __attribute__((aligned(16))) float2 s_tw[R > 1 ? R * C::TWP : 1];   // split twiddles
 
     const int b = remap_block((int)blockIdx.x, g.nblocks);
     if (b < 0) return;    // whole block exits together
+    if constexpr (R > 1) {
+        for (int e = (int)threadIdx.x; e < N; e += C::BLOCK)
+            s_tw[(e / N1) * C::TWP + e % N1] = make_float2(kSplitTw<N, R>.v[2 * e], kSplitTw<N, R>.v[2 * e + 1]);
+        __syncthreads();
+    }
 
     const int wloc = (int)threadIdx.x / T;
     const int t = (int)threadIdx.x - wloc * T;
     const int l = t / R;              // line (row or column) of this lane
     const int q = t - l * R;          // lane within the line
     const int ob = out_block<R>(q);   // output block of fft_split
+    const int lb = l / N1;            // block of this lane's line index
+    const float2 *tw = s_tw + (R > 1 ? q * C::TWP : 0);
     float *Sr = s_re + wloc * C::tile;
     float *Si = s_im + wloc * C::tile;
-    float *r0 = &s_red[0][wloc * T], *r1 = &s_red[1][wloc * T];
-    float *r2 = &s_red[2][wloc * T], *r3 = &s_red[3][wloc * T];
-    float *r4 = &s_red[4][wloc * T];
+    float *r0 = &s_red[0][wloc * C::RED], *r1 = &s_red[1][wloc * C::RED];
+    float *r2 = &s_red[2][wloc * C::RED], *r3 = &s_red[3][wloc * C::RED];
+    float *r4 = &s_red[4][wloc * C::RED];
 
     const int nWin = g.nWy * g.nWx;
     const long long gw = (long long)b * WPB + wloc;
@@ -612,11 +678,12 @@ xcorr_kernel(XcorrArgs g)
     const float enorm = 1.0f / sqrtf(ea * eb);
 
     if (valid) {
-        fft_split<N, R, false>(zr, zi, q);
+        fft_split<N, R, false>(zr, zi, q, tw);
 #pragma unroll
         for (int k = 0; k < N1; ++k) {
-            Sr[(k + N1 * ob) * PX + l] = zr[k];
-            Si[(k + N1 * ob) * PX + l] = zi[k];
+            const int e = tix<N>(k + N1 * ob, ob, l, lb);
+            Sr[e] = zr[k];
+            Si[e] = zi[k];
         }
     }
     __syncthreads();
@@ -626,60 +693,75 @@ xcorr_kernel(XcorrArgs g)
         float xr[N1], xi[N1];
 #pragma unroll
         for (int k = 0; k < N1; ++k) {
-            xr[k] = Sr[l * PX + k * R + q];
-            xi[k] = Si[l * PX + k * R + q];
+            const int e = tix<N>(l, lb, k * R + q, (k * R) / N1);
+            xr[k] = Sr[e];
+            xi[k] = Si[e];
         }
-        fft_split<N, R, false>(xr, xi, q);
+        fft_split<N, R, false>(xr, xi, q, tw);
 #pragma unroll
         for (int k = 0; k < N1; ++k) {
-            Sr[l * PX + k + N1 * ob] = xr[k];
-            Si[l * PX + k + N1 * ob] = xi[k];
+            const int e = tix<N>(l, lb, k + N1 * ob, ob);
+            Sr[e] = xr[k];
+            Si[e] = xi[k];
         }
     }
     __syncthreads();
 
     // ---- 3. column kx = l <= N/2: C = FB conj(FA), inverse column FFT ------
     if (valid && l <= N / 2) {
-        const int mx = (N - l) % N;
+        const int mx = (N - l) % N, mxb = mx / N1;
         float cr[N1], ci[N1];
 #pragma unroll
         for (int k = 0; k < N1; ++k) {
             const int ky = k * R + q;
-            const int my = (N - ky) % N;
-            const float zR = Sr[ky * PX + l], zI = Si[ky * PX + l];
-            const float mR = Sr[my * PX + mx], mI = Si[my * PX + mx];
+            // (N - ky) mod N = (N1 - k) R - q; affine in k except at ky = 0
+            const int my = k > 0 ? N - ky : (q == 0 ? 0 : N - q);
+            const int myb = k > 0 ? mirror_block<N1, R>(N1 - k, q) : (q == 0 ? 0 : R - 1);
+            const int ez = tix<N>(ky, (k * R) / N1, l, lb), em = tix<N>(my, myb, mx, mxb);
+            const float zR = Sr[ez], zI = Si[ez];
+            const float mR = Sr[em], mI = Si[em];
             // FA = (Z + conj(Zm)) / 2 ; FB = (Z - conj(Zm)) / (2i)
             const float ar = 0.5f * (zR + mR), ai = 0.5f * (zI - mI);
             const float br = 0.5f * (zI + mI), bi = -0.5f * (zR - mR);
             cr[k] = br * ar + bi * ai;
             ci[k] = bi * ar - br * ai;
         }
-        fft_split<N, R, true>(cr, ci, q);
+        fft_split<N, R, true>(cr, ci, q, tw);
 #pragma unroll
         for (int k = 0; k < N1; ++k) {
-            Sr[(k + N1 * ob) * PX + l] = cr[k];
-            Si[(k + N1 * ob) * PX + l] = ci[k];
+            const int e = tix<N>(k + N1 * ob, ob, l, lb);
+            Sr[e] = cr[k];
+            Si[e] = ci[k];
         }
     }
     __syncthreads();
 
     // ---- 4. rows j and j + N/2 (j = l < N/2): Hermitian rows -> real rows -----
-    if (valid && l < N / 2) {
-        const int ya = l, yb = l + N / 2;
-        float xr[N1], xi[N1];
+    const bool act4 = valid && l < N / 2;
+    const int ya = l, yb = l + N / 2;
+    float xr[N1], xi[N1];
+    if (act4) {
 #pragma unroll
         for (int k = 0; k < N1; ++k) {
+            // kx = k R + q; with 0 <= q < R the tests below fold to compile
+            // time except at k = 0 and k = N1/2
             const int kx = k * R + q;
-            const bool up = kx > N / 2;
-            const int kk = up ? N - kx : kx;
-            float g1r = Sr[ya * PX + kk], g1i = Si[ya * PX + kk];
-            float g2r = Sr[yb * PX + kk], g2i = Si[yb * PX + kk];
-            if (kk == 0 || kk == N / 2) { g1i = 0.0f; g2i = 0.0f; }
+            const bool up = 2 * k > N1 || (2 * k == N1 && q > 0);
+            const int kk = 2 * k < N1 ? kx : N - kx;        // N - kx = N/2 at kx = N/2
+            const int kb = 2 * k < N1 ? (k * R) / N1 : mirror_block<N1, R>(N1 - k, q);
+            const int ea = tix<N>(ya, lb, kk, kb), eb = tix<N>(yb, lb + R / 2, kk, kb);
+            float g1r = Sr[ea], g1i = Si[ea];
+            float g2r = Sr[eb], g2i = Si[eb];
+            if ((k == 0 || 2 * k == N1) && q == 0) { g1i = 0.0f; g2i = 0.0f; }   // kx = 0, N/2
             if (up) { g1i = -g1i; g2i = -g2i; }     // G(N-k) = conj(G(k)) per real row
             xr[k] = g1r - g2i;
             xi[k] = g1i + g2r;
         }
-        fft_split<N, R, true>(xr, xi, q);
+    }
+    // R > 1: the plane (pitch PX) overlays tile words other lines still read
+    if constexpr (R > 1) __syncthreads();
+    if (act4) {
+        fft_split<N, R, true>(xr, xi, q, tw);
         const float mul = 1.0f / (float)numel;
         // plane[(y + N/2) % N][(x + N/2) % N] = c[y][x] / numel
 #pragma unroll
