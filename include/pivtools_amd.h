@@ -189,6 +189,24 @@ int pivamd_warp(const float *d_imgA, const float *d_imgB, float *d_outA, float *
                 int nPX, const float *d_ctrY, const float *d_ctrX, int interp_mode,
                 int shared_predictor, int round_shifts, void *stream);
 
+/* pivamd_xcorr and pivamd_warp over 16-bit camera counts (uint16 frames,
+ * converted exactly to float on load): the same results as the float entry
+ * points on the converted frames, without a float copy of the frames in
+ * HBM.  The reference takes float frames only (pivtools_cli/lib/
+ * PIV_2d_cross_correlate.h:18-21, fused_warp.h:55-93); its caller converts
+ * the camera frames to float first. */
+int pivamd_xcorr_u16(const unsigned short *d_imgA, const unsigned short *d_imgB, const float *d_mask,
+                     int H, int W, int n_images, const float *d_ctrX, const float *d_ctrY, int nWy,
+                     int nWx, const float *d_wA, const float *d_wB, const float *d_planeWeight,
+                     int ensemble, int winY, int winX, int nPeaks, int finder, int impl, float *d_pkX,
+                     float *d_pkY, float *d_pkH, float *d_sx, float *d_sy, float *d_sxy,
+                     float *d_ratio, float *d_planes, void *d_work, unsigned long work_bytes,
+                     void *stream);
+int pivamd_warp_u16(const unsigned short *d_imgA, const unsigned short *d_imgB, float *d_outA,
+                    float *d_outB, const float *d_predDy, const float *d_predDx, int N, int H, int W,
+                    int nPY, int nPX, const float *d_ctrY, const float *d_ctrX, int interp_mode,
+                    int shared_predictor, int round_shifts, void *stream);
+
 /* Ensemble sums over device buffers.  triple = 1: bulkxcorr2d_accumulate_triple
  * (AB, AA, BB with optional mean removal and per-pair normalisation; sumW*
  * are the weight sums of the mean removal); triple = 0:

@@ -23,6 +23,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
+_LIB_CR = None
 
 _f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 _i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
@@ -35,60 +36,75 @@ def build() -> None:
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
-def lib() -> ctypes.CDLL:
-    global _LIB
+def lib(libm: str = "host") -> ctypes.CDLL:
+    """The C restatement.  libm="host": expf/logf from the host libm, as the
+    reference's C; libm="cr": correctly rounded expf/logf (the device
+    fitter's definition, liboracle_cr.so)."""
+    global _LIB, _LIB_CR
+    if libm == "cr":
+        if _LIB_CR is None:
+            path = os.path.join(HERE, "liboracle_cr.so")
+            if not os.path.isfile(path):
+                build()
+            _LIB_CR = _bind(ctypes.CDLL(path))
+        return _LIB_CR
     if _LIB is None:
         path = os.path.join(HERE, "liboracle.so")
         if not os.path.isfile(path):
             build()
-        L = ctypes.CDLL(path)
-        L.orc_bulkxcorr2d.restype = ctypes.c_ubyte
-        L.orc_bulkxcorr2d.argtypes = [
-            _f32p, _f32p, _f32p, _i32p, ctypes.c_int, _f32p, _f32p, _i32p, _f32p,
-            ctypes.c_int, _f32p, _i32p, ctypes.c_int, ctypes.c_int,
-            _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, ctypes.c_void_p,
-        ]
-        L.orc_lsqpeaklocate_lm.restype = None
-        L.orc_lsqpeaklocate_lm.argtypes = [
-            _f32p, _i32p, _f32p, ctypes.c_int, ctypes.c_int, _f32p, ctypes.c_void_p,
-        ]
-        L.orc_peak_detectability.restype = ctypes.c_float
-        L.orc_peak_detectability.argtypes = [_f32p, _i32p]
-        L.orc_convolve.restype = ctypes.c_uint
-        L.orc_convolve.argtypes = [_f32p, _f32p, _f32p, _i32p]
-        L.orc_fused_symmetric_warp_batch.restype = ctypes.c_int
-        L.orc_fused_symmetric_warp_batch.argtypes = [
-            _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, ctypes.c_int, ctypes.c_int,
-            ctypes.c_int, ctypes.c_int, ctypes.c_int, _f32p, _f32p, ctypes.c_int,
-            ctypes.c_int, ctypes.c_int,
-        ]
-        L.orc_bulkxcorr2d_accumulate_triple.restype = ctypes.c_ubyte
-        L.orc_bulkxcorr2d_accumulate_triple.argtypes = [
-            _f32p, _f32p, _f32p, _i32p, ctypes.c_int, _f32p, _f32p, _i32p,
-            _f32p, _f32p, _f32p, _f32p, _i32p, ctypes.c_void_p, ctypes.c_int,
-            ctypes.c_int, _f32p, _f32p, _f32p,
-        ]
-        L.orc_bulkxcorr2d_accumulate.restype = ctypes.c_ubyte
-        L.orc_bulkxcorr2d_accumulate.argtypes = [
-            _f32p, _f32p, _f32p, _i32p, ctypes.c_int, _f32p, _f32p, _i32p,
-            _f32p, _f32p, _i32p, ctypes.c_void_p, _f32p,
-        ]
-        L.orc_max_threads.restype = ctypes.c_int
-        _f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
-        _u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
-        L.orc_kspace_lm_fit_batch.restype = ctypes.c_int
-        L.orc_kspace_lm_fit_batch.argtypes = [
-            _f64p, _f64p, _f64p, _u8p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-            ctypes.c_int, ctypes.c_int, ctypes.c_int, _f64p, _i32p, _f64p, _f64p, _f64p, _f64p,
-            _f64p, _f64p, _i32p, _i32p, _u8p,
-        ]
-        L.orc_kspace_fft2_centered.restype = ctypes.c_int
-        L.orc_kspace_fft2_centered.argtypes = [_f64p, ctypes.c_int, ctypes.c_int, _f64p, _f64p]
-        L.orc_kspace_debug_prep.restype = ctypes.c_int
-        L.orc_kspace_debug_prep.argtypes = [_f64p, _f64p, _f64p, ctypes.c_void_p, ctypes.c_int,
-                                            ctypes.c_int, _f64p, _f64p]
-        _LIB = L
+        _LIB = _bind(ctypes.CDLL(path))
     return _LIB
+
+
+def _bind(L: ctypes.CDLL) -> ctypes.CDLL:
+    L.orc_bulkxcorr2d.restype = ctypes.c_ubyte
+    L.orc_bulkxcorr2d.argtypes = [
+        _f32p, _f32p, _f32p, _i32p, ctypes.c_int, _f32p, _f32p, _i32p, _f32p,
+        ctypes.c_int, _f32p, _i32p, ctypes.c_int, ctypes.c_int,
+        _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, ctypes.c_void_p,
+    ]
+    L.orc_lsqpeaklocate_lm.restype = None
+    L.orc_lsqpeaklocate_lm.argtypes = [
+        _f32p, _i32p, _f32p, ctypes.c_int, ctypes.c_int, _f32p, ctypes.c_void_p,
+    ]
+    L.orc_peak_detectability.restype = ctypes.c_float
+    L.orc_peak_detectability.argtypes = [_f32p, _i32p]
+    L.orc_convolve.restype = ctypes.c_uint
+    L.orc_convolve.argtypes = [_f32p, _f32p, _f32p, _i32p]
+    L.orc_fused_symmetric_warp_batch.restype = ctypes.c_int
+    L.orc_fused_symmetric_warp_batch.argtypes = [
+        _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, ctypes.c_int, ctypes.c_int,
+        ctypes.c_int, ctypes.c_int, ctypes.c_int, _f32p, _f32p, ctypes.c_int,
+        ctypes.c_int, ctypes.c_int,
+    ]
+    L.orc_bulkxcorr2d_accumulate_triple.restype = ctypes.c_ubyte
+    L.orc_bulkxcorr2d_accumulate_triple.argtypes = [
+        _f32p, _f32p, _f32p, _i32p, ctypes.c_int, _f32p, _f32p, _i32p,
+        _f32p, _f32p, _f32p, _f32p, _i32p, ctypes.c_void_p, ctypes.c_int,
+        ctypes.c_int, _f32p, _f32p, _f32p,
+    ]
+    L.orc_bulkxcorr2d_accumulate.restype = ctypes.c_ubyte
+    L.orc_bulkxcorr2d_accumulate.argtypes = [
+        _f32p, _f32p, _f32p, _i32p, ctypes.c_int, _f32p, _f32p, _i32p,
+        _f32p, _f32p, _i32p, ctypes.c_void_p, _f32p,
+    ]
+    L.orc_max_threads.restype = ctypes.c_int
+    if not hasattr(L, "orc_kspace_lm_fit_batch"):     # liboracle_cr.so: peak path only
+        return L
+    _f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+    _u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+    L.orc_kspace_lm_fit_batch.restype = ctypes.c_int
+    L.orc_kspace_lm_fit_batch.argtypes = [
+        _f64p, _f64p, _f64p, _u8p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+        ctypes.c_int, ctypes.c_int, ctypes.c_int, _f64p, _i32p, _f64p, _f64p, _f64p, _f64p,
+        _f64p, _f64p, _i32p, _i32p, _u8p,
+    ]
+    L.orc_kspace_fft2_centered.restype = ctypes.c_int
+    L.orc_kspace_fft2_centered.argtypes = [_f64p, ctypes.c_int, ctypes.c_int, _f64p, _f64p]
+    L.orc_kspace_debug_prep.restype = ctypes.c_int
+    L.orc_kspace_debug_prep.argtypes = [_f64p, _f64p, _f64p, ctypes.c_void_p, ctypes.c_int,
+                                        ctypes.c_int, _f64p, _f64p]
+    return L
 
 
 # ---------------------------------------------------------------------------
@@ -123,14 +139,14 @@ def bulkxcorr2d(img_a, img_b, mask, ctrs_x, ctrs_y, win, weight_a, weight_b,
     return o
 
 
-def lsqpeaklocate_lm(plane, n_peaks=1, fit_type=3, weight=None):
+def lsqpeaklocate_lm(plane, n_peaks=1, fit_type=3, weight=None, libm="host"):
     plane = np.ascontiguousarray(plane, np.float32)
     N = np.array(plane.shape, np.int32)
     loc = np.zeros(3 * n_peaks, np.float32)
     std = np.zeros(3 * n_peaks, np.float32)
     w = None if weight is None else np.ascontiguousarray(weight, np.float32)
-    lib().orc_lsqpeaklocate_lm(plane, N, loc, n_peaks, fit_type, std,
-                               None if w is None else w.ctypes.data)
+    lib(libm).orc_lsqpeaklocate_lm(plane, N, loc, n_peaks, fit_type, std,
+                                   None if w is None else w.ctypes.data)
     return loc, std
 
 

@@ -38,6 +38,19 @@
 #include <float.h>
 #include <math.h>
 #include <stdint.h>
+
+/* ORC_CR_LIBM (liboracle_cr.so): expf and logf correctly rounded, i.e.
+ * evaluated in double and rounded once -- the definition the device fitter
+ * uses.  Without it (liboracle.so) the host libm's expf/logf are called, as
+ * the reference's C does; glibc's are not correctly rounded for every input,
+ * so the two flavours can differ in the last bit of an exponential, and an
+ * LM trust decision sitting on its threshold can flip between them. */
+#ifdef ORC_CR_LIBM
+static inline float orc_expf_cr(float x) { return (float)exp((double)x); }
+static inline float orc_logf_cr(float x) { return (float)log((double)x); }
+#define expf orc_expf_cr
+#define logf orc_logf_cr
+#endif
 #include <stdlib.h>
 #include <string.h>
 #ifdef _OPENMP

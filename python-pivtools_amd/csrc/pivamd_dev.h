@@ -31,6 +31,16 @@ int pivamd_warp(const float *d_imgA, const float *d_imgB, float *d_outA, float *
                 const float *d_predDy, const float *d_predDx, int N, int H, int W, int nPY, int nPX,
                 const float *d_ctrY, const float *d_ctrX, int interp_mode, int shared_predictor,
                 int round_shifts, hipStream_t st);
+int pivamd_xcorr_u16(const unsigned short *d_imgA, const unsigned short *d_imgB, const float *d_mask,
+                     int H, int W, int n_images, const float *d_ctrX, const float *d_ctrY, int nWy,
+                     int nWx, const float *d_wA, const float *d_wB, const float *d_planeWeight,
+                     int ensemble, int winY, int winX, int nPeaks, int finder, int impl, float *d_pkX,
+                     float *d_pkY, float *d_pkH, float *d_sx, float *d_sy, float *d_sxy,
+                     float *d_ratio, float *d_planes, void *d_work, size_t work_bytes, hipStream_t st);
+int pivamd_warp_u16(const unsigned short *d_imgA, const unsigned short *d_imgB, float *d_outA,
+                    float *d_outB, const float *d_predDy, const float *d_predDx, int N, int H, int W,
+                    int nPY, int nPX, const float *d_ctrY, const float *d_ctrX, int interp_mode,
+                    int shared_predictor, int round_shifts, hipStream_t st);
 size_t pivamd_ensemble_workspace(int nWy, int nWx, int winY, int winX);
 int pivamd_accumulate(const float *d_A, const float *d_B, const float *d_mask, int H, int W, int n_images,
                       const float *d_ctrX, const float *d_ctrY, int nWy, int nWx, const float *d_wAab,

@@ -70,7 +70,8 @@ __device__ __forceinline__ float clamp_pos(float v, int size)
     return fminf(fmaxf(v, -16.0f), (float)size + 16.0f);
 }
 
-__device__ __forceinline__ float sample_bicubic(const float *img, float fy, float fx, int H, int W)
+template <typename TI>
+__device__ __forceinline__ float sample_bicubic(const TI *img, float fy, float fx, int H, int W)
 {
     fy = clamp_pos(fy, H);
     fx = clamp_pos(fx, W);
@@ -81,11 +82,11 @@ __device__ __forceinline__ float sample_bicubic(const float *img, float fy, floa
     keys4(fx - fxf, wx);
     float val = 0.0f;
     if (iy >= 0 && iy + 3 < H && ix >= 0 && ix + 3 < W) {
-        const float *p = img + (size_t)iy * W + ix;
+        const TI *p = img + (size_t)iy * W + ix;
 #pragma unroll
         for (int m = 0; m < 4; m++) {
 #pragma unroll
-            for (int n = 0; n < 4; n++) val += wy[m] * wx[n] * p[n];
+            for (int n = 0; n < 4; n++) val += wy[m] * wx[n] * (float)p[n];
             p += W;
         }
         return val;
@@ -98,7 +99,7 @@ __device__ __forceinline__ float sample_bicubic(const float *img, float fy, floa
         for (int n = 0; n < 4; n++) {
             const int col = ix + n;
             if (col < 0 || col >= W) continue;
-            val += wy[m] * wx[n] * img[(size_t)row * W + col];
+            val += wy[m] * wx[n] * (float)img[(size_t)row * W + col];
         }
     }
     return val;
@@ -117,7 +118,8 @@ __device__ __forceinline__ void lanczos_w(float d, const float *lut, float w[6])
     }
 }
 
-__device__ __forceinline__ float sample_lanczos(const float *img, float fy, float fx, int H, int W,
+template <typename TI>
+__device__ __forceinline__ float sample_lanczos(const TI *img, float fy, float fx, int H, int W,
                                                 const float *lut)
 {
     fy = clamp_pos(fy, H);
@@ -129,11 +131,11 @@ __device__ __forceinline__ float sample_lanczos(const float *img, float fy, floa
     lanczos_w(fx - fxf, lut, wx);
     float val = 0.0f;
     if (iy >= 0 && iy + 5 < H && ix >= 0 && ix + 5 < W) {
-        const float *p = img + (size_t)iy * W + ix;
+        const TI *p = img + (size_t)iy * W + ix;
 #pragma unroll
         for (int m = 0; m < 6; m++) {
 #pragma unroll
-            for (int n = 0; n < 6; n++) val += wy[m] * wx[n] * p[n];
+            for (int n = 0; n < 6; n++) val += wy[m] * wx[n] * (float)p[n];
             p += W;
         }
         return val;
@@ -146,7 +148,7 @@ __device__ __forceinline__ float sample_lanczos(const float *img, float fy, floa
         for (int n = 0; n < 6; n++) {
             const int col = ix + n;
             if (col < 0 || col >= W) continue;
-            val += wy[m] * wx[n] * img[(size_t)row * W + col];
+            val += wy[m] * wx[n] * (float)img[(size_t)row * W + col];
         }
     }
     return val;
@@ -154,12 +156,22 @@ __device__ __forceinline__ float sample_lanczos(const float *img, float fy, floa
 
 struct WarpArgs {
     const float *imgA, *imgB, *pdy, *pdx, *cy, *cx, *lut;
+    const unsigned short *imgA16, *imgB16;   // 16-bit counts instead of imgA/imgB
     float *outA, *outB;
     int H, W, nPY, nPX, predStride, mode, roundShifts;
 };
 
 constexpr int kPredMax = 1024;   // predictor cells a tile may stage in LDS
 
+template <typename TI>
+__device__ __forceinline__ const TI *warp_src(const WarpArgs &g, bool a)
+{
+    if constexpr (sizeof(TI) == 2) return a ? g.imgA16 : g.imgB16;
+    else return a ? g.imgA : g.imgB;
+}
+
+// TI: float frames, or 16-bit counts (converted exactly on load).
+template <typename TI>
 __global__ void __launch_bounds__(256) warp_kernel(WarpArgs g)
 {
     __shared__ float s_ly[kTileY];
@@ -215,8 +227,8 @@ __global__ void __launch_bounds__(256) warp_kernel(WarpArgs g)
     const int j = x0 + tx;
     if (j >= g.W) return;
     const size_t plane = (size_t)g.H * g.W;
-    const float *A = g.imgA + (size_t)img * plane;
-    const float *B = g.imgB + (size_t)img * plane;
+    const TI *A = warp_src<TI>(g, true) + (size_t)img * plane;
+    const TI *B = warp_src<TI>(g, false) + (size_t)img * plane;
 
     const float fx = s_lx[tx], fxf = floorf(fx);
     const int ix = (int)fxf - 1;
@@ -312,17 +324,20 @@ static const float *lanczos_lut_device()
 
 using namespace pivamd;
 
-extern "C" int pivamd_warp(const float *d_imgA, const float *d_imgB, float *d_outA, float *d_outB,
-                           const float *d_predDy, const float *d_predDx, int N, int H, int W,
-                           int nPY, int nPX, const float *d_ctrY, const float *d_ctrX,
-                           int interp_mode, int shared_predictor, int round_shifts, hipStream_t st)
+static int warp_entry(const float *d_imgA, const float *d_imgB, const unsigned short *d_imgA16,
+                      const unsigned short *d_imgB16, float *d_outA, float *d_outB, const float *d_predDy,
+                      const float *d_predDx, int N, int H, int W, int nPY, int nPX, const float *d_ctrY,
+                      const float *d_ctrX, int interp_mode, int shared_predictor, int round_shifts,
+                      hipStream_t st)
 {
     if (N <= 0 || H <= 0 || W <= 0 || nPY <= 0 || nPX <= 0) return PIVAMD_ERR_ARG;
-    if (!d_imgA || !d_imgB || !d_outA || !d_outB || !d_predDy || !d_predDx || !d_ctrY || !d_ctrX)
+    if (!(d_imgA16 ? d_imgB16 != nullptr : (d_imgA && d_imgB)) || !d_outA || !d_outB || !d_predDy ||
+        !d_predDx || !d_ctrY || !d_ctrX)
         return PIVAMD_ERR_ARG;
     if (N > 65535) return PIVAMD_ERR_ARG;
     WarpArgs g;
-    g.imgA = d_imgA; g.imgB = d_imgB; g.pdy = d_predDy; g.pdx = d_predDx;
+    g.imgA = d_imgA; g.imgB = d_imgB; g.imgA16 = d_imgA16; g.imgB16 = d_imgB16;
+    g.pdy = d_predDy; g.pdx = d_predDx;
     g.cy = d_ctrY; g.cx = d_ctrX; g.outA = d_outA; g.outB = d_outB;
     g.H = H; g.W = W; g.nPY = nPY; g.nPX = nPX;
     g.predStride = shared_predictor ? 0 : nPY * nPX;
@@ -334,6 +349,26 @@ extern "C" int pivamd_warp(const float *d_imgA, const float *d_imgB, float *d_ou
         if (!g.lut) return PIVAMD_ERR_NOMEM;
     }
     dim3 grid((W + kTileX - 1) / kTileX, (H + kTileY - 1) / kTileY, N);
-    hipLaunchKernelGGL(warp_kernel, grid, dim3(256), 0, st, g);
+    if (d_imgA16) hipLaunchKernelGGL(warp_kernel<unsigned short>, grid, dim3(256), 0, st, g);
+    else hipLaunchKernelGGL(warp_kernel<float>, grid, dim3(256), 0, st, g);
     return hipGetLastError() == hipSuccess ? 0 : PIVAMD_ERR_LAUNCH;
+}
+
+extern "C" int pivamd_warp(const float *d_imgA, const float *d_imgB, float *d_outA, float *d_outB,
+                           const float *d_predDy, const float *d_predDx, int N, int H, int W,
+                           int nPY, int nPX, const float *d_ctrY, const float *d_ctrX,
+                           int interp_mode, int shared_predictor, int round_shifts, hipStream_t st)
+{
+    return warp_entry(d_imgA, d_imgB, nullptr, nullptr, d_outA, d_outB, d_predDy, d_predDx, N, H, W, nPY,
+                      nPX, d_ctrY, d_ctrX, interp_mode, shared_predictor, round_shifts, st);
+}
+
+extern "C" int pivamd_warp_u16(const unsigned short *d_imgA, const unsigned short *d_imgB, float *d_outA,
+                               float *d_outB, const float *d_predDy, const float *d_predDx, int N, int H,
+                               int W, int nPY, int nPX, const float *d_ctrY, const float *d_ctrX,
+                               int interp_mode, int shared_predictor, int round_shifts, hipStream_t st)
+{
+    if (!d_imgA || !d_imgB) return PIVAMD_ERR_ARG;
+    return warp_entry(nullptr, nullptr, d_imgA, d_imgB, d_outA, d_outB, d_predDy, d_predDx, N, H, W, nPY,
+                      nPX, d_ctrY, d_ctrX, interp_mode, shared_predictor, round_shifts, st);
 }

@@ -38,6 +38,7 @@ namespace pivamd {
 
 struct XcorrArgs {
     const float *imgA, *imgB, *mask, *ctrX, *ctrY, *wA, *wB, *pw;
+    const unsigned short *imgA16, *imgB16;   // 16-bit counts instead of imgA/imgB when set
     float *pkX, *pkY, *pkH, *sx, *sy, *sxy, *ratio, *planes;
     float *rec, *work_planes;   // workspace: per-window peak records, multi-peak planes
     long long total;        // N_images * nWin
@@ -636,19 +637,22 @@ xcorr_kernel(XcorrArgs g)
     float zr[N1], zi[N1];
     float sa = 0.0f, sb = 0.0f;
     if (valid) {
-        const size_t img = (size_t)n * g.H * g.W;
-        const float *pa = g.imgA + img + (size_t)(row0 + q) * g.W + col0 + l;
-        const float *pb = g.imgB + img + (size_t)(row0 + q) * g.W + col0 + l;
+        const size_t img = (size_t)n * g.H * g.W + (size_t)(row0 + q) * g.W + col0 + l;
         const float *wa = g.wA + q * N + l, *wb = g.wB + q * N + l;
+        // 16-bit counts convert exactly, so both inputs give the same bits
+        auto load = [&](const auto *pa, const auto *pb) {
 #pragma unroll
-        for (int k = 0; k < N1; ++k) {
-            const float a = pa[(size_t)k * R * g.W] * wa[k * R * N];
-            const float bb = pb[(size_t)k * R * g.W] * wb[k * R * N];
-            zr[k] = a;
-            zi[k] = bb;
-            sa += a;
-            sb += bb;
-        }
+            for (int k = 0; k < N1; ++k) {
+                const float a = (float)pa[(size_t)k * R * g.W] * wa[k * R * N];
+                const float bb = (float)pb[(size_t)k * R * g.W] * wb[k * R * N];
+                zr[k] = a;
+                zi[k] = bb;
+                sa += a;
+                sb += bb;
+            }
+        };
+        if (g.imgA16) load(g.imgA16 + img, g.imgB16 + img);
+        else load(g.imgA + img, g.imgB + img);
     } else {
 #pragma unroll
         for (int k = 0; k < N1; ++k) { zr[k] = 0.0f; zi[k] = 0.0f; }
@@ -1102,15 +1106,19 @@ __global__ void __launch_bounds__((MfCfg<N>::BLOCK)) xcorr_mfma_kernel(XcorrArgs
     float sa = 0.0f, sb = 0.0f;
     if (valid) {
         const size_t img = (size_t)n * g.H * g.W;
+        auto load = [&](const auto *A, const auto *B) {
 #pragma unroll
-        for (int m = 0; m < EPL; ++m) {
-            const int e = t + T * m, y = e / N, x = e - (e / N) * N;
-            const size_t o = img + (size_t)(row0 + y) * g.W + col0 + x;
-            za[m] = g.imgA[o] * g.wA[e];
-            zb[m] = g.imgB[o] * g.wB[e];
-            sa += za[m];
-            sb += zb[m];
-        }
+            for (int m = 0; m < EPL; ++m) {
+                const int e = t + T * m, y = e / N, x = e - (e / N) * N;
+                const size_t o = img + (size_t)(row0 + y) * g.W + col0 + x;
+                za[m] = (float)A[o] * g.wA[e];
+                zb[m] = (float)B[o] * g.wB[e];
+                sa += za[m];
+                sb += zb[m];
+            }
+        };
+        if (g.imgA16) load(g.imgA16, g.imgB16);
+        else load(g.imgA, g.imgB);
     } else {
 #pragma unroll
         for (int m = 0; m < EPL; ++m) { za[m] = 0.0f; zb[m] = 0.0f; }
@@ -1365,18 +1373,21 @@ extern "C" int pivamd_set_dft_mode(int mode)
     return prev;
 }
 
-extern "C" int pivamd_xcorr(const float *d_imgA, const float *d_imgB, const float *d_mask, int H, int W,
-                            int n_images, const float *d_ctrX, const float *d_ctrY, int nWy, int nWx,
-                            const float *d_wA, const float *d_wB, const float *d_planeWeight, int ensemble,
-                            int winY, int winX, int nPeaks, int finder, int impl, float *d_pkX,
-                            float *d_pkY, float *d_pkH, float *d_sx, float *d_sy, float *d_sxy,
-                            float *d_ratio, float *d_planes, void *d_work, size_t work_bytes, hipStream_t st)
+static int xcorr_entry(const float *d_imgA, const float *d_imgB, const unsigned short *d_imgA16,
+                       const unsigned short *d_imgB16, const float *d_mask, int H, int W,
+                       int n_images, const float *d_ctrX, const float *d_ctrY, int nWy, int nWx,
+                       const float *d_wA, const float *d_wB, const float *d_planeWeight, int ensemble,
+                       int winY, int winX, int nPeaks, int finder, int impl, float *d_pkX,
+                       float *d_pkY, float *d_pkH, float *d_sx, float *d_sy, float *d_sxy,
+                       float *d_ratio, float *d_planes, void *d_work, size_t work_bytes, hipStream_t st)
 {
     if (n_images < 0 || nWy < 0 || nWx < 0 || nPeaks < 1 || H <= 0 || W <= 0) return PIVAMD_ERR_ARG;
     if (work_bytes < pivamd_xcorr_workspace(n_images, nWy, nWx, winY, winX, nPeaks, ensemble))
         return PIVAMD_ERR_ARG;
+    if (n_images > 0 && (d_imgA16 ? !d_imgB16 : (!d_imgA || !d_imgB))) return PIVAMD_ERR_ARG;
     XcorrArgs g;
-    g.imgA = d_imgA; g.imgB = d_imgB; g.mask = d_mask; g.ctrX = d_ctrX; g.ctrY = d_ctrY;
+    g.imgA = d_imgA; g.imgB = d_imgB; g.imgA16 = d_imgA16; g.imgB16 = d_imgB16;
+    g.mask = d_mask; g.ctrX = d_ctrX; g.ctrY = d_ctrY;
     g.wA = d_wA; g.wB = d_wB; g.pw = d_planeWeight;
     g.pkX = d_pkX; g.pkY = d_pkY; g.pkH = d_pkH; g.sx = d_sx; g.sy = d_sy; g.sxy = d_sxy;
     g.ratio = d_ratio; g.planes = d_planes;
@@ -1388,8 +1399,33 @@ extern "C" int pivamd_xcorr(const float *d_imgA, const float *d_imgB, const floa
     g.dft = (g_dft_mode == 1 && winY == winX && winY % 16 == 0 && winY <= 64) ? 1 : 0;
     if (!ensemble && (!d_pkX || !d_pkY || !d_pkH || !d_sx || !d_sy || !d_sxy || !d_ratio || !d_work))
         return PIVAMD_ERR_ARG;
-if (winY != winX || !pivamd_size_supported(winY, winX)) return PIVAMD_ERR_SIZE;
+    if (winY != winX || !pivamd_size_supported(winY, winX)) return PIVAMD_ERR_SIZE;
     return launch_xcorr(winY, g, st);
+}
+
+extern "C" int pivamd_xcorr(const float *d_imgA, const float *d_imgB, const float *d_mask, int H, int W,
+                            int n_images, const float *d_ctrX, const float *d_ctrY, int nWy, int nWx,
+                            const float *d_wA, const float *d_wB, const float *d_planeWeight, int ensemble,
+                            int winY, int winX, int nPeaks, int finder, int impl, float *d_pkX,
+                            float *d_pkY, float *d_pkH, float *d_sx, float *d_sy, float *d_sxy,
+                            float *d_ratio, float *d_planes, void *d_work, size_t work_bytes, hipStream_t st)
+{
+    return xcorr_entry(d_imgA, d_imgB, nullptr, nullptr, d_mask, H, W, n_images, d_ctrX, d_ctrY, nWy, nWx,
+                       d_wA, d_wB, d_planeWeight, ensemble, winY, winX, nPeaks, finder, impl, d_pkX, d_pkY,
+                       d_pkH, d_sx, d_sy, d_sxy, d_ratio, d_planes, d_work, work_bytes, st);
+}
+
+extern "C" int pivamd_xcorr_u16(const unsigned short *d_imgA, const unsigned short *d_imgB, const float *d_mask,
+                                int H, int W, int n_images, const float *d_ctrX, const float *d_ctrY, int nWy,
+                                int nWx, const float *d_wA, const float *d_wB, const float *d_planeWeight,
+                                int ensemble, int winY, int winX, int nPeaks, int finder, int impl, float *d_pkX,
+                                float *d_pkY, float *d_pkH, float *d_sx, float *d_sy, float *d_sxy,
+                                float *d_ratio, float *d_planes, void *d_work, size_t work_bytes, hipStream_t st)
+{
+    if (n_images > 0 && (!d_imgA || !d_imgB)) return PIVAMD_ERR_ARG;
+    return xcorr_entry(nullptr, nullptr, d_imgA, d_imgB, d_mask, H, W, n_images, d_ctrX, d_ctrY, nWy, nWx,
+                       d_wA, d_wB, d_planeWeight, ensemble, winY, winX, nPeaks, finder, impl, d_pkX, d_pkY,
+                       d_pkH, d_sx, d_sy, d_sxy, d_ratio, d_planes, d_work, work_bytes, st);
 }
 
 extern "C" int pivamd_peakfit(const float *d_planes, int n, int ny, int nx, int nPeaks, int finder,

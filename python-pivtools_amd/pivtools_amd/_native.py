@@ -60,8 +60,12 @@ def core() -> ctypes.CDLL:
             _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_size_t, _vp,
         ]
         L.pivamd_xcorr.restype = _i
+        L.pivamd_xcorr_u16.argtypes = L.pivamd_xcorr.argtypes
+        L.pivamd_xcorr_u16.restype = _i
         L.pivamd_warp.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp, _vp, _i, _i, _i, _vp]
         L.pivamd_warp.restype = _i
+        L.pivamd_warp_u16.argtypes = L.pivamd_warp.argtypes
+        L.pivamd_warp_u16.restype = _i
         L.pivamd_peakfit.argtypes = [_vp, _i, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp]
         L.pivamd_peakfit.restype = _i
         for name, argt in GLUE_SIGNATURES.items():

@@ -153,11 +153,10 @@ class MultipassPIV:
                     wts, rad = gaussian_weights_1d(g.sd, trunc)
                     b.smooth = (torch.as_tensor(wts, device=dev), rad)
             bufs.append(b)
-        self.imgf = torch.empty((2, n, self.H, self.W), dtype=f32, device=dev)
         self.warped = torch.empty((2, n, self.H, self.W), dtype=f32, device=dev)
         self._bufs = bufs
         self._n = n
-        self._sets[n] = (bufs, self.imgf, self.warped)
+        self._sets[n] = (bufs, self.warped)
 
     # -- one batch ---------------------------------------------------------------
     def run(self, img_a: torch.Tensor, img_b: torch.Tensor, keep_passes: bool = False, events=None):
@@ -174,21 +173,23 @@ class MultipassPIV:
         n = int(img_a.shape[0])
         if self._bufs is None or self._n != n:
             if n in self._sets:
-                self._bufs, self.imgf, self.warped = self._sets[n]
+                self._bufs, self.warped = self._sets[n]
                 self._n = n
             else:
                 self._alloc(n)
         st = self._stream()
         L = self.lib
-        npix = n * self.H * self.W
+        if img_a.dtype != img_b.dtype:
+            raise TypeError("the two frames of a pair must have the same dtype")
         if img_a.dtype == torch.float32:
-            A, B = img_a.contiguous(), img_b.contiguous()
+            u16 = False
         elif img_a.dtype in (torch.int16, torch.uint16):
-            _native.check(L.pivamd_u16_to_f32(_ptr(img_a.contiguous()), _ptr(self.imgf[0]), npix, st), "ingest")
-            _native.check(L.pivamd_u16_to_f32(_ptr(img_b.contiguous()), _ptr(self.imgf[1]), npix, st), "ingest")
-            A, B = self.imgf[0], self.imgf[1]
+            u16 = True      # the kernels convert the 16-bit counts on load (exact)
         else:
             raise TypeError(f"unsupported image dtype {img_a.dtype}")
+        A, B = img_a.contiguous(), img_b.contiguous()
+        xcorr_fn = L.pivamd_xcorr_u16 if u16 else L.pivamd_xcorr
+        warp_fn = L.pivamd_warp_u16 if u16 else L.pivamd_warp
 
         results = []
         prev = None
@@ -213,7 +214,7 @@ class MultipassPIV:
                 pred = o["pred"]
                 _native.check(L.pivamd_remap_cubic(_ptr(field), n, nyp, nxp, _ptr(b.maps[0]), _ptr(b.maps[1]),
                                                    ny, nx, _ptr(pred), st), "predictor remap")
-                _native.check(L.pivamd_warp(_ptr(A), _ptr(B), _ptr(self.warped[0]), _ptr(self.warped[1]),
+                _native.check(warp_fn(_ptr(A), _ptr(B), _ptr(self.warped[0]), _ptr(self.warped[1]),
                                             _ptr(field[0]), _ptr(field[1]), n, self.H, self.W, nyp, nxp,
                                             _ptr(prev.ctrs["ya"]), _ptr(prev.ctrs["xa"]), self.interp, 0, 0, st),
                               "warp")
@@ -222,7 +223,8 @@ class MultipassPIV:
                 ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
                 ev[0].record()
             prev_mode = L.pivamd_set_dft_mode(self.dft)
-            rc = L.pivamd_xcorr(_ptr(a_in), _ptr(b_in), None, self.H, self.W, n, _ptr(b.ctrs["x"]),
+            fn = xcorr_fn if p == 0 else L.pivamd_xcorr
+            rc = fn(_ptr(a_in), _ptr(b_in), None, self.H, self.W, n, _ptr(b.ctrs["x"]),
                                          _ptr(b.ctrs["y"]), ny, nx, _ptr(b.w), _ptr(b.w), _ptr(b.pw), 0,
                                          g.win[0], g.win[1], 1, self.finder, self.impl, _ptr(o["pkx"]),
                                          _ptr(o["pky"]), _ptr(o["pkh"]), _ptr(o["sx"]), _ptr(o["sy"]),

@@ -134,7 +134,7 @@ class PIVServer:
             return None
         H, W = spec.shape
         stage = 2 * self.max_batch * H * W * 2
-        est = stage + 2 * (2 * self.max_batch * H * W * 4) * 2   # staging + float frames + warped (+ fields)
+        est = stage + (2 * self.max_batch * H * W * 4) * 2   # staging + warped frames (+ fields, pass buffers)
         if self.bytes_in_use + est > self.budget and self.slots[i]:
             return None
         st = torch.cuda.Stream(device=self.dev)

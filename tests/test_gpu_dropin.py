@@ -69,12 +69,14 @@ def test_bulkxcorr2d_gauss3_all_sizes(n):
     _check_outputs(got, ref)
 
 
-def _refit_on_planes(planes, w, finder, n_peaks=1):
+def _refit_on_planes(planes, w, finder, n_peaks=1, libm="cr"):
     """Oracle fit of the device's own planes: separates the fitter from the
-    last-ulp FFT differences that feed it."""
+    last-ulp FFT differences that feed it.  libm="cr" is the oracle with
+    correctly rounded expf/logf, the device fitter's definition (exact
+    agreement expected); libm="host" is the reference's host-libm flavour."""
     pw = O.plane_weight(w)
     flat = planes.reshape(-1, planes.shape[-2], planes.shape[-1])
-    loc = np.array([O.lsqpeaklocate_lm(p, n_peaks, finder, pw)[0] for p in flat])
+    loc = np.array([O.lsqpeaklocate_lm(p, n_peaks, finder, pw, libm=libm)[0] for p in flat])
     return loc
 
 
@@ -108,6 +110,13 @@ def test_bulkxcorr2d_lm_fits(finder, impl, n):
         assert np.array_equal(np.isnan(gx), np.isnan(rx))
         ok = ~np.isnan(rx)
         assert np.abs(gx[ok] - rx[ok]).max(initial=0) < 1e-4
+        # host libm: its expf/logf are not correctly rounded for every input,
+        # so a window whose trust test sits on the threshold may flip (glibc
+        # vs correctly rounded differ in 2 of the 768 reference fixture fits)
+        hx = _refit_on_planes(got["planes"], w, finder, libm="host")[:, 1] - n / 2.0
+        assert np.sum(np.isnan(gx) != np.isnan(hx)) <= 1
+        both = ~np.isnan(gx) & ~np.isnan(hx)
+        assert np.median(np.abs(gx[both] - hx[both])) < 1e-6
     ex = ref["pk_x"].ravel()
     both = ~np.isnan(ex) & ~np.isnan(gx)
     tol = 0.005 if impl == 0 else 1e-3 * 5

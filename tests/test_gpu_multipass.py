@@ -127,6 +127,60 @@ def test_u16_ingest():
     np.testing.assert_array_equal(o.cpu().numpy(), x.astype(np.float32))
 
 
+def test_u16_frames_give_the_float_results():
+    """16-bit frames go straight into the first correlation and the warps
+    (converted on load); every pass must equal the run on the float frames,
+    bit for bit, including counts above 32767 (negative as int16)."""
+    shape = (160, 224)
+    sched = [(64, 64), (32, 32), (16, 16)]
+    A, B = [], []
+    for k in range(2):
+        a, b = S.multipass_pair(seed=700 + k, shape=shape)
+        A.append(a)
+        B.append(b)
+    A16 = torch.stack(A).to(torch.int32)
+    B16 = torch.stack(B).to(torch.int32)
+    A16[0, 5:9, 7:40] += 40000        # saturated streak above the int16 range
+    B16[1, 90:95, 100:140] += 50000
+    A16 = A16.clamp(0, 65535)
+    B16 = B16.clamp(0, 65535)
+    Af, Bf = A16.float().to(dev), B16.float().to(dev)
+    # uint16 data in an int16 tensor (two's complement for counts >= 32768)
+    Ai = torch.where(A16 >= 32768, A16 - 65536, A16).to(torch.int16).to(dev)
+    Bi = torch.where(B16 >= 32768, B16 - 65536, B16).to(torch.int16).to(dev)
+    mp = MultipassPIV(shape, sched, [50, 50, 50])
+    rf = mp.run(Af, Bf, keep_passes=True)
+    rf = [(r.ux.clone(), r.uy.clone(), r.nan_mask.clone(), r.peak_mag.clone()) for r in rf]
+    ri = mp.run(Ai, Bi, keep_passes=True)
+    torch.cuda.synchronize()
+    for p, (f, i) in enumerate(zip(rf, ri)):
+        for a, b in zip(f, (i.ux, i.uy, i.nan_mask, i.peak_mag)):
+            assert torch.equal(torch.nan_to_num(a, 1e30), torch.nan_to_num(b, 1e30)), p
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_warp_u16_equals_float_warp(mode):
+    L = _lib()
+    rng = np.random.default_rng(11)
+    n, H, W, nPY, nPX = 2, 70, 90, 5, 6
+    img = rng.integers(0, 65536, (2, n, H, W))
+    f = _t(img.astype(np.float32))
+    i16 = _t(np.where(img >= 32768, img - 65536, img).astype(np.int16), torch.int16)
+    pdy = _t(rng.normal(0, 2, (n, nPY, nPX)).astype(np.float32))
+    pdx = _t(rng.normal(0, 2, (n, nPY, nPX)).astype(np.float32))
+    cy = _t(np.linspace(-10, H + 10, nPY).astype(np.float32))
+    cx = _t(np.linspace(-10, W + 10, nPX).astype(np.float32))
+    outs = []
+    for fn, src in ((L.pivamd_warp, f), (L.pivamd_warp_u16, i16)):
+        oa = torch.empty((n, H, W), device=dev)
+        ob = torch.empty((n, H, W), device=dev)
+        assert fn(src[0].data_ptr(), src[1].data_ptr(), oa.data_ptr(), ob.data_ptr(), pdy.data_ptr(),
+                  pdx.data_ptr(), n, H, W, nPY, nPX, cy.data_ptr(), cx.data_ptr(), mode, 0, 0, _st()) == 0
+        outs.append((oa, ob))
+    torch.cuda.synchronize()
+    assert torch.equal(outs[0][0], outs[1][0]) and torch.equal(outs[0][1], outs[1][1])
+
+
 @pytest.mark.parametrize("finder", [3, 6])
 def test_multipass_loop_matches_oracle_loop(finder):
     shape = (192, 256)

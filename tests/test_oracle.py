@@ -44,6 +44,25 @@ def test_oracle_peakfit_matches_reference_c_bitwise(peak_gold, n, ft):
         assert np.array_equal(_bits(std), _bits(ref_std[k])), (k, std, ref_std[k])
 
 
+def test_cr_libm_flavour_against_reference_c(peak_gold):
+    """The oracle with correctly rounded expf/logf (the device fitter's
+    definition) on the reference C fixtures: identical NaN masks, positions
+    within 1e-5 px, and bitwise equal except where the host libm rounds an
+    exponential or logarithm differently (2 of the 768 fits)."""
+    n_diff = 0
+    for n in (16, 32, 64):
+        planes, w = peak_gold[f"n{n}_planes"], peak_gold[f"n{n}_weight"]
+        for ft in (3, 4, 5, 6):
+            ref_loc = peak_gold[f"n{n}_ft{ft}_loc"]
+            for k, pl in enumerate(planes):
+                loc, _ = O.lsqpeaklocate_lm(pl, 1, ft, w, libm="cr")
+                assert np.array_equal(np.isnan(loc), np.isnan(ref_loc[k]))
+                ok = ~np.isnan(loc)
+                assert np.all(np.abs(loc[ok] - ref_loc[k][ok]) < 1e-5)
+                n_diff += not np.array_equal(_bits(loc), _bits(ref_loc[k]))
+    assert n_diff <= 4
+
+
 @pytest.mark.parametrize("n", [16, 32, 64])
 def test_oracle_detectability_matches_reference_c(peak_gold, n):
     planes = peak_gold[f"n{n}_planes"]
