@@ -14,10 +14,14 @@
 //
 // Mapping: a 256-lane workgroup covers a 64-column x 16-row tile of one
 // image (grid.z = image).  Each lane owns one column and four rows.  The
-// tile's predictor indices and the predictor cells its stencils touch are
-// staged in LDS once; image taps take one bounds test per stencil (interior)
-// or one per tap (border), so a non-finite predictor can never produce an
-// out-of-range read.
+// tile's predictor indices, the predictor cells its stencils touch and the
+// predictor y weights and row offsets of each tile row are staged in LDS
+// once; image taps take one bounds test per stencil (interior) or one per
+// tap (border), and the half-shifts are clamped so that a non-finite
+// predictor can never produce an out-of-range read.  The image taps are read
+// straight from global memory (wide unaligned loads): per pixel the kernel
+// issues about 400 VALU instructions against about 8 loads, so it is VALU
+// bound, and staging the source window in LDS measured slower.
 #include <hip/hip_runtime.h>
 
 #include "pivamd_dev.h"
@@ -56,33 +60,31 @@ __device__ float pred_index(float coord, const float *ctrs, int nC)
     const float den = ctrs[seg + 1] - ctrs[seg];
     if (den < 1e-12f) return (float)seg;
     const float t = (coord - ctrs[seg]) / den;
-    return (float)seg + t;
+    // in [0, nC - 1] already for finite ascending centres; the clamp maps
+    // anything else (NaN centres) into range so later indexing stays safe
+    return fminf(fmaxf((float)seg + t, 0.0f), (float)(nC - 1));
 }
 
 // Bicubic tap sum (Keys a=-0.75, 4x4 at floor-1, zero outside the image),
 // term order m outer / n inner as fused_warp.c:70-105.  The whole stencil
-// is tested once; only border stencils test each tap.
-// Sample positions further than the stencil reach outside the image give 0;
-// clamping them to [-16, size + 16] keeps that value and keeps every index
-// computation in range for any input (non-finite or huge displacements).
-__device__ __forceinline__ float clamp_pos(float v, int size)
-{
-    return fminf(fmaxf(v, -16.0f), (float)size + 16.0f);
-}
+// is tested once; only border stencils test each tap.  The caller keeps the
+// shifts within +-kShiftMax, so every index computation is in range; hm3 =
+// max(H - 3, 0) and wm3 = max(W - 3, 0) make the interior test two unsigned
+// compares.
+__device__ __forceinline__ bool inside(int i0, int lim) { return (unsigned)i0 < (unsigned)lim; }
 
 template <typename TI>
-__device__ __forceinline__ float sample_bicubic(const TI *img, float fy, float fx, int H, int W)
+__device__ __forceinline__ float sample_bicubic(const TI *img, float fy, float fx, int H, int W, int hm3,
+                                                int wm3)
 {
-    fy = clamp_pos(fy, H);
-    fx = clamp_pos(fx, W);
     const float fyf = floorf(fy), fxf = floorf(fx);
     const int iy = (int)fyf - 1, ix = (int)fxf - 1;
     float wy[4], wx[4];
     keys4(fy - fyf, wy);
     keys4(fx - fxf, wx);
     float val = 0.0f;
-    if (iy >= 0 && iy + 3 < H && ix >= 0 && ix + 3 < W) {
-        const TI *p = img + (size_t)iy * W + ix;
+    if (inside(iy, hm3) && inside(ix, wm3)) {
+        const TI *p = img + (iy * W + ix);
 #pragma unroll
         for (int m = 0; m < 4; m++) {
 #pragma unroll
@@ -99,7 +101,7 @@ __device__ __forceinline__ float sample_bicubic(const TI *img, float fy, float f
         for (int n = 0; n < 4; n++) {
             const int col = ix + n;
             if (col < 0 || col >= W) continue;
-            val += wy[m] * wx[n] * (float)img[(size_t)row * W + col];
+            val += wy[m] * wx[n] * (float)img[row * W + col];
         }
     }
     return val;
@@ -119,19 +121,17 @@ __device__ __forceinline__ void lanczos_w(float d, const float *lut, float w[6])
 }
 
 template <typename TI>
-__device__ __forceinline__ float sample_lanczos(const TI *img, float fy, float fx, int H, int W,
-                                                const float *lut)
+__device__ __forceinline__ float sample_lanczos(const TI *img, float fy, float fx, int H, int W, int hm5,
+                                                int wm5, const float *lut)
 {
-    fy = clamp_pos(fy, H);
-    fx = clamp_pos(fx, W);
     const float fyf = floorf(fy), fxf = floorf(fx);
     const int iy = (int)fyf - 2, ix = (int)fxf - 2;
     float wy[6], wx[6];
     lanczos_w(fy - fyf, lut, wy);
     lanczos_w(fx - fxf, lut, wx);
     float val = 0.0f;
-    if (iy >= 0 && iy + 5 < H && ix >= 0 && ix + 5 < W) {
-        const TI *p = img + (size_t)iy * W + ix;
+    if (inside(iy, hm5) && inside(ix, wm5)) {
+        const TI *p = img + (iy * W + ix);
 #pragma unroll
         for (int m = 0; m < 6; m++) {
 #pragma unroll
@@ -148,7 +148,7 @@ __device__ __forceinline__ float sample_lanczos(const TI *img, float fy, float f
         for (int n = 0; n < 6; n++) {
             const int col = ix + n;
             if (col < 0 || col >= W) continue;
-            val += wy[m] * wx[n] * (float)img[(size_t)row * W + col];
+            val += wy[m] * wx[n] * (float)img[row * W + col];
         }
     }
     return val;
@@ -170,57 +170,84 @@ __device__ __forceinline__ const TI *warp_src(const WarpArgs &g, bool a)
     else return a ? g.imgA : g.imgB;
 }
 
+// Half-shifts beyond this put every tap outside any image (zero output);
+// clamping them keeps all position and index arithmetic in range for any
+// input, non-finite included, and leaves in-range shifts untouched.
+constexpr float kShiftMax = 1048576.0f;
+
 // TI: float frames, or 16-bit counts (converted exactly on load).
 template <typename TI>
 __global__ void __launch_bounds__(256) warp_kernel(WarpArgs g)
 {
     __shared__ float s_ly[kTileY];
     __shared__ float s_lx[kTileX];
+    __shared__ float4 s_wy[kTileY];     // predictor y weights of each tile row
+    __shared__ int4 s_ro[kTileY];       // predictor row offsets of each tile row
     __shared__ float s_pdy[kPredMax];
     __shared__ float s_pdx[kPredMax];
-    const int tx = threadIdx.x & (kTileX - 1);
-    const int ty = threadIdx.x / kTileX;
+    const int tid = (int)threadIdx.x;
+    const int tx = tid & (kTileX - 1);
+    const int ty = tid / kTileX;
     const int x0 = blockIdx.x * kTileX, y0 = blockIdx.y * kTileY;
     const int img = blockIdx.z;
     const int nx_t = min(kTileX, g.W - x0), ny_t = min(kTileY, g.H - y0);
-    if (threadIdx.x < kTileX) {
-        const int x = x0 + min((int)threadIdx.x, nx_t - 1);
-        s_lx[threadIdx.x] = pred_index((float)x, g.cx, g.nPX);
-    } else if (threadIdx.x < kTileX + kTileY) {
-        const int y = y0 + min((int)threadIdx.x - kTileX, ny_t - 1);
-        s_ly[threadIdx.x - kTileX] = pred_index((float)y, g.cy, g.nPY);
+    if (tid < kTileX) {
+        const int x = x0 + min(tid, nx_t - 1);
+        s_lx[tid] = pred_index((float)x, g.cx, g.nPX);
+    } else if (tid < kTileX + kTileY) {
+        const int y = y0 + min(tid - kTileX, ny_t - 1);
+        s_ly[tid - kTileX] = pred_index((float)y, g.cy, g.nPY);
     }
     __syncthreads();
-    // predictor cells the tile's stencils touch: integer min/max over the
-    // tile's own indices (no assumption on the order of the centres), so a
-    // staged index is always inside the staged rectangle
-    int fy0 = 0x7fffffff, fy1 = -0x7fffffff, fx0 = 0x7fffffff, fx1 = -0x7fffffff;
-#pragma unroll
-    for (int k = 0; k < kTileY; ++k) {
-        const int v = (int)floorf(s_ly[k]);
-        fy0 = min(fy0, v);
-        fy1 = max(fy1, v);
+    // The staged predictor rectangle spans the tile's smallest to largest
+    // predictor index.  pred_index is non-decreasing in the coordinate for
+    // ascending centres, so these sit at the tile ends; one compare per lane
+    // verifies it (NaN fails too).  A tile that fails reads the predictor
+    // from global memory with clamped indices instead.
+    bool bad = false;
+    if (tid < kTileX) {
+        const float v = s_lx[tid];
+        bad = !(v == v) || (tid + 1 < kTileX && !(v <= s_lx[tid + 1]));
+    } else if (tid < kTileX + kTileY) {
+        const int k = tid - kTileX;
+        const float v = s_ly[k];
+        bad = !(v == v) || (k + 1 < kTileY && !(v <= s_ly[k + 1]));
     }
-#pragma unroll 8
-    for (int k = 0; k < kTileX; ++k) {
-        const int v = (int)floorf(s_lx[k]);
-        fx0 = min(fx0, v);
-        fx1 = max(fx1, v);
+    const bool mono = !__syncthreads_or(bad);
+    int pr0 = 0, pr1 = -1, pc0 = 0, pc1 = -1;
+    if (mono) {
+        const int fy0 = (int)floorf(s_ly[0]), fy1 = (int)floorf(s_ly[kTileY - 1]);
+        const int fx0 = (int)floorf(s_lx[0]), fx1 = (int)floorf(s_lx[kTileX - 1]);
+        pr0 = min(max(fy0 - 1, 0), g.nPY - 1);
+        pr1 = min(max(fy1 + 2, 0), g.nPY - 1);
+        pc0 = min(max(fx0 - 1, 0), g.nPX - 1);
+        pc1 = min(max(fx1 + 2, 0), g.nPX - 1);
     }
-    const int pr0 = min(max(fy0 - 1, 0), g.nPY - 1);
-    const int pr1 = min(max(fy1 + 2, 0), g.nPY - 1);
-    const int pc0 = min(max(fx0 - 1, 0), g.nPX - 1);
-    const int pc1 = min(max(fx1 + 2, 0), g.nPX - 1);
     const int pcw = pc1 - pc0 + 1, prh = pr1 - pr0 + 1;
-    const bool staged = (pcw * prh <= kPredMax) && pr1 >= pr0 && pc1 >= pc0;
+    const bool staged = mono && pr1 >= pr0 && pc1 >= pc0 && pcw * prh <= kPredMax;
     const float *dyp = g.pdy + (size_t)img * g.predStride;
     const float *dxp = g.pdx + (size_t)img * g.predStride;
     if (staged) {
-        for (int e = threadIdx.x; e < pcw * prh; e += 256) {
+        for (int e = tid; e < pcw * prh; e += 256) {
             const int r = e / pcw, c = e - r * pcw;
             s_pdy[e] = dyp[(pr0 + r) * g.nPX + pc0 + c];
             s_pdx[e] = dxp[(pr0 + r) * g.nPX + pc0 + c];
         }
+    }
+    // y weights and row offsets of the predictor stencil, once per tile row
+    if (tid < kTileY) {
+        const float fiy = s_ly[tid], fiyf = floorf(fiy);
+        const int iyb = (int)fiyf - 1;
+        float wy[4];
+        keys4(fiy - fiyf, wy);
+        int ro[4];
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            const int rr = min(max(iyb + m, 0), g.nPY - 1);
+            ro[m] = staged ? (rr - pr0) * pcw - pc0 : rr * g.nPX;
+        }
+        s_wy[tid] = make_float4(wy[0], wy[1], wy[2], wy[3]);
+        s_ro[tid] = make_int4(ro[0], ro[1], ro[2], ro[3]);
     }
     __syncthreads();
 
@@ -229,6 +256,8 @@ __global__ void __launch_bounds__(256) warp_kernel(WarpArgs g)
     const size_t plane = (size_t)g.H * g.W;
     const TI *A = warp_src<TI>(g, true) + (size_t)img * plane;
     const TI *B = warp_src<TI>(g, false) + (size_t)img * plane;
+    const int hm3 = max(g.H - 3, 0), wm3 = max(g.W - 3, 0);
+    const int hm5 = max(g.H - 5, 0), wm5 = max(g.W - 5, 0);
 
     const float fx = s_lx[tx], fxf = floorf(fx);
     const int ix = (int)fxf - 1;
@@ -243,43 +272,43 @@ __global__ void __launch_bounds__(256) warp_kernel(WarpArgs g)
         const int ly = ty + r * kLanesY;
         const int i = y0 + ly;
         if (i >= g.H) break;
-        const float fiy = s_ly[ly], fiyf = floorf(fiy);
-        const int iyb = (int)fiyf - 1;
-        float wy[4];
-        keys4(fiy - fiyf, wy);
+        const float4 wy4 = s_wy[ly];
+        const int4 ro4 = s_ro[ly];
+        const float wy[4] = {wy4.x, wy4.y, wy4.z, wy4.w};
+        const int ro[4] = {ro4.x, ro4.y, ro4.z, ro4.w};
         float va = 0.0f, vb = 0.0f;
         if (staged) {
 #pragma unroll
             for (int m = 0; m < 4; m++) {
-                const int row = (min(max(iyb + m, 0), g.nPY - 1) - pr0) * pcw - pc0;
 #pragma unroll
                 for (int n = 0; n < 4; n++) {
                     const float w = wy[m] * wx[n];
-                    va += w * s_pdy[row + cols[n]];
-                    vb += w * s_pdx[row + cols[n]];
+                    va += w * s_pdy[ro[m] + cols[n]];
+                    vb += w * s_pdx[ro[m] + cols[n]];
                 }
             }
         } else {
 #pragma unroll
             for (int m = 0; m < 4; m++) {
-                const int row = min(max(iyb + m, 0), g.nPY - 1) * g.nPX;
 #pragma unroll
                 for (int n = 0; n < 4; n++) {
                     const float w = wy[m] * wx[n];
-                    va += w * dyp[row + cols[n]];
-                    vb += w * dxp[row + cols[n]];
+                    va += w * dyp[ro[m] + cols[n]];
+                    vb += w * dxp[ro[m] + cols[n]];
                 }
             }
         }
         float hy = 0.5f * va, hx = 0.5f * vb;
         if (g.roundShifts) { hy = roundf(hy); hx = roundf(hx); }
+        hy = fminf(fmaxf(hy, -kShiftMax), kShiftMax);
+        hx = fminf(fmaxf(hx, -kShiftMax), kShiftMax);
         const size_t o = (size_t)img * plane + (size_t)i * g.W + j;
         if (g.mode == 0) {
-            g.outA[o] = sample_bicubic(A, (float)i - hy, (float)j - hx, g.H, g.W);
-            g.outB[o] = sample_bicubic(B, (float)i + hy, (float)j + hx, g.H, g.W);
+            g.outA[o] = sample_bicubic(A, (float)i - hy, (float)j - hx, g.H, g.W, hm3, wm3);
+            g.outB[o] = sample_bicubic(B, (float)i + hy, (float)j + hx, g.H, g.W, hm3, wm3);
         } else {
-            g.outA[o] = sample_lanczos(A, (float)i - hy, (float)j - hx, g.H, g.W, g.lut);
-            g.outB[o] = sample_lanczos(B, (float)i + hy, (float)j + hx, g.H, g.W, g.lut);
+            g.outA[o] = sample_lanczos(A, (float)i - hy, (float)j - hx, g.H, g.W, hm5, wm5, g.lut);
+            g.outB[o] = sample_lanczos(B, (float)i + hy, (float)j + hx, g.H, g.W, hm5, wm5, g.lut);
         }
     }
 }
