@@ -247,7 +247,9 @@ def run_serving(args):
         pool.append([S.multipass_pair(seed=4000 + 10 * i + k, shape=sp.shape, device=dev) for k in range(4)])
     torch.cuda.synchronize()
     srv = PIVServer(specs, max_batch=args.serve_batch, max_slots_per_stream=2,
-                    hbm_budget_bytes=int(args.serve_hbm_gb * (1 << 30)))
+                    hbm_budget_bytes=int(args.serve_hbm_gb * (1 << 30)), autotune=args.autotune)
+    if srv.tuned:
+        log(f"[bench] autotuned variants {({n: r['variant'] for n, r in srv.tuned.items()})}")
     rng = np.random.default_rng(12345)
 
     def drive(rate, duration):
@@ -309,6 +311,10 @@ def main():
     ap.add_argument("--dft", choices=["fft", "split"], default="fft",
                     help="split: split-fp16 MFMA DFT for the 16/32/48/64 passes")
     ap.add_argument("--cpu-budget", type=float, default=60.0, help="0 disables the CPU baseline leg")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="keep the default correlation-kernel variants instead of timing them first")
+    ap.add_argument("--one-pair-steps", type=int, default=10,
+                    help="after the timed region, time the same loop fed one pair per step (0 disables)")
     ap.add_argument("--workload", choices=["multipass", "ensemble", "serving"], default="multipass")
     ap.add_argument("--serve-rates", type=float, nargs="+", default=[50, 200, 500, 1000, 2000])
     ap.add_argument("--serve-seconds", type=float, default=2.0)
@@ -341,6 +347,10 @@ def main():
     A, B = make_batch(args.pairs, dev)
     torch.cuda.synchronize()
     mp = MultipassPIV(SHAPE, SCHEDULE, OVERLAP, peak_finder=args.finder, peak_fit_impl=args.impl, dft=args.dft)
+    tuned = None
+    if args.autotune and args.dft == "fft":
+        from pivtools_amd import autotune
+        tuned = autotune.tune_xcorr([w[0] for w in SCHEDULE], reps=10, log=log)
     log("[bench] warmup")
     for _ in range(args.warmup):
         mp.run(A, B)
@@ -395,6 +405,26 @@ def main():
     gx = res_final.ux[0].cpu().numpy()
     gy = res_final.uy[0].cpu().numpy()
     gm = res_final.nan_mask[0].cpu().numpy().astype(bool)
+
+    # the same pass loop fed one pair per step (no batching), after the timed region
+    one = None
+    if args.one_pair_steps > 0:
+        a1, b1 = A[:1].contiguous(), B[:1].contiguous()
+        for _ in range(2):
+            mp.run(a1, b1)
+        torch.cuda.synchronize()
+        t1 = []
+        for _ in range(args.one_pair_steps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            mp.run(a1, b1)
+            torch.cuda.synchronize()
+            t1.append((time.perf_counter() - t0) * 1e3)
+        rate = 1e3 / float(np.mean(t1))
+        one = {"pairs_per_s": round(rate, 3), "latency_ms_p50": round(float(np.percentile(t1, 50)), 3),
+               "latency_ms_p99": round(float(np.percentile(t1, 99)), 3), "steps": args.one_pair_steps,
+               "batched_over_one_pair": round(value / rate, 3)}
+        log(f"[bench] one pair per step: {rate:.1f} pairs/s (batched / one-pair = {value / rate:.2f})")
     cpu = None
     acc = {}
     if args.cpu_budget > 0:
@@ -417,7 +447,8 @@ def main():
         "config": {"workload": "multipass_2160x2560_u16", "pairs_per_step": args.pairs,
                    "schedule": [w[0] for w in SCHEDULE], "overlap": 50, "peak_finder": args.finder,
                    "peak_fit_impl": args.impl, "dft": args.dft, "warp": "bicubic", "outlier": "median 5x5 eps 0.1 thr 2",
-                   "global_batch": args.pairs, "parallelism": "single-gpu"},
+                   "global_batch": args.pairs, "parallelism": "single-gpu",
+                   "xcorr_variants": ({str(n): r["variant"] for n, r in tuned.items()} if tuned else "default")},
         "latency_ms_p50": round(float(np.percentile(steps_ms, 50)), 3),
         "latency_ms_p99": round(float(np.percentile(steps_ms, 99)), 3),
         "vectors_per_s": round(value * mp.grids[-1].n_win[0] * mp.grids[-1].n_win[1], 1),
@@ -428,6 +459,8 @@ def main():
                      "note": "FP32 VALU FFT; peak = MI355X FP32 157.3 TFLOP/s"},
         "cpu_baseline": cpu,
         "accuracy": acc,
+        "one_pair_fp32": one,
+        "autotune": ({str(n): r for n, r in tuned.items()} if tuned else None),
     }
     print(json.dumps(line), flush=True)
 

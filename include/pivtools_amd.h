@@ -163,6 +163,20 @@ int pivamd_size_supported(int ny, int nx);
  * windows.  Returns the previous mode. */
 int pivamd_set_dft_mode(int mode);
 
+/* Launch variants of the correlation kernel per square window size N (no
+ * counterpart in the reference, whose C path has one build per size):
+ * R = 1, 2, 4 is the FP32 FFT with each line transform shared by R lanes,
+ * 16 the split-precision MFMA DFT.  _variants writes up to max_out codes
+ * and returns how many are built; _set_variant selects one for N
+ * (0 = default, process-wide, like bulkxcorr2d_set_peakfit_impl) and
+ * returns the previous selection, or -1 if v is not built for N;
+ * _default_variant and _get_variant return -1 for an unsupported N.  The
+ * autotuner (pivtools_amd/autotune.py) times them and keeps the fastest. */
+int pivamd_xcorr_variants(int N, int *out, int max_out);
+int pivamd_xcorr_default_variant(int N);
+int pivamd_xcorr_set_variant(int N, int v);
+int pivamd_xcorr_get_variant(int N);
+
 /* d_out = centre crop of the 2N zero-padded correlation of a and b (FP64). */
 int pivamd_convolve(const float *d_a, const float *d_b, int ny, int nx, float *d_out, void *stream);
 

@@ -17,6 +17,10 @@
 extern "C" {
 int pivamd_size_supported(int ny, int nx);
 int pivamd_set_dft_mode(int mode);
+int pivamd_xcorr_variants(int N, int *out, int max_out);
+int pivamd_xcorr_default_variant(int N);
+int pivamd_xcorr_set_variant(int N, int v);
+int pivamd_xcorr_get_variant(int N);
 int pivamd_convolve(const float *d_a, const float *d_b, int ny, int nx, float *d_out, hipStream_t st);
 int pivamd_plane_weight(const float *d_wB, int ny, int nx, float *d_out, hipStream_t st);
 int pivamd_xcorr(const float *d_imgA, const float *d_imgB, const float *d_mask, int H, int W,

@@ -44,12 +44,26 @@ struct XcorrArgs {
     long long total;        // N_images * nWin
     int H, W, nWy, nWx, nPeaks, finder, impl, ensemble, nblocks;
     int dft;                // 1: split-fp16 MFMA DFT (N = 16, 32, 48, 64), 0: FP32 FFT
+    int variant;            // FFT lanes per line R (0: default), kVariantMfma: the MFMA form
 };
 
 constexpr int kMaxLds = 158 * 1024;
 
-// Lanes per line transform for window size N.
+// Lanes per line transform (R) for window size N: the default, and the
+// values built as selectable variants (fft_reg holds N/R points per lane).
 constexpr int xc_radix(int N) { return N >= 64 ? 4 : (N >= 48 ? 2 : 1); }
+
+constexpr bool xc_radix_built(int N, int R)
+{
+    return R == 1 ? N <= 32
+         : R == 2 ? (N >= 16 && N <= 64)
+         : R == 4 ? (N >= 32 && N <= 128 && N != 24)
+         : false;
+}
+
+// Variant codes of pivamd_xcorr_set_variant: 0 = default, R = FP32 FFT with
+// R lanes per line, kVariantMfma = split-precision MFMA DFT.
+constexpr int kVariantMfma = 16;
 
 // Tile layout of the complex stages.  R = 1: element (r, c) at r*(N+1) + c.
 // R > 1: at r*TPX + SKEW*(r/N1) + c + PADC*(c/N1), i.e. rows padded to TPX,
@@ -75,6 +89,8 @@ constexpr XcLayout xc_layout(int N, int R)
          : (N == 64 && R == 4) ? XcLayout{76, 3, 8}
          : (N == 64 && R == 2) ? XcLayout{66, 1, 16}
          : (N == 48 && R == 2) ? XcLayout{50, 1, 0}
+         : (N == 48 && R == 4) ? XcLayout{52, 1, 0}
+         : (N == 24 && R == 2) ? XcLayout{30, 1, 0}
          : (N == 32 && R == 2) ? XcLayout{34, 1, 0}
          : (N == 16 && R == 2) ? XcLayout{17, 0, 8}
          : XcLayout{N + 1, 0, 0};
@@ -84,9 +100,9 @@ constexpr XcLayout xc_layout(int N, int R)
 // (row or column) transform, T = N*R lanes per window, WPB windows per
 // workgroup as LDS allows (one complex tile and RED reduction slots per
 // window).
-template <int N>
+template <int N, int R_ = xc_radix(N)>
 struct XcCfg {
-    static constexpr int R = xc_radix(N);
+    static constexpr int R = R_;
     static constexpr int N1 = N / R;
     static constexpr int T = N * R;
     static constexpr int PX = N + 1;      // plane pitch
@@ -108,10 +124,10 @@ struct XcCfg {
 
 // Tile word of element (r, c); rb = r / N1 and cb = c / N1 are passed in
 // by the caller, which knows them as compile-time or per-lane constants.
-template <int N>
+template <int N, int R>
 __device__ __forceinline__ int tix(int r, int rb, int c, int cb)
 {
-    using C = XcCfg<N>;
+    using C = XcCfg<N, R>;
     return r * C::TPX + C::SKEW * rb + c + C::PADC * cb;
 }
 
@@ -586,11 +602,11 @@ __device__ __forceinline__ void plane_epilogue(const XcorrArgs &g, const float *
     }
 }
 
-template <int N>
-__global__ void __launch_bounds__((XcCfg<N>::BLOCK))
+template <int N, int R_>
+__global__ void __launch_bounds__((XcCfg<N, R_>::BLOCK))
 xcorr_kernel(XcorrArgs g)
 {
-    using C = XcCfg<N>;
+    using C = XcCfg<N, R_>;
     constexpr int R = C::R, N1 = C::N1, T = C::T, PX = C::PX, WPB = C::WPB;
     constexpr int numel = N * N;
     __shared__ float s_re[WPB * C::tile];
@@ -685,7 +701,7 @@ xcorr_kernel(XcorrArgs g)
         fft_split<N, R, false>(zr, zi, q, tw);
 #pragma unroll
         for (int k = 0; k < N1; ++k) {
-            const int e = tix<N>(k + N1 * ob, ob, l, lb);
+            const int e = tix<N, R>(k + N1 * ob, ob, l, lb);
             Sr[e] = zr[k];
             Si[e] = zi[k];
         }
@@ -697,14 +713,14 @@ xcorr_kernel(XcorrArgs g)
         float xr[N1], xi[N1];
 #pragma unroll
         for (int k = 0; k < N1; ++k) {
-            const int e = tix<N>(l, lb, k * R + q, (k * R) / N1);
+            const int e = tix<N, R>(l, lb, k * R + q, (k * R) / N1);
             xr[k] = Sr[e];
             xi[k] = Si[e];
         }
         fft_split<N, R, false>(xr, xi, q, tw);
 #pragma unroll
         for (int k = 0; k < N1; ++k) {
-            const int e = tix<N>(l, lb, k + N1 * ob, ob);
+            const int e = tix<N, R>(l, lb, k + N1 * ob, ob);
             Sr[e] = xr[k];
             Si[e] = xi[k];
         }
@@ -721,7 +737,7 @@ xcorr_kernel(XcorrArgs g)
             // (N - ky) mod N = (N1 - k) R - q; affine in k except at ky = 0
             const int my = k > 0 ? N - ky : (q == 0 ? 0 : N - q);
             const int myb = k > 0 ? mirror_block<N1, R>(N1 - k, q) : (q == 0 ? 0 : R - 1);
-            const int ez = tix<N>(ky, (k * R) / N1, l, lb), em = tix<N>(my, myb, mx, mxb);
+            const int ez = tix<N, R>(ky, (k * R) / N1, l, lb), em = tix<N, R>(my, myb, mx, mxb);
             const float zR = Sr[ez], zI = Si[ez];
             const float mR = Sr[em], mI = Si[em];
             // FA = (Z + conj(Zm)) / 2 ; FB = (Z - conj(Zm)) / (2i)
@@ -733,7 +749,7 @@ xcorr_kernel(XcorrArgs g)
         fft_split<N, R, true>(cr, ci, q, tw);
 #pragma unroll
         for (int k = 0; k < N1; ++k) {
-            const int e = tix<N>(k + N1 * ob, ob, l, lb);
+            const int e = tix<N, R>(k + N1 * ob, ob, l, lb);
             Sr[e] = cr[k];
             Si[e] = ci[k];
         }
@@ -753,7 +769,7 @@ xcorr_kernel(XcorrArgs g)
             const bool up = 2 * k > N1 || (2 * k == N1 && q > 0);
             const int kk = 2 * k < N1 ? kx : N - kx;        // N - kx = N/2 at kx = N/2
             const int kb = 2 * k < N1 ? (k * R) / N1 : mirror_block<N1, R>(N1 - k, q);
-            const int ea = tix<N>(ya, lb, kk, kb), eb = tix<N>(yb, lb + R / 2, kk, kb);
+            const int ea = tix<N, R>(ya, lb, kk, kb), eb = tix<N, R>(yb, lb + R / 2, kk, kb);
             float g1r = Sr[ea], g1i = Si[ea];
             float g2r = Sr[eb], g2i = Si[eb];
             if ((k == 0 || 2 * k == N1) && q == 0) { g1i = 0.0f; g2i = 0.0f; }   // kx = 0, N/2
@@ -1260,21 +1276,37 @@ template <int N>
 int launch_main(XcorrArgs g, hipStream_t st);
 
 #ifdef PIVAMD_XCORR_SIZE
+template <int N, int R>
+static int launch_fft(XcorrArgs g, hipStream_t st)
+{
+    if constexpr (!xc_radix_built(N, R)) {
+        return PIVAMD_ERR_ARG;
+    } else {
+        using C = XcCfg<N, R>;
+        const long long nb = (g.total + C::WPB - 1) / C::WPB;
+        if (nb <= 0) return 0;
+        if (nb > (1LL << 30)) return PIVAMD_ERR_ARG;
+        g.nblocks = (int)nb;
+        const int grid = (int)((nb + 7) / 8 * 8);
+        hipLaunchKernelGGL((xcorr_kernel<N, R>), dim3(grid), dim3(C::BLOCK), 0, st, g);
+        return hipGetLastError() == hipSuccess ? 0 : PIVAMD_ERR_LAUNCH;
+    }
+}
+
 template <>
 int launch_main<PIVAMD_XCORR_SIZE>(XcorrArgs g, hipStream_t st)
 {
     constexpr int N = PIVAMD_XCORR_SIZE;
+    switch (g.variant) {
 #if (PIVAMD_XCORR_SIZE % 16 == 0) && (PIVAMD_XCORR_SIZE <= 64)
-    if (g.dft == 1) return launch_mfma(g, st);
+    case kVariantMfma: return launch_mfma(g, st);
 #endif
-    using C = XcCfg<N>;
-    const long long nb = (g.total + C::WPB - 1) / C::WPB;
-    if (nb <= 0) return 0;
-    if (nb > (1LL << 30)) return PIVAMD_ERR_ARG;
-    g.nblocks = (int)nb;
-    const int grid = (int)((nb + 7) / 8 * 8);
-    hipLaunchKernelGGL((xcorr_kernel<N>), dim3(grid), dim3(C::BLOCK), 0, st, g);
-    return hipGetLastError() == hipSuccess ? 0 : PIVAMD_ERR_LAUNCH;
+    case 1: return launch_fft<N, 1>(g, st);
+    case 2: return launch_fft<N, 2>(g, st);
+    case 4: return launch_fft<N, 4>(g, st);
+    case 0: return launch_fft<N, xc_radix(N)>(g, st);
+    default: return PIVAMD_ERR_ARG;
+    }
 }
 #endif
 
@@ -1366,6 +1398,47 @@ extern "C" size_t pivamd_xcorr_workspace(int n_images, int nWy, int nWx, int win
 // (other sizes keep the FFT).  Read when a launch is issued.
 static int g_dft_mode = 0;
 
+// Selected variant per window size (index N); 0 = default.
+static int g_variant[129];
+
+static bool variant_ok(int N, int v)
+{
+    if (!pivamd_size_supported(N, N)) return false;
+    if (v == 0) return true;
+    if (v == kVariantMfma) return N % 16 == 0 && N <= 64;
+    return (v == 1 || v == 2 || v == 4) && xc_radix_built(N, v);
+}
+
+extern "C" int pivamd_xcorr_variants(int N, int *out, int max_out)
+{
+    static const int cand[] = {1, 2, 4, kVariantMfma};
+    int n = 0;
+    for (int v : cand)
+        if (variant_ok(N, v)) {
+            if (out && n < max_out) out[n] = v;
+            ++n;
+        }
+    return n;
+}
+
+extern "C" int pivamd_xcorr_default_variant(int N)
+{
+    return pivamd_size_supported(N, N) ? xc_radix(N) : -1;
+}
+
+extern "C" int pivamd_xcorr_set_variant(int N, int v)
+{
+    if (!variant_ok(N, v)) return -1;
+    const int prev = g_variant[N];
+    g_variant[N] = v;
+    return prev;
+}
+
+extern "C" int pivamd_xcorr_get_variant(int N)
+{
+    return pivamd_size_supported(N, N) ? g_variant[N] : -1;
+}
+
 extern "C" int pivamd_set_dft_mode(int mode)
 {
     const int prev = g_dft_mode;
@@ -1397,6 +1470,7 @@ static int xcorr_entry(const float *d_imgA, const float *d_imgB, const unsigned 
     g.H = H; g.W = W; g.nWy = nWy; g.nWx = nWx; g.nPeaks = nPeaks; g.finder = finder;
     g.impl = impl; g.ensemble = ensemble; g.nblocks = 0;
     g.dft = (g_dft_mode == 1 && winY == winX && winY % 16 == 0 && winY <= 64) ? 1 : 0;
+    g.variant = g.dft ? kVariantMfma : (winY == winX && winY > 0 && winY <= 128 ? g_variant[winY] : 0);
     if (!ensemble && (!d_pkX || !d_pkY || !d_pkH || !d_sx || !d_sy || !d_sxy || !d_ratio || !d_work))
         return PIVAMD_ERR_ARG;
     if (winY != winX || !pivamd_size_supported(winY, winX)) return PIVAMD_ERR_SIZE;

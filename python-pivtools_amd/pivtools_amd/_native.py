@@ -49,6 +49,14 @@ def core() -> ctypes.CDLL:
         L.pivamd_size_supported.restype = _i
         L.pivamd_set_dft_mode.argtypes = [_i]
         L.pivamd_set_dft_mode.restype = _i
+        L.pivamd_xcorr_variants.argtypes = [_i, _vp, _i]
+        L.pivamd_xcorr_variants.restype = _i
+        L.pivamd_xcorr_default_variant.argtypes = [_i]
+        L.pivamd_xcorr_default_variant.restype = _i
+        L.pivamd_xcorr_set_variant.argtypes = [_i, _i]
+        L.pivamd_xcorr_set_variant.restype = _i
+        L.pivamd_xcorr_get_variant.argtypes = [_i]
+        L.pivamd_xcorr_get_variant.restype = _i
         L.pivamd_plane_weight.argtypes = [_vp, _i, _i, _vp, _vp]
         L.pivamd_plane_weight.restype = _i
         L.pivamd_convolve.argtypes = [_vp, _vp, _i, _i, _vp, _vp]

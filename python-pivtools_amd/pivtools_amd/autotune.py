@@ -1,0 +1,220 @@
+"""Autotuner for the correlation kernel's launch variants.
+
+Every square window size has several builds of the correlation kernel in
+the library (``pivamd_xcorr_variants``):
+
+* ``R`` = 1, 2 or 4: the FP32 FFT form with each row/column transform
+  shared by R lanes.  Larger R means more lanes (and waves) per window, so
+  more latency hiding, at the cost of cross-lane exchange and twiddle work;
+  it also changes the LDS tile layout and the windows per workgroup.
+* ``16``: the split-precision MFMA DFT form (16, 32, 48 and 64 points).
+
+Which one is fastest depends on the device and the size, so the library
+lets the caller pick per size (``pivamd_xcorr_set_variant``, process-wide,
+like the reference's peak-fit selector) and this module measures: for each
+size it runs every variant on a synthetic batch that fills the GPU, checks
+that the variant reproduces the default variant's displacements, and keeps
+the fastest.  Results can be cached in a JSON file keyed by the device and
+the library build, so a server pays the measurement once.
+
+The reference has no counterpart (its C path has one build per size); the
+variants all compute the same correlation, so a choice only moves rounding
+at the float32 level (checked here to 1e-3 px, and by the GPU tests to the
+parity tolerances).
+"""
+
+from __future__ import annotations
+
+import ctypes
+import json
+import os
+import time
+
+import numpy as np
+
+from . import _native
+from .window import compute_window_centers, window_weight
+
+MFMA = 16
+_TARGET_WINDOWS = {8: 262144, 12: 262144, 16: 262144, 24: 196608, 32: 131072, 48: 65536, 64: 49152,
+                   96: 20480, 128: 16384}
+
+
+def variants(n: int) -> list[int]:
+    """Variant codes built for window size n."""
+    L = _native.core()
+    buf = (ctypes.c_int * 8)()
+    k = L.pivamd_xcorr_variants(n, buf, 8)
+    return [int(buf[i]) for i in range(min(k, 8))]
+
+
+def default_variant(n: int) -> int:
+    return int(_native.core().pivamd_xcorr_default_variant(n))
+
+
+def get_variant(n: int) -> int:
+    return int(_native.core().pivamd_xcorr_get_variant(n))
+
+
+def set_variant(n: int, v: int) -> int:
+    """Select variant v for size n (0 = default); returns the previous one."""
+    prev = int(_native.core().pivamd_xcorr_set_variant(n, v))
+    if prev < 0:
+        raise ValueError(f"variant {v} is not built for {n}x{n} windows (have {variants(n)})")
+    return prev
+
+
+def apply(table: dict) -> None:
+    """Select the variants of a tuning table ({size: {"variant": v, ...}})."""
+    for n, rec in table.items():
+        set_variant(int(n), int(rec["variant"]))
+
+
+class _Case:
+    """A synthetic correlation call of one window size, on the device."""
+
+    def __init__(self, n: int, device, shape=(1024, 1024), seed: int = 5):
+        import torch
+
+        self.torch = torch
+        self.n = n
+        self.dev = device
+        wc = compute_window_centers(shape, (n, n), 50)
+        self.ny, self.nx = wc.n_win_y, wc.n_win_x
+        per_img = self.ny * self.nx
+        self.n_img = int(max(1, min(64, -(-_TARGET_WINDOWS.get(n, 65536) // per_img))))
+        g = torch.Generator(device="cpu")
+        g.manual_seed(seed + n)
+        # speckle-like frames: smoothed noise, B = A moved by (1.3, -0.7) px
+        a = torch.rand((self.n_img, shape[0] + 8, shape[1] + 8), generator=g)
+        a = torch.nn.functional.avg_pool2d(a[:, None], 3, 1, 1)[:, 0]
+        self.A = a[:, 4:-4, 4:-4].contiguous().to(device)
+        b = torch.roll(a, shifts=(1, -1), dims=(1, 2))
+        self.B = (0.7 * b[:, 4:-4, 4:-4] + 0.3 * a[:, 4:-4, 4:-4]).contiguous().to(device)
+        self.H, self.W = shape
+        self.cx = torch.as_tensor(wc.win_ctrs_x, device=device)
+        self.cy = torch.as_tensor(wc.win_ctrs_y, device=device)
+        self.w = torch.as_tensor(window_weight((n, n), "gaussian"), device=device)
+        self.pw = torch.empty((n, n), device=device)
+        L = _native.core()
+        st = torch.cuda.current_stream(device).cuda_stream
+        _native.check(L.pivamd_plane_weight(self.w.data_ptr(), n, n, self.pw.data_ptr(), st), "plane weight")
+        shp = (self.n_img, self.ny, self.nx)
+        self.out = {k: torch.empty(shp, device=device) for k in ("pkx", "pky", "pkh", "sx", "sy", "sxy", "ratio")}
+        ws = L.pivamd_xcorr_workspace(self.n_img, self.ny, self.nx, n, n, 1, 0)
+        self.work = torch.empty(max(ws, 4), dtype=torch.uint8, device=device)
+
+    @property
+    def windows(self) -> int:
+        return self.n_img * self.ny * self.nx
+
+    def run(self) -> None:
+        L = _native.core()
+        o = self.out
+        st = self.torch.cuda.current_stream(self.dev).cuda_stream
+        rc = L.pivamd_xcorr(self.A.data_ptr(), self.B.data_ptr(), None, self.H, self.W, self.n_img,
+                            self.cx.data_ptr(), self.cy.data_ptr(), self.ny, self.nx, self.w.data_ptr(),
+                            self.w.data_ptr(), self.pw.data_ptr(), 0, self.n, self.n, 1, 3, 0,
+                            o["pkx"].data_ptr(), o["pky"].data_ptr(), o["pkh"].data_ptr(), o["sx"].data_ptr(),
+                            o["sy"].data_ptr(), o["sxy"].data_ptr(), o["ratio"].data_ptr(), None,
+                            self.work.data_ptr(), self.work.numel(), st)
+        _native.check(rc, "bulkxcorr2d")
+
+    def result(self):
+        return self.out["pkx"].cpu().numpy().copy(), self.out["pky"].cpu().numpy().copy()
+
+    def time_ms(self, reps: int) -> float:
+        torch = self.torch
+        self.run()                                  # warm (first launch of this variant)
+        ts = []
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            self.run()
+            e1.record()
+            ts.append((e0, e1))
+        torch.cuda.synchronize(self.dev)
+        return float(np.median([a.elapsed_time(b) for a, b in ts]))
+
+
+def agree(ref, got, tol_px: float = 1e-3, max_flip: float = 1e-3) -> bool:
+    """Displacements of two variants agree: NaN masks nearly identical and
+    the common valid vectors within tol_px."""
+    (rx, ry), (gx, gy) = ref, got
+    rn, gn = np.isnan(rx), np.isnan(gx)
+    if np.mean(rn != gn) > max_flip:
+        return False
+    both = ~rn & ~gn
+    if not both.any():
+        return True
+    d = np.hypot(rx[both] - gx[both], ry[both] - gy[both])
+    return bool(np.mean(d > tol_px) <= max_flip)
+
+
+def _device_key(device) -> str:
+    import torch
+
+    p = torch.cuda.get_device_properties(device)
+    lib = os.path.join(_native.LIB_DIR, "libpivtools_amd.so")
+    st = os.stat(lib)
+    return f"{p.name}|{getattr(p, 'gcnArchName', '')}|{p.multi_processor_count}|{st.st_size}|{int(st.st_mtime)}"
+
+
+def tune_xcorr(sizes, device=None, reps: int = 10, cache: str | None = None, include_mfma: bool = True,
+               log=None) -> dict:
+    """Measure every variant of each window size, select the fastest that
+    agrees with the default, and return {size: {"variant", "default", "ms",
+    "windows"}} ("ms": median milliseconds per call, per variant).  With
+    ``cache`` (a JSON path), a table stored for this device and library
+    build is applied without measuring, and new results are written back."""
+    import torch
+
+    dev = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+    key = _device_key(dev)
+    stored = {}
+    if cache and os.path.isfile(cache):
+        try:
+            with open(cache) as f:
+                stored = json.load(f)
+        except (OSError, ValueError):
+            stored = {}
+    known = stored.get(key, {})
+    table = {}
+    for n in sorted({int(s) for s in sizes}):
+        if str(n) in known:
+            table[n] = known[str(n)]
+            set_variant(n, int(table[n]["variant"]))
+            continue
+        cands = [v for v in variants(n) if include_mfma or v != MFMA]
+        dflt = default_variant(n)
+        case = _Case(n, dev)
+        prev = set_variant(n, dflt)
+        try:
+            case.run()
+            torch.cuda.synchronize(dev)
+            ref = case.result()
+            ms = {}
+            ok = {}
+            for v in cands:
+                set_variant(n, v)
+                ms[v] = case.time_ms(reps)
+                ok[v] = agree(ref, case.result())
+        finally:
+            set_variant(n, prev)
+        good = [v for v in cands if ok[v]] or [dflt]
+        best = min(good, key=lambda v: ms.get(v, float("inf")))
+        set_variant(n, best)
+        table[n] = {"variant": best, "default": dflt, "windows": case.windows,
+                    "ms": {str(v): round(t, 4) for v, t in ms.items()},
+                    "rejected": [v for v in cands if not ok[v]]}
+        if log:
+            log(f"[autotune] {n}x{n}: {table[n]['ms']} ms -> variant {best}")
+        del case
+    if cache:
+        stored[key] = {**known, **{str(k): v for k, v in table.items()}}
+        os.makedirs(os.path.dirname(os.path.abspath(cache)), exist_ok=True)
+        tmp = cache + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump(stored, f, indent=1)
+        os.replace(tmp, cache)
+    return table

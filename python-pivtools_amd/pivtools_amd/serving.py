@@ -91,8 +91,17 @@ class _Slot:
 
 class PIVServer:
     def __init__(self, streams: list[StreamSpec], max_batch: int = 8, max_slots_per_stream: int = 3,
-                 hbm_budget_bytes: int = 64 << 30, use_graphs: bool = True, device=None):
+                 hbm_budget_bytes: int = 64 << 30, use_graphs: bool = True, device=None, autotune=False):
+        """``autotune``: False keeps the default correlation-kernel variants;
+        True times them for every window size of the streams first; a path
+        string does the same through that JSON cache (measured once per
+        device and library build)."""
         self.dev = torch.device(device) if device is not None else torch.device("cuda")
+        self.tuned = None
+        if autotune:
+            from . import autotune as AT
+            sizes = sorted({int(w[0] if isinstance(w, (tuple, list)) else w) for sp in streams for w in sp.window_sizes})
+            self.tuned = AT.tune_xcorr(sizes, device=self.dev, cache=autotune if isinstance(autotune, str) else None)
         self.specs = list(streams)
         self.max_batch = int(max_batch)
         self.max_slots = int(max_slots_per_stream)
