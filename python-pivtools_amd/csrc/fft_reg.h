@@ -181,8 +181,9 @@ __device__ __forceinline__ void fft_reg(float (&xr)[N], float (&xi)[N])
 namespace pivamd {
 
 // ---------------------------------------------------------------------------
-// One transform of length N shared by R lanes (R = 1, 2 or 4, the R lanes of
-// a line are consecutive and quad-aligned).
+// One transform of length N shared by R lanes (R = 1, 2 or 4: the R lanes of
+// a line are consecutive and quad-aligned; R = 8: lanes {0-3, 8-11} + 16m
+// or {4-7, 12-15} + 16m, see the R = 8 exchange below).
 //   in : lane q holds x[n1*R + q], n1 = 0..N/R-1
 //   out: lane q holds X[k1 + (N/R) * out_block(q)], k1 = 0..N/R-1
 // X[k1 + N1 k2] = sum_n2 W_R^(n2 k2) W_N^(n2 k1) FFT_N1(x[. R + n2])[k1]:
@@ -227,6 +228,12 @@ __device__ __forceinline__ float dpp_swap2(float v)
     return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, true));
 }
 
+__device__ __forceinline__ float dpp_ror8(float v)
+{
+    // row_ror:8 (lane i of a 16-lane row reads lane i ^ 8)
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x128, 0xF, 0xF, true));
+}
+
 // partner + sign * own, rounded once (the product is exact)
 __device__ __forceinline__ float xadd(float partner, float sign, float own)
 {
@@ -238,7 +245,8 @@ __device__ __forceinline__ float xadd(float partner, float sign, float own)
 template <int R>
 __device__ __forceinline__ int out_block(int q)
 {
-    if constexpr (R == 4) return ((q & 1) << 1) | (q >> 1);
+    if constexpr (R == 8) return ((q & 1) << 2) | (q & 2) | ((q >> 2) & 1);
+    else if constexpr (R == 4) return ((q & 1) << 1) | (q >> 1);
     else return q;
 }
 
@@ -264,6 +272,41 @@ __device__ __forceinline__ void fft_split(float (&xr)[N / R], float (&xi)[N / R]
             for (int k = 0; k < N1; ++k) {
                 xr[k] = xadd(dpp_swap1(xr[k]), sg, xr[k]);
                 xi[k] = xadd(dpp_swap1(xi[k]), sg, xi[k]);
+            }
+        } else if constexpr (R == 8) {
+            // radix-2 DIF over q in three exchange stages.  The caller maps
+            // q bit 2 to lane bit 3 (partner 8 lanes away: row_ror:8) and
+            // q bits 0-1 to the quad (quad_perm).  Stage A: lanes q >= 4 take
+            // (partner - own) * W_8^(q & 3); stage B: (partner - own) * W_4
+            // on q & 3 == 3; stage C: partner -+ own.  Lane q ends with the
+            // block bitrev3(q) = out_block<8>(q).
+            constexpr float h = 0.70710678118654752440f;
+            const int j = q & 3;
+            const bool hi = (q & 4) != 0;
+            float wr = 1.0f, wi = 0.0f;
+            if (hi) {
+                wr = j == 0 ? 1.0f : (j == 1 ? h : (j == 2 ? 0.0f : -h));
+                wi = j == 0 ? 0.0f : (j == 1 ? -h : (j == 2 ? -1.0f : -h));
+                if (INV) wi = -wi;
+            }
+            const float sa = hi ? -1.0f : 1.0f;
+            const float sb = (q & 2) ? -1.0f : 1.0f;
+            const float sc = (q & 1) ? -1.0f : 1.0f;
+            const bool rot = j == 3;
+#pragma unroll
+            for (int k = 0; k < N1; ++k) {
+                const float ur = xadd(dpp_ror8(xr[k]), sa, xr[k]);
+                const float ui = xadd(dpp_ror8(xi[k]), sa, xi[k]);
+                float tr = ur * wr - ui * wi;
+                float ti = ur * wi + ui * wr;
+                float vr = xadd(dpp_swap2(tr), sb, tr);
+                float vi = xadd(dpp_swap2(ti), sb, ti);
+                // W_4 = -i (forward) or +i (inverse) on q & 3 == 3
+                const float rr = INV ? -vi : vi, ri = INV ? vr : -vr;
+                vr = rot ? rr : vr;
+                vi = rot ? ri : vi;
+                xr[k] = xadd(dpp_swap1(vr), sc, vr);
+                xi[k] = xadd(dpp_swap1(vi), sc, vi);
             }
         } else {
             const float sa = (q & 2) ? -1.0f : 1.0f;

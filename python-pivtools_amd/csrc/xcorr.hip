@@ -58,6 +58,7 @@ constexpr bool xc_radix_built(int N, int R)
     return R == 1 ? N <= 32
          : R == 2 ? (N >= 16 && N <= 64)
          : R == 4 ? (N >= 32 && N <= 128 && N != 24)
+         : R == 8 ? N == 128
          : false;
 }
 
@@ -84,6 +85,7 @@ constexpr XcLayout xc_layout(int N, int R)
 {
     return R == 1 ? XcLayout{N + 1, 0, 0}
          : (N == 128 && R == 4) ? XcLayout{140, 3, 8}
+         : (N == 128 && R == 8) ? XcLayout{136, 1, 4}
          : (N == 96 && R == 4) ? XcLayout{100, 1, 8}
          : (N == 32 && R == 4) ? XcLayout{44, 3, 40}
          : (N == 64 && R == 4) ? XcLayout{76, 3, 8}
@@ -120,6 +122,19 @@ struct XcCfg {
     static constexpr int wpb_lds = (kMaxLds - lds_blk) / lds_win;
     static constexpr int WPB = wpb_thr < wpb_lds ? wpb_thr : wpb_lds;
     static constexpr int BLOCK = WPB * T;
+    // lane t of a window -> line l and lane-in-line q.  R = 8 puts q bit 2 on
+    // lane bit 3, so every exchange partner is one DPP away (xor 1, 2: quad
+    // permutes; xor 8: row_ror:8).
+    static __device__ __forceinline__ void lane_map(int t, int &l, int &q)
+    {
+        if constexpr (R == 8) {
+            q = (t & 3) | (((t >> 3) & 1) << 2);
+            l = ((t >> 2) & 1) | ((t >> 4) << 1);
+        } else {
+            l = t / R;
+            q = t - l * R;
+        }
+    }
 };
 
 // Tile word of element (r, c); rb = r / N1 and cb = c / N1 are passed in
@@ -624,8 +639,8 @@ xcorr_kernel(XcorrArgs g)
 
     const int wloc = (int)threadIdx.x / T;
     const int t = (int)threadIdx.x - wloc * T;
-    const int l = t / R;              // line (row or column) of this lane
-    const int q = t - l * R;          // lane within the line
+    int l, q;                         // line (row or column), lane within the line
+    C::lane_map(t, l, q);
     const int ob = out_block<R>(q);   // output block of fft_split
     const int lb = l / N1;            // block of this lane's line index
     const float2 *tw = s_tw + (R > 1 ? q * C::TWP : 0);
@@ -1304,6 +1319,7 @@ int launch_main<PIVAMD_XCORR_SIZE>(XcorrArgs g, hipStream_t st)
     case 1: return launch_fft<N, 1>(g, st);
     case 2: return launch_fft<N, 2>(g, st);
     case 4: return launch_fft<N, 4>(g, st);
+    case 8: return launch_fft<N, 8>(g, st);
     case 0: return launch_fft<N, xc_radix(N)>(g, st);
     default: return PIVAMD_ERR_ARG;
     }
@@ -1406,12 +1422,12 @@ static bool variant_ok(int N, int v)
     if (!pivamd_size_supported(N, N)) return false;
     if (v == 0) return true;
     if (v == kVariantMfma) return N % 16 == 0 && N <= 64;
-    return (v == 1 || v == 2 || v == 4) && xc_radix_built(N, v);
+    return (v == 1 || v == 2 || v == 4 || v == 8) && xc_radix_built(N, v);
 }
 
 extern "C" int pivamd_xcorr_variants(int N, int *out, int max_out)
 {
-    static const int cand[] = {1, 2, 4, kVariantMfma};
+    static const int cand[] = {1, 2, 4, 8, kVariantMfma};
     int n = 0;
     for (int v : cand)
         if (variant_ok(N, v)) {
