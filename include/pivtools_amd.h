@@ -165,8 +165,9 @@ int pivamd_set_dft_mode(int mode);
 
 /* Launch variants of the correlation kernel per square window size N (no
  * counterpart in the reference, whose C path has one build per size):
- * R = 1, 2, 4 is the FP32 FFT with each line transform shared by R lanes,
- * 16 the split-precision MFMA DFT.  _variants writes up to max_out codes
+ * R = 1, 2, 4, 8 is the FP32 FFT with each line transform shared by R
+ * lanes, 16 the split-precision MFMA DFT, 36 the 4-lane FFT as a persistent
+ * kernel that prefetches the next window (N >= 64).  _variants writes up to max_out codes
  * and returns how many are built; _set_variant selects one for N
  * (0 = default, process-wide, like bulkxcorr2d_set_peakfit_impl) and
  * returns the previous selection, or -1 if v is not built for N;

@@ -63,8 +63,11 @@ constexpr bool xc_radix_built(int N, int R)
 }
 
 // Variant codes of pivamd_xcorr_set_variant: 0 = default, R = FP32 FFT with
-// R lanes per line, kVariantMfma = split-precision MFMA DFT.
+// R lanes per line, kVariantMfma = split-precision MFMA DFT, kVariantPersist
+// + 4 = the persistent prefetching form of the 4-lane FFT (N >= 64, where a
+// workgroup holds one window).
 constexpr int kVariantMfma = 16;
+constexpr int kVariantPersist = 32;
 
 // Tile layout of the complex stages.  R = 1: element (r, c) at r*(N+1) + c.
 // R > 1: at r*TPX + SKEW*(r/N1) + c + PADC*(c/N1), i.e. rows padded to TPX,
@@ -617,77 +620,20 @@ __device__ __forceinline__ void plane_epilogue(const XcorrArgs &g, const float *
     }
 }
 
-template <int N, int R_>
-__global__ void __launch_bounds__((XcCfg<N, R_>::BLOCK))
-xcorr_kernel(XcorrArgs g)
+// Everything of one window after its weighted values are in registers:
+// mean removal, energy, the four transform stages and the epilogue.  zr/zi
+// hold the weighted A and B column samples (rows n1*R + q of column l), sa/sb
+// this lane's partial sums of them.  Called by all lanes of the workgroup
+// (it has barriers).
+template <int N, int R>
+__device__ __forceinline__ void xc_window(const XcorrArgs &g, float (&zr)[N / R], float (&zi)[N / R], float sa,
+                                          float sb, bool valid, long long gw, int n, int w, int t, int l, int q,
+                                          int ob, int lb, const float2 *tw, float *Sr, float *Si, float *r0,
+                                          float *r1, float *r2, float *r3, float *r4)
 {
-    using C = XcCfg<N, R_>;
-    constexpr int R = C::R, N1 = C::N1, T = C::T, PX = C::PX, WPB = C::WPB;
+    using C = XcCfg<N, R>;
+    constexpr int N1 = C::N1, T = C::T, PX = C::PX;
     constexpr int numel = N * N;
-    __shared__ float s_re[WPB * C::tile];
-    __shared__ float s_im[WPB * C::tile];
-    __shared__ float s_red[5][WPB * C::RED];
-    __shared__ __attribute__((aligned(16))) float2 s_tw[R > 1 ? R * C::TWP : 1];   // split twiddles
-
-    const int b = remap_block((int)blockIdx.x, g.nblocks);
-    if (b < 0) return;    // whole block exits together
-    if constexpr (R > 1) {
-        for (int e = (int)threadIdx.x; e < N; e += C::BLOCK)
-            s_tw[(e / N1) * C::TWP + e % N1] = make_float2(kSplitTw<N, R>.v[2 * e], kSplitTw<N, R>.v[2 * e + 1]);
-        __syncthreads();
-    }
-
-    const int wloc = (int)threadIdx.x / T;
-    const int t = (int)threadIdx.x - wloc * T;
-    int l, q;                         // line (row or column), lane within the line
-    C::lane_map(t, l, q);
-    const int ob = out_block<R>(q);   // output block of fft_split
-    const int lb = l / N1;            // block of this lane's line index
-    const float2 *tw = s_tw + (R > 1 ? q * C::TWP : 0);
-    float *Sr = s_re + wloc * C::tile;
-    float *Si = s_im + wloc * C::tile;
-    float *r0 = &s_red[0][wloc * C::RED], *r1 = &s_red[1][wloc * C::RED];
-    float *r2 = &s_red[2][wloc * C::RED], *r3 = &s_red[3][wloc * C::RED];
-    float *r4 = &s_red[4][wloc * C::RED];
-
-    const int nWin = g.nWy * g.nWx;
-    const long long gw = (long long)b * WPB + wloc;
-    int n = 0, w = 0, row0 = 0, col0 = 0;
-    bool valid = gw < g.total;
-    if (valid) {
-        n = (int)(gw / nWin);
-        w = (int)(gw - (long long)n * nWin);
-        const int ii = w % g.nWx, jj = w / g.nWx;
-        if (g.mask && g.mask[jj * g.nWx + ii] == 1.0f) valid = false;
-        row0 = (int)floor((double)g.ctrY[jj] - ((double)(float)N - 1.0) / 2.0 + 0.5);
-        col0 = (int)floor((double)g.ctrX[ii] - ((double)(float)N - 1.0) / 2.0 + 0.5);
-        if (row0 < 0 || col0 < 0 || row0 + N > g.H || col0 + N > g.W) valid = false;
-    }
-
-    // ---- 1. column x = l: rows n1*R + q, weight, mean, energy, FFT --------
-    float zr[N1], zi[N1];
-    float sa = 0.0f, sb = 0.0f;
-    if (valid) {
-        const size_t img = (size_t)n * g.H * g.W + (size_t)(row0 + q) * g.W + col0 + l;
-        const float *wa = g.wA + q * N + l, *wb = g.wB + q * N + l;
-        // 16-bit counts convert exactly, so both inputs give the same bits
-        auto load = [&](const auto *pa, const auto *pb) {
-#pragma unroll
-            for (int k = 0; k < N1; ++k) {
-                const float a = (float)pa[(size_t)k * R * g.W] * wa[k * R * N];
-                const float bb = (float)pb[(size_t)k * R * g.W] * wb[k * R * N];
-                zr[k] = a;
-                zi[k] = bb;
-                sa += a;
-                sb += bb;
-            }
-        };
-        if (g.imgA16) load(g.imgA16 + img, g.imgB16 + img);
-        else load(g.imgA + img, g.imgB + img);
-    } else {
-#pragma unroll
-        for (int k = 0; k < N1; ++k) { zr[k] = 0.0f; zi[k] = 0.0f; }
-    }
     win_sum2<T>(sa, sb, r0, r1, t);
     const float meanA = sa / (float)numel;
     const float meanB = sb / (float)numel;
@@ -809,6 +755,180 @@ xcorr_kernel(XcorrArgs g)
     __syncthreads();
 
     plane_epilogue<N, T>(g, Sr, PX, valid, gw, n, w, enorm, t, r0, r1, r2, r3, r4);
+}
+// Window geometry of global window gw (image n, window w, top-left corner);
+// false when masked, out of the image or past the end.
+template <int N>
+__device__ __forceinline__ bool xc_geom(const XcorrArgs &g, long long gw, int &n, int &w, int &row0, int &col0)
+{
+    n = 0; w = 0; row0 = 0; col0 = 0;
+    if (gw >= g.total) return false;
+    const int nWin = g.nWy * g.nWx;
+    n = (int)(gw / nWin);
+    w = (int)(gw - (long long)n * nWin);
+    const int ii = w % g.nWx, jj = w / g.nWx;
+    bool valid = true;
+    if (g.mask && g.mask[jj * g.nWx + ii] == 1.0f) valid = false;
+    row0 = (int)floor((double)g.ctrY[jj] - ((double)(float)N - 1.0) / 2.0 + 0.5);
+    col0 = (int)floor((double)g.ctrX[ii] - ((double)(float)N - 1.0) / 2.0 + 0.5);
+    if (row0 < 0 || col0 < 0 || row0 + N > g.H || col0 + N > g.W) valid = false;
+    return valid;
+}
+
+template <int N, int R_>
+__global__ void __launch_bounds__((XcCfg<N, R_>::BLOCK))
+xcorr_kernel(XcorrArgs g)
+{
+    using C = XcCfg<N, R_>;
+    constexpr int R = C::R, N1 = C::N1, T = C::T, WPB = C::WPB;
+    __shared__ float s_re[WPB * C::tile];
+    __shared__ float s_im[WPB * C::tile];
+    __shared__ float s_red[5][WPB * C::RED];
+    __shared__ __attribute__((aligned(16))) float2 s_tw[R > 1 ? R * C::TWP : 1];   // split twiddles
+
+    const int b = remap_block((int)blockIdx.x, g.nblocks);
+    if (b < 0) return;    // whole block exits together
+    if constexpr (R > 1) {
+        for (int e = (int)threadIdx.x; e < N; e += C::BLOCK)
+            s_tw[(e / N1) * C::TWP + e % N1] = make_float2(kSplitTw<N, R>.v[2 * e], kSplitTw<N, R>.v[2 * e + 1]);
+        __syncthreads();
+    }
+
+    const int wloc = (int)threadIdx.x / T;
+    const int t = (int)threadIdx.x - wloc * T;
+    int l, q;                         // line (row or column), lane within the line
+    C::lane_map(t, l, q);
+    const int ob = out_block<R>(q);   // output block of fft_split
+    const int lb = l / N1;            // block of this lane's line index
+    const float2 *tw = s_tw + (R > 1 ? q * C::TWP : 0);
+    float *Sr = s_re + wloc * C::tile;
+    float *Si = s_im + wloc * C::tile;
+    float *r0 = &s_red[0][wloc * C::RED], *r1 = &s_red[1][wloc * C::RED];
+    float *r2 = &s_red[2][wloc * C::RED], *r3 = &s_red[3][wloc * C::RED];
+    float *r4 = &s_red[4][wloc * C::RED];
+
+    const long long gw = (long long)b * WPB + wloc;
+    int n, w, row0, col0;
+    const bool valid = xc_geom<N>(g, gw, n, w, row0, col0);
+
+    // ---- 1. column x = l: rows n1*R + q, weight -----------------------------
+    float zr[N1], zi[N1];
+    float sa = 0.0f, sb = 0.0f;
+    if (valid) {
+        const size_t img = (size_t)n * g.H * g.W + (size_t)(row0 + q) * g.W + col0 + l;
+        const float *wa = g.wA + q * N + l, *wb = g.wB + q * N + l;
+        // 16-bit counts convert exactly, so both inputs give the same bits
+        auto load = [&](const auto *pa, const auto *pb) {
+#pragma unroll
+            for (int k = 0; k < N1; ++k) {
+                const float a = (float)pa[(size_t)k * R * g.W] * wa[k * R * N];
+                const float bb = (float)pb[(size_t)k * R * g.W] * wb[k * R * N];
+                zr[k] = a;
+                zi[k] = bb;
+                sa += a;
+                sb += bb;
+            }
+        };
+        if (g.imgA16) load(g.imgA16 + img, g.imgB16 + img);
+        else load(g.imgA + img, g.imgB + img);
+    } else {
+#pragma unroll
+        for (int k = 0; k < N1; ++k) { zr[k] = 0.0f; zi[k] = 0.0f; }
+    }
+    xc_window<N, R>(g, zr, zi, sa, sb, valid, gw, n, w, t, l, q, ob, lb, tw, Sr, Si, r0, r1, r2, r3, r4);
+}
+
+// Persistent form for one window per workgroup (WPB = 1): each workgroup
+// walks a contiguous run of windows (runs of one XCD are adjacent) and
+// loads the next window's raw samples into registers while the current one
+// goes through its transform stages, so the global-load latency of stage 1
+// is hidden behind LDS and VALU work.  TI: float or 16-bit frames (the raw
+// samples stay in their load registers until the next iteration converts
+// them).  The arithmetic per window is that of xcorr_kernel.
+template <int N, int R_, typename TI>
+__global__ void __launch_bounds__((XcCfg<N, R_>::BLOCK))
+xcorr_kernel_pf(XcorrArgs g)
+{
+    using C = XcCfg<N, R_>;
+    constexpr int R = C::R, N1 = C::N1;
+    static_assert(C::WPB == 1, "one window per workgroup");
+    __shared__ float s_re[C::tile];
+    __shared__ float s_im[C::tile];
+    __shared__ float s_red[5][C::RED];
+    __shared__ __attribute__((aligned(16))) float2 s_tw[R > 1 ? R * C::TWP : 1];
+
+    if constexpr (R > 1) {
+        for (int e = (int)threadIdx.x; e < N; e += C::BLOCK)
+            s_tw[(e / N1) * C::TWP + e % N1] = make_float2(kSplitTw<N, R>.v[2 * e], kSplitTw<N, R>.v[2 * e + 1]);
+        __syncthreads();
+    }
+    const int t = (int)threadIdx.x;
+    int l, q;
+    C::lane_map(t, l, q);
+    const int ob = out_block<R>(q);
+    const int lb = l / N1;
+    const float2 *tw = s_tw + (R > 1 ? q * C::TWP : 0);
+
+    // contiguous run of windows of this workgroup; runs of one XCD adjacent
+    const int P = (int)gridDim.x;
+    const int pidx = ((int)blockIdx.x & 7) * (P >> 3) + ((int)blockIdx.x >> 3);
+    const long long per = (g.total + P - 1) / P;
+    long long gw = (long long)pidx * per;
+    const long long gend = gw + per < g.total ? gw + per : g.total;
+    if (gw >= gend) return;    // whole block
+
+    const TI *srcA = nullptr, *srcB = nullptr;
+    if constexpr (sizeof(TI) == 2) { srcA = (const TI *)g.imgA16; srcB = (const TI *)g.imgB16; }
+    else { srcA = (const TI *)g.imgA; srcB = (const TI *)g.imgB; }
+    const float *wa = g.wA + q * N + l, *wb = g.wB + q * N + l;
+
+    TI ra[N1], rb[N1];     // raw samples of the window in flight
+    int n, w, row0, col0;
+    bool valid = xc_geom<N>(g, gw, n, w, row0, col0);
+    auto fetch = [&](bool v, int n_, int r0_, int c0_) {
+        if (v) {
+            const size_t img = (size_t)n_ * g.H * g.W + (size_t)(r0_ + q) * g.W + c0_ + l;
+#pragma unroll
+            for (int k = 0; k < N1; ++k) {
+                ra[k] = srcA[img + (size_t)k * R * g.W];
+                rb[k] = srcB[img + (size_t)k * R * g.W];
+            }
+        }
+    };
+    fetch(valid, n, row0, col0);
+    for (; gw < gend; ++gw) {
+        float zr[N1], zi[N1];
+        float sa = 0.0f, sb = 0.0f;
+        // An opaque zero added to the lane indices and weight pointers keeps
+        // their address arithmetic inside the loop: hoisted out of it, the
+        // per-element addresses would stay live across the loop and spill.
+        int z0;
+        asm volatile("v_mov_b32 %0, 0" : "=v"(z0));
+        if (valid) {
+            const float *wai = wa + z0, *wbi = wb + z0;
+#pragma unroll
+            for (int k = 0; k < N1; ++k) {
+                const float a = (float)ra[k] * wai[k * R * N];
+                const float bb = (float)rb[k] * wbi[k * R * N];
+                zr[k] = a;
+                zi[k] = bb;
+                sa += a;
+                sb += bb;
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < N1; ++k) { zr[k] = 0.0f; zi[k] = 0.0f; }
+        }
+        const bool cur_valid = valid;
+        const int cur_n = n, cur_w = w;
+        if (gw + 1 < gend) {
+            valid = xc_geom<N>(g, gw + 1, n, w, row0, col0);
+            fetch(valid, n, row0, col0);
+        }
+        xc_window<N, R>(g, zr, zi, sa, sb, cur_valid, gw, cur_n, cur_w, t + z0, l + z0, q + z0, ob + z0,
+                        lb + z0, tw + z0, s_re + z0, s_im + z0, s_red[0], s_red[1], s_red[2], s_red[3], s_red[4]);
+        __syncthreads();    // the next window overwrites the plane and the reduction slots
+    }
 }
 
 #ifdef PIVAMD_XCORR_COMMON
@@ -1308,11 +1428,47 @@ static int launch_fft(XcorrArgs g, hipStream_t st)
     }
 }
 
+template <int N, int R>
+static int launch_pf(XcorrArgs g, hipStream_t st)
+{
+    if constexpr (!xc_radix_built(N, R) || XcCfg<N, R>::WPB != 1) {
+        return PIVAMD_ERR_ARG;
+    } else {
+        using C = XcCfg<N, R>;
+        if (g.total <= 0) return 0;
+        static int occ[2] = {0, 0}, cus = 0;
+        auto kf = xcorr_kernel_pf<N, R, float>;
+        auto ku = xcorr_kernel_pf<N, R, unsigned short>;
+        const int which = g.imgA16 ? 1 : 0;
+        if (!cus) {
+            int dev = 0;
+            if (hipGetDevice(&dev) != hipSuccess ||
+                hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+                return PIVAMD_ERR_RUNTIME;
+        }
+        if (!occ[which]) {
+            int o = 0;
+            const hipError_t e = which ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, ku, C::BLOCK, 0)
+                                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, kf, C::BLOCK, 0);
+            if (e != hipSuccess || o < 1) return PIVAMD_ERR_RUNTIME;
+            occ[which] = o;
+        }
+        long long P = (long long)cus * occ[which];
+        if (P > g.total) P = g.total;
+        P = (P + 7) / 8 * 8;
+        g.nblocks = (int)P;
+        if (which) hipLaunchKernelGGL(ku, dim3((unsigned)P), dim3(C::BLOCK), 0, st, g);
+        else hipLaunchKernelGGL(kf, dim3((unsigned)P), dim3(C::BLOCK), 0, st, g);
+        return hipGetLastError() == hipSuccess ? 0 : PIVAMD_ERR_LAUNCH;
+    }
+}
+
 template <>
 int launch_main<PIVAMD_XCORR_SIZE>(XcorrArgs g, hipStream_t st)
 {
     constexpr int N = PIVAMD_XCORR_SIZE;
     switch (g.variant) {
+    case kVariantPersist + 4: return launch_pf<N, 4>(g, st);
 #if (PIVAMD_XCORR_SIZE % 16 == 0) && (PIVAMD_XCORR_SIZE <= 64)
     case kVariantMfma: return launch_mfma(g, st);
 #endif
@@ -1422,12 +1578,13 @@ static bool variant_ok(int N, int v)
     if (!pivamd_size_supported(N, N)) return false;
     if (v == 0) return true;
     if (v == kVariantMfma) return N % 16 == 0 && N <= 64;
+    if (v == kVariantPersist + 4) return xc_radix_built(N, 4) && N * 4 >= 256;   // WPB = 1
     return (v == 1 || v == 2 || v == 4 || v == 8) && xc_radix_built(N, v);
 }
 
 extern "C" int pivamd_xcorr_variants(int N, int *out, int max_out)
 {
-    static const int cand[] = {1, 2, 4, 8, kVariantMfma};
+    static const int cand[] = {1, 2, 4, 8, kVariantMfma, kVariantPersist + 4};
     int n = 0;
     for (int v : cand)
         if (variant_ok(N, v)) {

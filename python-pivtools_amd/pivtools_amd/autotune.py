@@ -8,6 +8,9 @@ the library (``pivamd_xcorr_variants``):
   more latency hiding, at the cost of cross-lane exchange and twiddle work;
   it also changes the LDS tile layout and the windows per workgroup.
 * ``16``: the split-precision MFMA DFT form (16, 32, 48 and 64 points).
+* ``36``: the 4-lane FFT as a persistent kernel (64, 96 and 128 points, one
+  window per workgroup) that loads the next window's samples while the
+  current one is transformed.
 
 Which one is fastest depends on the device and the size, so the library
 lets the caller pick per size (``pivamd_xcorr_set_variant``, process-wide,
@@ -28,7 +31,6 @@ from __future__ import annotations
 import ctypes
 import json
 import os
-import time
 
 import numpy as np
 
@@ -36,6 +38,7 @@ from . import _native
 from .window import compute_window_centers, window_weight
 
 MFMA = 16
+PERSIST = 32
 _TARGET_WINDOWS = {8: 262144, 12: 262144, 16: 262144, 24: 196608, 32: 131072, 48: 65536, 64: 49152,
                    96: 20480, 128: 16384}
 

@@ -10,8 +10,8 @@ import pytest
 from pivtools_amd import _native
 from pivtools_amd import autotune as AT
 
-BUILT = {8: [1], 12: [1], 16: [1, 2, 16], 24: [1, 2], 32: [1, 2, 4, 16], 48: [2, 4, 16], 64: [2, 4, 16],
-         96: [4], 128: [4, 8]}
+BUILT = {8: [1], 12: [1], 16: [1, 2, 16], 24: [1, 2], 32: [1, 2, 4, 16], 48: [2, 4, 16], 64: [2, 4, 16, 36],
+         96: [4, 36], 128: [4, 8, 36]}
 DEFAULT = {8: 1, 12: 1, 16: 1, 24: 1, 32: 1, 48: 2, 64: 4, 96: 4, 128: 4}
 
 
