@@ -580,43 +580,61 @@ __device__ __forceinline__ void plane_epilogue(const XcorrArgs &g, const float *
             }
         }
     }
-    if (t == 0 && gw < g.total) {
-        float *rec = g.rec + (size_t)gw * kRec;
-        if (!valid) {
-            rec[kRecFlag] = 0.0f;
-        } else {
-            float h = cd.v, det = (cd.p2 > 0.0f) ? cd.p1 / cd.p2 : 0.0f;
-            int i0 = 0, j0 = 0;
-            if (cd.c != 0x7fffffff) { i0 = N / 4 + cd.c / CW; j0 = N / 4 + cd.c % CW; }
-            if (cd.nan) {
-                seq_search(P, N, N, PX, &h, &i0, &j0);
-                det = seq_detect(P, N, N, PX);
-            }
-            g.ratio[(size_t)n * nWin + w] = det;
-            const bool gate = (h <= 0.0f) || i0 < 2 || i0 >= N - 2 || j0 < 2 || j0 >= N - 2;
-            if (gate) {
-                const size_t o = (size_t)n * nWin + w;
-                g.pkX[o] = __builtin_nanf("") - N / 2.0f;
-                g.pkY[o] = __builtin_nanf("") - N / 2.0f;
-                g.sx[o] = 0.0f; g.sy[o] = 0.0f; g.sxy[o] = 0.0f;
-                g.pkH[o] = 0.0f * enorm;
-                rec[kRecFlag] = 0.0f;
-            } else {
-#pragma unroll
-                for (int i = 0; i < PK; ++i)
-#pragma unroll
-                    for (int j = 0; j < PK; ++j) {
-                        const int r = i0 + i - 2, c = j0 + j - 2;
-                        const float v = P[r * PX + c];
-                        rec[i * PK + j] = g.pw ? v * g.pw[r * N + c] : v;
-                    }
-                rec[kRecH] = h;
-                rec[kRecI0] = (float)i0;
-                rec[kRecJ0] = (float)j0;
-                rec[kRecEnorm] = enorm;
-                rec[kRecFlag] = 1.0f;
-            }
+    // every lane of the window takes the merged summary (the shuffle tree
+    // already gave it to all lanes when T | 64)
+    if constexpr (64 % T != 0) {
+        if (t == 0) {
+            r0[0] = cd.v;
+            r1[0] = __int_as_float(cd.c);
+            r2[0] = cd.p1;
+            r3[0] = cd.p2;
+            r4[0] = (float)cd.nan;
         }
+        __syncthreads();
+        cd = Cand{r0[0], __float_as_int(r1[0]), r2[0], r3[0], r4[0] != 0.0f};
+    }
+    if (gw >= g.total) return;
+    float *rec = g.rec + (size_t)gw * kRec;
+    if (!valid) {
+        if (t == 0) rec[kRecFlag] = 0.0f;
+        return;
+    }
+    float h = cd.v, det = (cd.p2 > 0.0f) ? cd.p1 / cd.p2 : 0.0f;
+    int i0 = 0, j0 = 0;
+    if (cd.c != 0x7fffffff) { i0 = N / 4 + cd.c / CW; j0 = N / 4 + cd.c % CW; }
+    if (cd.nan) {
+        // a NaN in the plane: the sequential scan decides (rare, one lane)
+        if (t != 0) return;
+        seq_search(P, N, N, PX, &h, &i0, &j0);
+        det = seq_detect(P, N, N, PX);
+    }
+    const bool gate = (h <= 0.0f) || i0 < 2 || i0 >= N - 2 || j0 < 2 || j0 >= N - 2;
+    const size_t o = (size_t)n * nWin + w;
+    if (gate) {
+        if (t == 0) {
+            g.ratio[o] = det;
+            g.pkX[o] = __builtin_nanf("") - N / 2.0f;
+            g.pkY[o] = __builtin_nanf("") - N / 2.0f;
+            g.sx[o] = 0.0f; g.sy[o] = 0.0f; g.sxy[o] = 0.0f;
+            g.pkH[o] = 0.0f * enorm;
+            rec[kRecFlag] = 0.0f;
+        }
+        return;
+    }
+    // the 5x5 patch times the plane weight, spread over the window's lanes
+    const int nl = cd.nan ? 1 : T;
+    for (int e = t; e < PK * PK; e += nl) {
+        const int r = i0 + e / PK - 2, c = j0 + e % PK - 2;
+        const float v = P[r * PX + c];
+        rec[e] = g.pw ? v * g.pw[r * N + c] : v;
+    }
+    if (t == 0) {
+        g.ratio[o] = det;
+        rec[kRecH] = h;
+        rec[kRecI0] = (float)i0;
+        rec[kRecJ0] = (float)j0;
+        rec[kRecEnorm] = enorm;
+        rec[kRecFlag] = 1.0f;
     }
 }
 
