@@ -529,21 +529,32 @@ __device__ __forceinline__ void plane_epilogue(const XcorrArgs &g, const float *
     constexpr int CW = N / 2, NC = (N / 2) * (N / 2);
     Cand cd = {0.0f, 0x7fffffff, 0.0f, 0.0f, 0};
     if (valid) {
-        for (int c = t; c < NC; c += T) {
+        // Branch-free and unrolled, so the nine LDS reads of every candidate
+        // issue together.  A neighbour counts only if it is > v; fmaxf skips
+        // NaN neighbours, which a > test never counts either.  Candidates are
+        // visited in increasing c, with the same strict updates as the
+        // reference's loop (first maximum wins, top-2 for the ratio).
+#pragma unroll
+        for (int it = 0; it < (NC + T - 1) / T; ++it) {
+            const int c = t + it * T;
+            if constexpr (NC % T != 0) {
+                if (c >= NC) break;
+            }
             const int i = N / 4 + c / CW, j = N / 4 + c % CW;
-            const float v = P[i * PX + j];
-            if (v != v) { cd.nan = 1; continue; }
-            if (!(v > 0.0f)) continue;
-            bool is_max = true;
-#pragma unroll
-            for (int di = -1; di <= 1; ++di)
-#pragma unroll
-                for (int dj = -1; dj <= 1; ++dj)
-                    if (P[(i + di) * PX + (j + dj)] > v) is_max = false;
-            if (!is_max) continue;
-            if (v > cd.v) { cd.v = v; cd.c = c; }
-            if (v > cd.p1) { cd.p2 = cd.p1; cd.p1 = v; }
-            else if (v > cd.p2) cd.p2 = v;
+            const float *pc = P + i * PX + j;
+            const float v = pc[0];
+            const float m = fmaxf(fmaxf(fmaxf(pc[-PX - 1], pc[-PX]), fmaxf(pc[-PX + 1], pc[-1])),
+                                  fmaxf(fmaxf(pc[1], pc[PX - 1]), fmaxf(pc[PX], pc[PX + 1])));
+            const bool vnan = v != v;
+            cd.nan = cd.nan || vnan;
+            const bool cand = !vnan && v > 0.0f && !(m > v);
+            const bool best = cand && v > cd.v;
+            cd.v = best ? v : cd.v;
+            cd.c = best ? c : cd.c;
+            const bool g1 = cand && v > cd.p1;
+            const bool g2 = cand && !g1 && v > cd.p2;
+            cd.p2 = g1 ? cd.p1 : (g2 ? v : cd.p2);
+            cd.p1 = g1 ? v : cd.p1;
         }
     }
     if constexpr (64 % T == 0) {
