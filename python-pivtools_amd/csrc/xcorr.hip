@@ -654,15 +654,22 @@ __device__ __forceinline__ void plane_epilogue(const XcorrArgs &g, const float *
 // hold the weighted A and B column samples (rows n1*R + q of column l), sa/sb
 // this lane's partial sums of them.  Called by all lanes of the workgroup
 // (it has barriers).
+// With one lane per line (R = 1) and several windows per workgroup, the
+// inverse stages run packed: stage 3 has N/2 + 1 columns and stage 4 N/2 row
+// pairs per window, so the workgroup's lanes take them window after window
+// (sv: per-window valid flags in LDS; Sbr/Sbi: the workgroup's tiles) and the
+// waves left without work skip those stages.
 template <int N, int R>
 __device__ __forceinline__ void xc_window(const XcorrArgs &g, float (&zr)[N / R], float (&zi)[N / R], float sa,
                                           float sb, bool valid, long long gw, int n, int w, int t, int l, int q,
                                           int ob, int lb, const float2 *tw, float *Sr, float *Si, float *r0,
-                                          float *r1, float *r2, float *r3, float *r4)
+                                          float *r1, float *r2, float *r3, float *r4, float *Sbr = nullptr,
+                                          float *Sbi = nullptr, int *sv = nullptr, int wloc = 0)
 {
     using C = XcCfg<N, R>;
     constexpr int N1 = C::N1, T = C::T, PX = C::PX;
     constexpr int numel = N * N;
+    constexpr bool kPack = R == 1 && C::WPB > 1;
     win_sum2<T>(sa, sb, r0, r1, t);
     const float meanA = sa / (float)numel;
     const float meanB = sb / (float)numel;
@@ -696,6 +703,9 @@ __device__ __forceinline__ void xc_window(const XcorrArgs &g, float (&zr)[N / R]
             Si[e] = zi[k];
         }
     }
+    if constexpr (kPack) {
+        if (t == 0) sv[wloc] = valid ? 1 : 0;
+    }
     __syncthreads();
 
     // ---- 2. row ky = l --------------------------------------------------------
@@ -718,7 +728,7 @@ __device__ __forceinline__ void xc_window(const XcorrArgs &g, float (&zr)[N / R]
     __syncthreads();
 
     // ---- 3. column kx = l <= N/2: C = FB conj(FA), inverse column FFT ------
-    if (valid && l <= N / 2) {
+    auto stage3 = [&](int l, float *Sr, float *Si) {
         const int mx = (N - l) % N, mxb = mx / N1;
         float cr[N1], ci[N1];
 #pragma unroll
@@ -743,14 +753,22 @@ __device__ __forceinline__ void xc_window(const XcorrArgs &g, float (&zr)[N / R]
             Sr[e] = cr[k];
             Si[e] = ci[k];
         }
+    };
+    if constexpr (kPack) {
+        constexpr int C3 = N / 2 + 1;
+        const int v = (int)threadIdx.x;
+        if (v < C::WPB * C3) {
+            const int wl = v / C3;
+            if (sv[wl]) stage3(v - wl * C3, Sbr + wl * C::tile, Sbi + wl * C::tile);
+        }
+    } else {
+        if (valid && l <= N / 2) stage3(l, Sr, Si);
     }
     __syncthreads();
 
     // ---- 4. rows j and j + N/2 (j = l < N/2): Hermitian rows -> real rows -----
-    const bool act4 = valid && l < N / 2;
-    const int ya = l, yb = l + N / 2;
-    float xr[N1], xi[N1];
-    if (act4) {
+    auto load4 = [&](int l, const float *Sr, const float *Si, float (&xr)[N1], float (&xi)[N1]) {
+        const int ya = l, yb = l + N / 2;
 #pragma unroll
         for (int k = 0; k < N1; ++k) {
             // kx = k R + q; with 0 <= q < R the tests below fold to compile
@@ -767,10 +785,9 @@ __device__ __forceinline__ void xc_window(const XcorrArgs &g, float (&zr)[N / R]
             xr[k] = g1r - g2i;
             xi[k] = g1i + g2r;
         }
-    }
-    // R > 1: the plane (pitch PX) overlays tile words other lines still read
-    if constexpr (R > 1) __syncthreads();
-    if (act4) {
+    };
+    auto store4 = [&](int l, float *Sr, float (&xr)[N1], float (&xi)[N1]) {
+        const int ya = l, yb = l + N / 2;
         fft_split<N, R, true>(xr, xi, q, tw);
         const float mul = 1.0f / (float)numel;
         // plane[(y + N/2) % N][(x + N/2) % N] = c[y][x] / numel
@@ -780,6 +797,27 @@ __device__ __forceinline__ void xc_window(const XcorrArgs &g, float (&zr)[N / R]
             Sr[yb * PX + xs] = xr[k] * mul;
             Sr[ya * PX + xs] = xi[k] * mul;
         }
+    };
+    if constexpr (kPack) {
+        // R = 1: each row pair is read and written by the same lane (the
+        // plane overlays only that lane's two rows), so no barrier between
+        constexpr int C4 = N / 2;
+        const int v = (int)threadIdx.x;
+        if (v < C::WPB * C4) {
+            const int wl = v / C4;
+            if (sv[wl]) {
+                float xr[N1], xi[N1];
+                load4(v - wl * C4, Sbr + wl * C::tile, Sbi + wl * C::tile, xr, xi);
+                store4(v - wl * C4, Sbr + wl * C::tile, xr, xi);
+            }
+        }
+    } else {
+        const bool act4 = valid && l < N / 2;
+        float xr[N1], xi[N1];
+        if (act4) load4(l, Sr, Si, xr, xi);
+        // R > 1: the plane (pitch PX) overlays tile words other lines still read
+        if constexpr (R > 1) __syncthreads();
+        if (act4) store4(l, Sr, xr, xi);
     }
     __syncthreads();
 
@@ -836,6 +874,7 @@ xcorr_kernel(XcorrArgs g)
     float *r2 = &s_red[2][wloc * C::RED], *r3 = &s_red[3][wloc * C::RED];
     float *r4 = &s_red[4][wloc * C::RED];
 
+    __shared__ int s_valid[WPB];
     const long long gw = (long long)b * WPB + wloc;
     int n, w, row0, col0;
     const bool valid = xc_geom<N>(g, gw, n, w, row0, col0);
@@ -864,7 +903,8 @@ xcorr_kernel(XcorrArgs g)
 #pragma unroll
         for (int k = 0; k < N1; ++k) { zr[k] = 0.0f; zi[k] = 0.0f; }
     }
-    xc_window<N, R>(g, zr, zi, sa, sb, valid, gw, n, w, t, l, q, ob, lb, tw, Sr, Si, r0, r1, r2, r3, r4);
+    xc_window<N, R>(g, zr, zi, sa, sb, valid, gw, n, w, t, l, q, ob, lb, tw, Sr, Si, r0, r1, r2, r3, r4, s_re,
+                    s_im, s_valid, wloc);
 }
 
 // Persistent form for one window per workgroup (WPB = 1): each workgroup

@@ -164,9 +164,10 @@ def _device_key(device) -> str:
 
 
 def tune_xcorr(sizes, device=None, reps: int = 10, cache: str | None = None, include_mfma: bool = True,
-               log=None) -> dict:
+               log=None, margin: float = 0.03) -> dict:
     """Measure every variant of each window size, select the fastest that
-    agrees with the default, and return {size: {"variant", "default", "ms",
+    agrees with the default (the default unless one is faster by more than
+    ``margin``), and return {size: {"variant", "default", "ms",
     "windows"}} ("ms": median milliseconds per call, per variant).  With
     ``cache`` (a JSON path), a table stored for this device and library
     build is applied without measuring, and new results are written back."""
@@ -206,6 +207,10 @@ def tune_xcorr(sizes, device=None, reps: int = 10, cache: str | None = None, inc
             set_variant(n, prev)
         good = [v for v in cands if ok[v]] or [dflt]
         best = min(good, key=lambda v: ms.get(v, float("inf")))
+        # leave the default unless a variant is clearly faster: differences
+        # within the timing noise would make the choice flip between runs
+        if dflt in good and ms.get(best, 0.0) > (1.0 - margin) * ms.get(dflt, 0.0):
+            best = dflt
         set_variant(n, best)
         table[n] = {"variant": best, "default": dflt, "windows": case.windows,
                     "ms": {str(v): round(t, 4) for v, t in ms.items()},

@@ -54,7 +54,9 @@ def test_tuner_selects_and_caches(tmp_path):
             assert AT.get_variant(n) == rec["variant"]
             assert set(rec["ms"]) == {str(v) for v in AT.variants(n)}
             assert rec["rejected"] == []                  # every variant reproduces the default
-            assert rec["ms"][str(rec["variant"])] == min(rec["ms"][str(v)] for v in AT.variants(n))
+            fastest = min(rec["ms"][str(v)] for v in AT.variants(n))
+            chosen = rec["ms"][str(rec["variant"])]
+            assert chosen == fastest or (rec["variant"] == rec["default"] and fastest > 0.97 * chosen)
         stored = json.load(open(cache))
         assert len(stored) == 1
         # second call: taken from the cache, no measurement
