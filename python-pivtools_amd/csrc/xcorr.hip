@@ -160,7 +160,9 @@ __device__ __forceinline__ int mirror_block(int j, int q)
 // Sum of (a, b) over the T lanes of one window; every lane gets the same
 // bits.  T | 64: xor butterfly inside the wave.  64 | T: butterfly per wave,
 // then the per-wave partials through LDS in wave order.  Otherwise every
-// lane sums the T partials from LDS in lane order.
+// lane sums the T partials from LDS in lane order.  There is no barrier
+// after the LDS reads: a caller must not write ra/rb again before its next
+// workgroup barrier (consecutive calls use different slots).
 template <int T>
 __device__ __forceinline__ void win_sum2(float &a, float &b, float *ra, float *rb, int t)
 {
@@ -187,7 +189,6 @@ __device__ __forceinline__ void win_sum2(float &a, float &b, float *ra, float *r
             sa += ra[k];
             sb += rb[k];
         }
-        __syncthreads();
         a = sa;
         b = sb;
     } else {
@@ -199,7 +200,6 @@ __device__ __forceinline__ void win_sum2(float &a, float &b, float *ra, float *r
             sa += ra[k];
             sb += rb[k];
         }
-        __syncthreads();
         a = sa;
         b = sb;
     }
@@ -571,11 +571,12 @@ __device__ __forceinline__ void plane_epilogue(const XcorrArgs &g, const float *
             r4[t >> 6] = (float)cd.nan;
         }
         __syncthreads();
-        if (t == 0) {
-            for (int k = 1; k < T / 64; ++k) {
-                Cand o = {r0[k], __float_as_int(r1[k]), r2[k], r3[k], r4[k] != 0.0f};
-                cd.merge(o);
-            }
+        // every lane merges the per-wave summaries in wave order (the same
+        // bits in all lanes, no second barrier)
+        cd = Cand{r0[0], __float_as_int(r1[0]), r2[0], r3[0], r4[0] != 0.0f};
+        for (int k = 1; k < T / 64; ++k) {
+            Cand o = {r0[k], __float_as_int(r1[k]), r2[k], r3[k], r4[k] != 0.0f};
+            cd.merge(o);
         }
     } else {
         r0[t] = cd.v;
@@ -584,25 +585,11 @@ __device__ __forceinline__ void plane_epilogue(const XcorrArgs &g, const float *
         r3[t] = cd.p2;
         r4[t] = (float)cd.nan;
         __syncthreads();
-        if (t == 0) {
-            for (int k = 1; k < T; ++k) {
-                Cand o = {r0[k], __float_as_int(r1[k]), r2[k], r3[k], r4[k] != 0.0f};
-                cd.merge(o);
-            }
-        }
-    }
-    // every lane of the window takes the merged summary (the shuffle tree
-    // already gave it to all lanes when T | 64)
-    if constexpr (64 % T != 0) {
-        if (t == 0) {
-            r0[0] = cd.v;
-            r1[0] = __int_as_float(cd.c);
-            r2[0] = cd.p1;
-            r3[0] = cd.p2;
-            r4[0] = (float)cd.nan;
-        }
-        __syncthreads();
         cd = Cand{r0[0], __float_as_int(r1[0]), r2[0], r3[0], r4[0] != 0.0f};
+        for (int k = 1; k < T; ++k) {
+            Cand o = {r0[k], __float_as_int(r1[k]), r2[k], r3[k], r4[k] != 0.0f};
+            cd.merge(o);
+        }
     }
     if (gw >= g.total) return;
     float *rec = g.rec + (size_t)gw * kRec;
@@ -691,7 +678,7 @@ __device__ __forceinline__ void xc_window(const XcorrArgs &g, float (&zr)[N / R]
             }
         }
     }
-    win_sum2<T>(ea, eb, r0, r1, t);
+    win_sum2<T>(ea, eb, r2, r3, t);     // other slots: the first sum's reads need no barrier
     const float enorm = 1.0f / sqrtf(ea * eb);
 
     if (valid) {
@@ -1355,7 +1342,7 @@ __global__ void __launch_bounds__((MfCfg<N>::BLOCK)) xcorr_mfma_kernel(XcorrArgs
         ea += za[m] * za[m];
         eb += zb[m] * zb[m];
     }
-    win_sum2<T>(ea, eb, r0, r1, t);
+    win_sum2<T>(ea, eb, r2, r3, t);     // other slots: the first sum's reads need no barrier
     const float enorm = 1.0f / sqrtf(ea * eb);
     // scales: sqrt(max energy) -> <= 512 for the window, N^2 sqrt(eA eB) -> <= 512 for C
     // log2 bounds: sqrt(e) < 2^(ilogb + 1); |C| <= N^2 sqrt(eA eB) after the
