@@ -455,7 +455,7 @@ def main():
         "passes": passes,
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": traffic,
-                     "kernel": f"xcorr_kernel<{dom['window']},{dom['window']}> + fit_kernel",
+                     "kernel": f"{_xcorr_kernel_name(dom['window'], args.dft)} + fit_kernel",
                      "note": "FP32 VALU FFT; peak = MI355X FP32 157.3 TFLOP/s"},
         "cpu_baseline": cpu,
         "accuracy": acc,
@@ -463,6 +463,18 @@ def main():
         "autotune": ({str(n): r for n, r in tuned.items()} if tuned else None),
     }
     print(json.dumps(line), flush=True)
+
+
+def _xcorr_kernel_name(n: int, dft: str) -> str:
+    """Name of the correlation kernel that runs for n x n windows."""
+    from pivtools_amd import autotune
+
+    v = autotune.get_variant(n) or autotune.default_variant(n)
+    if (dft != "fft" and n in (16, 32, 48, 64)) or v == autotune.MFMA:
+        return f"xcorr_mfma_kernel<{n}>"
+    if v & autotune.PERSIST:
+        return f"xcorr_kernel_pf<{n},{v & ~autotune.PERSIST}>"
+    return f"xcorr_kernel<{n},{v}>"
 
 
 if __name__ == "__main__":
