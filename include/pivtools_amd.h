@@ -237,6 +237,26 @@ int pivamd_accumulate(const float *d_A, const float *d_B, const float *d_mask, i
                       float *d_outAB, float *d_outAA, float *d_outBB, void *d_work,
                       unsigned long work_bytes, void *stream);
 
+/* Camera payload ingest (device buffers).  The reference decodes frames on
+ * the host (pivtools_core/image_handling/readers/set_reader.py:1296-1350
+ * decode_mono12p, pivraw_reader.py:896-961 _sensor_pixels/_decode_into);
+ * these expand the payload in HBM into uint16 frames for pivamd_xcorr_u16 /
+ * pivamd_warp_u16.  Frame f's payload starts at d_raw + f * frame_stride
+ * (e.g. the .pivraw container's 4096-byte-aligned stride).
+ *  - Mono12p: H*W pixels (H*W even) in H*W*3/2 bytes, PFNC LSB-first packing;
+ *    d_out (n_frames, H, W).
+ *  - BayerRG8: an H x W mosaic (H, W even) summed over 2x2 superpixels;
+ *    d_out (n_frames, H/2, W/2).
+ * d_dark is null or one dark frame in the sensor type (uint16 H x W for
+ * Mono12p, uint8 H x W for BayerRG8) subtracted per photosite as
+ * max(p, d) - d from every frame of the call (the caller passes the dark of
+ * the frames' A/B role).  Returns PIVAMD_ERR_SIZE for an odd pixel count /
+ * odd mosaic, PIVAMD_ERR_ARG for a stride shorter than one payload. */
+int pivamd_unpack_mono12p(const void *d_raw, unsigned long frame_stride, int n_frames, int H, int W,
+                          const unsigned short *d_dark, unsigned short *d_out, void *stream);
+int pivamd_bayer_rg8_superpixel(const void *d_raw, unsigned long frame_stride, int n_frames, int H, int W,
+                                const unsigned char *d_dark, unsigned short *d_out, void *stream);
+
 /* Multipass glue (device buffers, planar fields [dy plane, dx plane]):
  * uint16 ingest, displacement gate + predictor add-back, normalised-median
  * test, nearest infill, edge padding, one axis of the Gaussian predictor

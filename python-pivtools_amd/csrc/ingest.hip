@@ -1,0 +1,210 @@
+// ingest.hip -- camera payloads decoded on the device.
+//
+// The reference decodes raw camera frames on the host before correlating
+// them (pivtools_core/image_handling/readers/set_reader.py:1296-1350
+// decode_mono12p; pivraw_reader.py:896-961 _sensor_pixels / _decode_into):
+//   Mono12p   PFNC LSB-first packing, two 12-bit pixels in three bytes:
+//             pixel j is bits [12j, 12j + 12) of the little-endian byte
+//             stream (p0 = b0 | (b1 & 0x0F) << 8, p1 = b1 >> 4 | b2 << 4);
+//   dark      max(p, d) - d per photosite in the sensor type, so the unsigned
+//             subtraction never wraps;
+//   BayerRG8  the 8-bit mosaic read as 2x2 superpixels, each the sum of its
+//             four photosites after their own dark subtraction.
+// Here the packed payload is what crosses PCIe (1.5 bytes per pixel instead
+// of 2) and these kernels expand it in HBM into the uint16 frames that the
+// correlation and warp kernels read directly.  Both kernels are HBM-bound
+// byte work: each lane moves whole aligned words when the buffers allow it
+// and falls back to byte loads otherwise (odd strides, ragged tails).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pivamd_dev.h"
+
+namespace pivamd {
+
+__device__ __forceinline__ uint32_t sub_dark(uint32_t p, uint32_t d) { return p > d ? p - d : 0u; }
+
+// One lane per group of 8 pixels (12 payload bytes).  Frame f of the batch
+// is blockIdx.y; its payload starts at raw + f * stride.  FAST: raw and
+// stride are 4-byte aligned, out and dark 16-byte aligned and npx % 8 == 0,
+// so a group is three dword loads and one 16-byte store.
+template <bool FAST, bool DARK>
+__global__ void __launch_bounds__(256) mono12p_kernel(const uint8_t *__restrict__ raw, size_t stride, size_t npx,
+                                                      const uint16_t *__restrict__ dark, uint16_t *__restrict__ out)
+{
+    const uint8_t *src = raw + (size_t)blockIdx.y * stride;
+    uint16_t *dst = out + (size_t)blockIdx.y * npx;
+    const size_t groups = (npx + 7) / 8;
+    const size_t step = (size_t)gridDim.x * blockDim.x;
+    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < groups; g += step) {
+        uint32_t p[8];
+        if (FAST) {
+            const uint32_t *s = (const uint32_t *)(src + 12 * g);
+            const uint32_t d0 = s[0], d1 = s[1], d2 = s[2];
+            p[0] = d0 & 0xFFFu;
+            p[1] = (d0 >> 12) & 0xFFFu;
+            p[2] = (d0 >> 24) | ((d1 & 0xFu) << 8);
+            p[3] = (d1 >> 4) & 0xFFFu;
+            p[4] = (d1 >> 16) & 0xFFFu;
+            p[5] = (d1 >> 28) | ((d2 & 0xFFu) << 4);
+            p[6] = (d2 >> 8) & 0xFFFu;
+            p[7] = d2 >> 20;
+            if (DARK) {
+                const uint4 dv = *(const uint4 *)(dark + 8 * g);
+                const uint32_t dw[4] = {dv.x, dv.y, dv.z, dv.w};
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    p[2 * k] = sub_dark(p[2 * k], dw[k] & 0xFFFFu);
+                    p[2 * k + 1] = sub_dark(p[2 * k + 1], dw[k] >> 16);
+                }
+            }
+            uint4 o;
+            o.x = p[0] | (p[1] << 16);
+            o.y = p[2] | (p[3] << 16);
+            o.z = p[4] | (p[5] << 16);
+            o.w = p[6] | (p[7] << 16);
+            *(uint4 *)(dst + 8 * g) = o;
+        } else {
+            // npx is even, so a group holds whole pixel pairs
+            const size_t left = npx - 8 * g;
+            const int cnt = left < 8 ? (int)left : 8;
+            for (int k = 0; k < cnt; k += 2) {
+                const uint8_t *b = src + 12 * g + 3 * (k / 2);
+                const uint32_t b0 = b[0], b1 = b[1], b2 = b[2];
+                uint32_t q0 = b0 | ((b1 & 0xFu) << 8);
+                uint32_t q1 = (b1 >> 4) | (b2 << 4);
+                if (DARK) {
+                    q0 = sub_dark(q0, dark[8 * g + k]);
+                    q1 = sub_dark(q1, dark[8 * g + k + 1]);
+                }
+                dst[8 * g + k] = (uint16_t)q0;
+                dst[8 * g + k + 1] = (uint16_t)q1;
+            }
+        }
+    }
+}
+
+// One lane per 4 superpixels of one output row.  Sensor frame (H, W) of
+// 8-bit photosites at raw + f * stride, output (H/2, W/2) uint16.  FAST: W
+// a multiple of 8, raw, stride and dark 8-byte aligned, out 8-byte aligned.
+template <bool FAST, bool DARK>
+__global__ void __launch_bounds__(256) bayer_rg8_kernel(const uint8_t *__restrict__ raw, size_t stride, int H, int W,
+                                                        const uint8_t *__restrict__ dark, uint16_t *__restrict__ out)
+{
+    const uint8_t *src = raw + (size_t)blockIdx.y * stride;
+    const int ho = H / 2, wo = W / 2;
+    uint16_t *dst = out + (size_t)blockIdx.y * ho * wo;
+    const int per_row = (wo + 3) / 4;
+    const size_t items = (size_t)ho * per_row;
+    const size_t step = (size_t)gridDim.x * blockDim.x;
+    for (size_t it = (size_t)blockIdx.x * blockDim.x + threadIdx.x; it < items; it += step) {
+        const int r = (int)(it / per_row);
+        const int c0 = (int)(it % per_row) * 4;
+        const size_t top = (size_t)(2 * r) * W, bot = top + W;
+        if (FAST) {
+            const uint2 t = *(const uint2 *)(src + top + 2 * c0);
+            const uint2 u = *(const uint2 *)(src + bot + 2 * c0);
+            const uint32_t tw[2] = {t.x, t.y}, uw[2] = {u.x, u.y};
+            uint32_t dtw[2] = {0u, 0u}, duw[2] = {0u, 0u};
+            if (DARK) {
+                const uint2 dt = *(const uint2 *)(dark + top + 2 * c0);
+                const uint2 du = *(const uint2 *)(dark + bot + 2 * c0);
+                dtw[0] = dt.x;
+                dtw[1] = dt.y;
+                duw[0] = du.x;
+                duw[1] = du.y;
+            }
+            uint32_t s[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int w = k >> 1, sh = (k & 1) * 16;
+                uint32_t a0 = (tw[w] >> sh) & 0xFFu, a1 = (tw[w] >> (sh + 8)) & 0xFFu;
+                uint32_t c0v = (uw[w] >> sh) & 0xFFu, c1v = (uw[w] >> (sh + 8)) & 0xFFu;
+                if (DARK) {
+                    a0 = sub_dark(a0, (dtw[w] >> sh) & 0xFFu);
+                    a1 = sub_dark(a1, (dtw[w] >> (sh + 8)) & 0xFFu);
+                    c0v = sub_dark(c0v, (duw[w] >> sh) & 0xFFu);
+                    c1v = sub_dark(c1v, (duw[w] >> (sh + 8)) & 0xFFu);
+                }
+                s[k] = a0 + a1 + c0v + c1v;
+            }
+            uint2 o;
+            o.x = s[0] | (s[1] << 16);
+            o.y = s[2] | (s[3] << 16);
+            *(uint2 *)(dst + (size_t)r * wo + c0) = o;
+        } else {
+            const int cnt = wo - c0 < 4 ? wo - c0 : 4;
+            for (int k = 0; k < cnt; ++k) {
+                const size_t x = 2 * (size_t)(c0 + k);
+                uint32_t a0 = src[top + x], a1 = src[top + x + 1], b0 = src[bot + x], b1 = src[bot + x + 1];
+                if (DARK) {
+                    a0 = sub_dark(a0, dark[top + x]);
+                    a1 = sub_dark(a1, dark[top + x + 1]);
+                    b0 = sub_dark(b0, dark[bot + x]);
+                    b1 = sub_dark(b1, dark[bot + x + 1]);
+                }
+                dst[(size_t)r * wo + c0 + k] = (uint16_t)(a0 + a1 + b0 + b1);
+            }
+        }
+    }
+}
+
+static inline bool aligned(const void *p, size_t a) { return ((uintptr_t)p % a) == 0; }
+
+static inline unsigned grid_x(size_t items)
+{
+    size_t g = (items + 255) / 256;
+    return (unsigned)(g > 4096 ? 4096 : (g ? g : 1));
+}
+
+}  // namespace pivamd
+
+using namespace pivamd;
+
+#define PIVAMD_LAUNCHED() (hipGetLastError() == hipSuccess ? 0 : PIVAMD_ERR_LAUNCH)
+
+extern "C" int pivamd_unpack_mono12p(const void *d_raw, size_t frame_stride, int n_frames, int H, int W,
+                                     const uint16_t *d_dark, uint16_t *d_out, hipStream_t st)
+{
+    if (n_frames < 0 || H < 0 || W < 0 || n_frames > 65535) return PIVAMD_ERR_ARG;
+    const size_t npx = (size_t)H * W;
+    if (n_frames == 0 || npx == 0) return 0;
+    if (npx % 2) return PIVAMD_ERR_SIZE;   // two pixels per three bytes
+    if (frame_stride < npx / 2 * 3 && n_frames > 1) return PIVAMD_ERR_ARG;
+    if (!d_raw || !d_out) return PIVAMD_ERR_ARG;
+    const uint8_t *raw = (const uint8_t *)d_raw;
+    const bool fast = npx % 8 == 0 && aligned(raw, 4) && frame_stride % 4 == 0 && aligned(d_out, 16) &&
+                      (!d_dark || aligned(d_dark, 16));
+    const dim3 grid(grid_x((npx + 7) / 8), (unsigned)n_frames);
+    if (fast) {
+        if (d_dark) hipLaunchKernelGGL((mono12p_kernel<true, true>), grid, dim3(256), 0, st, raw, frame_stride, npx, d_dark, d_out);
+        else hipLaunchKernelGGL((mono12p_kernel<true, false>), grid, dim3(256), 0, st, raw, frame_stride, npx, d_dark, d_out);
+    } else {
+        if (d_dark) hipLaunchKernelGGL((mono12p_kernel<false, true>), grid, dim3(256), 0, st, raw, frame_stride, npx, d_dark, d_out);
+        else hipLaunchKernelGGL((mono12p_kernel<false, false>), grid, dim3(256), 0, st, raw, frame_stride, npx, d_dark, d_out);
+    }
+    return PIVAMD_LAUNCHED();
+}
+
+extern "C" int pivamd_bayer_rg8_superpixel(const void *d_raw, size_t frame_stride, int n_frames, int H, int W,
+                                           const uint8_t *d_dark, uint16_t *d_out, hipStream_t st)
+{
+    if (n_frames < 0 || H < 0 || W < 0 || n_frames > 65535) return PIVAMD_ERR_ARG;
+    if (n_frames == 0 || H == 0 || W == 0) return 0;
+    if (H % 2 || W % 2) return PIVAMD_ERR_SIZE;   // 2x2 superpixels
+    if (frame_stride < (size_t)H * W && n_frames > 1) return PIVAMD_ERR_ARG;
+    if (!d_raw || !d_out) return PIVAMD_ERR_ARG;
+    const uint8_t *raw = (const uint8_t *)d_raw;
+    const bool fast = W % 8 == 0 && aligned(raw, 8) && frame_stride % 8 == 0 && aligned(d_out, 8) &&
+                      (!d_dark || aligned(d_dark, 8));
+    const size_t items = (size_t)(H / 2) * ((W / 2 + 3) / 4);
+    const dim3 grid(grid_x(items), (unsigned)n_frames);
+    if (fast) {
+        if (d_dark) hipLaunchKernelGGL((bayer_rg8_kernel<true, true>), grid, dim3(256), 0, st, raw, frame_stride, H, W, d_dark, d_out);
+        else hipLaunchKernelGGL((bayer_rg8_kernel<true, false>), grid, dim3(256), 0, st, raw, frame_stride, H, W, d_dark, d_out);
+    } else {
+        if (d_dark) hipLaunchKernelGGL((bayer_rg8_kernel<false, true>), grid, dim3(256), 0, st, raw, frame_stride, H, W, d_dark, d_out);
+        else hipLaunchKernelGGL((bayer_rg8_kernel<false, false>), grid, dim3(256), 0, st, raw, frame_stride, H, W, d_dark, d_out);
+    }
+    return PIVAMD_LAUNCHED();
+}

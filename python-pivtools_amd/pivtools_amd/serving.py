@@ -21,6 +21,11 @@ and window schedule fix its buffers and graphs.  Streams flagged
 ``ensemble`` also add every pair to that stream's ensemble accumulators
 (after the pass loop, on the stream's ensemble queue).
 
+A stream can also take its pairs as raw camera payloads (``pixel_format``
+"mono12p" or "bayer_rg8", with optional A and B dark frames): the slot then
+expands them on its own stream into the staging frames (``ingest.decode``),
+so only the packed bytes have to reach the GPU.
+
 Every pair is correlated independently of the others in its batch, so a
 pair's result does not depend on what it was batched with.  The serving tests
 check that a pair's result is bit-identical alone and in the mix.
@@ -33,6 +38,7 @@ from dataclasses import dataclass, field
 
 import torch
 
+from . import ingest
 from .ensemble import EnsemblePIV
 from .multipass import MultipassPIV
 
@@ -47,6 +53,13 @@ class StreamSpec:
     ensemble: bool = False
     ensemble_window: tuple = (32, 32)
     ensemble_overlap: int = 75
+    # "u16": requests carry 16-bit frames of ``shape``; "mono12p" /
+    # "bayer_rg8": requests carry the sensor payload (uint8, one frame's
+    # bytes) of a ``sensor_shape`` sensor, decoded into ``shape`` frames
+    pixel_format: str = "u16"
+    sensor_shape: tuple | None = None
+    dark_a: torch.Tensor | None = None     # sensor-type dark of the A frames (device)
+    dark_b: torch.Tensor | None = None     # and of the B frames
 
 
 @dataclass
@@ -103,6 +116,15 @@ class PIVServer:
             sizes = sorted({int(w[0] if isinstance(w, (tuple, list)) else w) for sp in streams for w in sp.window_sizes})
             self.tuned = AT.tune_xcorr(sizes, device=self.dev, cache=autotune if isinstance(autotune, str) else None)
         self.specs = list(streams)
+        for sp in self.specs:
+            if sp.pixel_format == "u16":
+                continue
+            if sp.pixel_format not in ingest.FORMATS or sp.sensor_shape is None:
+                raise ValueError(f"stream {sp.name}: pixel_format 'u16' or one of {ingest.FORMATS} "
+                                 "with its sensor_shape")
+            if tuple(ingest.frame_shape(sp.pixel_format, *sp.sensor_shape)) != tuple(sp.shape):
+                raise ValueError(f"stream {sp.name}: a {sp.sensor_shape} {sp.pixel_format} sensor gives "
+                                 f"{ingest.frame_shape(sp.pixel_format, *sp.sensor_shape)} frames, not {sp.shape}")
         self.max_batch = int(max_batch)
         self.max_slots = int(max_slots_per_stream)
         self.budget = int(hbm_budget_bytes)
@@ -126,10 +148,18 @@ class PIVServer:
     # -- admission ---------------------------------------------------------
     def submit(self, stream: int, a: torch.Tensor, b: torch.Tensor, t_arrival: float | None = None) -> Request:
         spec = self.specs[stream]
-        if tuple(a.shape) != tuple(spec.shape) or a.shape != b.shape:
-            raise ValueError(f"stream {spec.name} expects {spec.shape} frames, got {tuple(a.shape)}")
-        if a.dtype not in (torch.int16, torch.uint16) or b.dtype != a.dtype or not a.is_cuda:
-            raise TypeError("requests carry 16-bit camera counts (uint16 data) in device tensors")
+        if spec.pixel_format != "u16":
+            sh, sw = spec.sensor_shape
+            nbytes = ingest.payload_bytes(spec.pixel_format, sh, sw)
+            if a.dtype != torch.uint8 or b.dtype != torch.uint8 or not a.is_cuda or not b.is_cuda:
+                raise TypeError(f"stream {spec.name} takes {spec.pixel_format} payloads as uint8 device tensors")
+            if a.numel() != nbytes or b.numel() != nbytes or not a.is_contiguous() or not b.is_contiguous():
+                raise ValueError(f"stream {spec.name} expects contiguous {nbytes}-byte payloads")
+        else:
+            if tuple(a.shape) != tuple(spec.shape) or a.shape != b.shape:
+                raise ValueError(f"stream {spec.name} expects {spec.shape} frames, got {tuple(a.shape)}")
+            if a.dtype not in (torch.int16, torch.uint16) or b.dtype != a.dtype or not a.is_cuda:
+                raise TypeError("requests carry 16-bit camera counts (uint16 data) in device tensors")
         r = Request(self._rid, stream, a, b, time.perf_counter() if t_arrival is None else t_arrival)
         r.ready = torch.cuda.Event()
         r.ready.record(torch.cuda.current_stream(a.device))
@@ -174,10 +204,18 @@ class PIVServer:
                 slot.stream.wait_event(slot.ens_event)   # previous ensemble reads of the staging
             for r in batch:
                 slot.stream.wait_event(r.ready)          # frames produced on the caller's stream
+            spec = self.specs[i]
             for k in range(n):
                 r = batch[min(k, m - 1)]      # padding positions repeat the last real pair
-                slot.stage_a[k].copy_(r.a, non_blocking=True)
-                slot.stage_b[k].copy_(r.b, non_blocking=True)
+                if spec.pixel_format == "u16":
+                    slot.stage_a[k].copy_(r.a, non_blocking=True)
+                    slot.stage_b[k].copy_(r.b, non_blocking=True)
+                else:
+                    sh, sw = spec.sensor_shape
+                    ingest.decode(spec.pixel_format, r.a, sh, sw, n_frames=1, dark=spec.dark_a,
+                                  out=slot.stage_a[k:k + 1])
+                    ingest.decode(spec.pixel_format, r.b, sh, sw, n_frames=1, dark=spec.dark_b,
+                                  out=slot.stage_b[k:k + 1])
             for r in batch:
                 r.t_start = now
             a, b = slot.stage_a[:n], slot.stage_b[:n]
