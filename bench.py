@@ -300,6 +300,89 @@ def run_serving(args):
     print(json.dumps(line), flush=True)
 
 
+def host_fed(mp, A, B, steps: int, fmt: str) -> dict:
+    """Pairs/s when the batch starts in pinned host memory: per step, the
+    frames (16-bit) or Mono12p payloads cross PCIe on a copy stream, double
+    buffered so the next batch's copy overlaps this batch's pass loop; Mono12p
+    payloads are expanded on the device before the pass loop."""
+    import numpy as np
+    import torch
+    from pivtools_amd import ingest
+
+    n, H, W = A.shape
+    dev = A.device
+    if fmt == "mono12p":
+        def pack(x):     # 12-bit counts -> LSB-first pairs of pixels in three bytes
+            v = x.reshape(n, -1).to(torch.int32)
+            p0, p1 = v[:, 0::2], v[:, 1::2]
+            out = torch.stack([p0 & 0xFF, ((p0 >> 8) & 0xF) | ((p1 & 0xF) << 4), p1 >> 4], dim=-1)
+            return out.to(torch.uint8).reshape(n, -1)
+        host = [pack(A).cpu().pin_memory(), pack(B).cpu().pin_memory()]
+        frames = [torch.empty_like(A), torch.empty_like(B)]
+    else:
+        host = [A.cpu().pin_memory(), B.cpu().pin_memory()]
+    raw = [[torch.empty_like(h, device=dev) for h in host] for _ in range(2)]
+    # one copy stream per frame of the pair, so the two copies can use two DMA engines
+    copy_st = [torch.cuda.Stream(device=dev) for _ in host]
+    comp = torch.cuda.current_stream(dev)
+    copied, consumed = [[None, None], [None, None]], [None, None]
+
+    def step(s):
+        k = s % 2
+        for j, (d, h) in enumerate(zip(raw[k], host)):
+            with torch.cuda.stream(copy_st[j]):
+                if consumed[k] is not None:
+                    copy_st[j].wait_event(consumed[k])
+                d.copy_(h, non_blocking=True)
+                copied[k][j] = torch.cuda.Event()
+                copied[k][j].record(copy_st[j])
+        for j in range(len(host)):
+            comp.wait_event(copied[k][j])
+        if fmt == "mono12p":
+            for d, f in zip(raw[k], frames):
+                ingest.decode("mono12p", d, H, W, n_frames=n, out=f)
+            ev = torch.cuda.Event()
+            ev.record(comp)
+            consumed[k] = ev
+            mp.run(frames[0], frames[1])
+        else:
+            mp.run(raw[k][0], raw[k][1])
+            ev = torch.cuda.Event()
+            ev.record(comp)
+            consumed[k] = ev
+
+    for s in range(2):
+        step(s)
+    torch.cuda.synchronize(dev)
+    t0 = time.perf_counter()
+    for s in range(steps):
+        step(s)
+    torch.cuda.synchronize(dev)
+    dt = time.perf_counter() - t0
+    per_pair = sum(h.numel() * h.element_size() for h in host) / n
+    # the parts alone: the batch's copies (both streams), and the decode
+    torch.cuda.synchronize(dev)
+    t1 = time.perf_counter()
+    for j, (d, h) in enumerate(zip(raw[0], host)):
+        with torch.cuda.stream(copy_st[j]):
+            d.copy_(h, non_blocking=True)
+    torch.cuda.synchronize(dev)
+    copy_ms = (time.perf_counter() - t1) * 1e3
+    dec_ms = None
+    if fmt == "mono12p":
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for d, f in zip(raw[0], frames):
+            ingest.decode("mono12p", d, H, W, n_frames=n, out=f)
+        e1.record()
+        torch.cuda.synchronize(dev)
+        dec_ms = round(e0.elapsed_time(e1), 3)
+    return {"pairs_per_s": round(n * steps / dt, 3), "h2d_bytes_per_pair": int(per_pair),
+            "h2d_GB_per_s": round(per_pair * n * steps / dt / 1e9, 2), "steps": steps,
+            "batch_copy_alone_ms": round(copy_ms, 3),
+            "copy_alone_GB_per_s": round(per_pair * n / (copy_ms * 1e-3) / 1e9, 2), "decode_ms": dec_ms}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -315,6 +398,9 @@ def main():
                     help="keep the default correlation-kernel variants instead of timing them first")
     ap.add_argument("--one-pair-steps", type=int, default=10,
                     help="after the timed region, time the same loop fed one pair per step (0 disables)")
+    ap.add_argument("--host-fed-steps", type=int, default=4,
+                    help="after the timed region, time the loop fed from pinned host memory, 16-bit frames "
+                         "and Mono12p payloads (0 disables); reported beside value, never as value")
     ap.add_argument("--workload", choices=["multipass", "ensemble", "serving"], default="multipass")
     ap.add_argument("--serve-rates", type=float, nargs="+", default=[50, 200, 500, 1000, 2000])
     ap.add_argument("--serve-seconds", type=float, default=2.0)
@@ -425,6 +511,10 @@ def main():
                "latency_ms_p99": round(float(np.percentile(t1, 99)), 3), "steps": args.one_pair_steps,
                "batched_over_one_pair": round(value / rate, 3)}
         log(f"[bench] one pair per step: {rate:.1f} pairs/s (batched / one-pair = {value / rate:.2f})")
+    fed = None
+    if args.host_fed_steps > 0:
+        fed = {f: host_fed(mp, A, B, args.host_fed_steps, f) for f in ("u16", "mono12p")}
+        log(f"[bench] host-fed (PCIe-inclusive): {fed}")
     cpu = None
     acc = {}
     if args.cpu_budget > 0:
@@ -460,6 +550,7 @@ def main():
         "cpu_baseline": cpu,
         "accuracy": acc,
         "one_pair_fp32": one,
+        "host_fed": fed,
         "autotune": ({str(n): r for n, r in tuned.items()} if tuned else None),
     }
     print(json.dumps(line), flush=True)

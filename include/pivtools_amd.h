@@ -273,6 +273,17 @@ int pivamd_infill_nearest(const float *u, const float *mask, int N, int ny, int 
                           float px, float *uo, int *d_err, void *stream);
 int pivamd_pad_edge(const float *u, int N, int ny, int nx, int pre_y, int pre_x, int nyp, int nxp,
                     float *out, void *stream);
+/* Masked windows of a predictor field before it is padded
+ * (cpu_instantaneous.py:843-854 -> infilling.py:808-870 fill_masked_predictor):
+ * 'nearest' fills every masked cell (d_mask != 0) from the nearest unmasked
+ * cell in physical distance; 'harmonic' applies its least-squares solution
+ * as a linear map u[p][fill_idx[i]] = sum_j gt[j*nf + i] * u[p][bnd_idx[j]]
+ * (FP64, in place, every plane p < planes of plane_stride floats); gt is the
+ * mask geometry's map, computed once on the host. */
+int pivamd_fill_masked_nearest(const float *u, const float *d_mask, int N, int ny, int nx, float py,
+                               float px, float *uo, int *d_err, void *stream);
+int pivamd_fill_linear(float *u, int planes, long plane_stride, int nf, const int *d_fill_idx, int nb,
+                       const int *d_bnd_idx, const double *d_gt, void *stream);
 int pivamd_smooth_axis(const float *in, int planes, int ny, int nx, int axis, const double *d_wts,
                        int radius, float *out, void *stream);
 int pivamd_remap_cubic(const float *in, int N, int nyo, int nxo, const float *map_x,

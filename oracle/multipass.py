@@ -21,6 +21,16 @@ Pins:
                         per axis, replicate border).  Parity unpinned: only the
                         closed forms (constant, linear and integer-shift
                         fields) check it.
+  vector masks          load_images.py:952-985 restated as the reference's
+                        per-window loop; checked against hand-counted windows
+  predictor fill        infilling.py:808-870: 'nearest' through
+                        distance_transform_edt as the reference calls it;
+                        'harmonic' as the reference's difference-row least
+                        squares (infilling.py:144-260), checked against
+                        np.linalg.lstsq on the explicit equations, constant
+                        and interior-linear fields
+  predictor BCs         base.py:364-416 restated (row overwrite; unpinned
+                        beyond its own definition)
 """
 
 from __future__ import annotations
@@ -143,6 +153,103 @@ def infill_nearest(u, spacing, exclude=None):
     return out.astype(np.float32)
 
 
+def vector_masks_loop(image_shape, window_sizes, overlaps, pixel_mask, threshold=0.5):
+    """compute_vector_mask (load_images.py:827-1000, standard mode) as the
+    reference's per-window loop."""
+    from pivtools_amd.window import compute_window_centers
+
+    H, W = image_shape
+    out = []
+    for win, ov in zip(window_sizes, overlaps):
+        wy, wx = win
+        wc = compute_window_centers((H, W), (wy, wx), ov)
+        m = np.zeros((wc.n_win_y, wc.n_win_x), bool)
+        hy, hx = (wy - 1) / 2.0, (wx - 1) / 2.0
+        for iy in range(wc.n_win_y):
+            y0 = int(np.floor(wc.win_ctrs_y[iy] - hy + 0.5))
+            y1 = int(np.floor(wc.win_ctrs_y[iy] + hy + 0.5))
+            for ix in range(wc.n_win_x):
+                x0 = int(np.floor(wc.win_ctrs_x[ix] - hx + 0.5))
+                x1 = int(np.floor(wc.win_ctrs_x[ix] + hx + 0.5))
+                ya, yb, xa, xb = max(0, y0), min(H, y1 + 1), max(0, x0), min(W, x1 + 1)
+                if yb > ya and xb > xa:
+                    m[iy, ix] = np.sum(pixel_mask[ya:yb, xa:xb]) / (wy * wx) > threshold
+        out.append(m)
+    return out
+
+
+def _difference_rows(shape, spacing):
+    """First differences along y and x in physical units (infilling.py:144-179)."""
+    import scipy.sparse as sp
+
+    idx = np.arange(shape[0] * shape[1]).reshape(shape)
+    rows, cols, vals = [], [], []
+    r = 0
+    for axis, h in ((0, spacing[0]), (1, spacing[1])):
+        n = shape[axis]
+        if n <= 1:
+            continue
+        a = idx[:-1, :] if axis == 0 else idx[:, :-1]
+        b = idx[1:, :] if axis == 0 else idx[:, 1:]
+        m = a.size
+        rr = r + np.arange(m)
+        rows += [rr, rr]
+        cols += [a.ravel(), b.ravel()]
+        vals += [np.full(m, -1.0 / h), np.full(m, 1.0 / h)]
+        r += m
+    return sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                         shape=(r, shape[0] * shape[1]))
+
+
+def fill_harmonic(fields, fill, spacing):
+    """Least-squares first-difference fill of a stack (K, ny, nx), unmasked
+    nodes fixed (infilling.py:182-260 with no excluded nodes)."""
+    from scipy.sparse.linalg import splu
+
+    K, ny, nx = fields.shape
+    f = fill.ravel()
+    k = ~f
+    D = _difference_rows((ny, nx), spacing)
+    Du = D[:, f].tocsr()
+    keep = np.diff(Du.indptr) > 0
+    Du, Dk = Du[keep], D[:, k].tocsr()[keep]
+    A = (Du.T @ Du).tocsc()
+    Uk = fields.reshape(K, -1)[:, k].T.astype(np.float64)
+    Bm = Du.T @ (-(Dk @ Uk))
+    X = splu(A).solve(np.asarray(Bm)).reshape(-1, K)
+    out = fields.astype(np.float64).reshape(K, -1).copy()
+    out[:, f] = X.T
+    return out.reshape(K, ny, nx)
+
+
+def fill_masked_predictor(u, mask, spacing, method):
+    """infilling.py:808-870 on the planar field (2, N, ny, nx)."""
+    if method == "none" or mask is None or not mask.any():
+        return u
+    C, N, ny, nx = u.shape
+    stack = u.reshape(C * N, ny, nx).astype(np.float64)
+    if method == "harmonic":
+        out = fill_harmonic(stack, mask, spacing)
+    else:
+        donor = ~mask & np.all(np.isfinite(stack), axis=0)
+        iy, ix = ndi.distance_transform_edt(~donor, sampling=spacing, return_distances=False, return_indices=True)
+        out = stack.copy()
+        out[:, mask] = stack[:, iy, ix][:, mask]
+    return out.reshape(C, N, ny, nx).astype(np.float32)
+
+
+def apply_predictor_bcs(pad, ctrs_y_all, H, bcs):
+    """base.py:364-416: overwrite the padded rows beyond each BC position."""
+    for bc in bcs or []:
+        if bc["edge"] == "bottom":
+            rows = np.asarray(ctrs_y_all, np.float64) >= float(H - 1 - bc["y_position"])
+        else:
+            rows = np.asarray(ctrs_y_all, np.float64) <= float(bc["y_position"])
+        pad[0, :, rows, :] = bc["uy"]
+        pad[1, :, rows, :] = bc["ux"]
+    return pad
+
+
 def pad_edge(u, pre, post):
     return np.pad(u, ((0, 0), (0, 0), (pre[0], post[0]), (pre[1], post[1])), mode="edge")
 
@@ -227,16 +334,19 @@ def peaks_fp64(planes, pw):
     return row, col, np.where(ok, h0, 0.0)
 
 
-def xcorr_pass(A, B, g, w, finder, flavour):
+def xcorr_pass(A, B, g, w, finder, flavour, mask=None):
     """One bulkxcorr2d over all pairs: flavour 'oracle' (float32 C
     restatement) or 'fp64' (FP64 planes, FP64 3-point fit)."""
     ny, nx = g.n_win
     win = g.win
     if flavour == "oracle":
-        o = O.bulkxcorr2d(A, B, np.zeros((ny, nx), np.float32), g.ctrs_x, g.ctrs_y, win, w, w, 1, finder)
+        fm = np.zeros((ny, nx), np.float32) if mask is None else mask.astype(np.float32)
+        o = O.bulkxcorr2d(A, B, fm, g.ctrs_x, g.ctrs_y, win, w, w, 1, finder)
         return o["pk_x"][:, 0], o["pk_y"][:, 0], o["pk_h"][:, 0]
     if finder != 3:
         raise ValueError("the FP64 flavour fits with the 3-point estimator")
+    if mask is not None:
+        raise ValueError("the FP64 flavour runs without window masks")
     pw = O.plane_weight(w).astype(np.float64)
     N = A.shape[0]
     pkx = np.zeros((N, ny, nx))
@@ -255,7 +365,8 @@ def xcorr_pass(A, B, g, w, finder, flavour):
 
 
 def run_multipass(A, B, grids, window_type="gaussian", finder=3, smoothing=True, median=(0.1, 2.0, 5),
-                  final_infill="nearest", flavour="oracle", interp=0, keep_passes=False):
+                  final_infill="nearest", flavour="oracle", interp=0, keep_passes=False, vector_masks=None,
+                  predictor_fill="harmonic", predictor_bcs=None):
     """The whole pass loop on the CPU; A, B (N, H, W) float32."""
     from pivtools_amd.window import window_weight
 
@@ -276,15 +387,21 @@ def run_multipass(A, B, grids, window_type="gaussian", finder=3, smoothing=True,
             pred = remap_cubic(field, g.pred_map_x, g.pred_map_y)
             a_in, b_in = O.fused_warp_batch(A, B, field[0], field[1], prev_grid.ctrs_y_all, prev_grid.ctrs_x_all,
                                             interp, False, False)
-        pkx, pky, pkh = xcorr_pass(a_in, b_in, g, w, finder, flavour)
+        m = None
+        if vector_masks is not None and vector_masks[p] is not None and np.any(vector_masks[p]):
+            m = np.asarray(vector_masks[p], bool)
+        pkx, pky, pkh = xcorr_pass(a_in, b_in, g, w, finder, flavour, m)
         div = 2.0 if p == 0 else 4.0
-        u, nm, reason, pkg = gate(pkx, pky, pkh, pred, None, g.win[1] / div, g.win[0] / div)
-        u, nm, reason, pko = median_stage(u, pkg, nm, reason, None, *median)
+        u, nm, reason, pkg = gate(pkx, pky, pkh, pred, m, g.win[1] / div, g.win[0] / div)
+        u, nm, reason, pko = median_stage(u, pkg, nm, reason, m, *median)
         final = p == len(grids) - 1
+        sp_ = (float(g.spacing[0]), float(g.spacing[1]))
         if not final or final_infill == "nearest":
-            u = infill_nearest(u, (float(g.spacing[0]), float(g.spacing[1])))
+            u = infill_nearest(u, sp_, exclude=m)
         if not final:
-            prev_pad = pad_edge(u, g.n_pre, g.n_post)
+            src = fill_masked_predictor(u, m, sp_, predictor_fill)
+            prev_pad = apply_predictor_bcs(pad_edge(src, g.n_pre, g.n_post), g.ctrs_y_all, A.shape[1],
+                                           predictor_bcs)
         if keep_passes or final:
             out.append({"ux": u[1], "uy": u[0], "nan_mask": nm, "nan_reason": reason, "peak_mag": pko,
                         "predictor": pred})

@@ -10,6 +10,10 @@
 //   infill_kernel     nearest-donor fill in physical distance, ties to the
 //                     smallest column then row, as scipy's EDT resolves them
 //                     (infilling.py:114-141)
+//   infill_kernel     (second mode) nearest fill of masked windows of a
+//                     predictor field (infilling.py:808-870, 'nearest')
+//   fill_linear_kernel the same field's harmonic fill ('harmonic'), applied
+//                     as a precomputed linear map of the unmasked neighbours
 //   pad_kernel        edge padding into the predictor grid (:836-851)
 //   smooth_kernel     scipy gaussian_filter, mode 'nearest', axis 0 then 1 with
 //                     float32 rounding between axes (:1135-1147)
@@ -195,7 +199,7 @@ __global__ void median_kernel(const float *u, const float *pkh, const float *mas
 // One lane per cell; a ring search stops once the ring is farther than the
 // best distance found.  A cell with no donor anywhere sets *err.
 __global__ void infill_kernel(const float *u, const float *mask, int N, int ny, int nx, float py, float px,
-                              float *uo, int *err)
+                              int fill_masked, float *uo, int *err)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int nWin = ny * nx, total = N * nWin;
@@ -206,7 +210,10 @@ __global__ void infill_kernel(const float *u, const float *mask, int N, int ny, 
     const float y0 = uy[w], x0 = ux[w];
     uo[i] = y0;
     uo[(size_t)total + i] = x0;
-    const bool fill = (y0 != y0) || (x0 != x0);
+    // fill_masked = 0: fill NaN cells, masked cells are no donors (the
+    // infill of a pass); 1: fill the masked cells themselves from the
+    // unmasked ones (the masked-window fill of a predictor field)
+    const bool fill = fill_masked ? (mask && mask[w] != 0.0f) : ((y0 != y0) || (x0 != x0));
     if (!fill) return;
     double best = 1e300;
     int br = -1, bc = -1;
@@ -245,6 +252,24 @@ __global__ void infill_kernel(const float *u, const float *mask, int N, int ny, 
     }
     uo[i] = uy[br * nx + bc];
     uo[(size_t)total + i] = ux[br * nx + bc];
+}
+
+// A linear fill of chosen cells from chosen donor cells, the same in every
+// plane: u[p][fill_idx[i]] = sum_j gt[j][i] * u[p][bnd_idx[j]], in FP64,
+// rounded once.  The masked-window harmonic fill of a predictor field has
+// this form: its least-squares solution is a fixed linear map (the mask's
+// geometry only) of the unmasked cells next to the mask.  One lane per fill
+// cell (gt rows are read coalesced across lanes), one grid row per plane;
+// fill and donor cells are disjoint, so the plane is updated in place.
+__global__ void fill_linear_kernel(float *u, long plane_stride, int nf, const int *__restrict__ fill_idx, int nb,
+                                   const int *__restrict__ bnd_idx, const double *__restrict__ gt)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nf) return;
+    float *pl = u + (size_t)blockIdx.y * plane_stride;
+    double acc = 0.0;
+    for (int j = 0; j < nb; ++j) acc += gt[(size_t)j * nf + i] * (double)pl[bnd_idx[j]];
+    pl[fill_idx[i]] = (float)acc;
 }
 
 // np.pad(mode='edge') of the planar field (2, N, ny, nx) into (2, N, nyp, nxp).
@@ -393,7 +418,27 @@ extern "C" int pivamd_infill_nearest(const float *u, const float *mask, int N, i
     const size_t n = (size_t)N * ny * nx;
     if (n == 0) return 0;
     if (!(py > 0.0f) || !(px > 0.0f)) return PIVAMD_ERR_ARG;
-    hipLaunchKernelGGL(infill_kernel, dim3(nblk(n)), dim3(256), 0, st, u, mask, N, ny, nx, py, px, uo, d_err);
+    hipLaunchKernelGGL(infill_kernel, dim3(nblk(n)), dim3(256), 0, st, u, mask, N, ny, nx, py, px, 0, uo, d_err);
+    return PIVAMD_LAUNCHED();
+}
+
+extern "C" int pivamd_fill_masked_nearest(const float *u, const float *mask, int N, int ny, int nx, float py,
+                                          float px, float *uo, int *d_err, hipStream_t st)
+{
+    const size_t n = (size_t)N * ny * nx;
+    if (n == 0) return 0;
+    if (!mask || !(py > 0.0f) || !(px > 0.0f)) return PIVAMD_ERR_ARG;
+    hipLaunchKernelGGL(infill_kernel, dim3(nblk(n)), dim3(256), 0, st, u, mask, N, ny, nx, py, px, 1, uo, d_err);
+    return PIVAMD_LAUNCHED();
+}
+
+extern "C" int pivamd_fill_linear(float *u, int planes, long plane_stride, int nf, const int *fill_idx, int nb,
+                                  const int *bnd_idx, const double *gt, hipStream_t st)
+{
+    if (planes <= 0 || nf <= 0) return 0;
+    if (nb <= 0 || !fill_idx || !bnd_idx || !gt || planes > 65535) return PIVAMD_ERR_ARG;
+    hipLaunchKernelGGL(fill_linear_kernel, dim3((unsigned)((nf + 255) / 256), (unsigned)planes), dim3(256), 0, st, u,
+                       plane_stride, nf, fill_idx, nb, bnd_idx, gt);
     return PIVAMD_LAUNCHED();
 }
 

@@ -125,6 +125,8 @@ GLUE_SIGNATURES: dict = {
     "pivamd_pad_edge": [_vp, _i, _i, _i, _i, _i, _i, _i, _vp, _vp],
     "pivamd_smooth_axis": [_vp, _i, _i, _i, _i, _vp, _i, _vp, _vp],
     "pivamd_remap_cubic": [_vp, _i, _i, _i, _vp, _vp, _i, _i, _vp, _vp],
+    "pivamd_fill_masked_nearest": [_vp, _vp, _i, _i, _i, _f, _f, _vp, _vp, _vp],
+    "pivamd_fill_linear": [_vp, _i, ctypes.c_long, _i, _vp, _i, _vp, _vp, _vp],
     "pivamd_unpack_mono12p": [_vp, _sz, _i, _i, _i, _vp, _vp, _vp],
     "pivamd_bayer_rg8_superpixel": [_vp, _sz, _i, _i, _i, _vp, _vp, _vp],
 }

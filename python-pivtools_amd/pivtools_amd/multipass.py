@@ -11,6 +11,14 @@ after), the predictor added back, outliers flagged by the normalised median
 test, holes infilled by nearest donor and the field edge-padded into the next
 predictor (:555-865).
 
+Masked windows (a per-pass window mask, given directly or computed from a
+pixel mask as pivtools_core/image_handling/load_images.py:827-1000 does) are
+skipped by the correlation, invalid with reason -1, zero in the outputs and
+never infill donors; before a field becomes the next predictor its masked
+windows are filled from the unmasked ones ('harmonic', 'nearest' or 'none',
+infilling.py:808-870), and optional predictor boundary conditions overwrite
+padded rows (base.py:364-416).
+
 Torch provides device memory and the stream; every computation is a HIP
 kernel of libpivtools_amd.so launched through its C ABI on torch's current
 stream.  Buffers are allocated once per (batch size, schedule) and reused.
@@ -25,7 +33,9 @@ import numpy as np
 import torch
 
 from . import _native
+from .predictor_fill import harmonic_operator
 from .window import PassGrid, build_pass_grids, window_weight
+from .window import vector_masks as _vector_masks
 
 _vp = ctypes.c_void_p
 _i = ctypes.c_int
@@ -75,6 +85,8 @@ class _PassBuf:
     work: torch.Tensor | None = None
     maps: tuple | None = None
     smooth: tuple | None = None
+    fill_op: tuple | None = None       # harmonic predictor fill: (fill_idx, bnd_idx, gt) on the device
+    bc_rows: list = field(default_factory=list)   # (row slice of the padded grid, uy, ux)
 
 
 class MultipassPIV:
@@ -82,7 +94,16 @@ class MultipassPIV:
 
     def __init__(self, image_shape, window_sizes, overlaps, *, window_type="gaussian", peak_finder="gauss3",
                  peak_fit_impl="scalar", warp_interpolation="cubic", predictor_smoothing=True,
-                 median=(0.1, 2.0, 5), final_infill="nearest", dft="fft", device="cuda"):
+                 median=(0.1, 2.0, 5), final_infill="nearest", dft="fft", device="cuda",
+                 vector_masks=None, pixel_mask=None, mask_threshold=0.5, predictor_fill="harmonic",
+                 predictor_bcs=None):
+        """``vector_masks``: one (n_win_y, n_win_x) bool array per pass (True =
+        masked window), or ``pixel_mask``: an (H, W) bool image mask turned
+        into them with ``mask_threshold``.  ``predictor_fill``: 'harmonic',
+        'nearest' or 'none' for the masked windows of each predictor field.
+        ``predictor_bcs``: list of {"edge": "top"|"bottom", "y_position",
+        "ux", "uy"} overwriting the padded predictor rows beyond y_position.
+        """
         self.H, self.W = image_shape
         if dft not in ("fft", "split"):
             raise ValueError(f"dft {dft!r}: 'fft' (FP32 FFT) or 'split' (split-fp16 MFMA DFT, N <= 64)")
@@ -106,6 +127,34 @@ class MultipassPIV:
                 raise ValueError(f"window {win} has no device kernel")
         self.grids = build_pass_grids((self.H, self.W), self.window_sizes, self.overlaps)
         self.window_type = window_type
+        if vector_masks is not None and pixel_mask is not None:
+            raise ValueError("give vector_masks or pixel_mask, not both")
+        if pixel_mask is not None:
+            vector_masks = _vector_masks((self.H, self.W), self.window_sizes, self.overlaps, pixel_mask,
+                                         mask_threshold)
+        self.masks = None
+        if vector_masks is not None:
+            if len(vector_masks) != len(self.grids):
+                raise ValueError(f"{len(vector_masks)} vector masks for {len(self.grids)} passes")
+            self.masks = []
+            for m, g in zip(vector_masks, self.grids):
+                m = np.asarray(m).astype(bool)
+                if m.shape != g.n_win:
+                    raise ValueError(f"vector mask {m.shape} does not match the window grid {g.n_win}")
+                self.masks.append(m if m.any() else None)
+        if predictor_fill not in ("harmonic", "nearest", "none"):
+            raise ValueError(f"predictor_fill {predictor_fill!r}: 'harmonic', 'nearest' or 'none'")
+        self.predictor_fill = predictor_fill
+        # the harmonic fill's linear map per non-final masked pass (mask geometry only)
+        self._fill_host = {}
+        if self.masks is not None and predictor_fill == "harmonic":
+            for p, (m, g) in enumerate(zip(self.masks[:-1], self.grids[:-1])):
+                if m is not None:
+                    self._fill_host[p] = harmonic_operator(m, g.spacing)
+        self.predictor_bcs = list(predictor_bcs or [])
+        for bc in self.predictor_bcs:
+            if bc.get("edge") not in ("top", "bottom"):
+                raise ValueError(f"predictor BC edge {bc.get('edge')!r}: 'top' or 'bottom'")
         self._bufs: list[_PassBuf] | None = None
         self._n = 0
         # buffer sets per batch size, kept for the life of the object: a HIP
@@ -127,6 +176,21 @@ class MultipassPIV:
                       (("x", g.ctrs_x), ("y", g.ctrs_y), ("xa", g.ctrs_x_all), ("ya", g.ctrs_y_all))}
             w = window_weight(g.win, self.window_type)
             b.w = torch.as_tensor(w, device=dev)
+            m = self.masks[p] if self.masks is not None else None
+            final = p == len(self.grids) - 1
+            if m is not None:
+                b.mask = torch.as_tensor(m.astype(np.float32), device=dev)
+                if p in self._fill_host:
+                    b.fill_op = tuple(torch.as_tensor(x, device=dev) for x in self._fill_host[p])
+            if not final:
+                ya = np.asarray(g.ctrs_y_all, np.float64)
+                for bc in self.predictor_bcs:
+                    if bc["edge"] == "bottom":
+                        rows = np.flatnonzero(ya >= float(self.H - 1 - bc["y_position"]))
+                    else:
+                        rows = np.flatnonzero(ya <= float(bc["y_position"]))
+                    if rows.size:
+                        b.bc_rows.append((slice(int(rows[0]), int(rows[-1]) + 1), float(bc["uy"]), float(bc["ux"])))
             b.pw = torch.empty(g.win, dtype=f32, device=dev)
             _native.check(self.lib.pivamd_plane_weight(_ptr(b.w), g.win[0], g.win[1], _ptr(b.pw), st),
                           "plane weight")
@@ -136,6 +200,8 @@ class MultipassPIV:
             b.out["u"] = torch.empty((2,) + shape, dtype=f32, device=dev)
             b.out["u2"] = torch.empty((2,) + shape, dtype=f32, device=dev)
             b.out["u3"] = torch.empty((2,) + shape, dtype=f32, device=dev)
+            if b.mask is not None and not final and self.predictor_fill != "none":
+                b.out["u4"] = torch.empty((2,) + shape, dtype=f32, device=dev)
             b.out["nanm"] = torch.empty(shape, dtype=torch.uint8, device=dev)
             b.out["reason"] = torch.empty(shape, dtype=torch.int8, device=dev)
             b.out["err"] = torch.zeros(1, dtype=torch.int32, device=dev)
@@ -222,9 +288,13 @@ class MultipassPIV:
             if events is not None:
                 ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
                 ev[0].record()
+            if b.mask is not None:
+                # masked windows are skipped by the correlation and keep these zeros
+                for k in ("pkx", "pky", "pkh", "sx", "sy", "sxy", "ratio"):
+                    o[k].zero_()
             prev_mode = L.pivamd_set_dft_mode(self.dft)
             fn = xcorr_fn if p == 0 else L.pivamd_xcorr
-            rc = fn(_ptr(a_in), _ptr(b_in), None, self.H, self.W, n, _ptr(b.ctrs["x"]),
+            rc = fn(_ptr(a_in), _ptr(b_in), _ptr(b.mask), self.H, self.W, n, _ptr(b.ctrs["x"]),
                                          _ptr(b.ctrs["y"]), ny, nx, _ptr(b.w), _ptr(b.w), _ptr(b.pw), 0,
                                          g.win[0], g.win[1], 1, self.finder, self.impl, _ptr(o["pkx"]),
                                          _ptr(o["pky"]), _ptr(o["pkh"]), _ptr(o["sx"]), _ptr(o["sy"]),
@@ -235,12 +305,12 @@ class MultipassPIV:
                 ev[1].record()
                 events.append((p, ev))
             div = 2.0 if p == 0 else 4.0
-            _native.check(L.pivamd_gate(_ptr(o["pkx"]), _ptr(o["pky"]), _ptr(o["pkh"]), _ptr(pred), None, n,
+            _native.check(L.pivamd_gate(_ptr(o["pkx"]), _ptr(o["pky"]), _ptr(o["pkh"]), _ptr(pred), _ptr(b.mask), n,
                                         ny * nx, g.win[1] / div, g.win[0] / div, _ptr(o["u"]), _ptr(o["nanm"]),
                                         _ptr(o["reason"]), _ptr(o["pkh_g"]), st), "gate")
             if self.median is not None:
                 eps, thr, size = self.median
-                _native.check(L.pivamd_median(_ptr(o["u"]), _ptr(o["pkh_g"]), None, n, ny, nx, eps, thr, size,
+                _native.check(L.pivamd_median(_ptr(o["u"]), _ptr(o["pkh_g"]), _ptr(b.mask), n, ny, nx, eps, thr, size,
                                               _ptr(o["nanm"]), _ptr(o["reason"]), _ptr(o["u2"]), _ptr(o["pkh_o"]),
                                               st), "median test")
                 u = o["u2"]
@@ -248,14 +318,31 @@ class MultipassPIV:
                 raise NotImplementedError("the device pass loop runs with the median test")
             if not final or self.final_infill == "nearest":
                 o["err"].zero_()
-                _native.check(L.pivamd_infill_nearest(_ptr(u), None, n, ny, nx, float(g.spacing[0]),
+                _native.check(L.pivamd_infill_nearest(_ptr(u), _ptr(b.mask), n, ny, nx, float(g.spacing[0]),
                                                       float(g.spacing[1]), _ptr(o["u3"]), _ptr(o["err"]), st),
                               "infill")
                 u = o["u3"]
             if not final:
                 nyp, nxp = len(g.ctrs_y_all), len(g.ctrs_x_all)
-                _native.check(L.pivamd_pad_edge(_ptr(u), n, ny, nx, g.n_pre[0], g.n_pre[1], nyp, nxp,
+                src = u
+                if "u4" in o:
+                    # the predictor's copy of the field with its masked windows filled;
+                    # the pass result keeps zeros there
+                    src = o["u4"]
+                    if self.predictor_fill == "nearest":
+                        _native.check(L.pivamd_fill_masked_nearest(_ptr(u), _ptr(b.mask), n, ny, nx,
+                                                                   float(g.spacing[0]), float(g.spacing[1]),
+                                                                   _ptr(src), _ptr(o["err"]), st), "predictor fill")
+                    else:
+                        src.copy_(u)
+                        fi, bi, gt = b.fill_op
+                        _native.check(L.pivamd_fill_linear(_ptr(src), 2 * n, ny * nx, fi.numel(), _ptr(fi),
+                                                           bi.numel(), _ptr(bi), _ptr(gt), st), "predictor fill")
+                _native.check(L.pivamd_pad_edge(_ptr(src), n, ny, nx, g.n_pre[0], g.n_pre[1], nyp, nxp,
                                                 _ptr(o["pad"]), st), "pad")
+                for rows, uy_bc, ux_bc in b.bc_rows:
+                    o["pad"][0, :, rows, :].fill_(uy_bc)
+                    o["pad"][1, :, rows, :].fill_(ux_bc)
             if keep_passes or final:
                 results.append(PassResult(ux=u[1], uy=u[0], nan_mask=o["nanm"], nan_reason=o["reason"],
                                           peak_mag=o["pkh_o"], peak_ratio=o["ratio"], predictor=pred,

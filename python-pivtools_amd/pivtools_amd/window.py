@@ -6,6 +6,8 @@ so the numbers handed to the kernels are the reference's numbers:
 
 * ``compute_window_centers``  pivtools_core/window_utils.py:142-273
 * ``window_weight``           pivtools_cli/piv/piv_backend/base.py:52-135
+* ``vector_masks``            pivtools_core/image_handling/load_images.py:827-1000
+                              (compute_vector_mask, standard mode)
 * ``PassGrid`` / ``build_pass_grids``
                               pivtools_cli/piv/piv_backend/base.py:137-300
                               (instantaneous: first-pass kernel (0, 0), no BCs)
@@ -153,3 +155,40 @@ def build_pass_grids(image_shape, window_sizes, overlaps) -> list[PassGrid]:
             g.pred_map_y = iy.reshape(wy.shape).astype(np.float32)
         grids.append(g)
     return grids
+
+
+def vector_masks(image_shape, window_sizes, overlaps, pixel_mask, threshold: float = 0.5) -> list:
+    """Masked windows of every pass, (n_win_y, n_win_x) bool each.
+
+    A window covers rows floor(c - (h-1)/2 + 0.5) .. floor(c + (h-1)/2 + 0.5)
+    (and the same in x) clipped to the image; it is masked when its count of
+    masked pixels divided by h*w exceeds ``threshold``
+    (pivtools_core/image_handling/load_images.py:952-985).  The counts come
+    from a summed-area table instead of one slice sum per window; they are
+    exact integers either way, and the division and comparison are the
+    reference's float64 ones.
+    """
+    H, W = image_shape
+    pm = np.asarray(pixel_mask).astype(bool)
+    if pm.shape != (H, W):
+        raise ValueError(f"pixel mask {pm.shape} does not match the image shape {(H, W)}")
+    sat = np.zeros((H + 1, W + 1), np.int64)
+    sat[1:, 1:] = np.cumsum(np.cumsum(pm, axis=0, dtype=np.int64), axis=1)
+    out = []
+    for win, ov in zip(window_sizes, overlaps):
+        wy, wx = (win, win) if np.ndim(win) == 0 else win
+        wc = compute_window_centers((H, W), (wy, wx), ov)
+        cy = np.asarray(wc.win_ctrs_y, np.float64)
+        cx = np.asarray(wc.win_ctrs_x, np.float64)
+        y0 = np.floor(cy - (wy - 1) / 2.0 + 0.5).astype(np.int64)
+        y1 = np.floor(cy + (wy - 1) / 2.0 + 0.5).astype(np.int64)
+        x0 = np.floor(cx - (wx - 1) / 2.0 + 0.5).astype(np.int64)
+        x1 = np.floor(cx + (wx - 1) / 2.0 + 0.5).astype(np.int64)
+        ya, yb = np.maximum(0, y0), np.minimum(H, y1 + 1)
+        xa, xb = np.maximum(0, x0), np.minimum(W, x1 + 1)
+        cnt = (sat[yb[:, None], xb[None, :]] - sat[ya[:, None], xb[None, :]]
+               - sat[yb[:, None], xa[None, :]] + sat[ya[:, None], xa[None, :]])
+        frac = cnt.astype(np.float64) / float(wy * wx)
+        nonempty = (yb > ya)[:, None] & (xb > xa)[None, :]
+        out.append(nonempty & (frac > threshold))
+    return out

@@ -215,3 +215,109 @@ def test_multipass_loop_matches_oracle_loop(finder):
             # next pass warps differently there; the field agrees elsewhere
             assert flips <= 0.05, (p, flips)
             assert np.median(d) < 1e-3 and np.mean(d > 0.005) < 0.15, (p, np.mean(d > 0.005))
+
+
+def _blob_mask(ny, nx, seed):
+    """A disk inside the grid and a rectangle on its right edge."""
+    rng = np.random.default_rng(seed)
+    yy, xx = np.mgrid[0:ny, 0:nx]
+    cy, cx = rng.uniform(0.3, 0.7) * ny, rng.uniform(0.25, 0.5) * nx
+    m = (yy - cy) ** 2 + (xx - cx) ** 2 < (0.2 * min(ny, nx)) ** 2
+    m[ny // 3: ny // 3 + 3, nx - 2:] = True
+    return m
+
+
+def test_masked_glue_kernels_match_oracle():
+    """Gate, median, infill (masked donors excluded) and both predictor fills
+    with a window mask, against the oracle."""
+    L = _lib()
+    N, ny, nx = 3, 21, 26
+    rng = np.random.default_rng(11)
+    m = _blob_mask(ny, nx, 4)
+    dm = _t(m.astype(np.float32))
+    pkx = rng.normal(0, 2, (N, ny, nx)).astype(np.float32)
+    pky = rng.normal(0, 2, (N, ny, nx)).astype(np.float32)
+    pkx[0, 3, 4] = np.nan
+    pkh = rng.uniform(0.1, 1, (N, ny, nx)).astype(np.float32)
+    pred = rng.normal(0, 1, (2, N, ny, nx)).astype(np.float32)
+    ru, rnm, rrs, rpk = M.gate(pkx, pky, pkh, pred, m, 5.0, 5.0)
+    u = torch.empty((2, N, ny, nx), device=dev)
+    nm = torch.empty((N, ny, nx), dtype=torch.uint8, device=dev)
+    rs = torch.empty((N, ny, nx), dtype=torch.int8, device=dev)
+    ph = torch.empty((N, ny, nx), device=dev)
+    assert L.pivamd_gate(_t(pkx).data_ptr(), _t(pky).data_ptr(), _t(pkh).data_ptr(), _t(pred).data_ptr(),
+                         dm.data_ptr(), N, ny * nx, 5.0, 5.0, u.data_ptr(), nm.data_ptr(), rs.data_ptr(),
+                         ph.data_ptr(), _st()) == 0
+    mu, mnm, mrs, mpk = M.median_stage(ru, rpk, rnm, rrs, m, 0.1, 2.0, 5)
+    uo = torch.empty_like(u)
+    po = torch.empty_like(ph)
+    assert L.pivamd_median(u.data_ptr(), ph.data_ptr(), dm.data_ptr(), N, ny, nx, 0.1, 2.0, 5, nm.data_ptr(),
+                           rs.data_ptr(), uo.data_ptr(), po.data_ptr(), _st()) == 0
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(nm.cpu().numpy().astype(bool), mnm)
+    np.testing.assert_array_equal(rs.cpu().numpy(), mrs)
+    np.testing.assert_array_equal(uo.cpu().numpy(), mu)
+    assert (mrs[:, m] == -1).all() and np.array_equal(mrs != 0, mnm)
+    sp_ = (16.0, 12.0)
+    ri = M.infill_nearest(mu, sp_, exclude=m)
+    inf = torch.empty_like(u)
+    err = torch.zeros(1, dtype=torch.int32, device=dev)
+    assert L.pivamd_infill_nearest(uo.data_ptr(), dm.data_ptr(), N, ny, nx, sp_[0], sp_[1], inf.data_ptr(),
+                                   err.data_ptr(), _st()) == 0
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(inf.cpu().numpy(), ri)
+    # predictor fills of the infilled field (zeros at masked windows)
+    near = torch.empty_like(u)
+    assert L.pivamd_fill_masked_nearest(inf.data_ptr(), dm.data_ptr(), N, ny, nx, sp_[0], sp_[1],
+                                        near.data_ptr(), err.data_ptr(), _st()) == 0
+    from pivtools_amd.predictor_fill import harmonic_operator
+    fi, bi, gt = harmonic_operator(m, sp_)
+    harm = inf.clone()
+    dfi, dbi, dgt = _t(fi, torch.int32), _t(bi, torch.int32), _t(gt, torch.float64)
+    assert L.pivamd_fill_linear(harm.data_ptr(), 2 * N, ny * nx, fi.size, dfi.data_ptr(), bi.size, dbi.data_ptr(),
+                                dgt.data_ptr(), _st()) == 0
+    torch.cuda.synchronize()
+    assert int(err.item()) == 0
+    np.testing.assert_array_equal(near.cpu().numpy(), M.fill_masked_predictor(ri, m, sp_, "nearest"))
+    rh = M.fill_masked_predictor(ri, m, sp_, "harmonic")
+    np.testing.assert_allclose(harm.cpu().numpy(), rh, rtol=2e-7, atol=1e-7)
+    np.testing.assert_array_equal(harm.cpu().numpy()[:, :, ~m], ri[:, :, ~m])
+
+
+@pytest.mark.parametrize("fill,bcs", [("harmonic", None), ("nearest", None), ("none", None),
+                                      ("harmonic", [{"edge": "bottom", "y_position": 40, "ux": 1.5, "uy": 0.0}])])
+def test_masked_multipass_matches_oracle_loop(fill, bcs):
+    """The pass loop with a pixel mask (a disk and a strip on the image edge)
+    against the oracle loop: identical NaN masks and reasons, masked
+    windows zero, the field within the float tolerance of the unmasked loop."""
+    shape = (192, 256)
+    sched = [(64, 64), (32, 32), (16, 16)]
+    print(f"masked multipass fill={fill} bcs={bcs}", flush=True)
+    A, B = [], []
+    for k in range(2):
+        rng = np.random.default_rng(500 + k)
+        x, y, inten = S._positions(rng, shape, 0.03)
+        dx, dy = S.lamb_oseen(x, y, shape, rc=40.0, peak=3.0, uniform=(1.5, 0.0))
+        A.append(S.render(shape, x, y, inten).numpy())
+        B.append(S.render(shape, x + dx, y + dy, inten).numpy())
+    A = np.stack(A).astype(np.float32)
+    B = np.stack(B).astype(np.float32)
+    yy, xx = np.mgrid[0:shape[0], 0:shape[1]]
+    pm = ((yy - 90) ** 2 + (xx - 110) ** 2 < 35 ** 2) | (xx >= 236)
+    mp = MultipassPIV(shape, sched, [50, 50, 50], pixel_mask=pm, predictor_fill=fill, predictor_bcs=bcs)
+    res = mp.run(_t(A), _t(B), keep_passes=True)
+    torch.cuda.synchronize()
+    mp.check_infill()
+    grids = build_pass_grids(shape, sched, [50, 50, 50])
+    vm = M.vector_masks_loop(shape, sched, [50, 50, 50], pm, 0.5)
+    ref = M.run_multipass(A, B, grids, keep_passes=True, vector_masks=vm, predictor_fill=fill, predictor_bcs=bcs)
+    for p, (g, r) in enumerate(zip(res, ref)):
+        assert vm[p].any()
+        gx, gy = g.ux.cpu().numpy(), g.uy.cpu().numpy()
+        gm = g.nan_mask.cpu().numpy().astype(bool)
+        np.testing.assert_array_equal(gm, r["nan_mask"])
+        np.testing.assert_array_equal(g.nan_reason.cpu().numpy(), r["nan_reason"])
+        assert (gx[:, vm[p]] == 0).all() and (gy[:, vm[p]] == 0).all()
+        same = ~gm
+        d = np.hypot(gx[same] - r["ux"][same], gy[same] - r["uy"][same])
+        assert np.sqrt(np.mean(d ** 2)) < 0.005, (p, d.max())

@@ -120,3 +120,103 @@ def test_infill_nearest_tie_rule_matches_scipy():
             cand = np.nonzero(d == d.min())[0]
             k = cand[np.lexsort((ys[cand], xs[cand]))[0]]
             assert ref[0, 0, r, c] == u[0, 0, ys[k], xs[k]]
+
+
+def test_vector_masks_summed_area_equals_the_window_loop():
+    """The device loop's summed-area vector masks equal the reference's
+    per-window count, at every threshold, on blob and stripe masks."""
+    from pivtools_amd.window import vector_masks
+
+    shape = (160, 200)
+    yy, xx = np.mgrid[0:shape[0], 0:shape[1]]
+    masks = [((yy - 70) ** 2 + (xx - 90) ** 2 < 30 ** 2), (xx % 37 < 5) | (yy > 150),
+             np.random.default_rng(2).random(shape) < 0.3]
+    sched, ov = [(64, 64), (32, 32), (16, 16)], [50, 75, 50]
+    for pm in masks:
+        for thr in (0.0, 0.25, 0.5, 1.0):
+            a = vector_masks(shape, sched, ov, pm, thr)
+            b = M.vector_masks_loop(shape, sched, ov, pm, thr)
+            for x, y in zip(a, b):
+                np.testing.assert_array_equal(x, y)
+
+
+def test_vector_mask_threshold_is_strict():
+    """Exactly half a window masked is not more than 0.5 (hand count)."""
+    shape = (64, 64)
+    pm = np.zeros(shape, bool)
+    pm[:, :16] = True                       # the first 32x32 window: 16 of 32 columns
+    m = M.vector_masks_loop(shape, [(32, 32)], [50], pm, 0.5)[0]
+    assert not m[0, 0]
+    m = M.vector_masks_loop(shape, [(32, 32)], [50], pm, 0.49)[0]
+    assert m[0, 0] and not m[0, 1]          # the second window (x 16..47) holds none
+
+
+def _lstsq_fill(field, fill, spacing):
+    """Brute force: every first-difference equation that touches a fill
+    node, written out and solved with np.linalg.lstsq."""
+    ny, nx = field.shape
+    unk = {(r, c): i for i, (r, c) in enumerate(zip(*np.nonzero(fill)))}
+    rows, rhs = [], []
+    for (r, c), (r2, c2), h in ([((r, c), (r + 1, c), spacing[0]) for r in range(ny - 1) for c in range(nx)]
+                                + [((r, c), (r, c + 1), spacing[1]) for r in range(ny) for c in range(nx - 1)]):
+        if not (fill[r, c] or fill[r2, c2]):
+            continue
+        row = np.zeros(len(unk))
+        b = 0.0
+        for (rr, cc), s in (((r, c), -1.0 / h), ((r2, c2), 1.0 / h)):
+            if fill[rr, cc]:
+                row[unk[(rr, cc)]] += s
+            else:
+                b -= s * field[rr, cc]
+        rows.append(row)
+        rhs.append(b)
+    x = np.linalg.lstsq(np.array(rows), np.array(rhs), rcond=None)[0]
+    out = field.astype(np.float64).copy()
+    for (r, c), i in unk.items():
+        out[r, c] = x[i]
+    return out
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_harmonic_fill_against_lstsq_and_closed_forms(seed):
+    """The oracle's harmonic fill and the device path's host-side linear map
+    (predictor_fill.harmonic_operator) against explicit least squares, a
+    constant field and a linear field under an interior mask."""
+    from pivtools_amd.predictor_fill import harmonic_operator
+
+    rng = np.random.default_rng(seed)
+    ny, nx, sp_ = 7, 9, (16.0, 8.0)
+    fill = rng.random((ny, nx)) < 0.3
+    fill[0, 0] = True
+    f = rng.normal(0, 1, (ny, nx))
+    ref = _lstsq_fill(f, fill, sp_)
+    got = M.fill_harmonic(f[None], fill, sp_)[0]
+    np.testing.assert_allclose(got, ref, rtol=0, atol=1e-10)
+    fi, bi, gt = harmonic_operator(fill, sp_)
+    x = f.ravel().copy()
+    x[fi] = gt.T @ f.ravel()[bi]
+    np.testing.assert_allclose(x.reshape(ny, nx), ref, rtol=0, atol=1e-10)
+    np.testing.assert_allclose(M.fill_harmonic(np.full((1, ny, nx), 2.5), fill, sp_), 2.5, atol=1e-12)
+    inner = np.zeros((ny, nx), bool)
+    inner[2:5, 3:7] = True
+    yy, xx = np.mgrid[0:ny, 0:nx]
+    lin = 0.3 + 0.05 * xx * sp_[1] - 0.02 * yy * sp_[0]
+    np.testing.assert_allclose(M.fill_harmonic(lin[None], inner, sp_)[0], lin, atol=1e-12)
+
+
+def test_harmonic_operator_refuses_an_all_masked_grid():
+    from pivtools_amd.predictor_fill import harmonic_operator
+
+    with pytest.raises(ValueError):
+        harmonic_operator(np.ones((4, 5), bool), (8.0, 8.0))
+    fi, bi, gt = harmonic_operator(np.zeros((4, 5), bool), (8.0, 8.0))
+    assert fi.size == 0
+
+
+def test_predictor_bcs_overwrite_rows_beyond_the_position():
+    pad = np.zeros((2, 1, 6, 4), np.float32)
+    ya = np.array([-10.0, 5.0, 20.0, 35.0, 50.0, 70.0])
+    M.apply_predictor_bcs(pad, ya, 64, [{"edge": "bottom", "y_position": 20, "ux": 1.5, "uy": -0.5},
+                                        {"edge": "top", "y_position": 5, "ux": 0.25, "uy": 0.0}])
+    np.testing.assert_array_equal(pad[1, 0, :, 0], [0.25, 0.25, 0, 0, 1.5, 1.5])
+    np.testing.assert_array_equal(pad[0, 0, :, 0], [0, 0, 0, 0, -0.5, -0.5])
