@@ -269,10 +269,30 @@ int pivamd_gate(const float *pkx, const float *pky, const float *pkh, const floa
 int pivamd_median(const float *u, const float *pkh, const float *mask, int N, int ny, int nx,
                   float eps, float thr, int size, unsigned char *nanm, signed char *reason,
                   float *uo, float *pkh_out, void *stream);
+/* The outlier stage in general: the OR of the median test (use_median),
+ * the peak-magnitude test pkh < peak_thr or NaN (use_peak,
+ * outlier_detection.py:47-84) and precomputed flags d_extra (nullable),
+ * never at masked windows, followed by the median stage's bookkeeping.
+ * pivamd_median is this stage with the median test alone. */
+int pivamd_outlier_stage(const float *u, const float *pkh, const float *mask, int N, int ny, int nx,
+                         int use_median, float eps, float thr, int size, int use_peak, float peak_thr,
+                         const unsigned char *d_extra, unsigned char *nanm, signed char *reason, float *uo,
+                         float *pkh_out, void *stream);
 int pivamd_infill_nearest(const float *u, const float *mask, int N, int ny, int nx, float py,
                           float px, float *uo, int *d_err, void *stream);
 int pivamd_pad_edge(const float *u, int N, int ny, int nx, int pre_y, int pre_x, int nyp, int nxp,
                     float *out, void *stream);
+/* Least-squares infill of the NaN vectors of each image
+ * (infilling.py:144-260 _fill_least_squares via infill_harmonic (order 1)
+ * and infill_biharmonic (order 2)); d_mask (nullable) marks excluded cells
+ * (masked windows), neither donors nor filled.  FP64 normal equations solved
+ * by conjugate gradients to a relative residual of 1e-13, one workgroup per
+ * image; d_work is pivamd_infill_ls_workspace() bytes.  *d_err |= 2 if a
+ * solve misses the reference's 1e-6 acceptance, |= 4 for fill cells no
+ * difference row ties to a valid cell (left at 0; the reference raises). */
+unsigned long pivamd_infill_ls_workspace(int N, int ny, int nx);
+int pivamd_infill_ls(const float *u, const float *d_mask, int N, int ny, int nx, float py, float px,
+                     int order, float *uo, void *d_work, unsigned long work_bytes, int *d_err, void *stream);
 /* Masked windows of a predictor field before it is padded
  * (cpu_instantaneous.py:843-854 -> infilling.py:808-870 fill_masked_predictor):
  * 'nearest' fills every masked cell (d_mask != 0) from the nearest unmasked

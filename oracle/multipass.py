@@ -29,6 +29,11 @@ Pins:
                         squares (infilling.py:144-260), checked against
                         np.linalg.lstsq on the explicit equations, constant
                         and interior-linear fields
+  least-squares infill  infilling.py:144-260 restated (difference rows,
+                        normal equations, sparse LU), order 1 and 2 with
+                        excluded cells; checked against np.linalg.lstsq on
+                        the explicit equations and the fields each order
+                        reproduces exactly (constant; linear for order 2)
   predictor BCs         base.py:364-416 restated (row overwrite; unpinned
                         beyond its own definition)
 """
@@ -110,13 +115,31 @@ def median_outliers(ux, uy, eps=0.1, thr=2.0, size=5, masked=None):
 
 
 def median_stage(u, pkh, nanm, reason, mask=None, eps=0.1, thr=2.0, size=5):
-    """Outlier stage plus the bookkeeping of cpu_instantaneous.py:639-757."""
+    """Median test plus the bookkeeping of cpu_instantaneous.py:639-757."""
+    return outlier_stage(u, pkh, nanm, reason, mask,
+                         [{"type": "median_2d", "epsilon": eps, "threshold": thr, "size": size}])
+
+
+def outlier_stage(u, pkh, nanm, reason, mask, methods):
+    """apply_outlier_detection (outlier_detection.py:420-500) per image with
+    the median_2d and peak_mag methods, then the bookkeeping of
+    cpu_instantaneous.py:639-757."""
     N = u.shape[1]
     nanm = nanm.copy()
     reason = reason.copy()
     m = np.zeros(u.shape[2:], bool) if mask is None else mask.astype(bool)
     for i in range(N):
-        om = median_outliers(u[1, i], u[0, i], eps, thr, size, masked=m if mask is not None else None)
+        om = np.zeros(m.shape, bool)
+        for mc in methods:
+            t = mc["type"]
+            if t == "median_2d":
+                om |= median_outliers(u[1, i], u[0, i], mc.get("epsilon", 0.1), mc.get("threshold", 2.0),
+                                      mc.get("size", 5), masked=m if mask is not None else None)
+            elif t == "peak_mag":
+                om |= (pkh[i] < mc.get("threshold", 0.5)) | np.isnan(pkh[i])
+            else:
+                raise ValueError(t)
+        om &= ~m
         reason[i][om & (reason[i] == 0)] = 10
         nanm[i] |= om
     hn = np.isnan(pkh)
@@ -178,40 +201,52 @@ def vector_masks_loop(image_shape, window_sizes, overlaps, pixel_mask, threshold
     return out
 
 
-def _difference_rows(shape, spacing):
-    """First differences along y and x in physical units (infilling.py:144-179)."""
+def _difference_rows(shape, spacing, order=1):
+    """Differences of ``order`` along y and x in physical units: (-1, 1)/h
+    or (1, -2, 1)/h^2 at every stencil position inside the grid
+    (infilling.py:144-179)."""
     import scipy.sparse as sp
 
+    stencil = (-1.0, 1.0) if order == 1 else (1.0, -2.0, 1.0)
     idx = np.arange(shape[0] * shape[1]).reshape(shape)
     rows, cols, vals = [], [], []
     r = 0
     for axis, h in ((0, spacing[0]), (1, spacing[1])):
         n = shape[axis]
-        if n <= 1:
+        if n <= order:
             continue
-        a = idx[:-1, :] if axis == 0 else idx[:, :-1]
-        b = idx[1:, :] if axis == 0 else idx[:, 1:]
-        m = a.size
-        rr = r + np.arange(m)
-        rows += [rr, rr]
-        cols += [a.ravel(), b.ravel()]
-        vals += [np.full(m, -1.0 / h), np.full(m, 1.0 / h)]
+        m = None
+        rr = None
+        for j, coef in enumerate(stencil):
+            sl = [slice(None), slice(None)]
+            sl[axis] = slice(j, n - order + j)
+            c = idx[tuple(sl)].ravel()
+            if rr is None:
+                m = c.size
+                rr = r + np.arange(m)
+            rows.append(rr)
+            cols.append(c)
+            vals.append(np.full(m, coef / h ** order))
         r += m
     return sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
                          shape=(r, shape[0] * shape[1]))
 
 
-def fill_harmonic(fields, fill, spacing):
-    """Least-squares first-difference fill of a stack (K, ny, nx), unmasked
-    nodes fixed (infilling.py:182-260 with no excluded nodes)."""
+def fill_least_squares(fields, fill, spacing, order=1, exclude=None):
+    """Least-squares difference fill of a stack (K, ny, nx): equations that
+    touch a fill node and no excluded node, other nodes fixed, normal
+    equations by sparse LU in FP64 (infilling.py:182-260)."""
     from scipy.sparse.linalg import splu
 
     K, ny, nx = fields.shape
-    f = fill.ravel()
-    k = ~f
-    D = _difference_rows((ny, nx), spacing)
+    f = np.asarray(fill, bool).ravel()
+    e = np.zeros_like(f) if exclude is None else np.asarray(exclude, bool).ravel()
+    k = ~f & ~e
+    D = _difference_rows((ny, nx), spacing, order)
     Du = D[:, f].tocsr()
-    keep = np.diff(Du.indptr) > 0
+    touches_fill = np.diff(Du.indptr) > 0
+    touches_excl = np.diff(D[:, e].tocsr().indptr) > 0
+    keep = touches_fill & ~touches_excl
     Du, Dk = Du[keep], D[:, k].tocsr()[keep]
     A = (Du.T @ Du).tocsc()
     Uk = fields.reshape(K, -1)[:, k].T.astype(np.float64)
@@ -220,6 +255,29 @@ def fill_harmonic(fields, fill, spacing):
     out = fields.astype(np.float64).reshape(K, -1).copy()
     out[:, f] = X.T
     return out.reshape(K, ny, nx)
+
+
+def fill_harmonic(fields, fill, spacing):
+    return fill_least_squares(fields, fill, spacing, 1)
+
+
+def infill_ls(u, spacing, order, exclude=None):
+    """infill_harmonic / infill_biharmonic per image (infilling.py:336-410):
+    NaN vectors filled, excluded cells (and stray non-finite ones) neither
+    donors nor filled; the result is stored back as float32."""
+    out = u.astype(np.float32).copy()
+    for i in range(u.shape[1]):
+        ux, uy = u[1, i], u[0, i]
+        fill = np.isnan(ux) | np.isnan(uy)
+        if not fill.any():
+            continue
+        excl = ~fill & ~(np.isfinite(ux) & np.isfinite(uy))
+        if exclude is not None:
+            excl |= exclude
+        st = fill_least_squares(np.stack([uy, ux]).astype(np.float64), fill, spacing, order, excl)
+        out[0, i] = st[0].astype(np.float32)
+        out[1, i] = st[1].astype(np.float32)
+    return out
 
 
 def fill_masked_predictor(u, mask, spacing, method):
@@ -366,7 +424,7 @@ def xcorr_pass(A, B, g, w, finder, flavour, mask=None):
 
 def run_multipass(A, B, grids, window_type="gaussian", finder=3, smoothing=True, median=(0.1, 2.0, 5),
                   final_infill="nearest", flavour="oracle", interp=0, keep_passes=False, vector_masks=None,
-                  predictor_fill="harmonic", predictor_bcs=None):
+                  predictor_fill="harmonic", predictor_bcs=None, mid_infill="nearest", outlier_methods=None):
     """The whole pass loop on the CPU; A, B (N, H, W) float32."""
     from pivtools_amd.window import window_weight
 
@@ -393,11 +451,17 @@ def run_multipass(A, B, grids, window_type="gaussian", finder=3, smoothing=True,
         pkx, pky, pkh = xcorr_pass(a_in, b_in, g, w, finder, flavour, m)
         div = 2.0 if p == 0 else 4.0
         u, nm, reason, pkg = gate(pkx, pky, pkh, pred, m, g.win[1] / div, g.win[0] / div)
-        u, nm, reason, pko = median_stage(u, pkg, nm, reason, m, *median)
+        if outlier_methods is None:
+            u, nm, reason, pko = median_stage(u, pkg, nm, reason, m, *median)
+        else:
+            u, nm, reason, pko = outlier_stage(u, pkg, nm, reason, m, outlier_methods)
         final = p == len(grids) - 1
         sp_ = (float(g.spacing[0]), float(g.spacing[1]))
-        if not final or final_infill == "nearest":
+        method = mid_infill if not final else final_infill
+        if method == "nearest":
             u = infill_nearest(u, sp_, exclude=m)
+        elif method in ("harmonic", "biharmonic"):
+            u = infill_ls(u, sp_, 1 if method == "harmonic" else 2, exclude=m)
         if not final:
             src = fill_masked_predictor(u, m, sp_, predictor_fill)
             prev_pad = apply_predictor_bcs(pad_edge(src, g.n_pre, g.n_post), g.ctrs_y_all, A.shape[1],

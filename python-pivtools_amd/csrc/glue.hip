@@ -156,9 +156,13 @@ __device__ __forceinline__ bool median_outlier(const float *uy, const float *ux,
 
 // Normalised-median outlier test on u (planar, masked cells read as NaN),
 // then the end-of-stage bookkeeping of cpu_instantaneous.py:726-757.
+// The outlier methods combine by OR (outlier_detection.py:420-500
+// apply_outlier_detection): the median test (use_median), the peak
+// magnitude test pkh < peak_thr or NaN (use_peak, :47-84) and flags other
+// kernels computed into extra (nullable); masked windows are never flagged.
 __global__ void median_kernel(const float *u, const float *pkh, const float *mask, int N, int ny, int nx,
-                              float eps, float thr, int size, uint8_t *nanm, int8_t *reason, float *uo,
-                              float *pkh_out)
+                              int use_median, float eps, float thr, int size, int use_peak, float peak_thr,
+                              const uint8_t *extra, uint8_t *nanm, int8_t *reason, float *uo, float *pkh_out)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int nWin = ny * nx, total = N * nWin;
@@ -168,9 +172,13 @@ __global__ void median_kernel(const float *u, const float *pkh, const float *mas
     const float *ux = u + (size_t)total + (size_t)n * nWin;
     const bool self_masked = mask && mask[w] != 0.0f;
     bool outlier = false;
+    const float hv = pkh[i];
     if (!self_masked) {
-        outlier = size == 3 ? median_outlier<3>(uy, ux, mask, r, c, ny, nx, eps, thr)
-                            : median_outlier<5>(uy, ux, mask, r, c, ny, nx, eps, thr);
+        if (use_median)
+            outlier = size == 3 ? median_outlier<3>(uy, ux, mask, r, c, ny, nx, eps, thr)
+                                : median_outlier<5>(uy, ux, mask, r, c, ny, nx, eps, thr);
+        if (use_peak) outlier = outlier || !(hv >= peak_thr);
+        if (extra) outlier = outlier || extra[i] != 0;
     }
     bool nm = nanm[i] != 0;
     int8_t rs = reason[i];
@@ -178,7 +186,6 @@ __global__ void median_kernel(const float *u, const float *pkh, const float *mas
         nm = true;
         if (rs == 0) rs = 10;
     }
-    const float hv = pkh[i];
     if (hv != hv) {
         nm = true;
         if (rs == 0) rs = 13;
@@ -407,8 +414,21 @@ extern "C" int pivamd_median(const float *u, const float *pkh, const float *mask
     if (size != 3 && size != 5) return PIVAMD_ERR_ARG;
     const size_t n = (size_t)N * ny * nx;
     if (n == 0) return 0;
-    hipLaunchKernelGGL(median_kernel, dim3(nblk(n)), dim3(256), 0, st, u, pkh, mask, N, ny, nx, eps, thr, size,
-                       nanm, reason, uo, pkh_out);
+    hipLaunchKernelGGL(median_kernel, dim3(nblk(n)), dim3(256), 0, st, u, pkh, mask, N, ny, nx, 1, eps, thr, size,
+                       0, 0.0f, (const uint8_t *)nullptr, nanm, reason, uo, pkh_out);
+    return PIVAMD_LAUNCHED();
+}
+
+extern "C" int pivamd_outlier_stage(const float *u, const float *pkh, const float *mask, int N, int ny, int nx,
+                                    int use_median, float eps, float thr, int size, int use_peak, float peak_thr,
+                                    const uint8_t *extra, uint8_t *nanm, int8_t *reason, float *uo,
+                                    float *pkh_out, hipStream_t st)
+{
+    if (use_median && size != 3 && size != 5) return PIVAMD_ERR_ARG;
+    const size_t n = (size_t)N * ny * nx;
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(median_kernel, dim3(nblk(n)), dim3(256), 0, st, u, pkh, mask, N, ny, nx, use_median, eps, thr,
+                       size, use_peak, peak_thr, extra, nanm, reason, uo, pkh_out);
     return PIVAMD_LAUNCHED();
 }
 

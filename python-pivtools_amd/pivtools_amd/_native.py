@@ -76,6 +76,8 @@ def core() -> ctypes.CDLL:
         L.pivamd_warp_u16.restype = _i
         L.pivamd_peakfit.argtypes = [_vp, _i, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp]
         L.pivamd_peakfit.restype = _i
+        L.pivamd_infill_ls_workspace.argtypes = [_i, _i, _i]
+        L.pivamd_infill_ls_workspace.restype = ctypes.c_ulong
         for name, argt in GLUE_SIGNATURES.items():
             f = getattr(L, name)
             f.argtypes = argt
@@ -122,10 +124,12 @@ GLUE_SIGNATURES: dict = {
     "pivamd_gate": [_vp, _vp, _vp, _vp, _vp, _i, _i, _f, _f, _vp, _vp, _vp, _vp, _vp],
     "pivamd_median": [_vp, _vp, _vp, _i, _i, _i, _f, _f, _i, _vp, _vp, _vp, _vp, _vp],
     "pivamd_infill_nearest": [_vp, _vp, _i, _i, _i, _f, _f, _vp, _vp, _vp],
+    "pivamd_outlier_stage": [_vp, _vp, _vp, _i, _i, _i, _i, _f, _f, _i, _i, _f, _vp, _vp, _vp, _vp, _vp, _vp],
     "pivamd_pad_edge": [_vp, _i, _i, _i, _i, _i, _i, _i, _vp, _vp],
     "pivamd_smooth_axis": [_vp, _i, _i, _i, _i, _vp, _i, _vp, _vp],
     "pivamd_remap_cubic": [_vp, _i, _i, _i, _vp, _vp, _i, _i, _vp, _vp],
     "pivamd_fill_masked_nearest": [_vp, _vp, _i, _i, _i, _f, _f, _vp, _vp, _vp],
+    "pivamd_infill_ls": [_vp, _vp, _i, _i, _i, _f, _f, _i, _vp, _vp, ctypes.c_ulong, _vp, _vp],
     "pivamd_fill_linear": [_vp, _i, ctypes.c_long, _i, _vp, _i, _vp, _vp, _vp],
     "pivamd_unpack_mono12p": [_vp, _sz, _i, _i, _i, _vp, _vp, _vp],
     "pivamd_bayer_rg8_superpixel": [_vp, _sz, _i, _i, _i, _vp, _vp, _vp],

@@ -94,15 +94,19 @@ class MultipassPIV:
 
     def __init__(self, image_shape, window_sizes, overlaps, *, window_type="gaussian", peak_finder="gauss3",
                  peak_fit_impl="scalar", warp_interpolation="cubic", predictor_smoothing=True,
-                 median=(0.1, 2.0, 5), final_infill="nearest", dft="fft", device="cuda",
+                 median=(0.1, 2.0, 5), final_infill="nearest", dft="fft", device="cuda", mid_infill="nearest",
                  vector_masks=None, pixel_mask=None, mask_threshold=0.5, predictor_fill="harmonic",
-                 predictor_bcs=None):
+                 predictor_bcs=None, outlier_methods=None):
         """``vector_masks``: one (n_win_y, n_win_x) bool array per pass (True =
         masked window), or ``pixel_mask``: an (H, W) bool image mask turned
         into them with ``mask_threshold``.  ``predictor_fill``: 'harmonic',
         'nearest' or 'none' for the masked windows of each predictor field.
         ``predictor_bcs``: list of {"edge": "top"|"bottom", "y_position",
         "ux", "uy"} overwriting the padded predictor rows beyond y_position.
+        ``outlier_methods``: the reference's method list (config
+        outlier_detection.methods), e.g. [{"type": "peak_mag", "threshold":
+        0.25}, {"type": "median_2d", "epsilon": 0.1, "threshold": 2}]; when
+        given it replaces ``median``; an empty list runs no outlier test.
         """
         self.H, self.W = image_shape
         if dft not in ("fft", "split"):
@@ -117,9 +121,33 @@ class MultipassPIV:
         self.interp = 0 if warp_interpolation == "cubic" else 1
         self.smoothing = bool(predictor_smoothing)
         self.median = median
-        if final_infill not in ("nearest", "none"):
+        self.outliers = {"use_median": 0, "eps": 0.1, "thr": 2.0, "size": 5, "use_peak": 0, "peak_thr": 0.5}
+        if outlier_methods is None:
+            if median is not None:
+                self.outliers.update(use_median=1, eps=float(median[0]), thr=float(median[1]), size=int(median[2]))
+        else:
+            for mcfg in outlier_methods:
+                t = str(mcfg.get("type", "")).lower()
+                if t == "median_2d":
+                    self.outliers.update(use_median=1, eps=float(mcfg.get("epsilon", 0.1)),
+                                         thr=float(mcfg.get("threshold", 2.0)), size=int(mcfg.get("size", 5)))
+                elif t == "peak_mag":
+                    # outlier_detection.py:463-470: a method entry without a threshold gets 0.5
+                    self.outliers.update(use_peak=1, peak_thr=float(mcfg.get("threshold", 0.5)))
+                elif t in ("sigma", "div_vort"):
+                    raise NotImplementedError(f"outlier method {t!r} is not on the device path")
+                else:
+                    raise ValueError(f"Unknown outlier detection method: {t}")
+        if self.outliers["use_median"] and self.outliers["size"] not in (3, 5):
+            raise NotImplementedError("the device median test has 3x3 and 5x5 neighbourhoods")
+        # infilling.py:698-790 apply_infilling: the device path has the
+        # nearest-donor and the least-squares (harmonic, biharmonic) methods
+        if final_infill not in ("nearest", "harmonic", "biharmonic", "none"):
             raise NotImplementedError(f"final-pass infill '{final_infill}' is not on the device path")
+        if mid_infill not in ("nearest", "harmonic", "biharmonic"):
+            raise NotImplementedError(f"mid-pass infill '{mid_infill}' is not on the device path")
         self.final_infill = final_infill
+        self.mid_infill = mid_infill
         self.device = torch.device(device)
         self.lib = _native.core()
         for win in self.window_sizes:
@@ -209,6 +237,9 @@ class MultipassPIV:
             b.out["pad"] = torch.empty((2, n, nyp, nxp), dtype=f32, device=dev)
             b.out["pad_s"] = torch.empty((2, n, nyp, nxp), dtype=f32, device=dev)
             b.out["pad_t"] = torch.empty((2, n, nyp, nxp), dtype=f32, device=dev)
+            if (self.final_infill if final else self.mid_infill) in ("harmonic", "biharmonic"):
+                b.out["ls_work"] = torch.empty(max(4, self.lib.pivamd_infill_ls_workspace(n, ny, nx)),
+                                               dtype=torch.uint8, device=dev)
             ws = self.lib.pivamd_xcorr_workspace(n, ny, nx, g.win[0], g.win[1], 1, 0)
             b.work = torch.empty(max(ws, 4), dtype=torch.uint8, device=dev)
             if p > 0:
@@ -308,19 +339,25 @@ class MultipassPIV:
             _native.check(L.pivamd_gate(_ptr(o["pkx"]), _ptr(o["pky"]), _ptr(o["pkh"]), _ptr(pred), _ptr(b.mask), n,
                                         ny * nx, g.win[1] / div, g.win[0] / div, _ptr(o["u"]), _ptr(o["nanm"]),
                                         _ptr(o["reason"]), _ptr(o["pkh_g"]), st), "gate")
-            if self.median is not None:
-                eps, thr, size = self.median
-                _native.check(L.pivamd_median(_ptr(o["u"]), _ptr(o["pkh_g"]), _ptr(b.mask), n, ny, nx, eps, thr, size,
-                                              _ptr(o["nanm"]), _ptr(o["reason"]), _ptr(o["u2"]), _ptr(o["pkh_o"]),
-                                              st), "median test")
-                u = o["u2"]
-            else:
-                raise NotImplementedError("the device pass loop runs with the median test")
-            if not final or self.final_infill == "nearest":
+            oc = self.outliers
+            _native.check(L.pivamd_outlier_stage(_ptr(o["u"]), _ptr(o["pkh_g"]), _ptr(b.mask), n, ny, nx,
+                                                 oc["use_median"], oc["eps"], oc["thr"], oc["size"], oc["use_peak"],
+                                                 oc["peak_thr"], None, _ptr(o["nanm"]), _ptr(o["reason"]),
+                                                 _ptr(o["u2"]), _ptr(o["pkh_o"]), st), "outlier stage")
+            u = o["u2"]
+            method = self.final_infill if final else self.mid_infill
+            if method == "nearest":
                 o["err"].zero_()
                 _native.check(L.pivamd_infill_nearest(_ptr(u), _ptr(b.mask), n, ny, nx, float(g.spacing[0]),
                                                       float(g.spacing[1]), _ptr(o["u3"]), _ptr(o["err"]), st),
                               "infill")
+                u = o["u3"]
+            elif method in ("harmonic", "biharmonic"):
+                o["err"].zero_()
+                wk = o["ls_work"]
+                _native.check(L.pivamd_infill_ls(_ptr(u), _ptr(b.mask), n, ny, nx, float(g.spacing[0]),
+                                                 float(g.spacing[1]), 1 if method == "harmonic" else 2,
+                                                 _ptr(o["u3"]), _ptr(wk), wk.numel(), _ptr(o["err"]), st), "infill")
                 u = o["u3"]
             if not final:
                 nyp, nxp = len(g.ctrs_y_all), len(g.ctrs_x_all)
@@ -351,8 +388,17 @@ class MultipassPIV:
         return results
 
     def check_infill(self) -> None:
-        """Raise if any infill of the last batch size found no donor (a fully
-        invalid field; that field is then carried on as zero displacement)."""
-        for b in self._bufs or []:
-            if int(b.out["err"].item()) != 0:
-                raise ValueError("infilling has no donor points: 100% of vectors invalid")
+        """Raise as the reference's infill would for the last batch: a field
+        with no donor (nearest; carried on as zero displacement), a
+        least-squares fill that missed the 1e-6 residual acceptance, or fill
+        cells no difference equation ties to a valid cell (left at 0)."""
+        for p, b in enumerate(self._bufs or []):
+            e = int(b.out["err"].item())
+            if e & 1:
+                raise ValueError(f"pass {p}: infilling has no donor points: 100% of vectors invalid")
+            if e & 4:
+                raise ValueError(f"pass {p}: least-squares infill: a group of fill cells is not linked "
+                                 "to any valid vector")
+            if e & 2:
+                raise np.linalg.LinAlgError(f"pass {p}: least-squares infill did not reach a relative "
+                                            "residual of 1e-6")

@@ -321,3 +321,102 @@ def test_masked_multipass_matches_oracle_loop(fill, bcs):
         same = ~gm
         d = np.hypot(gx[same] - r["ux"][same], gy[same] - r["uy"][same])
         assert np.sqrt(np.mean(d ** 2)) < 0.005, (p, d.max())
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_least_squares_infill_matches_oracle(order):
+    """Harmonic / biharmonic infill (conjugate gradients on the device,
+    sparse LU in the oracle): isolated outliers, a cluster, a large hole, a
+    hole open to the frame edge, excluded (masked) cells."""
+    L = _lib()
+    N, ny, nx = 3, 40, 52
+    sp_ = (16.0, 12.0)
+    rng = np.random.default_rng(20 + order)
+    yy, xx = np.mgrid[0:ny, 0:nx]
+    u = np.stack([np.sin(xx / 7.0 + k) + 0.1 * rng.normal(size=(ny, nx)) for k in range(2 * N)])
+    u = u.reshape(2, N, ny, nx).astype(np.float32)
+    ex = np.zeros((ny, nx), bool)
+    ex[5:9, 30:36] = True
+    u[:, :, ex] = 0.0
+    holes = rng.random((N, ny, nx)) < 0.04
+    holes[1, 15:27, 10:24] = True            # a large hole
+    holes[2, 0:5, 40:52] = True              # open to the frame edge
+    holes[:, ex] = False
+    for i in range(N):
+        u[1, i][holes[i]] = np.nan
+    u[0, 0, 3, 3] = np.nan                   # one component only: the vector is filled whole
+    print(f"infill_ls order={order} holes={int(holes.sum())}", flush=True)
+    du = _t(u)
+    out = torch.empty_like(du)
+    ws = L.pivamd_infill_ls_workspace(N, ny, nx)
+    work = torch.empty(ws, dtype=torch.uint8, device=dev)
+    err = torch.zeros(1, dtype=torch.int32, device=dev)
+    assert L.pivamd_infill_ls(du.data_ptr(), _t(ex.astype(np.float32)).data_ptr(), N, ny, nx, sp_[0], sp_[1],
+                              order, out.data_ptr(), work.data_ptr(), ws, err.data_ptr(), _st()) == 0
+    torch.cuda.synchronize()
+    assert int(err.item()) == 0
+    ref = M.infill_ls(u, sp_, order, exclude=ex)
+    got = out.cpu().numpy()
+    assert np.isfinite(got).all()
+    np.testing.assert_allclose(got, ref, rtol=2e-6, atol=2e-6)
+
+
+def test_least_squares_infill_flags_an_unlinked_group():
+    """A NaN vector walled in by excluded cells and the frame corner has no
+    difference equation to a valid cell: flagged (the reference raises)."""
+    L = _lib()
+    N, ny, nx = 1, 8, 8
+    u = np.ones((2, N, ny, nx), np.float32)
+    u[:, 0, 0, 0] = np.nan
+    ex = np.zeros((ny, nx), bool)
+    ex[0, 1] = ex[1, 0] = ex[1, 1] = True
+    du = _t(u)
+    out = torch.empty_like(du)
+    ws = L.pivamd_infill_ls_workspace(N, ny, nx)
+    work = torch.empty(ws, dtype=torch.uint8, device=dev)
+    err = torch.zeros(1, dtype=torch.int32, device=dev)
+    assert L.pivamd_infill_ls(du.data_ptr(), _t(ex.astype(np.float32)).data_ptr(), N, ny, nx, 8.0, 8.0, 1,
+                              out.data_ptr(), work.data_ptr(), ws, err.data_ptr(), _st()) == 0
+    torch.cuda.synchronize()
+    assert int(err.item()) & 4
+
+
+@pytest.mark.parametrize("finder", [3, 6])
+def test_shipped_default_configuration_matches_oracle_loop(finder):
+    """The reference's shipped outlier and infill settings: peak magnitude
+    0.25 and the median test, biharmonic infill mid-pass and final."""
+    shape = (192, 256)
+    sched = [(64, 64), (32, 32), (16, 16)]
+    methods = [{"type": "peak_mag", "threshold": 0.25},
+               {"type": "median_2d", "epsilon": 0.1, "threshold": 2}]
+    A, B = [], []
+    for k in range(2):
+        rng = np.random.default_rng(700 + k)
+        x, y, inten = S._positions(rng, shape, 0.03)
+        dx, dy = S.lamb_oseen(x, y, shape, rc=40.0, peak=3.0, uniform=(1.5, 0.0))
+        A.append(S.render(shape, x, y, inten).numpy())
+        B.append(S.render(shape, x + dx, y + dy, inten).numpy())
+    A = np.stack(A).astype(np.float32)
+    B = np.stack(B).astype(np.float32)
+    print(f"shipped defaults finder={finder}", flush=True)
+    mp = MultipassPIV(shape, sched, [50, 50, 50], peak_finder=finder, outlier_methods=methods,
+                      mid_infill="biharmonic", final_infill="biharmonic")
+    res = mp.run(_t(A), _t(B), keep_passes=True)
+    torch.cuda.synchronize()
+    mp.check_infill()
+    grids = build_pass_grids(shape, sched, [50, 50, 50])
+    ref = M.run_multipass(A, B, grids, finder=finder, keep_passes=True, outlier_methods=methods,
+                          mid_infill="biharmonic", final_infill="biharmonic")
+    for p, (g, r) in enumerate(zip(res, ref)):
+        gx, gy = g.ux.cpu().numpy(), g.uy.cpu().numpy()
+        gm = g.nan_mask.cpu().numpy().astype(bool)
+        assert np.isfinite(gx).all() and np.isfinite(gy).all()
+        flips = np.mean(gm != r["nan_mask"])
+        d = np.hypot(gx - r["ux"], gy - r["uy"])
+        if finder == 3:
+            assert flips == 0.0, (p, flips)
+            np.testing.assert_array_equal(g.nan_reason.cpu().numpy(), r["nan_reason"])
+            assert np.sqrt(np.mean(d ** 2)) < 0.005, (p, d.max())
+        else:
+            assert flips <= 0.05, (p, flips)
+            assert np.median(d) < 1e-3 and np.mean(d > 0.005) < 0.15, (p, np.mean(d > 0.005))

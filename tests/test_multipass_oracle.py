@@ -220,3 +220,72 @@ def test_predictor_bcs_overwrite_rows_beyond_the_position():
                                         {"edge": "top", "y_position": 5, "ux": 0.25, "uy": 0.0}])
     np.testing.assert_array_equal(pad[1, 0, :, 0], [0.25, 0.25, 0, 0, 1.5, 1.5])
     np.testing.assert_array_equal(pad[0, 0, :, 0], [0, 0, 0, 0, -0.5, -0.5])
+
+
+def _lstsq_fill_general(field, fill, spacing, order, exclude):
+    """Brute force for any order with excluded cells: the explicit difference
+    equations touching a fill cell and no excluded cell, by np.linalg.lstsq."""
+    ny, nx = field.shape
+    unk = {(r, c): i for i, (r, c) in enumerate(zip(*np.nonzero(fill)))}
+    st = (-1.0, 1.0) if order == 1 else (1.0, -2.0, 1.0)
+    eqs = []
+    for r in range(ny):
+        for c in range(nx):
+            for axis, h in ((0, spacing[0]), (1, spacing[1])):
+                cells = [(r + j, c) if axis == 0 else (r, c + j) for j in range(order + 1)]
+                if any(a >= ny or b >= nx for a, b in cells):
+                    continue
+                eqs.append((cells, [s / h ** order for s in st]))
+    rows, rhs = [], []
+    for cells, co in eqs:
+        if not any(fill[a, b] for a, b in cells) or any(exclude[a, b] for a, b in cells):
+            continue
+        row = np.zeros(len(unk))
+        bv = 0.0
+        for (a, b), s in zip(cells, co):
+            if fill[a, b]:
+                row[unk[(a, b)]] += s
+            else:
+                bv -= s * field[a, b]
+        rows.append(row)
+        rhs.append(bv)
+    x = np.linalg.lstsq(np.array(rows), np.array(rhs), rcond=None)[0]
+    out = field.astype(np.float64).copy()
+    for (a, b), i in unk.items():
+        out[a, b] = x[i]
+    return out
+
+
+@pytest.mark.parametrize("order", [1, 2])
+@pytest.mark.parametrize("seed", [0, 1])
+def test_least_squares_infill_against_lstsq_and_exact_fields(order, seed):
+    rng = np.random.default_rng(10 + seed)
+    ny, nx, sp_ = 8, 9, (16.0, 12.0)
+    fill = rng.random((ny, nx)) < 0.2
+    excl = (rng.random((ny, nx)) < 0.1) & ~fill
+    f = rng.normal(0, 1, (ny, nx))
+    ref = _lstsq_fill_general(f, fill, sp_, order, excl)
+    got = M.fill_least_squares(f[None], fill, sp_, order, excl)[0]
+    np.testing.assert_allclose(got[~excl], ref[~excl], rtol=0, atol=1e-9)
+    yy, xx = np.mgrid[0:ny, 0:nx]
+    const = np.full((1, ny, nx), -0.75)
+    np.testing.assert_allclose(M.fill_least_squares(const, fill, sp_, order, excl), const, atol=1e-12)
+    if order == 2:
+        # the biharmonic fill reproduces a linear field for any hole, open edges included
+        lin = 0.4 + 0.03 * xx * sp_[1] - 0.01 * yy * sp_[0]
+        np.testing.assert_allclose(M.fill_least_squares(lin[None], fill, sp_, 2, excl)[0], lin, atol=1e-10)
+
+
+def test_infill_ls_fills_nan_vectors_only():
+    rng = np.random.default_rng(3)
+    u = rng.normal(0, 1, (2, 2, 6, 7)).astype(np.float32)
+    u[:, 0, 2, 3] = np.nan
+    u[1, 1, 0, 0] = np.nan
+    ex = np.zeros((6, 7), bool)
+    ex[4, 4] = True
+    out = M.infill_ls(u, (8.0, 8.0), 2, exclude=ex)
+    assert np.isfinite(out).all()
+    # a vector with one NaN component is filled whole (both components)
+    keep = np.broadcast_to(~np.isnan(u).any(axis=0), u.shape)
+    np.testing.assert_array_equal(out[keep], u[keep])
+    assert out[0, 1, 0, 0] != u[0, 1, 0, 0]
