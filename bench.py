@@ -63,7 +63,7 @@ def make_batch(n_pairs, device):
     return A, B
 
 
-def cpu_baseline(A_dev, B_dev, gpu_final, finder, budget_s):
+def cpu_baseline(A_dev, B_dev, gpu_final, finder, budget_s, loop_kw=None):
     """Oracle (C restatement, OpenMP) timed on a bounded sample of the same
     workload -- one full-size pair -- plus the accuracy of the device result
     for that pair against the FP64 NumPy oracle."""
@@ -78,7 +78,8 @@ def cpu_baseline(A_dev, B_dev, gpu_final, finder, budget_s):
     a = A_dev[0:1].cpu().numpy().astype(np.float32)
     b = B_dev[0:1].cpu().numpy().astype(np.float32)
     t0 = time.perf_counter()
-    ref = M.run_multipass(a, b, grids, finder=finder)[0]
+    loop_kw = loop_kw or {}
+    ref = M.run_multipass(a, b, grids, finder=finder, **loop_kw)[0]
     t_cpu = time.perf_counter() - t0
     out = {"value": round(1.0 / t_cpu, 4), "unit": "pairs/s", "cores": int(O.lib().orc_max_threads()),
            "kind": "port", "sample": "1 pair of the same workload (2160x2560, 4 passes), C oracle + NumPy glue",
@@ -91,7 +92,7 @@ def cpu_baseline(A_dev, B_dev, gpu_final, finder, budget_s):
     acc = {"rms_px_vs_oracle": rms_oracle, "mask_flip_frac_vs_oracle": flips}
     if finder == 3 and budget_s > 0:
         t0 = time.perf_counter()
-        r64 = M.run_multipass(a, b, grids, finder=3, flavour="fp64")[0]
+        r64 = M.run_multipass(a, b, grids, finder=3, flavour="fp64", **loop_kw)[0]
         fx, fy, fm = r64["ux"][0], r64["uy"][0], r64["nan_mask"][0]
         same = ~gm & ~fm
         acc["rms_px_vs_fp64"] = float(np.sqrt(np.mean((gx[same] - fx[same]) ** 2 + (gy[same] - fy[same]) ** 2)))
@@ -398,6 +399,10 @@ def main():
                     help="keep the default correlation-kernel variants instead of timing them first")
     ap.add_argument("--one-pair-steps", type=int, default=10,
                     help="after the timed region, time the same loop fed one pair per step (0 disables)")
+    ap.add_argument("--infill", choices=["nearest", "harmonic", "biharmonic"], default="nearest",
+                    help="infill of invalid vectors, mid-pass and final")
+    ap.add_argument("--peak-mag", type=float, default=None,
+                    help="add the peak-magnitude outlier test with this threshold")
     ap.add_argument("--host-fed-steps", type=int, default=4,
                     help="after the timed region, time the loop fed from pinned host memory, 16-bit frames "
                          "and Mono12p payloads (0 disables); reported beside value, never as value")
@@ -432,7 +437,12 @@ def main():
     log(f"[bench] generating {args.pairs} pairs of {SHAPE} uint16 on the device")
     A, B = make_batch(args.pairs, dev)
     torch.cuda.synchronize()
-    mp = MultipassPIV(SHAPE, SCHEDULE, OVERLAP, peak_finder=args.finder, peak_fit_impl=args.impl, dft=args.dft)
+    methods = None
+    if args.peak_mag is not None:
+        methods = [{"type": "peak_mag", "threshold": args.peak_mag},
+                   {"type": "median_2d", "epsilon": 0.1, "threshold": 2, "size": 5}]
+    mp = MultipassPIV(SHAPE, SCHEDULE, OVERLAP, peak_finder=args.finder, peak_fit_impl=args.impl, dft=args.dft,
+                      outlier_methods=methods, mid_infill=args.infill, final_infill=args.infill)
     tuned = None
     if args.autotune and args.dft == "fft":
         from pivtools_amd import autotune
@@ -519,7 +529,8 @@ def main():
     acc = {}
     if args.cpu_budget > 0:
         log("[bench] CPU oracle leg on one pair")
-        cpu, acc = cpu_baseline(A, B, (gx, gy, gm), mp.finder, args.cpu_budget)
+        cpu, acc = cpu_baseline(A, B, (gx, gy, gm), mp.finder, args.cpu_budget,
+                                {"outlier_methods": methods, "mid_infill": args.infill, "final_infill": args.infill})
 
     line = {
         "metric": "image pairs/sec + p50/p99 pair latency at <=0.01 px RMS error, 1 GPU",
@@ -536,7 +547,9 @@ def main():
         "data": "synthetic",
         "config": {"workload": "multipass_2160x2560_u16", "pairs_per_step": args.pairs,
                    "schedule": [w[0] for w in SCHEDULE], "overlap": 50, "peak_finder": args.finder,
-                   "peak_fit_impl": args.impl, "dft": args.dft, "warp": "bicubic", "outlier": "median 5x5 eps 0.1 thr 2",
+                   "peak_fit_impl": args.impl, "dft": args.dft, "warp": "bicubic",
+                   "outlier": "median 5x5 eps 0.1 thr 2" + (f" + peak_mag {args.peak_mag}" if args.peak_mag is not None else ""),
+                   "infill": args.infill,
                    "global_batch": args.pairs, "parallelism": "single-gpu",
                    "xcorr_variants": ({str(n): r["variant"] for n, r in tuned.items()} if tuned else "default")},
         "latency_ms_p50": round(float(np.percentile(steps_ms, 50)), 3),
