@@ -278,6 +278,19 @@ int pivamd_outlier_stage(const float *u, const float *pkh, const float *mask, in
                          int use_median, float eps, float thr, int size, int use_peak, float peak_thr,
                          const unsigned char *d_extra, unsigned char *nanm, signed char *reason, float *uo,
                          float *pkh_out, void *stream);
+/* The sigma (outlier_detection.py:244-313; field 0 = vector norm, 1 = ux,
+ * 2 = uy) and divergence/vorticity (:316-373; NaN thresholds = 6 * 1.4826 *
+ * MAD of the image's finite values) outlier methods on the gated planar
+ * field u, masked windows read as NaN; flags are OR-ed into d_extra (one
+ * byte per window) for pivamd_outlier_stage.  d_work is
+ * pivamd_outlier_methods_workspace() bytes. */
+unsigned long pivamd_outlier_methods_workspace(int N, int ny, int nx);
+int pivamd_outlier_sigma(const float *u, const float *d_mask, int N, int ny, int nx, float sigma_thr,
+                         int field, unsigned char *d_extra, void *d_work, unsigned long work_bytes,
+                         void *stream);
+int pivamd_outlier_div_vort(const float *u, const float *d_mask, int N, int ny, int nx, float div_thr,
+                            float vort_thr, unsigned char *d_extra, void *d_work, unsigned long work_bytes,
+                            void *stream);
 int pivamd_infill_nearest(const float *u, const float *mask, int N, int ny, int nx, float py,
                           float px, float *uo, int *d_err, void *stream);
 int pivamd_pad_edge(const float *u, int N, int ny, int nx, int pre_y, int pre_x, int nyp, int nxp,

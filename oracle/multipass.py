@@ -120,6 +120,53 @@ def median_stage(u, pkh, nanm, reason, mask=None, eps=0.1, thr=2.0, size=5):
                          [{"type": "median_2d", "epsilon": eps, "threshold": thr, "size": size}])
 
 
+def sigma_outliers(field, k):
+    """outlier_detection.py:244-313 on one scalar field: 8-neighbour mean and
+    standard deviation from scipy.ndimage.uniform_filter (the call the
+    reference makes), float32 throughout."""
+    v = np.asarray(field).astype(np.float32)
+    fin = np.isfinite(v)
+    v0 = np.where(fin, v, np.float32(0.0)).astype(np.float32)
+    cnt = fin.astype(np.float32)
+    s9 = ndi.uniform_filter(v0, size=3, mode="constant", cval=0.0) * np.float32(9.0)
+    q9 = ndi.uniform_filter(v0 * v0, size=3, mode="constant", cval=0.0) * np.float32(9.0)
+    c9 = ndi.uniform_filter(cnt, size=3, mode="constant", cval=0.0) * np.float32(9.0)
+    s8 = s9 - v0
+    q8 = q9 - v0 * v0
+    c8 = c9 - cnt
+    with np.errstate(invalid="ignore", divide="ignore"):
+        m8 = s8 / c8
+        var = q8 / c8 - m8 * m8
+    var = np.maximum(var, np.float32(0.0))
+    sd = np.sqrt(var)
+    with np.errstate(invalid="ignore"):
+        return (np.abs(v - m8) > np.float32(k) * sd) | ~fin | ~np.isfinite(m8) | (sd == 0) | (c8 < 1)
+
+
+def div_vort_outliers(ux, uy, div_thresh=None, vort_thresh=None):
+    """outlier_detection.py:316-373: periodic central differences, thresholds
+    6 * 1.4826 * MAD of the finite values unless given."""
+    h = np.float32(0.5)
+    dudx = h * (np.roll(ux, -1, 1) - np.roll(ux, 1, 1))
+    dudy = h * (np.roll(ux, -1, 0) - np.roll(ux, 1, 0))
+    dvdx = h * (np.roll(uy, -1, 1) - np.roll(uy, 1, 1))
+    dvdy = h * (np.roll(uy, -1, 0) - np.roll(uy, 1, 0))
+    div = dudx + dvdy
+    vort = dvdx - dudy
+
+    def auto(x):
+        f = x[np.isfinite(x)]
+        if f.size == 0:
+            return np.float32(np.inf)
+        med = np.median(f)
+        return np.float32(6) * (np.float32(1.4826) * np.median(np.abs(f - med)))
+
+    td = auto(div) if div_thresh is None else np.float32(div_thresh)
+    tv = auto(vort) if vort_thresh is None else np.float32(vort_thresh)
+    with np.errstate(invalid="ignore"):
+        return (np.abs(div) > td) | (np.abs(vort) > tv)
+
+
 def outlier_stage(u, pkh, nanm, reason, mask, methods):
     """apply_outlier_detection (outlier_detection.py:420-500) per image with
     the median_2d and peak_mag methods, then the bookkeeping of
@@ -137,6 +184,16 @@ def outlier_stage(u, pkh, nanm, reason, mask, methods):
                                       mc.get("size", 5), masked=m if mask is not None else None)
             elif t == "peak_mag":
                 om |= (pkh[i] < mc.get("threshold", 0.5)) | np.isnan(pkh[i])
+            elif t in ("sigma", "div_vort"):
+                ux = u[1, i].astype(np.float32)
+                uy = u[0, i].astype(np.float32)
+                if mask is not None:
+                    ux = np.where(m, np.float32(np.nan), ux)
+                    uy = np.where(m, np.float32(np.nan), uy)
+                if t == "sigma":
+                    om |= sigma_outliers(np.sqrt(ux ** 2 + uy ** 2), mc.get("sigma_threshold", 2.0))
+                else:
+                    om |= div_vort_outliers(ux, uy, mc.get("div_thresh"), mc.get("vort_thresh"))
             else:
                 raise ValueError(t)
         om &= ~m

@@ -78,6 +78,8 @@ def core() -> ctypes.CDLL:
         L.pivamd_peakfit.restype = _i
         L.pivamd_infill_ls_workspace.argtypes = [_i, _i, _i]
         L.pivamd_infill_ls_workspace.restype = ctypes.c_ulong
+        L.pivamd_outlier_methods_workspace.argtypes = [_i, _i, _i]
+        L.pivamd_outlier_methods_workspace.restype = ctypes.c_ulong
         for name, argt in GLUE_SIGNATURES.items():
             f = getattr(L, name)
             f.argtypes = argt
@@ -130,6 +132,8 @@ GLUE_SIGNATURES: dict = {
     "pivamd_remap_cubic": [_vp, _i, _i, _i, _vp, _vp, _i, _i, _vp, _vp],
     "pivamd_fill_masked_nearest": [_vp, _vp, _i, _i, _i, _f, _f, _vp, _vp, _vp],
     "pivamd_infill_ls": [_vp, _vp, _i, _i, _i, _f, _f, _i, _vp, _vp, ctypes.c_ulong, _vp, _vp],
+    "pivamd_outlier_sigma": [_vp, _vp, _i, _i, _i, _f, _i, _vp, _vp, ctypes.c_ulong, _vp],
+    "pivamd_outlier_div_vort": [_vp, _vp, _i, _i, _i, _f, _f, _vp, _vp, ctypes.c_ulong, _vp],
     "pivamd_fill_linear": [_vp, _i, ctypes.c_long, _i, _vp, _i, _vp, _vp, _vp],
     "pivamd_unpack_mono12p": [_vp, _sz, _i, _i, _i, _vp, _vp, _vp],
     "pivamd_bayer_rg8_superpixel": [_vp, _sz, _i, _i, _i, _vp, _vp, _vp],

@@ -121,7 +121,8 @@ class MultipassPIV:
         self.interp = 0 if warp_interpolation == "cubic" else 1
         self.smoothing = bool(predictor_smoothing)
         self.median = median
-        self.outliers = {"use_median": 0, "eps": 0.1, "thr": 2.0, "size": 5, "use_peak": 0, "peak_thr": 0.5}
+        self.outliers = {"use_median": 0, "eps": 0.1, "thr": 2.0, "size": 5, "use_peak": 0, "peak_thr": 0.5,
+                         "extra": []}
         if outlier_methods is None:
             if median is not None:
                 self.outliers.update(use_median=1, eps=float(median[0]), thr=float(median[1]), size=int(median[2]))
@@ -134,8 +135,13 @@ class MultipassPIV:
                 elif t == "peak_mag":
                     # outlier_detection.py:463-470: a method entry without a threshold gets 0.5
                     self.outliers.update(use_peak=1, peak_thr=float(mcfg.get("threshold", 0.5)))
-                elif t in ("sigma", "div_vort"):
-                    raise NotImplementedError(f"outlier method {t!r} is not on the device path")
+                elif t == "sigma":
+                    # outlier_detection.py:477-481 (combine 'norm': the vector norm)
+                    self.outliers["extra"].append(("sigma", float(mcfg.get("sigma_threshold", 2.0))))
+                elif t == "div_vort":
+                    dt, vt = mcfg.get("div_thresh"), mcfg.get("vort_thresh")
+                    self.outliers["extra"].append(("div_vort", float("nan") if dt is None else float(dt),
+                                                   float("nan") if vt is None else float(vt)))
                 else:
                     raise ValueError(f"Unknown outlier detection method: {t}")
         if self.outliers["use_median"] and self.outliers["size"] not in (3, 5):
@@ -233,6 +239,10 @@ class MultipassPIV:
             b.out["nanm"] = torch.empty(shape, dtype=torch.uint8, device=dev)
             b.out["reason"] = torch.empty(shape, dtype=torch.int8, device=dev)
             b.out["err"] = torch.zeros(1, dtype=torch.int32, device=dev)
+            if self.outliers["extra"]:
+                b.out["extra"] = torch.empty(shape, dtype=torch.uint8, device=dev)
+                b.out["om_work"] = torch.empty(max(4, self.lib.pivamd_outlier_methods_workspace(n, ny, nx)),
+                                               dtype=torch.uint8, device=dev)
             nyp, nxp = len(g.ctrs_y_all), len(g.ctrs_x_all)
             b.out["pad"] = torch.empty((2, n, nyp, nxp), dtype=f32, device=dev)
             b.out["pad_s"] = torch.empty((2, n, nyp, nxp), dtype=f32, device=dev)
@@ -340,9 +350,22 @@ class MultipassPIV:
                                         ny * nx, g.win[1] / div, g.win[0] / div, _ptr(o["u"]), _ptr(o["nanm"]),
                                         _ptr(o["reason"]), _ptr(o["pkh_g"]), st), "gate")
             oc = self.outliers
+            extra = None
+            if oc["extra"]:
+                extra = o["extra"]
+                extra.zero_()
+                wk = o["om_work"]
+                for m in oc["extra"]:
+                    if m[0] == "sigma":
+                        rc = L.pivamd_outlier_sigma(_ptr(o["u"]), _ptr(b.mask), n, ny, nx, m[1], 0, _ptr(extra),
+                                                    _ptr(wk), wk.numel(), st)
+                    else:
+                        rc = L.pivamd_outlier_div_vort(_ptr(o["u"]), _ptr(b.mask), n, ny, nx, m[1], m[2],
+                                                       _ptr(extra), _ptr(wk), wk.numel(), st)
+                    _native.check(rc, f"{m[0]} outliers")
             _native.check(L.pivamd_outlier_stage(_ptr(o["u"]), _ptr(o["pkh_g"]), _ptr(b.mask), n, ny, nx,
                                                  oc["use_median"], oc["eps"], oc["thr"], oc["size"], oc["use_peak"],
-                                                 oc["peak_thr"], None, _ptr(o["nanm"]), _ptr(o["reason"]),
+                                                 oc["peak_thr"], _ptr(extra), _ptr(o["nanm"]), _ptr(o["reason"]),
                                                  _ptr(o["u2"]), _ptr(o["pkh_o"]), st), "outlier stage")
             u = o["u2"]
             method = self.final_infill if final else self.mid_infill

@@ -420,3 +420,86 @@ def test_shipped_default_configuration_matches_oracle_loop(finder):
         else:
             assert flips <= 0.05, (p, flips)
             assert np.median(d) < 1e-3 and np.mean(d > 0.005) < 0.15, (p, np.mean(d > 0.005))
+
+
+@pytest.mark.parametrize("with_mask", [False, True])
+def test_sigma_and_div_vort_flags_match_oracle(with_mask):
+    """The sigma (vector norm and per component) and divergence/vorticity
+    flags against the oracle, bit for bit, on fields with planted outliers,
+    gated zeros and masked windows; fixed and automatic thresholds."""
+    L = _lib()
+    N, ny, nx = 3, 37, 45
+    rng = np.random.default_rng(77)
+    yy, xx = np.mgrid[0:ny, 0:nx]
+    u = np.stack([np.stack([np.sin(xx / 6.0 + i) + 0.05 * rng.normal(size=(ny, nx)),
+                            np.cos(yy / 5.0 - i) + 0.05 * rng.normal(size=(ny, nx))]) for i in range(N)], axis=1)
+    u = u.astype(np.float32)
+    hit = rng.random((N, ny, nx)) < 0.03
+    u[:, hit] += rng.normal(0, 4, (2, int(hit.sum()))).astype(np.float32)
+    u[:, rng.random((N, ny, nx)) < 0.02] = 0.0            # gated vectors read as zero
+    m = _blob_mask(ny, nx, 9) if with_mask else None
+    dm = _t(m.astype(np.float32)) if with_mask else None
+    du = _t(u)
+    ws = L.pivamd_outlier_methods_workspace(N, ny, nx)
+    work = torch.empty(ws, dtype=torch.uint8, device=dev)
+    mp_ = None if dm is None else dm.data_ptr()
+    for name, call, ref in [
+        ("sigma-norm", lambda e: L.pivamd_outlier_sigma(du.data_ptr(), mp_, N, ny, nx, 2.0, 0, e.data_ptr(),
+                                                        work.data_ptr(), ws, _st()),
+         lambda ux, uy: M.sigma_outliers(np.sqrt(ux ** 2 + uy ** 2), 2.0)),
+        ("sigma-ux", lambda e: L.pivamd_outlier_sigma(du.data_ptr(), mp_, N, ny, nx, 1.5, 1, e.data_ptr(),
+                                                      work.data_ptr(), ws, _st()),
+         lambda ux, uy: M.sigma_outliers(ux, 1.5)),
+        ("divvort-auto", lambda e: L.pivamd_outlier_div_vort(du.data_ptr(), mp_, N, ny, nx, float("nan"),
+                                                             float("nan"), e.data_ptr(), work.data_ptr(), ws, _st()),
+         lambda ux, uy: M.div_vort_outliers(ux, uy)),
+        ("divvort-fixed", lambda e: L.pivamd_outlier_div_vort(du.data_ptr(), mp_, N, ny, nx, 0.8, float("nan"),
+                                                              e.data_ptr(), work.data_ptr(), ws, _st()),
+         lambda ux, uy: M.div_vort_outliers(ux, uy, 0.8, None)),
+    ]:
+        print(f"outlier method {name} mask={with_mask}", flush=True)
+        e = torch.zeros((N, ny, nx), dtype=torch.uint8, device=dev)
+        assert call(e) == 0
+        torch.cuda.synchronize()
+        got = e.cpu().numpy().astype(bool)
+        for i in range(N):
+            ux, uy = u[1, i].copy(), u[0, i].copy()
+            if with_mask:
+                ux[m] = np.nan
+                uy[m] = np.nan
+            want = ref(ux, uy)
+            assert want.sum() > 0
+            np.testing.assert_array_equal(got[i], want, err_msg=f"{name} image {i}")
+
+
+def test_all_four_outlier_methods_in_the_loop_match_oracle():
+    shape = (192, 256)
+    sched = [(64, 64), (32, 32), (16, 16)]
+    methods = [{"type": "peak_mag", "threshold": 0.2}, {"type": "median_2d"},
+               {"type": "sigma", "sigma_threshold": 3.0}, {"type": "div_vort"}]
+    A, B = [], []
+    for k in range(2):
+        rng = np.random.default_rng(900 + k)
+        x, y, inten = S._positions(rng, shape, 0.03)
+        dx, dy = S.lamb_oseen(x, y, shape, rc=40.0, peak=3.0, uniform=(1.5, 0.0))
+        A.append(S.render(shape, x, y, inten).numpy())
+        B.append(S.render(shape, x + dx, y + dy, inten).numpy())
+    A = np.stack(A).astype(np.float32)
+    B = np.stack(B).astype(np.float32)
+    yy, xx = np.mgrid[0:shape[0], 0:shape[1]]
+    pm = (yy - 120) ** 2 + (xx - 60) ** 2 < 30 ** 2
+    mp = MultipassPIV(shape, sched, [50, 50, 50], outlier_methods=methods, pixel_mask=pm,
+                      mid_infill="harmonic", final_infill="biharmonic")
+    res = mp.run(_t(A), _t(B), keep_passes=True)
+    torch.cuda.synchronize()
+    mp.check_infill()
+    grids = build_pass_grids(shape, sched, [50, 50, 50])
+    vm = M.vector_masks_loop(shape, sched, [50, 50, 50], pm, 0.5)
+    ref = M.run_multipass(A, B, grids, keep_passes=True, outlier_methods=methods, vector_masks=vm,
+                          mid_infill="harmonic", final_infill="biharmonic")
+    for p, (g, r) in enumerate(zip(res, ref)):
+        gm = g.nan_mask.cpu().numpy().astype(bool)
+        np.testing.assert_array_equal(gm, r["nan_mask"])
+        np.testing.assert_array_equal(g.nan_reason.cpu().numpy(), r["nan_reason"])
+        d = np.hypot(g.ux.cpu().numpy() - r["ux"], g.uy.cpu().numpy() - r["uy"])
+        assert np.sqrt(np.mean(d ** 2)) < 0.005, (p, d.max())

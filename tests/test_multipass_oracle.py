@@ -289,3 +289,60 @@ def test_infill_ls_fills_nan_vectors_only():
     keep = np.broadcast_to(~np.isnan(u).any(axis=0), u.shape)
     np.testing.assert_array_equal(out[keep], u[keep])
     assert out[0, 1, 0, 0] != u[0, 1, 0, 0]
+
+
+def test_sigma_outliers_against_a_neighbour_loop():
+    """The 8-neighbour mean/std (uniform_filter form) against explicit
+    neighbour lists in FP64; nodes within 1e-4 of the threshold are skipped
+    (float32 against float64 can differ there)."""
+    rng = np.random.default_rng(4)
+    ny, nx, k = 13, 17, 2.0
+    v = rng.normal(1, 0.3, (ny, nx)).astype(np.float32)
+    v[3, 4] = 9.0
+    v[0, 5] = np.nan
+    got = M.sigma_outliers(v, k)
+    checked = 0
+    for r in range(ny):
+        for c in range(nx):
+            nb = [float(v[rr, cc]) for rr in range(r - 1, r + 2) for cc in range(c - 1, c + 2)
+                  if (rr, cc) != (r, c) and 0 <= rr < ny and 0 <= cc < nx and np.isfinite(v[rr, cc])]
+            if not np.isfinite(v[r, c]) or not nb:
+                assert got[r, c]
+                continue
+            m = np.mean(nb)
+            sd = np.sqrt(max(np.mean(np.square(nb)) - m * m, 0.0))
+            d = abs(float(v[r, c]) - m) - k * sd
+            if abs(d) < 1e-4:
+                continue
+            assert got[r, c] == (d > 0 or sd == 0), (r, c)
+            checked += 1
+    assert checked > 150 and got[3, 4]
+
+
+def test_div_vort_against_explicit_differences():
+    rng = np.random.default_rng(6)
+    ny, nx = 10, 12
+    ux = rng.normal(0, 1, (ny, nx)).astype(np.float32)
+    uy = rng.normal(0, 1, (ny, nx)).astype(np.float32)
+    ux[4, 5] = 25.0
+    div = np.zeros((ny, nx))
+    vort = np.zeros((ny, nx))
+    for r in range(ny):
+        for c in range(nx):
+            dudx = 0.5 * (ux[r, (c + 1) % nx] - ux[r, (c - 1) % nx])
+            dudy = 0.5 * (ux[(r + 1) % ny, c] - ux[(r - 1) % ny, c])
+            dvdx = 0.5 * (uy[r, (c + 1) % nx] - uy[r, (c - 1) % nx])
+            dvdy = 0.5 * (uy[(r + 1) % ny, c] - uy[(r - 1) % ny, c])
+            div[r, c], vort[r, c] = dudx + dvdy, dvdx - dudy
+
+    def thr(x):
+        s = np.sort(x.ravel())
+        med = (s[(s.size - 1) // 2] + s[s.size // 2]) / 2
+        d = np.sort(np.abs(x.ravel() - med))
+        return 6 * 1.4826 * (d[(d.size - 1) // 2] + d[d.size // 2]) / 2
+
+    want = (np.abs(div) > thr(div)) | (np.abs(vort) > thr(vort))
+    got = M.div_vort_outliers(ux, uy)
+    assert want.sum() >= 4
+    np.testing.assert_array_equal(got, want)
+    np.testing.assert_array_equal(M.div_vort_outliers(ux, uy, 1e9, 1e9), np.zeros((ny, nx), bool))
