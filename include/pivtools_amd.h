@@ -277,7 +277,30 @@ int pivamd_median(const float *u, const float *pkh, const float *mask, int N, in
 int pivamd_outlier_stage(const float *u, const float *pkh, const float *mask, int N, int ny, int nx,
                          int use_median, float eps, float thr, int size, int use_peak, float peak_thr,
                          const unsigned char *d_extra, unsigned char *nanm, signed char *reason, float *uo,
-                         float *pkh_out, void *stream);
+                         float *pkh_out, unsigned char *choice, void *stream);
+/* Several peaks per window (nPeaks > 1) and secondary-peak substitution
+ * (cpu_instantaneous.py:586-760).  pivamd_gate_peaks gates every peak of the
+ * (N, P, nWin) bulkxcorr2d outputs and adds the predictor (pxa, pya, pha),
+ * and takes the primary peak as pivamd_gate does (choice = 1).
+ * pivamd_outlier_detect is the outlier test without the final bookkeeping:
+ * d_active null = the first round (reason 10), else re-judging the windows
+ * of a substitution round (reason 12).  pivamd_peak_select is one round
+ * (windows still invalid move to their next peak; usel / pk_sel gather every
+ * window's current peak); pivamd_peak_final closes the stage. */
+int pivamd_gate_peaks(const float *pkx, const float *pky, const float *pkh, const float *pred,
+                      const float *mask, int N, int P, int nWin, float tx, float ty, float *pxa, float *pya,
+                      float *pha, float *u, unsigned char *nanm, signed char *reason, float *pkh0,
+                      unsigned char *choice, void *stream);
+int pivamd_outlier_detect(const float *u, const float *pkh, const float *mask, int N, int ny, int nx,
+                          int use_median, float eps, float thr, int size, int use_peak, float peak_thr,
+                          const unsigned char *d_extra, const unsigned char *d_active, unsigned char *nanm,
+                          signed char *reason, void *stream);
+int pivamd_peak_select(const float *pxa, const float *pya, const float *pha, int N, int P, int nWin,
+                       unsigned char *nanm, signed char *reason, unsigned char *choice, float *usel,
+                       float *pk_sel, unsigned char *active, void *stream);
+int pivamd_peak_final(const float *usel, const float *pk_sel, const float *mask, int N, int nWin,
+                      unsigned char *nanm, signed char *reason, unsigned char *choice, float *uo,
+                      float *pkh_out, void *stream);
 /* The sigma (outlier_detection.py:244-313; field 0 = vector norm, 1 = ux,
  * 2 = uy) and divergence/vorticity (:316-373; NaN thresholds = 6 * 1.4826 *
  * MAD of the image's finite values) outlier methods on the gated planar

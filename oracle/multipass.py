@@ -114,6 +114,108 @@ def median_outliers(ux, uy, eps=0.1, thr=2.0, size=5, masked=None):
     return out
 
 
+def _detect(ux, uy, pkh, mask, methods):
+    """apply_outlier_detection on one image's (ux, uy) (NaN allowed)."""
+    m = np.zeros(ux.shape, bool) if mask is None else mask
+    om = np.zeros(ux.shape, bool)
+    ux = ux.astype(np.float32)
+    uy = uy.astype(np.float32)
+    if mask is not None:
+        ux = np.where(m, np.float32(np.nan), ux)
+        uy = np.where(m, np.float32(np.nan), uy)
+    for mc in methods:
+        t = mc["type"]
+        if t == "median_2d":
+            om |= median_outliers(ux, uy, mc.get("epsilon", 0.1), mc.get("threshold", 2.0), mc.get("size", 5),
+                                  masked=m if mask is not None else None)
+        elif t == "peak_mag":
+            om |= (pkh < mc.get("threshold", 0.5)) | np.isnan(pkh)
+        elif t == "sigma":
+            with np.errstate(invalid="ignore"):
+                om |= sigma_outliers(np.sqrt(ux ** 2 + uy ** 2), mc.get("sigma_threshold", 2.0))
+        elif t == "div_vort":
+            om |= div_vort_outliers(ux, uy, mc.get("div_thresh"), mc.get("vort_thresh"))
+        else:
+            raise ValueError(t)
+    return om & ~m
+
+
+def peaks_stage(pkx, pky, pkh, pred, mask, tx, ty, methods, secondary):
+    """Several peaks per window (cpu_instantaneous.py:586-760): pk* (N, P,
+    ny, nx).  Gate every peak, add the predictor, judge the primary peak,
+    then (secondary) move invalid windows to their next peak and re-judge
+    them; returns u (NaN at invalid, 0 at masked), nan_mask, nan_reason,
+    the chosen peak's height (NaN at invalid), peak_choice and the gated
+    heights of every peak."""
+    pkx = pkx.astype(np.float32).copy()
+    pky = pky.astype(np.float32).copy()
+    pkh = pkh.astype(np.float32).copy()
+    N, P, ny, nx = pkx.shape
+    m = np.zeros((ny, nx), bool) if mask is None else mask.astype(bool)
+    mb = np.broadcast_to(m, (N, ny, nx))
+    large = (np.abs(pkx) > tx) | (np.abs(pky) > ty)
+    fit_fail0 = np.isnan(pkx[:, 0]) | np.isnan(pky[:, 0])
+    large0 = large[:, 0]
+    inv = np.broadcast_to(mb[:, None], pkx.shape) | large
+    pkx[inv] = np.nan
+    pky[inv] = np.nan
+    pkh[inv] = np.nan
+    if pred is not None:
+        pkx += pred[1][:, None]
+        pky += pred[0][:, None]
+    ux = pkx[:, 0].copy()
+    uy = pky[:, 0].copy()
+    nm = np.isnan(ux) | np.isnan(uy) | mb
+    ux[nm] = 0.0
+    uy[nm] = 0.0
+    choice = np.ones((N, ny, nx), np.int64)
+    reason = np.zeros((N, ny, nx), np.int8)
+    reason[fit_fail0] = 1
+    reason[large0] = 6
+    if methods:
+        for i in range(N):
+            om = _detect(ux[i], uy[i], pkh[i, 0], mask, methods)
+            nm[i] |= om
+            reason[i][om & (reason[i] == 0)] = 10
+    if secondary:
+        for _ in range(1, P):
+            act = nm & (choice < P)
+            if not act.any():
+                break
+            choice[act] += 1
+            idx = (choice - 1)[:, None]
+            ux = np.take_along_axis(pkx, idx, axis=1)[:, 0]
+            uy = np.take_along_axis(pky, idx, axis=1)[:, 0]
+            sub_nan = np.isnan(ux) | np.isnan(uy)
+            nm[act] = sub_nan[act]
+            reason[act] = np.where(sub_nan[act], 13, 0)
+            if methods:
+                ph = np.take_along_axis(pkh, idx, axis=1)[:, 0]
+                for i in range(N):
+                    if act[i].any():
+                        om = _detect(ux[i], uy[i], ph[i], mask, methods)
+                        rf = act[i] & om & ~sub_nan[i]
+                        nm[i] |= rf
+                        reason[i][rf] = 12
+        nm |= mb
+    idx = (choice - 1)[:, None]
+    ppm = np.take_along_axis(pkh, idx, axis=1)[:, 0]
+    nm |= np.isnan(ppm)
+    reason[np.isnan(ppm) & (reason == 0)] = 13
+    reason[nm & (reason == 0)] = 9
+    reason[mb] = -1
+    ux = ux.copy()
+    uy = uy.copy()
+    ux[nm] = np.nan
+    uy[nm] = np.nan
+    ppm = ppm.copy()
+    ppm[nm] = np.nan
+    ux[mb] = 0.0
+    uy[mb] = 0.0
+    choice[nm] = 0
+    return np.stack([uy, ux]).astype(np.float32), nm, reason, ppm, choice, pkh
+
+
 def median_stage(u, pkh, nanm, reason, mask=None, eps=0.1, thr=2.0, size=5):
     """Median test plus the bookkeeping of cpu_instantaneous.py:639-757."""
     return outlier_stage(u, pkh, nanm, reason, mask,
@@ -449,14 +551,16 @@ def peaks_fp64(planes, pw):
     return row, col, np.where(ok, h0, 0.0)
 
 
-def xcorr_pass(A, B, g, w, finder, flavour, mask=None):
+def xcorr_pass(A, B, g, w, finder, flavour, mask=None, n_peaks=1):
     """One bulkxcorr2d over all pairs: flavour 'oracle' (float32 C
     restatement) or 'fp64' (FP64 planes, FP64 3-point fit)."""
     ny, nx = g.n_win
     win = g.win
     if flavour == "oracle":
         fm = np.zeros((ny, nx), np.float32) if mask is None else mask.astype(np.float32)
-        o = O.bulkxcorr2d(A, B, fm, g.ctrs_x, g.ctrs_y, win, w, w, 1, finder)
+        o = O.bulkxcorr2d(A, B, fm, g.ctrs_x, g.ctrs_y, win, w, w, n_peaks, finder)
+        if n_peaks > 1:
+            return o["pk_x"], o["pk_y"], o["pk_h"]
         return o["pk_x"][:, 0], o["pk_y"][:, 0], o["pk_h"][:, 0]
     if finder != 3:
         raise ValueError("the FP64 flavour fits with the 3-point estimator")
@@ -481,7 +585,8 @@ def xcorr_pass(A, B, g, w, finder, flavour, mask=None):
 
 def run_multipass(A, B, grids, window_type="gaussian", finder=3, smoothing=True, median=(0.1, 2.0, 5),
                   final_infill="nearest", flavour="oracle", interp=0, keep_passes=False, vector_masks=None,
-                  predictor_fill="harmonic", predictor_bcs=None, mid_infill="nearest", outlier_methods=None):
+                  predictor_fill="harmonic", predictor_bcs=None, mid_infill="nearest", outlier_methods=None,
+                  num_peaks=1, secondary_peak=False):
     """The whole pass loop on the CPU; A, B (N, H, W) float32."""
     from pivtools_amd.window import window_weight
 
@@ -505,13 +610,22 @@ def run_multipass(A, B, grids, window_type="gaussian", finder=3, smoothing=True,
         m = None
         if vector_masks is not None and vector_masks[p] is not None and np.any(vector_masks[p]):
             m = np.asarray(vector_masks[p], bool)
-        pkx, pky, pkh = xcorr_pass(a_in, b_in, g, w, finder, flavour, m)
+        pkx, pky, pkh = xcorr_pass(a_in, b_in, g, w, finder, flavour, m, num_peaks)
         div = 2.0 if p == 0 else 4.0
-        u, nm, reason, pkg = gate(pkx, pky, pkh, pred, m, g.win[1] / div, g.win[0] / div)
-        if outlier_methods is None:
-            u, nm, reason, pko = median_stage(u, pkg, nm, reason, m, *median)
+        choice = None
+        if num_peaks > 1:
+            methods = outlier_methods
+            if methods is None:
+                methods = [] if median is None else [{"type": "median_2d", "epsilon": median[0],
+                                                      "threshold": median[1], "size": median[2]}]
+            u, nm, reason, pko, choice, _ = peaks_stage(pkx, pky, pkh, pred, m, g.win[1] / div, g.win[0] / div,
+                                                        methods, secondary_peak)
         else:
-            u, nm, reason, pko = outlier_stage(u, pkg, nm, reason, m, outlier_methods)
+            u, nm, reason, pkg = gate(pkx, pky, pkh, pred, m, g.win[1] / div, g.win[0] / div)
+            if outlier_methods is None:
+                u, nm, reason, pko = median_stage(u, pkg, nm, reason, m, *median)
+            else:
+                u, nm, reason, pko = outlier_stage(u, pkg, nm, reason, m, outlier_methods)
         final = p == len(grids) - 1
         sp_ = (float(g.spacing[0]), float(g.spacing[1]))
         method = mid_infill if not final else final_infill
@@ -525,6 +639,6 @@ def run_multipass(A, B, grids, window_type="gaussian", finder=3, smoothing=True,
                                            predictor_bcs)
         if keep_passes or final:
             out.append({"ux": u[1], "uy": u[0], "nan_mask": nm, "nan_reason": reason, "peak_mag": pko,
-                        "predictor": pred})
+                        "predictor": pred, "peak_choice": choice})
         prev_grid = g
     return out

@@ -71,6 +71,118 @@ __global__ void gate_kernel(const float *pkx, const float *pky, const float *pkh
     pkh_out[i] = h;
 }
 
+// Multi-peak gate (nPeaks > 1, cpu_instantaneous.py:586-635): pk fields
+// (N, P, nWin) as bulkxcorr2d lays them out.  Every peak is NaN when its
+// window is masked or the peak lies outside the gate, then the predictor is
+// added (pxa, pya, pha keep all peaks); the primary peak becomes u (0 at
+// invalid vectors), nan_mask and reasons as in gate_kernel; choice = 1.
+__global__ void gate_peaks_kernel(const float *pkx, const float *pky, const float *pkh, const float *pred,
+                                  const float *mask, int N, int P, int nWin, float tx, float ty, float *pxa,
+                                  float *pya, float *pha, float *u, uint8_t *nanm, int8_t *reason, float *pkh0,
+                                  uint8_t *choice)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int total = N * nWin;
+    if (i >= total) return;
+    const int n = i / nWin, w = i % nWin;
+    const bool masked = mask && mask[w] != 0.0f;
+    const float nan = __builtin_nanf("");
+    bool fit_fail0 = false, large0 = false;
+    for (int p = 0; p < P; ++p) {
+        const size_t k = ((size_t)n * P + p) * nWin + w;
+        float x = pkx[k], y = pky[k], h = pkh[k];
+        const bool large = (fabsf(x) > tx) || (fabsf(y) > ty);
+        if (p == 0) {
+            fit_fail0 = (x != x) || (y != y);
+            large0 = large;
+        }
+        if (masked || large) {
+            x = nan;
+            y = nan;
+            h = nan;
+        }
+        if (pred) {
+            y += pred[i];
+            x += pred[(size_t)total + i];
+        }
+        pxa[k] = x;
+        pya[k] = y;
+        pha[k] = h;
+        if (p == 0) {
+            const bool nm = (x != x) || (y != y) || masked;
+            u[i] = nm ? 0.0f : y;
+            u[(size_t)total + i] = nm ? 0.0f : x;
+            nanm[i] = nm ? 1 : 0;
+            pkh0[i] = h;
+        }
+    }
+    int8_t r = 0;
+    if (fit_fail0) r = 1;
+    if (large0) r = 6;
+    reason[i] = r;
+    choice[i] = 1;
+}
+
+// One substitution round (cpu_instantaneous.py:677-699): windows still
+// invalid with peaks left move to their next peak; every window's current
+// peak is gathered into usel / pk_sel (NaN where that peak is invalid), and
+// the moved windows are re-judged from scratch (reason 13 if the new peak is
+// NaN, else valid).
+__global__ void peak_select_kernel(const float *pxa, const float *pya, const float *pha, int N, int P, int nWin,
+                                   uint8_t *nanm, int8_t *reason, uint8_t *choice, float *usel, float *pk_sel,
+                                   uint8_t *active)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int total = N * nWin;
+    if (i >= total) return;
+    const int n = i / nWin, w = i % nWin;
+    const bool act = nanm[i] != 0 && choice[i] < P;
+    int ch = choice[i];
+    if (act) ch += 1;
+    choice[i] = (uint8_t)ch;
+    const size_t k = ((size_t)n * P + (ch - 1)) * nWin + w;
+    const float x = pxa[k], y = pya[k];
+    usel[i] = y;
+    usel[(size_t)total + i] = x;
+    pk_sel[i] = pha[k];
+    active[i] = act ? 1 : 0;
+    if (act) {
+        const bool sub_nan = (x != x) || (y != y);
+        nanm[i] = sub_nan ? 1 : 0;
+        reason[i] = sub_nan ? 13 : 0;
+    }
+}
+
+// After the substitution rounds (cpu_instantaneous.py:724-764): masked
+// windows stay invalid, a NaN height of the chosen peak invalidates, the
+// remaining reasons are filled in, and the outputs take NaN (0 at masked
+// windows, choice 0) at invalid vectors.
+__global__ void peak_final_kernel(const float *usel, const float *pk_sel, const float *mask, int N, int nWin,
+                                  uint8_t *nanm, int8_t *reason, uint8_t *choice, float *uo, float *pkh_out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int total = N * nWin;
+    if (i >= total) return;
+    const int w = i % nWin;
+    const bool masked = mask && mask[w] != 0.0f;
+    bool nm = nanm[i] != 0 || masked;
+    int8_t rs = reason[i];
+    const float hv = pk_sel[i];
+    if (hv != hv) {
+        nm = true;
+        if (rs == 0) rs = 13;
+    }
+    if (nm && rs == 0) rs = 9;
+    if (masked) rs = -1;
+    nanm[i] = nm ? 1 : 0;
+    reason[i] = rs;
+    const float nanv = __builtin_nanf("");
+    uo[i] = masked ? 0.0f : (nm ? nanv : usel[i]);
+    uo[(size_t)total + i] = masked ? 0.0f : (nm ? nanv : usel[(size_t)total + i]);
+    pkh_out[i] = nm ? nanv : hv;
+    if (nm) choice[i] = 0;
+}
+
 // nanmedian of K values held in registers (bottleneck semantics: NaNs
 // dropped, the middle element for an odd count, the mean of the middle pair
 // for an even one, NaN when nothing is left).  Fully unrolled rank
@@ -160,9 +272,15 @@ __device__ __forceinline__ bool median_outlier(const float *uy, const float *ux,
 // apply_outlier_detection): the median test (use_median), the peak
 // magnitude test pkh < peak_thr or NaN (use_peak, :47-84) and flags other
 // kernels computed into extra (nullable); masked windows are never flagged.
+//
+// mode 0: detection and the final bookkeeping (choice, nullable, gets 0 at
+// invalid vectors and 1 elsewhere); mode 1: detection only (the first round
+// before secondary peaks); mode 2: re-judging a substitution round, only the
+// windows flagged in active whose new peak is not NaN (reason 12).
 __global__ void median_kernel(const float *u, const float *pkh, const float *mask, int N, int ny, int nx,
                               int use_median, float eps, float thr, int size, int use_peak, float peak_thr,
-                              const uint8_t *extra, uint8_t *nanm, int8_t *reason, float *uo, float *pkh_out)
+                              const uint8_t *extra, int mode, const uint8_t *active, uint8_t *nanm, int8_t *reason,
+                              float *uo, float *pkh_out, uint8_t *choice)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int nWin = ny * nx, total = N * nWin;
@@ -173,6 +291,7 @@ __global__ void median_kernel(const float *u, const float *pkh, const float *mas
     const bool self_masked = mask && mask[w] != 0.0f;
     bool outlier = false;
     const float hv = pkh[i];
+    if (mode == 2 && !(active[i] && uy[w] == uy[w] && ux[w] == ux[w])) return;
     if (!self_masked) {
         if (use_median)
             outlier = size == 3 ? median_outlier<3>(uy, ux, mask, r, c, ny, nx, eps, thr)
@@ -182,9 +301,21 @@ __global__ void median_kernel(const float *u, const float *pkh, const float *mas
     }
     bool nm = nanm[i] != 0;
     int8_t rs = reason[i];
+    if (mode == 2) {
+        if (outlier) {
+            nanm[i] = 1;
+            reason[i] = 12;
+        }
+        return;
+    }
     if (outlier) {
         nm = true;
         if (rs == 0) rs = 10;
+    }
+    if (mode == 1) {
+        nanm[i] = nm ? 1 : 0;
+        reason[i] = rs;
+        return;
     }
     if (hv != hv) {
         nm = true;
@@ -198,6 +329,7 @@ __global__ void median_kernel(const float *u, const float *pkh, const float *mas
     uo[i] = self_masked ? 0.0f : (nm ? nanv : uy[w]);
     uo[(size_t)total + i] = self_masked ? 0.0f : (nm ? nanv : ux[w]);
     pkh_out[i] = nm ? nanv : hv;
+    if (choice) choice[i] = nm ? 0 : 1;
 }
 
 // Nearest-donor fill of NaN cells (both components from the same donor).
@@ -415,20 +547,72 @@ extern "C" int pivamd_median(const float *u, const float *pkh, const float *mask
     const size_t n = (size_t)N * ny * nx;
     if (n == 0) return 0;
     hipLaunchKernelGGL(median_kernel, dim3(nblk(n)), dim3(256), 0, st, u, pkh, mask, N, ny, nx, 1, eps, thr, size,
-                       0, 0.0f, (const uint8_t *)nullptr, nanm, reason, uo, pkh_out);
+                       0, 0.0f, (const uint8_t *)nullptr, 0, (const uint8_t *)nullptr, nanm, reason, uo, pkh_out,
+                       (uint8_t *)nullptr);
     return PIVAMD_LAUNCHED();
 }
 
 extern "C" int pivamd_outlier_stage(const float *u, const float *pkh, const float *mask, int N, int ny, int nx,
                                     int use_median, float eps, float thr, int size, int use_peak, float peak_thr,
                                     const uint8_t *extra, uint8_t *nanm, int8_t *reason, float *uo,
-                                    float *pkh_out, hipStream_t st)
+                                    float *pkh_out, uint8_t *choice, hipStream_t st)
 {
     if (use_median && size != 3 && size != 5) return PIVAMD_ERR_ARG;
     const size_t n = (size_t)N * ny * nx;
     if (n == 0) return 0;
     hipLaunchKernelGGL(median_kernel, dim3(nblk(n)), dim3(256), 0, st, u, pkh, mask, N, ny, nx, use_median, eps, thr,
-                       size, use_peak, peak_thr, extra, nanm, reason, uo, pkh_out);
+                       size, use_peak, peak_thr, extra, 0, (const uint8_t *)nullptr, nanm, reason, uo, pkh_out,
+                       choice);
+    return PIVAMD_LAUNCHED();
+}
+
+extern "C" int pivamd_outlier_detect(const float *u, const float *pkh, const float *mask, int N, int ny, int nx,
+                                     int use_median, float eps, float thr, int size, int use_peak, float peak_thr,
+                                     const uint8_t *extra, const uint8_t *active, uint8_t *nanm, int8_t *reason,
+                                     hipStream_t st)
+{
+    if (use_median && size != 3 && size != 5) return PIVAMD_ERR_ARG;
+    const size_t n = (size_t)N * ny * nx;
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(median_kernel, dim3(nblk(n)), dim3(256), 0, st, u, pkh, mask, N, ny, nx, use_median, eps, thr,
+                       size, use_peak, peak_thr, extra, active ? 2 : 1, active, nanm, reason, (float *)nullptr,
+                       (float *)nullptr, (uint8_t *)nullptr);
+    return PIVAMD_LAUNCHED();
+}
+
+extern "C" int pivamd_gate_peaks(const float *pkx, const float *pky, const float *pkh, const float *pred,
+                                 const float *mask, int N, int P, int nWin, float tx, float ty, float *pxa, float *pya,
+                                 float *pha, float *u, uint8_t *nanm, int8_t *reason, float *pkh0, uint8_t *choice,
+                                 hipStream_t st)
+{
+    const size_t n = (size_t)N * nWin;
+    if (n == 0) return 0;
+    if (P < 1 || P > 255) return PIVAMD_ERR_ARG;
+    hipLaunchKernelGGL(gate_peaks_kernel, dim3(nblk(n)), dim3(256), 0, st, pkx, pky, pkh, pred, mask, N, P, nWin, tx,
+                       ty, pxa, pya, pha, u, nanm, reason, pkh0, choice);
+    return PIVAMD_LAUNCHED();
+}
+
+extern "C" int pivamd_peak_select(const float *pxa, const float *pya, const float *pha, int N, int P, int nWin,
+                                  uint8_t *nanm, int8_t *reason, uint8_t *choice, float *usel, float *pk_sel,
+                                  uint8_t *active, hipStream_t st)
+{
+    const size_t n = (size_t)N * nWin;
+    if (n == 0) return 0;
+    if (P < 1 || P > 255) return PIVAMD_ERR_ARG;
+    hipLaunchKernelGGL(peak_select_kernel, dim3(nblk(n)), dim3(256), 0, st, pxa, pya, pha, N, P, nWin, nanm, reason,
+                       choice, usel, pk_sel, active);
+    return PIVAMD_LAUNCHED();
+}
+
+extern "C" int pivamd_peak_final(const float *usel, const float *pk_sel, const float *mask, int N, int nWin,
+                                 uint8_t *nanm, int8_t *reason, uint8_t *choice, float *uo, float *pkh_out,
+                                 hipStream_t st)
+{
+    const size_t n = (size_t)N * nWin;
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(peak_final_kernel, dim3(nblk(n)), dim3(256), 0, st, usel, pk_sel, mask, N, nWin, nanm, reason,
+                       choice, uo, pkh_out);
     return PIVAMD_LAUNCHED();
 }
 

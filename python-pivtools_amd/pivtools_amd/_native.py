@@ -126,7 +126,13 @@ GLUE_SIGNATURES: dict = {
     "pivamd_gate": [_vp, _vp, _vp, _vp, _vp, _i, _i, _f, _f, _vp, _vp, _vp, _vp, _vp],
     "pivamd_median": [_vp, _vp, _vp, _i, _i, _i, _f, _f, _i, _vp, _vp, _vp, _vp, _vp],
     "pivamd_infill_nearest": [_vp, _vp, _i, _i, _i, _f, _f, _vp, _vp, _vp],
-    "pivamd_outlier_stage": [_vp, _vp, _vp, _i, _i, _i, _i, _f, _f, _i, _i, _f, _vp, _vp, _vp, _vp, _vp, _vp],
+    "pivamd_outlier_stage": [_vp, _vp, _vp, _i, _i, _i, _i, _f, _f, _i, _i, _f, _vp, _vp, _vp, _vp, _vp, _vp,
+                             _vp],
+    "pivamd_outlier_detect": [_vp, _vp, _vp, _i, _i, _i, _i, _f, _f, _i, _i, _f, _vp, _vp, _vp, _vp, _vp],
+    "pivamd_gate_peaks": [_vp, _vp, _vp, _vp, _vp, _i, _i, _i, _f, _f, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                          _vp],
+    "pivamd_peak_select": [_vp, _vp, _vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "pivamd_peak_final": [_vp, _vp, _vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp],
     "pivamd_pad_edge": [_vp, _i, _i, _i, _i, _i, _i, _i, _vp, _vp],
     "pivamd_smooth_axis": [_vp, _i, _i, _i, _i, _vp, _i, _vp, _vp],
     "pivamd_remap_cubic": [_vp, _i, _i, _i, _vp, _vp, _i, _i, _vp, _vp],

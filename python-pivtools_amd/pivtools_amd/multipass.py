@@ -61,7 +61,12 @@ def gaussian_weights_1d(sigma: float, truncate: float) -> tuple[np.ndarray, int]
 
 @dataclass
 class PassResult:
-    """Per-pass device outputs (cpu_instantaneous.py:871-906 fields)."""
+    """Per-pass device outputs, batched over the pairs: the fields of the
+    reference's PIVPassResult (pivtools_cli/piv/piv_result.py:7-30, filled at
+    cpu_instantaneous.py:876-891).  ``peak_mag`` holds every peak's height,
+    NaN where the peak was gated out (N, num_peaks, ny, nx);
+    ``primary_peak_mag`` the chosen peak's height, NaN at invalid vectors;
+    ``peak_choice`` the 1-based chosen peak, 0 at invalid vectors."""
 
     ux: torch.Tensor
     uy: torch.Tensor
@@ -72,6 +77,11 @@ class PassResult:
     predictor: torch.Tensor | None
     window_size: tuple
     n_windows: tuple
+    primary_peak_mag: torch.Tensor | None = None
+    peak_choice: torch.Tensor | None = None
+    b_mask: np.ndarray | None = None
+    win_ctrs_x: np.ndarray | None = None
+    win_ctrs_y: np.ndarray | None = None
 
 
 @dataclass
@@ -96,7 +106,7 @@ class MultipassPIV:
                  peak_fit_impl="scalar", warp_interpolation="cubic", predictor_smoothing=True,
                  median=(0.1, 2.0, 5), final_infill="nearest", dft="fft", device="cuda", mid_infill="nearest",
                  vector_masks=None, pixel_mask=None, mask_threshold=0.5, predictor_fill="harmonic",
-                 predictor_bcs=None, outlier_methods=None):
+                 predictor_bcs=None, outlier_methods=None, num_peaks=1, secondary_peak=False):
         """``vector_masks``: one (n_win_y, n_win_x) bool array per pass (True =
         masked window), or ``pixel_mask``: an (H, W) bool image mask turned
         into them with ``mask_threshold``.  ``predictor_fill``: 'harmonic',
@@ -107,6 +117,9 @@ class MultipassPIV:
         outlier_detection.methods), e.g. [{"type": "peak_mag", "threshold":
         0.25}, {"type": "median_2d", "epsilon": 0.1, "threshold": 2}]; when
         given it replaces ``median``; an empty list runs no outlier test.
+        ``num_peaks``: peaks found and fitted per window; with
+        ``secondary_peak`` an invalid vector is replaced by its next peak that
+        passes the outlier test (cpu_instantaneous.py:664-722).
         """
         self.H, self.W = image_shape
         if dft not in ("fft", "split"):
@@ -121,6 +134,10 @@ class MultipassPIV:
         self.interp = 0 if warp_interpolation == "cubic" else 1
         self.smoothing = bool(predictor_smoothing)
         self.median = median
+        self.num_peaks = int(num_peaks)
+        if not 1 <= self.num_peaks <= 255:
+            raise ValueError(f"num_peaks {num_peaks}: 1..255")
+        self.secondary = bool(secondary_peak) and self.num_peaks > 1
         self.outliers = {"use_median": 0, "eps": 0.1, "thr": 2.0, "size": 5, "use_peak": 0, "peak_thr": 0.5,
                          "extra": []}
         if outlier_methods is None:
@@ -229,8 +246,18 @@ class MultipassPIV:
             _native.check(self.lib.pivamd_plane_weight(_ptr(b.w), g.win[0], g.win[1], _ptr(b.pw), st),
                           "plane weight")
             shape = (n, ny, nx)
-            b.out = {k: torch.empty(shape, dtype=f32, device=dev)
-                     for k in ("pkx", "pky", "pkh", "sx", "sy", "sxy", "ratio", "pkh_g", "pkh_o")}
+            P = self.num_peaks
+            pshape = (n, P, ny, nx)
+            b.out = {k: torch.empty(shape, dtype=f32, device=dev) for k in ("ratio", "pkh_g", "pkh_o")}
+            b.out.update({k: torch.empty(pshape, dtype=f32, device=dev) for k in ("pkx", "pky", "pkh", "sx", "sy",
+                                                                                  "sxy")})
+            b.out["choice"] = torch.empty(shape, dtype=torch.uint8, device=dev)
+            if P > 1:
+                b.out.update({k: torch.empty(pshape, dtype=f32, device=dev) for k in ("pxa", "pya", "pha")})
+            if self.secondary:
+                b.out["usel"] = torch.empty((2,) + shape, dtype=f32, device=dev)
+                b.out["pk_sel"] = torch.empty(shape, dtype=f32, device=dev)
+                b.out["active"] = torch.empty(shape, dtype=torch.uint8, device=dev)
             b.out["u"] = torch.empty((2,) + shape, dtype=f32, device=dev)
             b.out["u2"] = torch.empty((2,) + shape, dtype=f32, device=dev)
             b.out["u3"] = torch.empty((2,) + shape, dtype=f32, device=dev)
@@ -250,7 +277,7 @@ class MultipassPIV:
             if (self.final_infill if final else self.mid_infill) in ("harmonic", "biharmonic"):
                 b.out["ls_work"] = torch.empty(max(4, self.lib.pivamd_infill_ls_workspace(n, ny, nx)),
                                                dtype=torch.uint8, device=dev)
-            ws = self.lib.pivamd_xcorr_workspace(n, ny, nx, g.win[0], g.win[1], 1, 0)
+            ws = self.lib.pivamd_xcorr_workspace(n, ny, nx, g.win[0], g.win[1], P, 0)
             b.work = torch.empty(max(ws, 4), dtype=torch.uint8, device=dev)
             if p > 0:
                 b.maps = (torch.as_tensor(g.pred_map_x, device=dev), torch.as_tensor(g.pred_map_y, device=dev))
@@ -337,7 +364,7 @@ class MultipassPIV:
             fn = xcorr_fn if p == 0 else L.pivamd_xcorr
             rc = fn(_ptr(a_in), _ptr(b_in), _ptr(b.mask), self.H, self.W, n, _ptr(b.ctrs["x"]),
                                          _ptr(b.ctrs["y"]), ny, nx, _ptr(b.w), _ptr(b.w), _ptr(b.pw), 0,
-                                         g.win[0], g.win[1], 1, self.finder, self.impl, _ptr(o["pkx"]),
+                                         g.win[0], g.win[1], self.num_peaks, self.finder, self.impl, _ptr(o["pkx"]),
                                          _ptr(o["pky"]), _ptr(o["pkh"]), _ptr(o["sx"]), _ptr(o["sy"]),
                                          _ptr(o["sxy"]), _ptr(o["ratio"]), None, _ptr(b.work), b.work.numel(), st)
             L.pivamd_set_dft_mode(prev_mode)
@@ -346,27 +373,47 @@ class MultipassPIV:
                 ev[1].record()
                 events.append((p, ev))
             div = 2.0 if p == 0 else 4.0
-            _native.check(L.pivamd_gate(_ptr(o["pkx"]), _ptr(o["pky"]), _ptr(o["pkh"]), _ptr(pred), _ptr(b.mask), n,
-                                        ny * nx, g.win[1] / div, g.win[0] / div, _ptr(o["u"]), _ptr(o["nanm"]),
-                                        _ptr(o["reason"]), _ptr(o["pkh_g"]), st), "gate")
+            nw = ny * nx
+            if self.num_peaks == 1:
+                _native.check(L.pivamd_gate(_ptr(o["pkx"]), _ptr(o["pky"]), _ptr(o["pkh"]), _ptr(pred),
+                                            _ptr(b.mask), n, nw, g.win[1] / div, g.win[0] / div, _ptr(o["u"]),
+                                            _ptr(o["nanm"]), _ptr(o["reason"]), _ptr(o["pkh_g"]), st), "gate")
+                peak_mag = o["pkh_g"].view(n, 1, ny, nx)
+            else:
+                _native.check(L.pivamd_gate_peaks(_ptr(o["pkx"]), _ptr(o["pky"]), _ptr(o["pkh"]), _ptr(pred),
+                                                  _ptr(b.mask), n, self.num_peaks, nw, g.win[1] / div,
+                                                  g.win[0] / div, _ptr(o["pxa"]), _ptr(o["pya"]), _ptr(o["pha"]),
+                                                  _ptr(o["u"]), _ptr(o["nanm"]), _ptr(o["reason"]),
+                                                  _ptr(o["pkh_g"]), _ptr(o["choice"]), st), "gate")
+                peak_mag = o["pha"]
             oc = self.outliers
-            extra = None
-            if oc["extra"]:
-                extra = o["extra"]
-                extra.zero_()
-                wk = o["om_work"]
-                for m in oc["extra"]:
-                    if m[0] == "sigma":
-                        rc = L.pivamd_outlier_sigma(_ptr(o["u"]), _ptr(b.mask), n, ny, nx, m[1], 0, _ptr(extra),
-                                                    _ptr(wk), wk.numel(), st)
-                    else:
-                        rc = L.pivamd_outlier_div_vort(_ptr(o["u"]), _ptr(b.mask), n, ny, nx, m[1], m[2],
-                                                       _ptr(extra), _ptr(wk), wk.numel(), st)
-                    _native.check(rc, f"{m[0]} outliers")
-            _native.check(L.pivamd_outlier_stage(_ptr(o["u"]), _ptr(o["pkh_g"]), _ptr(b.mask), n, ny, nx,
-                                                 oc["use_median"], oc["eps"], oc["thr"], oc["size"], oc["use_peak"],
-                                                 oc["peak_thr"], _ptr(extra), _ptr(o["nanm"]), _ptr(o["reason"]),
-                                                 _ptr(o["u2"]), _ptr(o["pkh_o"]), st), "outlier stage")
+            extra = self._extra_methods(o, o["u"], b.mask, n, ny, nx, st)
+            if not self.secondary:
+                _native.check(L.pivamd_outlier_stage(_ptr(o["u"]), _ptr(o["pkh_g"]), _ptr(b.mask), n, ny, nx,
+                                                     oc["use_median"], oc["eps"], oc["thr"], oc["size"],
+                                                     oc["use_peak"], oc["peak_thr"], _ptr(extra), _ptr(o["nanm"]),
+                                                     _ptr(o["reason"]), _ptr(o["u2"]), _ptr(o["pkh_o"]),
+                                                     _ptr(o["choice"]), st), "outlier stage")
+            else:
+                # first round on the primary peak, then one substitution round per further peak
+                _native.check(L.pivamd_outlier_detect(_ptr(o["u"]), _ptr(o["pkh_g"]), _ptr(b.mask), n, ny, nx,
+                                                      oc["use_median"], oc["eps"], oc["thr"], oc["size"],
+                                                      oc["use_peak"], oc["peak_thr"], _ptr(extra), None,
+                                                      _ptr(o["nanm"]), _ptr(o["reason"]), st), "outlier stage")
+                for _ in range(1, self.num_peaks):
+                    _native.check(L.pivamd_peak_select(_ptr(o["pxa"]), _ptr(o["pya"]), _ptr(o["pha"]), n,
+                                                       self.num_peaks, nw, _ptr(o["nanm"]), _ptr(o["reason"]),
+                                                       _ptr(o["choice"]), _ptr(o["usel"]), _ptr(o["pk_sel"]),
+                                                       _ptr(o["active"]), st), "peak select")
+                    ex = self._extra_methods(o, o["usel"], b.mask, n, ny, nx, st)
+                    _native.check(L.pivamd_outlier_detect(_ptr(o["usel"]), _ptr(o["pk_sel"]), _ptr(b.mask), n, ny,
+                                                          nx, oc["use_median"], oc["eps"], oc["thr"], oc["size"],
+                                                          oc["use_peak"], oc["peak_thr"], _ptr(ex),
+                                                          _ptr(o["active"]), _ptr(o["nanm"]), _ptr(o["reason"]),
+                                                          st), "secondary outlier stage")
+                _native.check(L.pivamd_peak_final(_ptr(o["usel"]), _ptr(o["pk_sel"]), _ptr(b.mask), n, nw,
+                                                  _ptr(o["nanm"]), _ptr(o["reason"]), _ptr(o["choice"]),
+                                                  _ptr(o["u2"]), _ptr(o["pkh_o"]), st), "peak final")
             u = o["u2"]
             method = self.final_infill if final else self.mid_infill
             if method == "nearest":
@@ -405,10 +452,32 @@ class MultipassPIV:
                     o["pad"][1, :, rows, :].fill_(ux_bc)
             if keep_passes or final:
                 results.append(PassResult(ux=u[1], uy=u[0], nan_mask=o["nanm"], nan_reason=o["reason"],
-                                          peak_mag=o["pkh_o"], peak_ratio=o["ratio"], predictor=pred,
-                                          window_size=g.win, n_windows=(ny, nx)))
+                                          peak_mag=peak_mag, peak_ratio=o["ratio"], predictor=pred,
+                                          window_size=g.win, n_windows=(ny, nx), primary_peak_mag=o["pkh_o"],
+                                          peak_choice=o["choice"],
+                                          b_mask=(self.masks[p] if self.masks is not None and self.masks[p] is not None
+                                                  else np.zeros((ny, nx), bool)),
+                                          win_ctrs_x=g.ctrs_x, win_ctrs_y=g.ctrs_y))
             prev = b
         return results
+
+    def _extra_methods(self, o, u, mask, n, ny, nx, st):
+        """Flags of the sigma / div_vort methods on the planar field u, or None."""
+        if not self.outliers["extra"]:
+            return None
+        L = self.lib
+        extra = o["extra"]
+        extra.zero_()
+        wk = o["om_work"]
+        for m in self.outliers["extra"]:
+            if m[0] == "sigma":
+                rc = L.pivamd_outlier_sigma(_ptr(u), _ptr(mask), n, ny, nx, m[1], 0, _ptr(extra), _ptr(wk),
+                                            wk.numel(), st)
+            else:
+                rc = L.pivamd_outlier_div_vort(_ptr(u), _ptr(mask), n, ny, nx, m[1], m[2], _ptr(extra), _ptr(wk),
+                                               wk.numel(), st)
+            _native.check(rc, f"{m[0]} outliers")
+        return extra
 
     def check_infill(self) -> None:
         """Raise as the reference's infill would for the last batch: a field

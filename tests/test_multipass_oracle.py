@@ -346,3 +346,49 @@ def test_div_vort_against_explicit_differences():
     assert want.sum() >= 4
     np.testing.assert_array_equal(got, want)
     np.testing.assert_array_equal(M.div_vort_outliers(ux, uy, 1e9, 1e9), np.zeros((ny, nx), bool))
+
+
+def test_peaks_stage_without_substitution_equals_the_single_peak_stage():
+    rng = np.random.default_rng(8)
+    N, P, ny, nx = 2, 3, 11, 13
+    pkx = rng.normal(0, 1.5, (N, P, ny, nx)).astype(np.float32)
+    pky = rng.normal(0, 1.5, (N, P, ny, nx)).astype(np.float32)
+    pkh = rng.uniform(0.05, 1, (N, P, ny, nx)).astype(np.float32)
+    pkx[0, 0, 3, 3] = np.nan
+    pred = rng.normal(0, 0.5, (2, N, ny, nx)).astype(np.float32)
+    m = np.zeros((ny, nx), bool)
+    m[5, 6] = True
+    methods = [{"type": "median_2d"}, {"type": "peak_mag", "threshold": 0.1}]
+    u, nm, rs, ppm, ch, _ = M.peaks_stage(pkx, pky, pkh, pred, m, 4.0, 4.0, methods, False)
+    u1, nm1, rs1, pk1 = M.gate(pkx[:, 0], pky[:, 0], pkh[:, 0], pred, m, 4.0, 4.0)
+    u1, nm1, rs1, pk1 = M.outlier_stage(u1, pk1, nm1, rs1, m, methods)
+    np.testing.assert_array_equal(nm, nm1)
+    np.testing.assert_array_equal(rs, rs1)
+    np.testing.assert_array_equal(u, u1)
+    np.testing.assert_array_equal(ppm, pk1)
+    np.testing.assert_array_equal(ch, np.where(nm, 0, 1))
+
+
+def test_secondary_peak_substitution_by_hand():
+    """Hand-built windows: an outlier rescued by its second peak, one whose
+    further peaks are an outlier and then gated out, a gated primary with a
+    good second peak, and a masked window."""
+    N, P, ny, nx = 1, 3, 9, 9
+    pkx = np.full((N, P, ny, nx), 1.0, np.float32)
+    pky = np.full((N, P, ny, nx), 0.5, np.float32)
+    pkh = np.full((N, P, ny, nx), 0.8, np.float32)
+    pkx[0, :, 4, 4] = [3.5, 1.02, 0.0]          # outlier, then a consistent second peak
+    pkx[0, :, 2, 2] = [3.5, -3.0, 9.0]          # outlier, outlier, gated out (|x| > 4)
+    pkx[0, :, 6, 6] = [9.0, 0.98, 0.0]          # gated primary, good second peak
+    m = np.zeros((ny, nx), bool)
+    m[0, 8] = True
+    u, nm, rs, ppm, ch, pha = M.peaks_stage(pkx, pky, pkh, None, m, 4.0, 4.0, [{"type": "median_2d"}], True)
+    assert not nm[0, 4, 4] and ch[0, 4, 4] == 2 and rs[0, 4, 4] == 0 and u[1, 0, 4, 4] == np.float32(1.02)
+    assert nm[0, 2, 2] and ch[0, 2, 2] == 0 and rs[0, 2, 2] == 13 and np.isnan(u[1, 0, 2, 2])
+    assert not nm[0, 6, 6] and ch[0, 6, 6] == 2 and u[1, 0, 6, 6] == np.float32(0.98)
+    assert nm[0, 0, 8] and rs[0, 0, 8] == -1 and u[1, 0, 0, 8] == 0.0 and ch[0, 0, 8] == 0
+    assert np.isnan(pha[0, 0, 6, 6]) and np.isnan(pha[0, 2, 2, 2])
+    assert np.array_equal(rs != 0, nm)
+    # without substitution the same windows stay invalid
+    u2, nm2, rs2, _, ch2, _ = M.peaks_stage(pkx, pky, pkh, None, m, 4.0, 4.0, [{"type": "median_2d"}], False)
+    assert nm2[0, 4, 4] and rs2[0, 4, 4] == 10 and rs2[0, 6, 6] == 6
