@@ -82,6 +82,11 @@ class PassResult:
     b_mask: np.ndarray | None = None
     win_ctrs_x: np.ndarray | None = None
     win_ctrs_y: np.ndarray | None = None
+    # dump_correlation_planes: {"planes" (N, ny, nx, h, w) raw planes, zero
+    # at skipped windows; "pk_loc_x_raw", "pk_loc_y_raw", "pk_height_raw",
+    # "sx", "sy", "sxy" (N, P, ny, nx) and "pk_ratio_raw" (N, ny, nx) as the
+    # correlation returned them; "peak_finder"} (cpu_instantaneous.py:555-572)
+    debug_dump: dict | None = None
 
 
 @dataclass
@@ -106,7 +111,8 @@ class MultipassPIV:
                  peak_fit_impl="scalar", warp_interpolation="cubic", predictor_smoothing=True,
                  median=(0.1, 2.0, 5), final_infill="nearest", dft="fft", device="cuda", mid_infill="nearest",
                  vector_masks=None, pixel_mask=None, mask_threshold=0.5, predictor_fill="harmonic",
-                 predictor_bcs=None, outlier_methods=None, num_peaks=1, secondary_peak=False):
+                 predictor_bcs=None, outlier_methods=None, num_peaks=1, secondary_peak=False,
+                 dump_correlation_planes=False):
         """``vector_masks``: one (n_win_y, n_win_x) bool array per pass (True =
         masked window), or ``pixel_mask``: an (H, W) bool image mask turned
         into them with ``mask_threshold``.  ``predictor_fill``: 'harmonic',
@@ -117,6 +123,9 @@ class MultipassPIV:
         outlier_detection.methods), e.g. [{"type": "peak_mag", "threshold":
         0.25}, {"type": "median_2d", "epsilon": 0.1, "threshold": 2}]; when
         given it replaces ``median``; an empty list runs no outlier test.
+        ``dump_correlation_planes``: keep every pass's raw correlation
+        planes and peak outputs in ``PassResult.debug_dump`` (the reference's
+        debug dump; N*windows*h*w floats per pass).
         ``num_peaks``: peaks found and fitted per window; with
         ``secondary_peak`` an invalid vector is replaced by its next peak that
         passes the outlier test (cpu_instantaneous.py:664-722).
@@ -138,6 +147,7 @@ class MultipassPIV:
         if not 1 <= self.num_peaks <= 255:
             raise ValueError(f"num_peaks {num_peaks}: 1..255")
         self.secondary = bool(secondary_peak) and self.num_peaks > 1
+        self.dump_planes = bool(dump_correlation_planes)
         self.outliers = {"use_median": 0, "eps": 0.1, "thr": 2.0, "size": 5, "use_peak": 0, "peak_thr": 0.5,
                          "extra": []}
         if outlier_methods is None:
@@ -252,6 +262,8 @@ class MultipassPIV:
             b.out.update({k: torch.empty(pshape, dtype=f32, device=dev) for k in ("pkx", "pky", "pkh", "sx", "sy",
                                                                                   "sxy")})
             b.out["choice"] = torch.empty(shape, dtype=torch.uint8, device=dev)
+            if self.dump_planes:
+                b.out["planes"] = torch.empty((n, ny, nx) + tuple(g.win), dtype=f32, device=dev)
             if P > 1:
                 b.out.update({k: torch.empty(pshape, dtype=f32, device=dev) for k in ("pxa", "pya", "pha")})
             if self.secondary:
@@ -360,13 +372,16 @@ class MultipassPIV:
                 # masked windows are skipped by the correlation and keep these zeros
                 for k in ("pkx", "pky", "pkh", "sx", "sy", "sxy", "ratio"):
                     o[k].zero_()
+            if self.dump_planes:
+                o["planes"].zero_()     # skipped (masked, out-of-image) windows dump zero planes
             prev_mode = L.pivamd_set_dft_mode(self.dft)
             fn = xcorr_fn if p == 0 else L.pivamd_xcorr
             rc = fn(_ptr(a_in), _ptr(b_in), _ptr(b.mask), self.H, self.W, n, _ptr(b.ctrs["x"]),
                                          _ptr(b.ctrs["y"]), ny, nx, _ptr(b.w), _ptr(b.w), _ptr(b.pw), 0,
                                          g.win[0], g.win[1], self.num_peaks, self.finder, self.impl, _ptr(o["pkx"]),
                                          _ptr(o["pky"]), _ptr(o["pkh"]), _ptr(o["sx"]), _ptr(o["sy"]),
-                                         _ptr(o["sxy"]), _ptr(o["ratio"]), None, _ptr(b.work), b.work.numel(), st)
+                                         _ptr(o["sxy"]), _ptr(o["ratio"]), _ptr(o.get("planes")), _ptr(b.work),
+                                         b.work.numel(), st)
             L.pivamd_set_dft_mode(prev_mode)
             _native.check(rc, "bulkxcorr2d")
             if events is not None:
@@ -457,7 +472,12 @@ class MultipassPIV:
                                           peak_choice=o["choice"],
                                           b_mask=(self.masks[p] if self.masks is not None and self.masks[p] is not None
                                                   else np.zeros((ny, nx), bool)),
-                                          win_ctrs_x=g.ctrs_x, win_ctrs_y=g.ctrs_y))
+                                          win_ctrs_x=g.ctrs_x, win_ctrs_y=g.ctrs_y,
+                                          debug_dump=None if not self.dump_planes else {
+                                              "planes": o["planes"], "pk_loc_x_raw": o["pkx"],
+                                              "pk_loc_y_raw": o["pky"], "pk_height_raw": o["pkh"],
+                                              "pk_ratio_raw": o["ratio"], "sx": o["sx"], "sy": o["sy"],
+                                              "sxy": o["sxy"], "peak_finder": int(self.finder)}))
             prev = b
         return results
 

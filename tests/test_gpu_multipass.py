@@ -623,3 +623,59 @@ def test_multi_peak_and_secondary_substitution_match_oracle_loop(num_peaks, seco
             assert np.median(d[same]) < 1e-3, p
     if density < 0.02:
         assert subst > 0
+
+
+def test_correlation_plane_dump(tmp_path):
+    """dump_correlation_planes: the raw planes of every pass (zero at masked
+    windows) with the peak outputs, as the reference's debug dump: each valid
+    window's argmax sits on its raw peak (the reference's own check), pass 0's
+    planes match the oracle's within 1e-5 relative L2, and the npz and .mat
+    writers take the result."""
+    import oracle as O
+    from pivtools_amd.results import pair_result, save_corr_plane_dump, save_piv_result
+    from pivtools_amd.window import window_weight
+
+    shape = (192, 256)
+    sched = [(64, 64), (32, 32)]
+    A, B = [], []
+    for k in range(2):
+        rng = np.random.default_rng(40 + k)
+        x, y, inten = S._positions(rng, shape, 0.03)
+        dx, dy = S.lamb_oseen(x, y, shape, rc=40.0, peak=2.0, uniform=(1.5, -0.5))
+        A.append(S.render(shape, x, y, inten).numpy())
+        B.append(S.render(shape, x + dx, y + dy, inten).numpy())
+    A = np.stack(A).astype(np.float32)
+    B = np.stack(B).astype(np.float32)
+    grids = build_pass_grids(shape, sched, [50, 50])
+    vm = [np.zeros(g.n_win, bool) for g in grids]
+    vm[0][0, 0:2] = True
+    vm[1][1:3, 2] = True
+    mp = MultipassPIV(shape, sched, [50, 50], vector_masks=vm, dump_correlation_planes=True)
+    res = mp.run(_t(A), _t(B), keep_passes=True)
+    torch.cuda.synchronize()
+    for p, (g, r) in enumerate(zip(grids, res)):
+        dd = r.debug_dump
+        planes = dd["planes"].cpu().numpy()
+        ny, nx = g.n_win
+        assert planes.shape == (2, ny, nx) + g.win
+        assert not planes[:, vm[p]].any()
+        px = dd["pk_loc_x_raw"].cpu().numpy()[:, 0]
+        py = dd["pk_loc_y_raw"].cpu().numpy()[:, 0]
+        nm = r.nan_mask.cpu().numpy().astype(bool)
+        for n in range(2):
+            for j in range(ny):
+                for i in range(nx):
+                    if nm[n, j, i]:
+                        continue
+                    ar, ac = np.unravel_index(np.argmax(planes[n, j, i]), g.win)
+                    assert abs(ar - (py[n, j, i] + g.win[0] // 2)) <= 1.0
+                    assert abs(ac - (px[n, j, i] + g.win[1] // 2)) <= 1.0
+    w = window_weight(grids[0].win, "gaussian")
+    o = O.bulkxcorr2d(A, B, vm[0].astype(np.float32), grids[0].ctrs_x, grids[0].ctrs_y, grids[0].win, w, w, 1, 3,
+                      want_planes=True)
+    got = res[0].debug_dump["planes"].cpu().numpy()
+    rel = np.linalg.norm(got - o["planes"]) / np.linalg.norm(o["planes"])
+    assert rel < 1e-5, rel
+    pr = pair_result(res, 1)
+    assert save_corr_plane_dump(pr, tmp_path, 3) is not None
+    assert save_piv_result(pr, tmp_path, 3).endswith("B00003.mat")
