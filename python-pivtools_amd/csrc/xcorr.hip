@@ -125,6 +125,20 @@ struct XcCfg {
     static constexpr int wpb_lds = (kMaxLds - lds_blk) / lds_win;
     static constexpr int WPB = wpb_thr < wpb_lds ? wpb_thr : wpb_lds;
     static constexpr int BLOCK = WPB * T;
+    // Window tiles in the workgroup's LDS.  LDS serves a wave in two halves
+    // of 32 lanes over 32 banks.  With one lane per line and H = 32 / T
+    // windows per half wave (N = 8, 16, 32), the tile size N (N + 1) puts
+    // those H windows N banks apart, which keeps the column and row stages
+    // free of conflicts.  The packed inverse stages hold 2H windows per half
+    // wave; every second group of H windows is therefore moved by another
+    // 16 / H banks (and each pair of groups by 32 words, so tiles never
+    // overlap), which puts the 2H windows 16 / H banks apart.
+    static constexpr int H = (R == 1 && T <= 32 && 32 % T == 0) ? 32 / T : 0;
+    static __host__ __device__ constexpr int wbase(int wl)
+    {
+        return wl * tile + (H ? 32 * (wl / (2 * H)) + (16 / H) * ((wl / H) % 2) : 0);
+    }
+    static constexpr int tiles = wbase(WPB - 1) + tile;
     // lane t of a window -> line l and lane-in-line q.  R = 8 puts q bit 2 on
     // lane bit 3, so every exchange partner is one DPP away (xor 1, 2: quad
     // permutes; xor 8: row_ror:8).
@@ -742,12 +756,20 @@ __device__ __forceinline__ void xc_window(const XcorrArgs &g, float (&zr)[N / R]
         }
     };
     if constexpr (kPack) {
-        constexpr int C3 = N / 2 + 1;
+        // columns 0 .. N/2-1 of every window, window after window (2H whole
+        // windows per half wave, at distinct bank offsets), then the N/2
+        // column of every window
+        constexpr int C3 = N / 2;
         const int v = (int)threadIdx.x;
+        int wl = -1, col = 0;
         if (v < C::WPB * C3) {
-            const int wl = v / C3;
-            if (sv[wl]) stage3(v - wl * C3, Sbr + wl * C::tile, Sbi + wl * C::tile);
+            wl = v / C3;
+            col = v - wl * C3;
+        } else if (v < C::WPB * C3 + C::WPB) {
+            wl = v - C::WPB * C3;
+            col = N / 2;
         }
+        if (wl >= 0 && sv[wl]) stage3(col, Sbr + C::wbase(wl), Sbi + C::wbase(wl));
     } else {
         if (valid && l <= N / 2) stage3(l, Sr, Si);
     }
@@ -794,8 +816,8 @@ __device__ __forceinline__ void xc_window(const XcorrArgs &g, float (&zr)[N / R]
             const int wl = v / C4;
             if (sv[wl]) {
                 float xr[N1], xi[N1];
-                load4(v - wl * C4, Sbr + wl * C::tile, Sbi + wl * C::tile, xr, xi);
-                store4(v - wl * C4, Sbr + wl * C::tile, xr, xi);
+                load4(v - wl * C4, Sbr + C::wbase(wl), Sbi + C::wbase(wl), xr, xi);
+                store4(v - wl * C4, Sbr + C::wbase(wl), xr, xi);
             }
         }
     } else {
@@ -835,8 +857,8 @@ xcorr_kernel(XcorrArgs g)
 {
     using C = XcCfg<N, R_>;
     constexpr int R = C::R, N1 = C::N1, T = C::T, WPB = C::WPB;
-    __shared__ float s_re[WPB * C::tile];
-    __shared__ float s_im[WPB * C::tile];
+    __shared__ float s_re[C::tiles];
+    __shared__ float s_im[C::tiles];
     __shared__ float s_red[5][WPB * C::RED];
     __shared__ __attribute__((aligned(16))) float2 s_tw[R > 1 ? R * C::TWP : 1];   // split twiddles
 
@@ -855,8 +877,8 @@ xcorr_kernel(XcorrArgs g)
     const int ob = out_block<R>(q);   // output block of fft_split
     const int lb = l / N1;            // block of this lane's line index
     const float2 *tw = s_tw + (R > 1 ? q * C::TWP : 0);
-    float *Sr = s_re + wloc * C::tile;
-    float *Si = s_im + wloc * C::tile;
+    float *Sr = s_re + C::wbase(wloc);
+    float *Si = s_im + C::wbase(wloc);
     float *r0 = &s_red[0][wloc * C::RED], *r1 = &s_red[1][wloc * C::RED];
     float *r2 = &s_red[2][wloc * C::RED], *r3 = &s_red[3][wloc * C::RED];
     float *r4 = &s_red[4][wloc * C::RED];
