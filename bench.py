@@ -399,6 +399,8 @@ def main():
                     help="keep the default correlation-kernel variants instead of timing them first")
     ap.add_argument("--one-pair-steps", type=int, default=10,
                     help="after the timed region, time the same loop fed one pair per step (0 disables)")
+    ap.add_argument("--warp", choices=["exact", "fma"], default="fma",
+                    help="warp arithmetic: the reference's bit for bit, or separable fused multiply-adds")
     ap.add_argument("--infill", choices=["nearest", "harmonic", "biharmonic"], default="nearest",
                     help="infill of invalid vectors, mid-pass and final")
     ap.add_argument("--peak-mag", type=float, default=None,
@@ -442,7 +444,8 @@ def main():
         methods = [{"type": "peak_mag", "threshold": args.peak_mag},
                    {"type": "median_2d", "epsilon": 0.1, "threshold": 2, "size": 5}]
     mp = MultipassPIV(SHAPE, SCHEDULE, OVERLAP, peak_finder=args.finder, peak_fit_impl=args.impl, dft=args.dft,
-                      outlier_methods=methods, mid_infill=args.infill, final_infill=args.infill)
+                      outlier_methods=methods, mid_infill=args.infill, final_infill=args.infill,
+                      warp_arithmetic=args.warp)
     tuned = None
     if args.autotune and args.dft == "fft":
         from pivtools_amd import autotune
@@ -547,7 +550,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "multipass_2160x2560_u16", "pairs_per_step": args.pairs,
                    "schedule": [w[0] for w in SCHEDULE], "overlap": 50, "peak_finder": args.finder,
-                   "peak_fit_impl": args.impl, "dft": args.dft, "warp": "bicubic",
+                   "peak_fit_impl": args.impl, "dft": args.dft, "warp": "bicubic" + (" (separable FMA)" if args.warp == "fma" else " (reference arithmetic)"),
                    "outlier": "median 5x5 eps 0.1 thr 2" + (f" + peak_mag {args.peak_mag}" if args.peak_mag is not None else ""),
                    "infill": args.infill,
                    "global_batch": args.pairs, "parallelism": "single-gpu",

@@ -198,7 +198,10 @@ int pivamd_xcorr(const float *d_imgA, const float *d_imgB, const float *d_mask, 
 unsigned long pivamd_xcorr_workspace(int n_images, int nWy, int nWx, int winY, int winX, int nPeaks,
                                      int ensemble);
 
-/* One fused_symmetric_warp_batch over device buffers. */
+/* One fused_symmetric_warp_batch over device buffers.  interp_mode 0:
+ * bicubic and 1: Lanczos-3, both bit-identical to fused_warp.c; 2: the same
+ * bicubic evaluated separably with fused multiply-adds (last-bit differences
+ * only, about half the instructions).  Other values: PIVAMD_ERR_ARG. */
 int pivamd_warp(const float *d_imgA, const float *d_imgB, float *d_outA, float *d_outB,
                 const float *d_predDy, const float *d_predDx, int N, int H, int W, int nPY,
                 int nPX, const float *d_ctrY, const float *d_ctrX, int interp_mode,

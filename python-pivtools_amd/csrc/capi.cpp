@@ -452,7 +452,9 @@ int fused_symmetric_warp_batch(const float *imgs_a, const float *imgs_b, float *
     bool ok = h2d(dA, imgs_a, img) && h2d(dB, imgs_b, img) && h2d(dy, pred_dy, pred) &&
               h2d(dx, pred_dx, pred) && h2d(cy, ctrs_y, nPY * sizeof(float)) &&
               h2d(cx, ctrs_x, nPX * sizeof(float));
-    ok = ok && pivamd_warp(dA, dB, oA, oB, dy, dx, N, H, W, nPY, nPX, cy, cx, interp_mode,
+    // fused_warp.c:514: 0 is bicubic, anything else Lanczos-3 (both in the
+    // reference's exact arithmetic)
+    ok = ok && pivamd_warp(dA, dB, oA, oB, dy, dx, N, H, W, nPY, nPX, cy, cx, interp_mode == 0 ? 0 : 1,
                            shared_predictor, round_shifts, g_ctx.st) == 0;
     ok = ok && d2h(outs_a, oA, img) && d2h(outs_b, oB, img) && sync();
     return ok ? 0 : 1;

@@ -10,7 +10,11 @@
 //            bicubic (4x4, floor-1) or Lanczos-3 (6x6, floor-2, 4097-entry
 //            LUT with lerp), zero outside the image.
 // The arithmetic is the reference's scalar path (impl 0) term for term with
-// contraction off, so every output pixel is bit-identical to it.
+// contraction off, so every output pixel is bit-identical to it.  A second
+// bicubic form (mode 2) evaluates the same interpolants separably with fused
+// multiply-adds: rows of four taps are summed first and the 4x4 products
+// wy*wx are never formed, for about half the instructions; its results
+// differ from the reference's in the last bits only.
 //
 // Mapping: a 256-lane workgroup covers a 64-column x 16-row tile of one
 // image (grid.z = image).  Each lane owns one column and four rows.  The
@@ -107,6 +111,57 @@ __device__ __forceinline__ float sample_bicubic(const TI *img, float fy, float f
     return val;
 }
 
+// Keys a = -0.75 weights with fused multiply-adds (the same polynomials).
+__device__ __forceinline__ void keys4_fma(float d, float w[4])
+{
+    const float s0 = 1.0f + d, q2 = 1.0f - d, s3 = 2.0f - d;
+    w[1] = __builtin_fmaf(__builtin_fmaf(1.25f, d, -2.25f) * d, d, 1.0f);
+    w[2] = __builtin_fmaf(__builtin_fmaf(1.25f, q2, -2.25f) * q2, q2, 1.0f);
+    w[0] = __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(-0.75f, s0, 3.75f), s0, -6.0f), s0, 3.0f);
+    w[3] = __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(-0.75f, s3, 3.75f), s3, -6.0f), s3, 3.0f);
+}
+
+// The separable bicubic sample: sum_m wy[m] * (sum_n wx[n] * img[m][n]),
+// zero outside the image.
+template <typename TI>
+__device__ __forceinline__ float sample_bicubic_fast(const TI *img, float fy, float fx, int H, int W, int hm3,
+                                                     int wm3)
+{
+    const float fyf = floorf(fy), fxf = floorf(fx);
+    const int iy = (int)fyf - 1, ix = (int)fxf - 1;
+    float wy[4], wx[4];
+    keys4_fma(fy - fyf, wy);
+    keys4_fma(fx - fxf, wx);
+    float val = 0.0f;
+    if (inside(iy, hm3) && inside(ix, wm3)) {
+        const TI *p = img + (iy * W + ix);
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            float rs = wx[0] * (float)p[0];
+            rs = __builtin_fmaf(wx[1], (float)p[1], rs);
+            rs = __builtin_fmaf(wx[2], (float)p[2], rs);
+            rs = __builtin_fmaf(wx[3], (float)p[3], rs);
+            val = __builtin_fmaf(wy[m], rs, val);
+            p += W;
+        }
+        return val;
+    }
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+        const int row = iy + m;
+        if (row < 0 || row >= H) continue;
+        float rs = 0.0f;
+#pragma unroll
+        for (int n = 0; n < 4; n++) {
+            const int col = ix + n;
+            if (col < 0 || col >= W) continue;
+            rs = __builtin_fmaf(wx[n], (float)img[row * W + col], rs);
+        }
+        val = __builtin_fmaf(wy[m], rs, val);
+    }
+    return val;
+}
+
 __device__ __forceinline__ void lanczos_w(float d, const float *lut, float w[6])
 {
     const float pos = d * (float)kLanczosLut;
@@ -175,8 +230,9 @@ __device__ __forceinline__ const TI *warp_src(const WarpArgs &g, bool a)
 // input, non-finite included, and leaves in-range shifts untouched.
 constexpr float kShiftMax = 1048576.0f;
 
-// TI: float frames, or 16-bit counts (converted exactly on load).
-template <typename TI>
+// TI: float frames, or 16-bit counts (converted exactly on load).  FAST:
+// the separable fused-multiply-add form of mode 2.
+template <typename TI, bool FAST>
 __global__ void __launch_bounds__(256) warp_kernel(WarpArgs g)
 {
     __shared__ float s_ly[kTileY];
@@ -277,7 +333,24 @@ __global__ void __launch_bounds__(256) warp_kernel(WarpArgs g)
         const float wy[4] = {wy4.x, wy4.y, wy4.z, wy4.w};
         const int ro[4] = {ro4.x, ro4.y, ro4.z, ro4.w};
         float va = 0.0f, vb = 0.0f;
-        if (staged) {
+        if constexpr (FAST) {
+            // separate loops keep the staged reads LDS reads (no flat pointer)
+            auto pred_sum = [&](const float *py, const float *px) {
+#pragma unroll
+                for (int m = 0; m < 4; m++) {
+                    float ry = wx[0] * py[ro[m] + cols[0]], rx = wx[0] * px[ro[m] + cols[0]];
+#pragma unroll
+                    for (int n = 1; n < 4; n++) {
+                        ry = __builtin_fmaf(wx[n], py[ro[m] + cols[n]], ry);
+                        rx = __builtin_fmaf(wx[n], px[ro[m] + cols[n]], rx);
+                    }
+                    va = __builtin_fmaf(wy[m], ry, va);
+                    vb = __builtin_fmaf(wy[m], rx, vb);
+                }
+            };
+            if (staged) pred_sum(s_pdy, s_pdx);
+            else pred_sum(dyp, dxp);
+        } else if (staged) {
 #pragma unroll
             for (int m = 0; m < 4; m++) {
 #pragma unroll
@@ -303,7 +376,10 @@ __global__ void __launch_bounds__(256) warp_kernel(WarpArgs g)
         hy = fminf(fmaxf(hy, -kShiftMax), kShiftMax);
         hx = fminf(fmaxf(hx, -kShiftMax), kShiftMax);
         const size_t o = (size_t)img * plane + (size_t)i * g.W + j;
-        if (g.mode == 0) {
+        if constexpr (FAST) {
+            g.outA[o] = sample_bicubic_fast(A, (float)i - hy, (float)j - hx, g.H, g.W, hm3, wm3);
+            g.outB[o] = sample_bicubic_fast(B, (float)i + hy, (float)j + hx, g.H, g.W, hm3, wm3);
+        } else if (g.mode == 0) {
             g.outA[o] = sample_bicubic(A, (float)i - hy, (float)j - hx, g.H, g.W, hm3, wm3);
             g.outB[o] = sample_bicubic(B, (float)i + hy, (float)j + hx, g.H, g.W, hm3, wm3);
         } else {
@@ -370,7 +446,8 @@ static int warp_entry(const float *d_imgA, const float *d_imgB, const unsigned s
     g.cy = d_ctrY; g.cx = d_ctrX; g.outA = d_outA; g.outB = d_outB;
     g.H = H; g.W = W; g.nPY = nPY; g.nPX = nPX;
     g.predStride = shared_predictor ? 0 : nPY * nPX;
-    g.mode = interp_mode == 0 ? 0 : 1;
+    if (interp_mode < 0 || interp_mode > 2) return PIVAMD_ERR_ARG;
+    g.mode = interp_mode;
     g.roundShifts = round_shifts ? 1 : 0;
     g.lut = nullptr;
     if (g.mode == 1) {
@@ -378,8 +455,13 @@ static int warp_entry(const float *d_imgA, const float *d_imgB, const unsigned s
         if (!g.lut) return PIVAMD_ERR_NOMEM;
     }
     dim3 grid((W + kTileX - 1) / kTileX, (H + kTileY - 1) / kTileY, N);
-    if (d_imgA16) hipLaunchKernelGGL(warp_kernel<unsigned short>, grid, dim3(256), 0, st, g);
-    else hipLaunchKernelGGL(warp_kernel<float>, grid, dim3(256), 0, st, g);
+    if (g.mode == 2) {
+        if (d_imgA16) hipLaunchKernelGGL((warp_kernel<unsigned short, true>), grid, dim3(256), 0, st, g);
+        else hipLaunchKernelGGL((warp_kernel<float, true>), grid, dim3(256), 0, st, g);
+    } else {
+        if (d_imgA16) hipLaunchKernelGGL((warp_kernel<unsigned short, false>), grid, dim3(256), 0, st, g);
+        else hipLaunchKernelGGL((warp_kernel<float, false>), grid, dim3(256), 0, st, g);
+    }
     return hipGetLastError() == hipSuccess ? 0 : PIVAMD_ERR_LAUNCH;
 }
 

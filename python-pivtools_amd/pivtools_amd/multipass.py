@@ -112,7 +112,7 @@ class MultipassPIV:
                  median=(0.1, 2.0, 5), final_infill="nearest", dft="fft", device="cuda", mid_infill="nearest",
                  vector_masks=None, pixel_mask=None, mask_threshold=0.5, predictor_fill="harmonic",
                  predictor_bcs=None, outlier_methods=None, num_peaks=1, secondary_peak=False,
-                 dump_correlation_planes=False):
+                 dump_correlation_planes=False, warp_arithmetic="fma"):
         """``vector_masks``: one (n_win_y, n_win_x) bool array per pass (True =
         masked window), or ``pixel_mask``: an (H, W) bool image mask turned
         into them with ``mask_threshold``.  ``predictor_fill``: 'harmonic',
@@ -123,6 +123,10 @@ class MultipassPIV:
         outlier_detection.methods), e.g. [{"type": "peak_mag", "threshold":
         0.25}, {"type": "median_2d", "epsilon": 0.1, "threshold": 2}]; when
         given it replaces ``median``; an empty list runs no outlier test.
+        ``warp_arithmetic``: 'fma' (default) evaluates the bicubic warp's
+        interpolants separably with fused multiply-adds (last-bit differences
+        in the warped frames, fewer instructions); 'exact' warps with the
+        reference's arithmetic, bit for bit.  Lanczos-3 is always exact.
         ``dump_correlation_planes``: keep every pass's raw correlation
         planes and peak outputs in ``PassResult.debug_dump`` (the reference's
         debug dump; N*windows*h*w floats per pass).
@@ -141,6 +145,10 @@ class MultipassPIV:
         self.finder = PEAK_FINDERS[peak_finder] if isinstance(peak_finder, str) else int(peak_finder)
         self.impl = 1 if peak_fit_impl == "batch" else 0
         self.interp = 0 if warp_interpolation == "cubic" else 1
+        if warp_arithmetic not in ("exact", "fma"):
+            raise ValueError(f"warp_arithmetic {warp_arithmetic!r}: 'exact' or 'fma'")
+        if warp_arithmetic == "fma" and self.interp == 0:
+            self.interp = 2                      # Lanczos-3 keeps the reference's arithmetic
         self.smoothing = bool(predictor_smoothing)
         self.median = median
         self.num_peaks = int(num_peaks)

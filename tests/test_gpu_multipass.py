@@ -679,3 +679,117 @@ def test_correlation_plane_dump(tmp_path):
     pr = pair_result(res, 1)
     assert save_corr_plane_dump(pr, tmp_path, 3) is not None
     assert save_piv_result(pr, tmp_path, 3).endswith("B00003.mat")
+
+
+def _warp_fp64(img_a, img_b, pdy, pdx, cy, cx):
+    """FP64 evaluation of the symmetric bicubic warp (fused_warp.c's
+    formulas, exact arithmetic): np.interp predictor index with clamp,
+    bicubic predictor with replicate border, bicubic taps zero outside."""
+    def keys(d):
+        d = np.asarray(d, np.float64)
+        s0, q2, s3 = 1 + d, 1 - d, 2 - d
+        return np.stack([((-0.75 * s0 + 3.75) * s0 - 6) * s0 + 3, (1.25 * d - 2.25) * d * d + 1,
+                         (1.25 * q2 - 2.25) * q2 * q2 + 1, ((-0.75 * s3 + 3.75) * s3 - 6) * s3 + 3])
+    n, H, W = img_a.shape
+    nPY, nPX = pdy.shape[1:]
+    fy = np.interp(np.arange(H), cy, np.arange(nPY))
+    fx = np.interp(np.arange(W), cx, np.arange(nPX))
+    iy, ix = np.floor(fy).astype(int) - 1, np.floor(fx).astype(int) - 1
+    wy, wx = keys(fy - np.floor(fy)), keys(fx - np.floor(fx))
+    outs = []
+    for k in range(n):
+        d = []
+        for P in (pdy[k], pdx[k]):
+            acc = np.zeros((H, W))
+            for m in range(4):
+                rows = np.clip(iy + m, 0, nPY - 1)
+                for q in range(4):
+                    cols = np.clip(ix + q, 0, nPX - 1)
+                    acc += wy[m][:, None] * wx[q][None, :] * P[rows][:, cols]
+            d.append(acc)
+        hy, hx = 0.5 * d[0], 0.5 * d[1]
+        ii, jj = np.meshgrid(np.arange(H), np.arange(W), indexing="ij")
+        pair = []
+        for img, sgn in ((img_a[k], -1), (img_b[k], 1)):
+            py, px = ii + sgn * hy, jj + sgn * hx
+            by, bx = np.floor(py).astype(int) - 1, np.floor(px).astype(int) - 1
+            ky, kx = keys(py - np.floor(py)), keys(px - np.floor(px))
+            val = np.zeros((H, W))
+            for m in range(4):
+                r = by + m
+                for q in range(4):
+                    c = bx + q
+                    ok = (r >= 0) & (r < H) & (c >= 0) & (c < W)
+                    val += np.where(ok, ky[m] * kx[q] * img.astype(np.float64)[np.clip(r, 0, H - 1),
+                                                                                  np.clip(c, 0, W - 1)], 0.0)
+            pair.append(val)
+        outs.append(pair)
+    return outs
+
+
+def test_fma_warp_against_exact_warp_and_fp64():
+    """The separable fused-multiply-add warp (mode 2) against the
+    reference-exact warp (mode 0) and an FP64 evaluation of the same
+    interpolants: both float32 forms stay within 1e-5 of full scale of the
+    FP64 result (about 5e-6 measured, mostly from the float32 predictor
+    index both share), and the FMA form is no less accurate than the exact
+    one."""
+    L = _lib()
+    rng = np.random.default_rng(12)
+    n, H, W, nPY, nPX = 2, 70, 90, 5, 6
+    img = rng.integers(0, 4096, (2, n, H, W))
+    i16 = _t(img.astype(np.int16), torch.int16)
+    pdy_h = rng.normal(0, 2, (n, nPY, nPX)).astype(np.float32)
+    pdx_h = rng.normal(0, 2, (n, nPY, nPX)).astype(np.float32)
+    cy_h = np.linspace(-10, H + 10, nPY).astype(np.float32)
+    cx_h = np.linspace(-10, W + 10, nPX).astype(np.float32)
+    pdy, pdx, cy, cx = _t(pdy_h), _t(pdx_h), _t(cy_h), _t(cx_h)
+    outs = {}
+    for mode in (0, 2):
+        oa = torch.empty((n, H, W), device=dev)
+        ob = torch.empty((n, H, W), device=dev)
+        assert L.pivamd_warp_u16(i16[0].data_ptr(), i16[1].data_ptr(), oa.data_ptr(), ob.data_ptr(),
+                                 pdy.data_ptr(), pdx.data_ptr(), n, H, W, nPY, nPX, cy.data_ptr(), cx.data_ptr(),
+                                 mode, 0, 0, _st()) == 0
+        outs[mode] = (oa.cpu().numpy(), ob.cpu().numpy())
+    ref = _warp_fp64(img[0], img[1], pdy_h.astype(np.float64), pdx_h.astype(np.float64), cy_h.astype(np.float64),
+                     cx_h.astype(np.float64))
+    scale = 4095.0
+    errs = {}
+    for mode in (0, 2):
+        e = max(np.abs(outs[mode][s][k] - ref[k][s]).max() for k in range(n) for s in range(2))
+        errs[mode] = e / scale
+    assert errs[0] < 1e-5 and errs[2] < 1e-5, errs
+    assert errs[2] <= 1.5 * errs[0] + 1e-7, errs
+    d = max(np.abs(outs[2][s] - outs[0][s]).max() for s in range(2)) / scale
+    assert 0 < d < 1.5e-5, d                 # at most the two errors added
+
+
+def test_exact_warp_loop_matches_oracle_loop():
+    """The pass loop with the reference-exact warp (the default is the FMA
+    warp, covered by test_multipass_loop_matches_oracle_loop) against the
+    oracle loop: identical NaN masks and reasons, displacements within the
+    float tolerance."""
+    shape = (192, 256)
+    sched = [(64, 64), (32, 32), (16, 16)]
+    A, B = [], []
+    for k in range(2):
+        rng = np.random.default_rng(300 + k)
+        x, y, inten = S._positions(rng, shape, 0.03)
+        dx, dy = S.lamb_oseen(x, y, shape, rc=40.0, peak=3.0, uniform=(1.5, 0.0))
+        A.append(S.render(shape, x, y, inten).numpy())
+        B.append(S.render(shape, x + dx, y + dy, inten).numpy())
+    A = np.stack(A).astype(np.float32)
+    B = np.stack(B).astype(np.float32)
+    mp = MultipassPIV(shape, sched, [50, 50, 50], warp_arithmetic="exact")
+    res = mp.run(_t(A), _t(B), keep_passes=True)
+    torch.cuda.synchronize()
+    grids = build_pass_grids(shape, sched, [50, 50, 50])
+    ref = M.run_multipass(A, B, grids, keep_passes=True)
+    for p, (g, r) in enumerate(zip(res, ref)):
+        gm = g.nan_mask.cpu().numpy().astype(bool)
+        np.testing.assert_array_equal(gm, r["nan_mask"])
+        np.testing.assert_array_equal(g.nan_reason.cpu().numpy(), r["nan_reason"])
+        same = ~gm
+        d = np.hypot(g.ux.cpu().numpy()[same] - r["ux"][same], g.uy.cpu().numpy()[same] - r["uy"][same])
+        assert np.sqrt(np.mean(d ** 2)) < 0.005, (p, d.max())
