@@ -30,6 +30,9 @@
 
 namespace pivamd {
 
+// The glue restates NumPy float32 expressions: no fused multiply-adds.
+#pragma clang fp contract(off)
+
 __global__ void u16_to_f32_kernel(const uint16_t *in, float *out, size_t n)
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -183,31 +186,50 @@ __global__ void peak_final_kernel(const float *usel, const float *pk_sel, const 
     if (nm) choice[i] = 0;
 }
 
+// Batcher's odd-even merge sort of K registers (any K; 132 compare-exchanges
+// for K = 24, 19 for K = 8), fully unrolled so every index is a constant.
+template <int K>
+__device__ __forceinline__ void sort_net(float (&w)[K])
+{
+#pragma unroll
+    for (int p = 1; p < K; p += p)
+#pragma unroll
+        for (int k = p; k > 0; k /= 2)
+#pragma unroll
+            for (int j = k % p; j + k < K; j += k + k)
+#pragma unroll
+                for (int i = 0; i < k && i < K - j - k; ++i)
+                    if ((j + i) / (p + p) == (j + i + k) / (p + p)) {
+                        const float a = w[j + i], b = w[j + i + k];
+                        w[j + i] = fminf(a, b);
+                        w[j + i + k] = fmaxf(a, b);
+                    }
+}
+
 // nanmedian of K values held in registers (bottleneck semantics: NaNs
 // dropped, the middle element for an odd count, the mean of the middle pair
-// for an even one, NaN when nothing is left).  Fully unrolled rank
-// selection: every index is a compile-time constant, nothing spills.
+// for an even one, NaN when nothing is left).  NaNs become +inf and sort
+// last; the n values in front are the finite ones (and any real +inf, equal
+// to the stand-ins), so the middle of those is the median.
 template <int K>
 __device__ __forceinline__ float nanmedian_reg(const float (&v)[K])
 {
+    float w[K];
     int n = 0;
 #pragma unroll
-    for (int i = 0; i < K; ++i) n += (v[i] == v[i]) ? 1 : 0;
-    if (n == 0) return __builtin_nanf("");
-    const int klo = (n - 1) >> 1, khi = n >> 1;
-    float lo = 0.0f, hi = 0.0f;
-#pragma unroll
     for (int i = 0; i < K; ++i) {
-        int rank = 0;
-#pragma unroll
-        for (int j = 0; j < K; ++j) {
-            if (j == i) continue;
-            // stable rank among the non-NaN values (NaN compares false)
-            rank += (v[j] < v[i] || (j < i && v[j] == v[i])) ? 1 : 0;
-        }
         const bool ok = v[i] == v[i];
-        lo = (ok && rank == klo) ? v[i] : lo;
-        hi = (ok && rank == khi) ? v[i] : hi;
+        n += ok ? 1 : 0;
+        w[i] = ok ? v[i] : __builtin_inff();
+    }
+    if (n == 0) return __builtin_nanf("");
+    sort_net<K>(w);
+    const int klo = (n - 1) >> 1, khi = n >> 1;
+    float lo = w[0], hi = w[0];
+#pragma unroll
+    for (int i = 1; i < K; ++i) {
+        lo = i == klo ? w[i] : lo;
+        hi = i == khi ? w[i] : hi;
     }
     return (n & 1) ? lo : 0.5f * (lo + hi);
 }
