@@ -217,6 +217,7 @@ struct WarpArgs {
 };
 
 constexpr int kPredMax = 1024;   // predictor cells a tile may stage in LDS
+constexpr int kRowSums = 8;      // predictor rows whose x sums a FAST tile keeps in LDS
 
 template <typename TI>
 __device__ __forceinline__ const TI *warp_src(const WarpArgs &g, bool a)
@@ -241,6 +242,10 @@ __global__ void __launch_bounds__(256) warp_kernel(WarpArgs g)
     __shared__ int4 s_ro[kTileY];       // predictor row offsets of each tile row
     __shared__ float s_pdy[kPredMax];
     __shared__ float s_pdx[kPredMax];
+    // FAST: x sums sum_n wx[n] P[r][c_n] of each staged predictor row r at
+    // each tile column; a pixel then needs only sum_m wy[m] * S[r_m]
+    __shared__ float s_sy[FAST ? kRowSums * kTileX : 1];
+    __shared__ float s_sx[FAST ? kRowSums * kTileX : 1];
     const int tid = (int)threadIdx.x;
     const int tx = tid & (kTileX - 1);
     const int ty = tid / kTileX;
@@ -290,7 +295,9 @@ __global__ void __launch_bounds__(256) warp_kernel(WarpArgs g)
             s_pdx[e] = dxp[(pr0 + r) * g.nPX + pc0 + c];
         }
     }
+    const bool rowsums = FAST && staged && prh <= kRowSums;
     // y weights and row offsets of the predictor stencil, once per tile row
+    // (FAST with row sums: the staged row index instead of the offset)
     if (tid < kTileY) {
         const float fiy = s_ly[tid], fiyf = floorf(fiy);
         const int iyb = (int)fiyf - 1;
@@ -300,12 +307,36 @@ __global__ void __launch_bounds__(256) warp_kernel(WarpArgs g)
 #pragma unroll
         for (int m = 0; m < 4; m++) {
             const int rr = min(max(iyb + m, 0), g.nPY - 1);
-            ro[m] = staged ? (rr - pr0) * pcw - pc0 : rr * g.nPX;
+            ro[m] = rowsums ? rr - pr0 : (staged ? (rr - pr0) * pcw - pc0 : rr * g.nPX);
         }
         s_wy[tid] = make_float4(wy[0], wy[1], wy[2], wy[3]);
         s_ro[tid] = make_int4(ro[0], ro[1], ro[2], ro[3]);
     }
     __syncthreads();
+
+    if constexpr (FAST) {
+        if (rowsums) {
+            const float fxs = s_lx[tx], fxsf = floorf(fxs);
+            const int ixs = (int)fxsf - 1;
+            float wxs[4];
+            keys4_fma(fxs - fxsf, wxs);
+            int cs[4];
+#pragma unroll
+            for (int n = 0; n < 4; n++) cs[n] = min(max(ixs + n, 0), g.nPX - 1) - pc0;
+            for (int r = ty; r < prh; r += kLanesY) {
+                const float *py = s_pdy + r * pcw, *px = s_pdx + r * pcw;
+                float ry = wxs[0] * py[cs[0]], rx = wxs[0] * px[cs[0]];
+#pragma unroll
+                for (int n = 1; n < 4; n++) {
+                    ry = __builtin_fmaf(wxs[n], py[cs[n]], ry);
+                    rx = __builtin_fmaf(wxs[n], px[cs[n]], rx);
+                }
+                s_sy[r * kTileX + tx] = ry;
+                s_sx[r * kTileX + tx] = rx;
+            }
+            __syncthreads();
+        }
+    }
 
     const int j = x0 + tx;
     if (j >= g.W) return;
@@ -348,8 +379,18 @@ __global__ void __launch_bounds__(256) warp_kernel(WarpArgs g)
                     vb = __builtin_fmaf(wy[m], rx, vb);
                 }
             };
-            if (staged) pred_sum(s_pdy, s_pdx);
-            else pred_sum(dyp, dxp);
+            if (rowsums) {
+                // the same terms in the same order as pred_sum, the x sums read back
+#pragma unroll
+                for (int m = 0; m < 4; m++) {
+                    va = __builtin_fmaf(wy[m], s_sy[ro[m] * kTileX + tx], va);
+                    vb = __builtin_fmaf(wy[m], s_sx[ro[m] * kTileX + tx], vb);
+                }
+            } else if (staged) {
+                pred_sum(s_pdy, s_pdx);
+            } else {
+                pred_sum(dyp, dxp);
+            }
         } else if (staged) {
 #pragma unroll
             for (int m = 0; m < 4; m++) {
