@@ -142,8 +142,8 @@ def _detect(ux, uy, pkh, mask, methods):
 
 def peaks_stage(pkx, pky, pkh, pred, mask, tx, ty, methods, secondary):
     """Several peaks per window (cpu_instantaneous.py:586-760): pk* (N, P,
-    ny, nx).  Gate every peak, add the predictor, judge the primary peak,
-    then (secondary) move invalid windows to their next peak and re-judge
+    ny, nx).  Gate every peak, add the predictor, test the primary peak,
+    then (secondary) move invalid windows to their next peak and re-test
     them; returns u (NaN at invalid, 0 at masked), nan_mask, nan_reason,
     the chosen peak's height (NaN at invalid), peak_choice and the gated
     heights of every peak."""

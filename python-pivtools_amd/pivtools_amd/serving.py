@@ -60,6 +60,9 @@ class StreamSpec:
     sensor_shape: tuple | None = None
     dark_a: torch.Tensor | None = None     # sensor-type dark of the A frames (device)
     dark_b: torch.Tensor | None = None     # and of the B frames
+    # further MultipassPIV options of the stream (outlier_methods,
+    # mid_infill, final_infill, vector_masks / pixel_mask, num_peaks, ...)
+    piv_options: dict = field(default_factory=dict)
 
 
 @dataclass
@@ -179,7 +182,7 @@ class PIVServer:
         st = torch.cuda.Stream(device=self.dev)
         with torch.cuda.stream(st):
             mp = MultipassPIV(spec.shape, spec.window_sizes, spec.overlaps, peak_finder=spec.peak_finder,
-                              device=self.dev)
+                              device=self.dev, **spec.piv_options)
             sa = torch.zeros((self.max_batch, H, W), dtype=torch.int16, device=self.dev)
             sb = torch.zeros_like(sa)
         slot = _Slot(mp, st, sa, sb)

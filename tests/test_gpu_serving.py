@@ -99,3 +99,30 @@ def test_graph_replays_match_eager_repeatedly():
         torch.cuda.synchronize()
         assert _same(ux, res.ux[0]) and _same(uy, res.uy[0])
     assert 1 in srv.slots[0][0].graphs
+
+
+def test_stream_with_shipped_options_replays_its_graph_exactly():
+    """A stream configured like the reference's shipped settings (peak
+    magnitude and median tests, biharmonic infill, a pixel mask, several
+    peaks with substitution) served through captured graphs equals eager runs
+    of the same options, pair for pair, over several replays."""
+    shape = (256, 256)
+    yy, xx = np.mgrid[0:shape[0], 0:shape[1]]
+    opts = {"outlier_methods": [{"type": "peak_mag", "threshold": 0.25},
+                                {"type": "median_2d", "epsilon": 0.1, "threshold": 2}],
+            "mid_infill": "biharmonic", "final_infill": "biharmonic",
+            "pixel_mask": (yy - 120) ** 2 + (xx - 100) ** 2 < 30 ** 2,
+            "num_peaks": 2, "secondary_peak": True}
+    spec = StreamSpec("shipped", shape, [(64, 64), (32, 32), (16, 16)], [50, 50, 50], piv_options=opts)
+    srv = PIVServer([spec], max_batch=2, max_slots_per_stream=1)
+    eager = MultipassPIV(shape, spec.window_sizes, spec.overlaps, **opts)
+    for k in range(5):
+        a, b = _pair(spec, 8300 + k)
+        r = srv.submit(0, a, b)
+        srv.drain()
+        res = eager.run(a[None], b[None])[-1]
+        ux, uy, nm = r.result()
+        torch.cuda.synchronize()
+        assert _same(ux, res.ux[0]) and _same(uy, res.uy[0]) and _same(nm, res.nan_mask[0]), k
+    eager.check_infill()
+    assert 1 in srv.slots[0][0].graphs
