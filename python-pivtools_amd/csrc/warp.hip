@@ -123,9 +123,27 @@ __device__ __forceinline__ void keys4_fma(float d, float w[4])
 
 // The separable bicubic sample: sum_m wy[m] * (sum_n wx[n] * img[m][n]),
 // zero outside the image.
+typedef uint32_t u32x3 __attribute__((ext_vector_type(3)));
+
+// A frame as a raw buffer (one 12-byte load per row of taps): the base is
+// the frame start rounded down to 4 bytes, so every offset below is aligned.
+struct FrameBuf {
+    __amdgpu_buffer_rsrc_t rsrc;
+    uintptr_t base;        // frame start rounded down to 4 bytes
+};
+
+__device__ __forceinline__ FrameBuf frame_buf(const void *frame, size_t bytes)
+{
+    FrameBuf f;
+    f.base = (uintptr_t)frame & ~(uintptr_t)3;
+    const int n = (int)(bytes + ((uintptr_t)frame & 3));
+    f.rsrc = __builtin_amdgcn_make_buffer_rsrc((void *)f.base, (short)0, n, 0x00020000);
+    return f;
+}
+
 template <typename TI>
 __device__ __forceinline__ float sample_bicubic_fast(const TI *img, float fy, float fx, int H, int W, int hm3,
-                                                     int wm3)
+                                                     int wm3, const FrameBuf &fb)
 {
     const float fyf = floorf(fy), fxf = floorf(fx);
     const int iy = (int)fyf - 1, ix = (int)fxf - 1;
@@ -134,6 +152,35 @@ __device__ __forceinline__ float sample_bicubic_fast(const TI *img, float fy, fl
     keys4_fma(fx - fxf, wx);
     float val = 0.0f;
     if (inside(iy, hm3) && inside(ix, wm3)) {
+        if constexpr (sizeof(TI) == 2) {
+            // 16-bit taps: each row's four taps lie in the three aligned
+            // dwords from their address rounded down to 4 bytes, at a 0- or
+            // 16-bit offset that alignbit takes out.  Aligned reads cost the
+            // texture addresser less than 2-byte-aligned 8-byte reads.  The
+            // run may extend 2 bytes either side of the taps: it is used only
+            // when that stays inside the frame.
+            const uintptr_t a0 = (uintptr_t)(img + (iy * W + ix));
+            const uintptr_t rowb = (uintptr_t)W * 2u;
+            const uintptr_t lo = (uintptr_t)img, hi = (uintptr_t)(img + (size_t)H * W);
+            if ((a0 & ~(uintptr_t)3) >= lo && a0 + 3 * rowb + 12 <= hi) {
+#pragma unroll
+                for (int m = 0; m < 4; m++) {
+                    const uintptr_t am = a0 + m * rowb;
+                    const int off = (int)((am & ~(uintptr_t)3) - fb.base);
+                    const u32x3 v = __builtin_amdgcn_raw_buffer_load_b96(fb.rsrc, off, 0, 0);
+                    const uint32_t sh = (uint32_t)(am & 2u) * 8u;
+                    const uint32_t d0 = v.x, d1 = v.y, d2 = v.z;
+                    const uint32_t t01 = __builtin_amdgcn_alignbit(d1, d0, sh);
+                    const uint32_t t23 = __builtin_amdgcn_alignbit(d2, d1, sh);
+                    float rs = wx[0] * (float)(t01 & 0xffffu);
+                    rs = __builtin_fmaf(wx[1], (float)(t01 >> 16), rs);
+                    rs = __builtin_fmaf(wx[2], (float)(t23 & 0xffffu), rs);
+                    rs = __builtin_fmaf(wx[3], (float)(t23 >> 16), rs);
+                    val = __builtin_fmaf(wy[m], rs, val);
+                }
+                return val;
+            }
+        }
         const TI *p = img + (iy * W + ix);
 #pragma unroll
         for (int m = 0; m < 4; m++) {
@@ -345,6 +392,7 @@ __global__ void __launch_bounds__(256) warp_kernel(WarpArgs g)
     const TI *B = warp_src<TI>(g, false) + (size_t)img * plane;
     const int hm3 = max(g.H - 3, 0), wm3 = max(g.W - 3, 0);
     const int hm5 = max(g.H - 5, 0), wm5 = max(g.W - 5, 0);
+    const FrameBuf fbA = frame_buf(A, plane * sizeof(TI)), fbB = frame_buf(B, plane * sizeof(TI));
 
     const float fx = s_lx[tx], fxf = floorf(fx);
     const int ix = (int)fxf - 1;
@@ -418,8 +466,8 @@ __global__ void __launch_bounds__(256) warp_kernel(WarpArgs g)
         hx = fminf(fmaxf(hx, -kShiftMax), kShiftMax);
         const size_t o = (size_t)img * plane + (size_t)i * g.W + j;
         if constexpr (FAST) {
-            g.outA[o] = sample_bicubic_fast(A, (float)i - hy, (float)j - hx, g.H, g.W, hm3, wm3);
-            g.outB[o] = sample_bicubic_fast(B, (float)i + hy, (float)j + hx, g.H, g.W, hm3, wm3);
+            g.outA[o] = sample_bicubic_fast(A, (float)i - hy, (float)j - hx, g.H, g.W, hm3, wm3, fbA);
+            g.outB[o] = sample_bicubic_fast(B, (float)i + hy, (float)j + hx, g.H, g.W, hm3, wm3, fbB);
         } else if (g.mode == 0) {
             g.outA[o] = sample_bicubic(A, (float)i - hy, (float)j - hx, g.H, g.W, hm3, wm3);
             g.outB[o] = sample_bicubic(B, (float)i + hy, (float)j + hx, g.H, g.W, hm3, wm3);
