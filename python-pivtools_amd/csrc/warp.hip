@@ -296,7 +296,16 @@ __global__ void __launch_bounds__(256) warp_kernel(WarpArgs g)
     const int tid = (int)threadIdx.x;
     const int tx = tid & (kTileX - 1);
     const int ty = tid / kTileX;
-    const int x0 = blockIdx.x * kTileX, y0 = blockIdx.y * kTileY;
+    // Workgroups go to the XCDs (and their shader engines) round robin in
+    // launch order, so launch column x of tile row y lands on XCD
+    // (x + y * (cols mod 8)) mod 8.  When that pins an XCD to the same tile
+    // columns down the whole frame (cols a multiple of 8, as at 1024 and
+    // 2048 pixels), the engines holding the border columns fall behind and
+    // the warp runs at half speed.  Rotating tile row y by (cols mod 8) + 1
+    // columns moves every XCD on by one column per tile row for any width.
+    const int rot = (int)(gridDim.x & 7u) + 1;
+    const int bx = (int)((blockIdx.x + blockIdx.y * rot) % gridDim.x);
+    const int x0 = bx * kTileX, y0 = blockIdx.y * kTileY;
     const int img = blockIdx.z;
     const int nx_t = min(kTileX, g.W - x0), ny_t = min(kTileY, g.H - y0);
     if (tid < kTileX) {

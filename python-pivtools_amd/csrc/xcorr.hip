@@ -51,7 +51,7 @@ constexpr int kMaxLds = 158 * 1024;
 
 // Lanes per line transform (R) for window size N: the default, and the
 // values built as selectable variants (fft_reg holds N/R points per lane).
-constexpr int xc_radix(int N) { return N >= 64 ? 4 : (N >= 48 ? 2 : 1); }
+constexpr int xc_radix(int N) { return N >= 48 ? 4 : 1; }
 
 constexpr bool xc_radix_built(int N, int R)
 {

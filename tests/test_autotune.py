@@ -12,7 +12,7 @@ from pivtools_amd import autotune as AT
 
 BUILT = {8: [1], 12: [1], 16: [1, 2, 16], 24: [1, 2], 32: [1, 2, 4, 16], 48: [2, 4, 16], 64: [2, 4, 16, 36],
          96: [4, 36], 128: [4, 8, 36]}
-DEFAULT = {8: 1, 12: 1, 16: 1, 24: 1, 32: 1, 48: 2, 64: 4, 96: 4, 128: 4}
+DEFAULT = {8: 1, 12: 1, 16: 1, 24: 1, 32: 1, 48: 4, 64: 4, 96: 4, 128: 4}
 
 
 @pytest.mark.parametrize("n", sorted(BUILT))
