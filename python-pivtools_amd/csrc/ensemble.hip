@@ -329,6 +329,9 @@ __global__ void __launch_bounds__((EnsCfg<N>::BLOCK)) ens_kernel_reg(EnsArgs g)
     auto flush = [&]() {
         if (valid) {
             double *row = acc + (size_t)ky * HW;
+            // an opaque base keeps the 4 x HW bin addresses out of the image
+            // loop: hoisted, they held about 140 registers for the whole call
+            asm volatile("" : "+v"(row));
 #pragma unroll
             for (int k = 0; k < HW; ++k) {
                 double *p0 = row + k, *p1 = row + N * HW + k, *p2 = row + 2 * N * HW + k, *p3 = row + 3 * N * HW + k;
@@ -337,6 +340,9 @@ __global__ void __launch_bounds__((EnsCfg<N>::BLOCK)) ens_kernel_reg(EnsArgs g)
                 *p2 = (first ? 0.0 : *p2) + (double)aAA[k];
                 *p3 = (first ? 0.0 : *p3) + (double)aBB[k];
                 aRe[k] = 0.0f; aIm[k] = 0.0f; aAA[k] = 0.0f; aBB[k] = 0.0f;
+                // one bin at a time: hoisting every bin's FP64 loads would
+                // raise the kernel's register peak for a once-per-flush step
+                asm volatile("" ::: "memory");
             }
         }
         first = false;
@@ -348,40 +354,85 @@ __global__ void __launch_bounds__((EnsCfg<N>::BLOCK)) ens_kernel_reg(EnsArgs g)
         // are re-read from the cache each image instead of pinned in VGPRs
         asm volatile("" ::: "memory");
         float z1r[N], z1i[N];
-        float sAB_B = 0.0f, sAA = 0.0f, sBB = 0.0f;
-        const float *pa = g.imgA + (size_t)n * plane + (size_t)row0 * g.W + col0 + t;
-        const float *pb = g.imgB + (size_t)n * plane + (size_t)row0 * g.W + col0 + t;
-        if (valid) {
+        // The frames as buffer resources based at image n: one 32-bit lane
+        // offset serves all N rows (the row step is a uniform offset), where
+        // 64-bit row addresses cost two registers per row and load.
+        const __amdgpu_buffer_rsrc_t rA = __builtin_amdgcn_make_buffer_rsrc(
+            (void *)(g.imgA + (size_t)n * plane), (short)0, (int)(plane * sizeof(float)), 0x00020000);
+        const __amdgpu_buffer_rsrc_t rB = __builtin_amdgcn_make_buffer_rsrc(
+            (void *)(g.imgB + (size_t)n * plane), (short)0, (int)(plane * sizeof(float)), 0x00020000);
+        const int voff = (row0 * g.W + col0 + t) * (int)sizeof(float);
+        auto pa = [&](int y) {
+            return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rA, voff, y * g.W * 4, 0));
+        };
+        auto pb = [&](int y) {
+            return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rB, voff, y * g.W * 4, 0));
+        };
+        float mB = 0.0f, mAA = 0.0f, mBB = 0.0f;
+        float eAA = 0.0f, eBB = 0.0f, dummy = 0.0f;
+        if constexpr (SHARED) {
+            // wBab is wBB and wAab is wAA (and their sums agree), so the AB
+            // operands are the auto operands: two weight columns per image
+            // instead of four, the same values bit for bit
+            float sAA = 0.0f, sBB = 0.0f;
+            if (valid) {
+#pragma unroll
+                for (int y = 0; y < N; ++y) {
+                    z1r[y] = pb(y);
+                    z1i[y] = pa(y);
+                    if (g.meanSub) {
+                        sAA += z1i[y] * g.wAA[y * N + t];
+                        sBB += z1r[y] * g.wBB[y * N + t];
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int y = 0; y < N; ++y) { z1r[y] = 0.0f; z1i[y] = 0.0f; }
+            }
+            if (g.meanSub) {
+                ens_sum3(sBB, sAA, dummy, r0, r1, r2, t, T);
+                mAA = (g.sumWAA > 0.0f) ? sAA / g.sumWAA : 0.0f;
+                mBB = (g.sumWBB > 0.0f) ? sBB / g.sumWBB : 0.0f;
+            }
 #pragma unroll
             for (int y = 0; y < N; ++y) {
-                z1r[y] = pb[(size_t)y * g.W];
-                z1i[y] = pa[(size_t)y * g.W];
-                if (g.meanSub) {
-                    sAB_B += z1r[y] * g.wBab[y * N + t];
-                    sAA += z1i[y] * g.wAA[y * N + t];
-                    sBB += z1r[y] * g.wBB[y * N + t];
-                }
+                z1r[y] = (z1r[y] - mBB) * g.wBB[y * N + t];
+                z1i[y] = (z1i[y] - mAA) * g.wAA[y * N + t];
+                eAA += z1i[y] * z1i[y];
+                eBB += z1r[y] * z1r[y];
             }
         } else {
+            float sAB_B = 0.0f, sAA = 0.0f, sBB = 0.0f;
+            if (valid) {
 #pragma unroll
-            for (int y = 0; y < N; ++y) { z1r[y] = 0.0f; z1i[y] = 0.0f; }
-        }
-        float mB = 0.0f, mAA = 0.0f, mBB = 0.0f;
-        if (g.meanSub) {
-            ens_sum3(sAB_B, sAA, sBB, r0, r1, r2, t, T);
-            mB = (g.sumWBab > 0.0f) ? sAB_B / g.sumWBab : 0.0f;
-            mAA = (g.sumWAA > 0.0f) ? sAA / g.sumWAA : 0.0f;
-            mBB = (g.sumWBB > 0.0f) ? sBB / g.sumWBB : 0.0f;
-        }
-        float eAA = 0.0f, eBB = 0.0f, dummy = 0.0f;
+                for (int y = 0; y < N; ++y) {
+                    z1r[y] = pb(y);
+                    z1i[y] = pa(y);
+                    if (g.meanSub) {
+                        sAB_B += z1r[y] * g.wBab[y * N + t];
+                        sAA += z1i[y] * g.wAA[y * N + t];
+                        sBB += z1r[y] * g.wBB[y * N + t];
+                    }
+                }
+            } else {
 #pragma unroll
-        for (int y = 0; y < N; ++y) {
-            const float b = z1r[y], a = z1i[y];
-            z1r[y] = (b - mB) * g.wBab[y * N + t];
-            z1i[y] = (a - mAA) * g.wAab[y * N + t];
-            const float pA = (a - mAA) * g.wAA[y * N + t], pB = (b - mBB) * g.wBB[y * N + t];
-            eAA += pA * pA;
-            eBB += pB * pB;
+                for (int y = 0; y < N; ++y) { z1r[y] = 0.0f; z1i[y] = 0.0f; }
+            }
+            if (g.meanSub) {
+                ens_sum3(sAB_B, sAA, sBB, r0, r1, r2, t, T);
+                mB = (g.sumWBab > 0.0f) ? sAB_B / g.sumWBab : 0.0f;
+                mAA = (g.sumWAA > 0.0f) ? sAA / g.sumWAA : 0.0f;
+                mBB = (g.sumWBB > 0.0f) ? sBB / g.sumWBB : 0.0f;
+            }
+#pragma unroll
+            for (int y = 0; y < N; ++y) {
+                const float b = z1r[y], a = z1i[y];
+                z1r[y] = (b - mB) * g.wBab[y * N + t];
+                z1i[y] = (a - mAA) * g.wAab[y * N + t];
+                const float pA = (a - mAA) * g.wAA[y * N + t], pB = (b - mBB) * g.wBB[y * N + t];
+                eAA += pA * pA;
+                eBB += pB * pB;
+            }
         }
         float sAB = 1.0f, sA = 1.0f, sB = 1.0f;
         bool use = valid;
@@ -414,7 +465,7 @@ __global__ void __launch_bounds__((EnsCfg<N>::BLOCK)) ens_kernel_reg(EnsArgs g)
             float z2r[N], z2i[N];
 #pragma unroll
             for (int y = 0; y < N; ++y) {
-                const float a = valid ? pa[(size_t)y * g.W] : 0.0f, b = valid ? pb[(size_t)y * g.W] : 0.0f;
+                const float a = valid ? pa(y) : 0.0f, b = valid ? pb(y) : 0.0f;
                 z2r[y] = (a - mAA) * g.wAA[y * N + t];
                 z2i[y] = (b - mBB) * g.wBB[y * N + t];
             }
@@ -505,6 +556,8 @@ static int launch_ens(EnsArgs g, hipStream_t st)
     g.nblocks = nb;
     const int grid = (nb + 7) / 8 * 8;
     if constexpr (N <= 32) {
+        // the register form reads each frame as one buffer resource
+        if ((size_t)g.H * g.W * sizeof(float) > (size_t)0x7fffffff) return PIVAMD_ERR_ARG;
         if (g.sharedW)
             hipLaunchKernelGGL((ens_kernel_reg<N, true>), dim3(grid), dim3(C::BLOCK), 0, st, g);
         else
@@ -546,7 +599,7 @@ extern "C" int pivamd_accumulate(const float *d_A, const float *d_B, const float
     g.H = H; g.W = W; g.nImages = n_images; g.nWy = nWy; g.nWx = nWx; g.fitY = fitY; g.fitX = fitX;
     g.meanSub = triple ? meanSub : 0; g.perPairNorm = triple ? perPairNorm : 0; g.triple = triple;
     g.sumWBab = sumWBab; g.sumWAA = sumWAA; g.sumWBB = sumWBB; g.nblocks = 0;
-    g.sharedW = triple && d_wAab == d_wAA && d_wBab == d_wBB;
+    g.sharedW = triple && d_wAab == d_wAA && d_wBab == d_wBB && sumWBab == sumWBB;
     switch (winY) {
     case 8: return launch_ens<8>(g, st);
     case 12: return launch_ens<12>(g, st);
