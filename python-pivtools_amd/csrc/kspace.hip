@@ -452,8 +452,11 @@ struct KsSharedFit : KsShared<TPB, NS> {
     double scal[4];
 };
 
+// Two waves per SIMD: unbounded, the per-lane FP64 normal equations and
+// Cholesky factor took about 310 registers and one workgroup per CU; at 256
+// the compiler spills about 20 values (scratch, outside the bin sweeps).
 template <int TPB, bool GLOBAL, bool KX4, bool KY4>
-__global__ __launch_bounds__(TPB) void kspace_kernel(KsArgs a)
+__global__ __launch_bounds__(TPB, 2) void kspace_kernel(KsArgs a)
 {
     constexpr int K = 7 + (KX4 ? 1 : 0) + (KY4 ? 1 : 0);
     constexpr int NT = K * (K + 1) / 2;
