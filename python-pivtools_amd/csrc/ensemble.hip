@@ -485,26 +485,50 @@ __global__ void __launch_bounds__((EnsCfg<N>::BLOCK)) ens_kernel_reg(EnsArgs g)
             }
         }
         __syncthreads();
-        if ((n + 1) % kEnsFlush == 0) flush();
+        // the call's last images go straight to the readout below
+        if ((n + 1) % kEnsFlush == 0 && n + 1 < g.nImages) flush();
     }
-    if (first || g.nImages % kEnsFlush != 0) flush();
-    __syncthreads();   // rows written by every lane before the column readout
+    // This call's totals: the registers, plus the FP64 sums when an earlier
+    // flush spilled into them (the same value the flush-and-read-back gave).
+    // They are staged row by row in the window's LDS tile, so the column
+    // readout never goes through HBM; calls of up to kEnsFlush images touch
+    // no FP64 sums at all.
+    if (!first && valid) {
+        const double *row = acc + (size_t)ky * HW;
+        asm volatile("" : "+v"(row));
+#pragma unroll
+        for (int k = 0; k < HW; ++k) {
+            aRe[k] = (float)(row[k] + (double)aRe[k]);
+            aIm[k] = (float)(row[N * HW + k] + (double)aIm[k]);
+            aAA[k] = (float)(row[2 * N * HW + k] + (double)aAA[k]);
+            aBB[k] = (float)(row[3 * N * HW + k] + (double)aBB[k]);
+            asm volatile("" ::: "memory");
+        }
+    }
 
     // ---- readout: one c2r per accumulated spectrum, shift, crop, += -------
     const int oh = g.fitY, ow = g.fitX;
     const int sy = (N - oh) / 2, sx = (N - ow) / 2;
     for (int which = 0; which < (g.triple ? 3 : 1); ++which) {
+        if (valid) {
+#pragma unroll
+            for (int k = 0; k < HW; ++k) {
+                if (which == 0) {
+                    Sr[ky * PX + k] = aRe[k];
+                    Si[ky * PX + k] = aIm[k];
+                } else {
+                    Sr[ky * PX + k] = which == 1 ? aAA[k] : aBB[k];
+                }
+            }
+        }
+        __syncthreads();
         if (valid && t < HW) {
+            // each lane reads and then rewrites only its own column t
             float cr[N], ci[N];
 #pragma unroll
             for (int k = 0; k < N; ++k) {
-                if (which == 0) {
-                    cr[k] = (float)acc[(size_t)(0 * N + k) * HW + t];
-                    ci[k] = (float)acc[(size_t)(1 * N + k) * HW + t];
-                } else {
-                    cr[k] = (float)acc[(size_t)((which + 1) * N + k) * HW + t];
-                    ci[k] = 0.0f;
-                }
+                cr[k] = Sr[k * PX + t];
+                ci[k] = which == 0 ? Si[k * PX + t] : 0.0f;
             }
             fft_reg<N, true>(cr, ci);
 #pragma unroll
