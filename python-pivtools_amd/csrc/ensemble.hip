@@ -555,15 +555,24 @@ __global__ void __launch_bounds__((EnsCfg<N>::BLOCK)) ens_kernel_reg(EnsArgs g)
             }
             fft_reg<N, true>(xr, xi);
             const float mul = 1.0f / (float)numel;
-            float *out = which == 0 ? g.outAB : (which == 1 ? g.outAA : g.outBB);
-            out += (size_t)w * oh * ow;
+            // the shifted rows ya and yb go back into this lane's own rows of
+            // the tile, so the plane leaves in row order below
 #pragma unroll
             for (int x = 0; x < N; ++x) {
                 const int xs = (x + N / 2) % N;
-                if (xs >= sx && xs < sx + ow) {
-                    if (yb >= sy && yb < sy + oh) out[(yb - sy) * ow + (xs - sx)] += xr[x] * mul;
-                    if (ya >= sy && ya < sy + oh) out[(ya - sy) * ow + (xs - sx)] += xi[x] * mul;
-                }
+                Sr[yb * PX + xs] = xr[x] * mul;
+                Sr[ya * PX + xs] = xi[x] * mul;
+            }
+        }
+        __syncthreads();
+        if (valid) {
+            // the cropped plane added to the caller's output, consecutive
+            // lanes on consecutive addresses (row-per-lane stores left most
+            // of every written line untouched)
+            float *out = (which == 0 ? g.outAB : (which == 1 ? g.outAA : g.outBB)) + (size_t)w * oh * ow;
+            for (int e = t; e < oh * ow; e += T) {
+                const int y = e / ow, x = e - y * ow;
+                out[e] += Sr[(y + sy) * PX + x + sx];
             }
         }
         __syncthreads();
