@@ -102,6 +102,11 @@ def cpu_baseline(A_dev, B_dev, gpu_final, finder, budget_s, loop_kw=None):
 
 
 ENS_SHAPE = (1024, 1024)
+# HBM bytes per 64-pair accumulate launch (1024x1024 f32, 15625 windows of
+# 32x32): rocprofv3 FETCH_SIZE 539 MiB (x2.0) + WRITE_SIZE 183 MiB (x1.0),
+# the copy-kernel calibration of the multipass counters; recorded in
+# profiles/kernel_stats_ensemble_1024_w32_o75.md.
+ENS_TRAFFIC_64_PAIRS = 1.130e9 + 0.192e9
 ENS_WIN = (32, 32)
 
 
@@ -188,7 +193,8 @@ def run_ensemble(args):
                    "batch": batch},
         "accumulate_ms": round(acc_ms, 3), "finalize_ms": round(fin_ms, 3),
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
-                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+                     "traffic": ENS_TRAFFIC_64_PAIRS if batch == 64 else None,
                      "kernel": "ens_kernel<32> (accumulate)",
                      "note": "algorithmic flops 10N^2log2N + 12N^2 + 18N(N/2+1) per window and pair"},
         "cpu_baseline": cpu, "fit": stats,
