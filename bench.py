@@ -192,11 +192,15 @@ def run_ensemble(args):
                    "overlap": 75, "windows": nwin, "bg_method": "correlation", "kspace_floor": "coloured",
                    "batch": batch},
         "accumulate_ms": round(acc_ms, 3), "finalize_ms": round(fin_ms, 3),
-        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
+        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
                      "traffic": ENS_TRAFFIC_64_PAIRS if batch == 64 else None,
-                     "kernel": "ens_kernel<32> (accumulate)",
-                     "note": "algorithmic flops 10N^2log2N + 12N^2 + 18N(N/2+1) per window and pair"},
+                     "traffic_note": "HBM bytes of ONE 64-pair accumulate launch, a profile-derived "
+                                     "estimate (FETCH_SIZE x2.0 + WRITE_SIZE of a separate rocprofv3 run), "
+                                     "not measured in this run",
+                     "kernel": "ens_kernel_reg<32,true> (accumulate)",
+                     "note": "algorithmic flops 10N^2log2N + 12N^2 + 18N(N/2+1) per window and pair; "
+                             "FP32 VALU work (register FFT), no MFMA"},
         "cpu_baseline": cpu, "fit": stats,
     }
     print(json.dumps(line), flush=True)
@@ -521,7 +525,7 @@ def run_mixed(args):
         "latency_ms_p99": fft["latency_ms_p99"],
         "fft": fft,
         "split": legs["split"],
-        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
+        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
                      "kernel": f"{_xcorr_kernel_name(dom['window'], 'fft')} + fit_kernel ({dom['image']})",
                      "note": "FP32 VALU FFT; peak = MI355X FP32 157.3 TFLOP/s"},
@@ -796,7 +800,7 @@ def main():
         "latency_ms_p99": round(float(np.percentile(steps_ms, 99)), 3),
         "vectors_per_s": round(value * mp.grids[-1].n_win[0] * mp.grids[-1].n_win[1], 1),
         "passes": passes,
-        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
+        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": traffic,
                      "kernel": f"{_xcorr_kernel_name(dom['window'], args.dft)} + fit_kernel",
                      "note": "FP32 VALU FFT; peak = MI355X FP32 157.3 TFLOP/s"},

@@ -91,7 +91,9 @@ float peak_detectability(const float *xcorr, const int *N);
  * pivtools_cli/piv/piv_backend/lib_guard.py:6-26). */
 int pivtools_cpu_supported(void);
 
-/* peak_locate_lm_batch.h:39-51.  Lane width W = peakfit_batch_lanes(). */
+/* peak_locate_lm_batch.h:39-51.  Lane width W = peakfit_batch_lanes() = 16,
+ * the reference's PEAKFIT_MAX_LANES (:26-28): fits min(L_real, 16) planes and
+ * writes exactly 3*W floats to each of peak_loc and std_dev. */
 void lsqpeaklocate_lm_batch(const float *planes, int L_real, const int *N, int iFitType,
                             float *peak_loc, float *std_dev, const float *fPlaneWeight);
 int peakfit_batch_available(void);
@@ -102,8 +104,8 @@ int peakfit_batch_lanes(void);
  * sizes up to 128 (192/256 return 4). */
 unsigned convolve(const float *w1, const float *w2, float *conv, const int *N);
 unsigned xcorr(const float *w1, const float *w2, float *corr, const int *N);
-/* BUILD-SIMD.md verify step: windows per batched call (here, per workgroup
- * for a 32x32 window). */
+/* BUILD-SIMD.md verify step.  Returns 16, equal to peakfit_batch_lanes(),
+ * which the reference's batch gate requires (PIV_2d_cross_correlate.c:231-239). */
 int codelet_lanes(void);
 
 /* ======================================================================= */

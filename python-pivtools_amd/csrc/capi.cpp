@@ -83,7 +83,11 @@ bool sync() { return hipStreamSynchronize(g_ctx.st) == hipSuccess; }
 static volatile int g_kernel_timing = 0;
 static double g_t_fft = 0.0, g_t_fit = 0.0;
 static volatile int g_peakfit_impl = 0;
-static const int kBatchLanes = 64;
+// Lane width of the batch fitter's output arrays.  The reference publishes
+// PEAKFIT_MAX_LANES = 16 as the bound callers size their buffers by
+// (peak_locate_lm_batch.h:26-28, PIV_2d_cross_correlate.c:149), so the
+// drop-in reports 16 and writes exactly 3 x 16 floats per output array.
+static const int kBatchLanes = 16;
 
 extern "C" {
 

@@ -65,3 +65,14 @@ def test_every_device_entry_point_has_argtypes():
     L = _native.core()
     missing = [n for n in _declared() if n.startswith("pivamd_") and getattr(L, n).argtypes is None]
     assert not missing, missing
+
+
+def test_batch_lane_width_within_reference_bound():
+    """peak_locate_lm_batch.h:26-28: PEAKFIT_MAX_LANES = 16 bounds the lane
+    width callers size their output arrays by, and the reference's batch gate
+    requires peakfit_batch_lanes() == codelet_lanes()
+    (PIV_2d_cross_correlate.c:231-239).  Neither call touches the device."""
+    L = _native.bulkxcorr2d_lib()
+    assert 1 <= L.peakfit_batch_lanes() <= 16
+    assert L.peakfit_batch_lanes() == L.codelet_lanes()
+    assert L.peakfit_batch_available() == 1

@@ -240,11 +240,15 @@ def test_lsqpeaklocate_lm_matches_reference_fixtures(ft, n):
 
 
 @pytest.mark.parametrize("ft", [4, 5, 6])
-def test_lsqpeaklocate_lm_batch_matches_reference_batch(ft):
+@pytest.mark.parametrize("n", [16, 32, 64])
+def test_lsqpeaklocate_lm_batch_matches_reference_batch(ft, n):
+    """The batch flavour (the reference's config default) at the scalar bar:
+    identical NaN masks and positions within 1e-4 px of the reference C's
+    own batch fitter (peak_locate_lm_batch.c, fixtures from oracle/_ref)."""
     g = np.load(os.path.join(GOLD, "peakfit_ref.npz"))
     L = _native.bulkxcorr2d_lib()
     W = L.peakfit_batch_lanes()
-    n = 32
+    assert W == 16
     planes, w = g[f"n{n}_planes"], g[f"n{n}_weight"]
     ref = g[f"n{n}_ft{ft}_batch_loc"]
     got = np.zeros((len(planes), 3), np.float32)
@@ -255,9 +259,33 @@ def test_lsqpeaklocate_lm_batch_matches_reference_batch(ft):
         L.lsqpeaklocate_lm_batch(chunk.reshape(-1), len(chunk), np.array([n, n], np.int32), ft, loc, std,
                                  w.ctypes.data)
         got[s:s + len(chunk)] = loc.reshape(3, W)[:, :len(chunk)].T
-    both = ~np.isnan(ref[:, 0]) & ~np.isnan(got[:, 0])
-    assert np.abs(got[both, :2] - ref[both, :2]).max(initial=0) < 1e-3
-    assert np.mean(np.isnan(ref[:, 0]) != np.isnan(got[:, 0])) < 0.05
+    np.testing.assert_array_equal(np.isnan(got[:, 0]), np.isnan(ref[:, 0]))
+    ok = ~np.isnan(ref[:, 0])
+    assert np.abs(got[ok, :2] - ref[ok, :2]).max(initial=0) < 1e-4
+    np.testing.assert_array_equal(got[ok, 2], ref[ok, 2])
+
+
+def test_lsqpeaklocate_lm_batch_writes_exactly_three_lane_widths():
+    """A caller that sizes its outputs by PEAKFIT_MAX_LANES (3 x 16 floats,
+    PIV_2d_cross_correlate.c:149) must find the guard words behind them
+    untouched, also when L_real exceeds the lane width."""
+    g = np.load(os.path.join(GOLD, "peakfit_ref.npz"))
+    L = _native.bulkxcorr2d_lib()
+    W = L.peakfit_batch_lanes()
+    assert W <= 16 and L.codelet_lanes() == W
+    n = 32
+    planes = np.ascontiguousarray(g[f"n{n}_planes"][:W + 4])
+    canary = np.float32(-12345.5)
+    for l_real in (3, W, W + 4):
+        loc = np.full(3 * 16 + 8, canary, np.float32)
+        std = np.full(3 * 16 + 8, canary, np.float32)
+        L.lsqpeaklocate_lm_batch(planes.reshape(-1), l_real, np.array([n, n], np.int32), 5, loc, std,
+                                 g[f"n{n}_weight"].ctypes.data)
+        assert (loc[3 * W:] == canary).all() and (std[3 * W:] == canary).all(), l_real
+        assert not (loc[:3 * W] == canary).any()
+        lanes = loc[:3 * W].reshape(3, W)
+        assert np.isnan(lanes[0, min(l_real, W):]).all()
+        assert (lanes[2, min(l_real, W):] == 0).all()
 
 
 @pytest.mark.parametrize("ft", [3, 4, 5, 6])
