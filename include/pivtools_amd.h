@@ -196,6 +196,22 @@ int pivamd_xcorr(const float *d_imgA, const float *d_imgB, const float *d_mask, 
                  float *d_pkH, float *d_sx, float *d_sy, float *d_sxy, float *d_ratio,
                  float *d_planes, void *d_work, unsigned long work_bytes, void *stream);
 
+/* No counterpart in the reference: pivamd_xcorr without the workspace
+ * argument, for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  It
+ * makes no allocation: the peak records (pivamd_xcorr_workspace() bytes) live
+ * in a file-scope device array of pivamd_xcorr_noalloc_max_bytes() = 64 MiB
+ * per GPU, held until the process ends; a larger call returns PIVAMD_ERR_SIZE
+ * (2) with a message.  All calls on one GPU share that array, so calls that
+ * overlap on one GPU (other streams or threads) race. */
+int pivamd_xcorr_noalloc(const float *d_imgA, const float *d_imgB, const float *d_mask, int H, int W,
+                         int n_images, const float *d_ctrX, const float *d_ctrY, int nWy, int nWx,
+                         const float *d_wA, const float *d_wB, const float *d_planeWeight, int ensemble,
+                         int winY, int winX, int nPeaks, int finder, int impl, float *d_pkX,
+                         float *d_pkY, float *d_pkH, float *d_sx, float *d_sy, float *d_sxy,
+                         float *d_ratio, float *d_planes, void *stream);
+unsigned long pivamd_xcorr_noalloc_max_bytes(void);
+
 /* Device workspace bytes pivamd_xcorr needs (0 in ensemble mode). */
 unsigned long pivamd_xcorr_workspace(int n_images, int nWy, int nWx, int winY, int winX, int nPeaks,
                                      int ensemble);

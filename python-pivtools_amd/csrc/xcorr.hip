@@ -28,6 +28,8 @@
 // contiguous run of windows, which keeps overlapping windows in one L2.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <stdio.h>
+#include <mutex>
 #include <type_traits>
 
 #include "fft_reg.h"
@@ -1751,6 +1753,62 @@ extern "C" int pivamd_xcorr_u16(const unsigned short *d_imgA, const unsigned sho
     return xcorr_entry(nullptr, nullptr, d_imgA, d_imgB, d_mask, H, W, n_images, d_ctrX, d_ctrY, nWy, nWx,
                        d_wA, d_wB, d_planeWeight, ensemble, winY, winX, nPeaks, finder, impl, d_pkX, d_pkY,
                        d_pkH, d_sx, d_sy, d_sxy, d_ratio, d_planes, d_work, work_bytes, st);
+}
+
+// ---- pivamd_xcorr_noalloc: the correlation with no workspace argument -----
+// Callers that cannot allocate device memory (code under HIP graph capture,
+// tight serving loops, harnesses that forbid hipMalloc) get the peak records
+// (and, for nPeaks > 1, the planes the multi-peak kernel reads) from this
+// file-scope device array instead of a caller workspace.  Each GPU the
+// library runs on holds its own copy until the process ends.  Every record a
+// kernel reads is written earlier in the same call (the validity flag of each
+// window included), so nothing in it needs clearing between calls; calls that
+// overlap on one GPU (other streams or threads) share it and race.
+constexpr size_t kNoallocScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static void *g_scratch_dev[kMaxDevices];
+static std::mutex g_scratch_mu;
+
+// Device address of the current device's scratch array, looked up once per
+// device (hipGetSymbolAddress with that device current).  The array's name
+// is only a host-side stand-in: host code passes this address, never the name.
+static void *noalloc_scratch()
+{
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return nullptr;
+        g_scratch_dev[dev] = p;
+    }
+    return g_scratch_dev[dev];
+}
+
+extern "C" size_t pivamd_xcorr_noalloc_max_bytes(void) { return kNoallocScratchBytes; }
+
+extern "C" int pivamd_xcorr_noalloc(const float *d_imgA, const float *d_imgB, const float *d_mask, int H, int W,
+                                    int n_images, const float *d_ctrX, const float *d_ctrY, int nWy, int nWx,
+                                    const float *d_wA, const float *d_wB, const float *d_planeWeight,
+                                    int ensemble, int winY, int winX, int nPeaks, int finder, int impl,
+                                    float *d_pkX, float *d_pkY, float *d_pkH, float *d_sx, float *d_sy,
+                                    float *d_sxy, float *d_ratio, float *d_planes, hipStream_t st)
+{
+    if (n_images < 0 || nWy < 0 || nWx < 0 || nPeaks < 1) return PIVAMD_ERR_ARG;
+    const size_t need = pivamd_xcorr_workspace(n_images, nWy, nWx, winY, winX, nPeaks, ensemble);
+    if (need > kNoallocScratchBytes) {
+        fprintf(stderr, "pivamd_xcorr_noalloc: %zu bytes of peak records for %d images x %d x %d windows "
+                        "exceed the %zu-byte device scratch; split the call or use pivamd_xcorr\n",
+                need, n_images, nWy, nWx, kNoallocScratchBytes);
+        return PIVAMD_ERR_SIZE;
+    }
+    void *work = noalloc_scratch();
+    if (!work) return PIVAMD_ERR_RUNTIME;
+    return xcorr_entry(d_imgA, d_imgB, nullptr, nullptr, d_mask, H, W, n_images, d_ctrX, d_ctrY, nWy, nWx,
+                       d_wA, d_wB, d_planeWeight, ensemble, winY, winX, nPeaks, finder, impl, d_pkX, d_pkY,
+                       d_pkH, d_sx, d_sy, d_sxy, d_ratio, d_planes, work, kNoallocScratchBytes, st);
 }
 
 extern "C" int pivamd_peakfit(const float *d_planes, int n, int ny, int nx, int nPeaks, int finder,

@@ -69,6 +69,10 @@ def core() -> ctypes.CDLL:
         ]
         L.pivamd_xcorr.restype = _i
         L.pivamd_xcorr_u16.argtypes = L.pivamd_xcorr.argtypes
+        L.pivamd_xcorr_noalloc.argtypes = L.pivamd_xcorr.argtypes[:27] + [_vp]
+        L.pivamd_xcorr_noalloc.restype = _i
+        L.pivamd_xcorr_noalloc_max_bytes.argtypes = []
+        L.pivamd_xcorr_noalloc_max_bytes.restype = ctypes.c_size_t
         L.pivamd_xcorr_u16.restype = _i
         L.pivamd_warp.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp, _vp, _i, _i, _i, _vp]
         L.pivamd_warp.restype = _i
