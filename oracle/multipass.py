@@ -493,8 +493,16 @@ def _cubic(x):
 
 
 def remap_cubic(field, map_x, map_y):
-    """Bicubic remap on the 1/32 lattice with replicate border (cv2.remap
-    INTER_CUBIC as published); field (2, N, nyo, nxo) -> (2, N, ny, nx)."""
+    """Bicubic remap with replicate border, field (2, N, nyo, nxo) -> (2, N,
+    ny, nx): cv2.remap(INTER_CUBIC, BORDER_REPLICATE) as published
+    (cpu_instantaneous.py:1050-1071 calls it on float32 maps).  The map is
+    rounded to the 1/32 lattice (X = rint(32 x), ties to even), the stencil
+    starts at (X >> 5) - 1, and the per-axis Keys a = -0.75 weights of the
+    fraction (X & 31) / 32 form the 4 x 4 products wy[a] * wx[b].  A stencil
+    inside the field (0 <= sx <= nxo - 4 and 0 <= sy <= nyo - 4) is summed row
+    by row (four left-to-right row sums added in order); one that reaches a
+    border reads clamped indices and adds its 16 products one at a time in
+    row-major order."""
     C, N, nyo, nxo = field.shape
     ny, nx = map_x.shape
     X = np.rint(map_x.astype(np.float32) * np.float32(32)).astype(np.int64)
@@ -508,8 +516,16 @@ def remap_cubic(field, map_x, map_y):
             rows = np.clip(sy + np.arange(4), 0, nyo - 1)
             cols = np.clip(sx + np.arange(4), 0, nxo - 1)
             w2 = (wy[:, None] * wx[None, :]).astype(np.float32)
-            patch = field[:, :, rows][:, :, :, cols]
-            out[:, :, r, c] = np.sum(np.sum(patch * w2, axis=-1, dtype=np.float32), axis=-1, dtype=np.float32)
+            prod = field[:, :, rows][:, :, :, cols] * w2          # (C, N, 4, 4) float32 products
+            if 0 <= sx <= nxo - 4 and 0 <= sy <= nyo - 4:
+                rs = ((prod[..., 0] + prod[..., 1]) + prod[..., 2]) + prod[..., 3]
+                acc = ((rs[..., 0] + rs[..., 1]) + rs[..., 2]) + rs[..., 3]
+            else:
+                acc = np.zeros((C, N), np.float32)
+                for a in range(4):
+                    for b in range(4):
+                        acc = acc + prod[..., a, b]
+            out[:, :, r, c] = acc
     return out
 
 

@@ -487,7 +487,8 @@ __device__ __forceinline__ void cubic_coeffs(float x, float *w)
 // Bicubic remap of the planar field (2, N, nyo, nxo) at fractional indices
 // (map_y, map_x) of the new grid (ny, nx): coordinates rounded to 1/32 px
 // (round half to even), per-axis cubic weights on that lattice, replicate
-// border.
+// border, with the summation order of cv2.remap(INTER_CUBIC) for stencils
+// inside the field and for those that reach a border.
 __global__ void remap_kernel(const float *in, int N, int nyo, int nxo, const float *map_x, const float *map_y,
                              int ny, int nx, float *out)
 {
@@ -502,6 +503,10 @@ __global__ void remap_kernel(const float *in, int N, int nyo, int nxo, const flo
     float wx[4], wy[4];
     cubic_coeffs((float)(X & 31) * (1.0f / 32.0f), wx);
     cubic_coeffs((float)(Y & 31) * (1.0f / 32.0f), wy);
+    // a stencil inside the field is summed row by row (four row sums added
+    // in order), one reaching a border adds its 16 clamped-index products
+    // one at a time in row-major order
+    const bool inside = sx >= 0 && sy >= 0 && sx <= nxo - 4 && sy <= nyo - 4;
     float sum = 0.0f;
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
@@ -511,9 +516,11 @@ __global__ void remap_kernel(const float *in, int N, int nyo, int nxo, const flo
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
             const int cc = min(max(sx + b, 0), nxo - 1);
-            rs += row[cc] * (wy[a] * wx[b]);
+            const float p = row[cc] * (wy[a] * wx[b]);
+            if (inside) rs += p;
+            else sum += p;
         }
-        sum += rs;
+        if (inside) sum += rs;
     }
     out[i] = sum;
 }

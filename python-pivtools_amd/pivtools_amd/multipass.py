@@ -112,7 +112,7 @@ class MultipassPIV:
                  median=(0.1, 2.0, 5), final_infill="nearest", dft="fft", device="cuda", mid_infill="nearest",
                  vector_masks=None, pixel_mask=None, mask_threshold=0.5, predictor_fill="harmonic",
                  predictor_bcs=None, outlier_methods=None, num_peaks=1, secondary_peak=False,
-                 dump_correlation_planes=False, warp_arithmetic="fma"):
+                 dump_correlation_planes=False, warp_arithmetic="fma", max_batch=None):
         """``vector_masks``: one (n_win_y, n_win_x) bool array per pass (True =
         masked window), or ``pixel_mask``: an (H, W) bool image mask turned
         into them with ``mask_threshold``.  ``predictor_fill``: 'harmonic',
@@ -130,6 +130,12 @@ class MultipassPIV:
         ``dump_correlation_planes``: keep every pass's raw correlation
         planes and peak outputs in ``PassResult.debug_dump`` (the reference's
         debug dump; N*windows*h*w floats per pass).
+        ``max_batch``: when given, every batch-size dependent buffer is
+        allocated once for max_batch pairs and each batch size of at most
+        max_batch uses the leading part of it (so a caller can capture a
+        graph per batch size at the memory cost of one; results of one run
+        are overwritten by the next run of any size); ``footprint_bytes()``
+        gives that memory without allocating it.
         ``num_peaks``: peaks found and fitted per window; with
         ``secondary_peak`` an invalid vector is replaced by its next peak that
         passes the outlier test (cpu_instantaneous.py:664-722).
@@ -224,6 +230,11 @@ class MultipassPIV:
         for bc in self.predictor_bcs:
             if bc.get("edge") not in ("top", "bottom"):
                 raise ValueError(f"predictor BC edge {bc.get('edge')!r}: 'top' or 'bottom'")
+        self.max_batch = None if max_batch is None else int(max_batch)
+        if self.max_batch is not None and self.max_batch < 1:
+            raise ValueError(f"max_batch {max_batch}: at least 1")
+        self._const = None
+        self._pool = None
         self._bufs: list[_PassBuf] | None = None
         self._n = 0
         # buffer sets per batch size, kept for the life of the object: a HIP
@@ -234,23 +245,27 @@ class MultipassPIV:
     def _stream(self):
         return torch.cuda.current_stream(self.device).cuda_stream
 
-    def _alloc(self, n):
-        dev, f32 = self.device, torch.float32
+    def _constants(self):
+        """Per-pass tensors that do not depend on the batch size (window
+        centres, weights, plane weight, masks, predictor maps and
+        smoothing weights), built once and shared by every buffer set."""
+        if self._const is not None:
+            return self._const
+        dev = self.device
         st = self._stream()
-        bufs = []
+        const = []
         for p, g in enumerate(self.grids):
-            b = _PassBuf(g)
-            ny, nx = g.n_win
-            b.ctrs = {k: torch.as_tensor(v, device=dev) for k, v in
+            c = _PassBuf(g)
+            c.ctrs = {k: torch.as_tensor(v, device=dev) for k, v in
                       (("x", g.ctrs_x), ("y", g.ctrs_y), ("xa", g.ctrs_x_all), ("ya", g.ctrs_y_all))}
             w = window_weight(g.win, self.window_type)
-            b.w = torch.as_tensor(w, device=dev)
+            c.w = torch.as_tensor(w, device=dev)
             m = self.masks[p] if self.masks is not None else None
             final = p == len(self.grids) - 1
             if m is not None:
-                b.mask = torch.as_tensor(m.astype(np.float32), device=dev)
+                c.mask = torch.as_tensor(m.astype(np.float32), device=dev)
                 if p in self._fill_host:
-                    b.fill_op = tuple(torch.as_tensor(x, device=dev) for x in self._fill_host[p])
+                    c.fill_op = tuple(torch.as_tensor(x, device=dev) for x in self._fill_host[p])
             if not final:
                 ya = np.asarray(g.ctrs_y_all, np.float64)
                 for bc in self.predictor_bcs:
@@ -259,55 +274,115 @@ class MultipassPIV:
                     else:
                         rows = np.flatnonzero(ya <= float(bc["y_position"]))
                     if rows.size:
-                        b.bc_rows.append((slice(int(rows[0]), int(rows[-1]) + 1), float(bc["uy"]), float(bc["ux"])))
-            b.pw = torch.empty(g.win, dtype=f32, device=dev)
-            _native.check(self.lib.pivamd_plane_weight(_ptr(b.w), g.win[0], g.win[1], _ptr(b.pw), st),
+                        c.bc_rows.append((slice(int(rows[0]), int(rows[-1]) + 1), float(bc["uy"]), float(bc["ux"])))
+            c.pw = torch.empty(g.win, dtype=torch.float32, device=dev)
+            _native.check(self.lib.pivamd_plane_weight(_ptr(c.w), g.win[0], g.win[1], _ptr(c.pw), st),
                           "plane weight")
-            shape = (n, ny, nx)
-            P = self.num_peaks
-            pshape = (n, P, ny, nx)
-            b.out = {k: torch.empty(shape, dtype=f32, device=dev) for k in ("ratio", "pkh_g", "pkh_o")}
-            b.out.update({k: torch.empty(pshape, dtype=f32, device=dev) for k in ("pkx", "pky", "pkh", "sx", "sy",
-                                                                                  "sxy")})
-            b.out["choice"] = torch.empty(shape, dtype=torch.uint8, device=dev)
-            if self.dump_planes:
-                b.out["planes"] = torch.empty((n, ny, nx) + tuple(g.win), dtype=f32, device=dev)
-            if P > 1:
-                b.out.update({k: torch.empty(pshape, dtype=f32, device=dev) for k in ("pxa", "pya", "pha")})
-            if self.secondary:
-                b.out["usel"] = torch.empty((2,) + shape, dtype=f32, device=dev)
-                b.out["pk_sel"] = torch.empty(shape, dtype=f32, device=dev)
-                b.out["active"] = torch.empty(shape, dtype=torch.uint8, device=dev)
-            b.out["u"] = torch.empty((2,) + shape, dtype=f32, device=dev)
-            b.out["u2"] = torch.empty((2,) + shape, dtype=f32, device=dev)
-            b.out["u3"] = torch.empty((2,) + shape, dtype=f32, device=dev)
-            if b.mask is not None and not final and self.predictor_fill != "none":
-                b.out["u4"] = torch.empty((2,) + shape, dtype=f32, device=dev)
-            b.out["nanm"] = torch.empty(shape, dtype=torch.uint8, device=dev)
-            b.out["reason"] = torch.empty(shape, dtype=torch.int8, device=dev)
-            b.out["err"] = torch.zeros(1, dtype=torch.int32, device=dev)
-            if self.outliers["extra"]:
-                b.out["extra"] = torch.empty(shape, dtype=torch.uint8, device=dev)
-                b.out["om_work"] = torch.empty(max(4, self.lib.pivamd_outlier_methods_workspace(n, ny, nx)),
-                                               dtype=torch.uint8, device=dev)
-            nyp, nxp = len(g.ctrs_y_all), len(g.ctrs_x_all)
-            b.out["pad"] = torch.empty((2, n, nyp, nxp), dtype=f32, device=dev)
-            b.out["pad_s"] = torch.empty((2, n, nyp, nxp), dtype=f32, device=dev)
-            b.out["pad_t"] = torch.empty((2, n, nyp, nxp), dtype=f32, device=dev)
-            if (self.final_infill if final else self.mid_infill) in ("harmonic", "biharmonic"):
-                b.out["ls_work"] = torch.empty(max(4, self.lib.pivamd_infill_ls_workspace(n, ny, nx)),
-                                               dtype=torch.uint8, device=dev)
-            ws = self.lib.pivamd_xcorr_workspace(n, ny, nx, g.win[0], g.win[1], P, 0)
-            b.work = torch.empty(max(ws, 4), dtype=torch.uint8, device=dev)
             if p > 0:
-                b.maps = (torch.as_tensor(g.pred_map_x, device=dev), torch.as_tensor(g.pred_map_y, device=dev))
-                b.out["pred"] = torch.empty((2,) + shape, dtype=f32, device=dev)
+                c.maps = (torch.as_tensor(g.pred_map_x, device=dev), torch.as_tensor(g.pred_map_y, device=dev))
                 if self.smoothing and g.sd > 0:
                     trunc = (g.ksize[0] - 1) / (2 * g.sd)
                     wts, rad = gaussian_weights_1d(g.sd, trunc)
-                    b.smooth = (torch.as_tensor(wts, device=dev), rad)
-            bufs.append(b)
-        self.warped = torch.empty((2, n, self.H, self.W), dtype=f32, device=dev)
+                    c.smooth = (torch.as_tensor(wts, device=dev), rad)
+            const.append(c)
+        self._const = const
+        return const
+
+    def _buffer_specs(self, n):
+        """(pass, name, shape, dtype) of every batch-size dependent buffer of
+        an n-pair batch; pass -1 holds the warped frame pair."""
+        f32, u8 = torch.float32, torch.uint8
+        specs = []
+        for p, g in enumerate(self.grids):
+            ny, nx = g.n_win
+            final = p == len(self.grids) - 1
+            m = self.masks[p] if self.masks is not None else None
+            shape = (n, ny, nx)
+            P = self.num_peaks
+            pshape = (n, P, ny, nx)
+            add = lambda name, shp, dt=f32: specs.append((p, name, tuple(int(x) for x in shp), dt))  # noqa: E731
+            for k in ("ratio", "pkh_g", "pkh_o"):
+                add(k, shape)
+            for k in ("pkx", "pky", "pkh", "sx", "sy", "sxy"):
+                add(k, pshape)
+            add("choice", shape, u8)
+            if self.dump_planes:
+                add("planes", (n, ny, nx) + tuple(g.win))
+            if P > 1:
+                for k in ("pxa", "pya", "pha"):
+                    add(k, pshape)
+            if self.secondary:
+                add("usel", (2,) + shape)
+                add("pk_sel", shape)
+                add("active", shape, u8)
+            for k in ("u", "u2", "u3"):
+                add(k, (2,) + shape)
+            if m is not None and m.any() and not final and self.predictor_fill != "none":
+                add("u4", (2,) + shape)
+            add("nanm", shape, u8)
+            add("reason", shape, torch.int8)
+            add("err", (1,), torch.int32)
+            if self.outliers["extra"]:
+                add("extra", shape, u8)
+                add("om_work", (max(4, self.lib.pivamd_outlier_methods_workspace(n, ny, nx)),), u8)
+            nyp, nxp = len(g.ctrs_y_all), len(g.ctrs_x_all)
+            for k in ("pad", "pad_s", "pad_t"):
+                add(k, (2, n, nyp, nxp))
+            if (self.final_infill if final else self.mid_infill) in ("harmonic", "biharmonic"):
+                add("ls_work", (max(4, self.lib.pivamd_infill_ls_workspace(n, ny, nx)),), u8)
+            add("work", (max(4, self.lib.pivamd_xcorr_workspace(n, ny, nx, g.win[0], g.win[1], P, 0)),), u8)
+            if p > 0:
+                add("pred", (2,) + shape)
+        specs.append((-1, "warped", (2, n, self.H, self.W), f32))
+        return specs
+
+    def footprint_bytes(self, n=None) -> int:
+        """Device bytes of the buffers of one n-pair batch (n = max_batch by
+        default), without allocating them; with max_batch set this is all
+        the batch-size dependent memory the object ever holds."""
+        n = self.max_batch if n is None else int(n)
+        if n is None:
+            raise ValueError("give n or construct with max_batch")
+        total = 0
+        for _, _, shape, dt in self._buffer_specs(n):
+            total += int(np.prod(shape)) * torch.empty((), dtype=dt).element_size()
+        return total
+
+    def _alloc(self, n):
+        dev = self.device
+        const = self._constants()
+        if self.max_batch is not None:
+            if n > self.max_batch:
+                raise ValueError(f"batch of {n} pairs exceeds max_batch={self.max_batch}")
+            if self._pool is None:
+                # one flat allocation per buffer at the largest batch; every
+                # batch size takes views of their leading elements
+                self._pool = {(p, k): torch.empty(int(np.prod(shp)), dtype=dt, device=dev)
+                              for p, k, shp, dt in self._buffer_specs(self.max_batch)}
+
+            def new(p, k, shp, dt):
+                flat = self._pool[(p, k)]
+                need = int(np.prod(shp))
+                if need > flat.numel():
+                    raise RuntimeError(f"buffer {k} of pass {p}: {need} > {flat.numel()} elements")
+                return flat[:need].view(shp)
+        else:
+            def new(p, k, shp, dt):
+                return torch.empty(shp, dtype=dt, device=dev)
+        bufs = [_PassBuf(c.grid, ctrs=c.ctrs, w=c.w, pw=c.pw, mask=c.mask, maps=c.maps, smooth=c.smooth,
+                         fill_op=c.fill_op, bc_rows=c.bc_rows) for c in const]
+        warped = None
+        for p, k, shp, dt in self._buffer_specs(n):
+            t = new(p, k, shp, dt)
+            if p < 0:
+                warped = t
+            elif k == "work":
+                bufs[p].work = t
+            else:
+                bufs[p].out[k] = t
+        for b in bufs:
+            b.out["err"].zero_()
+        self.warped = warped
         self._bufs = bufs
         self._n = n
         self._sets[n] = (bufs, self.warped)

@@ -15,6 +15,15 @@ a polling loop:
   backlog gets more batches in flight; slots of different streams and of the
   same stream run concurrently on their own HIP streams.
 
+Admission is by HBM budget.  A slot is charged its real device bytes: its
+16-bit staging frames plus every buffer of its pass loop at ``max_batch``
+pairs (``MultipassPIV.footprint_bytes``; the slot allocates them once and
+every batch size uses their leading part).  A slot is created only when it
+fits; when it does not and a stream waits with no slot at all, idle slots of
+other streams without waiting work are released until it fits.  A stream
+whose single slot exceeds the budget is refused at construction.  Batches
+are never padded: each slot captures one graph per batch size it has run.
+
 A request that arrives while others are on the GPU is admitted as soon as a
 slot of its stream is free.  Batches never mix streams: a stream's image size
 and window schedule fix its buffers and graphs.  Streams flagged
@@ -145,6 +154,13 @@ class PIVServer:
                 self.ens.append(None)
                 self.ens_stream.append(None)
         self.bytes_in_use = 0
+        self.stats = {"batches": 0, "pairs": 0, "padded_pairs": 0, "graphs_captured": 0, "slots_created": 0,
+                      "evicted_slots": 0, "peak_bytes": 0}
+        for i, sp in enumerate(self.specs):
+            need = self._slot_bytes(i, self._slot_piv(i))
+            if need > self.budget:
+                raise ValueError(f"stream {sp.name}: one slot needs {need / 2**30:.2f} GiB of HBM "
+                                 f"(max_batch {self.max_batch}), more than the {self.budget / 2**30:.2f} GiB budget")
         self._rid = 0
         self.completed: list[Request] = []
 
@@ -170,38 +186,60 @@ class PIVServer:
         self.pending[stream].append(r)
         return r
 
+    def _slot_piv(self, i: int) -> MultipassPIV:
+        spec = self.specs[i]
+        return MultipassPIV(spec.shape, spec.window_sizes, spec.overlaps, peak_finder=spec.peak_finder,
+                            device=self.dev, max_batch=self.max_batch, **spec.piv_options)
+
+    def _slot_bytes(self, i: int, mp: MultipassPIV) -> int:
+        H, W = self.specs[i].shape
+        return 2 * self.max_batch * H * W * 2 + mp.footprint_bytes()
+
+    def _evict_for(self, i: int, need: int) -> None:
+        """Release idle slots of other streams that have no waiting pairs
+        (their pass-loop buffers and graphs) until ``need`` more bytes fit."""
+        for j in sorted(range(len(self.specs)), key=lambda j: -len(self.slots[j])):
+            if j == i or self.pending[j]:
+                continue
+            for s in list(self.slots[j]):
+                if self.bytes_in_use + need <= self.budget:
+                    return
+                if s.batch or (s.ens_event is not None and not s.ens_event.query()):
+                    continue
+                self.slots[j].remove(s)
+                self.bytes_in_use -= s.nbytes
+                self.stats["evicted_slots"] += 1
+
     def _new_slot(self, i: int) -> _Slot | None:
         spec = self.specs[i]
         if len(self.slots[i]) >= self.max_slots:
             return None
-        H, W = spec.shape
-        stage = 2 * self.max_batch * H * W * 2
-        est = stage + (2 * self.max_batch * H * W * 4) * 2   # staging + warped frames (+ fields, pass buffers)
-        if self.bytes_in_use + est > self.budget and self.slots[i]:
+        mp = self._slot_piv(i)
+        est = self._slot_bytes(i, mp)
+        if self.bytes_in_use + est > self.budget and not self.slots[i]:
+            self._evict_for(i, est)
+        if self.bytes_in_use + est > self.budget:
             return None
+        H, W = spec.shape
         st = torch.cuda.Stream(device=self.dev)
         with torch.cuda.stream(st):
-            mp = MultipassPIV(spec.shape, spec.window_sizes, spec.overlaps, peak_finder=spec.peak_finder,
-                              device=self.dev, **spec.piv_options)
             sa = torch.zeros((self.max_batch, H, W), dtype=torch.int16, device=self.dev)
             sb = torch.zeros_like(sa)
         slot = _Slot(mp, st, sa, sb)
         slot.nbytes = est
         self.bytes_in_use += est
+        self.stats["peak_bytes"] = max(self.stats["peak_bytes"], self.bytes_in_use)
+        self.stats["slots_created"] += 1
         self.slots[i].append(slot)
         return slot
 
     # -- execution -----------------------------------------------------------
-    def _bucket(self, n: int) -> int:
-        b = 1
-        while b < n:
-            b *= 2
-        return min(b, self.max_batch)
-
     def _launch(self, i: int, slot: _Slot, batch: list[Request]) -> None:
         m = len(batch)
-        n = self._bucket(m)            # graphs exist per power-of-two batch size
+        n = m                          # no padding: one graph per batch size
         now = time.perf_counter()
+        self.stats["batches"] += 1
+        self.stats["pairs"] += m
         with torch.cuda.stream(slot.stream):
             if slot.ens_event is not None:
                 slot.stream.wait_event(slot.ens_event)   # previous ensemble reads of the staging
@@ -209,7 +247,7 @@ class PIVServer:
                 slot.stream.wait_event(r.ready)          # frames produced on the caller's stream
             spec = self.specs[i]
             for k in range(n):
-                r = batch[min(k, m - 1)]      # padding positions repeat the last real pair
+                r = batch[k]
                 if spec.pixel_format == "u16":
                     slot.stage_a[k].copy_(r.a, non_blocking=True)
                     slot.stage_b[k].copy_(r.b, non_blocking=True)
@@ -234,6 +272,7 @@ class PIVServer:
                     res = slot.mp.run(a, b)[-1]
                 slot.graphs[n] = g
                 slot.outs[n] = (res.ux, res.uy, res.nan_mask)
+                self.stats["graphs_captured"] += 1
                 g.replay()
                 out = slot.outs[n]
             else:

@@ -61,6 +61,8 @@ def test_results_alone_equal_results_in_mix(use_graphs):
     n_graphs = sum(len(s.graphs) for slots in srv.slots for s in slots)
     if use_graphs:
         assert n_graphs >= 3
+    assert srv.stats["pairs"] == 30 and srv.stats["padded_pairs"] == 0
+    assert srv.stats["peak_bytes"] <= srv.budget
     alone = [MultipassPIV(s.shape, s.window_sizes, s.overlaps, peak_finder=s.peak_finder) for s in SPECS]
     for r in reqs:
         ux, uy, nm = r.result()
@@ -126,3 +128,78 @@ def test_stream_with_shipped_options_replays_its_graph_exactly():
         assert _same(ux, res.ux[0]) and _same(uy, res.uy[0]) and _same(nm, res.nan_mask[0]), k
     eager.check_infill()
     assert 1 in srv.slots[0][0].graphs
+
+
+def _poisson_serve(srv, plan, rate, seed):
+    """Submit (stream, pair) in plan order at Poisson arrival times of the
+    given mean rate, polling the server in between."""
+    import time
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate, len(plan))
+    reqs = []
+    t_next = time.perf_counter()
+    for (i, a, b), gap in zip(plan, gaps):
+        t_next += gap
+        while time.perf_counter() < t_next:
+            srv.poll()
+        reqs.append(srv.submit(i, a, b))
+        srv.poll()
+    srv.drain()
+    torch.cuda.synchronize()
+    return reqs
+
+
+def test_full_size_streams_alone_equal_mix_under_poisson_arrivals():
+    """The serving workload's stream sizes: a 2160 x 2560 stream with the
+    4-pass 128 -> 16 schedule and a 2048 x 2048 stream with 64 -> 16, pairs
+    submitted at Poisson arrival times (mean 400 pairs/s) while earlier
+    batches run.  Every result is bit-identical to the pair run alone."""
+    specs = [StreamSpec("cam_2560", (2160, 2560), [(128, 128), (64, 64), (32, 32), (16, 16)], [50] * 4),
+             StreamSpec("cam_2048", (2048, 2048), [(64, 64), (32, 32), (16, 16)], [50] * 3)]
+    srv = PIVServer(specs, max_batch=4, max_slots_per_stream=2)
+    plan = []
+    for k in range(10):
+        i = k % 2
+        a, b = S.multipass_pair(seed=9400 + k, shape=specs[i].shape, device=dev)
+        plan.append((i, a, b))
+    print("serving: 10 full-size pairs", flush=True)
+    reqs = _poisson_serve(srv, plan, 400.0, 3)
+    assert len(srv.completed) == 10 and srv.stats["padded_pairs"] == 0
+    alone = [MultipassPIV(s.shape, s.window_sizes, s.overlaps) for s in specs]
+    for r in reqs:
+        ux, uy, nm = r.result()
+        res = alone[r.stream].run(r.a[None], r.b[None])[-1]
+        torch.cuda.synchronize()
+        assert _same(ux, res.ux[0]) and _same(uy, res.uy[0]) and _same(nm, res.nan_mask[0]), (r.rid, r.stream)
+        assert r.latency > 0
+    assert srv.stats["peak_bytes"] <= srv.budget
+
+
+def test_hbm_budget_admission_and_eviction():
+    """A slot is charged its staging plus pass-loop buffers at max_batch;
+    a budget below one slot is refused up front; with room for one slot at a
+    time, a stream that waits with no slot gets the idle slot of the other
+    stream released, and both streams' results stay exact."""
+    specs = [StreamSpec("a", (256, 256), [(64, 64), (32, 32), (16, 16)], [50] * 3),
+             StreamSpec("b", (192, 320), [(32, 32), (16, 16)], [50] * 2)]
+    probe = PIVServer(specs, max_batch=2, max_slots_per_stream=1)
+    sizes = [probe._slot_bytes(i, probe._slot_piv(i)) for i in range(2)]
+    mp = probe._slot_piv(0)
+    assert sizes[0] == 2 * 2 * 256 * 256 * 2 + mp.footprint_bytes(2)
+    with pytest.raises(ValueError):
+        PIVServer(specs, max_batch=2, hbm_budget_bytes=min(sizes) - 1)
+    srv = PIVServer(specs, max_batch=2, max_slots_per_stream=2, hbm_budget_bytes=max(sizes) + min(sizes) // 2)
+    alone = [MultipassPIV(s.shape, s.window_sizes, s.overlaps) for s in specs]
+    for rnd in range(3):
+        for i in (0, 1):
+            reqs = [srv.submit(i, *_pair(specs[i], 9600 + 10 * rnd + 3 * i + k)) for k in range(3)]
+            srv.drain()
+            assert len(srv.slots[i]) == 1 and len(srv.slots[1 - i]) == 0
+            assert srv.bytes_in_use <= srv.budget
+            for r in reqs:
+                ux, uy, nm = r.result()
+                res = alone[i].run(r.a[None], r.b[None])[-1]
+                torch.cuda.synchronize()
+                assert _same(ux, res.ux[0]) and _same(nm, res.nan_mask[0])
+    assert srv.stats["evicted_slots"] >= 5
+    assert srv.stats["peak_bytes"] <= srv.budget

@@ -89,6 +89,86 @@ def test_remap_closed_forms():
     np.testing.assert_array_equal(M.remap_cubic(field, mi, mj)[0, 0, 0], lin[[0, 3, 8], [0, 4, 10]])
 
 
+def _keys64(t, a=-0.75):
+    t = abs(t)
+    if t <= 1:
+        return (a + 2) * t ** 3 - (a + 3) * t ** 2 + 1
+    if t < 2:
+        return a * t ** 3 - 5 * a * t ** 2 + 8 * a * t - 4 * a
+    return 0.0
+
+
+def _remap_fp64_padded(f, x, y, pad=6):
+    """FP64 bicubic value at lattice position (x, y) computed on an
+    explicitly edge-padded copy of f (np.pad mode='edge'): an independent
+    statement of the replicate border, without index clamping."""
+    fp = np.pad(f.astype(np.float64), pad, mode="edge")
+    ix, iy = int(np.floor(x)), int(np.floor(y))
+    v = 0.0
+    for r in range(iy - 1, iy + 3):
+        for c in range(ix - 1, ix + 3):
+            v += _keys64(y - r) * _keys64(x - c) * fp[r + pad, c + pad]
+    return v
+
+
+def test_remap_rounds_to_the_nearest_1_32_lattice_point_ties_to_even():
+    """cv2 converts the float maps to X = cvRound(32 x) (round half to even)
+    and interpolates at X / 32.  Positions a little off the lattice give the
+    lattice point's value bit for bit (a floor rule fails for positions just
+    below a lattice point), and positions on the tie k/32 + 1/64 take the
+    even one of k and k + 1 (a round-half-up rule fails for even k)."""
+    rng = np.random.default_rng(21)
+    ny, nx = 10, 12
+    f = rng.normal(size=(ny, nx)).astype(np.float32)
+    field = f[None, None]
+    ks = np.arange(64, 200)                     # lattice indices 2.0 .. 6.2 px
+    on = (ks / 32.0).astype(np.float32)
+    y0 = np.full_like(on, np.float32(4.25))
+    ref = M.remap_cubic(field, on[None], y0[None])[0, 0, 0]
+    for d in (-1 / 128, -1 / 96, 1 / 96, 1 / 128):
+        off = (ks / 32.0 + d).astype(np.float32)
+        np.testing.assert_array_equal(M.remap_cubic(field, off[None], y0[None])[0, 0, 0], ref)
+    tie = (ks / 32.0 + 1 / 64).astype(np.float32)
+    even = ks + (ks % 2)                        # round half to even
+    want = M.remap_cubic(field, (even / 32.0).astype(np.float32)[None], y0[None])[0, 0, 0]
+    got = M.remap_cubic(field, tie[None], y0[None])[0, 0, 0]
+    np.testing.assert_array_equal(got, want)
+    # the same along y
+    gy = M.remap_cubic(field, np.full_like(tie, np.float32(5.5))[None], tie[None])[0, 0, 0]
+    wy = M.remap_cubic(field, np.full_like(tie, np.float32(5.5))[None], (even / 32.0).astype(np.float32)[None])
+    np.testing.assert_array_equal(gy, wy[0, 0, 0])
+    # and every one of them against FP64 Keys weights at the rounded position
+    for k, v in zip(even[:20], got[:20]):
+        assert abs(v - _remap_fp64_padded(f, k / 32.0, 4.25)) < 2e-5
+
+
+def test_remap_border_is_replicate():
+    """Stencils that reach each border (and positions outside the field)
+    read the edge values: against an FP64 evaluation on an edge-padded copy,
+    which a zero, mirror or off-by-one clamp fails."""
+    rng = np.random.default_rng(22)
+    ny, nx = 9, 11
+    f = rng.normal(size=(ny, nx)).astype(np.float32)
+    pts = [(-0.75, 4.0), (0.3, 4.5), (nx - 1.6, 3.25), (nx - 0.2, 2.0), (nx + 0.9, 6.5),
+           (5.5, -1.4), (4.25, 0.6), (6.0, ny - 1.3), (3.75, ny + 0.4), (-1.2, -1.7), (nx + 0.3, ny + 0.8),
+           (0.0, 0.0), (nx - 1.0, ny - 1.0)]
+    mx = np.array([[p[0] for p in pts]], np.float32)
+    my = np.array([[p[1] for p in pts]], np.float32)
+    out = M.remap_cubic(f[None, None], mx, my)[0, 0, 0]
+    for (x, y), v in zip(pts, out):
+        xl = np.rint(np.float32(x) * 32) / 32.0
+        yl = np.rint(np.float32(y) * 32) / 32.0
+        ref = _remap_fp64_padded(f, xl, yl)
+        assert abs(v - ref) < 2e-5 * max(1.0, abs(ref)), (x, y, v, ref)
+    # far outside: the corner value exactly weighted by a partition of unity
+    far = M.remap_cubic(f[None, None], np.array([[-5.0]], np.float32), np.array([[-5.0]], np.float32))
+    assert abs(far[0, 0, 0, 0] - f[0, 0]) < 1e-6
+    # at integer positions on the border the value is the sample itself
+    edge = M.remap_cubic(f[None, None], np.array([[0.0, nx - 1.0]], np.float32),
+                         np.array([[ny - 1.0, 0.0]], np.float32))[0, 0, 0]
+    np.testing.assert_array_equal(edge, [f[ny - 1, 0], f[0, nx - 1]])
+
+
 def test_gate_reason_invariant():
     rng = np.random.default_rng(2)
     pkx = rng.normal(0, 5, (2, 6, 7)).astype(np.float32)
@@ -100,6 +180,97 @@ def test_gate_reason_invariant():
     assert reason[0, 1, 1] == 1
     u, nm, reason, pk = M.median_stage(u, pk, nm, reason)
     assert np.array_equal(reason != 0, nm)
+
+
+def _smooth_field(ny=9, nx=11, seed=31):
+    rng = np.random.default_rng(seed)
+    pkx = (2.0 + 0.02 * rng.normal(size=(1, ny, nx))).astype(np.float32)
+    pky = (-1.0 + 0.02 * rng.normal(size=(1, ny, nx))).astype(np.float32)
+    pkh = np.ones((1, ny, nx), np.float32)
+    return pkx, pky, pkh
+
+
+MEDIAN = [{"type": "median_2d", "epsilon": 0.1, "threshold": 2.0, "size": 5}]
+
+
+def test_nan_reason_known_answers_and_precedence():
+    """Every single-peak code of nan_reason_codes.py:18-25 where the stage
+    that sets it is known by construction (cpu_instantaneous.py:576-757):
+    masked -1 overrides everything (also a masked large displacement), the
+    fit's NaN sentinel 1, the large-displacement gate 6 (win/4 after pass 0),
+    a primary-peak outlier 10, a NaN peak height 13 on an otherwise valid
+    vector, and a fit failure the median test also flags keeps 1 (the
+    outlier code is written only where no earlier stage claimed the vector)."""
+    pkx, pky, pkh = _smooth_field()
+    mask = np.zeros((9, 11), bool)
+    mask[1, 1] = True
+    pkx[0, 1, 1] = 100.0                 # masked and large: masked wins
+    pkx[0, 2, 2] = np.nan                # fit failure
+    pky[0, 3, 3] = 9.5                   # |dy| > 8: large displacement
+    pkx[0, 4, 5] = 6.0                   # isolated wild vector inside the gate
+    pkh[0, 6, 7] = np.nan                # valid vector, NaN peak height
+    u, nm, reason, ph = M.gate(pkx, pky, pkh, None, mask, 8.0, 8.0)
+    assert reason[0, 2, 2] == 1 and reason[0, 3, 3] == 6 and reason[0, 1, 1] == 6
+    u, nm, reason, ph = M.outlier_stage(u, ph, nm, reason, mask, MEDIAN)
+    want = np.zeros((1, 9, 11), np.int8)
+    want[0, 1, 1] = -1
+    want[0, 2, 2] = 1
+    want[0, 3, 3] = 6
+    want[0, 4, 5] = 10
+    want[0, 6, 7] = 13
+    # the four corners see 4 of their 9 guard cells, fewer than the 6 the
+    # median test requires (outlier_detection.py:20-25, 221-240): outliers too
+    want[0, [0, 0, -1, -1], [0, -1, 0, -1]] = 10
+    np.testing.assert_array_equal(reason, want)
+    np.testing.assert_array_equal(nm, want != 0)
+    assert np.isnan(u[:, nm & ~mask[None]]).all() and (u[:, 0, 1, 1] == 0).all()
+    assert np.isnan(ph[nm]).all() and not np.isnan(ph[~nm]).any()
+
+
+def test_nan_reason_unclassified_and_masked_block():
+    """Code 9: a vector in nan_mask that no stage claimed -- here a NaN
+    predictor with the outlier test off (outlier_methods = []).  And the
+    reference's masked-block test (unit-tests/test_nan_reason.py:97-106): a
+    block of masked windows is exactly -1 and -1 appears nowhere else."""
+    pkx, pky, pkh = _smooth_field()
+    pred = np.zeros((2, 1, 9, 11), np.float32)
+    pred[1, 0, 5, 3] = np.nan
+    mask = np.zeros((9, 11), bool)
+    mask[1:3, 1:4] = True
+    u, nm, reason, ph = M.gate(pkx, pky, pkh, pred, mask, 8.0, 8.0)
+    u, nm, reason, ph = M.outlier_stage(u, ph, nm, reason, mask, [])
+    assert reason[0, 5, 3] == 9
+    np.testing.assert_array_equal(reason[0] == -1, mask)
+    np.testing.assert_array_equal(reason != 0, nm)
+    assert set(np.unique(reason)) == {-1, 0, 9}
+
+
+def test_nan_reason_secondary_peak_codes():
+    """Several peaks with secondary substitution (cpu_instantaneous.py:664-722):
+    a primary outlier whose second peak fits the field is rescued (0); one
+    whose second peak is an outlier too gets 12; one whose second peak is
+    the fit's NaN sentinel gets 13; a primary fit failure with a good second
+    peak is rescued; a masked window stays -1."""
+    pkx1, pky1, pkh1 = _smooth_field()
+    P = 2
+    pkx = np.stack([pkx1, pkx1 + 0.01], axis=1)
+    pky = np.stack([pky1, pky1 - 0.01], axis=1)
+    pkh = np.stack([pkh1, 0.5 * pkh1], axis=1)
+    mask = np.zeros((9, 11), bool)
+    mask[7, 9] = True
+    pkx[0, 0, 2, 3] = 6.5                        # primary outlier, second peak good
+    pkx[0, 0, 4, 5], pkx[0, 1, 4, 5] = 6.5, -4.0  # both outliers
+    pkx[0, 0, 6, 2], pkx[0, 1, 6, 2] = 6.0, np.nan
+    pkx[0, 0, 3, 8] = np.nan                     # primary fit failure, second good
+    u, nm, reason, ph, choice, _ = M.peaks_stage(pkx, pky, pkh, None, mask, 8.0, 8.0, MEDIAN, True)
+    assert reason[0, 2, 3] == 0 and choice[0, 2, 3] == 2 and not nm[0, 2, 3]
+    assert abs(u[1, 0, 2, 3] - pkx[0, 1, 2, 3]) < 1e-6
+    assert reason[0, 4, 5] == 12
+    assert reason[0, 6, 2] == 13
+    assert reason[0, 3, 8] == 0 and choice[0, 3, 8] == 2
+    assert reason[0, 7, 9] == -1
+    np.testing.assert_array_equal(reason != 0, nm)
+    assert P == 2
 
 
 def test_infill_nearest_tie_rule_matches_scipy():
@@ -392,3 +563,22 @@ def test_secondary_peak_substitution_by_hand():
     # without substitution the same windows stay invalid
     u2, nm2, rs2, _, ch2, _ = M.peaks_stage(pkx, pky, pkh, None, m, 4.0, 4.0, [{"type": "median_2d"}], False)
     assert nm2[0, 4, 4] and rs2[0, 4, 4] == 10 and rs2[0, 6, 6] == 6
+
+
+def test_oracle_loop_converges_on_poiseuille_flow():
+    """unit-tests/test_multipass_convergence.py:398-430, the end-to-end pin
+    of the predictor remap and warp: three passes 64 -> 32 -> 16 on the
+    Poiseuille pair recover u_x to a median error below 0.3 px and u_y to
+    below 0.15 px over the interior rows, on every pass after the first."""
+    import poiseuille as P
+
+    a, b = P.pair()
+    grids = build_pass_grids(P.SHAPE, P.SCHEDULE, P.OVERLAP)
+    res = M.run_multipass(a[None].astype(np.float32), b[None].astype(np.float32), grids, finder=6,
+                          outlier_methods=P.OUTLIERS, mid_infill="nearest", final_infill="biharmonic",
+                          keep_passes=True)
+    for p in (1, 2):
+        ux, uy, nm = res[p]["ux"][0], res[p]["uy"][0], res[p]["nan_mask"][0]
+        ex, ey = P.errors(ux, uy, ~nm & np.isfinite(ux))
+        assert ex < 0.3 and ey < 0.15, (p, ex, ey)
+    assert ex < 0.05
