@@ -820,6 +820,8 @@ def _xcorr_kernel_name(n: int, dft: str) -> str:
     v = autotune.get_variant(n) or autotune.default_variant(n)
     if (dft != "fft" and n in (16, 32, 48, 64)) or v == autotune.MFMA:
         return f"xcorr_mfma_kernel<{n}>"
+    if v & autotune.HALF:
+        return f"xcorr_kernel_half<{n},{v & ~autotune.HALF}>"
     if v & autotune.PERSIST:
         return f"xcorr_kernel_pf<{n},{v & ~autotune.PERSIST}>"
     return f"xcorr_kernel<{n},{v}>"

@@ -67,9 +67,11 @@ constexpr bool xc_radix_built(int N, int R)
 // Variant codes of pivamd_xcorr_set_variant: 0 = default, R = FP32 FFT with
 // R lanes per line, kVariantMfma = split-precision MFMA DFT, kVariantPersist
 // + 4 = the persistent prefetching form of the 4-lane FFT (N >= 64, where a
-// workgroup holds one window).
+// workgroup holds one window), kVariantHalf + 4 = the 4-lane FFT through
+// one real LDS plane per window (N = 64, 96, 128: more windows per CU).
 constexpr int kVariantMfma = 16;
 constexpr int kVariantPersist = 32;
+constexpr int kVariantHalf = 64;     // + R: xcorr_kernel_half (one real LDS plane per window)
 
 // Tile layout of the complex stages.  R = 1: element (r, c) at r*(N+1) + c.
 // R > 1: at r*TPX + SKEW*(r/N1) + c + PADC*(c/N1), i.e. rows padded to TPX,
@@ -834,6 +836,174 @@ __device__ __forceinline__ void xc_window(const XcorrArgs &g, float (&zr)[N / R]
 
     plane_epilogue<N, T>(g, Sr, PX, valid, gw, n, w, enorm, t, r0, r1, r2, r3, r4);
 }
+// xc_window with one real LDS plane of C::tile words per window instead of
+// the complex pair (72 KB instead of 144 KB at N = 128).  Every exchange
+// between stages moves the real parts through the plane, then the imaginary
+// parts, with a barrier between, so two 512-lane workgroups (16 waves) share
+// a CU where the complex tile allowed one.  The arithmetic is xc_window's, in
+// the same order; split-line layout only (R > 1, one window per workgroup).
+template <int N, int R>
+__device__ __forceinline__ void xc_window_half(const XcorrArgs &g, float (&zr)[N / R], float (&zi)[N / R],
+                                               float sa, float sb, bool valid, long long gw, int n, int w, int t,
+                                               int l, int q, int ob, int lb, const float2 *tw, float *S, float *r0,
+                                               float *r1, float *r2, float *r3, float *r4)
+{
+    using C = XcCfg<N, R>;
+    constexpr int N1 = C::N1, T = C::T, PX = C::PX;
+    constexpr int numel = N * N;
+    static_assert(R > 1 && C::WPB == 1, "split-line layout, one window per workgroup");
+    win_sum2<T>(sa, sb, r0, r1, t);
+    const float meanA = sa / (float)numel;
+    const float meanB = sb / (float)numel;
+    float ea = 0.0f, eb = 0.0f;
+    if (valid) {
+        if (!g.ensemble) {
+#pragma unroll
+            for (int k = 0; k < N1; ++k) {
+                zr[k] -= meanA;
+                zi[k] -= meanB;
+                ea += zr[k] * zr[k];
+                eb += zi[k] * zi[k];
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < N1; ++k) {
+                ea += zr[k] * zr[k];
+                eb += zi[k] * zi[k];
+            }
+        }
+    }
+    win_sum2<T>(ea, eb, r2, r3, t);
+    const float enorm = 1.0f / sqrtf(ea * eb);
+
+    // ---- 1 -> 2: column spectra to rows, real parts then imaginary parts ----
+    if (valid) {
+        fft_split<N, R, false>(zr, zi, q, tw);
+#pragma unroll
+        for (int k = 0; k < N1; ++k) S[tix<N, R>(k + N1 * ob, ob, l, lb)] = zr[k];
+    }
+    __syncthreads();
+    float xr[N1], xi[N1];
+    if (valid) {
+#pragma unroll
+        for (int k = 0; k < N1; ++k) xr[k] = S[tix<N, R>(l, lb, k * R + q, (k * R) / N1)];
+    }
+    __syncthreads();
+    if (valid) {
+#pragma unroll
+        for (int k = 0; k < N1; ++k) S[tix<N, R>(k + N1 * ob, ob, l, lb)] = zi[k];
+    }
+    __syncthreads();
+    if (valid) {
+#pragma unroll
+        for (int k = 0; k < N1; ++k) xi[k] = S[tix<N, R>(l, lb, k * R + q, (k * R) / N1)];
+        // ---- 2. row ky = l ----
+        fft_split<N, R, false>(xr, xi, q, tw);
+    }
+    __syncthreads();
+
+    // ---- 2 -> 3: column kx = l <= N/2 and its mirror, C = FB conj(FA) -----
+    if (valid) {
+#pragma unroll
+        for (int k = 0; k < N1; ++k) S[tix<N, R>(l, lb, k + N1 * ob, ob)] = xr[k];
+    }
+    __syncthreads();
+    const bool act3 = valid && l <= N / 2;
+    const int mx = (N - l) % N, mxb = mx / N1;
+    auto ez_of = [&](int k) { return tix<N, R>(k * R + q, (k * R) / N1, l, lb); };
+    auto em_of = [&](int k) {
+        const int ky = k * R + q;
+        const int my = k > 0 ? N - ky : (q == 0 ? 0 : N - q);
+        const int myb = k > 0 ? mirror_block<N1, R>(N1 - k, q) : (q == 0 ? 0 : R - 1);
+        return tix<N, R>(my, myb, mx, mxb);
+    };
+    float cr[N1], ci[N1];     // first zR + mR and zR - mR, then the cross spectrum
+    if (act3) {
+#pragma unroll
+        for (int k = 0; k < N1; ++k) {
+            const float zR = S[ez_of(k)], mR = S[em_of(k)];
+            cr[k] = zR + mR;
+            ci[k] = zR - mR;
+        }
+    }
+    __syncthreads();
+    if (valid) {
+#pragma unroll
+        for (int k = 0; k < N1; ++k) S[tix<N, R>(l, lb, k + N1 * ob, ob)] = xi[k];
+    }
+    __syncthreads();
+    if (act3) {
+#pragma unroll
+        for (int k = 0; k < N1; ++k) {
+            const float zI = S[ez_of(k)], mI = S[em_of(k)];
+            // FA = (Z + conj(Zm)) / 2 ; FB = (Z - conj(Zm)) / (2i)
+            const float ar = 0.5f * cr[k], ai = 0.5f * (zI - mI);
+            const float br = 0.5f * (zI + mI), bi = -0.5f * ci[k];
+            cr[k] = br * ar + bi * ai;
+            ci[k] = bi * ar - br * ai;
+        }
+        // ---- 3. inverse column FFT ----
+        fft_split<N, R, true>(cr, ci, q, tw);
+    }
+    __syncthreads();
+
+    // ---- 3 -> 4: rows j and j + N/2 (j = l < N/2): Hermitian rows -> real rows
+    if (act3) {
+#pragma unroll
+        for (int k = 0; k < N1; ++k) S[tix<N, R>(k + N1 * ob, ob, l, lb)] = cr[k];
+    }
+    __syncthreads();
+    const bool act4 = valid && l < N / 2;
+    const int ya = l, yb = l + N / 2;
+    auto kk_of = [&](int k) {
+        const int kx = k * R + q;
+        return 2 * k < N1 ? kx : N - kx;
+    };
+    auto kb_of = [&](int k) { return 2 * k < N1 ? (k * R) / N1 : mirror_block<N1, R>(N1 - k, q); };
+    float g1[N1], g2[N1];     // real parts of rows ya and yb, then the packed row
+    if (act4) {
+#pragma unroll
+        for (int k = 0; k < N1; ++k) {
+            g1[k] = S[tix<N, R>(ya, lb, kk_of(k), kb_of(k))];
+            g2[k] = S[tix<N, R>(yb, lb + R / 2, kk_of(k), kb_of(k))];
+        }
+    }
+    __syncthreads();
+    if (act3) {
+#pragma unroll
+        for (int k = 0; k < N1; ++k) S[tix<N, R>(k + N1 * ob, ob, l, lb)] = ci[k];
+    }
+    __syncthreads();
+    if (act4) {
+#pragma unroll
+        for (int k = 0; k < N1; ++k) {
+            const bool up = 2 * k > N1 || (2 * k == N1 && q > 0);
+            float g1i = S[tix<N, R>(ya, lb, kk_of(k), kb_of(k))];
+            float g2i = S[tix<N, R>(yb, lb + R / 2, kk_of(k), kb_of(k))];
+            if ((k == 0 || 2 * k == N1) && q == 0) { g1i = 0.0f; g2i = 0.0f; }   // kx = 0, N/2
+            if (up) { g1i = -g1i; g2i = -g2i; }     // G(N-k) = conj(G(k)) per real row
+            const float g1r = g1[k], g2r = g2[k];
+            g1[k] = g1r - g2i;
+            g2[k] = g1i + g2r;
+        }
+        // ---- 4. inverse row FFT ----
+        fft_split<N, R, true>(g1, g2, q, tw);
+    }
+    __syncthreads();    // the plane (pitch PX) overlays tile words other lines still read
+    if (act4) {
+        const float mul = 1.0f / (float)numel;
+#pragma unroll
+        for (int k = 0; k < N1; ++k) {
+            const int xs = (k + N1 * ob + N / 2) % N;
+            S[yb * PX + xs] = g1[k] * mul;
+            S[ya * PX + xs] = g2[k] * mul;
+        }
+    }
+    __syncthreads();
+
+    plane_epilogue<N, T>(g, S, PX, valid, gw, n, w, enorm, t, r0, r1, r2, r3, r4);
+}
+
 // Window geometry of global window gw (image n, window w, top-left corner);
 // false when masked, out of the image or past the end.
 template <int N>
@@ -916,6 +1086,62 @@ xcorr_kernel(XcorrArgs g)
     }
     xc_window<N, R>(g, zr, zi, sa, sb, valid, gw, n, w, t, l, q, ob, lb, tw, Sr, Si, r0, r1, r2, r3, r4, s_re,
                     s_im, s_valid, wloc);
+}
+
+// The correlation kernel with xc_window_half: one window per 512-lane
+// workgroup in a single real plane, bounded to 128 VGPRs (four waves per
+// SIMD) so that two workgroups share a CU.
+template <int N, int R_>
+__global__ void __launch_bounds__((XcCfg<N, R_>::BLOCK), (2 * XcCfg<N, R_>::BLOCK / 256))
+xcorr_kernel_half(XcorrArgs g)
+{
+    using C = XcCfg<N, R_>;
+    constexpr int R = C::R, N1 = C::N1;
+    static_assert(C::WPB == 1 && R > 1, "one window per workgroup, split lines");
+    __shared__ float s_pl[C::tile];
+    __shared__ float s_red[5][C::RED];
+    __shared__ __attribute__((aligned(16))) float2 s_tw[R * C::TWP];
+
+    const int b = remap_block((int)blockIdx.x, g.nblocks);
+    if (b < 0) return;    // whole block exits together
+    for (int e = (int)threadIdx.x; e < N; e += C::BLOCK)
+        s_tw[(e / N1) * C::TWP + e % N1] = make_float2(kSplitTw<N, R>.v[2 * e], kSplitTw<N, R>.v[2 * e + 1]);
+    __syncthreads();
+
+    const int t = (int)threadIdx.x;
+    int l, q;
+    C::lane_map(t, l, q);
+    const int ob = out_block<R>(q);
+    const int lb = l / N1;
+    const float2 *tw = s_tw + q * C::TWP;
+    const long long gw = (long long)b;
+    int n, w, row0, col0;
+    const bool valid = xc_geom<N>(g, gw, n, w, row0, col0);
+
+    float zr[N1], zi[N1];
+    float sa = 0.0f, sb = 0.0f;
+    if (valid) {
+        const size_t img = (size_t)n * g.H * g.W + (size_t)(row0 + q) * g.W + col0 + l;
+        const float *wa = g.wA + q * N + l, *wb = g.wB + q * N + l;
+        auto load = [&](const auto *pa, const auto *pb) {
+#pragma unroll
+            for (int k = 0; k < N1; ++k) {
+                const float a = (float)pa[(size_t)k * R * g.W] * wa[k * R * N];
+                const float bb = (float)pb[(size_t)k * R * g.W] * wb[k * R * N];
+                zr[k] = a;
+                zi[k] = bb;
+                sa += a;
+                sb += bb;
+            }
+        };
+        if (g.imgA16) load(g.imgA16 + img, g.imgB16 + img);
+        else load(g.imgA + img, g.imgB + img);
+    } else {
+#pragma unroll
+        for (int k = 0; k < N1; ++k) { zr[k] = 0.0f; zi[k] = 0.0f; }
+    }
+    xc_window_half<N, R>(g, zr, zi, sa, sb, valid, gw, n, w, t, l, q, ob, lb, tw, s_pl, s_red[0], s_red[1],
+                         s_red[2], s_red[3], s_red[4]);
 }
 
 // Persistent form for one window per workgroup (WPB = 1): each workgroup
@@ -1509,6 +1735,23 @@ static int launch_fft(XcorrArgs g, hipStream_t st)
 }
 
 template <int N, int R>
+static int launch_half(XcorrArgs g, hipStream_t st)
+{
+    if constexpr (!xc_radix_built(N, R) || XcCfg<N, R>::WPB != 1 || R == 1) {
+        return PIVAMD_ERR_ARG;
+    } else {
+        using C = XcCfg<N, R>;
+        const long long nb = g.total;
+        if (nb <= 0) return 0;
+        if (nb > (1LL << 30)) return PIVAMD_ERR_ARG;
+        g.nblocks = (int)nb;
+        const int grid = (int)((nb + 7) / 8 * 8);
+        hipLaunchKernelGGL((xcorr_kernel_half<N, R>), dim3(grid), dim3(C::BLOCK), 0, st, g);
+        return hipGetLastError() == hipSuccess ? 0 : PIVAMD_ERR_LAUNCH;
+    }
+}
+
+template <int N, int R>
 static int launch_pf(XcorrArgs g, hipStream_t st)
 {
     if constexpr (!xc_radix_built(N, R) || XcCfg<N, R>::WPB != 1) {
@@ -1549,6 +1792,7 @@ int launch_main<PIVAMD_XCORR_SIZE>(XcorrArgs g, hipStream_t st)
     constexpr int N = PIVAMD_XCORR_SIZE;
     switch (g.variant) {
     case kVariantPersist + 4: return launch_pf<N, 4>(g, st);
+    case kVariantHalf + 4: return launch_half<N, 4>(g, st);
 #if (PIVAMD_XCORR_SIZE % 16 == 0) && (PIVAMD_XCORR_SIZE <= 64)
     case kVariantMfma: return launch_mfma(g, st);
 #endif
@@ -1556,7 +1800,9 @@ int launch_main<PIVAMD_XCORR_SIZE>(XcorrArgs g, hipStream_t st)
     case 2: return launch_fft<N, 2>(g, st);
     case 4: return launch_fft<N, 4>(g, st);
     case 8: return launch_fft<N, 8>(g, st);
-    case 0: return launch_fft<N, xc_radix(N)>(g, st);
+    case 0:
+        if constexpr (N == 64 || N == 128) return launch_half<N, 4>(g, st);   // more windows per CU
+        else return launch_fft<N, xc_radix(N)>(g, st);
     default: return PIVAMD_ERR_ARG;
     }
 }
@@ -1659,12 +1905,13 @@ static bool variant_ok(int N, int v)
     if (v == 0) return true;
     if (v == kVariantMfma) return N % 16 == 0 && N <= 64;
     if (v == kVariantPersist + 4) return xc_radix_built(N, 4) && N * 4 >= 256;   // WPB = 1
+    if (v == kVariantHalf + 4) return N == 64 || N == 96 || N == 128;
     return (v == 1 || v == 2 || v == 4 || v == 8) && xc_radix_built(N, v);
 }
 
 extern "C" int pivamd_xcorr_variants(int N, int *out, int max_out)
 {
-    static const int cand[] = {1, 2, 4, 8, kVariantMfma, kVariantPersist + 4};
+    static const int cand[] = {1, 2, 4, 8, kVariantMfma, kVariantPersist + 4, kVariantHalf + 4};
     int n = 0;
     for (int v : cand)
         if (variant_ok(N, v)) {
@@ -1676,7 +1923,8 @@ extern "C" int pivamd_xcorr_variants(int N, int *out, int max_out)
 
 extern "C" int pivamd_xcorr_default_variant(int N)
 {
-    return pivamd_size_supported(N, N) ? xc_radix(N) : -1;
+    if (!pivamd_size_supported(N, N)) return -1;
+    return (N == 64 || N == 128) ? kVariantHalf + 4 : xc_radix(N);
 }
 
 extern "C" int pivamd_xcorr_set_variant(int N, int v)

@@ -11,6 +11,9 @@ the library (``pivamd_xcorr_variants``):
 * ``36``: the 4-lane FFT as a persistent kernel (64, 96 and 128 points, one
   window per workgroup) that loads the next window's samples while the
   current one is transformed.
+* ``68``: the 4-lane FFT with one real LDS plane per window instead of the
+  complex tile (96 and 128 points): the stage exchanges move real then
+  imaginary parts, and two workgroups fit a CU.
 
 Which one is fastest depends on the device and the size, so the library
 lets the caller pick per size (``pivamd_xcorr_set_variant``, process-wide,
@@ -39,6 +42,7 @@ from .window import compute_window_centers, window_weight
 
 MFMA = 16
 PERSIST = 32
+HALF = 64
 _TARGET_WINDOWS = {8: 262144, 12: 262144, 16: 262144, 24: 196608, 32: 131072, 48: 65536, 64: 49152,
                    96: 20480, 128: 16384}
 
