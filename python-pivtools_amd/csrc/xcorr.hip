@@ -67,8 +67,9 @@ constexpr bool xc_radix_built(int N, int R)
 // Variant codes of pivamd_xcorr_set_variant: 0 = default, R = FP32 FFT with
 // R lanes per line, kVariantMfma = split-precision MFMA DFT, kVariantPersist
 // + 4 = the persistent prefetching form of the 4-lane FFT (N >= 64, where a
-// workgroup holds one window), kVariantHalf + 4 = the 4-lane FFT through
-// one real LDS plane per window (N = 64, 96, 128: more windows per CU).
+// workgroup holds one window), kVariantHalf + R = the R-lane FFT through
+// one real LDS plane per window (R = 4: N = 64, 96, 128; R = 1: N <= 32):
+// more windows per CU.
 constexpr int kVariantMfma = 16;
 constexpr int kVariantPersist = 32;
 constexpr int kVariantHalf = 64;     // + R: xcorr_kernel_half (one real LDS plane per window)
@@ -1004,6 +1005,175 @@ __device__ __forceinline__ void xc_window_half(const XcorrArgs &g, float (&zr)[N
     plane_epilogue<N, T>(g, S, PX, valid, gw, n, w, enorm, t, r0, r1, r2, r3, r4);
 }
 
+// xc_window_half for one lane per line (R = 1) with several windows per
+// workgroup: one real LDS plane per window, real then imaginary parts at
+// every stage exchange, and the inverse stages packed over the workgroup's
+// windows as in xc_window (sv: per-window valid flags; Sb: the workgroup's
+// planes).  Half the LDS of the complex tiles, so twice the workgroups fit
+// a CU by LDS.  Same arithmetic in the same order as xc_window.
+template <int N>
+__device__ __forceinline__ void xc_window_half1(const XcorrArgs &g, float (&zr)[N], float (&zi)[N], float sa,
+                                                float sb, bool valid, long long gw, int n, int w, int t, int l,
+                                                float *S, float *Sb, int *sv, int wloc, float *r0, float *r1,
+                                                float *r2, float *r3, float *r4)
+{
+    using C = XcCfg<N, 1>;
+    constexpr int T = C::T, PX = C::PX, TPX = C::TPX, WPB = C::WPB;
+    constexpr int numel = N * N;
+    static_assert(C::R == 1 && WPB > 1 && TPX == PX, "one lane per line, packed inverse stages");
+    win_sum2<T>(sa, sb, r0, r1, t);
+    const float meanA = sa / (float)numel;
+    const float meanB = sb / (float)numel;
+    float ea = 0.0f, eb = 0.0f;
+    if (valid) {
+        if (!g.ensemble) {
+#pragma unroll
+            for (int k = 0; k < N; ++k) {
+                zr[k] -= meanA;
+                zi[k] -= meanB;
+                ea += zr[k] * zr[k];
+                eb += zi[k] * zi[k];
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < N; ++k) {
+                ea += zr[k] * zr[k];
+                eb += zi[k] * zi[k];
+            }
+        }
+    }
+    win_sum2<T>(ea, eb, r2, r3, t);
+    const float enorm = 1.0f / sqrtf(ea * eb);
+    if (t == 0) sv[wloc] = valid ? 1 : 0;
+
+    // ---- 1 -> 2: column spectra to rows ------------------------------------
+    if (valid) {
+        fft_split<N, 1, false>(zr, zi, 0, nullptr);
+#pragma unroll
+        for (int k = 0; k < N; ++k) S[k * TPX + l] = zr[k];
+    }
+    __syncthreads();
+    float xr[N], xi[N];
+    if (valid) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) xr[k] = S[l * TPX + k];
+    }
+    __syncthreads();
+    if (valid) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) S[k * TPX + l] = zi[k];
+    }
+    __syncthreads();
+    if (valid) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) xi[k] = S[l * TPX + k];
+        fft_split<N, 1, false>(xr, xi, 0, nullptr);      // ---- 2. row ky = l
+    }
+    __syncthreads();
+
+    // ---- 2 -> 3: packed columns kx <= N/2 of every window -------------------
+    if (valid) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) S[l * TPX + k] = xr[k];
+    }
+    constexpr int C3 = N / 2;
+    const int v = (int)threadIdx.x;
+    int wl = -1, col = 0;
+    if (v < WPB * C3) {
+        wl = v / C3;
+        col = v - wl * C3;
+    } else if (v < WPB * C3 + WPB) {
+        wl = v - WPB * C3;
+        col = N / 2;
+    }
+    __syncthreads();
+    const bool act3 = wl >= 0 && sv[wl];
+    float *S3 = Sb + C::wbase(wl < 0 ? 0 : wl);
+    const int mx = (N - col) % N;
+    float cr[N], ci[N];
+    if (act3) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            const int my = (N - k) % N;
+            const float zR = S3[k * TPX + col], mR = S3[my * TPX + mx];
+            cr[k] = zR + mR;
+            ci[k] = zR - mR;
+        }
+    }
+    __syncthreads();
+    if (valid) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) S[l * TPX + k] = xi[k];
+    }
+    __syncthreads();
+    if (act3) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            const int my = (N - k) % N;
+            const float zI = S3[k * TPX + col], mI = S3[my * TPX + mx];
+            // FA = (Z + conj(Zm)) / 2 ; FB = (Z - conj(Zm)) / (2i)
+            const float ar = 0.5f * cr[k], ai = 0.5f * (zI - mI);
+            const float br = 0.5f * (zI + mI), bi = -0.5f * ci[k];
+            cr[k] = br * ar + bi * ai;
+            ci[k] = bi * ar - br * ai;
+        }
+        fft_split<N, 1, true>(cr, ci, 0, nullptr);       // ---- 3. inverse column FFT
+    }
+    __syncthreads();
+
+    // ---- 3 -> 4: packed row pairs j, j + N/2 of every window ----------------
+    if (act3) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) S3[k * TPX + col] = cr[k];
+    }
+    constexpr int C4 = N / 2;
+    const int w4 = v < WPB * C4 ? v / C4 : -1;
+    const int j4 = v - (w4 < 0 ? 0 : w4) * C4;
+    __syncthreads();
+    const bool act4 = w4 >= 0 && sv[w4];
+    float *S4 = Sb + C::wbase(w4 < 0 ? 0 : w4);
+    const int ya = j4, yb = j4 + N / 2;
+    float g1[N], g2[N];
+    if (act4) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            const int kk = 2 * k < N ? k : N - k;
+            g1[k] = S4[ya * TPX + kk];
+            g2[k] = S4[yb * TPX + kk];
+        }
+    }
+    __syncthreads();
+    if (act3) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) S3[k * TPX + col] = ci[k];
+    }
+    __syncthreads();
+    if (act4) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            const int kk = 2 * k < N ? k : N - k;
+            float g1i = S4[ya * TPX + kk], g2i = S4[yb * TPX + kk];
+            if (k == 0 || 2 * k == N) { g1i = 0.0f; g2i = 0.0f; }     // kx = 0, N/2
+            if (2 * k > N) { g1i = -g1i; g2i = -g2i; }                // G(N-k) = conj(G(k))
+            const float g1r = g1[k], g2r = g2[k];
+            g1[k] = g1r - g2i;
+            g2[k] = g1i + g2r;
+        }
+        fft_split<N, 1, true>(g1, g2, 0, nullptr);       // ---- 4. inverse row FFT
+        // the plane rows ya, yb overlay exactly the tile rows this lane read
+        const float mul = 1.0f / (float)numel;
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            const int xs = (k + N / 2) % N;
+            S4[yb * PX + xs] = g1[k] * mul;
+            S4[ya * PX + xs] = g2[k] * mul;
+        }
+    }
+    __syncthreads();
+
+    plane_epilogue<N, T>(g, S, PX, valid, gw, n, w, enorm, t, r0, r1, r2, r3, r4);
+}
+
 // Window geometry of global window gw (image n, window w, top-left corner);
 // false when masked, out of the image or past the end.
 template <int N>
@@ -1142,6 +1312,56 @@ xcorr_kernel_half(XcorrArgs g)
     }
     xc_window_half<N, R>(g, zr, zi, sa, sb, valid, gw, n, w, t, l, q, ob, lb, tw, s_pl, s_red[0], s_red[1],
                          s_red[2], s_red[3], s_red[4]);
+}
+
+// The correlation kernel with xc_window_half1 (one lane per line, several
+// windows per workgroup, one real plane per window).
+template <int N>
+__global__ void __launch_bounds__((XcCfg<N, 1>::BLOCK), 3)
+xcorr_kernel_half1(XcorrArgs g)
+{
+    using C = XcCfg<N, 1>;
+    constexpr int WPB = C::WPB;
+    __shared__ float s_pl[C::tiles];
+    __shared__ float s_red[5][WPB * C::RED];
+    __shared__ int s_valid[WPB];
+
+    const int b = remap_block((int)blockIdx.x, g.nblocks);
+    if (b < 0) return;    // whole block exits together
+    const int wloc = (int)threadIdx.x / N;
+    const int t = (int)threadIdx.x - wloc * N;
+    const int l = t;
+    float *S = s_pl + C::wbase(wloc);
+    float *r0 = &s_red[0][wloc * C::RED], *r1 = &s_red[1][wloc * C::RED];
+    float *r2 = &s_red[2][wloc * C::RED], *r3 = &s_red[3][wloc * C::RED];
+    float *r4 = &s_red[4][wloc * C::RED];
+    const long long gw = (long long)b * WPB + wloc;
+    int n, w, row0, col0;
+    const bool valid = xc_geom<N>(g, gw, n, w, row0, col0);
+
+    float zr[N], zi[N];
+    float sa = 0.0f, sb = 0.0f;
+    if (valid) {
+        const size_t img = (size_t)n * g.H * g.W + (size_t)row0 * g.W + col0 + l;
+        const float *wa = g.wA + l, *wb = g.wB + l;
+        auto load = [&](const auto *pa, const auto *pb) {
+#pragma unroll
+            for (int k = 0; k < N; ++k) {
+                const float a = (float)pa[(size_t)k * g.W] * wa[k * N];
+                const float bb = (float)pb[(size_t)k * g.W] * wb[k * N];
+                zr[k] = a;
+                zi[k] = bb;
+                sa += a;
+                sb += bb;
+            }
+        };
+        if (g.imgA16) load(g.imgA16 + img, g.imgB16 + img);
+        else load(g.imgA + img, g.imgB + img);
+    } else {
+#pragma unroll
+        for (int k = 0; k < N; ++k) { zr[k] = 0.0f; zi[k] = 0.0f; }
+    }
+    xc_window_half1<N>(g, zr, zi, sa, sb, valid, gw, n, w, t, l, S, s_pl, s_valid, wloc, r0, r1, r2, r3, r4);
 }
 
 // Persistent form for one window per workgroup (WPB = 1): each workgroup
@@ -1751,6 +1971,23 @@ static int launch_half(XcorrArgs g, hipStream_t st)
     }
 }
 
+template <int N>
+static int launch_half1(XcorrArgs g, hipStream_t st)
+{
+    if constexpr (!xc_radix_built(N, 1) || XcCfg<N, 1>::WPB < 2) {
+        return PIVAMD_ERR_ARG;
+    } else {
+        using C = XcCfg<N, 1>;
+        const long long nb = (g.total + C::WPB - 1) / C::WPB;
+        if (nb <= 0) return 0;
+        if (nb > (1LL << 30)) return PIVAMD_ERR_ARG;
+        g.nblocks = (int)nb;
+        const int grid = (int)((nb + 7) / 8 * 8);
+        hipLaunchKernelGGL((xcorr_kernel_half1<N>), dim3(grid), dim3(C::BLOCK), 0, st, g);
+        return hipGetLastError() == hipSuccess ? 0 : PIVAMD_ERR_LAUNCH;
+    }
+}
+
 template <int N, int R>
 static int launch_pf(XcorrArgs g, hipStream_t st)
 {
@@ -1793,6 +2030,7 @@ int launch_main<PIVAMD_XCORR_SIZE>(XcorrArgs g, hipStream_t st)
     switch (g.variant) {
     case kVariantPersist + 4: return launch_pf<N, 4>(g, st);
     case kVariantHalf + 4: return launch_half<N, 4>(g, st);
+    case kVariantHalf + 1: return launch_half1<N>(g, st);
 #if (PIVAMD_XCORR_SIZE % 16 == 0) && (PIVAMD_XCORR_SIZE <= 64)
     case kVariantMfma: return launch_mfma(g, st);
 #endif
@@ -1802,6 +2040,7 @@ int launch_main<PIVAMD_XCORR_SIZE>(XcorrArgs g, hipStream_t st)
     case 8: return launch_fft<N, 8>(g, st);
     case 0:
         if constexpr (N == 64 || N == 128) return launch_half<N, 4>(g, st);   // more windows per CU
+        else if constexpr (N == 32) return launch_half1<N>(g, st);
         else return launch_fft<N, xc_radix(N)>(g, st);
     default: return PIVAMD_ERR_ARG;
     }
@@ -1906,12 +2145,13 @@ static bool variant_ok(int N, int v)
     if (v == kVariantMfma) return N % 16 == 0 && N <= 64;
     if (v == kVariantPersist + 4) return xc_radix_built(N, 4) && N * 4 >= 256;   // WPB = 1
     if (v == kVariantHalf + 4) return N == 64 || N == 96 || N == 128;
+    if (v == kVariantHalf + 1) return xc_radix_built(N, 1);
     return (v == 1 || v == 2 || v == 4 || v == 8) && xc_radix_built(N, v);
 }
 
 extern "C" int pivamd_xcorr_variants(int N, int *out, int max_out)
 {
-    static const int cand[] = {1, 2, 4, 8, kVariantMfma, kVariantPersist + 4, kVariantHalf + 4};
+    static const int cand[] = {1, 2, 4, 8, kVariantMfma, kVariantPersist + 4, kVariantHalf + 1, kVariantHalf + 4};
     int n = 0;
     for (int v : cand)
         if (variant_ok(N, v)) {
@@ -1924,7 +2164,7 @@ extern "C" int pivamd_xcorr_variants(int N, int *out, int max_out)
 extern "C" int pivamd_xcorr_default_variant(int N)
 {
     if (!pivamd_size_supported(N, N)) return -1;
-    return (N == 64 || N == 128) ? kVariantHalf + 4 : xc_radix(N);
+    return (N == 64 || N == 128) ? kVariantHalf + 4 : N == 32 ? kVariantHalf + 1 : xc_radix(N);
 }
 
 extern "C" int pivamd_xcorr_set_variant(int N, int v)

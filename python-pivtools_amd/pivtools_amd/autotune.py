@@ -11,9 +11,10 @@ the library (``pivamd_xcorr_variants``):
 * ``36``: the 4-lane FFT as a persistent kernel (64, 96 and 128 points, one
   window per workgroup) that loads the next window's samples while the
   current one is transformed.
-* ``68``: the 4-lane FFT with one real LDS plane per window instead of the
-  complex tile (96 and 128 points): the stage exchanges move real then
-  imaginary parts, and two workgroups fit a CU.
+* ``64 + R``: the R-lane FFT with one real LDS plane per window instead of
+  the complex tile (``68``: 64, 96 and 128 points; ``65``: sizes up to 32
+  points, inverse stages packed over the workgroup's windows): the stage
+  exchanges move real then imaginary parts, so more windows fit a CU.
 
 Which one is fastest depends on the device and the size, so the library
 lets the caller pick per size (``pivamd_xcorr_set_variant``, process-wide,
