@@ -537,6 +537,65 @@ def run_mixed(args):
     print(json.dumps(line), flush=True)
 
 
+def serving_leg(A, B, piv_options, rate, seconds, max_batch, one_pair_rate):
+    """The same workload through the continuous-batching server: two camera
+    streams of 2160 x 2560 pairs with the 128 -> 16 schedule (captured
+    graphs, several slots per stream).  Reports per-request latency at one
+    Poisson offered rate and the saturated rate with every request queued at
+    once; the latter over the one-pair-per-step rate is the serving form's
+    gain over feeding the pass loop one pair at a time."""
+    import numpy as np
+    import torch
+    from pivtools_amd.serving import PIVServer, StreamSpec
+
+    spec = dict(shape=SHAPE, window_sizes=SCHEDULE, overlaps=OVERLAP, piv_options=dict(piv_options))
+    specs = [StreamSpec("cam_a", **spec), StreamSpec("cam_b", **spec)]
+    srv = PIVServer(specs, max_batch=max_batch, max_slots_per_stream=3)
+    rng = np.random.default_rng(777)
+    n_pool = A.shape[0]
+
+    def burst(n):
+        srv.completed.clear()
+        t0 = time.perf_counter()
+        reqs = [srv.submit(k % 2, A[k % n_pool], B[k % n_pool], t_arrival=t0) for k in range(n)]
+        srv.drain()
+        torch.cuda.synchronize()
+        return reqs, max(r.t_done for r in reqs) - t0
+
+    def poisson(r, dur):
+        srv.completed.clear()
+        arr = np.cumsum(rng.exponential(1.0 / r, int(r * dur * 1.5) + 10))
+        arr = arr[arr < dur]
+        reqs, k = [], 0
+        t0 = time.perf_counter()
+        while k < len(arr) or srv.busy():
+            now = time.perf_counter() - t0
+            while k < len(arr) and arr[k] <= now:
+                reqs.append(srv.submit(k % 2, A[k % n_pool], B[k % n_pool], t_arrival=t0 + arr[k]))
+                k += 1
+            srv.poll()
+        torch.cuda.synchronize()
+        lat = np.array([q.latency for q in reqs]) * 1e3
+        return {"offered_pairs_s": r, "pairs": len(reqs),
+                "achieved_pairs_s": round(len(reqs) / (max(q.t_done for q in reqs) - t0), 2),
+                "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3)}
+
+    log("[bench] serving leg: warmup (graph capture per slot and batch size)")
+    for _ in range(3):
+        burst(4 * max_batch)
+    poisson(rate, 0.5)
+    reqs, span = burst(8 * n_pool)
+    sat = len(reqs) / span
+    lat = poisson(rate, seconds)
+    out = {"streams": 2, "shape": list(SHAPE), "schedule": [w[0] for w in SCHEDULE], "max_batch": max_batch,
+           "slots_per_stream": 3, "graphs": True, "saturated_pairs_s": round(sat, 2),
+           "saturated_pairs": len(reqs), "at_rate": lat, "stats": dict(srv.stats)}
+    if one_pair_rate:
+        out["serving_over_one_pair"] = round(sat / one_pair_rate, 3)
+    log(f"[bench] serving leg: {out}")
+    return out
+
+
 def host_fed(mp, A, B, steps: int, fmt: str) -> dict:
     """Pairs/s when the batch starts in pinned host memory: per step, the
     frames (16-bit) or Mono12p payloads cross PCIe on a copy stream, double
@@ -650,6 +709,10 @@ def main():
     ap.add_argument("--serve-rates", type=float, nargs="+", default=[50, 200, 500, 1000, 2000])
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--serve-batch", type=int, default=8)
+    ap.add_argument("--serve-leg-seconds", type=float, default=2.0,
+                    help="multipass workload: seconds of the Poisson serving leg (0 skips the serving leg)")
+    ap.add_argument("--serve-leg-rate", type=float, default=1000.0,
+                    help="multipass workload: offered pairs/s of the serving leg")
     ap.add_argument("--serve-hbm-gb", type=float, default=64.0)
     ap.add_argument("--ens-pairs", type=int, default=2000)
     ap.add_argument("--ens-distinct", type=int, default=64)
@@ -765,6 +828,12 @@ def main():
                "latency_ms_p99": round(float(np.percentile(t1, 99)), 3), "steps": args.one_pair_steps,
                "batched_over_one_pair": round(value / rate, 3)}
         log(f"[bench] one pair per step: {rate:.1f} pairs/s (batched / one-pair = {value / rate:.2f})")
+    serve = None
+    if args.serve_leg_seconds > 0:
+        serve = serving_leg(A, B, {"outlier_methods": methods, "mid_infill": args.infill,
+                                   "final_infill": args.infill,
+                                   "warp_arithmetic": args.warp}, args.serve_leg_rate, args.serve_leg_seconds,
+                            args.serve_batch, one["pairs_per_s"] if one else None)
     fed = None
     if args.host_fed_steps > 0:
         fed = {f: host_fed(mp, A, B, args.host_fed_steps, f) for f in ("u16", "mono12p")}
@@ -807,6 +876,7 @@ def main():
         "cpu_baseline": cpu,
         "accuracy": acc,
         "one_pair_fp32": one,
+        "serving": serve,
         "host_fed": fed,
         "autotune": ({str(n): r for n, r in tuned.items()} if tuned else None),
     }

@@ -366,3 +366,23 @@ def test_convolve_and_xcorr_dropins():
         assert np.linalg.norm(c - r64) / np.linalg.norm(r64) < 1e-5
     assert L.xcorr(np.zeros((20, 20), np.float32), np.zeros((20, 20), np.float32),
                    np.zeros((20, 20), np.float32), np.array([20, 20], np.int32)) == 4
+
+
+def test_plane_weight_cache_follows_the_weight():
+    """bulkxcorr2d recomputes the plane weight only when wB changes: three
+    calls of one size with weights gaussian, top-hat, gaussian each match
+    the oracle for their own weight."""
+    n = 32
+    shape = (160, 192)
+    a, b = _pair(41, shape)
+    wc = compute_window_centers(shape, (n, n), 50)
+    mask = np.zeros((wc.n_win_y, wc.n_win_x), np.float32)
+    for w in (window_weight((n, n), "gaussian"), np.ones((n, n), np.float32),
+              window_weight((n, n), "gaussian")):
+        got = dropin_bulkxcorr2d(a, b, mask, wc.win_ctrs_x, wc.win_ctrs_y, (n, n), w, w, 1, 3)
+        ref = O.bulkxcorr2d(a, b, mask, wc.win_ctrs_x, wc.win_ctrs_y, (n, n), w, w, 1, 3)
+        _check_outputs(got, ref)
+        ok = ~np.isnan(ref["pk_x"])
+        # the widths see the plane weight's curvature across the patch
+        np.testing.assert_allclose(got["sx"][ok], ref["sx"][ok], rtol=1e-4)
+        np.testing.assert_allclose(got["sy"][ok], ref["sy"][ok], rtol=1e-4)
