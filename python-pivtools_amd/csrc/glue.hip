@@ -5,7 +5,7 @@
 // they are small kernels so a whole pass stays on the device:
 //   gate_kernel       displacement gate N/2 (first pass) or N/4, NaN sentinel,
 //                     predictor add-back, nan_mask / nan_reason (:576-635)
-//   median_kernel     normalised-median test, 5x5, vector norm, 3x3 guard
+//   median_kernel     normalised-median test (any odd size), vector norm, 3x3 guard
 //                     (outlier_detection.py:87-243) and the reason codes (:726-757)
 //   infill_kernel     nearest-donor fill in physical distance, ties to the
 //                     smallest column then row, as scipy's EDT resolves them
@@ -288,6 +288,84 @@ __device__ __forceinline__ bool median_outlier(const float *uy, const float *ux,
     return exceeded || present < 6;
 }
 
+// The k-th smallest (0-based) of the values f(0..K-1) that are not NaN,
+// found by counting (O(K^2) compares, nothing stored): v_i is the k-th when
+// fewer than k + 1 values are below it and more than k are at or below it.
+template <typename F>
+__device__ __forceinline__ float kth_finite(F f, int K, int k)
+{
+    float best = __builtin_nanf("");
+    for (int i = 0; i < K; ++i) {
+        const float vi = f(i);
+        if (vi != vi) continue;
+        int lt = 0, le = 0;
+        for (int j = 0; j < K; ++j) {
+            const float vj = f(j);
+            lt += vj < vi ? 1 : 0;
+            le += vj <= vi ? 1 : 0;
+        }
+        if (lt <= k && k < le) best = vi;
+    }
+    return best;
+}
+
+// nanmedian (bottleneck semantics, as nanmedian_reg) of f(0..K-1).
+template <typename F>
+__device__ __forceinline__ float nanmedian_count(F f, int K)
+{
+    int n = 0;
+    for (int i = 0; i < K; ++i) {
+        const float v = f(i);
+        n += v == v ? 1 : 0;
+    }
+    if (n == 0) return __builtin_nanf("");
+    const float lo = kth_finite(f, K, (n - 1) >> 1);
+    if (n & 1) return lo;
+    const float hi = kth_finite(f, K, n >> 1);
+    return 0.5f * (lo + hi);
+}
+
+// The normalised-median test for any odd neighbourhood size (the reference
+// accepts any odd size >= 3, outlier_detection.py:170-171): the same
+// arithmetic as median_outlier<SIZE>, with the neighbours read by index and
+// the medians selected by counting instead of a sorting network.
+__device__ bool median_outlier_any(const float *uy, const float *ux, const float *mask, int r, int c, int ny,
+                                   int nx, float eps, float thr, int size)
+{
+    const int h = size / 2, K = size * size - 1;
+    // neighbour j (row-major, the centre skipped), NaN outside or masked
+    auto nb = [&](int j, const float *f) {
+        const int e = j < K / 2 ? j : j + 1;
+        const int rr = r + e / size - h, cc = c + e % size - h;
+        if (rr < 0 || rr >= ny || cc < 0 || cc >= nx) return __builtin_nanf("");
+        const int q = rr * nx + cc;
+        if (mask && mask[q] != 0.0f) return __builtin_nanf("");
+        return f[q];
+    };
+    const float mu = nanmedian_count([&](int j) { return nb(j, ux); }, K);
+    const float mv = nanmedian_count([&](int j) { return nb(j, uy); }, K);
+    const float x0 = ux[r * nx + c], y0 = uy[r * nx + c];
+    const float ddx = x0 - mu, ddy = y0 - mv;
+    const float r0 = sqrtf(ddx * ddx + ddy * ddy);
+    const float rm = nanmedian_count([&](int j) {
+        const float ex = nb(j, ux) - mu, ey = nb(j, uy) - mv;
+        return sqrtf(ex * ex + ey * ey);
+    }, K);
+    const float nr = r0 / (rm + eps);
+    const bool exceeded = (nr > thr) || !isfinite(nr);
+    int present = 0;
+    for (int dr = -1; dr <= 1; ++dr)
+        for (int dc = -1; dc <= 1; ++dc) {
+            const int rr = r + dr, cc = c + dc;
+            if (rr < 0 || rr >= ny || cc < 0 || cc >= nx) continue;
+            const int q = rr * nx + cc;
+            const bool m = mask && mask[q] != 0.0f;
+            const float a = m ? __builtin_nanf("") : ux[q], b = m ? __builtin_nanf("") : uy[q];
+            if ((isfinite(a) && isfinite(b)) || m) ++present;
+        }
+    return exceeded || present < 6;
+}
+
 // Normalised-median outlier test on u (planar, masked cells read as NaN),
 // then the end-of-stage bookkeeping of cpu_instantaneous.py:726-757.
 // The outlier methods combine by OR (outlier_detection.py:420-500
@@ -316,8 +394,10 @@ __global__ void median_kernel(const float *u, const float *pkh, const float *mas
     if (mode == 2 && !(active[i] && uy[w] == uy[w] && ux[w] == ux[w])) return;
     if (!self_masked) {
         if (use_median)
-            outlier = size == 3 ? median_outlier<3>(uy, ux, mask, r, c, ny, nx, eps, thr)
-                                : median_outlier<5>(uy, ux, mask, r, c, ny, nx, eps, thr);
+            outlier = size == 3   ? median_outlier<3>(uy, ux, mask, r, c, ny, nx, eps, thr)
+                      : size == 5 ? median_outlier<5>(uy, ux, mask, r, c, ny, nx, eps, thr)
+                      : size == 7 ? median_outlier<7>(uy, ux, mask, r, c, ny, nx, eps, thr)
+                                  : median_outlier_any(uy, ux, mask, r, c, ny, nx, eps, thr, size);
         if (use_peak) outlier = outlier || !(hv >= peak_thr);
         if (extra) outlier = outlier || extra[i] != 0;
     }
@@ -572,7 +652,7 @@ extern "C" int pivamd_median(const float *u, const float *pkh, const float *mask
                              float eps, float thr, int size, uint8_t *nanm, int8_t *reason, float *uo,
                              float *pkh_out, hipStream_t st)
 {
-    if (size != 3 && size != 5) return PIVAMD_ERR_ARG;
+    if (size < 3 || size % 2 == 0) return PIVAMD_ERR_ARG;
     const size_t n = (size_t)N * ny * nx;
     if (n == 0) return 0;
     hipLaunchKernelGGL(median_kernel, dim3(nblk(n)), dim3(256), 0, st, u, pkh, mask, N, ny, nx, 1, eps, thr, size,
@@ -586,7 +666,7 @@ extern "C" int pivamd_outlier_stage(const float *u, const float *pkh, const floa
                                     const uint8_t *extra, uint8_t *nanm, int8_t *reason, float *uo,
                                     float *pkh_out, uint8_t *choice, hipStream_t st)
 {
-    if (use_median && size != 3 && size != 5) return PIVAMD_ERR_ARG;
+    if (use_median && (size < 3 || size % 2 == 0)) return PIVAMD_ERR_ARG;
     const size_t n = (size_t)N * ny * nx;
     if (n == 0) return 0;
     hipLaunchKernelGGL(median_kernel, dim3(nblk(n)), dim3(256), 0, st, u, pkh, mask, N, ny, nx, use_median, eps, thr,
@@ -600,7 +680,7 @@ extern "C" int pivamd_outlier_detect(const float *u, const float *pkh, const flo
                                      const uint8_t *extra, const uint8_t *active, uint8_t *nanm, int8_t *reason,
                                      hipStream_t st)
 {
-    if (use_median && size != 3 && size != 5) return PIVAMD_ERR_ARG;
+    if (use_median && (size < 3 || size % 2 == 0)) return PIVAMD_ERR_ARG;
     const size_t n = (size_t)N * ny * nx;
     if (n == 0) return 0;
     hipLaunchKernelGGL(median_kernel, dim3(nblk(n)), dim3(256), 0, st, u, pkh, mask, N, ny, nx, use_median, eps, thr,

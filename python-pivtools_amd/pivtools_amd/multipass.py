@@ -185,8 +185,9 @@ class MultipassPIV:
                                                    float("nan") if vt is None else float(vt)))
                 else:
                     raise ValueError(f"Unknown outlier detection method: {t}")
-        if self.outliers["use_median"] and self.outliers["size"] not in (3, 5):
-            raise NotImplementedError("the device median test has 3x3 and 5x5 neighbourhoods")
+        if self.outliers["use_median"] and (self.outliers["size"] < 3 or self.outliers["size"] % 2 == 0):
+            # outlier_detection.py:170-171
+            raise ValueError(f"size must be an odd integer >= 3, got {self.outliers['size']!r}")
         # infilling.py:698-790 apply_infilling: the device path has the
         # nearest-donor and the least-squares (harmonic, biharmonic) methods
         if final_infill not in ("nearest", "harmonic", "biharmonic", "none"):

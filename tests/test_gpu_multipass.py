@@ -65,6 +65,40 @@ def test_gate_and_median_match_oracle():
     assert np.array_equal(mrs != 0, mnm)
 
 
+@pytest.mark.parametrize("size", [3, 5, 7, 9, 11])
+def test_median_any_odd_size_matches_oracle(size):
+    """outlier_detection.py:170-171 accepts any odd size >= 3: 3, 5 and 7 run
+    sorting networks, larger sizes the counting selection; all equal the
+    oracle's nanmedian test, with NaN holes and a window mask."""
+    L = _lib()
+    rng = np.random.default_rng(40 + size)
+    N, ny, nx = 2, 23, 29
+    u = rng.normal(0.0, 1.0, (2, N, ny, nx)).astype(np.float32)
+    u[1] += 3.0
+    u[:, rng.uniform(size=(N, ny, nx)) < 0.07] = np.nan
+    u[1, 0, 11, 13] += 7.0
+    mask = np.zeros((ny, nx), bool)
+    mask[4:7, 20:24] = True
+    ph = rng.uniform(0.2, 1.0, (N, ny, nx)).astype(np.float32)
+    nm0 = np.isnan(u[0]) | np.isnan(u[1])
+    rs0 = np.where(nm0, 1, 0).astype(np.int8)
+    du, dph, dm = _t(u), _t(ph), _t(mask.astype(np.float32))
+    nm = _t(nm0.astype(np.uint8), torch.uint8)
+    rs = _t(rs0, torch.int8)
+    uo = torch.empty_like(du)
+    ph2 = torch.empty_like(dph)
+    assert L.pivamd_median(du.data_ptr(), dph.data_ptr(), dm.data_ptr(), N, ny, nx, 0.1, 2.0, size,
+                           nm.data_ptr(), rs.data_ptr(), uo.data_ptr(), ph2.data_ptr(), _st()) == 0
+    mu, mnm, mrs, mph = M.median_stage(u, ph, nm0, rs0, mask=mask, size=size)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(nm.cpu().numpy().astype(bool), mnm)
+    np.testing.assert_array_equal(rs.cpu().numpy(), mrs)
+    np.testing.assert_array_equal(uo.cpu().numpy(), mu)
+    assert mnm[0, 11, 13]
+    assert L.pivamd_median(du.data_ptr(), dph.data_ptr(), None, N, ny, nx, 0.1, 2.0, 4, nm.data_ptr(),
+                           rs.data_ptr(), uo.data_ptr(), ph2.data_ptr(), _st()) != 0
+
+
 @pytest.mark.parametrize("spacing", [(16.0, 16.0), (8.0, 16.0)])
 def test_infill_matches_scipy_edt(spacing):
     L = _lib()
@@ -904,3 +938,32 @@ def test_device_loop_converges_on_poiseuille_flow():
     both = np.isfinite(ux) & np.isfinite(rx)
     assert np.median(np.abs(ux[both] - rx[both])) < 0.005
     assert np.mean(nm != ref[-1]["nan_mask"][0]) < 0.02
+
+
+@pytest.mark.parametrize("size", [7, 9])
+def test_loop_with_large_median_neighbourhood_matches_oracle_loop(size):
+    """The pass loop with a 7 x 7 or 9 x 9 median test (gauss3): identical
+    NaN masks and reasons and RMS < 0.005 px against the oracle loop."""
+    shape = (192, 256)
+    sched = [(64, 64), (32, 32), (16, 16)]
+    A, B = [], []
+    for k in range(2):
+        rng = np.random.default_rng(900 + k)
+        x, y, inten = S._positions(rng, shape, 0.03)
+        dx, dy = S.lamb_oseen(x, y, shape, rc=40.0, peak=3.0, uniform=(1.5, 0.0))
+        A.append(S.render(shape, x, y, inten).numpy())
+        B.append(S.render(shape, x + dx, y + dy, inten).numpy())
+    A = np.stack(A).astype(np.float32)
+    B = np.stack(B).astype(np.float32)
+    mp = MultipassPIV(shape, sched, [50, 50, 50], median=(0.1, 2.0, size), warp_arithmetic="exact")
+    res = mp.run(_t(A), _t(B), keep_passes=True)
+    torch.cuda.synchronize()
+    grids = build_pass_grids(shape, sched, [50, 50, 50])
+    ref = M.run_multipass(A, B, grids, median=(0.1, 2.0, size), keep_passes=True)
+    for p, (g, r) in enumerate(zip(res, ref)):
+        gm = g.nan_mask.cpu().numpy().astype(bool)
+        np.testing.assert_array_equal(gm, r["nan_mask"])
+        np.testing.assert_array_equal(g.nan_reason.cpu().numpy(), r["nan_reason"])
+        same = ~gm
+        d = np.hypot(g.ux.cpu().numpy()[same] - r["ux"][same], g.uy.cpu().numpy()[same] - r["uy"][same])
+        assert np.sqrt(np.mean(d ** 2)) < 0.005, (p, d.max())

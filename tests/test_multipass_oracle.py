@@ -31,12 +31,32 @@ def _median_brute(ux, uy, eps, thr, size):
                     if 0 <= rr < ny and 0 <= cc < nx:
                         nbx.append(ux[rr, cc])
                         nby.append(uy[rr, cc])
+            nbx, nby = np.array(nbx), np.array(nby)
+            keep = np.isfinite(nbx) & np.isfinite(nby)
+            nbx, nby = nbx[keep], nby[keep]
             mu, mv = np.median(nbx), np.median(nby)
             r0 = np.hypot(ux[r, c] - mu, uy[r, c] - mv)
-            rm = np.median(np.hypot(np.array(nbx) - mu, np.array(nby) - mv))
-            pres = sum(1 for dr in (-1, 0, 1) for dc in (-1, 0, 1) if 0 <= r + dr < ny and 0 <= c + dc < nx)
+            rm = np.median(np.hypot(nbx - mu, nby - mv))
+            pres = sum(1 for dr in (-1, 0, 1) for dc in (-1, 0, 1) if 0 <= r + dr < ny and 0 <= c + dc < nx
+                       and np.isfinite(ux[r + dr, c + dc]))
             out[r, c] = (r0 / (rm + eps) > thr) or pres < 6
     return out
+
+
+@pytest.mark.parametrize("size", [3, 5, 7, 9])
+def test_median_test_vs_bruteforce_any_size(size):
+    """outlier_detection.py:87-243 accepts any odd size >= 3."""
+    rng = np.random.default_rng(10 + size)
+    ux = rng.normal(3, 0.2, (15, 19)).astype(np.float32)
+    uy = rng.normal(-1, 0.2, (15, 19)).astype(np.float32)
+    ux[rng.uniform(size=ux.shape) < 0.08] = np.nan
+    uy[np.isnan(ux)] = np.nan
+    ux[7, 9] += 5
+    got = M.median_outliers(ux, uy, size=size)
+    ref = _median_brute(ux.astype(np.float64), uy.astype(np.float64), 0.1, 2.0, size)
+    fin = np.isfinite(ux)
+    assert np.array_equal(got[fin], ref[fin])
+    assert got[7, 9]
 
 
 def test_median_test_vs_bruteforce():
