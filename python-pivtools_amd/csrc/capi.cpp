@@ -245,7 +245,8 @@ static unsigned char run_accumulate(const float *A, const float *B, const float 
     const int nWy = nWindows[0], nWx = nWindows[1];
     const int wy = nWindowSize[0], wx = nWindowSize[1];
     const int fy = nFit ? nFit[0] : wy, fx = nFit ? nFit[1] : wx;
-    if (!pivamd_size_supported(wy, wx)) return kErrNomem;
+    // the device ensemble sums are built for square windows
+    if (wy != wx || !pivamd_size_supported(wy, wx)) return kErrNomem;
     if (nWy <= 0 || nWx <= 0) return kErrNone;
     const size_t numel = (size_t)wy * wx, nWin = (size_t)nWy * nWx;
     float sWB = 0.0f, sWAA = 0.0f, sWBB = 0.0f;

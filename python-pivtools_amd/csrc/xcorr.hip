@@ -1558,6 +1558,197 @@ __global__ void __launch_bounds__(256) multipeak_kernel(const float *planes, con
                    s_r[3], s_r[4], out);
 }
 
+// ---- rectangular windows (NY != NX, each a built size) ---------------------
+// The codelet engine binds row length W and column length H independently
+// (pivtools_cli/lib/codelet_fft.c:52-70).  Rectangular windows take this
+// general kernel: one 256-lane workgroup per window, the complex tile
+// z = a + ib in LDS, and each 1D transform stage as direct DFT sums over the
+// line (twiddles from a per-length table computed in FP64 and rounded), every
+// lane accumulating its share of the stage's outputs in registers before the
+// tile is overwritten.  It does O(N) work per output instead of the FFT's
+// O(log N), so it trades speed for generality; the plane it writes (raw,
+// fftshifted, / numel) goes to the multi-peak search and fit.
+constexpr int kRectT = 256;
+constexpr int kRectMaxE = (128 * 96 + kRectT - 1) / kRectT;   // outputs per lane at most
+
+__device__ __forceinline__ void rect_sum2(float &a, float &b, float *red, int t)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        a += __shfl_xor(a, o, 64);
+        b += __shfl_xor(b, o, 64);
+    }
+    if ((t & 63) == 0) {
+        red[t >> 6] = a;
+        red[4 + (t >> 6)] = b;
+    }
+    __syncthreads();
+    a = ((red[0] + red[1]) + red[2]) + red[3];
+    b = ((red[4] + red[5]) + red[6]) + red[7];
+    __syncthreads();
+}
+
+// One in-place stage of 1D DFTs over the tile: along rows (length NX) or
+// columns (length NY); inverse selects the + sign.  real_only: keep only the
+// real part of each output (the last inverse stage).
+__device__ __forceinline__ void rect_stage(float *Sr, float *Si, int NY, int NX, bool rows, bool inverse,
+                                           const float2 *twr, const float2 *twc, int t)
+{
+    const int L = rows ? NX : NY;
+    const float2 *tw = rows ? twr : twc;
+    const int total = NY * NX;
+    const float sg = inverse ? 1.0f : -1.0f;
+    float outr[kRectMaxE], outi[kRectMaxE];
+#pragma unroll
+    for (int m = 0; m < kRectMaxE; ++m) {
+        const int e = t + m * kRectT;
+        float xr = 0.0f, xi = 0.0f;
+        if (e < total) {
+            const int y = e / NX, x = e - y * NX;
+            const int k = rows ? x : y;                 // output frequency on the line
+            const int base = rows ? y * NX : x, step = rows ? 1 : NX;
+            int idx = 0;
+            for (int j = 0; j < L; ++j) {
+                const float ar = Sr[base + j * step], ai = Si[base + j * step];
+                const float c = tw[idx].x, sn = sg * tw[idx].y;
+                xr += ar * c - ai * sn;
+                xi += ar * sn + ai * c;
+                idx += k;
+                if (idx >= L) idx -= L;
+            }
+        }
+        outr[m] = xr;
+        outi[m] = xi;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < kRectMaxE; ++m) {
+        const int e = t + m * kRectT;
+        if (e < total) {
+            Sr[e] = outr[m];
+            Si[e] = outi[m];
+        }
+    }
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(kRectT) xcorr_rect_kernel(XcorrArgs g, int NY, int NX)
+{
+    __shared__ float s_rect[2 * 128 * 96];
+    __shared__ float s_red[8];
+    __shared__ float2 s_twr[128], s_twc[128];
+    const int t = (int)threadIdx.x;
+    const long long gw = blockIdx.x;
+    if (gw >= g.total) return;
+    const int total = NY * NX;
+    float *Sr = s_rect, *Si = s_rect + total;
+    for (int j = t; j < NX; j += kRectT) {
+        double sn, cs;
+        sincospi(2.0 * (double)j / (double)NX, &sn, &cs);
+        s_twr[j] = make_float2((float)cs, (float)sn);
+    }
+    for (int j = t; j < NY; j += kRectT) {
+        double sn, cs;
+        sincospi(2.0 * (double)j / (double)NY, &sn, &cs);
+        s_twc[j] = make_float2((float)cs, (float)sn);
+    }
+    const int nWin = g.nWy * g.nWx;
+    const int n = (int)(gw / nWin), w = (int)(gw - (long long)n * nWin);
+    const int ii = w % g.nWx, jj = w / g.nWx;
+    bool valid = !(g.mask && g.mask[jj * g.nWx + ii] == 1.0f);
+    const int row0 = (int)floor((double)g.ctrY[jj] - ((double)(float)NY - 1.0) / 2.0 + 0.5);
+    const int col0 = (int)floor((double)g.ctrX[ii] - ((double)(float)NX - 1.0) / 2.0 + 0.5);
+    if (row0 < 0 || col0 < 0 || row0 + NY > g.H || col0 + NX > g.W) valid = false;
+    if (!valid) {                       // uniform over the block
+        if (!g.ensemble && t == 0) g.rec[(size_t)gw * kRec + kRecFlag] = 0.0f;
+        return;
+    }
+    // window samples times the weights, A real and B imaginary
+    float sa = 0.0f, sb = 0.0f;
+    const size_t img = (size_t)n * g.H * g.W;
+    for (int e = t; e < total; e += kRectT) {
+        const int y = e / NX, x = e - y * NX;
+        const size_t o = img + (size_t)(row0 + y) * g.W + col0 + x;
+        const float a = (g.imgA16 ? (float)g.imgA16[o] : g.imgA[o]) * g.wA[e];
+        const float b = (g.imgA16 ? (float)g.imgB16[o] : g.imgB[o]) * g.wB[e];
+        Sr[e] = a;
+        Si[e] = b;
+        sa += a;
+        sb += b;
+    }
+    rect_sum2(sa, sb, s_red, t);
+    const float meanA = sa / (float)total, meanB = sb / (float)total;
+    float ea = 0.0f, eb = 0.0f;
+    for (int e = t; e < total; e += kRectT) {
+        float a = Sr[e], b = Si[e];
+        if (!g.ensemble) {
+            a -= meanA;
+            b -= meanB;
+            Sr[e] = a;
+            Si[e] = b;
+        }
+        ea += a * a;
+        eb += b * b;
+    }
+    rect_sum2(ea, eb, s_red, t);
+    const float enorm = 1.0f / sqrtf(ea * eb);
+
+    rect_stage(Sr, Si, NY, NX, true, false, s_twr, s_twc, t);     // rows, forward
+    rect_stage(Sr, Si, NY, NX, false, false, s_twr, s_twc, t);    // columns, forward
+    // C = FB conj(FA), FA = (Z + conj(Zm)) / 2, FB = (Z - conj(Zm)) / (2i)
+    {
+        float cr[kRectMaxE], ci[kRectMaxE];
+#pragma unroll
+        for (int m = 0; m < kRectMaxE; ++m) {
+            const int e = t + m * kRectT;
+            float vr = 0.0f, vi = 0.0f;
+            if (e < total) {
+                const int ky = e / NX, kx = e - ky * NX;
+                const int em = ((NY - ky) % NY) * NX + (NX - kx) % NX;
+                const float zR = Sr[e], zI = Si[e], mR = Sr[em], mI = Si[em];
+                const float ar = 0.5f * (zR + mR), ai = 0.5f * (zI - mI);
+                const float br = 0.5f * (zI + mI), bi = -0.5f * (zR - mR);
+                vr = br * ar + bi * ai;
+                vi = bi * ar - br * ai;
+            }
+            cr[m] = vr;
+            ci[m] = vi;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < kRectMaxE; ++m) {
+            const int e = t + m * kRectT;
+            if (e < total) {
+                Sr[e] = cr[m];
+                Si[e] = ci[m];
+            }
+        }
+        __syncthreads();
+    }
+    rect_stage(Sr, Si, NY, NX, false, true, s_twr, s_twc, t);     // columns, inverse
+    // rows, inverse, real part only: the fftshifted plane / numel
+    float *dst = g.ensemble ? g.planes + (size_t)gw * total : g.work_planes + (size_t)gw * total;
+    const float mul = 1.0f / (float)total;
+    for (int e = t; e < total; e += kRectT) {
+        const int y = e / NX, x = e - y * NX;
+        float v = 0.0f;
+        int idx = 0;
+        for (int j = 0; j < NX; ++j) {
+            v += Sr[y * NX + j] * s_twr[idx].x - Si[y * NX + j] * s_twr[idx].y;
+            idx += x;
+            if (idx >= NX) idx -= NX;
+        }
+        const int o = ((y + NY / 2) % NY) * NX + (x + NX / 2) % NX;
+        dst[o] = v * mul;
+        if (!g.ensemble && g.planes) g.planes[(size_t)gw * total + o] = v * mul;
+    }
+    if (!g.ensemble && t == 0) {
+        float *rec = g.rec + (size_t)gw * kRec;
+        rec[kRecEnorm] = enorm;
+        rec[kRecFlag] = 1.0f;
+    }
+}
+
 // numel / (wB (*) wB) on the centre crop of the 2N zero-padded correlation
 // (xcorr.c:22-89, PIV_2d_cross_correlate.c:219-222), summed in FP64.
 __global__ void plane_weight_kernel(const float *wB, int NY, int NX, float *out)
@@ -2095,13 +2286,32 @@ static int launch_xcorr(int N, XcorrArgs g, hipStream_t st)
 
 using namespace pivamd;
 
+static bool built_axis(int n)
+{
+    return n == 8 || n == 12 || n == 16 || n == 24 || n == 32 || n == 48 || n == 64 || n == 96 || n == 128;
+}
+
+// Square windows of a built size run the FFT kernels; rectangular ones whose
+// row and column lengths are both built sizes run xcorr_rect_kernel (its LDS
+// tile holds up to 128 x 96).
 extern "C" int pivamd_size_supported(int ny, int nx)
 {
-#define PIVAMD_SQ(N) if (ny == N && nx == N) return 1;
-    PIVAMD_SQ(8) PIVAMD_SQ(12) PIVAMD_SQ(16) PIVAMD_SQ(24) PIVAMD_SQ(32)
-    PIVAMD_SQ(48) PIVAMD_SQ(64) PIVAMD_SQ(96) PIVAMD_SQ(128)
-#undef PIVAMD_SQ
-    return 0;
+    if (!built_axis(ny) || !built_axis(nx)) return 0;
+    return (ny == nx || ny * nx <= 128 * 96) ? 1 : 0;
+}
+
+// Correlation of rectangular windows: xcorr_rect_kernel, then the search
+// and fit of every plane (multipeak_kernel, one peak or several).
+static int launch_rect(int NY, int NX, XcorrArgs g, hipStream_t st)
+{
+    if (g.total <= 0) return 0;
+    if (g.total > (1LL << 31) - 1) return PIVAMD_ERR_ARG;
+    hipLaunchKernelGGL(xcorr_rect_kernel, dim3((unsigned)g.total), dim3(kRectT), 0, st, g, NY, NX);
+    if (hipGetLastError() != hipSuccess) return PIVAMD_ERR_LAUNCH;
+    if (g.ensemble) return 0;
+    hipLaunchKernelGGL(multipeak_kernel, dim3((unsigned)g.total), dim3(256), (size_t)NY * NX * sizeof(float), st,
+                       g.work_planes, g.rec, g.total, g.nWy * g.nWx, NY, NX, g.nPeaks, g.finder, g.pw, g);
+    return hipGetLastError() == hipSuccess ? 0 : PIVAMD_ERR_LAUNCH;
 }
 
 extern "C" int pivamd_convolve(const float *d_a, const float *d_b, int ny, int nx, float *d_out,
@@ -2126,7 +2336,7 @@ extern "C" size_t pivamd_xcorr_workspace(int n_images, int nWy, int nWx, int win
     if (ensemble) return 0;
     const size_t total = (size_t)n_images * nWy * nWx;
     size_t bytes = total * kRec * sizeof(float);
-    if (nPeaks > 1) bytes += total * (size_t)winY * winX * sizeof(float);
+    if (nPeaks > 1 || winY != winX) bytes += total * (size_t)winY * winX * sizeof(float);
     return bytes;
 }
 
@@ -2206,7 +2416,8 @@ static int xcorr_entry(const float *d_imgA, const float *d_imgB, const unsigned 
     g.pkX = d_pkX; g.pkY = d_pkY; g.pkH = d_pkH; g.sx = d_sx; g.sy = d_sy; g.sxy = d_sxy;
     g.ratio = d_ratio; g.planes = d_planes;
     g.rec = (float *)d_work;
-    g.work_planes = nPeaks > 1 && d_work ? (float *)d_work + (size_t)n_images * nWy * nWx * kRec : nullptr;
+    g.work_planes = (nPeaks > 1 || winY != winX) && d_work ? (float *)d_work + (size_t)n_images * nWy * nWx * kRec
+                                                            : nullptr;
     g.total = (long long)n_images * nWy * nWx;
     g.H = H; g.W = W; g.nWy = nWy; g.nWx = nWx; g.nPeaks = nPeaks; g.finder = finder;
     g.impl = impl; g.ensemble = ensemble; g.nblocks = 0;
@@ -2214,7 +2425,8 @@ static int xcorr_entry(const float *d_imgA, const float *d_imgB, const unsigned 
     g.variant = g.dft ? kVariantMfma : (winY == winX && winY > 0 && winY <= 128 ? g_variant[winY] : 0);
     if (!ensemble && (!d_pkX || !d_pkY || !d_pkH || !d_sx || !d_sy || !d_sxy || !d_ratio || !d_work))
         return PIVAMD_ERR_ARG;
-    if (winY != winX || !pivamd_size_supported(winY, winX)) return PIVAMD_ERR_SIZE;
+    if (!pivamd_size_supported(winY, winX)) return PIVAMD_ERR_SIZE;
+    if (winY != winX) return launch_rect(winY, winX, g, st);
     return launch_xcorr(winY, g, st);
 }
 

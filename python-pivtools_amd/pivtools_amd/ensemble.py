@@ -62,8 +62,9 @@ class EnsemblePIV:
         if per_pair_normalization and bg_method != "window_mean":
             raise ValueError("per-pair normalisation requires the window_mean background method")
         L = _native.core()
-        if not L.pivamd_size_supported(int(window_size[0]), int(window_size[1])):
-            raise ValueError(f"window {window_size} has no device kernel")
+        if (int(window_size[0]) != int(window_size[1]) or
+                not L.pivamd_size_supported(int(window_size[0]), int(window_size[1]))):
+            raise ValueError(f"window {window_size} has no device ensemble kernel (square built sizes)")
         self.L = L
         self.dev = torch.device(device) if device is not None else torch.device("cuda")
         self.H, self.W = (int(image_shape[0]), int(image_shape[1]))

@@ -56,6 +56,10 @@ def test_reference_style_bindings_load():
     assert _native.kspacefit_lib().kspace_lm_fit_batch.restype is ctypes.c_int
     assert _native.core().pivamd_size_supported(32, 32) == 1
     assert _native.core().pivamd_size_supported(20, 20) == 0
+    assert _native.core().pivamd_size_supported(32, 64) == 1        # rectangular, both axes built
+    assert _native.core().pivamd_size_supported(128, 128) == 1
+    assert _native.core().pivamd_size_supported(128, 96) == 1
+    assert _native.core().pivamd_size_supported(48, 20) == 0
 
 
 def test_every_device_entry_point_has_argtypes():

@@ -386,3 +386,63 @@ def test_plane_weight_cache_follows_the_weight():
         # the widths see the plane weight's curvature across the patch
         np.testing.assert_allclose(got["sx"][ok], ref["sx"][ok], rtol=1e-4)
         np.testing.assert_allclose(got["sy"][ok], ref["sy"][ok], rtol=1e-4)
+
+
+RECT = [(32, 64), (64, 32), (16, 48), (128, 96), (24, 8), (96, 128)]
+
+
+@pytest.mark.parametrize("win", RECT)
+def test_bulkxcorr2d_rectangular_windows(win):
+    """Rectangular windows (row length W and column length H bound
+    independently, codelet_fft.c:52-70): identical NaN masks and integer
+    peaks, planes within 1e-5 relative L2 of the oracle and 5e-6 of FP64,
+    displacements within 0.005 px; a masked window is left untouched."""
+    shape = (3 * win[0] + 11, 3 * win[1] + 7)
+    a, b = _pair(sum(win), shape, shift=(1.3, -0.7))
+    wc = compute_window_centers(shape, win, 50)
+    w = window_weight(win, "gaussian")
+    mask = np.zeros((wc.n_win_y, wc.n_win_x), np.float32)
+    mask[0, 1] = 1.0
+    got = dropin_bulkxcorr2d(a, b, mask, wc.win_ctrs_x, wc.win_ctrs_y, win, w, w, 1, 3, want_planes=True,
+                             prefill=-3.0)
+    ref = O.bulkxcorr2d(a, b, mask, wc.win_ctrs_x, wc.win_ctrs_y, win, w, w, 1, 3, want_planes=True)
+    assert got["err"] == 0
+    assert (got["pk_x"][0, 0, 0, 1] == -3.0) and (got["planes"][0, 0, 1] == 0).all()
+    got["pk_x"][0, 0, 0, 1] = ref["pk_x"][0, 0, 0, 1]
+    for k in ("pk_y", "pk_h", "sx", "sy", "sxy"):
+        got[k][0, 0, 0, 1] = ref[k][0, 0, 0, 1]
+    got["ratio"][0, 0, 1] = ref["ratio"][0, 0, 1]
+    _check_outputs(got, ref)
+    gi = np.argmax(got["planes"].reshape(-1, win[0] * win[1]), axis=1)
+    ri = np.argmax(ref["planes"].reshape(-1, win[0] * win[1]), axis=1)
+    live = mask.reshape(-1) == 0
+    assert np.array_equal(gi[live], ri[live])
+    p64, _ = O.planes_fp64(a, b, wc.win_ctrs_x, wc.win_ctrs_y, win, w.astype(np.float64), w.astype(np.float64))
+    g = got["planes"].reshape(-1, *win).astype(np.float64)[live]
+    assert np.linalg.norm(g - p64[live]) / np.linalg.norm(p64[live]) < 5e-6
+
+
+@pytest.mark.parametrize("finder,n_peaks", [(6, 1), (5, 1), (3, 3)])
+def test_bulkxcorr2d_rectangular_fits_and_peaks(finder, n_peaks):
+    """LM fits and several peaks on rectangular windows: the device fit of
+    the device plane equals the oracle fit of that plane (identical NaN
+    masks, 1e-4 px); end to end within 0.005 px of the oracle."""
+    win = (32, 64)
+    shape = (140, 260)
+    a, b = _pair(77, shape, shift=(0.6, 2.2))
+    wc = compute_window_centers(shape, win, 50)
+    w = window_weight(win, "gaussian")
+    mask = np.zeros((wc.n_win_y, wc.n_win_x), np.float32)
+    got = dropin_bulkxcorr2d(a, b, mask, wc.win_ctrs_x, wc.win_ctrs_y, win, w, w, n_peaks, finder, want_planes=True)
+    ref = O.bulkxcorr2d(a, b, mask, wc.win_ctrs_x, wc.win_ctrs_y, win, w, w, n_peaks, finder, want_planes=True)
+    assert got["err"] == 0
+    loc = _refit_on_planes(got["planes"], w, finder, n_peaks)
+    gx = got["pk_x"][:, 0].ravel()
+    rx = loc[:, n_peaks] - win[1] / 2.0     # column of the first peak
+    assert np.array_equal(np.isnan(gx), np.isnan(rx))
+    ok = ~np.isnan(rx)
+    assert np.abs(gx[ok] - rx[ok]).max(initial=0) < 1e-4
+    ex = ref["pk_x"][:, 0].ravel()
+    both = ~np.isnan(ex) & ~np.isnan(gx)
+    assert np.abs(gx[both] - ex[both]).max(initial=0) < 0.005
+    assert np.mean(np.isnan(ex) != np.isnan(gx)) < 0.06
