@@ -55,7 +55,12 @@ def ensemble_pass(A, B, ctrs_x, ctrs_y, win, weight, bg_method="correlation", en
     cy, cx = h // 2, w // 2
     nfac = np.sqrt(np.maximum(R[0][:, cy, cx] * R[1][:, cy, cx], np.float32(1e-12)))[:, None, None]
     planes = [(r / nfac).astype(np.float64) for r in R]
-    pw = None if p_centred is None else np.tile(np.asarray(p_centred, np.float64).ravel(), nwin)
+    if p_centred is None:
+        pw = None
+    elif np.ndim(p_centred) == 2 and np.shape(p_centred)[0] == nwin and np.shape(p_centred)[1] == h * w:
+        pw = np.asarray(p_centred, np.float64).ravel()          # one floor shape per window
+    else:
+        pw = np.tile(np.asarray(p_centred, np.float64).ravel(), nwin)
     ret, o = O.kspace_fit(planes[0], planes[1], planes[2], mask, win, p_win=pw, kx4=kx4, ky4=ky4)
     if ret != 0:
         raise RuntimeError(f"oracle kspace fit failed ({ret})")
@@ -90,3 +95,54 @@ def ensemble_pass(A, B, ctrs_x, ctrs_y, win, weight, bg_method="correlation", en
     return {"ux": ux.reshape(shp), "uy": uy.reshape(shp), "UU": uu.reshape(shp), "VV": vv.reshape(shp),
             "UV": uv.reshape(shp), "peakheight": peak.reshape(shp), "nan_reason": reason.reshape(shp),
             "planes": planes, "fit": o}
+
+
+def ensemble_multipass(A, B, grids, weight_type="square", bg_method="correlation", floor_fn=None,
+                       rounding=False, smoothing=False, mid_infill="nearest", final_infill="biharmonic",
+                       interp_mode=0):
+    """Multi-pass ensemble (cpu_ensemble.py:912-960,1445-1660,
+    single_pass_accumulator.py:820-890,1146-1400): pass p > 0 warps every
+    pair with the previous pass's infilled field as one shared predictor
+    (edge-padded onto the previous padded centres, optionally smoothed and
+    rounded to even integers), correlates the warped pairs on its own grid,
+    and adds the cubic remap of the predictor at its window centres back to
+    the fitted displacements.  floor_fn(pred_win or None, win) gives the
+    coloured floor shape (None: flat)."""
+    import multipass as M
+    from pivtools_amd.window import window_weight
+
+    A = np.ascontiguousarray(A, np.float32)
+    B = np.ascontiguousarray(B, np.float32)
+    out, field = [], None
+    for p, g in enumerate(grids):
+        Ap, Bp, win = A, B, None
+        if p > 0:
+            gp = grids[p - 1]
+            pad = M.pad_edge(field, gp.n_pre, gp.n_post)
+            if smoothing and g.sd > 0:
+                pad = M.smooth(pad, g.sd, g.ksize)
+            if rounding:
+                pad = (2.0 * np.round(pad / 2.0)).astype(np.float32)
+            win = M.remap_cubic(pad, g.pred_map_x, g.pred_map_y)[:, 0]
+            if rounding:
+                win = (2.0 * np.round(win / 2.0)).astype(np.float32)
+            Ap, Bp = O.fused_warp_batch(A, B, pad[0, 0], pad[1, 0], np.asarray(gp.ctrs_y_all, np.float32),
+                                        np.asarray(gp.ctrs_x_all, np.float32), interp_mode, True, rounding)
+        w = window_weight(g.win, weight_type)
+        pc = None if floor_fn is None else floor_fn(win, g.win)
+        r = ensemble_pass(Ap, Bp, g.ctrs_x, g.ctrs_y, g.win, w, bg_method=bg_method, p_centred=pc)
+        if win is not None:
+            r["ux"] = (r["ux"] + win[1]).astype(np.float32)
+            r["uy"] = (r["uy"] + win[0]).astype(np.float32)
+        r["predictor"] = win
+        u = np.stack([r["uy"], r["ux"]])[:, None].astype(np.float32)
+        method = final_infill if p == len(grids) - 1 else mid_infill
+        if method != "none" and np.isnan(u).any():
+            if method == "nearest":
+                u = M.infill_nearest(u, g.spacing).astype(np.float32)
+            else:
+                u = M.infill_ls(u, g.spacing, 1 if method == "harmonic" else 2).astype(np.float32)
+            r["uy"], r["ux"] = u[0, 0], u[1, 0]
+        field = u
+        out.append(r)
+    return out

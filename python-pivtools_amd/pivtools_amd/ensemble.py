@@ -14,8 +14,16 @@ Follows the reference's streaming ensemble pass:
 Every step is a HIP kernel of libpivtools_amd.so on torch's current stream;
 only the floor shape P is host work (one small analytic model per pass).
 The plane sums stay in HBM for the whole ensemble; the fitter inputs are
-FP64.  Multi-pass ensembles (predictor warp between passes) reuse the
-instantaneous warp and are not wired here.
+FP64.
+
+``EnsembleMultipass`` chains passes as the reference's ensemble does
+(``cpu_ensemble.py:912-960,1445-1660``, ``single_pass_accumulator.py:
+820-890,1146-1400``): pass p > 0 warps every pair with the previous pass's
+field as one shared predictor (edge-padded, optionally smoothed and rounded
+to even integers), accumulates the warped pairs on its own grid, takes the
+per-window half-shift fractions of the remapped predictor into the coloured
+floor, adds the predictor back to the fitted displacements, and infills
+failed fits before they feed the next pass.
 """
 
 from __future__ import annotations
@@ -26,7 +34,7 @@ import numpy as np
 import torch
 
 from . import _native, floor_psd
-from .window import compute_window_centers, window_weight
+from .window import build_pass_grids, compute_window_centers, window_weight
 
 BG_METHODS = {"correlation", "window_mean", "correlation+window_mean", "correlation+dc_zero", "none"}
 
@@ -101,6 +109,18 @@ class EnsemblePIV:
         self.work = torch.empty(max(wb, 8), dtype=torch.uint8, device=d)
         self._f32 = [None, None]
         self.n_images = 0
+        self.predictor = None      # (2, ny, nx) [dy, dx] at the window centres (pass > 0)
+
+    def set_predictor(self, pred: torch.Tensor) -> None:
+        """The predictor the pairs of this pass were warped with, at this
+        pass's window centres (planar (2, ny, nx), dy then dx): it sets the
+        coloured floor's per-window half-shift fractions and is added back to
+        the fitted displacements (single_pass_accumulator.py:834-878,
+        1146-1162)."""
+        pred = torch.as_tensor(pred, device=self.dev, dtype=torch.float32).reshape(2, self.ny, self.nx)
+        if not bool(torch.isfinite(pred).all()):
+            raise ValueError("non-finite predictor: cannot derive the coloured-floor fractions")
+        self.predictor = pred.contiguous()
 
     # ------------------------------------------------------------------
     def _stream(self):
@@ -145,15 +165,25 @@ class EnsemblePIV:
 
     # ------------------------------------------------------------------
     def floor_shape(self):
-        """Per-pass floor shape P in the centred layout (one shared shape:
-        a single pass has no predictor, so every window sits at f = 0)."""
+        """Floor shape P in the centred layout: one shared (h, w) shape when
+        every window sits at half-shift fraction 0 (no predictor, or an
+        even-integer one), else one shape per window (n, h * w) from the
+        fractions frac(pred / 2) of the predictor at the window centres."""
         corr = self.win
         env_auto = floor_psd.pair_envelope(self.w_host, self.w_host, corr)
         env_for_floor = env_auto if self.envelope_divide else np.ones_like(env_auto)
+        if self.predictor is None:
+            fx = fy = np.zeros(1)
+        else:
+            pr = self.predictor.double().cpu().numpy()
+            fx = floor_psd.frac_distance(pr[1].ravel() / 2.0)
+            fy = floor_psd.frac_distance(pr[0].ravel() / 2.0)
+        n_fracs = 2 if max(fx.max(), fy.max()) == 0.0 else 21
         fs, grid = floor_psd.build_p_grid(env_for_floor, self.warp_kernel, "window_mean" in self.bg_method,
-                                          self.w_host, n_fracs=2)
-        p = floor_psd.interp_p(fs, grid, [0.0], [0.0])[0].astype(np.float32)
-        return p.astype(np.float64)
+                                          self.w_host, n_fracs=n_fracs)
+        if n_fracs == 2:
+            return floor_psd.interp_p(fs, grid, [0.0], [0.0])[0].astype(np.float32).astype(np.float64)
+        return floor_psd.interp_p(fs, grid, fx, fy).astype(np.float32).astype(np.float64)
 
     def finalize(self) -> EnsembleResult:
         if self.n_images == 0:
@@ -202,14 +232,168 @@ class EnsemblePIV:
             None if pw is None else pw.data_ptr(), n, h, w, self.kx4, self.ky4, gauss.data_ptr(),
             status.data_ptr(), init.data_ptr(), diag["gain"].data_ptr(), diag["N0"].data_ptr(),
             diag["b4x"].data_ptr(), diag["b4y"].data_ptr(), diag["cost_per_pt"].data_ptr(),
-            diag["n_valid"].data_ptr(), diag["iter"].data_ptr(), diag["conv"].data_ptr(), 1,
-            work.data_ptr(), work.numel(), st), "kspace_fit")
+            diag["n_valid"].data_ptr(), diag["iter"].data_ptr(), diag["conv"].data_ptr(),
+            1 if pw is None or pw.dim() == 1 else 0, work.data_ptr(), work.numel(), st), "kspace_fit")
         out = [torch.empty(n, dtype=f32, device=d) for _ in range(6)]
         reason = torch.empty(n, dtype=torch.int32, device=d)
         _native.check(L.pivamd_ensemble_extract(
             gauss.data_ptr(), status.data_ptr(), self.mask_u8.data_ptr(), n, h, w, *[o.data_ptr() for o in out],
             reason.data_ptr(), st), "ensemble_extract")
+        if self.predictor is not None:
+            # the fit measured the residual of the warped pairs
+            out[0] += self.predictor[1].reshape(-1)
+            out[1] += self.predictor[0].reshape(-1)
         shp = (self.ny, self.nx)
         return EnsembleResult(out[0].view(shp), out[1].view(shp), out[2].view(shp), out[3].view(shp),
                               out[4].view(shp), out[5].view(shp), reason.view(shp), gauss, status, init, diag,
                               tuple(p.view(n, h, w) for p in planes), self.n_images)
+
+
+class EnsembleMultipass:
+    """Multi-pass ensemble PIV over pairs resident on the device.
+
+    ``window_sizes`` / ``overlaps`` give the standard-mode passes (the
+    'single' sum-window run type is not built).  Pass p > 0 warps every pair
+    by the previous pass's field used as one shared predictor
+    (``fused_symmetric_warp_batch(shared_predictor=1, round_shifts)``,
+    ``cpu_ensemble.py:1445-1484``):
+
+    * the field is edge-padded onto the previous pass's padded centres,
+      Gaussian-smoothed when ``predictor_smoothing`` (the reference's
+      ensemble default is off) and rounded to the nearest even integer when
+      ``predictor_rounding`` (``cpu_ensemble.py:1534-1601``);
+    * the cubic remap of it onto this pass's centres (rounded the same way,
+      ``:1620-1640``) is the predictor recorded for the pass: it sets the
+      coloured floor's per-window fractions and is added back to the fitted
+      displacements;
+    * failed fits (NaN) are infilled before they feed the next pass:
+      ``mid_infill`` between passes (default nearest), ``final_infill`` on
+      the last pass (default biharmonic, or 'none'),
+      ``single_pass_accumulator.py:1330-1400``.
+    """
+
+    def __init__(self, image_shape, window_sizes, overlaps, *, window_type="square", bg_method="correlation",
+                 kspace_floor="coloured", kspace_shape="gaussian", envelope_divide=False,
+                 predictor_smoothing=False, predictor_rounding=False, warp_interpolation="cubic",
+                 mid_infill="nearest", final_infill="biharmonic", device=None):
+        self.H, self.W = (int(image_shape[0]), int(image_shape[1]))
+        self.window_sizes = [tuple(w) if np.ndim(w) else (w, w) for w in window_sizes]
+        self.overlaps = list(overlaps)
+        if len(self.overlaps) != len(self.window_sizes):
+            raise ValueError("one overlap per pass")
+        if mid_infill not in ("nearest", "harmonic", "biharmonic"):
+            raise ValueError(f"mid-pass infill {mid_infill!r}: nearest, harmonic or biharmonic")
+        if final_infill not in ("nearest", "harmonic", "biharmonic", "none"):
+            raise ValueError(f"final-pass infill {final_infill!r}")
+        if warp_interpolation not in ("cubic", "lanczos"):
+            raise ValueError(f"warp_interpolation {warp_interpolation!r}: cubic or lanczos")
+        self.grids = build_pass_grids((self.H, self.W), self.window_sizes, self.overlaps)
+        self.opts = dict(window_type=window_type, bg_method=bg_method, kspace_floor=kspace_floor,
+                         kspace_shape=kspace_shape, envelope_divide=envelope_divide,
+                         warp_kernel=warp_interpolation)
+        self.smoothing = bool(predictor_smoothing)
+        self.rounding = bool(predictor_rounding)
+        self.interp = 0 if warp_interpolation == "cubic" else 1
+        self.mid_infill, self.final_infill = mid_infill, final_infill
+        self.dev = torch.device(device) if device is not None else torch.device("cuda")
+        self.L = _native.core()
+        import pivtools_amd.multipass  # noqa: F401  (binds the glue entry points)
+
+    def _st(self):
+        return torch.cuda.current_stream(self.dev).cuda_stream
+
+    def _predictor(self, p, field):
+        """Padded (2, 1, nyp, nxp) shared predictor of pass p from the
+        previous pass's field (2, 1, ny, nx), and its remap onto pass p's
+        window centres (2, ny, nx)."""
+        from .multipass import gaussian_weights_1d
+        L, st, d = self.L, self._st(), self.dev
+        gp, g = self.grids[p - 1], self.grids[p]
+        ny, nx = gp.n_win
+        nyp, nxp = len(gp.ctrs_y_all), len(gp.ctrs_x_all)
+        pad = torch.empty((2, 1, nyp, nxp), dtype=torch.float32, device=d)
+        _native.check(L.pivamd_pad_edge(field.data_ptr(), 1, ny, nx, gp.n_pre[0], gp.n_pre[1], nyp, nxp,
+                                        pad.data_ptr(), st), "pad")
+        if self.smoothing and g.sd > 0:
+            wts, rad = gaussian_weights_1d(g.sd, (g.ksize[0] - 1) / (2 * g.sd))
+            dw = torch.as_tensor(wts, device=d)
+            tmp, sm = torch.empty_like(pad), torch.empty_like(pad)
+            _native.check(L.pivamd_smooth_axis(pad.data_ptr(), 2, nyp, nxp, 0, dw.data_ptr(), rad, tmp.data_ptr(),
+                                               st), "smooth")
+            _native.check(L.pivamd_smooth_axis(tmp.data_ptr(), 2, nyp, nxp, 1, dw.data_ptr(), rad, sm.data_ptr(),
+                                               st), "smooth")
+            pad = sm
+        if self.rounding:
+            pad = 2.0 * torch.round(pad / 2.0)        # half to even, as np.round
+        mx = torch.as_tensor(g.pred_map_x, device=d)
+        my = torch.as_tensor(g.pred_map_y, device=d)
+        nyn, nxn = g.n_win
+        win = torch.empty((2, 1, nyn, nxn), dtype=torch.float32, device=d)
+        _native.check(L.pivamd_remap_cubic(pad.data_ptr(), 1, nyp, nxp, mx.data_ptr(), my.data_ptr(), nyn, nxn,
+                                           win.data_ptr(), st), "predictor remap")
+        if self.rounding:
+            win = 2.0 * torch.round(win / 2.0)
+        return pad.contiguous(), win.reshape(2, nyn, nxn).contiguous()
+
+    def _infill(self, p, res):
+        method = self.final_infill if p == len(self.grids) - 1 else self.mid_infill
+        g = self.grids[p]
+        ny, nx = g.n_win
+        u = torch.stack([res.uy, res.ux]).reshape(2, 1, ny, nx).contiguous()
+        if method == "none" or not bool(torch.isnan(u).any()):
+            return u
+        L, st = self.L, self._st()
+        out = torch.empty_like(u)
+        err = torch.zeros(1, dtype=torch.int32, device=self.dev)
+        if method == "nearest":
+            _native.check(L.pivamd_infill_nearest(u.data_ptr(), None, 1, ny, nx, float(g.spacing[0]),
+                                                  float(g.spacing[1]), out.data_ptr(), err.data_ptr(), st), "infill")
+        else:
+            wk = torch.empty(max(4, L.pivamd_infill_ls_workspace(1, ny, nx)), dtype=torch.uint8, device=self.dev)
+            _native.check(L.pivamd_infill_ls(u.data_ptr(), None, 1, ny, nx, float(g.spacing[0]), float(g.spacing[1]),
+                                             1 if method == "harmonic" else 2, out.data_ptr(), wk.data_ptr(),
+                                             wk.numel(), err.data_ptr(), st), "infill")
+        if int(err.item()) != 0:
+            raise RuntimeError(f"ensemble pass {p + 1}: infill found no valid donor")
+        res.uy.copy_(out[0, 0])
+        res.ux.copy_(out[1, 0])
+        return out
+
+    def run(self, img_a: torch.Tensor, img_b: torch.Tensor, batch: int = 64) -> list:
+        """All passes over the resident pairs (n, H, W) (float32 or 16-bit
+        counts); returns one EnsembleResult per pass (displacements with the
+        predictor added back, failed fits infilled)."""
+        if img_a.shape != img_b.shape or img_a.dim() != 3 or tuple(img_a.shape[1:]) != (self.H, self.W):
+            raise ValueError(f"expected two (n, {self.H}, {self.W}) stacks, got {tuple(img_a.shape)}")
+        n = int(img_a.shape[0])
+        u16 = img_a.dtype in (torch.int16, torch.uint16)
+        L, d = self.L, self.dev
+        warp_fn = L.pivamd_warp_u16 if u16 else L.pivamd_warp
+        results, field = [], None
+        wa = wb = None
+        for p, g in enumerate(self.grids):
+            ens = EnsemblePIV((self.H, self.W), g.win, self.overlaps[p], device=d, **self.opts)
+            pad = None
+            if p > 0:
+                pad, win = self._predictor(p, field)
+                ens.set_predictor(win)
+                gp = self.grids[p - 1]
+                cy = torch.as_tensor(gp.ctrs_y_all.astype(np.float32), device=d)
+                cx = torch.as_tensor(gp.ctrs_x_all.astype(np.float32), device=d)
+                if wa is None:
+                    wa = torch.empty((min(batch, n), self.H, self.W), dtype=torch.float32, device=d)
+                    wb = torch.empty_like(wa)
+            for s in range(0, n, batch):
+                a, b = img_a[s:s + batch].contiguous(), img_b[s:s + batch].contiguous()
+                m = int(a.shape[0])
+                if pad is not None:
+                    _native.check(warp_fn(a.data_ptr(), b.data_ptr(), wa.data_ptr(), wb.data_ptr(),
+                                          pad[0].data_ptr(), pad[1].data_ptr(), m, self.H, self.W, pad.shape[2],
+                                          pad.shape[3], cy.data_ptr(), cx.data_ptr(), self.interp, 1,
+                                          int(self.rounding), self._st()), "shared-predictor warp")
+                    a, b = wa[:m], wb[:m]
+                ens.accumulate(a, b)
+            res = ens.finalize()
+            field = self._infill(p, res)
+            results.append(res)
+        return results

@@ -128,3 +128,56 @@ def test_stress_recovery_uniform_shear():
     assert abs(ux - np.mean(dxs)) < 0.05, (ux, np.mean(dxs))
     assert abs(uu - np.var(dxs)) < 0.05, (uu, np.var(dxs))
     assert abs(vv - np.var(dys)) < 0.05, (vv, np.var(dys))
+
+
+def _floor_fn(mp_ens):
+    """The floor shape the device used, for the oracle chain (pinned
+    separately against the brute-force quadratic form)."""
+    def fn(win_pred, win):
+        e = EnsemblePIV((mp_ens.H, mp_ens.W), win, 50, "square", device=dev)
+        if win_pred is not None:
+            e.set_predictor(torch.as_tensor(win_pred, device=dev))
+        return e.floor_shape()
+    return fn
+
+
+@pytest.mark.parametrize("rounding", [False, True])
+def test_multipass_ensemble_matches_oracle(rounding):
+    """Two ensemble passes, 32 x 32 then 16 x 16 at 50 %, the second warped
+    with the first pass's infilled field as one shared predictor (optionally
+    rounded to even integers): pass planes within 1e-5 relative L2 of the
+    oracle chain's, identical fit statuses on pass 0, and displacements with
+    the predictor added back within 2e-3 px where both fits succeed."""
+    from pivtools_amd.ensemble import EnsembleMultipass
+    import ensemble_oracle as EO2
+
+    shape = (160, 192)
+    A, B = _pairs(10, shape, seed0=800, log_scale=2.0)
+    mp = EnsembleMultipass(shape, [(32, 32), (16, 16)], [50, 50], predictor_rounding=rounding,
+                           mid_infill="nearest", final_infill="none")
+    print(f"multipass ensemble rounding={rounding}", flush=True)
+    res = mp.run(torch.as_tensor(A, device=dev), torch.as_tensor(B, device=dev), batch=4)
+    torch.cuda.synchronize()
+    ref = EO2.ensemble_multipass(A, B, mp.grids, floor_fn=_floor_fn(mp), rounding=rounding,
+                                 mid_infill="nearest", final_infill="none")
+    for p, (r, o) in enumerate(zip(res, ref)):
+        st, ost = r.status.cpu().numpy(), o["fit"]["status"]
+        assert np.mean(st != ost) < 0.02, p
+        ux, uy = r.ux.cpu().numpy(), r.uy.cpu().numpy()
+        both = np.isfinite(ux) & np.isfinite(o["ux"]) & (st.reshape(ux.shape) == ost.reshape(ux.shape))
+        assert both.mean() > 0.8
+        d = np.hypot(ux[both] - o["ux"][both], uy[both] - o["uy"][both])
+        if p == 0:
+            for dp, rp in zip([x.cpu().numpy() for x in r.planes], o["planes"]):
+                assert _rel(dp, rp) < 1e-5
+            assert d.max() < 1e-3
+        else:
+            # a fit that flips between the chains changes the infilled
+            # predictor around it, so the next pass warps differently there
+            assert np.median(d) < 1e-4 and np.mean(d < 2e-3) > 0.95, (np.median(d), np.mean(d < 2e-3))
+    # the second pass measured the residual of warped pairs: the added-back
+    # field follows the flow, not the residual
+    assert np.nanmedian(np.abs(res[1].ux.cpu().numpy())) > 0.5
+    if rounding:
+        w = ref[1]["predictor"]
+        assert np.all(np.abs(w / 2.0 - np.round(w / 2.0)) == 0)

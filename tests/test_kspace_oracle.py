@@ -168,3 +168,34 @@ def test_noisy_batch_statuses_converge():
         ok = o["status"] == 0
         assert ok.sum() >= 30
         assert np.all(o["gauss"][ok, 9:11] >= 0)
+
+
+def test_multipass_ensemble_oracle_recovers_a_uniform_shift():
+    """The oracle's two-pass ensemble chain on pairs displaced by a uniform
+    (dy, dx) = (-1.3, 4.6) px: pass 0 measures it directly, pass 1 warps
+    by it (a shared predictor) and measures a residual near 0, and the
+    added-back field equals the shift; with even-integer rounding the
+    warp moves whole pixels and the residual carries the rest."""
+    import ensemble_oracle as EO
+    from pivtools_amd import synthetic as S
+    from pivtools_amd.window import build_pass_grids
+
+    shape = (128, 128)
+    A, B = [], []
+    for k in range(6):
+        rng = np.random.default_rng(70 + k)
+        x, y, inten = S._positions(rng, shape, 0.05)
+        A.append(S.render(shape, x, y, inten).numpy())
+        B.append(S.render(shape, x + 4.6, y - 1.3, inten).numpy())
+    A, B = np.stack(A), np.stack(B)
+    grids = build_pass_grids(shape, [(32, 32), (32, 32)], [50, 50])
+    for rounding in (False, True):
+        out = EO.ensemble_multipass(A, B, grids, rounding=rounding, final_infill="none")
+        for p, r in enumerate(out):
+            ux, uy = r["ux"][1:-1, 1:-1], r["uy"][1:-1, 1:-1]
+            assert abs(np.nanmedian(ux) - 4.6) < 0.05 and abs(np.nanmedian(uy) + 1.3) < 0.05, (rounding, p)
+        pred = out[1]["predictor"]
+        if rounding:
+            assert np.all(pred[1] == 4.0) and np.all(pred[0] == -2.0)
+        else:
+            assert abs(np.median(pred[1]) - 4.6) < 0.05
