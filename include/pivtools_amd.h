@@ -212,6 +212,31 @@ int pivamd_xcorr_noalloc(const float *d_imgA, const float *d_imgB, const float *
                          float *d_ratio, float *d_planes, void *stream);
 unsigned long pivamd_xcorr_noalloc_max_bytes(void);
 
+/* No counterpart in the reference (its caller makes one bulkxcorr2d call per
+ * batch, pivtools_cli/piv/piv_backend/cpu_instantaneous.py:505-528): one
+ * ragged launch of the correlation for up to 8 jobs -- pivamd_xcorr calls of
+ * one square window size over different batches (image sizes, window grids,
+ * buffers) -- whose windows share one grid of workgroups, then each job's
+ * fit.  Per job the results equal pivamd_xcorr's.  FP32 FFT forms only (the
+ * split DFT mode is not used); returns PIVAMD_ERR_ARG for more than 8 jobs
+ * or a job's missing buffer, PIVAMD_ERR_SIZE for a non-square or unbuilt
+ * size.  pivamd_xcorr_job_size() is sizeof(PivamdXcorrJob). */
+typedef struct PivamdXcorrJob {
+    const void *imgA, *imgB;        /* frames (n_images, H, W): float32, or 16-bit counts when u16 != 0 */
+    int u16;
+    int H, W, n_images;
+    const float *mask, *ctrX, *ctrY; /* mask nullable */
+    int nWy, nWx;
+    const float *wA, *wB, *planeWeight;
+    float *pkX, *pkY, *pkH, *sx, *sy, *sxy, *ratio;
+    float *planes;                   /* nullable */
+    void *work;                      /* pivamd_xcorr_workspace() bytes */
+    unsigned long work_bytes;
+} PivamdXcorrJob;
+int pivamd_xcorr_ragged(const PivamdXcorrJob *jobs, int n_jobs, int winY, int winX, int nPeaks,
+                        int finder, int impl, void *stream);
+int pivamd_xcorr_job_size(void);
+
 /* Device workspace bytes pivamd_xcorr needs (0 in ensemble mode). */
 unsigned long pivamd_xcorr_workspace(int n_images, int nWy, int nWx, int winY, int winX, int nPeaks,
                                      int ensemble);

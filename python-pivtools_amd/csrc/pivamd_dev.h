@@ -14,6 +14,21 @@
 #define PIVAMD_ERR_NOMEM 4
 #define PIVAMD_ERR_RUNTIME 5
 
+// One correlation call of a ragged launch (pivamd_xcorr_ragged); the same
+// layout as the PivamdXcorrJob of the public header.
+typedef struct PivamdXcorrJob {
+    const void *imgA, *imgB;        /* frames (n_images, H, W): float32, or 16-bit counts when u16 != 0 */
+    int u16;
+    int H, W, n_images;
+    const float *mask, *ctrX, *ctrY; /* mask nullable */
+    int nWy, nWx;
+    const float *wA, *wB, *planeWeight;
+    float *pkX, *pkY, *pkH, *sx, *sy, *sxy, *ratio;
+    float *planes;                   /* nullable */
+    void *work;                      /* pivamd_xcorr_workspace() bytes */
+    unsigned long work_bytes;
+} PivamdXcorrJob;
+
 extern "C" {
 int pivamd_size_supported(int ny, int nx);
 int pivamd_set_dft_mode(int mode);

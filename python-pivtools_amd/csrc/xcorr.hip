@@ -29,6 +29,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <stdio.h>
+#include <string.h>
 #include <mutex>
 #include <type_traits>
 
@@ -50,6 +51,30 @@ struct XcorrArgs {
 };
 
 constexpr int kMaxLds = 158 * 1024;
+
+// One ragged correlation launch: the correlation calls (jobs) of one square
+// window size over different batches -- other image sizes, window grids,
+// buffers -- share one grid of workgroups.  Job j owns workgroups
+// [start[j], start[j + 1]); the table travels by value in the kernel
+// arguments, so a captured HIP graph replays it as recorded.
+constexpr int kMaxRaggedJobs = 8;
+struct RaggedArgs {
+    XcorrArgs job[kMaxRaggedJobs];
+    int start[kMaxRaggedJobs + 1];
+    int n_jobs;
+    int nblocks;
+};
+
+// Job of workgroup b (uniform over the workgroup).
+__device__ __forceinline__ int ragged_job(const RaggedArgs &ra, int b)
+{
+    int j = 0;
+#pragma unroll
+    for (int k = 1; k < kMaxRaggedJobs; ++k) j += (k < ra.n_jobs && b >= ra.start[k]) ? 1 : 0;
+    return j;
+}
+
+
 
 // Lanes per line transform (R) for window size N: the default, and the
 // values built as selectable variants (fft_reg holds N/R points per lane).
@@ -1194,8 +1219,7 @@ __device__ __forceinline__ bool xc_geom(const XcorrArgs &g, long long gw, int &n
 }
 
 template <int N, int R_>
-__global__ void __launch_bounds__((XcCfg<N, R_>::BLOCK))
-xcorr_kernel(XcorrArgs g)
+__device__ __forceinline__ void xcorr_body(const XcorrArgs &g, const int b)
 {
     using C = XcCfg<N, R_>;
     constexpr int R = C::R, N1 = C::N1, T = C::T, WPB = C::WPB;
@@ -1204,8 +1228,6 @@ xcorr_kernel(XcorrArgs g)
     __shared__ float s_red[5][WPB * C::RED];
     __shared__ __attribute__((aligned(16))) float2 s_tw[R > 1 ? R * C::TWP : 1];   // split twiddles
 
-    const int b = remap_block((int)blockIdx.x, g.nblocks);
-    if (b < 0) return;    // whole block exits together
     if constexpr (R > 1) {
         for (int e = (int)threadIdx.x; e < N; e += C::BLOCK)
             s_tw[(e / N1) * C::TWP + e % N1] = make_float2(kSplitTw<N, R>.v[2 * e], kSplitTw<N, R>.v[2 * e + 1]);
@@ -1258,12 +1280,20 @@ xcorr_kernel(XcorrArgs g)
                     s_im, s_valid, wloc);
 }
 
+template <int N, int R_>
+__global__ void __launch_bounds__((XcCfg<N, R_>::BLOCK))
+xcorr_kernel(XcorrArgs g)
+{
+    const int b = remap_block((int)blockIdx.x, g.nblocks);
+    if (b < 0) return;    // whole block exits together
+    xcorr_body<N, R_>(g, b);
+}
+
 // The correlation kernel with xc_window_half: one window per 512-lane
 // workgroup in a single real plane, bounded to 128 VGPRs (four waves per
 // SIMD) so that two workgroups share a CU.
 template <int N, int R_>
-__global__ void __launch_bounds__((XcCfg<N, R_>::BLOCK), (2 * XcCfg<N, R_>::BLOCK / 256))
-xcorr_kernel_half(XcorrArgs g)
+__device__ __forceinline__ void xcorr_half_body(const XcorrArgs &g, const int b)
 {
     using C = XcCfg<N, R_>;
     constexpr int R = C::R, N1 = C::N1;
@@ -1272,8 +1302,6 @@ xcorr_kernel_half(XcorrArgs g)
     __shared__ float s_red[5][C::RED];
     __shared__ __attribute__((aligned(16))) float2 s_tw[R * C::TWP];
 
-    const int b = remap_block((int)blockIdx.x, g.nblocks);
-    if (b < 0) return;    // whole block exits together
     for (int e = (int)threadIdx.x; e < N; e += C::BLOCK)
         s_tw[(e / N1) * C::TWP + e % N1] = make_float2(kSplitTw<N, R>.v[2 * e], kSplitTw<N, R>.v[2 * e + 1]);
     __syncthreads();
@@ -1314,11 +1342,19 @@ xcorr_kernel_half(XcorrArgs g)
                          s_red[2], s_red[3], s_red[4]);
 }
 
+template <int N, int R_>
+__global__ void __launch_bounds__((XcCfg<N, R_>::BLOCK), (2 * XcCfg<N, R_>::BLOCK / 256))
+xcorr_kernel_half(XcorrArgs g)
+{
+    const int b = remap_block((int)blockIdx.x, g.nblocks);
+    if (b < 0) return;    // whole block exits together
+    xcorr_half_body<N, R_>(g, b);
+}
+
 // The correlation kernel with xc_window_half1 (one lane per line, several
 // windows per workgroup, one real plane per window).
 template <int N>
-__global__ void __launch_bounds__((XcCfg<N, 1>::BLOCK), 3)
-xcorr_kernel_half1(XcorrArgs g)
+__device__ __forceinline__ void xcorr_half1_body(const XcorrArgs &g, const int b)
 {
     using C = XcCfg<N, 1>;
     constexpr int WPB = C::WPB;
@@ -1326,8 +1362,6 @@ xcorr_kernel_half1(XcorrArgs g)
     __shared__ float s_red[5][WPB * C::RED];
     __shared__ int s_valid[WPB];
 
-    const int b = remap_block((int)blockIdx.x, g.nblocks);
-    if (b < 0) return;    // whole block exits together
     const int wloc = (int)threadIdx.x / N;
     const int t = (int)threadIdx.x - wloc * N;
     const int l = t;
@@ -1362,6 +1396,48 @@ xcorr_kernel_half1(XcorrArgs g)
         for (int k = 0; k < N; ++k) { zr[k] = 0.0f; zi[k] = 0.0f; }
     }
     xc_window_half1<N>(g, zr, zi, sa, sb, valid, gw, n, w, t, l, S, s_pl, s_valid, wloc, r0, r1, r2, r3, r4);
+}
+
+template <int N>
+__global__ void __launch_bounds__((XcCfg<N, 1>::BLOCK), 3)
+xcorr_kernel_half1(XcorrArgs g)
+{
+    const int b = remap_block((int)blockIdx.x, g.nblocks);
+    if (b < 0) return;    // whole block exits together
+    xcorr_half1_body<N>(g, b);
+}
+
+// Ragged forms of the three kernels: workgroup b of the flattened grid
+// (renumbered so each XCD takes a contiguous run) runs tile b - start[j]
+// of its job j.
+template <int N, int R_>
+__global__ void __launch_bounds__((XcCfg<N, R_>::BLOCK))
+xcorr_ragged(RaggedArgs ra)
+{
+    const int b = remap_block((int)blockIdx.x, ra.nblocks);
+    if (b < 0) return;
+    const int j = ragged_job(ra, b);
+    xcorr_body<N, R_>(ra.job[j], b - ra.start[j]);
+}
+
+template <int N, int R_>
+__global__ void __launch_bounds__((XcCfg<N, R_>::BLOCK), (2 * XcCfg<N, R_>::BLOCK / 256))
+xcorr_ragged_half(RaggedArgs ra)
+{
+    const int b = remap_block((int)blockIdx.x, ra.nblocks);
+    if (b < 0) return;
+    const int j = ragged_job(ra, b);
+    xcorr_half_body<N, R_>(ra.job[j], b - ra.start[j]);
+}
+
+template <int N>
+__global__ void __launch_bounds__((XcCfg<N, 1>::BLOCK), 3)
+xcorr_ragged_half1(RaggedArgs ra)
+{
+    const int b = remap_block((int)blockIdx.x, ra.nblocks);
+    if (b < 0) return;
+    const int j = ragged_job(ra, b);
+    xcorr_half1_body<N>(ra.job[j], b - ra.start[j]);
 }
 
 // Persistent form for one window per workgroup (WPB = 1): each workgroup
@@ -2127,6 +2203,11 @@ static int launch_mfma(XcorrArgs g, hipStream_t st)
 template <int N>
 int launch_main(XcorrArgs g, hipStream_t st);
 
+// The ragged launch of one square size (kernel chosen by variant as in
+// launch_main; the FFT forms only); same translation unit.
+template <int N>
+int launch_ragged_main(RaggedArgs &ra, int variant, hipStream_t st);
+
 #ifdef PIVAMD_XCORR_SIZE
 template <int N, int R>
 static int launch_fft(XcorrArgs g, hipStream_t st)
@@ -2214,6 +2295,64 @@ static int launch_pf(XcorrArgs g, hipStream_t st)
     }
 }
 
+// Tiles of every job at WPB windows per workgroup, then one launch.
+template <int WPB, typename K>
+static int launch_ragged_kernel(RaggedArgs &ra, K kernel, int block, hipStream_t st)
+{
+    long long acc = 0;
+    for (int j = 0; j < ra.n_jobs; ++j) {
+        ra.start[j] = (int)acc;
+        acc += (ra.job[j].total + WPB - 1) / WPB;
+        if (acc > (1LL << 30)) return PIVAMD_ERR_ARG;
+    }
+    ra.start[ra.n_jobs] = (int)acc;
+    ra.nblocks = (int)acc;
+    if (acc == 0) return 0;
+    const int grid = (int)((acc + 7) / 8 * 8);
+    hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, st, ra);
+    return hipGetLastError() == hipSuccess ? 0 : PIVAMD_ERR_LAUNCH;
+}
+
+template <int N, int R>
+static int ragged_fft(RaggedArgs &ra, hipStream_t st)
+{
+    if constexpr (!xc_radix_built(N, R)) return PIVAMD_ERR_ARG;
+    else return launch_ragged_kernel<XcCfg<N, R>::WPB>(ra, xcorr_ragged<N, R>, XcCfg<N, R>::BLOCK, st);
+}
+
+template <int N>
+static int ragged_half(RaggedArgs &ra, hipStream_t st)
+{
+    if constexpr (!xc_radix_built(N, 4) || XcCfg<N, 4>::WPB != 1) return PIVAMD_ERR_ARG;
+    else return launch_ragged_kernel<1>(ra, xcorr_ragged_half<N, 4>, XcCfg<N, 4>::BLOCK, st);
+}
+
+template <int N>
+static int ragged_half1(RaggedArgs &ra, hipStream_t st)
+{
+    if constexpr (!xc_radix_built(N, 1) || XcCfg<N, 1>::WPB < 2) return PIVAMD_ERR_ARG;
+    else return launch_ragged_kernel<XcCfg<N, 1>::WPB>(ra, xcorr_ragged_half1<N>, XcCfg<N, 1>::BLOCK, st);
+}
+
+template <>
+int launch_ragged_main<PIVAMD_XCORR_SIZE>(RaggedArgs &ra, int variant, hipStream_t st)
+{
+    constexpr int N = PIVAMD_XCORR_SIZE;
+    switch (variant) {
+    case kVariantHalf + 4: return ragged_half<N>(ra, st);
+    case kVariantHalf + 1: return ragged_half1<N>(ra, st);
+    case 1: return ragged_fft<N, 1>(ra, st);
+    case 2: return ragged_fft<N, 2>(ra, st);
+    case 4: return ragged_fft<N, 4>(ra, st);
+    case 8: return ragged_fft<N, 8>(ra, st);
+    case 0:
+        if constexpr (N == 64 || N == 128) return ragged_half<N>(ra, st);
+        else if constexpr (N == 32) return ragged_half1<N>(ra, st);
+        else return ragged_fft<N, xc_radix(N)>(ra, st);
+    default: return PIVAMD_ERR_ARG;    // the MFMA and persistent forms have no ragged launch
+    }
+}
+
 template <>
 int launch_main<PIVAMD_XCORR_SIZE>(XcorrArgs g, hipStream_t st)
 {
@@ -2248,6 +2387,16 @@ template <> int launch_main<48>(XcorrArgs, hipStream_t);
 template <> int launch_main<64>(XcorrArgs, hipStream_t);
 template <> int launch_main<96>(XcorrArgs, hipStream_t);
 template <> int launch_main<128>(XcorrArgs, hipStream_t);
+
+template <> int launch_ragged_main<8>(RaggedArgs &, int, hipStream_t);
+template <> int launch_ragged_main<12>(RaggedArgs &, int, hipStream_t);
+template <> int launch_ragged_main<16>(RaggedArgs &, int, hipStream_t);
+template <> int launch_ragged_main<24>(RaggedArgs &, int, hipStream_t);
+template <> int launch_ragged_main<32>(RaggedArgs &, int, hipStream_t);
+template <> int launch_ragged_main<48>(RaggedArgs &, int, hipStream_t);
+template <> int launch_ragged_main<64>(RaggedArgs &, int, hipStream_t);
+template <> int launch_ragged_main<96>(RaggedArgs &, int, hipStream_t);
+template <> int launch_ragged_main<128>(RaggedArgs &, int, hipStream_t);
 
 // Correlation kernel, then the fit (nPeaks == 1) or multi-peak kernel.
 static int launch_xcorr(int N, XcorrArgs g, hipStream_t st)
@@ -2453,6 +2602,71 @@ extern "C" int pivamd_xcorr_u16(const unsigned short *d_imgA, const unsigned sho
     return xcorr_entry(nullptr, nullptr, d_imgA, d_imgB, d_mask, H, W, n_images, d_ctrX, d_ctrY, nWy, nWx,
                        d_wA, d_wB, d_planeWeight, ensemble, winY, winX, nPeaks, finder, impl, d_pkX, d_pkY,
                        d_pkH, d_sx, d_sy, d_sxy, d_ratio, d_planes, d_work, work_bytes, st);
+}
+
+// ---- pivamd_xcorr_ragged: several correlation calls of one size, one launch
+extern "C" int pivamd_xcorr_job_size(void) { return (int)sizeof(PivamdXcorrJob); }
+
+extern "C" int pivamd_xcorr_ragged(const PivamdXcorrJob *jobs, int n_jobs, int winY, int winX, int nPeaks,
+                                   int finder, int impl, hipStream_t st)
+{
+    if (!jobs || n_jobs < 1 || n_jobs > kMaxRaggedJobs || nPeaks < 1) return PIVAMD_ERR_ARG;
+    if (winY != winX || !pivamd_size_supported(winY, winX)) return PIVAMD_ERR_SIZE;
+    const int N = winY;
+    RaggedArgs ra;
+    memset(&ra, 0, sizeof(ra));
+    ra.n_jobs = n_jobs;
+    for (int j = 0; j < n_jobs; ++j) {
+        const PivamdXcorrJob &J = jobs[j];
+        if (J.n_images < 0 || J.nWy < 0 || J.nWx < 0 || J.H <= 0 || J.W <= 0) return PIVAMD_ERR_ARG;
+        if (J.n_images > 0 && (!J.imgA || !J.imgB)) return PIVAMD_ERR_ARG;
+        if (!J.pkX || !J.pkY || !J.pkH || !J.sx || !J.sy || !J.sxy || !J.ratio || !J.work) return PIVAMD_ERR_ARG;
+        if (J.work_bytes < pivamd_xcorr_workspace(J.n_images, J.nWy, J.nWx, N, N, nPeaks, 0)) return PIVAMD_ERR_ARG;
+        XcorrArgs &g = ra.job[j];
+        g.imgA = J.u16 ? nullptr : (const float *)J.imgA;
+        g.imgB = J.u16 ? nullptr : (const float *)J.imgB;
+        g.imgA16 = J.u16 ? (const unsigned short *)J.imgA : nullptr;
+        g.imgB16 = J.u16 ? (const unsigned short *)J.imgB : nullptr;
+        g.mask = J.mask; g.ctrX = J.ctrX; g.ctrY = J.ctrY;
+        g.wA = J.wA; g.wB = J.wB; g.pw = J.planeWeight;
+        g.pkX = J.pkX; g.pkY = J.pkY; g.pkH = J.pkH; g.sx = J.sx; g.sy = J.sy; g.sxy = J.sxy;
+        g.ratio = J.ratio; g.planes = J.planes;
+        g.rec = (float *)J.work;
+        g.work_planes = nPeaks > 1 ? (float *)J.work + (size_t)J.n_images * J.nWy * J.nWx * kRec : nullptr;
+        g.total = (long long)J.n_images * J.nWy * J.nWx;
+        g.H = J.H; g.W = J.W; g.nWy = J.nWy; g.nWx = J.nWx; g.nPeaks = nPeaks; g.finder = finder;
+        g.impl = impl; g.ensemble = 0; g.nblocks = 0; g.dft = 0; g.variant = 0;
+    }
+    const int variant = g_variant[N];
+    int rc;
+    switch (N) {
+    case 8: rc = launch_ragged_main<8>(ra, variant, st); break;
+    case 12: rc = launch_ragged_main<12>(ra, variant, st); break;
+    case 16: rc = launch_ragged_main<16>(ra, variant, st); break;
+    case 24: rc = launch_ragged_main<24>(ra, variant, st); break;
+    case 32: rc = launch_ragged_main<32>(ra, variant, st); break;
+    case 48: rc = launch_ragged_main<48>(ra, variant, st); break;
+    case 64: rc = launch_ragged_main<64>(ra, variant, st); break;
+    case 96: rc = launch_ragged_main<96>(ra, variant, st); break;
+    case 128: rc = launch_ragged_main<128>(ra, variant, st); break;
+    default: return PIVAMD_ERR_SIZE;
+    }
+    if (rc != 0) return rc;
+    for (int j = 0; j < n_jobs; ++j) {
+        const XcorrArgs &g = ra.job[j];
+        if (g.total <= 0) continue;
+        const int nWin = g.nWy * g.nWx;
+        if (nPeaks == 1) {
+            const long long fb = (g.total + 255) / 256;
+            hipLaunchKernelGGL(fit_kernel, dim3((unsigned)fb), dim3(256), 0, st, g.rec, g.total, nWin, N, N,
+                               g.finder, g.impl, g.pkX, g.pkY, g.pkH, g.sx, g.sy, g.sxy);
+        } else {
+            hipLaunchKernelGGL(multipeak_kernel, dim3((unsigned)g.total), dim3(256), (size_t)N * N * sizeof(float),
+                               st, g.work_planes, g.rec, g.total, nWin, N, N, nPeaks, g.finder, g.pw, g);
+        }
+        if (hipGetLastError() != hipSuccess) return PIVAMD_ERR_LAUNCH;
+    }
+    return 0;
 }
 
 // ---- pivamd_xcorr_noalloc: the correlation with no workspace argument -----

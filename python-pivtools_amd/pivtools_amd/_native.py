@@ -40,6 +40,21 @@ def lib_path(name: str) -> str:
     return path
 
 
+class XcorrJob(ctypes.Structure):
+    """PivamdXcorrJob of include/pivtools_amd.h: one correlation call of a
+    ragged launch (device pointers)."""
+
+    _fields_ = [("imgA", ctypes.c_void_p), ("imgB", ctypes.c_void_p), ("u16", ctypes.c_int),
+                ("H", ctypes.c_int), ("W", ctypes.c_int), ("n_images", ctypes.c_int),
+                ("mask", ctypes.c_void_p), ("ctrX", ctypes.c_void_p), ("ctrY", ctypes.c_void_p),
+                ("nWy", ctypes.c_int), ("nWx", ctypes.c_int),
+                ("wA", ctypes.c_void_p), ("wB", ctypes.c_void_p), ("planeWeight", ctypes.c_void_p),
+                ("pkX", ctypes.c_void_p), ("pkY", ctypes.c_void_p), ("pkH", ctypes.c_void_p),
+                ("sx", ctypes.c_void_p), ("sy", ctypes.c_void_p), ("sxy", ctypes.c_void_p),
+                ("ratio", ctypes.c_void_p), ("planes", ctypes.c_void_p), ("work", ctypes.c_void_p),
+                ("work_bytes", ctypes.c_ulong)]
+
+
 def core() -> ctypes.CDLL:
     """libpivtools_amd.so with argtypes of the device-pointer API."""
     global _CORE
@@ -69,6 +84,10 @@ def core() -> ctypes.CDLL:
         ]
         L.pivamd_xcorr.restype = _i
         L.pivamd_xcorr_u16.argtypes = L.pivamd_xcorr.argtypes
+        L.pivamd_xcorr_ragged.argtypes = [ctypes.POINTER(XcorrJob), _i, _i, _i, _i, _i, _i, _vp]
+        L.pivamd_xcorr_ragged.restype = _i
+        L.pivamd_xcorr_job_size.argtypes = []
+        L.pivamd_xcorr_job_size.restype = _i
         L.pivamd_xcorr_noalloc.argtypes = L.pivamd_xcorr.argtypes[:27] + [_vp]
         L.pivamd_xcorr_noalloc.restype = _i
         L.pivamd_xcorr_noalloc_max_bytes.argtypes = []

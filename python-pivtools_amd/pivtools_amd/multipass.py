@@ -27,6 +27,7 @@ stream.  Buffers are allocated once per (batch size, schedule) and reused.
 from __future__ import annotations
 
 import ctypes
+from types import SimpleNamespace
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -389,15 +390,8 @@ class MultipassPIV:
         self._sets[n] = (bufs, self.warped)
 
     # -- one batch ---------------------------------------------------------------
-    def run(self, img_a: torch.Tensor, img_b: torch.Tensor, keep_passes: bool = False, events=None):
-        """Correlate a batch of pairs (N, H, W) already on the device.
-
-        ``img_a``/``img_b`` are float32, or 16-bit counts (uint16 data in an
-        int16 or uint16 tensor).  Returns the list of ``PassResult`` (all
-        passes if ``keep_passes``, else the final one) as device tensors.
-        ``events`` (optional list) receives, per pass, a pair of recorded
-        torch.cuda.Events bracketing the correlation launch on this stream.
-        """
+    def _begin(self, img_a: torch.Tensor, img_b: torch.Tensor, keep_passes: bool, events):
+        """Checks, buffer set of the batch size and the per-run state."""
         if img_a.shape != img_b.shape or img_a.dim() != 3 or tuple(img_a.shape[1:]) != (self.H, self.W):
             raise ValueError(f"expected two (N, {self.H}, {self.W}) stacks, got {tuple(img_a.shape)}")
         n = int(img_a.shape[0])
@@ -418,152 +412,222 @@ class MultipassPIV:
         else:
             raise TypeError(f"unsupported image dtype {img_a.dtype}")
         A, B = img_a.contiguous(), img_b.contiguous()
-        xcorr_fn = L.pivamd_xcorr_u16 if u16 else L.pivamd_xcorr
-        warp_fn = L.pivamd_warp_u16 if u16 else L.pivamd_warp
+        return SimpleNamespace(A=A, B=B, n=n, u16=u16, keep=keep_passes, events=events, results=[], prev=None,
+                               pred=None, ev=None)
 
-        results = []
-        prev = None
-        for p, b in enumerate(self._bufs):
-            g = b.grid
-            ny, nx = g.n_win
-            o = b.out
-            final = p == len(self._bufs) - 1
-            pred = None
-            if p == 0:
-                a_in, b_in = A, B
-            else:
-                field = prev.out["pad"]
-                nyp, nxp = field.shape[2], field.shape[3]
-                if b.smooth is not None:
-                    wts, rad = b.smooth
-                    _native.check(L.pivamd_smooth_axis(_ptr(field), 2 * n, nyp, nxp, 0, _ptr(wts), rad,
-                                                       _ptr(prev.out["pad_t"]), st), "smooth")
-                    _native.check(L.pivamd_smooth_axis(_ptr(prev.out["pad_t"]), 2 * n, nyp, nxp, 1, _ptr(wts), rad,
-                                                       _ptr(prev.out["pad_s"]), st), "smooth")
-                    field = prev.out["pad_s"]
-                pred = o["pred"]
-                _native.check(L.pivamd_remap_cubic(_ptr(field), n, nyp, nxp, _ptr(b.maps[0]), _ptr(b.maps[1]),
-                                                   ny, nx, _ptr(pred), st), "predictor remap")
-                _native.check(warp_fn(_ptr(A), _ptr(B), _ptr(self.warped[0]), _ptr(self.warped[1]),
-                                            _ptr(field[0]), _ptr(field[1]), n, self.H, self.W, nyp, nxp,
-                                            _ptr(prev.ctrs["ya"]), _ptr(prev.ctrs["xa"]), self.interp, 0, 0, st),
-                              "warp")
-                a_in, b_in = self.warped[0], self.warped[1]
-            if events is not None:
-                ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-                ev[0].record()
-            if b.mask is not None:
-                # masked windows are skipped by the correlation and keep these zeros
-                for k in ("pkx", "pky", "pkh", "sx", "sy", "sxy", "ratio"):
-                    o[k].zero_()
-            if self.dump_planes:
-                o["planes"].zero_()     # skipped (masked, out-of-image) windows dump zero planes
-            prev_mode = L.pivamd_set_dft_mode(self.dft)
-            fn = xcorr_fn if p == 0 else L.pivamd_xcorr
-            rc = fn(_ptr(a_in), _ptr(b_in), _ptr(b.mask), self.H, self.W, n, _ptr(b.ctrs["x"]),
-                                         _ptr(b.ctrs["y"]), ny, nx, _ptr(b.w), _ptr(b.w), _ptr(b.pw), 0,
-                                         g.win[0], g.win[1], self.num_peaks, self.finder, self.impl, _ptr(o["pkx"]),
-                                         _ptr(o["pky"]), _ptr(o["pkh"]), _ptr(o["sx"]), _ptr(o["sy"]),
-                                         _ptr(o["sxy"]), _ptr(o["ratio"]), _ptr(o.get("planes")), _ptr(b.work),
-                                         b.work.numel(), st)
-            L.pivamd_set_dft_mode(prev_mode)
-            _native.check(rc, "bulkxcorr2d")
-            if events is not None:
-                ev[1].record()
-                events.append((p, ev))
-            div = 2.0 if p == 0 else 4.0
-            nw = ny * nx
-            if self.num_peaks == 1:
-                _native.check(L.pivamd_gate(_ptr(o["pkx"]), _ptr(o["pky"]), _ptr(o["pkh"]), _ptr(pred),
-                                            _ptr(b.mask), n, nw, g.win[1] / div, g.win[0] / div, _ptr(o["u"]),
-                                            _ptr(o["nanm"]), _ptr(o["reason"]), _ptr(o["pkh_g"]), st), "gate")
-                peak_mag = o["pkh_g"].view(n, 1, ny, nx)
-            else:
-                _native.check(L.pivamd_gate_peaks(_ptr(o["pkx"]), _ptr(o["pky"]), _ptr(o["pkh"]), _ptr(pred),
-                                                  _ptr(b.mask), n, self.num_peaks, nw, g.win[1] / div,
-                                                  g.win[0] / div, _ptr(o["pxa"]), _ptr(o["pya"]), _ptr(o["pha"]),
-                                                  _ptr(o["u"]), _ptr(o["nanm"]), _ptr(o["reason"]),
-                                                  _ptr(o["pkh_g"]), _ptr(o["choice"]), st), "gate")
-                peak_mag = o["pha"]
-            oc = self.outliers
-            extra = self._extra_methods(o, o["u"], b.mask, n, ny, nx, st)
-            if not self.secondary:
-                _native.check(L.pivamd_outlier_stage(_ptr(o["u"]), _ptr(o["pkh_g"]), _ptr(b.mask), n, ny, nx,
-                                                     oc["use_median"], oc["eps"], oc["thr"], oc["size"],
-                                                     oc["use_peak"], oc["peak_thr"], _ptr(extra), _ptr(o["nanm"]),
-                                                     _ptr(o["reason"]), _ptr(o["u2"]), _ptr(o["pkh_o"]),
-                                                     _ptr(o["choice"]), st), "outlier stage")
-            else:
-                # first round on the primary peak, then one substitution round per further peak
-                _native.check(L.pivamd_outlier_detect(_ptr(o["u"]), _ptr(o["pkh_g"]), _ptr(b.mask), n, ny, nx,
-                                                      oc["use_median"], oc["eps"], oc["thr"], oc["size"],
-                                                      oc["use_peak"], oc["peak_thr"], _ptr(extra), None,
-                                                      _ptr(o["nanm"]), _ptr(o["reason"]), st), "outlier stage")
-                for _ in range(1, self.num_peaks):
-                    _native.check(L.pivamd_peak_select(_ptr(o["pxa"]), _ptr(o["pya"]), _ptr(o["pha"]), n,
-                                                       self.num_peaks, nw, _ptr(o["nanm"]), _ptr(o["reason"]),
-                                                       _ptr(o["choice"]), _ptr(o["usel"]), _ptr(o["pk_sel"]),
-                                                       _ptr(o["active"]), st), "peak select")
-                    ex = self._extra_methods(o, o["usel"], b.mask, n, ny, nx, st)
-                    _native.check(L.pivamd_outlier_detect(_ptr(o["usel"]), _ptr(o["pk_sel"]), _ptr(b.mask), n, ny,
-                                                          nx, oc["use_median"], oc["eps"], oc["thr"], oc["size"],
-                                                          oc["use_peak"], oc["peak_thr"], _ptr(ex),
-                                                          _ptr(o["active"]), _ptr(o["nanm"]), _ptr(o["reason"]),
-                                                          st), "secondary outlier stage")
-                _native.check(L.pivamd_peak_final(_ptr(o["usel"]), _ptr(o["pk_sel"]), _ptr(b.mask), n, nw,
-                                                  _ptr(o["nanm"]), _ptr(o["reason"]), _ptr(o["choice"]),
-                                                  _ptr(o["u2"]), _ptr(o["pkh_o"]), st), "peak final")
-            u = o["u2"]
-            method = self.final_infill if final else self.mid_infill
-            if method == "nearest":
-                o["err"].zero_()
-                _native.check(L.pivamd_infill_nearest(_ptr(u), _ptr(b.mask), n, ny, nx, float(g.spacing[0]),
-                                                      float(g.spacing[1]), _ptr(o["u3"]), _ptr(o["err"]), st),
-                              "infill")
-                u = o["u3"]
-            elif method in ("harmonic", "biharmonic"):
-                o["err"].zero_()
-                wk = o["ls_work"]
-                _native.check(L.pivamd_infill_ls(_ptr(u), _ptr(b.mask), n, ny, nx, float(g.spacing[0]),
-                                                 float(g.spacing[1]), 1 if method == "harmonic" else 2,
-                                                 _ptr(o["u3"]), _ptr(wk), wk.numel(), _ptr(o["err"]), st), "infill")
-                u = o["u3"]
-            if not final:
-                nyp, nxp = len(g.ctrs_y_all), len(g.ctrs_x_all)
-                src = u
-                if "u4" in o:
-                    # the predictor's copy of the field with its masked windows filled;
-                    # the pass result keeps zeros there
-                    src = o["u4"]
-                    if self.predictor_fill == "nearest":
-                        _native.check(L.pivamd_fill_masked_nearest(_ptr(u), _ptr(b.mask), n, ny, nx,
-                                                                   float(g.spacing[0]), float(g.spacing[1]),
-                                                                   _ptr(src), _ptr(o["err"]), st), "predictor fill")
-                    else:
-                        src.copy_(u)
-                        fi, bi, gt = b.fill_op
-                        _native.check(L.pivamd_fill_linear(_ptr(src), 2 * n, ny * nx, fi.numel(), _ptr(fi),
-                                                           bi.numel(), _ptr(bi), _ptr(gt), st), "predictor fill")
-                _native.check(L.pivamd_pad_edge(_ptr(src), n, ny, nx, g.n_pre[0], g.n_pre[1], nyp, nxp,
-                                                _ptr(o["pad"]), st), "pad")
-                for rows, uy_bc, ux_bc in b.bc_rows:
-                    o["pad"][0, :, rows, :].fill_(uy_bc)
-                    o["pad"][1, :, rows, :].fill_(ux_bc)
-            if keep_passes or final:
-                results.append(PassResult(ux=u[1], uy=u[0], nan_mask=o["nanm"], nan_reason=o["reason"],
-                                          peak_mag=peak_mag, peak_ratio=o["ratio"], predictor=pred,
-                                          window_size=g.win, n_windows=(ny, nx), primary_peak_mag=o["pkh_o"],
-                                          peak_choice=o["choice"],
-                                          b_mask=(self.masks[p] if self.masks is not None and self.masks[p] is not None
-                                                  else np.zeros((ny, nx), bool)),
-                                          win_ctrs_x=g.ctrs_x, win_ctrs_y=g.ctrs_y,
-                                          debug_dump=None if not self.dump_planes else {
-                                              "planes": o["planes"], "pk_loc_x_raw": o["pkx"],
-                                              "pk_loc_y_raw": o["pky"], "pk_height_raw": o["pkh"],
-                                              "pk_ratio_raw": o["ratio"], "sx": o["sx"], "sy": o["sy"],
-                                              "sxy": o["sxy"], "peak_finder": int(self.finder)}))
-            prev = b
-        return results
+    def _pre(self, c, p):
+        """Pass p up to its correlation: predictor smoothing and remap and
+        the warp (p > 0).  Returns the frames the correlation reads."""
+        st, L, n = self._stream(), self.lib, c.n
+        A, B = c.A, c.B
+        b = self._bufs[p]
+        prev = c.prev
+        warp_fn = L.pivamd_warp_u16 if c.u16 else L.pivamd_warp
+        g = b.grid
+        ny, nx = g.n_win
+        o = b.out
+        final = p == len(self._bufs) - 1
+        pred = None
+        if p == 0:
+            a_in, b_in = A, B
+        else:
+            field = prev.out["pad"]
+            nyp, nxp = field.shape[2], field.shape[3]
+            if b.smooth is not None:
+                wts, rad = b.smooth
+                _native.check(L.pivamd_smooth_axis(_ptr(field), 2 * n, nyp, nxp, 0, _ptr(wts), rad,
+                                                   _ptr(prev.out["pad_t"]), st), "smooth")
+                _native.check(L.pivamd_smooth_axis(_ptr(prev.out["pad_t"]), 2 * n, nyp, nxp, 1, _ptr(wts), rad,
+                                                   _ptr(prev.out["pad_s"]), st), "smooth")
+                field = prev.out["pad_s"]
+            pred = o["pred"]
+            _native.check(L.pivamd_remap_cubic(_ptr(field), n, nyp, nxp, _ptr(b.maps[0]), _ptr(b.maps[1]),
+                                               ny, nx, _ptr(pred), st), "predictor remap")
+            _native.check(warp_fn(_ptr(A), _ptr(B), _ptr(self.warped[0]), _ptr(self.warped[1]),
+                                        _ptr(field[0]), _ptr(field[1]), n, self.H, self.W, nyp, nxp,
+                                        _ptr(prev.ctrs["ya"]), _ptr(prev.ctrs["xa"]), self.interp, 0, 0, st),
+                          "warp")
+            a_in, b_in = self.warped[0], self.warped[1]
+        c.pred = pred
+        return a_in, b_in
+
+    def _correlate(self, c, p, a_in, b_in):
+        """The correlation call of pass p (one launch plus the fit)."""
+        st, L, n = self._stream(), self.lib, c.n
+        b = self._bufs[p]
+        g = b.grid
+        ny, nx = g.n_win
+        o = b.out
+        events = c.events
+        xcorr_fn = L.pivamd_xcorr_u16 if c.u16 else L.pivamd_xcorr
+        if events is not None:
+            ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+            ev[0].record()
+        self._zero_skipped(p)
+        prev_mode = L.pivamd_set_dft_mode(self.dft)
+        fn = xcorr_fn if p == 0 else L.pivamd_xcorr
+        rc = fn(_ptr(a_in), _ptr(b_in), _ptr(b.mask), self.H, self.W, n, _ptr(b.ctrs["x"]),
+                                     _ptr(b.ctrs["y"]), ny, nx, _ptr(b.w), _ptr(b.w), _ptr(b.pw), 0,
+                                     g.win[0], g.win[1], self.num_peaks, self.finder, self.impl, _ptr(o["pkx"]),
+                                     _ptr(o["pky"]), _ptr(o["pkh"]), _ptr(o["sx"]), _ptr(o["sy"]),
+                                     _ptr(o["sxy"]), _ptr(o["ratio"]), _ptr(o.get("planes")), _ptr(b.work),
+                                     b.work.numel(), st)
+        L.pivamd_set_dft_mode(prev_mode)
+        _native.check(rc, "bulkxcorr2d")
+        if events is not None:
+            ev[1].record()
+            events.append((p, ev))
+
+    def _zero_skipped(self, p):
+        """Outputs of windows the correlation skips (masked, out of the
+        image) keep zeros; the plane dump holds zero planes there."""
+        b = self._bufs[p]
+        o = b.out
+        if b.mask is not None:
+            for k in ("pkx", "pky", "pkh", "sx", "sy", "sxy", "ratio"):
+                o[k].zero_()
+        if self.dump_planes:
+            o["planes"].zero_()
+
+    def _job(self, c, p, a_in, b_in):
+        """The correlation call of pass p as a ragged-launch job."""
+        b = self._bufs[p]
+        g = b.grid
+        o = b.out
+        J = _native.XcorrJob()
+        J.imgA, J.imgB = _ptr(a_in), _ptr(b_in)
+        J.u16 = 1 if (p == 0 and c.u16) else 0
+        J.H, J.W, J.n_images = self.H, self.W, c.n
+        J.mask, J.ctrX, J.ctrY = _ptr(b.mask), _ptr(b.ctrs["x"]), _ptr(b.ctrs["y"])
+        J.nWy, J.nWx = g.n_win
+        J.wA = J.wB = _ptr(b.w)
+        J.planeWeight = _ptr(b.pw)
+        J.pkX, J.pkY, J.pkH = _ptr(o["pkx"]), _ptr(o["pky"]), _ptr(o["pkh"])
+        J.sx, J.sy, J.sxy, J.ratio = _ptr(o["sx"]), _ptr(o["sy"]), _ptr(o["sxy"]), _ptr(o["ratio"])
+        J.planes = _ptr(o.get("planes"))
+        J.work, J.work_bytes = _ptr(b.work), b.work.numel()
+        return J
+
+    def _post(self, c, p):
+        """Pass p after its correlation: gate, outlier tests, infill, the
+        padded field for the next pass and the pass result."""
+        st, L, n = self._stream(), self.lib, c.n
+        b = self._bufs[p]
+        g = b.grid
+        ny, nx = g.n_win
+        o = b.out
+        final = p == len(self._bufs) - 1
+        pred = c.pred
+        keep_passes = c.keep
+        results = c.results
+        div = 2.0 if p == 0 else 4.0
+        nw = ny * nx
+        if self.num_peaks == 1:
+            _native.check(L.pivamd_gate(_ptr(o["pkx"]), _ptr(o["pky"]), _ptr(o["pkh"]), _ptr(pred),
+                                        _ptr(b.mask), n, nw, g.win[1] / div, g.win[0] / div, _ptr(o["u"]),
+                                        _ptr(o["nanm"]), _ptr(o["reason"]), _ptr(o["pkh_g"]), st), "gate")
+            peak_mag = o["pkh_g"].view(n, 1, ny, nx)
+        else:
+            _native.check(L.pivamd_gate_peaks(_ptr(o["pkx"]), _ptr(o["pky"]), _ptr(o["pkh"]), _ptr(pred),
+                                              _ptr(b.mask), n, self.num_peaks, nw, g.win[1] / div,
+                                              g.win[0] / div, _ptr(o["pxa"]), _ptr(o["pya"]), _ptr(o["pha"]),
+                                              _ptr(o["u"]), _ptr(o["nanm"]), _ptr(o["reason"]),
+                                              _ptr(o["pkh_g"]), _ptr(o["choice"]), st), "gate")
+            peak_mag = o["pha"]
+        oc = self.outliers
+        extra = self._extra_methods(o, o["u"], b.mask, n, ny, nx, st)
+        if not self.secondary:
+            _native.check(L.pivamd_outlier_stage(_ptr(o["u"]), _ptr(o["pkh_g"]), _ptr(b.mask), n, ny, nx,
+                                                 oc["use_median"], oc["eps"], oc["thr"], oc["size"],
+                                                 oc["use_peak"], oc["peak_thr"], _ptr(extra), _ptr(o["nanm"]),
+                                                 _ptr(o["reason"]), _ptr(o["u2"]), _ptr(o["pkh_o"]),
+                                                 _ptr(o["choice"]), st), "outlier stage")
+        else:
+            # first round on the primary peak, then one substitution round per further peak
+            _native.check(L.pivamd_outlier_detect(_ptr(o["u"]), _ptr(o["pkh_g"]), _ptr(b.mask), n, ny, nx,
+                                                  oc["use_median"], oc["eps"], oc["thr"], oc["size"],
+                                                  oc["use_peak"], oc["peak_thr"], _ptr(extra), None,
+                                                  _ptr(o["nanm"]), _ptr(o["reason"]), st), "outlier stage")
+            for _ in range(1, self.num_peaks):
+                _native.check(L.pivamd_peak_select(_ptr(o["pxa"]), _ptr(o["pya"]), _ptr(o["pha"]), n,
+                                                   self.num_peaks, nw, _ptr(o["nanm"]), _ptr(o["reason"]),
+                                                   _ptr(o["choice"]), _ptr(o["usel"]), _ptr(o["pk_sel"]),
+                                                   _ptr(o["active"]), st), "peak select")
+                ex = self._extra_methods(o, o["usel"], b.mask, n, ny, nx, st)
+                _native.check(L.pivamd_outlier_detect(_ptr(o["usel"]), _ptr(o["pk_sel"]), _ptr(b.mask), n, ny,
+                                                      nx, oc["use_median"], oc["eps"], oc["thr"], oc["size"],
+                                                      oc["use_peak"], oc["peak_thr"], _ptr(ex),
+                                                      _ptr(o["active"]), _ptr(o["nanm"]), _ptr(o["reason"]),
+                                                      st), "secondary outlier stage")
+            _native.check(L.pivamd_peak_final(_ptr(o["usel"]), _ptr(o["pk_sel"]), _ptr(b.mask), n, nw,
+                                              _ptr(o["nanm"]), _ptr(o["reason"]), _ptr(o["choice"]),
+                                              _ptr(o["u2"]), _ptr(o["pkh_o"]), st), "peak final")
+        u = o["u2"]
+        method = self.final_infill if final else self.mid_infill
+        if method == "nearest":
+            o["err"].zero_()
+            _native.check(L.pivamd_infill_nearest(_ptr(u), _ptr(b.mask), n, ny, nx, float(g.spacing[0]),
+                                                  float(g.spacing[1]), _ptr(o["u3"]), _ptr(o["err"]), st),
+                          "infill")
+            u = o["u3"]
+        elif method in ("harmonic", "biharmonic"):
+            o["err"].zero_()
+            wk = o["ls_work"]
+            _native.check(L.pivamd_infill_ls(_ptr(u), _ptr(b.mask), n, ny, nx, float(g.spacing[0]),
+                                             float(g.spacing[1]), 1 if method == "harmonic" else 2,
+                                             _ptr(o["u3"]), _ptr(wk), wk.numel(), _ptr(o["err"]), st), "infill")
+            u = o["u3"]
+        if not final:
+            nyp, nxp = len(g.ctrs_y_all), len(g.ctrs_x_all)
+            src = u
+            if "u4" in o:
+                # the predictor's copy of the field with its masked windows filled;
+                # the pass result keeps zeros there
+                src = o["u4"]
+                if self.predictor_fill == "nearest":
+                    _native.check(L.pivamd_fill_masked_nearest(_ptr(u), _ptr(b.mask), n, ny, nx,
+                                                               float(g.spacing[0]), float(g.spacing[1]),
+                                                               _ptr(src), _ptr(o["err"]), st), "predictor fill")
+                else:
+                    src.copy_(u)
+                    fi, bi, gt = b.fill_op
+                    _native.check(L.pivamd_fill_linear(_ptr(src), 2 * n, ny * nx, fi.numel(), _ptr(fi),
+                                                       bi.numel(), _ptr(bi), _ptr(gt), st), "predictor fill")
+            _native.check(L.pivamd_pad_edge(_ptr(src), n, ny, nx, g.n_pre[0], g.n_pre[1], nyp, nxp,
+                                            _ptr(o["pad"]), st), "pad")
+            for rows, uy_bc, ux_bc in b.bc_rows:
+                o["pad"][0, :, rows, :].fill_(uy_bc)
+                o["pad"][1, :, rows, :].fill_(ux_bc)
+        if keep_passes or final:
+            results.append(PassResult(ux=u[1], uy=u[0], nan_mask=o["nanm"], nan_reason=o["reason"],
+                                      peak_mag=peak_mag, peak_ratio=o["ratio"], predictor=pred,
+                                      window_size=g.win, n_windows=(ny, nx), primary_peak_mag=o["pkh_o"],
+                                      peak_choice=o["choice"],
+                                      b_mask=(self.masks[p] if self.masks is not None and self.masks[p] is not None
+                                              else np.zeros((ny, nx), bool)),
+                                      win_ctrs_x=g.ctrs_x, win_ctrs_y=g.ctrs_y,
+                                      debug_dump=None if not self.dump_planes else {
+                                          "planes": o["planes"], "pk_loc_x_raw": o["pkx"],
+                                          "pk_loc_y_raw": o["pky"], "pk_height_raw": o["pkh"],
+                                          "pk_ratio_raw": o["ratio"], "sx": o["sx"], "sy": o["sy"],
+                                          "sxy": o["sxy"], "peak_finder": int(self.finder)}))
+        c.prev = b
+
+    def run(self, img_a: torch.Tensor, img_b: torch.Tensor, keep_passes: bool = False, events=None):
+        """Correlate a batch of pairs (N, H, W) already on the device.
+
+        ``img_a``/``img_b`` are float32, or 16-bit counts (uint16 data in an
+        int16 or uint16 tensor).  Returns the list of ``PassResult`` (all
+        passes if ``keep_passes``, else the final one) as device tensors.
+        ``events`` (optional list) receives, per pass, a pair of recorded
+        torch.cuda.Events bracketing the correlation launch on this stream.
+        """
+        c = self._begin(img_a, img_b, keep_passes, events)
+        for p in range(len(self._bufs)):
+            a_in, b_in = self._pre(c, p)
+            self._correlate(c, p, a_in, b_in)
+            self._post(c, p)
+        return c.results
 
     def _extra_methods(self, o, u, mask, n, ny, nx, st):
         """Flags of the sigma / div_vort methods on the planar field u, or None."""
@@ -598,3 +662,47 @@ class MultipassPIV:
             if e & 2:
                 raise np.linalg.LinAlgError(f"pass {p}: least-squares infill did not reach a relative "
                                             "residual of 1e-6")
+
+
+def run_ragged(items, keep_passes: bool = False):
+    """Several pass loops of one schedule over batches of different image
+    sizes (``items``: (MultipassPIV, img_a, img_b) triples on the current
+    stream), pass by pass: each pass correlates every batch in ONE ragged
+    launch (``pivamd_xcorr_ragged``: the windows of all batches share one
+    grid of workgroups) between the batches' own warps and glue.  Each
+    batch's results equal its own ``run``.  Returns one result list per item.
+    The schedules must match pass for pass (square windows; a rectangular
+    pass runs each batch's own correlation) and the peak settings must agree;
+    the split-DFT mode is not used."""
+    if not items:
+        return []
+    mps = [it[0] for it in items]
+    m0 = mps[0]
+    for m in mps[1:]:
+        if m.window_sizes != m0.window_sizes:
+            raise ValueError("ragged batches need the same window schedule")
+        if (m.num_peaks, m.finder, m.impl) != (m0.num_peaks, m0.finder, m0.impl):
+            raise ValueError("ragged batches need the same peak settings")
+        if m.device != m0.device:
+            raise ValueError("ragged batches need one device")
+    if len(items) > 8:
+        raise ValueError("at most 8 batches per ragged launch")
+    if any(m.dft for m in mps):
+        raise ValueError("the ragged launch runs the FP32 FFT forms (dft='fft')")
+    ctx = [m._begin(a, b, keep_passes, None) for m, a, b in items]
+    L = m0.lib
+    st = m0._stream()
+    for p, win in enumerate(m0.window_sizes):
+        ins = [m._pre(c, p) for m, c in zip(mps, ctx)]
+        if win[0] == win[1]:
+            for m in mps:
+                m._zero_skipped(p)
+            jobs = (_native.XcorrJob * len(mps))(*[m._job(c, p, a, b) for m, c, (a, b) in zip(mps, ctx, ins)])
+            _native.check(L.pivamd_xcorr_ragged(jobs, len(mps), win[0], win[1], m0.num_peaks, m0.finder, m0.impl,
+                                                st), "ragged bulkxcorr2d")
+        else:
+            for m, c, (a, b) in zip(mps, ctx, ins):
+                m._correlate(c, p, a, b)
+        for m, c in zip(mps, ctx):
+            m._post(c, p)
+    return [c.results for c in ctx]

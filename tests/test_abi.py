@@ -80,3 +80,8 @@ def test_batch_lane_width_within_reference_bound():
     assert 1 <= L.peakfit_batch_lanes() <= 16
     assert L.peakfit_batch_lanes() == L.codelet_lanes()
     assert L.peakfit_batch_available() == 1
+
+
+def test_ragged_job_layout_matches_the_library():
+    """The ctypes mirror of PivamdXcorrJob has the library's size."""
+    assert ctypes.sizeof(_native.XcorrJob) == _native.core().pivamd_xcorr_job_size()

@@ -996,3 +996,63 @@ def test_loop_with_rectangular_windows_matches_oracle_loop():
         same = ~gm
         d = np.hypot(g.ux.cpu().numpy()[same] - r["ux"][same], g.uy.cpu().numpy()[same] - r["uy"][same])
         assert np.sqrt(np.mean(d ** 2)) < 0.005, (p, d.max())
+
+
+def _lamb_batch(n, shape, seed, u16):
+    A, B = [], []
+    for k in range(n):
+        rng = np.random.default_rng(seed + k)
+        x, y, inten = S._positions(rng, shape, 0.03)
+        dx, dy = S.lamb_oseen(x, y, shape, rc=50.0, peak=3.0, uniform=(1.5, -0.5))
+        a, b = S.render(shape, x, y, inten).numpy(), S.render(shape, x + dx, y + dy, inten).numpy()
+        if u16:
+            a, b = np.round(a * 3000.0), np.round(b * 3000.0)
+        A.append(a)
+        B.append(b)
+    A, B = np.stack(A).astype(np.float32), np.stack(B).astype(np.float32)
+    if u16:
+        return _t(np.clip(A, 0, 4095).astype(np.int16), torch.int16), _t(np.clip(B, 0, 4095).astype(np.int16),
+                                                                        torch.int16)
+    return _t(A), _t(B)
+
+
+@pytest.mark.parametrize("sched", [[(64, 64), (32, 32), (16, 16)], [(128, 128), (64, 64), (32, 32), (16, 16)]])
+def test_ragged_launch_equals_separate_runs(sched):
+    """Batches of three image sizes (16-bit and float frames) through one
+    ragged correlation launch per pass give, batch by batch, exactly the
+    results of their own pass loops; and a captured HIP graph of the ragged
+    loop replays them exactly, with eager work between the replays."""
+    from pivtools_amd.multipass import run_ragged
+
+    ov = [50] * len(sched)
+    shapes = [((256, 320), 2, True), ((192, 448), 3, False), ((320, 256), 1, True)]
+    batches = [_lamb_batch(n, shp, 1200 + 10 * i, u16) for i, (shp, n, u16) in enumerate(shapes)]
+    alone = []
+    for (shp, n, u16), (a, b) in zip(shapes, batches):
+        r = MultipassPIV(shp, sched, ov).run(a, b, keep_passes=True)
+        alone.append([(x.ux.clone(), x.uy.clone(), x.nan_mask.clone(), x.nan_reason.clone()) for x in r])
+    mps = [MultipassPIV(shp, sched, ov) for shp, _, _ in shapes]
+    res = run_ragged([(m, a, b) for m, (a, b) in zip(mps, batches)], keep_passes=True)
+    torch.cuda.synchronize()
+
+    def same(res_list):
+        for got, ref in zip(res_list, alone):
+            for g, r in zip(got, ref):
+                for x, y in zip((g.ux, g.uy, g.nan_mask, g.nan_reason), r):
+                    assert np.array_equal(x.cpu().numpy(), y.cpu().numpy(), equal_nan=True)
+    same(res)
+    # graph capture of the whole ragged loop
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        run_ragged([(m, a, b) for m, (a, b) in zip(mps, batches)], keep_passes=True)
+    torch.cuda.current_stream().wait_stream(s)
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        gres = run_ragged([(m, a, b) for m, (a, b) in zip(mps, batches)], keep_passes=True)
+    other = MultipassPIV((192, 448), sched, ov)
+    for _ in range(3):
+        other.run(*batches[1])                    # unrelated eager work between replays
+        graph.replay()
+        torch.cuda.synchronize()
+        same(gres)
