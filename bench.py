@@ -488,6 +488,47 @@ def run_mixed(args):
         del mps
         torch.cuda.empty_cache()
 
+    # both image sizes through one ragged correlation launch per pass
+    from pivtools_amd.multipass import run_ragged
+    mps = [MultipassPIV(shape, MIXED_SCHEDULE, [50] * 4, peak_finder=args.finder, mid_infill=args.infill,
+                        final_infill=args.infill, warp_arithmetic=args.warp) for shape, _, _ in groups]
+    items = [(mp, A, B) for mp, (_, A, B) in zip(mps, groups)]
+    log("[bench] ragged: warmup")
+    for _ in range(args.warmup):
+        run_ragged(items)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        rres = run_ragged(items)
+    torch.cuda.synchronize()
+    rwall = time.perf_counter() - t0
+    exact = all(bool(torch.equal(r[-1].ux.nan_to_num(7.0), f.ux.nan_to_num(7.0))) and
+                bool(torch.equal(r[-1].nan_mask, f.nan_mask)) for r, f in zip(rres, finals["fft"]))
+    legs["ragged"] = {"pairs_per_s": round(args.mixed_pairs * args.steps / rwall, 3),
+                      "ms_per_step": round(rwall * 1e3 / args.steps, 3), "equals_separate_runs": exact}
+    # small batches (two pairs per image size): where one launch per pass matters
+    small = [(mp, A[:2], B[:2]) for mp, (_, A, B) in zip(mps, groups)]
+    for _ in range(2):
+        run_ragged(small)
+        for mp, a, b in small:
+            mp.run(a, b)
+    torch.cuda.synchronize()
+    tt = {}
+    for name in ("separate", "ragged"):
+        t0 = time.perf_counter()
+        for _ in range(20):
+            if name == "ragged":
+                run_ragged(small)
+            else:
+                for mp, a, b in small:
+                    mp.run(a, b)
+        torch.cuda.synchronize()
+        tt[name] = (time.perf_counter() - t0) / 20 * 1e3
+    legs["ragged"]["two_pairs_per_size_ms"] = {k: round(v, 3) for k, v in tt.items()}
+    log(f"[bench] ragged: {legs['ragged']}")
+    del mps, items
+    torch.cuda.empty_cache()
+
     # the whole loop: final-pass vectors of the split form against the FFT form
     loop = []
     for gi, (shape, _, _) in enumerate(groups):
@@ -525,6 +566,7 @@ def run_mixed(args):
         "latency_ms_p99": fft["latency_ms_p99"],
         "fft": fft,
         "split": legs["split"],
+        "ragged": legs["ragged"],
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
                      "kernel": f"{_xcorr_kernel_name(dom['window'], 'fft')} + fit_kernel ({dom['image']})",

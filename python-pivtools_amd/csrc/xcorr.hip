@@ -2346,7 +2346,7 @@ int launch_ragged_main<PIVAMD_XCORR_SIZE>(RaggedArgs &ra, int variant, hipStream
     case 4: return ragged_fft<N, 4>(ra, st);
     case 8: return ragged_fft<N, 8>(ra, st);
     case 0:
-        if constexpr (N == 64 || N == 128) return ragged_half<N>(ra, st);
+        if constexpr (N == 48 || N == 64 || N == 128) return ragged_half<N>(ra, st);
         else if constexpr (N == 32) return ragged_half1<N>(ra, st);
         else return ragged_fft<N, xc_radix(N)>(ra, st);
     default: return PIVAMD_ERR_ARG;    // the MFMA and persistent forms have no ragged launch
@@ -2369,7 +2369,7 @@ int launch_main<PIVAMD_XCORR_SIZE>(XcorrArgs g, hipStream_t st)
     case 4: return launch_fft<N, 4>(g, st);
     case 8: return launch_fft<N, 8>(g, st);
     case 0:
-        if constexpr (N == 64 || N == 128) return launch_half<N, 4>(g, st);   // more windows per CU
+        if constexpr (N == 48 || N == 64 || N == 128) return launch_half<N, 4>(g, st);   // more windows per CU
         else if constexpr (N == 32) return launch_half1<N>(g, st);
         else return launch_fft<N, xc_radix(N)>(g, st);
     default: return PIVAMD_ERR_ARG;
@@ -2503,7 +2503,7 @@ static bool variant_ok(int N, int v)
     if (v == 0) return true;
     if (v == kVariantMfma) return N % 16 == 0 && N <= 64;
     if (v == kVariantPersist + 4) return xc_radix_built(N, 4) && N * 4 >= 256;   // WPB = 1
-    if (v == kVariantHalf + 4) return N == 64 || N == 96 || N == 128;
+    if (v == kVariantHalf + 4) return N == 48 || N == 64 || N == 96 || N == 128;
     if (v == kVariantHalf + 1) return xc_radix_built(N, 1);
     return (v == 1 || v == 2 || v == 4 || v == 8) && xc_radix_built(N, v);
 }
@@ -2523,7 +2523,7 @@ extern "C" int pivamd_xcorr_variants(int N, int *out, int max_out)
 extern "C" int pivamd_xcorr_default_variant(int N)
 {
     if (!pivamd_size_supported(N, N)) return -1;
-    return (N == 64 || N == 128) ? kVariantHalf + 4 : N == 32 ? kVariantHalf + 1 : xc_radix(N);
+    return (N == 48 || N == 64 || N == 128) ? kVariantHalf + 4 : N == 32 ? kVariantHalf + 1 : xc_radix(N);
 }
 
 extern "C" int pivamd_xcorr_set_variant(int N, int v)

@@ -12,7 +12,7 @@ the library (``pivamd_xcorr_variants``):
   window per workgroup) that loads the next window's samples while the
   current one is transformed.
 * ``64 + R``: the R-lane FFT with one real LDS plane per window instead of
-  the complex tile (``68``: 64, 96 and 128 points; ``65``: sizes up to 32
+  the complex tile (``68``: 48, 64, 96 and 128 points; ``65``: sizes up to 32
   points, inverse stages packed over the workgroup's windows): the stage
   exchanges move real then imaginary parts, so more windows fit a CU.
 
