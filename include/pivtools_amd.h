@@ -382,6 +382,16 @@ int pivamd_infill_ls(const float *u, const float *d_mask, int N, int ny, int nx,
  * as a linear map u[p][fill_idx[i]] = sum_j gt[j*nf + i] * u[p][bnd_idx[j]]
  * (FP64, in place, every plane p < planes of plane_stride floats); gt is the
  * mask geometry's map, computed once on the host. */
+/* infilling.py:609-695 (infill_knn): NaN vectors (not excluded) become the
+ * k-nearest-neighbour regression of the valid vectors in physical
+ * coordinates, weights 1/distance (uniform != 0: equal), k = min(n_neighbors,
+ * donors) with n_neighbors <= 128, FP64 sums stored as float32; ties at equal
+ * distance in row-major order.  d_work: pivamd_infill_knn_workspace(N)
+ * bytes; *d_err |= 1 when an image with holes has no donor. */
+unsigned long pivamd_infill_knn_workspace(int N);
+int pivamd_infill_knn(const float *u, const float *d_excl, int N, int ny, int nx, float py, float px,
+                      int n_neighbors, int uniform, float *out, void *d_work, unsigned long work_bytes,
+                      int *d_err, void *stream);
 int pivamd_fill_masked_nearest(const float *u, const float *d_mask, int N, int ny, int nx, float py,
                                float px, float *uo, int *d_err, void *stream);
 int pivamd_fill_linear(float *u, int planes, long plane_stride, int nf, const int *d_fill_idx, int nb,

@@ -335,6 +335,54 @@ def infill_nearest(u, spacing, exclude=None):
     return out.astype(np.float32)
 
 
+def infill_knn(u, spacing, k=32, weights="distance", exclude=None, impl="brute"):
+    """infill_knn (infilling.py:609-695) per image: NaN vectors (not excluded)
+    become the k-nearest-neighbour regression of the valid vectors in
+    physical coordinates (x * px, y * py), k = min(k, donors), weights
+    1 / distance or uniform, FP64, stored as float32.  impl="sklearn" calls
+    KNeighborsRegressor (kd_tree) as the reference does; impl="brute" ranks
+    the donors by (distance, row-major index) and sums in that order -- the
+    rule the device follows where several donors share the k-th distance."""
+    py, px = float(spacing[0]), float(spacing[1])
+    out = u.astype(np.float32).copy()
+    ny, nx = u.shape[2:]
+    yy, xx = np.mgrid[0:ny, 0:nx]
+    X = np.c_[(xx * px).ravel(), (yy * py).ravel()].astype(np.float64)
+    for i in range(u.shape[1]):
+        ux, uy = u[1, i].ravel(), u[0, i].ravel()
+        ex = np.zeros(ny * nx, bool) if exclude is None else np.asarray(exclude, bool).ravel()
+        fill = (np.isnan(ux) | np.isnan(uy)) & ~ex
+        if not fill.any():
+            continue
+        donor = ~fill & ~ex & np.isfinite(ux) & np.isfinite(uy)
+        nd = int(donor.sum())
+        if nd == 0:
+            raise ValueError("no donor")
+        kk = min(k, nd)
+        Xd, Xq = X[donor], X[fill]
+        yd = np.c_[ux[donor].astype(np.float64), uy[donor].astype(np.float64)]
+        if impl == "sklearn":
+            from sklearn.neighbors import KNeighborsRegressor
+            knn = KNeighborsRegressor(n_neighbors=kk, weights=weights, algorithm="kd_tree", n_jobs=1)
+            pred = knn.fit(Xd, yd).predict(Xq)
+        else:
+            idx = np.flatnonzero(donor)
+            pred = np.empty((len(Xq), 2))
+            for q, xq in enumerate(Xq):
+                d = np.sqrt(((Xd - xq) ** 2).sum(axis=1))
+                order = np.lexsort((idx, d))[:kk]
+                wts = np.ones(kk) if weights == "uniform" else 1.0 / d[order]
+                sy = sx = sw = 0.0
+                for j, o in enumerate(order):
+                    sx += yd[o, 0] * wts[j]
+                    sy += yd[o, 1] * wts[j]
+                    sw += wts[j]
+                pred[q] = (sx / sw, sy / sw)
+        out[1, i].ravel()[fill] = pred[:, 0].astype(np.float32)
+        out[0, i].ravel()[fill] = pred[:, 1].astype(np.float32)
+    return out
+
+
 def vector_masks_loop(image_shape, window_sizes, overlaps, pixel_mask, threshold=0.5):
     """compute_vector_mask (load_images.py:827-1000, standard mode) as the
     reference's per-window loop."""
@@ -649,6 +697,8 @@ def run_multipass(A, B, grids, window_type="gaussian", finder=3, smoothing=True,
             u = infill_nearest(u, sp_, exclude=m)
         elif method in ("harmonic", "biharmonic"):
             u = infill_ls(u, sp_, 1 if method == "harmonic" else 2, exclude=m)
+        elif method == "knn":
+            u = infill_knn(u, sp_, exclude=m)
         if not final:
             src = fill_masked_predictor(u, m, sp_, predictor_fill)
             prev_pad = apply_predictor_bcs(pad_edge(src, g.n_pre, g.n_post), g.ctrs_y_all, A.shape[1],

@@ -358,3 +358,130 @@ extern "C" int pivamd_infill_ls(const float *u, const float *mask, int N, int ny
                            (double)px, uo, d_work, d_err);
     return hipGetLastError() == hipSuccess ? 0 : PIVAMD_ERR_LAUNCH;
 }
+
+// ---- k-nearest-neighbour regression infill --------------------------------
+// infilling.py:609-695 (infill_knn): each NaN vector (not excluded) becomes
+// the k-nearest-neighbour regression of the valid vectors in physical
+// coordinates (x * px, y * py), weights 1 / distance ('distance') or equal
+// ('uniform'), k = min(n_neighbors, valid donors) -- the prediction of
+// sklearn's KNeighborsRegressor, in FP64, stored as float32.  Neighbours at
+// equal distance are taken in row-major order (a tie at the k-th neighbour
+// is where a KD-tree may choose another donor of the same distance).  The
+// search grows square rings around the cell and stops once the ring cannot
+// hold anything nearer than the current k-th neighbour.  One lane per cell.
+namespace pivamd {
+
+constexpr int kKnnMax = 128;
+
+__global__ void knn_fill_kernel(const float *u, const float *excl, int N, int ny, int nx, float py, float px,
+                                int k, int uniform, int n_donor_min, const int *n_donors, float *out, int *err)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long per = (long long)ny * nx, total = (long long)N * per;
+    if (i >= total) return;
+    const int n = (int)(i / per), w = (int)(i - (long long)n * per);
+    const float *uy = u + (size_t)n * per, *ux = u + (size_t)total + (size_t)n * per;
+    float *oy = out + (size_t)n * per, *ox = out + (size_t)total + (size_t)n * per;
+    const float y0 = uy[w], x0 = ux[w];
+    const bool ex = excl && excl[w] != 0.0f;
+    const bool hole = !ex && !(y0 == y0 && x0 == x0);
+    oy[w] = y0;
+    ox[w] = x0;
+    if (!hole) return;
+    const int nd = n_donors[n];
+    if (nd < n_donor_min) {
+        atomicOr(err, 1);
+        return;
+    }
+    const int kk = k < nd ? k : nd;
+    double bd[kKnnMax];
+    int bi[kKnnMax];
+    int cnt = 0;
+    const int r0 = w / nx, c0 = w % nx;
+    const double dpy = (double)py, dpx = (double)px;
+    const double step = dpy < dpx ? dpy : dpx;
+    const int rmax = ny > nx ? ny : nx;
+    for (int rad = 1; rad <= rmax; ++rad) {
+        // nothing in this ring is nearer than rad * min(py, px)
+        if (cnt == kk && (double)rad * step > bd[kk - 1]) break;
+        for (int dr = -rad; dr <= rad; ++dr) {
+            const int rr = r0 + dr;
+            if (rr < 0 || rr >= ny) continue;
+            const bool edge = dr == -rad || dr == rad;
+            for (int dc = -rad; dc <= rad; dc += edge ? 1 : 2 * rad) {
+                const int cc = c0 + dc;
+                if (cc < 0 || cc >= nx) continue;
+                const int q = rr * nx + cc;
+                if (excl && excl[q] != 0.0f) continue;
+                const float vy = uy[q], vx = ux[q];
+                if (!(vy == vy && vx == vx) || isinf(vy) || isinf(vx)) continue;
+                const double ey = (double)dr * dpy, ex2 = (double)dc * dpx;
+                const double d = sqrt(ex2 * ex2 + ey * ey);
+                if (cnt == kk && (d > bd[kk - 1] || (d == bd[kk - 1] && q > bi[kk - 1]))) continue;
+                // insert (d, q), ordered by distance then row-major index
+                int pos = cnt < kk ? cnt : kk - 1;
+                while (pos > 0 && (bd[pos - 1] > d || (bd[pos - 1] == d && bi[pos - 1] > q))) {
+                    bd[pos] = bd[pos - 1];
+                    bi[pos] = bi[pos - 1];
+                    --pos;
+                }
+                bd[pos] = d;
+                bi[pos] = q;
+                if (cnt < kk) ++cnt;
+            }
+        }
+    }
+    double sy = 0.0, sx = 0.0, sw = 0.0;
+    for (int j = 0; j < cnt; ++j) {
+        const double wt = uniform ? 1.0 : 1.0 / bd[j];
+        sy += (double)uy[bi[j]] * wt;
+        sx += (double)ux[bi[j]] * wt;
+        sw += wt;
+    }
+    oy[w] = (float)(sy / sw);
+    ox[w] = (float)(sx / sw);
+}
+
+// Valid donors per image (finite, not excluded).
+__global__ void knn_count_kernel(const float *u, const float *excl, int N, int per, int *n_donors)
+{
+    const int n = blockIdx.x;
+    const long long total = (long long)N * per;
+    int c = 0;
+    for (int w = threadIdx.x; w < per; w += blockDim.x) {
+        const float a = u[(size_t)n * per + w], b = u[(size_t)total + (size_t)n * per + w];
+        c += (!(excl && excl[w] != 0.0f) && isfinite(a) && isfinite(b)) ? 1 : 0;
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) c += __shfl_xor(c, o, 64);
+    __shared__ int s[16];
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) t += s[k];
+        n_donors[n] = t;
+    }
+}
+
+}  // namespace pivamd
+
+// d_work: N ints.  *d_err |= 1 when an image has holes and no donor.
+extern "C" unsigned long pivamd_infill_knn_workspace(int N) { return (unsigned long)(N > 0 ? N : 1) * sizeof(int); }
+
+extern "C" int pivamd_infill_knn(const float *u, const float *d_excl, int N, int ny, int nx, float py, float px,
+                                 int n_neighbors, int uniform, float *out, void *d_work, unsigned long work_bytes,
+                                 int *d_err, hipStream_t st)
+{
+    using namespace pivamd;
+    if (N < 0 || ny <= 0 || nx <= 0 || !u || !out || !d_err || !d_work) return PIVAMD_ERR_ARG;
+    if (n_neighbors < 1 || n_neighbors > kKnnMax) return PIVAMD_ERR_ARG;
+    if (work_bytes < pivamd_infill_knn_workspace(N)) return PIVAMD_ERR_ARG;
+    if (N == 0) return 0;
+    int *nd = (int *)d_work;
+    hipLaunchKernelGGL(knn_count_kernel, dim3(N), dim3(1024), 0, st, u, d_excl, N, ny * nx, nd);
+    const long long total = (long long)N * ny * nx;
+    hipLaunchKernelGGL(knn_fill_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, u, d_excl, N, ny,
+                       nx, py, px, n_neighbors, uniform, 1, nd, out, d_err);
+    return hipGetLastError() == hipSuccess ? 0 : PIVAMD_ERR_LAUNCH;
+}

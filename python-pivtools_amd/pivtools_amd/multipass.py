@@ -105,6 +105,30 @@ class _PassBuf:
     bc_rows: list = field(default_factory=list)   # (row slice of the padded grid, uy, ux)
 
 
+def _infill_cfg(cfg, final):
+    """(method, parameters) of an infill option: a name, or the reference's
+    config form {"method": ..., "parameters": {...}} (infilling.py:698-790).
+    griddata_linear and rbf_local are not built on the device."""
+    if isinstance(cfg, dict):
+        method, params = str(cfg.get("method", "nearest")).lower(), dict(cfg.get("parameters", {}) or {})
+        if final and not cfg.get("enabled", True):
+            method = "none"
+    else:
+        method, params = str(cfg).lower(), {}
+    allowed = ("nearest", "harmonic", "biharmonic", "knn") + (("none",) if final else ())
+    if method not in allowed:
+        if method in ("griddata_linear", "rbf_local"):
+            raise NotImplementedError(f"infill '{method}' is not on the device path")
+        raise ValueError(f"infill method {method!r}: one of {allowed}")
+    if method == "knn":
+        k = int(params.get("n_neighbors", 32))
+        if not 1 <= k <= 128:
+            raise ValueError(f"knn n_neighbors {k}: 1..128 on the device")
+        if params.get("weights", "distance") not in ("distance", "uniform"):
+            raise ValueError(f"knn weights {params.get('weights')!r}: distance or uniform")
+    return method, params
+
+
 class MultipassPIV:
     """Multipass instantaneous PIV for batches of pairs of one image shape."""
 
@@ -191,12 +215,10 @@ class MultipassPIV:
             raise ValueError(f"size must be an odd integer >= 3, got {self.outliers['size']!r}")
         # infilling.py:698-790 apply_infilling: the device path has the
         # nearest-donor and the least-squares (harmonic, biharmonic) methods
-        if final_infill not in ("nearest", "harmonic", "biharmonic", "none"):
-            raise NotImplementedError(f"final-pass infill '{final_infill}' is not on the device path")
-        if mid_infill not in ("nearest", "harmonic", "biharmonic"):
-            raise NotImplementedError(f"mid-pass infill '{mid_infill}' is not on the device path")
-        self.final_infill = final_infill
-        self.mid_infill = mid_infill
+        # infilling.py:698-790 apply_infilling: a method name or the config
+        # form {"method": ..., "parameters": {...}}
+        self.final_infill, self.final_infill_params = _infill_cfg(final_infill, final=True)
+        self.mid_infill, self.mid_infill_params = _infill_cfg(mid_infill, final=False)
         self.device = torch.device(device)
         self.lib = _native.core()
         for win in self.window_sizes:
@@ -330,6 +352,8 @@ class MultipassPIV:
             nyp, nxp = len(g.ctrs_y_all), len(g.ctrs_x_all)
             for k in ("pad", "pad_s", "pad_t"):
                 add(k, (2, n, nyp, nxp))
+            if (self.final_infill if final else self.mid_infill) == "knn":
+                add("knn_work", (max(4, self.lib.pivamd_infill_knn_workspace(n)),), u8)
             if (self.final_infill if final else self.mid_infill) in ("harmonic", "biharmonic"):
                 add("ls_work", (max(4, self.lib.pivamd_infill_ls_workspace(n, ny, nx)),), u8)
             add("work", (max(4, self.lib.pivamd_xcorr_workspace(n, ny, nx, g.win[0], g.win[1], P, 0)),), u8)
@@ -569,6 +593,15 @@ class MultipassPIV:
             _native.check(L.pivamd_infill_nearest(_ptr(u), _ptr(b.mask), n, ny, nx, float(g.spacing[0]),
                                                   float(g.spacing[1]), _ptr(o["u3"]), _ptr(o["err"]), st),
                           "infill")
+            u = o["u3"]
+        elif method == "knn":
+            o["err"].zero_()
+            kp = self.final_infill_params if final else self.mid_infill_params
+            wk = o["knn_work"]
+            _native.check(L.pivamd_infill_knn(_ptr(u), _ptr(b.mask), n, ny, nx, float(g.spacing[0]),
+                                              float(g.spacing[1]), int(kp.get("n_neighbors", 32)),
+                                              1 if kp.get("weights", "distance") == "uniform" else 0,
+                                              _ptr(o["u3"]), _ptr(wk), wk.numel(), _ptr(o["err"]), st), "infill")
             u = o["u3"]
         elif method in ("harmonic", "biharmonic"):
             o["err"].zero_()

@@ -1056,3 +1056,77 @@ def test_ragged_launch_equals_separate_runs(sched):
         graph.replay()
         torch.cuda.synchronize()
         same(gres)
+
+
+@pytest.mark.parametrize("weights", ["distance", "uniform"])
+@pytest.mark.parametrize("spacing", [(16.0, 16.0), (8.0, 12.0)])
+def test_knn_infill_matches_oracle(weights, spacing):
+    """The device k-NN infill equals the oracle's (donors by distance, then
+    row-major index; FP64 sums) bit for bit, with holes, an excluded block,
+    k below and above the donor count; an image without donors flags the
+    error word."""
+    L = _lib()
+    rng = np.random.default_rng(61)
+    N, ny, nx = 3, 19, 26
+    u = rng.normal(size=(2, N, ny, nx)).astype(np.float32)
+    holes = rng.uniform(size=(N, ny, nx)) < 0.2
+    holes[1, 5:12, 3:14] = True
+    u[:, holes] = np.nan
+    ex = np.zeros((ny, nx), np.float32)
+    ex[2:4, 20:24] = 1.0
+    du, dex = _t(u), _t(ex)
+    wk = torch.empty(L.pivamd_infill_knn_workspace(N), dtype=torch.uint8, device=dev)
+    err = torch.zeros(1, dtype=torch.int32, device=dev)
+    for k in (1, 6, 32, 128):
+        out = torch.empty_like(du)
+        assert L.pivamd_infill_knn(du.data_ptr(), dex.data_ptr(), N, ny, nx, spacing[0], spacing[1], k,
+                                   1 if weights == "uniform" else 0, out.data_ptr(), wk.data_ptr(), wk.numel(),
+                                   err.data_ptr(), _st()) == 0
+        ref = M.infill_knn(u, spacing, k=k, weights=weights, exclude=ex.astype(bool))
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(out.cpu().numpy(), ref)
+        assert int(err.item()) == 0
+    v = np.full((2, 1, ny, nx), np.nan, np.float32)
+    dv = _t(v)
+    out = torch.empty_like(dv)
+    assert L.pivamd_infill_knn(dv.data_ptr(), None, 1, ny, nx, 16.0, 16.0, 8, 0, out.data_ptr(), wk.data_ptr(),
+                               wk.numel(), err.data_ptr(), _st()) == 0
+    torch.cuda.synchronize()
+    assert int(err.item()) & 1
+
+
+def test_loop_with_knn_infill_matches_oracle_loop():
+    """The pass loop with k-NN infill on every pass (the config form
+    {"method": "knn", "parameters": {"n_neighbors": 16}} at the end):
+    identical NaN masks and reasons, RMS < 0.005 px against the oracle loop."""
+    shape = (192, 256)
+    sched = [(64, 64), (32, 32), (16, 16)]
+    A, B = [], []
+    for k in range(2):
+        rng = np.random.default_rng(980 + k)
+        x, y, inten = S._positions(rng, shape, 0.03)
+        dx, dy = S.lamb_oseen(x, y, shape, rc=40.0, peak=3.0, uniform=(1.5, 0.0))
+        A.append(S.render(shape, x, y, inten).numpy())
+        B.append(S.render(shape, x + dx, y + dy, inten).numpy())
+    A = np.stack(A).astype(np.float32)
+    B = np.stack(B).astype(np.float32)
+    mp = MultipassPIV(shape, sched, [50, 50, 50], warp_arithmetic="exact", mid_infill="knn",
+                      final_infill={"method": "knn", "parameters": {"n_neighbors": 16}})
+    res = mp.run(_t(A), _t(B), keep_passes=True)
+    torch.cuda.synchronize()
+    mp.check_infill()
+    grids = build_pass_grids(shape, sched, [50, 50, 50])
+    import multipass as MM
+    orig = MM.infill_knn
+    MM.infill_knn = lambda u, sp, exclude=None: orig(u, sp, k=16 if u.shape[2] == grids[-1].n_win[0] else 32,
+                                                      exclude=exclude)
+    try:
+        ref = M.run_multipass(A, B, grids, keep_passes=True, mid_infill="knn", final_infill="knn")
+    finally:
+        MM.infill_knn = orig
+    for p, (g, r) in enumerate(zip(res, ref)):
+        gm = g.nan_mask.cpu().numpy().astype(bool)
+        np.testing.assert_array_equal(gm, r["nan_mask"])
+        np.testing.assert_array_equal(g.nan_reason.cpu().numpy(), r["nan_reason"])
+        d = np.hypot(g.ux.cpu().numpy() - r["ux"], g.uy.cpu().numpy() - r["uy"])
+        assert np.sqrt(np.mean(d ** 2)) < 0.005, (p, d.max())

@@ -602,3 +602,37 @@ def test_oracle_loop_converges_on_poiseuille_flow():
         ex, ey = P.errors(ux, uy, ~nm & np.isfinite(ux))
         assert ex < 0.3 and ey < 0.15, (p, ex, ey)
     assert ex < 0.05
+
+
+@pytest.mark.parametrize("weights", ["distance", "uniform"])
+@pytest.mark.parametrize("spacing", [(16.0, 16.0), (8.0, 12.0)])
+def test_knn_infill_oracle_against_sklearn(weights, spacing):
+    """The oracle's k-NN infill (donors ranked by distance, then row-major
+    index) equals sklearn's KNeighborsRegressor -- the library the reference
+    calls (infilling.py:609-695) -- at every hole whose k-th and (k+1)-th
+    donor distances differ (where they tie, a KD-tree may keep another donor
+    of the same distance); and a linear field is reproduced by neither
+    weighting only approximately, so the values are checked, not the field."""
+    rng = np.random.default_rng(31)
+    N, ny, nx = 2, 17, 23
+    u = rng.normal(size=(2, N, ny, nx)).astype(np.float32)
+    holes = rng.uniform(size=(N, ny, nx)) < 0.15
+    holes[0, 0:4, 0:5] = True
+    u[:, holes] = np.nan
+    ex = np.zeros((ny, nx), bool)
+    ex[8, 10:13] = True
+    for k in (1, 5, 32):
+        a = M.infill_knn(u, spacing, k=k, weights=weights, exclude=ex, impl="brute")
+        b = M.infill_knn(u, spacing, k=k, weights=weights, exclude=ex, impl="sklearn")
+        yy, xx = np.mgrid[0:ny, 0:nx]
+        for i in range(N):
+            fill = (np.isnan(u[0, i]) | np.isnan(u[1, i])) & ~ex
+            donor = ~fill & ~ex
+            dpts = np.c_[(xx * spacing[1])[donor], (yy * spacing[0])[donor]]
+            for r, c in zip(*np.nonzero(fill)):
+                d = np.sort(np.hypot(dpts[:, 0] - c * spacing[1], dpts[:, 1] - r * spacing[0]))
+                if k < len(d) and d[k - 1] == d[k]:
+                    continue
+                np.testing.assert_allclose(a[:, i, r, c], b[:, i, r, c], rtol=1e-6, atol=1e-7)
+            assert np.isfinite(a[:, i][:, fill]).all()
+            np.testing.assert_array_equal(a[:, i][:, ~fill], u[:, i][:, ~fill])
