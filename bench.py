@@ -363,7 +363,7 @@ def _peak_locations(lib, planes, pw, finder):
     return loc[:n], loc[n:2 * n]
 
 
-def mixed_accuracy(pairs, finder, log):
+def mixed_accuracy(pairs, finder, log, forms=("fft", "split")):
     """Correlation-plane relative L2 error and sub-pixel peak RMS error of the
     FP32 FFT and the split-fp16 MFMA DFT against FP64 planes, per window size
     and image size, on one pair of each size (first pass: unwarped frames)."""
@@ -379,7 +379,7 @@ def mixed_accuracy(pairs, finder, log):
         for win in MIXED_SCHEDULE:
             n = win[0]
             got = {}
-            for dft in ("fft", "split"):
+            for dft in forms:
                 mp = MultipassPIV(shape, [win], [50], peak_finder=finder, dft=dft, dump_correlation_planes=True)
                 res = mp.run(a[None], b[None], keep_passes=True)
                 got[dft] = res[0].debug_dump["planes"][0]
@@ -440,7 +440,9 @@ def run_mixed(args):
 
     legs = {}
     finals = {}
-    for dft in ("fft", "split"):
+    if "fft" not in args.mixed_dft:
+        args.mixed_dft = ["fft"] + list(args.mixed_dft)
+    for dft in args.mixed_dft:
         mps = [MultipassPIV(shape, MIXED_SCHEDULE, [50] * 4, peak_finder=args.finder, dft=dft,
                             mid_infill=args.infill, final_infill=args.infill, warp_arithmetic=args.warp)
                for shape, _, _ in groups]
@@ -529,16 +531,21 @@ def run_mixed(args):
     del mps, items
     torch.cuda.empty_cache()
 
-    # the whole loop: final-pass vectors of the split form against the FFT form
+    # the whole loop: final-pass vectors of each MFMA form against the FFT form
     loop = []
     for gi, (shape, _, _) in enumerate(groups):
-        f, s = finals["fft"][gi], finals["split"][gi]
-        ok = (f.nan_mask == 0) & (s.nan_mask == 0)
-        d2 = (f.ux - s.ux)[ok].double() ** 2 + (f.uy - s.uy)[ok].double() ** 2
-        loop.append({"image": f"{shape[0]}x{shape[1]}", "vectors": int(ok.sum()),
-                     "split_vs_fft_rms_px": float(f"{float(torch.sqrt(d2.mean())):.3e}")})
-    log(f"[bench] loop split vs fft: {loop}")
-    acc = mixed_accuracy([(shape, (A[0], B[0])) for shape, A, B in groups], args.finder, log)
+        row = {"image": f"{shape[0]}x{shape[1]}"}
+        for form in args.mixed_dft:
+            if form == "fft":
+                continue
+            f, s = finals["fft"][gi], finals[form][gi]
+            ok = (f.nan_mask == 0) & (s.nan_mask == 0)
+            d2 = (f.ux - s.ux)[ok].double() ** 2 + (f.uy - s.uy)[ok].double() ** 2
+            row["vectors"] = int(ok.sum())
+            row[f"{form}_vs_fft_rms_px"] = float(f"{float(torch.sqrt(d2.mean())):.3e}")
+        loop.append(row)
+    log(f"[bench] loop forms vs fft: {loop}")
+    acc = mixed_accuracy([(shape, (A[0], B[0])) for shape, A, B in groups], args.finder, log, args.mixed_dft)
 
     fft = legs["fft"]
     dom = max(fft["passes"], key=lambda d: d["ms"])
@@ -565,7 +572,7 @@ def run_mixed(args):
         "latency_ms_p50": fft["latency_ms_p50"],
         "latency_ms_p99": fft["latency_ms_p99"],
         "fft": fft,
-        "split": legs["split"],
+        **{form: legs[form] for form in args.mixed_dft if form != "fft"},
         "ragged": legs["ragged"],
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
@@ -729,8 +736,9 @@ def main():
     ap.add_argument("--pairs", type=int, default=64)
     ap.add_argument("--finder", default="gauss3")
     ap.add_argument("--impl", default="scalar")
-    ap.add_argument("--dft", choices=["fft", "split"], default="fft",
-                    help="split: split-fp16 MFMA DFT for the 16/32/48/64 passes")
+    ap.add_argument("--dft", choices=["fft", "split", "split-bf16", "split-bf16x3"], default="fft",
+                    help="split: split-fp16 MFMA DFT for the 16/32/48/64 passes; split-bf16(x3): split-bf16 "
+                         "MFMA DFT for the 16-point pass")
     ap.add_argument("--cpu-budget", type=float, default=60.0, help="0 disables the CPU baseline leg")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="keep the default correlation-kernel variants instead of timing them first")
@@ -746,6 +754,9 @@ def main():
                     help="after the timed region, time the loop fed from pinned host memory, 16-bit frames "
                          "and Mono12p payloads (0 disables); reported beside value, never as value")
     ap.add_argument("--workload", choices=["multipass", "ensemble", "serving", "mixed"], default="multipass")
+    ap.add_argument("--mixed-dft", nargs="+", default=["fft", "split"],
+                    choices=["fft", "split", "split-bf16", "split-bf16x3"],
+                    help="correlation forms the mixed workload times (the first is the reported one)")
     ap.add_argument("--mixed-pairs", type=int, default=256,
                     help="mixed workload: pairs per step, half 1024x1024 and half 2048x2048")
     ap.add_argument("--serve-rates", type=float, nargs="+", default=[50, 200, 500, 1000, 2000])
@@ -930,8 +941,14 @@ def _xcorr_kernel_name(n: int, dft: str) -> str:
     from pivtools_amd import autotune
 
     v = autotune.get_variant(n) or autotune.default_variant(n)
-    if (dft != "fft" and n in (16, 32, 48, 64)) or v == autotune.MFMA:
+    if dft != "fft" and n == 16:
+        return f"xcorr_gemm16_kernel<{ {'split': 0, 'split-bf16': 1, 'split-bf16x3': 2}[dft]}>"
+    if (dft == "split" and n in (32, 48, 64)) or v == autotune.MFMA:
         return f"xcorr_mfma_kernel<{n}>"
+    if autotune.GEMM <= v <= autotune.GEMM + 2:
+        return f"xcorr_gemm16_kernel<{v - autotune.GEMM}>"
+    if v == autotune.HALF + 1:
+        return f"xcorr_kernel_half1<{n}>"
     if v & autotune.HALF:
         return f"xcorr_kernel_half<{n},{v & ~autotune.HALF}>"
     if v & autotune.PERSIST:

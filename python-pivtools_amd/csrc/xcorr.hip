@@ -46,8 +46,8 @@ struct XcorrArgs {
     float *rec, *work_planes;   // workspace: per-window peak records, multi-peak planes
     long long total;        // N_images * nWin
     int H, W, nWy, nWx, nPeaks, finder, impl, ensemble, nblocks;
-    int dft;                // 1: split-fp16 MFMA DFT (N = 16, 32, 48, 64), 0: FP32 FFT
-    int variant;            // FFT lanes per line R (0: default), kVariantMfma: the MFMA form
+    int dft;                // 1: a split-precision MFMA DFT form, 0: FP32 FFT
+    int variant;            // FFT lanes per line R (0: default), or a kVariant* form
 };
 
 constexpr int kMaxLds = 158 * 1024;
@@ -98,6 +98,7 @@ constexpr bool xc_radix_built(int N, int R)
 constexpr int kVariantMfma = 16;
 constexpr int kVariantPersist = 32;
 constexpr int kVariantHalf = 64;     // + R: xcorr_kernel_half (one real LDS plane per window)
+constexpr int kVariantGemm = 96;     // + 0/1/2: N = 16 register-chained MFMA DFT, fp16 / bf16 / bf16 x3
 
 // Tile layout of the complex stages.  R = 1: element (r, c) at r*(N+1) + c.
 // R > 1: at r*TPX + SKEW*(r/N1) + c + PADC*(c/N1), i.e. rows padded to TPX,
@@ -2196,6 +2197,294 @@ static int launch_mfma(XcorrArgs g, hipStream_t st)
     hipLaunchKernelGGL((xcorr_mfma_kernel<N>), dim3(grid), dim3(C::BLOCK), 0, st, g);
     return hipGetLastError() == hipSuccess ? 0 : PIVAMD_ERR_LAUNCH;
 }
+
+#if PIVAMD_XCORR_SIZE == 16
+// ---------------------------------------------------------------------------
+// The 16-point correlation as register-chained 16x16x32 MFMA products.  One
+// wave per window; the DFT matrix fragments stay in registers, the window
+// batch is the grid, and no workgroup barrier or LDS staging sits between
+// the products:
+//   T   = Z F^T                  A = window (z = a + i b), B = constant
+//   Q   = F T                    A = constant,  B = T (the C registers)
+//   C   = FB conj(FA)            FA, FB from Q(k), Q(-k) by lane permutes
+//   U   = conj(F) C              A = constant,  B = C (the C registers)
+//   P^T = Re[conj(F) U^T]        B = U^T through a wave-private LDS transpose
+// The output tile of v_mfma_f32_16x16x32 puts rows 4g..4g+3 of column c in
+// lane 16g + c, and its B operand takes k-slots 8g..8g+7 in that lane.  Giving
+// the k-slots of lane group g the order (real rows 4g..4g+3, imaginary rows
+// 4g..4g+3), with the constant operand's slots in the same order, turns one
+// complex product into a single K = 32 MFMA per output part and split term,
+// fed straight from the previous product's accumulators.
+//
+// Precision forms (operands x = x0 + x1 [+ x2], FP32 accumulation, smaller
+// terms first):
+//   kGemmF16:    fp16 hi/lo, x0 y0 + x0 y1 + x1 y0 (about 22 bits; the
+//                power-of-two scales of the staged MFMA form keep fp16 in range)
+//   kGemmBf16:   bf16 hi/lo, the same three products (about 16 bits)
+//   kGemmBf16x3: bf16 hi/mid/lo, the six products of order <= 2 (about 24 bits)
+// ---------------------------------------------------------------------------
+constexpr int kGemmF16 = 0, kGemmBf16 = 1, kGemmBf16x3 = 2;
+constexpr int kGemmWindowsPerWave = 4;    // windows per wave: the constants are built once, and the
+                                          // epilogue runs 16 lanes per window
+constexpr int kGemmWaves = 4;             // waves per workgroup
+constexpr int kGemmTP = 20;               // transpose pitch: conflict-free writes and b128 reads
+
+typedef __bf16 gm_b8 __attribute__((ext_vector_type(8)));
+
+template <int P>
+struct GmT {
+    using E = _Float16;
+    using V = mf_h8;
+    static constexpr int NT = 2;
+};
+template <>
+struct GmT<kGemmBf16> {
+    using E = __bf16;
+    using V = gm_b8;
+    static constexpr int NT = 2;
+};
+template <>
+struct GmT<kGemmBf16x3> {
+    using E = __bf16;
+    using V = gm_b8;
+    static constexpr int NT = 3;
+};
+
+template <int P>
+__device__ __forceinline__ mf_f4 gm_mma(const typename GmT<P>::V &a, const typename GmT<P>::V &b, mf_f4 c)
+{
+    if constexpr (P == kGemmF16) return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+    else return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// Split operand: NT parts of one 8-slot fragment.
+template <int P>
+struct GmOp {
+    typename GmT<P>::V p[GmT<P>::NT];
+};
+
+// slot v of the split of x (FP32 value)
+template <int P>
+__device__ __forceinline__ void gm_split(float x, GmOp<P> &o, int v)
+{
+    using E = typename GmT<P>::E;
+    const E h = (E)x;
+    o.p[0][v] = h;
+    const float r = x - (float)h;
+    const E m = (E)r;
+    o.p[1][v] = m;
+    if constexpr (GmT<P>::NT == 3) o.p[2][v] = (E)(r - (float)m);
+}
+
+// A B over the split parts, products of order <= NT - 1, smaller terms first.
+template <int P>
+__device__ __forceinline__ mf_f4 gm_prod(const GmOp<P> &a, const GmOp<P> &b)
+{
+    mf_f4 c = {0.0f, 0.0f, 0.0f, 0.0f};
+    if constexpr (GmT<P>::NT == 3) {
+        c = gm_mma<P>(a.p[0], b.p[2], c);
+        c = gm_mma<P>(a.p[1], b.p[1], c);
+        c = gm_mma<P>(a.p[2], b.p[0], c);
+    }
+    c = gm_mma<P>(a.p[0], b.p[1], c);
+    c = gm_mma<P>(a.p[1], b.p[0], c);
+    c = gm_mma<P>(a.p[0], b.p[0], c);
+    return c;
+}
+
+// cos(2 pi t / 16), t = 0..15, rounded to float: the split parts of every
+// constant (at most 24 significant bits in all) come from these values
+__constant__ float gm_cos16[16] = {
+    1.0f, 0.92387953f, 0.70710678f, 0.38268343f, 0.0f, -0.38268343f, -0.70710678f, -0.92387953f,
+    -1.0f, -0.92387953f, -0.70710678f, -0.38268343f, 0.0f, 0.38268343f, 0.70710678f, 0.92387953f};
+
+// Constant fragment of lane (g, c) for the blocks [X | Y] (slots j < 4: X at
+// column 4g + j, j >= 4: Y at column 4g + j - 4), F[m][k] = exp(-2 pi i mk/16).
+// sel: 0 = [Fr | -Fi], 1 = [Fi | Fr], 2 = [Fr | Fi], 3 = [-Fi | Fr].  F is
+// symmetric, so the same fragment serves as the A operand (row c) and as the B
+// operand (column c).  Every entry is a cosine at a shifted angle: Fr(t) =
+// cos(t), -Fi(t) = sin(t) = cos(t - 4), Fi(t) = cos(t + 4) in units of 2 pi / 16.
+template <int P>
+__device__ __forceinline__ void gm_const(GmOp<P> &o, int sel, int g, int c)
+{
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+        const int t = c * (4 * g + (v & 3));
+        int sh;
+        if (v < 4) sh = (sel == 0 || sel == 2) ? 0 : sel == 1 ? 4 : 12;
+        else sh = (sel == 1 || sel == 3) ? 0 : sel == 0 ? 12 : 4;
+        gm_split<P>(gm_cos16[(t + sh) & 15], o, v);
+    }
+}
+
+template <int P>
+__global__ void __launch_bounds__(64 * kGemmWaves) xcorr_gemm16_kernel(XcorrArgs g)
+{
+    constexpr int N = 16, NN = 256, PX = N + 1, TP = kGemmTP, WV = kGemmWindowsPerWave;
+    constexpr int PT = N * PX;                            // one plane tile
+    // per wave: the U^T transpose (2 x 16 x TP) and the planes of its WV windows
+    __shared__ __attribute__((aligned(16))) float s_t[kGemmWaves][2 * N * TP + WV * PT];
+
+    const int b = remap_block((int)blockIdx.x, g.nblocks);
+    if (b < 0) return;
+    const int wv = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
+    const int gq = lane >> 4, cl = lane & 15;
+    float *S = s_t[wv];
+    float *Pls = s_t[wv] + 2 * N * TP;
+
+    GmOp<P> K1, K2, K3, K4;
+    gm_const<P>(K1, 0, gq, cl);
+    gm_const<P>(K2, 1, gq, cl);
+    gm_const<P>(K3, 2, gq, cl);
+    gm_const<P>(K4, 3, gq, cl);
+    // lanes holding Q(-k): rows (16 - 4g - j) & 15 of column (16 - c) & 15
+    const int mc = (16 - cl) & 15;
+    const int src0 = 16 * ((4 - gq) & 3) + mc, src1 = 16 * (3 - gq) + mc;
+
+    const int nWin = g.nWy * g.nWx;
+    const long long gw0 = ((long long)b * kGemmWaves + wv) * WV;
+    // the window that lane group gq finishes in the epilogue
+    bool my_valid = false;
+    int my_n = 0, my_w = 0;
+    float my_enorm = 0.0f;
+#pragma unroll
+    for (int it = 0; it < WV; ++it) {
+        const long long gw = gw0 + it;
+        if (gw >= g.total) break;           // wave-uniform; no barrier follows
+        const int n = (int)(gw / nWin);
+        const int w = (int)(gw - (long long)n * nWin);
+        bool valid = true;
+        int row0, col0;
+        {
+            const int ii = w % g.nWx, jj = w / g.nWx;
+            if (g.mask && g.mask[jj * g.nWx + ii] == 1.0f) valid = false;
+            row0 = (int)floor((double)g.ctrY[jj] - ((double)(float)N - 1.0) / 2.0 + 0.5);
+            col0 = (int)floor((double)g.ctrX[ii] - ((double)(float)N - 1.0) / 2.0 + 0.5);
+            if (row0 < 0 || col0 < 0 || row0 + N > g.H || col0 + N > g.W) valid = false;
+        }
+
+        // ---- window: lane (g, c) holds row c, columns 4g..4g+3 of A and B ----
+        float za[4] = {0.0f, 0.0f, 0.0f, 0.0f}, zb[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        const int e0 = cl * N + 4 * gq;
+        if (valid) {
+            const size_t o = (size_t)n * g.H * g.W + (size_t)(row0 + cl) * g.W + col0 + 4 * gq;
+            if (g.imgA16) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    za[j] = (float)g.imgA16[o + j];
+                    zb[j] = (float)g.imgB16[o + j];
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    za[j] = g.imgA[o + j];
+                    zb[j] = g.imgB[o + j];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                za[j] *= g.wA[e0 + j];
+                zb[j] *= g.wB[e0 + j];
+            }
+        }
+        float sa = (za[0] + za[1]) + (za[2] + za[3]), sb = (zb[0] + zb[1]) + (zb[2] + zb[3]);
+        win_sum2<64>(sa, sb, nullptr, nullptr, lane);
+        const float meanA = sa / (float)NN, meanB = sb / (float)NN;
+        float ea = 0.0f, eb = 0.0f;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (valid && !g.ensemble) {
+                za[j] -= meanA;
+                zb[j] -= meanB;
+            }
+            ea += za[j] * za[j];
+            eb += zb[j] * zb[j];
+        }
+        win_sum2<64>(ea, eb, nullptr, nullptr, lane);
+        const float enorm = 1.0f / sqrtf(ea * eb);
+        // the staged MFMA form's scales: window values <= 512, |C| <= 512
+        const float emax = fmaxf(ea, eb);
+        const bool scalable = emax > 0.0f && isfinite(emax);
+        const int ez = scalable ? 8 - ilogbf(sqrtf(emax)) : 0;
+        const bool cross = scalable && ea > 0.0f && eb > 0.0f;
+        const int lc = cross ? (ilogbf(sqrtf(ea)) + 1) + (ilogbf(sqrtf(eb)) + 1) + 2 * ez + 2 * (ilogbf((float)N) + 1)
+                             : 0;
+        const int ec = cross ? 9 - lc : 0;
+
+        GmOp<P> X;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            gm_split<P>(ldexpf(za[j], ez), X, j);
+            gm_split<P>(ldexpf(zb[j], ez), X, 4 + j);
+        }
+        // ---- T = Z F^T; Q = F T ---------------------------------------------
+        const mf_f4 tr = gm_prod<P>(X, K1), ti = gm_prod<P>(X, K2);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            gm_split<P>(tr[j], X, j);
+            gm_split<P>(ti[j], X, 4 + j);
+        }
+        const mf_f4 qr = gm_prod<P>(K1, X), qi = gm_prod<P>(K2, X);
+
+        // ---- C = FB conj(FA), scaled by 2^ec --------------------------------
+        // register j of Q(-k) comes from register (4 - j) & 3 of the partner lane
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int sj = (4 - j) & 3, src = j == 0 ? src0 : src1;
+            const float mR = __shfl(qr[sj], src, 64), mI = __shfl(qi[sj], src, 64);
+            const float zR = qr[j], zI = qi[j];
+            const float ar = 0.5f * (zR + mR), ai = 0.5f * (zI - mI);
+            const float br = 0.5f * (zI + mI), bi = -0.5f * (zR - mR);
+            gm_split<P>(ldexpf(br * ar + bi * ai, ec), X, j);
+            gm_split<P>(ldexpf(bi * ar - br * ai, ec), X, 4 + j);
+        }
+        // ---- U = conj(F) C; U^T through LDS ------------------------------------
+        const mf_f4 ur = gm_prod<P>(K3, X), ui = gm_prod<P>(K4, X);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            S[(4 * gq + j) * TP + cl] = ur[j];
+            S[N * TP + (4 * gq + j) * TP + cl] = ui[j];
+        }
+        // same wave: LDS operations complete in order, no barrier
+        const float4 vr = *(const float4 *)&S[cl * TP + 4 * gq];
+        const float4 vi = *(const float4 *)&S[N * TP + cl * TP + 4 * gq];
+        gm_split<P>(vr.x, X, 0); gm_split<P>(vr.y, X, 1); gm_split<P>(vr.z, X, 2); gm_split<P>(vr.w, X, 3);
+        gm_split<P>(vi.x, X, 4); gm_split<P>(vi.y, X, 5); gm_split<P>(vi.z, X, 6); gm_split<P>(vi.w, X, 7);
+        // ---- P^T = Re[conj(F) U^T]: lane (g, c) holds plane row c, columns 4g + j
+        const mf_f4 pt = gm_prod<P>(K3, X);
+        const int eback = -(2 * ez + ec);
+        const float mul = 1.0f / (float)NN;
+        float *Pl = Pls + it * PT;           // fftshifted plane tile, row stride PX
+        const int ys = (cl + N / 2) & 15;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) Pl[ys * PX + ((4 * gq + j + N / 2) & 15)] = ldexpf(pt[j], eback) * mul;
+        if (gq == it) {
+            my_valid = valid;
+            my_n = n;
+            my_w = w;
+            my_enorm = enorm;
+        }
+    }
+    // ---- epilogue: 16 lanes per window, the wave's windows side by side -----
+    const long long my_gw = gw0 + gq;
+    if (my_gw < g.total)
+        plane_epilogue<N, 16>(g, Pls + gq * PT, PX, my_valid, my_gw, my_n, my_w, my_enorm, cl, nullptr, nullptr,
+                              nullptr, nullptr, nullptr);
+}
+
+template <int P>
+static int launch_gemm16(XcorrArgs g, hipStream_t st)
+{
+    constexpr int per = kGemmWaves * kGemmWindowsPerWave;
+    const long long nb = (g.total + per - 1) / per;
+    if (nb <= 0) return 0;
+    if (nb > (1LL << 30)) return PIVAMD_ERR_ARG;
+    g.nblocks = (int)nb;
+    const int grid = (int)((nb + 7) / 8 * 8);
+    hipLaunchKernelGGL((xcorr_gemm16_kernel<P>), dim3(grid), dim3(64 * kGemmWaves), 0, st, g);
+    return hipGetLastError() == hipSuccess ? 0 : PIVAMD_ERR_LAUNCH;
+}
+#endif
 #endif
 
 // Launch of the correlation kernel for one square size; defined in the
@@ -2364,6 +2653,11 @@ int launch_main<PIVAMD_XCORR_SIZE>(XcorrArgs g, hipStream_t st)
 #if (PIVAMD_XCORR_SIZE % 16 == 0) && (PIVAMD_XCORR_SIZE <= 64)
     case kVariantMfma: return launch_mfma(g, st);
 #endif
+#if PIVAMD_XCORR_SIZE == 16
+    case kVariantGemm + kGemmF16: return launch_gemm16<kGemmF16>(g, st);
+    case kVariantGemm + kGemmBf16: return launch_gemm16<kGemmBf16>(g, st);
+    case kVariantGemm + kGemmBf16x3: return launch_gemm16<kGemmBf16x3>(g, st);
+#endif
     case 1: return launch_fft<N, 1>(g, st);
     case 2: return launch_fft<N, 2>(g, st);
     case 4: return launch_fft<N, 4>(g, st);
@@ -2491,7 +2785,9 @@ extern "C" size_t pivamd_xcorr_workspace(int n_images, int nWy, int nWx, int win
 
 // Transform used by pivamd_xcorr: 0 = FP32 FFT (default, the reference's
 // arithmetic class), 1 = split-fp16 MFMA DFT for 16/32/48/64-point windows
-// (other sizes keep the FFT).  Read when a launch is issued.
+// (16: the register-chained form, 32-64: the LDS-staged form), 2 = split-bf16
+// (hi/lo) and 3 = split-bf16 (hi/mid/lo) register-chained DFT for 16-point
+// windows; other sizes keep the FFT.  Read when a launch is issued.
 static int g_dft_mode = 0;
 
 // Selected variant per window size (index N); 0 = default.
@@ -2502,6 +2798,7 @@ static bool variant_ok(int N, int v)
     if (!pivamd_size_supported(N, N)) return false;
     if (v == 0) return true;
     if (v == kVariantMfma) return N % 16 == 0 && N <= 64;
+    if (v >= kVariantGemm && v <= kVariantGemm + 2) return N == 16;
     if (v == kVariantPersist + 4) return xc_radix_built(N, 4) && N * 4 >= 256;   // WPB = 1
     if (v == kVariantHalf + 4) return N == 48 || N == 64 || N == 96 || N == 128;
     if (v == kVariantHalf + 1) return xc_radix_built(N, 1);
@@ -2510,7 +2807,8 @@ static bool variant_ok(int N, int v)
 
 extern "C" int pivamd_xcorr_variants(int N, int *out, int max_out)
 {
-    static const int cand[] = {1, 2, 4, 8, kVariantMfma, kVariantPersist + 4, kVariantHalf + 1, kVariantHalf + 4};
+    static const int cand[] = {1, 2, 4, 8, kVariantMfma, kVariantPersist + 4, kVariantHalf + 1, kVariantHalf + 4,
+                               kVariantGemm, kVariantGemm + 1, kVariantGemm + 2};
     int n = 0;
     for (int v : cand)
         if (variant_ok(N, v)) {
@@ -2542,7 +2840,7 @@ extern "C" int pivamd_xcorr_get_variant(int N)
 extern "C" int pivamd_set_dft_mode(int mode)
 {
     const int prev = g_dft_mode;
-    if (mode == 0 || mode == 1) g_dft_mode = mode;
+    if (mode >= 0 && mode <= 3) g_dft_mode = mode;
     return prev;
 }
 
@@ -2570,8 +2868,13 @@ static int xcorr_entry(const float *d_imgA, const float *d_imgB, const unsigned 
     g.total = (long long)n_images * nWy * nWx;
     g.H = H; g.W = W; g.nWy = nWy; g.nWx = nWx; g.nPeaks = nPeaks; g.finder = finder;
     g.impl = impl; g.ensemble = ensemble; g.nblocks = 0;
-    g.dft = (g_dft_mode == 1 && winY == winX && winY % 16 == 0 && winY <= 64) ? 1 : 0;
-    g.variant = g.dft ? kVariantMfma : (winY == winX && winY > 0 && winY <= 128 ? g_variant[winY] : 0);
+    int dv = 0;
+    if (winY == winX && g_dft_mode == 1 && winY % 16 == 0 && winY <= 64)
+        dv = winY == 16 ? kVariantGemm : kVariantMfma;
+    else if (winY == winX && winY == 16 && (g_dft_mode == 2 || g_dft_mode == 3))
+        dv = kVariantGemm + g_dft_mode - 1;
+    g.dft = dv != 0 ? 1 : 0;
+    g.variant = dv != 0 ? dv : (winY == winX && winY > 0 && winY <= 128 ? g_variant[winY] : 0);
     if (!ensemble && (!d_pkX || !d_pkY || !d_pkH || !d_sx || !d_sy || !d_sxy || !d_ratio || !d_work))
         return PIVAMD_ERR_ARG;
     if (!pivamd_size_supported(winY, winX)) return PIVAMD_ERR_SIZE;

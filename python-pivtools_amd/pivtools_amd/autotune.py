@@ -7,7 +7,12 @@ the library (``pivamd_xcorr_variants``):
   shared by R lanes.  Larger R means more lanes (and waves) per window, so
   more latency hiding, at the cost of cross-lane exchange and twiddle work;
   it also changes the LDS tile layout and the windows per workgroup.
-* ``16``: the split-precision MFMA DFT form (16, 32, 48 and 64 points).
+* ``16``: the split-precision MFMA DFT form (16, 32, 48 and 64 points),
+  staged through LDS between its four products.
+* ``96``, ``97``, ``98`` (16 points): the split-precision DFT as
+  register-chained MFMA products, one wave per window, with fp16 hi/lo,
+  bf16 hi/lo (about 16 significant bits: selectable, never tuned in) and
+  bf16 hi/mid/lo operands.
 * ``36``: the 4-lane FFT as a persistent kernel (64, 96 and 128 points, one
   window per workgroup) that loads the next window's samples while the
   current one is transformed.
@@ -44,6 +49,8 @@ from .window import compute_window_centers, window_weight
 MFMA = 16
 PERSIST = 32
 HALF = 64
+GEMM = 96                      # + 0 fp16, + 1 bf16, + 2 bf16 x3 (16 points)
+MFMA_FORMS = (MFMA, GEMM, GEMM + 1, GEMM + 2)
 _TARGET_WINDOWS = {8: 262144, 12: 262144, 16: 262144, 24: 196608, 32: 131072, 48: 65536, 64: 49152,
                    96: 20480, 128: 16384}
 
@@ -51,9 +58,9 @@ _TARGET_WINDOWS = {8: 262144, 12: 262144, 16: 262144, 24: 196608, 32: 131072, 48
 def variants(n: int) -> list[int]:
     """Variant codes built for window size n."""
     L = _native.core()
-    buf = (ctypes.c_int * 8)()
-    k = L.pivamd_xcorr_variants(n, buf, 8)
-    return [int(buf[i]) for i in range(min(k, 8))]
+    buf = (ctypes.c_int * 16)()
+    k = L.pivamd_xcorr_variants(n, buf, 16)
+    return [int(buf[i]) for i in range(min(k, 16))]
 
 
 def default_variant(n: int) -> int:
@@ -194,7 +201,7 @@ def tune_xcorr(sizes, device=None, reps: int = 10, cache: str | None = None, inc
             table[n] = known[str(n)]
             set_variant(n, int(table[n]["variant"]))
             continue
-        cands = [v for v in variants(n) if include_mfma or v != MFMA]
+        cands = [v for v in variants(n) if v != GEMM + 1 and (include_mfma or v not in MFMA_FORMS)]
         dflt = default_variant(n)
         case = _Case(n, dev)
         prev = set_variant(n, dflt)

@@ -129,6 +129,11 @@ def _infill_cfg(cfg, final):
     return method, params
 
 
+# Correlation transform of the passes (pivamd_set_dft_mode): the FP32 FFT, or a
+# split-precision MFMA DFT; sizes a form does not cover keep the FFT.
+DFT_MODES = {"fft": 0, "split": 1, "split-bf16": 2, "split-bf16x3": 3}
+
+
 class MultipassPIV:
     """Multipass instantaneous PIV for batches of pairs of one image shape."""
 
@@ -166,9 +171,10 @@ class MultipassPIV:
         passes the outlier test (cpu_instantaneous.py:664-722).
         """
         self.H, self.W = image_shape
-        if dft not in ("fft", "split"):
-            raise ValueError(f"dft {dft!r}: 'fft' (FP32 FFT) or 'split' (split-fp16 MFMA DFT, N <= 64)")
-        self.dft = 1 if dft == "split" else 0
+        if dft not in DFT_MODES:
+            raise ValueError(f"dft {dft!r}: one of {sorted(DFT_MODES)} ('fft': FP32 FFT; 'split': split-fp16 "
+                             f"MFMA DFT for N <= 64; 'split-bf16', 'split-bf16x3': split-bf16 MFMA DFT for N = 16)")
+        self.dft = DFT_MODES[dft]
         self.window_sizes = [tuple(w) if np.ndim(w) else (w, w) for w in window_sizes]
         if len(overlaps) != len(self.window_sizes):
             raise ValueError("one overlap per pass")
