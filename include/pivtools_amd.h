@@ -392,6 +392,21 @@ unsigned long pivamd_infill_knn_workspace(int N);
 int pivamd_infill_knn(const float *u, const float *d_excl, int N, int ny, int nx, float py, float px,
                       int n_neighbors, int uniform, float *out, void *d_work, unsigned long work_bytes,
                       int *d_err, void *stream);
+/* infilling.py:505-597 (infill_rbf_local, scipy RBFInterpolator with
+ * `neighbors`): each NaN vector (not excluded) is interpolated from its k =
+ * min(neighbors, donors) nearest valid vectors (neighbors <= 128; physical
+ * coordinates; ties at the k-th distance in row-major order) by the kernel
+ * phi (0 linear, 1 thin_plate_spline, 2 cubic, 3 quintic, 4 multiquadric,
+ * 5 inverse_multiquadric, 6 inverse_quadratic, 7 gaussian; epsilon = 1 for
+ * 0-3 as scipy's default) plus scipy's default polynomial degree, smoothing
+ * on the kernel diagonal, solved in FP64 by LU with partial pivoting, stored
+ * as float32.  d_work: pivamd_infill_rbf_workspace(N, ny, nx) bytes;
+ * *d_err |= 1 (fewer than 4 donors in an image with holes), 2 (singular
+ * system), 4 (candidate buffer exceeded by a very anisotropic pitch). */
+unsigned long pivamd_infill_rbf_workspace(int N, int ny, int nx);
+int pivamd_infill_rbf(const float *u, const float *d_excl, int N, int ny, int nx, float py, float px,
+                      int neighbors, int kernel, double epsilon, double smoothing, float *out, void *d_work,
+                      unsigned long work_bytes, int *d_err, void *stream);
 int pivamd_fill_masked_nearest(const float *u, const float *d_mask, int N, int ny, int nx, float py,
                                float px, float *uo, int *d_err, void *stream);
 int pivamd_fill_linear(float *u, int planes, long plane_stride, int nf, const int *d_fill_idx, int nb,

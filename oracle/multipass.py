@@ -383,6 +383,131 @@ def infill_knn(u, spacing, k=32, weights="distance", exclude=None, impl="brute")
     return out
 
 
+# scipy.interpolate.RBFInterpolator's kernels, default polynomial degrees and
+# scale-invariant kernels (scipy 1.15 _rbfinterp.py:20-48; its kernel
+# functions: -r, r^2 log r, r^3, -r^5, -sqrt(r^2 + 1), 1/sqrt(r^2 + 1),
+# 1/(r^2 + 1), exp(-r^2)).
+def _rbf_phi(kernel, r):
+    r = np.asarray(r, np.float64)
+    if kernel == "linear":
+        return -r
+    if kernel == "thin_plate_spline":
+        with np.errstate(divide="ignore", invalid="ignore"):
+            return np.where(r == 0.0, 0.0, r * r * np.log(np.where(r == 0.0, 1.0, r)))
+    if kernel == "cubic":
+        return r ** 3
+    if kernel == "quintic":
+        return -r ** 5
+    if kernel == "multiquadric":
+        return -np.sqrt(r * r + 1.0)
+    if kernel == "inverse_multiquadric":
+        return 1.0 / np.sqrt(r * r + 1.0)
+    if kernel == "inverse_quadratic":
+        return 1.0 / (r * r + 1.0)
+    if kernel == "gaussian":
+        return np.exp(-r * r)
+    raise ValueError(kernel)
+
+
+_RBF_MIN_DEGREE = {"multiquadric": 0, "linear": 0, "thin_plate_spline": 1, "cubic": 1, "quintic": 2}
+_RBF_SCALE_INVARIANT = {"linear", "thin_plate_spline", "cubic", "quintic"}
+
+
+def _rbf_powers(degree):
+    """Monomial exponents (x, y) in scipy's order (_monomial_powers)."""
+    from itertools import combinations_with_replacement
+
+    out = []
+    for deg in range(degree + 1):
+        for mono in combinations_with_replacement(range(2), deg):
+            pw = [0, 0]
+            for var in mono:
+                pw[var] += 1
+            out.append(pw)
+    return np.array(out, dtype=np.int64).reshape(-1, 2)
+
+
+def infill_rbf_local(u, spacing, neighbors=64, kernel="thin_plate_spline", epsilon=None, smoothing=0.0,
+                     exclude=None, impl="brute"):
+    """infill_rbf_local (infilling.py:505-597) per image: NaN vectors (not
+    excluded) become the value of a local RBF interpolant of the valid
+    vectors in physical coordinates (x * px, y * py), one interpolant per
+    component, stored as float32.  impl="scipy" calls RBFInterpolator with
+    `neighbors` as the reference does.  impl="brute" restates it: the k =
+    min(neighbors, donors) nearest donors ranked by (distance, row-major
+    index) -- the rule the device follows where several share the k-th
+    distance -- taken in index order; the system [K(eps r) + s I, P; P^T, 0]
+    over monomials of (y - shift) / scale (shift, scale: mid-point and
+    half-range of the neighbourhood, a zero half-range taken as 1); solved by
+    LAPACK's gesv (numpy.linalg.solve, both components at once); evaluated at
+    the hole."""
+    if kernel not in ("linear", "thin_plate_spline", "cubic", "quintic", "multiquadric",
+                      "inverse_multiquadric", "inverse_quadratic", "gaussian"):
+        raise ValueError(kernel)
+    if epsilon is None:
+        if kernel not in _RBF_SCALE_INVARIANT:
+            raise ValueError("`epsilon` must be specified")
+        epsilon = 1.0
+    eps = float(epsilon)
+    powers = _rbf_powers(max(_RBF_MIN_DEGREE.get(kernel, -1), 0))
+    nm = len(powers)
+    py, px = float(spacing[0]), float(spacing[1])
+    out = u.astype(np.float32).copy()
+    ny, nx = u.shape[2:]
+    yy, xx = np.mgrid[0:ny, 0:nx]
+    X = np.c_[(xx * px).ravel(), (yy * py).ravel()].astype(np.float64)
+    for i in range(u.shape[1]):
+        ux, uy = u[1, i].ravel(), u[0, i].ravel()
+        ex = np.zeros(ny * nx, bool) if exclude is None else np.asarray(exclude, bool).ravel()
+        fill = (np.isnan(ux) | np.isnan(uy)) & ~ex
+        if not fill.any():
+            continue
+        donor = ~fill & ~ex & np.isfinite(ux) & np.isfinite(uy)
+        nd = int(donor.sum())
+        if nd < 4:
+            raise ValueError(f"Infilling method 'rbf_local' cannot run: {nd} valid donor point(s)")
+        Xd, Xq = X[donor], X[fill]
+        du, dv = ux[donor].astype(np.float64), uy[donor].astype(np.float64)
+        if impl == "scipy":
+            from scipy.interpolate import RBFInterpolator
+
+            pu = RBFInterpolator(Xd, du, kernel=kernel, epsilon=epsilon, neighbors=neighbors,
+                                 smoothing=smoothing)(Xq)
+            pv = RBFInterpolator(Xd, dv, kernel=kernel, epsilon=epsilon, neighbors=neighbors,
+                                 smoothing=smoothing)(Xq)
+        else:
+            k = min(int(neighbors), nd)
+            if nm > k:
+                raise ValueError("fewer neighbours than monomials")
+            idx = np.flatnonzero(donor)
+            pu, pv = np.empty(len(Xq)), np.empty(len(Xq))
+            for q, xq in enumerate(Xq):
+                d = np.sqrt(((Xd - xq) ** 2).sum(axis=1))
+                nbr = np.sort(np.lexsort((idx, d))[:k])
+                y = Xd[nbr]
+                mins, maxs = y.min(axis=0), y.max(axis=0)
+                shift, scale = (maxs + mins) / 2, (maxs - mins) / 2
+                scale[scale == 0.0] = 1.0
+                ye = y * eps
+                diff = ye[:, None, :] - ye[None, :, :]
+                lhs = np.zeros((k + nm, k + nm))
+                lhs[:k, :k] = _rbf_phi(kernel, np.sqrt((diff ** 2).sum(axis=-1)))
+                lhs[:k, :k] += np.eye(k) * float(smoothing)
+                pm = np.prod(((y - shift) / scale)[:, None, :] ** powers[None, :, :], axis=-1)
+                lhs[:k, k:] = pm
+                lhs[k:, :k] = pm.T
+                rhs = np.zeros((k + nm, 2))
+                rhs[:k, 0], rhs[:k, 1] = du[nbr], dv[nbr]
+                coef = np.linalg.solve(lhs, rhs)
+                de = xq * eps - ye
+                vec = np.r_[_rbf_phi(kernel, np.sqrt((de ** 2).sum(axis=1))),
+                            np.prod(((xq - shift) / scale)[None, :] ** powers, axis=-1)]
+                pu[q], pv[q] = vec @ coef[:, 0], vec @ coef[:, 1]
+        out[1, i].ravel()[fill] = np.asarray(pu, np.float64).astype(np.float32)
+        out[0, i].ravel()[fill] = np.asarray(pv, np.float64).astype(np.float32)
+    return out
+
+
 def vector_masks_loop(image_shape, window_sizes, overlaps, pixel_mask, threshold=0.5):
     """compute_vector_mask (load_images.py:827-1000, standard mode) as the
     reference's per-window loop."""
@@ -699,6 +824,14 @@ def run_multipass(A, B, grids, window_type="gaussian", finder=3, smoothing=True,
             u = infill_ls(u, sp_, 1 if method == "harmonic" else 2, exclude=m)
         elif method == "knn":
             u = infill_knn(u, sp_, exclude=m)
+        elif method == "rbf_local":
+            u = infill_rbf_local(u, sp_, exclude=m)
+        elif isinstance(method, dict) and method.get("method") == "rbf_local":
+            u = infill_rbf_local(u, sp_, exclude=m, **method.get("parameters", {}))
+        elif isinstance(method, dict) and method.get("method") == "knn":
+            kp = method.get("parameters", {})
+            u = infill_knn(u, sp_, k=kp.get("n_neighbors", 32), weights=kp.get("weights", "distance"),
+                           exclude=m)
         if not final:
             src = fill_masked_predictor(u, m, sp_, predictor_fill)
             prev_pad = apply_predictor_bcs(pad_edge(src, g.n_pre, g.n_post), g.ctrs_y_all, A.shape[1],

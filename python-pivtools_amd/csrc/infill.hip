@@ -485,3 +485,399 @@ extern "C" int pivamd_infill_knn(const float *u, const float *d_excl, int N, int
                        nx, py, px, n_neighbors, uniform, 1, nd, out, d_err);
     return hipGetLastError() == hipSuccess ? 0 : PIVAMD_ERR_LAUNCH;
 }
+
+// ---------------------------------------------------------------------------
+// Local RBF infill (infilling.py:505-597 infill_rbf_local: scipy's
+// RBFInterpolator with `neighbors`, one interpolator per component).  Each
+// hole (a NaN vector, not excluded) is interpolated from its k = min(k, donors)
+// nearest donors in physical coordinates (x = col * px, y = row * py), ranked
+// by distance and then row-major index (a KD-tree may keep another donor
+// where several share the k-th distance):
+//   phi(eps |y_i - y_j|) coefficients plus the monomials of degree <= deg of
+//   yhat = (y - shift) / scale (shift, scale: mid-point and half-range of the
+//   neighbourhood, a zero half-range taken as 1), the donors in ascending
+//   index order, smoothing added to the kernel diagonal:
+//     [ K + s I   P ] [c]   [d]
+//     [ P^T       0 ] [a] = [0]
+// solved by LU with partial pivoting in FP64 (LAPACK's dgesv in the
+// reference), then evaluated at the hole.  One workgroup per hole: the
+// candidates come from a rectangle that must hold the k nearest, the
+// neighbourhood from ranking them, and the (k + m) x (k + m + 2) augmented
+// system sits in LDS.  Kernels and default degrees are scipy's.
+// *err bits: 1 = fewer than 4 donors (the reference raises), or fewer than
+// the monomials; 2 = singular system; 4 = the candidate rectangle exceeds
+// the LDS buffer (very anisotropic pitch).
+// ---------------------------------------------------------------------------
+namespace pivamd {
+
+constexpr int kRbfMaxNbr = 128, kRbfMaxMono = 6;
+constexpr int kRbfThreads = 256;
+
+enum : int { kRbfLinear = 0, kRbfTps, kRbfCubic, kRbfQuintic, kRbfMq, kRbfImq, kRbfIq, kRbfGauss };
+
+__device__ __forceinline__ double rbf_phi(int kern, double r)
+{
+    switch (kern) {
+    case kRbfLinear: return -r;
+    case kRbfTps: return r == 0.0 ? 0.0 : r * r * log(r);
+    case kRbfCubic: return r * r * r;
+    case kRbfQuintic: return -(r * r * r * r * r);
+    case kRbfMq: return -sqrt(r * r + 1.0);
+    case kRbfImq: return 1.0 / sqrt(r * r + 1.0);
+    case kRbfIq: return 1.0 / (r * r + 1.0);
+    default: return exp(-(r * r));
+    }
+}
+
+__host__ __device__ constexpr int rbf_degree(int kern)
+{
+    return kern == kRbfTps || kern == kRbfCubic ? 1 : kern == kRbfQuintic ? 2 : 0;
+}
+
+// monomial m of degree <= 2 in scipy's order: 1, x, y, x^2, x y, y^2
+__device__ __forceinline__ double rbf_mono(int m, double x, double y)
+{
+    switch (m) {
+    case 0: return 1.0;
+    case 1: return x;
+    case 2: return y;
+    case 3: return x * x;
+    case 4: return x * y;
+    default: return y * y;
+    }
+}
+
+// Block-wide sum / min / max / argmax helpers (kRbfThreads lanes, 4 waves).
+__device__ __forceinline__ double rbf_wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+template <int OP>    // 0 sum, 1 min, 2 max
+__device__ double rbf_block_reduce(double v, double *red)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        const double w = __shfl_xor(v, o, 64);
+        v = OP == 0 ? v + w : OP == 1 ? fmin(v, w) : fmax(v, w);
+    }
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double r = red[0];
+#pragma unroll
+    for (int k = 1; k < kRbfThreads / 64; ++k) r = OP == 0 ? r + red[k] : OP == 1 ? fmin(r, red[k]) : fmax(r, red[k]);
+    return r;
+}
+
+// argmax of |v| over the lanes, first (smallest) row on ties; rows < 0 never win
+__device__ int rbf_block_argmax(double v, int row, double *red, int *redi)
+{
+    double a = row >= 0 ? fabs(v) : -1.0;
+    int r = row >= 0 ? row : 0x7fffffff;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        const double a2 = __shfl_xor(a, o, 64);
+        const int r2 = __shfl_xor(r, o, 64);
+        if (a2 > a || (a2 == a && r2 < r)) { a = a2; r = r2; }
+    }
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) {
+        red[threadIdx.x >> 6] = a;
+        redi[threadIdx.x >> 6] = r;
+    }
+    __syncthreads();
+    a = red[0];
+    r = redi[0];
+#pragma unroll
+    for (int k = 1; k < kRbfThreads / 64; ++k)
+        if (red[k] > a || (red[k] == a && redi[k] < r)) { a = red[k]; r = redi[k]; }
+    return a > 0.0 ? r : -1;
+}
+
+__global__ void rbf_prep_kernel(const float *u, const float *excl, int N, int per, float *out, int *hole_count,
+                                int *holes)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long total = (long long)N * per;
+    if (i >= total) return;
+    const int w = (int)(i % per);
+    const float a = u[i], b = u[total + i];
+    out[i] = a;
+    out[total + i] = b;
+    const bool ex = excl && excl[w] != 0.0f;
+    if (!ex && !(a == a && b == b)) holes[atomicAdd(hole_count, 1)] = (int)i;
+}
+
+__global__ void rbf_zero_kernel(int *p) { *p = 0; }
+
+// NMAX: largest system (k + monomials) the static LDS holds.
+template <int NMAX>
+__global__ void __launch_bounds__(kRbfThreads) rbf_fill_kernel(const float *u, const float *excl, int N, int ny,
+                                                               int nx, double py, double px, int k_req, int kern,
+                                                               double eps, double smooth, const int *n_donors,
+                                                               const int *hole_count, const int *holes, float *out,
+                                                               int *err)
+{
+    constexpr int PITCH = NMAX + 3;                       // 2 right-hand sides, odd pitch in doubles
+    __shared__ double s_a[NMAX * PITCH];                  // candidates, then the augmented system
+    __shared__ int s_nbr[kRbfMaxNbr], s_sel[kRbfMaxNbr];
+    __shared__ double s_x[kRbfMaxNbr], s_y[kRbfMaxNbr];   // neighbour coordinates times eps
+    __shared__ double s_red[kRbfThreads / 64];
+    __shared__ int s_redi[kRbfThreads / 64];
+    __shared__ int s_cnt;
+    __shared__ double s_piv;
+    constexpr int CAP = (NMAX * PITCH * 8) / 12;          // candidates: (distance, index) pairs
+    double *c_d = s_a;
+    int *c_q = (int *)(s_a + CAP);
+
+    const int t = (int)threadIdx.x;
+    const long long per = (long long)ny * nx, total = (long long)N * per;
+    const int nh = *hole_count;
+    const int deg = rbf_degree(kern);
+    const int nm = deg == 0 ? 1 : deg == 1 ? 3 : 6;
+    for (int h = blockIdx.x; h < nh; h += gridDim.x) {
+        const long long i = holes[h];
+        const int n = (int)(i / per), w = (int)(i - (long long)n * per);
+        const int r0 = w / nx, c0 = w % nx;
+        const float *uy = u + (size_t)n * per, *ux = u + (size_t)total + (size_t)n * per;
+        const int nd = n_donors[n];
+        const int k = k_req < nd ? k_req : nd;
+        if (nd < 4 || nm > k) {
+            if (t == 0) atomicOr(err, 1);
+            continue;                                      // block-uniform
+        }
+        const int ns = k + nm;
+        auto donor = [&](int q) {
+            if (excl && excl[q] != 0.0f) return false;
+            const float a = uy[q], b = ux[q];
+            return isfinite(a) && isfinite(b);
+        };
+        // ---- 1. Chebyshev radius R whose square holds >= k donors ----------
+        int cum = 0, R = 0;
+        const int rmax = ny > nx ? ny : nx;
+        while (cum < k && R < rmax) {
+            ++R;
+            int c = 0;
+            for (int e = t; e < 8 * R; e += kRbfThreads) {
+                // ring cell e: top row, bottom row, then the two side columns
+                int dr, dc;
+                if (e < 2 * R + 1) { dr = -R; dc = e - R; }
+                else if (e < 4 * R + 2) { dr = R; dc = e - (2 * R + 1) - R; }
+                else { const int f = e - (4 * R + 2); dr = -R + 1 + (f >> 1); dc = (f & 1) ? R : -R; }
+                const int rr = r0 + dr, cc = c0 + dc;
+                if (rr >= 0 && rr < ny && cc >= 0 && cc < nx && donor(rr * nx + cc)) ++c;
+            }
+            cum += (int)rbf_block_reduce<0>((double)c, s_red);
+        }
+        // ---- 2. candidates: every donor the k nearest can be -----------------
+        const double qx = (double)c0 * px, qy = (double)r0 * py;
+        // d_k <= R * |(py, px)|; the margin keeps the square's corner donors
+        // whose computed distance rounds above it
+        const double reach = (double)R * sqrt(py * py + px * px) * (1.0 + 1e-9);
+        const int hr = (int)fmin((double)ny, ceil(reach / py)), hc = (int)fmin((double)nx, ceil(reach / px));
+        const int rlo = r0 - hr < 0 ? 0 : r0 - hr, rhi = r0 + hr >= ny ? ny - 1 : r0 + hr;
+        const int clo = c0 - hc < 0 ? 0 : c0 - hc, chi = c0 + hc >= nx ? nx - 1 : c0 + hc;
+        const int rw = chi - clo + 1, cells = (rhi - rlo + 1) * rw;
+        if (t == 0) s_cnt = 0;
+        __syncthreads();
+        for (int e = t; e < cells; e += kRbfThreads) {
+            const int rr = rlo + e / rw, cc = clo + e % rw;
+            const int q = rr * nx + cc;
+            if (!donor(q)) continue;
+            const double dx = (double)cc * px - qx, dy = (double)rr * py - qy;
+            const double d = sqrt(dx * dx + dy * dy);
+            if (d > reach) continue;
+            const int slot = atomicAdd(&s_cnt, 1);
+            if (slot < CAP) {
+                c_d[slot] = d;
+                c_q[slot] = q;
+            }
+        }
+        __syncthreads();
+        const int M = s_cnt;
+        if (M > CAP || M < k) {
+            if (t == 0) atomicOr(err, 4);
+            __syncthreads();
+            continue;
+        }
+        // ---- 3. the k nearest by (distance, index), then in index order ------
+        for (int a = t; a < M; a += kRbfThreads) {
+            const double d = c_d[a];
+            const int q = c_q[a];
+            int rank = 0;
+            for (int b2 = 0; b2 < M; ++b2) {
+                const double d2 = c_d[b2];
+                rank += (d2 < d || (d2 == d && c_q[b2] < q)) ? 1 : 0;
+            }
+            if (rank < k) s_sel[rank] = q;
+        }
+        __syncthreads();
+        for (int a = t; a < k; a += kRbfThreads) {
+            const int q = s_sel[a];
+            int rank = 0;
+            for (int b2 = 0; b2 < k; ++b2) rank += s_sel[b2] < q ? 1 : 0;
+            s_nbr[rank] = q;
+        }
+        __syncthreads();
+        // ---- 4. neighbourhood coordinates, shift and scale ---------------------
+        double xmn = INFINITY, xmx = -INFINITY, ymn = INFINITY, ymx = -INFINITY;
+        for (int a = t; a < k; a += kRbfThreads) {
+            const int q = s_nbr[a];
+            const double x = (double)(q % nx) * px, y = (double)(q / nx) * py;
+            xmn = fmin(xmn, x); xmx = fmax(xmx, x);
+            ymn = fmin(ymn, y); ymx = fmax(ymx, y);
+        }
+        xmn = rbf_block_reduce<1>(xmn, s_red);
+        xmx = rbf_block_reduce<2>(xmx, s_red);
+        ymn = rbf_block_reduce<1>(ymn, s_red);
+        ymx = rbf_block_reduce<2>(ymx, s_red);
+        const double shx = (xmx + xmn) / 2.0, shy = (ymx + ymn) / 2.0;
+        double scx = (xmx - xmn) / 2.0, scy = (ymx - ymn) / 2.0;
+        if (scx == 0.0) scx = 1.0;
+        if (scy == 0.0) scy = 1.0;
+        for (int a = t; a < k; a += kRbfThreads) {
+            const int q = s_nbr[a];
+            s_x[a] = (double)(q % nx) * px * eps;
+            s_y[a] = (double)(q / nx) * py * eps;
+        }
+        __syncthreads();
+        // ---- 5. the augmented system [K + sI, P | d; P^T, 0 | 0] ---------------
+        const int ncol = ns + 2;
+        for (int e = t; e < ns * ncol; e += kRbfThreads) {
+            const int r = e / ncol, c = e - r * ncol;
+            double v;
+            if (r < k) {
+                const int q = s_nbr[r];
+                if (c < k) {
+                    const double dx = s_x[r] - s_x[c], dy = s_y[r] - s_y[c];
+                    v = rbf_phi(kern, sqrt(dx * dx + dy * dy));
+                    if (r == c) v += smooth;
+                } else if (c < ns) {
+                    const double x = (double)(q % nx) * px, y = (double)(q / nx) * py;
+                    v = rbf_mono(c - k, (x - shx) / scx, (y - shy) / scy);
+                } else {
+                    v = (double)(c == ns ? ux[q] : uy[q]);
+                }
+            } else {
+                if (c < k) {
+                    const int q = s_nbr[c];
+                    const double x = (double)(q % nx) * px, y = (double)(q / nx) * py;
+                    v = rbf_mono(r - k, (x - shx) / scx, (y - shy) / scy);
+                } else {
+                    v = 0.0;
+                }
+            }
+            s_a[r * PITCH + c] = v;
+        }
+        __syncthreads();
+        // ---- 6. LU with partial pivoting, eliminating into both right sides ----
+        bool singular = false;
+        for (int j = 0; j < ns; ++j) {
+            // pivot: largest |a_ij|, i >= j, first on ties (idamax)
+            double best = -1.0;
+            int brow = -1;
+            for (int r = j + t; r < ns; r += kRbfThreads) {
+                const double a = fabs(s_a[r * PITCH + j]);
+                if (a > best) { best = a; brow = r; }
+            }
+            const int p = rbf_block_argmax(best < 0.0 ? 0.0 : best, brow, s_red, s_redi);
+            if (p < 0) { singular = true; break; }          // block-uniform
+            if (p != j)
+                for (int c = t; c < ncol; c += kRbfThreads) {
+                    const double tmp = s_a[j * PITCH + c];
+                    s_a[j * PITCH + c] = s_a[p * PITCH + c];
+                    s_a[p * PITCH + c] = tmp;
+                }
+            __syncthreads();
+            if (t == 0) s_piv = 1.0 / s_a[j * PITCH + j];
+            __syncthreads();
+            const double rp = s_piv;
+            for (int r = j + 1 + t; r < ns; r += kRbfThreads) s_a[r * PITCH + j] *= rp;
+            __syncthreads();
+            const int rows = ns - j - 1, cols = ncol - j - 1;
+            for (int e = t; e < rows * cols; e += kRbfThreads) {
+                const int r = j + 1 + e / cols, c = j + 1 + e % cols;
+                s_a[r * PITCH + c] -= s_a[r * PITCH + j] * s_a[j * PITCH + c];
+            }
+            __syncthreads();
+        }
+        if (singular) {
+            if (t == 0) atomicOr(err, 2);
+            __syncthreads();
+            continue;
+        }
+        // ---- 7. back substitution (columns ns, ns + 1 become the coefficients) --
+        for (int j = ns - 1; j >= 0; --j) {
+            if (t < 2) s_a[j * PITCH + ns + t] /= s_a[j * PITCH + j];
+            __syncthreads();
+            for (int e = t; e < 2 * j; e += kRbfThreads) {
+                const int r = e >> 1, c = ns + (e & 1);
+                s_a[r * PITCH + c] -= s_a[r * PITCH + j] * s_a[j * PITCH + c];
+            }
+            __syncthreads();
+        }
+        // ---- 8. evaluate at the hole ------------------------------------------
+        const double xe = qx * eps, ye = qy * eps;
+        const double xh = (qx - shx) / scx, yh = (qy - shy) / scy;
+        double vx = 0.0, vy = 0.0;
+        for (int a = t; a < ns; a += kRbfThreads) {
+            double f;
+            if (a < k) {
+                const double dx = xe - s_x[a], dy = ye - s_y[a];
+                f = rbf_phi(kern, sqrt(dx * dx + dy * dy));
+            } else {
+                f = rbf_mono(a - k, xh, yh);
+            }
+            vx += f * s_a[a * PITCH + ns];
+            vy += f * s_a[a * PITCH + ns + 1];
+        }
+        vx = rbf_block_reduce<0>(vx, s_red);
+        vy = rbf_block_reduce<0>(vy, s_red);
+        if (t == 0) {
+            out[i] = (float)vy;
+            out[total + i] = (float)vx;
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace pivamd
+
+// d_work: N donor counts, the hole count and up to N ny nx hole indices.
+extern "C" unsigned long pivamd_infill_rbf_workspace(int N, int ny, int nx)
+{
+    const long long cells = (long long)(N > 0 ? N : 1) * (ny > 0 ? ny : 1) * (nx > 0 ? nx : 1);
+    return (unsigned long)((N > 0 ? N : 1) + 1 + cells) * sizeof(int);
+}
+
+extern "C" int pivamd_infill_rbf(const float *u, const float *d_excl, int N, int ny, int nx, float py, float px,
+                                 int neighbors, int kernel, double epsilon, double smoothing, float *out,
+                                 void *d_work, unsigned long work_bytes, int *d_err, hipStream_t st)
+{
+    using namespace pivamd;
+    if (N < 0 || ny <= 0 || nx <= 0 || !u || !out || !d_err || !d_work) return PIVAMD_ERR_ARG;
+    if (neighbors < 1 || neighbors > kRbfMaxNbr || kernel < kRbfLinear || kernel > kRbfGauss) return PIVAMD_ERR_ARG;
+    if (!(py > 0.0f) || !(px > 0.0f) || !(epsilon > 0.0) || !(smoothing >= 0.0)) return PIVAMD_ERR_ARG;
+    if (work_bytes < pivamd_infill_rbf_workspace(N, ny, nx)) return PIVAMD_ERR_ARG;
+    if (N == 0) return 0;
+    int *nd = (int *)d_work, *cnt = nd + N, *holes = cnt + 1;
+    const long long total = (long long)N * ny * nx;
+    if (total > 0x7fffffffLL) return PIVAMD_ERR_ARG;
+    hipLaunchKernelGGL(rbf_zero_kernel, dim3(1), dim3(1), 0, st, cnt);
+    hipLaunchKernelGGL(knn_count_kernel, dim3(N), dim3(1024), 0, st, u, d_excl, N, ny * nx, nd);
+    hipLaunchKernelGGL(rbf_prep_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, u, d_excl, N,
+                       ny * nx, out, cnt, holes);
+    const int nm = rbf_degree(kernel) == 0 ? 1 : rbf_degree(kernel) == 1 ? 3 : 6;
+    const unsigned grid = (unsigned)(total < 2048 ? total : 2048);
+    if (neighbors + nm <= 70)
+        hipLaunchKernelGGL(rbf_fill_kernel<70>, dim3(grid), dim3(kRbfThreads), 0, st, u, d_excl, N, ny, nx,
+                           (double)py, (double)px, neighbors, kernel, epsilon, smoothing, nd, cnt, holes, out, d_err);
+    else
+        hipLaunchKernelGGL(rbf_fill_kernel<kRbfMaxNbr + kRbfMaxMono>, dim3(grid), dim3(kRbfThreads), 0, st, u, d_excl,
+                           N, ny, nx, (double)py, (double)px, neighbors, kernel, epsilon, smoothing, nd, cnt, holes,
+                           out, d_err);
+    return hipGetLastError() == hipSuccess ? 0 : PIVAMD_ERR_LAUNCH;
+}

@@ -89,6 +89,11 @@ def core() -> ctypes.CDLL:
         L.pivamd_infill_knn.argtypes = [_vp, _vp, _i, _i, _i, ctypes.c_float, ctypes.c_float, _i, _i, _vp, _vp,
                                         ctypes.c_ulong, _vp, _vp]
         L.pivamd_infill_knn.restype = _i
+        L.pivamd_infill_rbf_workspace.argtypes = [_i, _i, _i]
+        L.pivamd_infill_rbf_workspace.restype = ctypes.c_ulong
+        L.pivamd_infill_rbf.argtypes = [_vp, _vp, _i, _i, _i, ctypes.c_float, ctypes.c_float, _i, _i,
+                                        ctypes.c_double, ctypes.c_double, _vp, _vp, ctypes.c_ulong, _vp, _vp]
+        L.pivamd_infill_rbf.restype = _i
         L.pivamd_xcorr_ragged.argtypes = [ctypes.POINTER(XcorrJob), _i, _i, _i, _i, _i, _i, _vp]
         L.pivamd_xcorr_ragged.restype = _i
         L.pivamd_xcorr_job_size.argtypes = []

@@ -108,16 +108,16 @@ class _PassBuf:
 def _infill_cfg(cfg, final):
     """(method, parameters) of an infill option: a name, or the reference's
     config form {"method": ..., "parameters": {...}} (infilling.py:698-790).
-    griddata_linear and rbf_local are not built on the device."""
+    griddata_linear is not built on the device."""
     if isinstance(cfg, dict):
         method, params = str(cfg.get("method", "nearest")).lower(), dict(cfg.get("parameters", {}) or {})
         if final and not cfg.get("enabled", True):
             method = "none"
     else:
         method, params = str(cfg).lower(), {}
-    allowed = ("nearest", "harmonic", "biharmonic", "knn") + (("none",) if final else ())
+    allowed = ("nearest", "harmonic", "biharmonic", "knn", "rbf_local") + (("none",) if final else ())
     if method not in allowed:
-        if method in ("griddata_linear", "rbf_local"):
+        if method == "griddata_linear":
             raise NotImplementedError(f"infill '{method}' is not on the device path")
         raise ValueError(f"infill method {method!r}: one of {allowed}")
     if method == "knn":
@@ -126,7 +126,29 @@ def _infill_cfg(cfg, final):
             raise ValueError(f"knn n_neighbors {k}: 1..128 on the device")
         if params.get("weights", "distance") not in ("distance", "uniform"):
             raise ValueError(f"knn weights {params.get('weights')!r}: distance or uniform")
+    if method == "rbf_local":
+        k = int(params.get("neighbors", 64))
+        if not 1 <= k <= 128:
+            raise ValueError(f"rbf_local neighbors {k}: 1..128 on the device")
+        kern = params.get("kernel", "thin_plate_spline")
+        if kern not in RBF_KERNELS:
+            raise ValueError(f"`kernel` must be one of {sorted(RBF_KERNELS)}.")
+        eps = params.get("epsilon", None)
+        if eps is None:
+            if kern not in ("linear", "thin_plate_spline", "cubic", "quintic"):
+                raise ValueError("`epsilon` must be specified if `kernel` is not one of "
+                                 "{'linear', 'thin_plate_spline', 'cubic', 'quintic'}.")
+            eps = 1.0
+        sm = float(params.get("smoothing", 0.0) or 0.0)
+        if not float(eps) > 0.0 or not sm >= 0.0:
+            raise ValueError(f"rbf_local epsilon {eps!r} must be > 0 and smoothing {sm!r} >= 0")
+        params = {"neighbors": k, "kernel": kern, "epsilon": float(eps), "smoothing": sm}
     return method, params
+
+
+# scipy RBFInterpolator kernel names -> pivamd_infill_rbf kernel codes
+RBF_KERNELS = {"linear": 0, "thin_plate_spline": 1, "cubic": 2, "quintic": 3, "multiquadric": 4,
+               "inverse_multiquadric": 5, "inverse_quadratic": 6, "gaussian": 7}
 
 
 # Correlation transform of the passes (pivamd_set_dft_mode): the FP32 FFT, or a
@@ -360,6 +382,8 @@ class MultipassPIV:
                 add(k, (2, n, nyp, nxp))
             if (self.final_infill if final else self.mid_infill) == "knn":
                 add("knn_work", (max(4, self.lib.pivamd_infill_knn_workspace(n)),), u8)
+            if (self.final_infill if final else self.mid_infill) == "rbf_local":
+                add("rbf_work", (max(4, self.lib.pivamd_infill_rbf_workspace(n, ny, nx)),), u8)
             if (self.final_infill if final else self.mid_infill) in ("harmonic", "biharmonic"):
                 add("ls_work", (max(4, self.lib.pivamd_infill_ls_workspace(n, ny, nx)),), u8)
             add("work", (max(4, self.lib.pivamd_xcorr_workspace(n, ny, nx, g.win[0], g.win[1], P, 0)),), u8)
@@ -609,6 +633,15 @@ class MultipassPIV:
                                               1 if kp.get("weights", "distance") == "uniform" else 0,
                                               _ptr(o["u3"]), _ptr(wk), wk.numel(), _ptr(o["err"]), st), "infill")
             u = o["u3"]
+        elif method == "rbf_local":
+            o["err"].zero_()
+            kp = self.final_infill_params if final else self.mid_infill_params
+            wk = o["rbf_work"]
+            _native.check(L.pivamd_infill_rbf(_ptr(u), _ptr(b.mask), n, ny, nx, float(g.spacing[0]),
+                                              float(g.spacing[1]), kp["neighbors"], RBF_KERNELS[kp["kernel"]],
+                                              kp["epsilon"], kp["smoothing"], _ptr(o["u3"]), _ptr(wk),
+                                              wk.numel(), _ptr(o["err"]), st), "infill")
+            u = o["u3"]
         elif method in ("harmonic", "biharmonic"):
             o["err"].zero_()
             wk = o["ls_work"]
@@ -691,8 +724,17 @@ class MultipassPIV:
         with no donor (nearest; carried on as zero displacement), a
         least-squares fill that missed the 1e-6 residual acceptance, or fill
         cells no difference equation ties to a valid cell (left at 0)."""
+        last = len(self.window_sizes) - 1
         for p, b in enumerate(self._bufs or []):
             e = int(b.out["err"].item())
+            if (self.final_infill if p == last else self.mid_infill) == "rbf_local" and e:
+                if e & 1:
+                    raise ValueError(f"pass {p}: Infilling method 'rbf_local' cannot run: fewer than 4 valid "
+                                     "donor points (or fewer than the polynomial terms)")
+                if e & 2:
+                    raise np.linalg.LinAlgError(f"pass {p}: rbf_local: Singular matrix.")
+                raise ValueError(f"pass {p}: rbf_local: the neighbour search exceeded its buffer "
+                                 "(grid pitch too anisotropic)")
             if e & 1:
                 raise ValueError(f"pass {p}: infilling has no donor points: 100% of vectors invalid")
             if e & 4:

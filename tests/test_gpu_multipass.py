@@ -1130,3 +1130,100 @@ def test_loop_with_knn_infill_matches_oracle_loop():
         np.testing.assert_array_equal(g.nan_reason.cpu().numpy(), r["nan_reason"])
         d = np.hypot(g.ux.cpu().numpy() - r["ux"], g.uy.cpu().numpy() - r["uy"])
         assert np.sqrt(np.mean(d ** 2)) < 0.005, (p, d.max())
+
+
+@pytest.mark.parametrize("kernel,eps,smoothing", [("thin_plate_spline", None, 0.0), ("cubic", None, 0.01),
+                                                  ("quintic", None, 0.0), ("linear", None, 0.0),
+                                                  ("multiquadric", 0.05, 0.0), ("gaussian", 0.05, 0.0),
+                                                  ("inverse_multiquadric", 0.05, 0.0),
+                                                  ("inverse_quadratic", 0.05, 0.02)])
+@pytest.mark.parametrize("spacing", [(16.0, 16.0), (8.0, 12.0)])
+def test_rbf_infill_matches_oracle(kernel, eps, smoothing, spacing):
+    """The device local RBF infill against the oracle's (same neighbourhoods:
+    distance, then row-major index; FP64 systems, LU with partial pivoting on
+    the device, LAPACK gesv in the oracle): float32 results within 2e-6
+    relative (the two FP64 solves round differently), with holes, an
+    excluded block and neighbour counts below and above the donor count."""
+    from pivtools_amd.multipass import RBF_KERNELS
+    L = _lib()
+    rng = np.random.default_rng(67)
+    N, ny, nx = 2, 17, 23
+    yy, xx = np.mgrid[0:ny, 0:nx]
+    u = np.stack([np.sin(0.3 * xx + 0.2 * yy), np.cos(0.25 * yy) * 0.7])[:, None].repeat(N, 1)
+    u = (u + 0.1 * rng.normal(size=u.shape)).astype(np.float32)
+    holes = rng.uniform(size=(N, ny, nx)) < 0.12
+    holes[1, 5:9, 3:10] = True
+    u[:, holes] = np.nan
+    ex = np.zeros((ny, nx), np.float32)
+    ex[2:4, 18:21] = 1.0
+    du, dex = _t(u), _t(ex)
+    wk = torch.empty(L.pivamd_infill_rbf_workspace(N, ny, nx), dtype=torch.uint8, device=dev)
+    err = torch.zeros(1, dtype=torch.int32, device=dev)
+    for k in ((20, 64, 128) if kernel == "quintic" else (6, 20, 64, 128)):
+        print(f"rbf {kernel} k={k} spacing={spacing}", flush=True)
+        out = torch.empty_like(du)
+        err.zero_()
+        assert L.pivamd_infill_rbf(du.data_ptr(), dex.data_ptr(), N, ny, nx, spacing[0], spacing[1], k,
+                                   RBF_KERNELS[kernel], 1.0 if eps is None else eps, smoothing, out.data_ptr(),
+                                   wk.data_ptr(), wk.numel(), err.data_ptr(), _st()) == 0
+        ref = M.infill_rbf_local(u, spacing, neighbors=k, kernel=kernel, epsilon=eps, smoothing=smoothing,
+                                 exclude=ex.astype(bool))
+        torch.cuda.synchronize()
+        assert int(err.item()) == 0
+        got = out.cpu().numpy()
+        np.testing.assert_array_equal(np.isnan(got), np.isnan(ref))
+        np.testing.assert_allclose(got, ref, rtol=2e-6, atol=2e-6)
+        fill = np.isnan(u) & ~ex.astype(bool)[None, None]
+        print(f"  exact {np.mean(got[fill] == ref[fill]):.3f} of {fill.sum()} filled values", flush=True)
+
+
+def test_rbf_infill_error_words():
+    """Fewer than four donors flags bit 1; the other cells pass through."""
+    L = _lib()
+    ny, nx = 9, 11
+    v = np.full((2, 1, ny, nx), np.nan, np.float32)
+    v[:, 0, 0, :3] = 1.0
+    dv = _t(v)
+    out = torch.empty_like(dv)
+    wk = torch.empty(L.pivamd_infill_rbf_workspace(1, ny, nx), dtype=torch.uint8, device=dev)
+    err = torch.zeros(1, dtype=torch.int32, device=dev)
+    assert L.pivamd_infill_rbf(dv.data_ptr(), None, 1, ny, nx, 16.0, 16.0, 64, 1, 1.0, 0.0, out.data_ptr(),
+                               wk.data_ptr(), wk.numel(), err.data_ptr(), _st()) == 0
+    torch.cuda.synchronize()
+    assert int(err.item()) & 1
+    np.testing.assert_array_equal(out.cpu().numpy()[:, 0, 0, :3], 1.0)
+    # arguments the device refuses
+    assert L.pivamd_infill_rbf(dv.data_ptr(), None, 1, ny, nx, 16.0, 16.0, 129, 1, 1.0, 0.0, out.data_ptr(),
+                               wk.data_ptr(), wk.numel(), err.data_ptr(), _st()) != 0
+    assert L.pivamd_infill_rbf(dv.data_ptr(), None, 1, ny, nx, 16.0, 16.0, 64, 8, 1.0, 0.0, out.data_ptr(),
+                               wk.data_ptr(), wk.numel(), err.data_ptr(), _st()) != 0
+
+
+def test_loop_with_rbf_infill_matches_oracle_loop():
+    """The pass loop with the reference's rbf_local infill (thin-plate
+    spline, 64 neighbours) on the final pass and nearest in between:
+    identical NaN masks and reasons, RMS < 0.005 px against the oracle loop."""
+    shape = (192, 256)
+    sched = [(64, 64), (32, 32), (16, 16)]
+    A, B = [], []
+    for k in range(2):
+        rng = np.random.default_rng(990 + k)
+        x, y, inten = S._positions(rng, shape, 0.03)
+        dx, dy = S.lamb_oseen(x, y, shape, rc=40.0, peak=3.0, uniform=(1.5, 0.0))
+        A.append(S.render(shape, x, y, inten).numpy())
+        B.append(S.render(shape, x + dx, y + dy, inten).numpy())
+    A = np.stack(A).astype(np.float32)
+    B = np.stack(B).astype(np.float32)
+    mp = MultipassPIV(shape, sched, [50, 50, 50], warp_arithmetic="exact", final_infill="rbf_local")
+    res = mp.run(_t(A), _t(B), keep_passes=True)
+    torch.cuda.synchronize()
+    mp.check_infill()
+    grids = build_pass_grids(shape, sched, [50, 50, 50])
+    ref = M.run_multipass(A, B, grids, keep_passes=True, final_infill="rbf_local")
+    for p, (g, r) in enumerate(zip(res, ref)):
+        gm = g.nan_mask.cpu().numpy().astype(bool)
+        np.testing.assert_array_equal(gm, r["nan_mask"])
+        np.testing.assert_array_equal(g.nan_reason.cpu().numpy(), r["nan_reason"])
+        d = np.hypot(g.ux.cpu().numpy() - r["ux"], g.uy.cpu().numpy() - r["uy"])
+        assert np.sqrt(np.mean(d ** 2)) < 0.005, (p, d.max())
+    print(f"final-pass vectors filled: {int(res[-1].nan_mask.sum())}", flush=True)

@@ -636,3 +636,88 @@ def test_knn_infill_oracle_against_sklearn(weights, spacing):
                 np.testing.assert_allclose(a[:, i, r, c], b[:, i, r, c], rtol=1e-6, atol=1e-7)
             assert np.isfinite(a[:, i][:, fill]).all()
             np.testing.assert_array_equal(a[:, i][:, ~fill], u[:, i][:, ~fill])
+
+
+RBF_EPS = {"linear": None, "thin_plate_spline": None, "cubic": None, "quintic": None, "multiquadric": 0.4,
+           "inverse_multiquadric": 0.4, "inverse_quadratic": 0.4, "gaussian": 0.4}
+
+
+def _rbf_case(seed, ny=15, nx=19, frac=0.12):
+    rng = np.random.default_rng(seed)
+    yy, xx = np.mgrid[0:ny, 0:nx]
+    u = np.stack([np.sin(0.3 * xx + 0.2 * yy) + 0.1 * rng.normal(size=(ny, nx)),
+                  np.cos(0.25 * yy) * 0.7 + 0.1 * rng.normal(size=(ny, nx))])[:, None].astype(np.float32)
+    u = np.concatenate([u, u[:, :, ::-1, :]], axis=1)          # two images
+    holes = rng.uniform(size=u.shape[1:]) < frac
+    holes[0, 0:3, 0:4] = True
+    u[:, holes] = np.nan
+    ex = np.zeros((ny, nx), bool)
+    ex[7, 9:12] = True
+    return u, ex
+
+
+@pytest.mark.parametrize("kernel", sorted(RBF_EPS))
+@pytest.mark.parametrize("spacing,smoothing", [((1.0, 1.0), 0.0), ((8.0, 12.0), 0.0), ((1.0, 1.0), 0.05)])
+def test_rbf_local_oracle_against_scipy(kernel, spacing, smoothing):
+    """The oracle's local RBF infill equals scipy's RBFInterpolator with
+    `neighbors` -- what the reference calls (infilling.py:505-597) -- at every
+    hole whose k-th and (k+1)-th donor distances differ (where they tie, the
+    KD-tree may keep another donor of the same distance).  Smoothing > 0 also
+    pins the kernels' signs (without it c * phi and (-c) * (-phi) agree)."""
+    u, ex = _rbf_case(7)
+    eps = RBF_EPS[kernel]
+    ny, nx = u.shape[2:]
+    yy, xx = np.mgrid[0:ny, 0:nx]
+    # 8 neighbours can lie on two grid lines, where the quadratic monomials
+    # of the quintic lose rank (scipy raises "Singular matrix")
+    for k in ((20, 64) if kernel == "quintic" else (8, 20, 64)):
+        a = M.infill_rbf_local(u, spacing, neighbors=k, kernel=kernel, epsilon=eps, smoothing=smoothing,
+                               exclude=ex, impl="brute")
+        b = M.infill_rbf_local(u, spacing, neighbors=k, kernel=kernel, epsilon=eps, smoothing=smoothing,
+                               exclude=ex, impl="scipy")
+        checked = 0
+        for i in range(u.shape[1]):
+            fill = (np.isnan(u[0, i]) | np.isnan(u[1, i])) & ~ex
+            donor = ~fill & ~ex
+            dpts = np.c_[(xx * spacing[1])[donor], (yy * spacing[0])[donor]]
+            for r, c in zip(*np.nonzero(fill)):
+                d = np.sort(np.sqrt((dpts[:, 0] - c * spacing[1]) ** 2 + (dpts[:, 1] - r * spacing[0]) ** 2))
+                if k < len(d) and d[k - 1] == d[k]:
+                    continue
+                np.testing.assert_allclose(a[:, i, r, c], b[:, i, r, c], rtol=2e-6, atol=2e-6)
+                checked += 1
+            assert np.isfinite(a[:, i][:, fill]).all()
+            np.testing.assert_array_equal(a[:, i][:, ~fill], u[:, i][:, ~fill])
+        assert checked >= 5, checked
+
+
+@pytest.mark.parametrize("kernel,field", [("thin_plate_spline", "linear"), ("cubic", "linear"),
+                                          ("quintic", "quadratic"), ("linear", "constant"),
+                                          ("gaussian", "constant")])
+def test_rbf_local_reproduces_its_polynomials(kernel, field):
+    """An interpolant with polynomial terms of degree d reproduces every
+    polynomial of degree <= d (the RBF coefficients vanish): linear fields
+    for the thin-plate spline and cubic, quadratics for the quintic,
+    constants for the rest."""
+    ny, nx = 13, 16
+    yy, xx = np.mgrid[0:ny, 0:nx].astype(np.float64)
+    X, Y = xx * 3.0, yy * 2.0
+    f = {"linear": 0.3 + 0.02 * X - 0.05 * Y, "quadratic": 0.1 + 0.01 * X * X - 0.004 * X * Y + 0.02 * Y * Y,
+         "constant": 0.75 + 0 * X}[field]
+    u = np.stack([f, -f])[:, None].astype(np.float32)
+    holes = np.zeros((1, ny, nx), bool)
+    holes[0, 5:8, 6:9] = True
+    holes[0, 0, 0] = holes[0, 12, 15] = True
+    u[:, holes] = np.nan
+    got = M.infill_rbf_local(u, (2.0, 3.0), neighbors=30, kernel=kernel, epsilon=RBF_EPS[kernel])
+    want = np.stack([f, -f])[:, None].astype(np.float32)
+    np.testing.assert_allclose(got[:, holes], want[:, holes], rtol=0, atol=2e-6)
+
+
+def test_rbf_local_needs_four_donors():
+    u = np.full((2, 1, 5, 5), np.nan, np.float32)
+    u[:, 0, 0, 0:3] = 1.0
+    with pytest.raises(ValueError):
+        M.infill_rbf_local(u, (1.0, 1.0))
+    with pytest.raises(ValueError):
+        M.infill_rbf_local(u, (1.0, 1.0), impl="scipy")
