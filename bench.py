@@ -98,7 +98,75 @@ def cpu_baseline(A_dev, B_dev, gpu_final, finder, budget_s, loop_kw=None):
         acc["rms_px_vs_fp64"] = float(np.sqrt(np.mean((gx[same] - fx[same]) ** 2 + (gy[same] - fy[same]) ** 2)))
         acc["mask_flip_frac_vs_fp64"] = float(np.mean(gm != fm))
         acc["fp64_oracle_s"] = round(time.perf_counter() - t0, 2)
+    cal = warp_calibration(a, b, grids)
+    if cal is not None:
+        out["calibration"] = cal
     return out, acc
+
+
+def warp_calibration(a, b, grids, reps=3):
+    """R = t(oracle) / t(reference C) for the one reference kernel of the
+    path that compiles from its own sources: fused_symmetric_warp_batch
+    (bicubic), timed on the same pair, predictor grid and thread count as
+    the oracle's warp (median of `reps`).  The reference build (AVX2
+    interior sampler, and its scalar form) is the checker's, made from the
+    reference sources in the build container; None where it is absent.  The
+    correlation (libbulkxcorr2d) is not built from the reference: its FFT
+    includes generated codelet headers."""
+    import ctypes
+    import numpy as np
+    import oracle as O
+
+    here = os.path.join(ROOT, "oracle", "_ref")
+    simd, scal = os.path.join(here, "libref_warp_simd.so"), os.path.join(here, "libref_warp.so")
+    if not (os.path.isfile(simd) and os.path.isfile(scal)):
+        return None
+    g = grids[1]
+    cy = np.ascontiguousarray(g.ctrs_y_all, np.float32)
+    cx = np.ascontiguousarray(g.ctrs_x_all, np.float32)
+    rng = np.random.default_rng(5)
+    dy = rng.uniform(-2, 2, (1, len(cy), len(cx))).astype(np.float32)
+    dx = rng.uniform(-2, 2, (1, len(cy), len(cx))).astype(np.float32)
+    A = np.ascontiguousarray(a, np.float32)
+    B = np.ascontiguousarray(b, np.float32)
+    N, H, W = A.shape
+    f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+
+    def ref_ms(path, impl):
+        lib = ctypes.CDLL(path)
+        lib.fused_symmetric_warp_batch.argtypes = [f32p] * 6 + [ctypes.c_int] * 5 + [f32p, f32p] + \
+            [ctypes.c_int] * 3
+        lib.fused_warp_set_impl.argtypes = [ctypes.c_int]
+        lib.fused_warp_set_impl(impl)
+        oa, ob = np.zeros_like(A), np.zeros_like(B)
+        ts = []
+        for _ in range(reps + 1):
+            t0 = time.perf_counter()
+            r = lib.fused_symmetric_warp_batch(A, B, oa, ob, dy, dx, N, H, W, len(cy), len(cx), cy, cx, 0, 0, 0)
+            ts.append(time.perf_counter() - t0)
+            if r != 0:
+                raise RuntimeError(f"reference warp failed ({r})")
+        return 1e3 * float(np.median(ts[1:])), oa
+
+    def orc_ms():
+        ts = []
+        for _ in range(reps + 1):
+            t0 = time.perf_counter()
+            oa, _ = O.fused_warp_batch(A, B, dy, dx, cy, cx, interp_mode=0)
+            ts.append(time.perf_counter() - t0)
+        return 1e3 * float(np.median(ts[1:])), oa
+
+    t_simd, w_simd = ref_ms(simd, 1)
+    t_scal, w_scal = ref_ms(scal, 0)
+    t_orc, w_orc = orc_ms()
+    return {"kernel": "fused_symmetric_warp_batch, bicubic, 1 pair of 2160x2560, pass-1 predictor grid",
+            "threads": int(O.lib().orc_max_threads()),
+            "reference_simd_ms": round(t_simd, 2), "reference_scalar_ms": round(t_scal, 2),
+            "oracle_ms": round(t_orc, 2), "R_vs_simd": round(t_orc / t_simd, 3),
+            "R_vs_scalar": round(t_orc / t_scal, 3),
+            "oracle_equals_reference_scalar": bool(np.array_equal(w_orc, w_scal)),
+            "max_abs_simd_vs_scalar": float(np.max(np.abs(w_simd - w_scal))),
+            "correlation": "reference libbulkxcorr2d not built (generated codelet headers); no R"}
 
 
 ENS_SHAPE = (1024, 1024)
