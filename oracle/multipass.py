@@ -5,10 +5,12 @@ pivtools_cli/piv/piv_backend/cpu_instantaneous.py:381-1147 around the two C
 calls, with the C calls taken from oracle.py (liboracle.so) or, for the
 FP64 flavour, from FP64 NumPy planes.
 
-Pins:
-  gate / reason codes   restated from cpu_instantaneous.py:576-757 (unpinned
-                        beyond the reference's own invariants; tested for
-                        (nan_reason != 0) == nan_mask)
+Pins (each checked in tests/test_multipass_oracle.py so that a plausible
+mistake fails):
+  gate / reason codes   restated from cpu_instantaneous.py:576-757; known
+                        answers for every code (-1, 1, 6, 9, 10, 12, 13), their
+                        precedence, the corner guard rule, the secondary-peak
+                        rescue and unit-tests/test_nan_reason.py's masked block
   median test           outlier_detection.py:87-243 restated; checked against
                         a brute-force per-node loop in the tests
   nearest infill        scipy.ndimage.distance_transform_edt, as the reference
@@ -17,10 +19,12 @@ Pins:
                         it (cpu_instantaneous.py:1135-1147)
   predictor remap       cv2.remap(INTER_CUBIC, BORDER_REPLICATE) is not
                         importable here: restated from OpenCV's published
-                        algorithm (float map -> 1/32 fixed point, Keys a=-0.75
-                        per axis, replicate border).  Parity unpinned: only the
-                        closed forms (constant, linear and integer-shift
-                        fields) check it.
+                        algorithm (float map -> 1/32 fixed point with ties to
+                        even, Keys a=-0.75 per axis, replicate border, row sums
+                        inside the field and term-by-term sums at a border);
+                        pinned by positions off the lattice and on its ties,
+                        every border against an FP64 evaluation on an
+                        edge-padded copy, and closed-form fields
   vector masks          load_images.py:952-985 restated as the reference's
                         per-window loop; checked against hand-counted windows
   predictor fill        infilling.py:808-870: 'nearest' through
@@ -34,8 +38,16 @@ Pins:
                         excluded cells; checked against np.linalg.lstsq on
                         the explicit equations and the fields each order
                         reproduces exactly (constant; linear for order 2)
-  predictor BCs         base.py:364-416 restated (row overwrite; unpinned
-                        beyond its own definition)
+  predictor BCs         base.py:364-416 restated (row overwrite), checked by
+                        hand-computed rows with a centre on each threshold
+  k-NN infill           infilling.py:600-695; equals sklearn's
+                        KNeighborsRegressor where no tie sits at the k-th
+                        distance
+  local RBF infill      infilling.py:505-597; equals scipy's RBFInterpolator
+                        (neighbors) where no tie sits at the k-th distance,
+                        and reproduces the polynomials of its degree
+  end to end            the Poiseuille bound of
+                        unit-tests/test_multipass_convergence.py:398-430
 """
 
 from __future__ import annotations

@@ -405,12 +405,15 @@ def test_harmonic_operator_refuses_an_all_masked_grid():
 
 
 def test_predictor_bcs_overwrite_rows_beyond_the_position():
+    """base.py:364-416 by hand: a bottom BC at y_position 20 of a 64-row
+    image sets rows with centre >= 43, a top BC at 5 rows with centre <= 5;
+    a centre on each threshold pins the inclusive comparisons."""
     pad = np.zeros((2, 1, 6, 4), np.float32)
-    ya = np.array([-10.0, 5.0, 20.0, 35.0, 50.0, 70.0])
+    ya = np.array([-10.0, 5.0, 20.0, 43.0, 50.0, 70.0])
     M.apply_predictor_bcs(pad, ya, 64, [{"edge": "bottom", "y_position": 20, "ux": 1.5, "uy": -0.5},
                                         {"edge": "top", "y_position": 5, "ux": 0.25, "uy": 0.0}])
-    np.testing.assert_array_equal(pad[1, 0, :, 0], [0.25, 0.25, 0, 0, 1.5, 1.5])
-    np.testing.assert_array_equal(pad[0, 0, :, 0], [0, 0, 0, 0, -0.5, -0.5])
+    np.testing.assert_array_equal(pad[1, 0, :, 0], [0.25, 0.25, 0, 1.5, 1.5, 1.5])
+    np.testing.assert_array_equal(pad[0, 0, :, 0], [0, 0, 0, -0.5, -0.5, -0.5])
 
 
 def _lstsq_fill_general(field, fill, spacing, order, exclude):
