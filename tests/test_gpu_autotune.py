@@ -21,7 +21,8 @@ from test_gpu_dropin import _check_outputs, _pair
 
 pytestmark = pytest.mark.gpu
 
-CASES = [(n, v) for n in (8, 12, 16, 24, 32, 48, 64, 96, 128) for v in AT.variants(n) if v != AT.MFMA]
+# the FP32 variants; the split-precision MFMA forms have their own bars in test_gpu_split_dft
+CASES = [(n, v) for n in (8, 12, 16, 24, 32, 48, 64, 96, 128) for v in AT.variants(n) if v not in AT.MFMA_FORMS]
 
 
 @pytest.mark.parametrize("n,v", CASES)
@@ -52,9 +53,10 @@ def test_tuner_selects_and_caches(tmp_path):
             rec = table[n]
             assert rec["variant"] in AT.variants(n)
             assert AT.get_variant(n) == rec["variant"]
-            assert set(rec["ms"]) == {str(v) for v in AT.variants(n)}
+            # every variant but bf16 hi/lo (about 16 bits: selectable, never tuned in)
+            assert set(rec["ms"]) == {str(v) for v in AT.variants(n) if v != AT.GEMM + 1}
             assert rec["rejected"] == []                  # every variant reproduces the default
-            fastest = min(rec["ms"][str(v)] for v in AT.variants(n))
+            fastest = min(rec["ms"].values())
             chosen = rec["ms"][str(rec["variant"])]
             assert chosen == fastest or (rec["variant"] == rec["default"] and fastest > 0.97 * chosen)
         stored = json.load(open(cache))
