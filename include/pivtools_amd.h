@@ -268,6 +268,14 @@ int pivamd_warp_u16(const unsigned short *d_imgA, const unsigned short *d_imgB, 
                     int nPY, int nPX, const float *d_ctrY, const float *d_ctrX, int interp_mode,
                     int shared_predictor, int round_shifts, void *stream);
 
+/* Form of the interp_mode 2 warp, process-global (no counterpart in the
+ * reference beyond its own form selector, fused_warp.h:21-22): 1 (default)
+ * stages each tile's source rectangles of A and B in LDS as floats, 0 reads
+ * every tap from global memory.  Both give the same values (signed zeros
+ * aside).  set returns PIVAMD_ERR_ARG for other values. */
+int pivamd_warp_set_variant(int v);
+int pivamd_warp_get_variant(void);
+
 /* Ensemble sums over device buffers.  triple = 1: bulkxcorr2d_accumulate_triple
  * (AB, AA, BB with optional mean removal and per-pair normalisation; sumW*
  * are the weight sums of the mean removal); triple = 0:

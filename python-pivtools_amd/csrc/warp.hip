@@ -22,10 +22,14 @@
 // predictor y weights and row offsets of each tile row are staged in LDS
 // once; image taps take one bounds test per stencil (interior) or one per
 // tap (border), and the half-shifts are clamped so that a non-finite
-// predictor can never produce an out-of-range read.  The image taps are read
-// straight from global memory (wide unaligned loads): per pixel the kernel
-// issues about 400 VALU instructions against about 8 loads, so it is VALU
-// bound, and staging the source window in LDS measured slower.
+// predictor can never produce an out-of-range read.  The reference-exact
+// forms read the taps straight from global memory (wide unaligned loads):
+// they are VALU bound at about 400 instructions per pixel, and staging the
+// source window in LDS measured slower for them.  The FMA form (mode 2) by
+// default stages it: a 512-lane workgroup covers a 64 x 32 tile, converts
+// the source rectangles of A and B to float in LDS once, and each tap is
+// then an LDS read, which took the form from about 310 to about 215 VALU
+// instructions per pixel pair.
 #include <hip/hip_runtime.h>
 
 #include "pivamd_dev.h"
@@ -111,14 +115,18 @@ __device__ __forceinline__ float sample_bicubic(const TI *img, float fy, float f
     return val;
 }
 
-// Keys a = -0.75 weights with fused multiply-adds (the same polynomials).
+// Keys a = -0.75 weights with fused multiply-adds: the same four cubics
+// expanded in powers of d (0 <= d < 1) and evaluated by Horner's rule,
+// 11 operations instead of 17:
+//   w0 = -0.75 d^3 + 1.5 d^2 - 0.75 d     w1 = 1.25 d^3 - 2.25 d^2 + 1
+//   w2 = -1.25 d^3 + 1.5 d^2 + 0.75 d     w3 = 0.75 d^3 - 0.75 d^2
 __device__ __forceinline__ void keys4_fma(float d, float w[4])
 {
-    const float s0 = 1.0f + d, q2 = 1.0f - d, s3 = 2.0f - d;
-    w[1] = __builtin_fmaf(__builtin_fmaf(1.25f, d, -2.25f) * d, d, 1.0f);
-    w[2] = __builtin_fmaf(__builtin_fmaf(1.25f, q2, -2.25f) * q2, q2, 1.0f);
-    w[0] = __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(-0.75f, s0, 3.75f), s0, -6.0f), s0, 3.0f);
-    w[3] = __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(-0.75f, s3, 3.75f), s3, -6.0f), s3, 3.0f);
+    const float d2 = d * d;
+    w[0] = __builtin_fmaf(__builtin_fmaf(-0.75f, d, 1.5f), d, -0.75f) * d;
+    w[1] = __builtin_fmaf(__builtin_fmaf(1.25f, d, -2.25f), d2, 1.0f);
+    w[2] = __builtin_fmaf(__builtin_fmaf(-1.25f, d, 1.5f), d, 0.75f) * d;
+    w[3] = __builtin_fmaf(0.75f, d, -0.75f) * d2;
 }
 
 // The separable bicubic sample: sum_m wy[m] * (sum_n wx[n] * img[m][n]),
@@ -264,7 +272,31 @@ struct WarpArgs {
 };
 
 constexpr int kPredMax = 1024;   // predictor cells a tile may stage in LDS
+constexpr int kPredMaxStaged = 256;   // the same with the source frames staged too
 constexpr int kRowSums = 8;      // predictor rows whose x sums a FAST tile keeps in LDS
+
+// Source staging (the FAST form's default): each frame's source rectangle of
+// the tile, converted to float, zero outside the image, kRegH rows of kRegP
+// columns.  3*kRegP + 3 = 255 keeps the 4x4 taps of a stencil within the
+// 8-bit dword offsets of ds_read2_b32 from one base address.
+constexpr int kRegP = 84, kRegH = 48;
+// The staged form's tile: 64 x 32 pixels for a 512-lane workgroup (each lane
+// one column and four rows), so the per-tile set-up and the source halo are
+// shared by twice the pixels of the unstaged 64 x 16 tile.
+constexpr int kTileYS = 32, kLanesYS = 8;
+// Predictor magnitudes above this (or non-finite) leave the tile unstaged, so
+// a staged row lies within kStageRowsMax rows of the frame.
+constexpr float kStageShiftMax = 4096.0f;
+constexpr int kStageRowsMax = 4300;
+
+template <typename TI>
+__device__ __forceinline__ float load_tap(__amdgpu_buffer_rsrc_t r, uint32_t off_bytes)
+{
+    if constexpr (sizeof(TI) == 2)
+        return (float)__builtin_amdgcn_raw_buffer_load_b16(r, off_bytes, 0, 0);
+    else
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off_bytes, 0, 0));
+}
 
 template <typename TI>
 __device__ __forceinline__ const TI *warp_src(const WarpArgs &g, bool a)
@@ -279,20 +311,36 @@ __device__ __forceinline__ const TI *warp_src(const WarpArgs &g, bool a)
 constexpr float kShiftMax = 1048576.0f;
 
 // TI: float frames, or 16-bit counts (converted exactly on load).  FAST:
-// the separable fused-multiply-add form of mode 2.
-template <typename TI, bool FAST>
-__global__ void __launch_bounds__(256) warp_kernel(WarpArgs g)
+// the separable fused-multiply-add form of mode 2.  STAGE (FAST only): the
+// tile's source rectangles of A and B are staged in LDS as floats first, so
+// each tap is an LDS read instead of a global load, an unpack and a convert.
+// The rectangles are bounded from the staged predictor cells: a Keys bicubic
+// stencil has at most 0.4454 negative weight mass (1.375^2 = 1 + 2 * 0.4453),
+// so the interpolated predictor stays within half the cells' range beyond
+// their extremes.  Zeros stand for pixels outside the image, which gives the
+// border stencils' sums exactly; a stencil outside the rectangle (never, for
+// finite arithmetic) reads global memory as the unstaged form does.
+template <typename TI, bool FAST, bool STAGE = false>
+__global__ void __launch_bounds__(STAGE ? 512 : 256, STAGE ? 8 : 1) warp_kernel(WarpArgs g)
 {
-    __shared__ float s_ly[kTileY];
+    static_assert(FAST || !STAGE, "staging is a form of the FMA warp");
+    constexpr int kPM = STAGE ? kPredMaxStaged : kPredMax;
+    constexpr int TY = STAGE ? kTileYS : kTileY;      // tile rows
+    constexpr int LY = STAGE ? kLanesYS : kLanesY;    // lane rows (TY / LY rows per lane)
+    constexpr int NT = kTileX * LY;                   // lanes
+    __shared__ float s_ly[TY];
     __shared__ float s_lx[kTileX];
-    __shared__ float4 s_wy[kTileY];     // predictor y weights of each tile row
-    __shared__ int4 s_ro[kTileY];       // predictor row offsets of each tile row
-    __shared__ float s_pdy[kPredMax];
-    __shared__ float s_pdx[kPredMax];
+    __shared__ float4 s_wy[TY];         // predictor y weights of each tile row
+    __shared__ int4 s_ro[TY];           // predictor row offsets of each tile row
+    __shared__ float s_pdy[kPM];
+    __shared__ float s_pdx[kPM];
     // FAST: x sums sum_n wx[n] P[r][c_n] of each staged predictor row r at
     // each tile column; a pixel then needs only sum_m wy[m] * S[r_m]
     __shared__ float s_sy[FAST ? kRowSums * kTileX : 1];
     __shared__ float s_sx[FAST ? kRowSums * kTileX : 1];
+    // STAGE: the source rectangles (A then B) and their origins and heights
+    __shared__ float s_reg[STAGE ? 2 * kRegH * kRegP : 1];
+    __shared__ int s_rg[8];
     const int tid = (int)threadIdx.x;
     const int tx = tid & (kTileX - 1);
     const int ty = tid / kTileX;
@@ -305,13 +353,13 @@ __global__ void __launch_bounds__(256) warp_kernel(WarpArgs g)
     // columns moves every XCD on by one column per tile row for any width.
     const int rot = (int)(gridDim.x & 7u) + 1;
     const int bx = (int)((blockIdx.x + blockIdx.y * rot) % gridDim.x);
-    const int x0 = bx * kTileX, y0 = blockIdx.y * kTileY;
+    const int x0 = bx * kTileX, y0 = blockIdx.y * TY;
     const int img = blockIdx.z;
-    const int nx_t = min(kTileX, g.W - x0), ny_t = min(kTileY, g.H - y0);
+    const int nx_t = min(kTileX, g.W - x0), ny_t = min(TY, g.H - y0);
     if (tid < kTileX) {
         const int x = x0 + min(tid, nx_t - 1);
         s_lx[tid] = pred_index((float)x, g.cx, g.nPX);
-    } else if (tid < kTileX + kTileY) {
+    } else if (tid < kTileX + TY) {
         const int y = y0 + min(tid - kTileX, ny_t - 1);
         s_ly[tid - kTileX] = pred_index((float)y, g.cy, g.nPY);
     }
@@ -325,15 +373,15 @@ __global__ void __launch_bounds__(256) warp_kernel(WarpArgs g)
     if (tid < kTileX) {
         const float v = s_lx[tid];
         bad = !(v == v) || (tid + 1 < kTileX && !(v <= s_lx[tid + 1]));
-    } else if (tid < kTileX + kTileY) {
+    } else if (tid < kTileX + TY) {
         const int k = tid - kTileX;
         const float v = s_ly[k];
-        bad = !(v == v) || (k + 1 < kTileY && !(v <= s_ly[k + 1]));
+        bad = !(v == v) || (k + 1 < TY && !(v <= s_ly[k + 1]));
     }
     const bool mono = !__syncthreads_or(bad);
     int pr0 = 0, pr1 = -1, pc0 = 0, pc1 = -1;
     if (mono) {
-        const int fy0 = (int)floorf(s_ly[0]), fy1 = (int)floorf(s_ly[kTileY - 1]);
+        const int fy0 = (int)floorf(s_ly[0]), fy1 = (int)floorf(s_ly[TY - 1]);
         const int fx0 = (int)floorf(s_lx[0]), fx1 = (int)floorf(s_lx[kTileX - 1]);
         pr0 = min(max(fy0 - 1, 0), g.nPY - 1);
         pr1 = min(max(fy1 + 2, 0), g.nPY - 1);
@@ -341,11 +389,11 @@ __global__ void __launch_bounds__(256) warp_kernel(WarpArgs g)
         pc1 = min(max(fx1 + 2, 0), g.nPX - 1);
     }
     const int pcw = pc1 - pc0 + 1, prh = pr1 - pr0 + 1;
-    const bool staged = mono && pr1 >= pr0 && pc1 >= pc0 && pcw * prh <= kPredMax;
+    const bool staged = mono && pr1 >= pr0 && pc1 >= pc0 && pcw * prh <= kPM;
     const float *dyp = g.pdy + (size_t)img * g.predStride;
     const float *dxp = g.pdx + (size_t)img * g.predStride;
     if (staged) {
-        for (int e = tid; e < pcw * prh; e += 256) {
+        for (int e = tid; e < pcw * prh; e += NT) {
             const int r = e / pcw, c = e - r * pcw;
             s_pdy[e] = dyp[(pr0 + r) * g.nPX + pc0 + c];
             s_pdx[e] = dxp[(pr0 + r) * g.nPX + pc0 + c];
@@ -354,7 +402,7 @@ __global__ void __launch_bounds__(256) warp_kernel(WarpArgs g)
     const bool rowsums = FAST && staged && prh <= kRowSums;
     // y weights and row offsets of the predictor stencil, once per tile row
     // (FAST with row sums: the staged row index instead of the offset)
-    if (tid < kTileY) {
+    if (tid < TY) {
         const float fiy = s_ly[tid], fiyf = floorf(fiy);
         const int iyb = (int)fiyf - 1;
         float wy[4];
@@ -371,6 +419,52 @@ __global__ void __launch_bounds__(256) warp_kernel(WarpArgs g)
     __syncthreads();
 
     if constexpr (FAST) {
+        if constexpr (STAGE) {
+            // wave 0: bound the tile's half-shifts from the staged predictor
+            // cells and place the source rectangles of A and B
+            if (tid < 64) {
+                int ok = 0, r0a = 0, c0a = 0, ha = 0, r0b = 0, c0b = 0, hb = 0;
+                if (staged) {
+                    float mny = INFINITY, mxy = -INFINITY, mnx = INFINITY, mxx = -INFINITY;
+                    bool bad = false;
+                    for (int e = tid; e < pcw * prh; e += 64) {
+                        const float vy = s_pdy[e], vx = s_pdx[e];
+                        bad = bad || !(fabsf(vy) <= kStageShiftMax) || !(fabsf(vx) <= kStageShiftMax);
+                        mny = fminf(mny, vy); mxy = fmaxf(mxy, vy);
+                        mnx = fminf(mnx, vx); mxx = fmaxf(mxx, vx);
+                    }
+#pragma unroll
+                    for (int o = 32; o; o >>= 1) {
+                        mny = fminf(mny, __shfl_xor(mny, o));
+                        mxy = fmaxf(mxy, __shfl_xor(mxy, o));
+                        mnx = fminf(mnx, __shfl_xor(mnx, o));
+                        mxx = fmaxf(mxx, __shfl_xor(mxx, o));
+                    }
+                    if (!__any(bad)) {
+                        // half the interpolant's bound, widened by one pixel
+                        // for rounding and round_shifts
+                        const float ry = mxy - mny, rx = mxx - mnx;
+                        const float hyl = 0.5f * (mny - 0.5f * ry) - 1.0f, hyh = 0.5f * (mxy + 0.5f * ry) + 1.0f;
+                        const float hxl = 0.5f * (mnx - 0.5f * rx) - 1.0f, hxh = 0.5f * (mxx + 0.5f * rx) + 1.0f;
+                        const float ylo = (float)y0, yhi = (float)(y0 + ny_t - 1);
+                        const float xlo = (float)x0, xhi = (float)(x0 + nx_t - 1);
+                        r0a = (int)floorf(ylo - hyh) - 1;
+                        c0a = (int)floorf(xlo - hxh) - 1;
+                        r0b = (int)floorf(ylo + hyl) - 1;
+                        c0b = (int)floorf(xlo + hxl) - 1;
+                        ha = (int)floorf(yhi - hyl) + 3 - r0a;
+                        hb = (int)floorf(yhi + hyh) + 3 - r0b;
+                        const int wa = (int)floorf(xhi - hxl) + 3 - c0a;
+                        const int wb = (int)floorf(xhi + hxh) + 3 - c0b;
+                        ok = ha <= kRegH && hb <= kRegH && wa <= kRegP && wb <= kRegP;
+                    }
+                }
+                if (tid == 0) {
+                    s_rg[0] = ok; s_rg[1] = r0a; s_rg[2] = c0a; s_rg[3] = ha;
+                    s_rg[4] = r0b; s_rg[5] = c0b; s_rg[6] = hb; s_rg[7] = 0;
+                }
+            }
+        }
         if (rowsums) {
             const float fxs = s_lx[tx], fxsf = floorf(fxs);
             const int ixs = (int)fxsf - 1;
@@ -379,7 +473,7 @@ __global__ void __launch_bounds__(256) warp_kernel(WarpArgs g)
             int cs[4];
 #pragma unroll
             for (int n = 0; n < 4; n++) cs[n] = min(max(ixs + n, 0), g.nPX - 1) - pc0;
-            for (int r = ty; r < prh; r += kLanesY) {
+            for (int r = ty; r < prh; r += LY) {
                 const float *py = s_pdy + r * pcw, *px = s_pdx + r * pcw;
                 float ry = wxs[0] * py[cs[0]], rx = wxs[0] * px[cs[0]];
 #pragma unroll
@@ -390,12 +484,14 @@ __global__ void __launch_bounds__(256) warp_kernel(WarpArgs g)
                 s_sy[r * kTileX + tx] = ry;
                 s_sx[r * kTileX + tx] = rx;
             }
-            __syncthreads();
         }
+        if (rowsums || STAGE) __syncthreads();
     }
 
     const int j = x0 + tx;
-    if (j >= g.W) return;
+    if constexpr (!STAGE) {
+        if (j >= g.W) return;
+    }
     const size_t plane = (size_t)g.H * g.W;
     const TI *A = warp_src<TI>(g, true) + (size_t)img * plane;
     const TI *B = warp_src<TI>(g, false) + (size_t)img * plane;
@@ -411,11 +507,8 @@ __global__ void __launch_bounds__(256) warp_kernel(WarpArgs g)
 #pragma unroll
     for (int n = 0; n < 4; n++) cols[n] = min(max(ix + n, 0), g.nPX - 1);
 
-#pragma unroll
-    for (int r = 0; r < kTileY / kLanesY; ++r) {
-        const int ly = ty + r * kLanesY;
-        const int i = y0 + ly;
-        if (i >= g.H) break;
+    // the half-shifts (hy, hx) of tile row ly at this lane's column
+    auto half_shifts = [&](int ly, float &hy, float &hx) {
         const float4 wy4 = s_wy[ly];
         const int4 ro4 = s_ro[ly];
         const float wy[4] = {wy4.x, wy4.y, wy4.z, wy4.w};
@@ -469,10 +562,120 @@ __global__ void __launch_bounds__(256) warp_kernel(WarpArgs g)
                 }
             }
         }
-        float hy = 0.5f * va, hx = 0.5f * vb;
+        hy = 0.5f * va;
+        hx = 0.5f * vb;
         if (g.roundShifts) { hy = roundf(hy); hx = roundf(hx); }
         hy = fminf(fmaxf(hy, -kShiftMax), kShiftMax);
         hx = fminf(fmaxf(hx, -kShiftMax), kShiftMax);
+    };
+
+    if constexpr (STAGE) {
+        const int ok = s_rg[0];
+        const int r0a = s_rg[1], c0a = s_rg[2], ha = s_rg[3];
+        const int r0b = s_rg[4], c0b = s_rg[5], hb = s_rg[6];
+        // lane -> (column c of the rectangle, row phase grp); rows step by
+        // kGroups, and every lane stages kSteps rows of each rectangle
+        // (kGroups * kSteps = kRegH; rows past its height are never read).
+        // Byte offsets are 32-bit: a row above the frame wraps to 2^31 or
+        // more and a column outside it is moved there, so both lie beyond
+        // the frame's num_records and read 0, as does a row below the frame
+        // (the host keeps kStageRowsMax rows either side addressable this
+        // way).  All loads of both rectangles are issued before the first
+        // LDS write, and the half-shifts are computed while they are in flight.
+        constexpr int kGroups = NT / kRegP;
+        constexpr int kSteps = kRegH / kGroups;
+        static_assert(kGroups * kSteps == kRegH, "staged rows");
+        const bool stager = ok && tid < kGroups * kRegP;
+        const int grp = tid / kRegP;
+        const int c = tid - grp * kRegP;
+        const uint32_t rstep = (uint32_t)(kGroups * g.W) * (uint32_t)sizeof(TI);
+        float tv[2][kSteps];
+        if (stager) {
+            const int bytes = (int)(plane * sizeof(TI));
+#pragma unroll
+            for (int f = 0; f < 2; f++) {
+                const int r0 = f ? r0b : r0a, c0 = f ? c0b : c0a;
+                const __amdgpu_buffer_rsrc_t rs =
+                    __builtin_amdgcn_make_buffer_rsrc((void *)(f ? B : A), (short)0, bytes, 0x00020000);
+                const int cc = c0 + c;
+                uint32_t off = (uint32_t)((r0 + grp) * g.W + cc) * (uint32_t)sizeof(TI);
+                if ((unsigned)cc >= (unsigned)g.W) off = 0x80000000u;
+#pragma unroll
+                for (int k = 0; k < kSteps; k++) {
+                    tv[f][k] = load_tap<TI>(rs, off);
+                    off += rstep;
+                }
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        float hys[TY / LY], hxs[TY / LY];
+#pragma unroll
+        for (int r = 0; r < TY / LY; ++r) half_shifts(ty + r * LY, hys[r], hxs[r]);
+        __builtin_amdgcn_sched_barrier(0);
+        if (stager) {
+#pragma unroll
+            for (int f = 0; f < 2; f++) {
+                float *dst = s_reg + f * (kRegH * kRegP) + grp * kRegP + c;
+#pragma unroll
+                for (int k = 0; k < kSteps; k++) dst[k * kGroups * kRegP] = tv[f][k];
+            }
+        }
+        __syncthreads();
+        if (j >= g.W) return;
+        // one tap sum from the staged rectangle R (origin r0, c0; h rows),
+        // the same terms in the same order as sample_bicubic_fast
+        auto sample_staged = [&](const float *R, int r0, int c0, int h, const TI *src, const FrameBuf &fb,
+                                 float fy, float fxp) -> float {
+            const float fyf = floorf(fy), fxpf = floorf(fxp);
+            const int ry = (int)fyf - 1 - r0, rx = (int)fxpf - 1 - c0;
+            if (ok && (unsigned)ry <= (unsigned)(h - 4) && (unsigned)rx <= (unsigned)(kRegP - 4)) {
+                float wys[4], wxs[4];
+                keys4_fma(fy - fyf, wys);
+                keys4_fma(fxp - fxpf, wxs);
+                // ry < kRegH and rx < kRegP here: a 24-bit multiply-add
+                const float *p = R + (__umul24((unsigned)ry, (unsigned)kRegP) + (unsigned)rx);
+                float val = 0.0f;
+#pragma unroll
+                for (int m = 0; m < 4; m++) {
+                    float rs = wxs[0] * p[m * kRegP];
+                    rs = __builtin_fmaf(wxs[1], p[m * kRegP + 1], rs);
+                    rs = __builtin_fmaf(wxs[2], p[m * kRegP + 2], rs);
+                    rs = __builtin_fmaf(wxs[3], p[m * kRegP + 3], rs);
+                    val = __builtin_fmaf(wys[m], rs, val);
+                }
+                return val;
+            }
+            return sample_bicubic_fast(src, fy, fxp, g.H, g.W, hm3, wm3, fb);
+        };
+        // output planes as raw buffers based at this image: one 32-bit lane
+        // offset, the row step a uniform offset
+        const int obytes = (int)(plane * sizeof(float));
+        const __amdgpu_buffer_rsrc_t oa =
+            __builtin_amdgcn_make_buffer_rsrc((void *)(g.outA + (size_t)img * plane), (short)0, obytes, 0x00020000);
+        const __amdgpu_buffer_rsrc_t ob =
+            __builtin_amdgcn_make_buffer_rsrc((void *)(g.outB + (size_t)img * plane), (short)0, obytes, 0x00020000);
+        const int vo = ((y0 + ty) * g.W + j) * (int)sizeof(float);
+#pragma unroll
+        for (int r = 0; r < TY / LY; ++r) {
+            const int i = y0 + ty + r * LY;
+            if (i >= g.H) break;
+            const int so = r * LY * g.W * (int)sizeof(float);
+            const float va = sample_staged(s_reg, r0a, c0a, ha, A, fbA, (float)i - hys[r], (float)j - hxs[r]);
+            const float vb = sample_staged(s_reg + kRegH * kRegP, r0b, c0b, hb, B, fbB, (float)i + hys[r],
+                                           (float)j + hxs[r]);
+            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, va), oa, vo, so, 0);
+            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, vb), ob, vo, so, 0);
+        }
+        return;
+    }
+
+#pragma unroll
+    for (int r = 0; r < kTileY / kLanesY; ++r) {
+        const int ly = ty + r * kLanesY;
+        const int i = y0 + ly;
+        if (i >= g.H) break;
+        float hy, hx;
+        half_shifts(ly, hy, hx);
         const size_t o = (size_t)img * plane + (size_t)i * g.W + j;
         if constexpr (FAST) {
             g.outA[o] = sample_bicubic_fast(A, (float)i - hy, (float)j - hx, g.H, g.W, hm3, wm3, fbA);
@@ -527,6 +730,20 @@ static const float *lanczos_lut_device()
 
 using namespace pivamd;
 
+// Form of the FMA warp (mode 2): 1 stages each tile's source rectangles in
+// LDS (the default), 0 reads every tap from global memory.  Process-global,
+// like the reference's fused_warp_set_impl selector.
+static int g_warp_variant = 1;
+
+extern "C" int pivamd_warp_set_variant(int v)
+{
+    if (v < 0 || v > 1) return PIVAMD_ERR_ARG;
+    g_warp_variant = v;
+    return PIVAMD_OK;
+}
+
+extern "C" int pivamd_warp_get_variant(void) { return g_warp_variant; }
+
 static int warp_entry(const float *d_imgA, const float *d_imgB, const unsigned short *d_imgA16,
                       const unsigned short *d_imgB16, float *d_outA, float *d_outB, const float *d_predDy,
                       const float *d_predDx, int N, int H, int W, int nPY, int nPX, const float *d_ctrY,
@@ -553,7 +770,18 @@ static int warp_entry(const float *d_imgA, const float *d_imgB, const unsigned s
         if (!g.lut) return PIVAMD_ERR_NOMEM;
     }
     dim3 grid((W + kTileX - 1) / kTileX, (H + kTileY - 1) / kTileY, N);
-    if (g.mode == 2) {
+    // the staged form addresses a frame with 32-bit byte offsets
+    // (and the output planes, 4 bytes per pixel, below 2 GiB); rows within
+    // kStageRowsMax of the frame stay below 2^31 bytes either side of it
+    const size_t row_bytes = (size_t)W * (d_imgA16 ? 2 : 4);
+    const bool stage = g_warp_variant == 1 && (size_t)H * W < ((size_t)1 << 29) &&
+                       ((size_t)H + kStageRowsMax) * row_bytes < ((size_t)1 << 31);
+    if (g.mode == 2 && stage) {
+        dim3 grid_s((W + kTileX - 1) / kTileX, (H + kTileYS - 1) / kTileYS, N);
+        const int nt = kTileX * kLanesYS;
+        if (d_imgA16) hipLaunchKernelGGL((warp_kernel<unsigned short, true, true>), grid_s, dim3(nt), 0, st, g);
+        else hipLaunchKernelGGL((warp_kernel<float, true, true>), grid_s, dim3(nt), 0, st, g);
+    } else if (g.mode == 2) {
         if (d_imgA16) hipLaunchKernelGGL((warp_kernel<unsigned short, true>), grid, dim3(256), 0, st, g);
         else hipLaunchKernelGGL((warp_kernel<float, true>), grid, dim3(256), 0, st, g);
     } else {

@@ -107,6 +107,10 @@ def core() -> ctypes.CDLL:
         L.pivamd_warp.restype = _i
         L.pivamd_warp_u16.argtypes = L.pivamd_warp.argtypes
         L.pivamd_warp_u16.restype = _i
+        L.pivamd_warp_set_variant.argtypes = [_i]
+        L.pivamd_warp_set_variant.restype = _i
+        L.pivamd_warp_get_variant.argtypes = []
+        L.pivamd_warp_get_variant.restype = _i
         L.pivamd_peakfit.argtypes = [_vp, _i, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp]
         L.pivamd_peakfit.restype = _i
         L.pivamd_infill_ls_workspace.argtypes = [_i, _i, _i]

@@ -772,6 +772,65 @@ def _warp_fp64(img_a, img_b, pdy, pdx, cy, cx):
     return outs
 
 
+def _warp_both_forms(L, frames, pdy, pdx, cy, cx, n, H, W, shared, rnd):
+    """The FMA warp (mode 2) unstaged (form 0) and LDS-staged (form 1)."""
+    u16 = frames.dtype == torch.int16
+    fn = L.pivamd_warp_u16 if u16 else L.pivamd_warp
+    outs = []
+    try:
+        for v in (0, 1):
+            assert L.pivamd_warp_set_variant(v) == 0
+            oa = torch.full((n, H, W), -7.0, device=dev)
+            ob = torch.full((n, H, W), -7.0, device=dev)
+            assert fn(frames[0].data_ptr(), frames[1].data_ptr(), oa.data_ptr(), ob.data_ptr(), pdy.data_ptr(),
+                      pdx.data_ptr(), n, H, W, len(cy), len(cx), cy.data_ptr(), cx.data_ptr(), 2, shared, rnd,
+                      _st()) == 0
+            outs.append((oa, ob))
+        torch.cuda.synchronize()
+    finally:
+        L.pivamd_warp_set_variant(1)
+    return outs
+
+
+@pytest.mark.parametrize("case", ["random", "smooth", "nan", "huge", "round", "shared", "float"])
+def test_staged_fma_warp_equals_unstaged(case):
+    """The LDS-staged FMA warp (the default) gives the unstaged form's values
+    at every pixel: random predictors whose stencils reach every border,
+    a smooth field on a larger frame, tiles that fall back to global reads
+    (a NaN predictor cell, shifts beyond the staging bound), rounded shifts,
+    a shared predictor and float frames."""
+    L = _lib()
+    assert L.pivamd_warp_get_variant() == 1
+    rng = np.random.default_rng({"random": 21, "smooth": 22, "nan": 23, "huge": 24, "round": 25,
+                                 "shared": 26, "float": 27}[case])
+    n, H, W, nPY, nPX = 2, 70, 90, 5, 6
+    if case == "smooth":
+        n, H, W, nPY, nPX = 2, 333, 517, 11, 17
+    img = rng.integers(0, 4096, (2, n, H, W))
+    frames = (_t(img.astype(np.float32)) if case == "float"
+              else _t(img.astype(np.int16), torch.int16))
+    shared = 1 if case == "shared" else 0
+    npred = 1 if shared else n
+    if case == "smooth":
+        yy, xx = np.meshgrid(np.linspace(0, 1, nPY), np.linspace(0, 1, nPX), indexing="ij")
+        pdy_h = np.stack([3.0 * np.sin(2 * yy + k) + 1.0 for k in range(npred)]).astype(np.float32)
+        pdx_h = np.stack([4.0 * np.cos(3 * xx - k) - 2.0 for k in range(npred)]).astype(np.float32)
+    else:
+        pdy_h = rng.normal(0, 2, (npred, nPY, nPX)).astype(np.float32)
+        pdx_h = rng.normal(0, 2, (npred, nPY, nPX)).astype(np.float32)
+    if case == "nan":
+        pdy_h[0, 2, 3] = np.nan
+    if case == "huge":
+        pdx_h[1] += 5000.0
+    cy = _t(np.linspace(-10, H + 10, nPY).astype(np.float32))
+    cx = _t(np.linspace(-10, W + 10, nPX).astype(np.float32))
+    (a0, b0), (a1, b1) = _warp_both_forms(L, frames, _t(pdy_h), _t(pdx_h), cy, cx, n, H, W, shared,
+                                          1 if case == "round" else 0)
+    for u, s_ in ((a0, a1), (b0, b1)):
+        assert torch.equal(torch.nan_to_num(u, 1e30), torch.nan_to_num(s_, 1e30))
+    assert (a1 != -7.0).all() and (b1 != -7.0).all()     # every pixel written
+
+
 def test_fma_warp_against_exact_warp_and_fp64():
     """The separable fused-multiply-add warp (mode 2) against the
     reference-exact warp (mode 0) and an FP64 evaluation of the same
