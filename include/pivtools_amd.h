@@ -311,6 +311,37 @@ int pivamd_unpack_mono12p(const void *d_raw, unsigned long frame_stride, int n_f
 int pivamd_bayer_rg8_superpixel(const void *d_raw, unsigned long frame_stride, int n_frames, int H, int W,
                                 const unsigned char *d_dark, unsigned short *d_out, void *stream);
 
+/* DaVis .set stream decode (device buffers): the reference decodes a frame
+ * stream's pixels, applies its file-backed corrections and its geometry on
+ * the host (pivtools_core/image_handling/readers/set_reader.py:1196-1480
+ * pixel encodings, :477-585 corrections and geometries, :1907-2000 the read
+ * path); this does all three in one pass.  encoding: 0 mono-10p,
+ * 1 mono-10pmsb, 2 mono-12p, 3 mono-12pmsb, 4 mono-12packed, 5 raw-16-bit,
+ * 6 raw-8-bit.  d_scmos, d_dark, d_bgnd: null or one H x W uint16 image in
+ * sensor coordinates each, applied in that order as clip(x - S + 300, 0,
+ * 65535), max(x - D, 0), max(x - B, 0).  geometry: 0 none, 1 rotate-180,
+ * 2 vertical-mirror, 3 horizontal mirror, 4 rotate-cw-90, 5 rotate-ccw-90,
+ * 6 rotate-cw-90 + vertical mirror, 7 rotate-ccw-90 + vertical mirror; 4-7
+ * write W x H frames.  Frame f's payload starts at d_raw + f * frame_stride;
+ * d_out holds n_frames output frames.  PIVAMD_ERR_SIZE when H x W does not
+ * fill whole groups of the packing.  pivamd_davis_payload_bytes gives one
+ * frame's payload bytes (0 for such a size or an unknown encoding). */
+unsigned long pivamd_davis_payload_bytes(int encoding, int H, int W);
+int pivamd_davis_decode(const void *d_raw, unsigned long frame_stride, int n_frames, int H, int W,
+                        int encoding, const unsigned short *d_scmos, const unsigned short *d_dark,
+                        const unsigned short *d_bgnd, int geometry, unsigned short *d_out, void *stream);
+
+/* The serial decompressors of the DaVis containers, host memory (no device
+ * work).  pivamd_davis_tokens decodes nvals values of the DaVis token stream
+ * (.im7 pack 1, IMX .set entries; davis_tokens.py:46-130) from data[*off_io]
+ * with running value *prev_io, updating both and *written; returns 0, or
+ * 1 truncated, 2 a 0x7f run past nvals, 3 a value outside 0..65535.
+ * pivamd_lz4_block decompresses one LZ4 block (.im7 pack 20, LZ4 .set
+ * entries; im7_reader.py:270-291) into dst and returns its size, or -1. */
+int pivamd_davis_tokens(const unsigned char *data, unsigned long n, unsigned long *off_io,
+                        unsigned short *out, unsigned long nvals, long long *prev_io, unsigned long *written);
+long long pivamd_lz4_block(const unsigned char *src, unsigned long n, unsigned char *dst, unsigned long cap);
+
 /* Multipass glue (device buffers, planar fields [dy plane, dx plane]):
  * uint16 ingest, displacement gate + predictor add-back, normalised-median
  * test, nearest infill, edge padding, one axis of the Gaussian predictor

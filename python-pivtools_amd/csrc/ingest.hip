@@ -157,6 +157,108 @@ static inline unsigned grid_x(size_t items)
     return (unsigned)(g > 4096 ? 4096 : (g ? g : 1));
 }
 
+
+// DaVis .set stream decode: the packed pixel encodings of a frame stream
+// (set_reader.py:1196-1480), its file-backed corrections in LaVision's order
+// (intensity-correction-1, dark-image-subtraction, background-subtraction,
+// each clipped: set_reader.py:477-565) and at most one geometry
+// (set_reader.py:566-585), fused into one pass.  Each lane writes one output
+// pixel: the geometry maps it to its sensor pixel k, which is cut out of the
+// two payload bytes that hold it.  Encodings, by sensor pixel k:
+//   0 mono-10p       LE16 at 5(k/4) + k%4, >> 2(k%4), & 0x3FF
+//   1 mono-10pmsb    BE16 at 5(k/4) + k%4, >> 6 - 2(k%4), & 0x3FF
+//   2 mono-12p       LE16 at 3(k/2) + k%2: & 0xFFF, or >> 4
+//   3 mono-12pmsb    BE16 at 3(k/2) + k%2: >> 4, or & 0xFFF
+//   4 mono-12packed  b0 << 4 | b1 & 0xF, or b2 << 4 | b1 >> 4
+//   5 raw-16-bit     LE16 at 2k
+//   6 raw-8-bit      byte k
+// Geometries (output (i, j) of an H x W sensor frame x):
+//   0 none x[i][j]; 1 rotate-180 x[H-1-i][W-1-j]; 2 vertical-mirror x[H-1-i][j];
+//   3 horizontal mirror x[i][W-1-j]; 4 rotate-cw-90 x[H-1-j][i];
+//   5 rotate-ccw-90 x[j][W-1-i]; 6 rotate-cw-90 + vertical mirror x[H-1-j][W-1-i];
+//   7 rotate-ccw-90 + vertical mirror x[j][i].
+// The 90-degree geometries give a W x H output.
+struct DavisArgs {
+    const uint8_t *raw;
+    size_t stride;                 // payload bytes from one frame to the next
+    const uint16_t *scmos, *dark, *bgnd;   // sensor-shape corrections, nullable
+    uint16_t *out;
+    int H, W, enc, geo;
+};
+
+__device__ __forceinline__ uint32_t le16(const uint8_t *b) { return (uint32_t)b[0] | (uint32_t)b[1] << 8; }
+__device__ __forceinline__ uint32_t be16(const uint8_t *b) { return (uint32_t)b[0] << 8 | (uint32_t)b[1]; }
+
+template <int ENC>
+__device__ __forceinline__ uint32_t davis_pixel(const uint8_t *src, size_t k)
+{
+    if constexpr (ENC == 0) {
+        const uint32_t m = (uint32_t)(k & 3);
+        return (le16(src + 5 * (k >> 2) + m) >> (2 * m)) & 0x3FFu;
+    } else if constexpr (ENC == 1) {
+        const uint32_t m = (uint32_t)(k & 3);
+        return (be16(src + 5 * (k >> 2) + m) >> (6 - 2 * m)) & 0x3FFu;
+    } else if constexpr (ENC == 2) {
+        const uint32_t w = le16(src + 3 * (k >> 1) + (k & 1));
+        return (k & 1) ? w >> 4 : w & 0xFFFu;
+    } else if constexpr (ENC == 3) {
+        const uint32_t w = be16(src + 3 * (k >> 1) + (k & 1));
+        return (k & 1) ? w & 0xFFFu : w >> 4;
+    } else if constexpr (ENC == 4) {
+        const uint8_t *b = src + 3 * (k >> 1);
+        return (k & 1) ? ((uint32_t)b[2] << 4 | (uint32_t)b[1] >> 4) : ((uint32_t)b[0] << 4 | ((uint32_t)b[1] & 0xFu));
+    } else if constexpr (ENC == 5) {
+        return le16(src + 2 * k);
+    } else {
+        return src[k];
+    }
+}
+
+template <int ENC>
+__global__ void __launch_bounds__(256) davis_decode_kernel(DavisArgs g)
+{
+    const bool swap = g.geo >= 4;
+    const int Ho = swap ? g.W : g.H, Wo = swap ? g.H : g.W;
+    const size_t npx = (size_t)g.H * g.W;
+    const uint8_t *src = g.raw + (size_t)blockIdx.y * g.stride;
+    uint16_t *dst = g.out + (size_t)blockIdx.y * npx;
+    const size_t step = (size_t)gridDim.x * blockDim.x;
+    for (size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x; o < npx; o += step) {
+        const int i = (int)(o / (size_t)Wo), j = (int)(o - (size_t)i * Wo);
+        int r, c;
+        switch (g.geo) {
+        case 1: r = g.H - 1 - i; c = g.W - 1 - j; break;
+        case 2: r = g.H - 1 - i; c = j; break;
+        case 3: r = i; c = g.W - 1 - j; break;
+        case 4: r = g.H - 1 - j; c = i; break;
+        case 5: r = j; c = g.W - 1 - i; break;
+        case 6: r = g.H - 1 - j; c = g.W - 1 - i; break;
+        case 7: r = j; c = i; break;
+        default: r = i; c = j; break;
+        }
+        (void)Ho;
+        const size_t k = (size_t)r * g.W + c;
+        int32_t v = (int32_t)davis_pixel<ENC>(src, k);
+        if (g.scmos) v = min(max(v - (int32_t)g.scmos[k] + 300, 0), 65535);
+        if (g.dark) v = max(v - (int32_t)g.dark[k], 0);
+        if (g.bgnd) v = max(v - (int32_t)g.bgnd[k], 0);
+        dst[o] = (uint16_t)v;
+    }
+}
+
+// Payload bytes of one frame of n pixels in encoding enc (0 when the pixel
+// count does not fill whole groups).
+__host__ __device__ inline size_t davis_payload_bytes(int enc, size_t n)
+{
+    switch (enc) {
+    case 0: case 1: return n % 4 ? 0 : n / 4 * 5;
+    case 2: case 3: case 4: return n % 2 ? 0 : n / 2 * 3;
+    case 5: return 2 * n;
+    case 6: return n;
+    default: return 0;
+    }
+}
+
 }  // namespace pivamd
 
 using namespace pivamd;
@@ -205,6 +307,42 @@ extern "C" int pivamd_bayer_rg8_superpixel(const void *d_raw, size_t frame_strid
     } else {
         if (d_dark) hipLaunchKernelGGL((bayer_rg8_kernel<false, true>), grid, dim3(256), 0, st, raw, frame_stride, H, W, d_dark, d_out);
         else hipLaunchKernelGGL((bayer_rg8_kernel<false, false>), grid, dim3(256), 0, st, raw, frame_stride, H, W, d_dark, d_out);
+    }
+    return PIVAMD_LAUNCHED();
+}
+
+
+extern "C" unsigned long pivamd_davis_payload_bytes(int encoding, int H, int W)
+{
+    if (H <= 0 || W <= 0) return 0;
+    return davis_payload_bytes(encoding, (size_t)H * W);
+}
+
+extern "C" int pivamd_davis_decode(const void *d_raw, size_t frame_stride, int n_frames, int H, int W,
+                                   int encoding, const uint16_t *d_scmos, const uint16_t *d_dark,
+                                   const uint16_t *d_bgnd, int geometry, uint16_t *d_out, hipStream_t st)
+{
+    if (n_frames < 0 || H < 0 || W < 0 || n_frames > 65535) return PIVAMD_ERR_ARG;
+    if (encoding < 0 || encoding > 6 || geometry < 0 || geometry > 7) return PIVAMD_ERR_ARG;
+    const size_t npx = (size_t)H * W;
+    if (n_frames == 0 || npx == 0) return 0;
+    const size_t need = davis_payload_bytes(encoding, npx);
+    if (need == 0) return PIVAMD_ERR_SIZE;    // the packing needs whole groups
+    if (frame_stride < need && n_frames > 1) return PIVAMD_ERR_ARG;
+    if (!d_raw || !d_out) return PIVAMD_ERR_ARG;
+    DavisArgs g;
+    g.raw = (const uint8_t *)d_raw; g.stride = frame_stride;
+    g.scmos = d_scmos; g.dark = d_dark; g.bgnd = d_bgnd; g.out = d_out;
+    g.H = H; g.W = W; g.enc = encoding; g.geo = geometry;
+    const dim3 grid(grid_x(npx), (unsigned)n_frames);
+    switch (encoding) {
+    case 0: hipLaunchKernelGGL(davis_decode_kernel<0>, grid, dim3(256), 0, st, g); break;
+    case 1: hipLaunchKernelGGL(davis_decode_kernel<1>, grid, dim3(256), 0, st, g); break;
+    case 2: hipLaunchKernelGGL(davis_decode_kernel<2>, grid, dim3(256), 0, st, g); break;
+    case 3: hipLaunchKernelGGL(davis_decode_kernel<3>, grid, dim3(256), 0, st, g); break;
+    case 4: hipLaunchKernelGGL(davis_decode_kernel<4>, grid, dim3(256), 0, st, g); break;
+    case 5: hipLaunchKernelGGL(davis_decode_kernel<5>, grid, dim3(256), 0, st, g); break;
+    default: hipLaunchKernelGGL(davis_decode_kernel<6>, grid, dim3(256), 0, st, g); break;
     }
     return PIVAMD_LAUNCHED();
 }

@@ -1,0 +1,135 @@
+// readers.cpp -- the serial decompressors of the DaVis containers, on the host.
+//
+// The camera containers the reference reads (pivtools_core/image_handling/
+// readers/) store a frame either packed (decoded on the device by
+// pivamd_davis_decode) or compressed with a serial code whose token
+// boundaries depend on the tokens themselves.  Those are decoded here, on
+// the host, into the packed payload that is then copied to the device:
+//
+//   DaVis token stream  .im7 pack 1 and IMX-compressed .set entries
+//                       (davis_tokens.py:1-136): tokens update a running
+//                       16-bit value --
+//                         int8 byte (not 0x7f/0x80/0x81)  signed delta
+//                         0x80 lo hi                      one absolute value
+//                         0x7f n (lo hi)*n                n absolute values
+//                         0x81 nibbles...                 signed 4-bit deltas,
+//                           high nibble first; a low nibble 0x8 ends the run
+//                           (a high nibble 0x8 is the delta -8).
+//   LZ4 block           .im7 pack 20 and LZ4-compressed .set entries
+//                       (im7_reader.py:343-363): the LZ4 block format, one
+//                       block with no frame header (a sequence is a token
+//                       byte, literal length 15 + 255-run extension, the
+//                       literals, a little-endian 16-bit match offset and
+//                       match length 4 + nibble + 255-run extension; the last
+//                       sequence carries literals only).
+#include <stddef.h>
+#include <stdint.h>
+#include <string.h>
+
+namespace {
+
+enum TokenStatus { kTokOk = 0, kTokTruncated = 1, kTokOverrun = 2, kTokRange = 3 };
+
+}  // namespace
+
+// Decode nvals running values from data[off:] into out.  prev is the running
+// value before the first token.  On return *off_io and *prev_io hold the
+// offset after the last token read and the last value, *written the values
+// written; the result is 0 or the first failure: 1 the stream ended inside a
+// token, 2 a 0x7f run passes nvals, 3 the value left 0..65535.
+extern "C" int pivamd_davis_tokens(const uint8_t *data, size_t n, size_t *off_io, uint16_t *out,
+                                   size_t nvals, long long *prev_io, size_t *written)
+{
+    size_t off = *off_io, i = 0;
+    long long prev = *prev_io;
+    int status = kTokOk;
+    while (i < nvals) {
+        if (off >= n) { status = kTokTruncated; break; }
+        const int b = data[off++];
+        if (b == 0x81) {
+            bool stop = false;
+            while (i < nvals && !stop) {
+                if (off >= n) { status = kTokTruncated; break; }
+                const int byte = data[off++];
+                const int hi = byte >> 4;
+                prev += hi >= 8 ? hi - 16 : hi;
+                if (prev < 0 || prev > 65535) { status = kTokRange; break; }
+                out[i++] = (uint16_t)prev;
+                if (i >= nvals) break;
+                const int lo = byte & 0xF;
+                if (lo == 8) { stop = true; break; }
+                prev += lo >= 8 ? lo - 16 : lo;
+                if (prev < 0 || prev > 65535) { status = kTokRange; break; }
+                out[i++] = (uint16_t)prev;
+            }
+            if (status != kTokOk) break;
+        } else if (b == 0x80) {
+            if (off + 2 > n) { status = kTokTruncated; break; }
+            prev = (long long)data[off] | ((long long)data[off + 1] << 8);
+            off += 2;
+            out[i++] = (uint16_t)prev;
+        } else if (b == 0x7F) {
+            if (off >= n) { status = kTokTruncated; break; }
+            const size_t cnt = data[off++];
+            if (i + cnt > nvals) { status = kTokOverrun; break; }
+            if (off + 2 * cnt > n) { status = kTokTruncated; break; }
+            for (size_t k = 0; k < cnt; ++k) {
+                prev = (long long)data[off] | ((long long)data[off + 1] << 8);
+                off += 2;
+                out[i++] = (uint16_t)prev;
+            }
+        } else {
+            prev += b >= 128 ? b - 256 : b;
+            if (prev < 0 || prev > 65535) { status = kTokRange; break; }
+            out[i++] = (uint16_t)prev;
+        }
+    }
+    *off_io = off;
+    *prev_io = prev;
+    *written = i;
+    return status;
+}
+
+// Decompress one LZ4 block of n bytes into dst (capacity cap).  Returns the
+// decompressed size, or -1 for a malformed block (a length running past the
+// input, a match offset of 0 or before the output start, output past cap).
+extern "C" long long pivamd_lz4_block(const uint8_t *src, size_t n, uint8_t *dst, size_t cap)
+{
+    size_t ip = 0, op = 0;
+    while (ip < n) {
+        const unsigned token = src[ip++];
+        size_t lit = token >> 4;
+        if (lit == 15) {
+            unsigned char s;
+            do {
+                if (ip >= n) return -1;
+                s = src[ip++];
+                lit += s;
+            } while (s == 255);
+        }
+        if (lit > n - ip || lit > cap - op) return -1;
+        memcpy(dst + op, src + ip, lit);
+        ip += lit;
+        op += lit;
+        if (ip == n) break;                      // the last sequence: literals only
+        if (n - ip < 2) return -1;
+        const size_t offset = (size_t)src[ip] | ((size_t)src[ip + 1] << 8);
+        ip += 2;
+        if (offset == 0 || offset > op) return -1;
+        size_t mlen = (token & 15u) + 4;
+        if ((token & 15u) == 15) {
+            unsigned char s;
+            do {
+                if (ip >= n) return -1;
+                s = src[ip++];
+                mlen += s;
+            } while (s == 255);
+        }
+        if (mlen > cap - op) return -1;
+        // byte by byte: a match may overlap its own output (offset < mlen)
+        const uint8_t *m = dst + op - offset;
+        for (size_t k = 0; k < mlen; ++k) dst[op + k] = m[k];
+        op += mlen;
+    }
+    return (long long)op;
+}

@@ -107,6 +107,13 @@ def core() -> ctypes.CDLL:
         L.pivamd_warp.restype = _i
         L.pivamd_warp_u16.argtypes = L.pivamd_warp.argtypes
         L.pivamd_warp_u16.restype = _i
+        L.pivamd_davis_payload_bytes.argtypes = [_i, _i, _i]
+        L.pivamd_davis_payload_bytes.restype = ctypes.c_ulong
+        L.pivamd_davis_tokens.argtypes = [_vp, _sz, ctypes.POINTER(_sz), _vp, _sz, ctypes.POINTER(ctypes.c_longlong),
+                                          ctypes.POINTER(_sz)]
+        L.pivamd_davis_tokens.restype = _i
+        L.pivamd_lz4_block.argtypes = [_vp, _sz, _vp, _sz]
+        L.pivamd_lz4_block.restype = ctypes.c_longlong
         L.pivamd_warp_set_variant.argtypes = [_i]
         L.pivamd_warp_set_variant.restype = _i
         L.pivamd_warp_get_variant.argtypes = []
@@ -180,6 +187,7 @@ GLUE_SIGNATURES: dict = {
     "pivamd_fill_linear": [_vp, _i, ctypes.c_long, _i, _vp, _i, _vp, _vp, _vp],
     "pivamd_unpack_mono12p": [_vp, _sz, _i, _i, _i, _vp, _vp, _vp],
     "pivamd_bayer_rg8_superpixel": [_vp, _sz, _i, _i, _i, _vp, _vp, _vp],
+    "pivamd_davis_decode": [_vp, _sz, _i, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _vp],
 }
 
 
