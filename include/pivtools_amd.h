@@ -446,6 +446,23 @@ unsigned long pivamd_infill_rbf_workspace(int N, int ny, int nx);
 int pivamd_infill_rbf(const float *u, const float *d_excl, int N, int ny, int nx, float py, float px,
                       int neighbors, int kernel, double epsilon, double smoothing, float *out, void *d_work,
                       unsigned long work_bytes, int *d_err, void *stream);
+
+/* Linear griddata infill (infilling.py:440-503 infill_griddata_linear:
+ * scipy griddata 'linear' over the valid vectors in physical coordinates,
+ * 'nearest' outside their convex hull).  Each NaN vector (not excluded) is
+ * the barycentric interpolation in a Delaunay triangle of the donors: the
+ * first triangle, in row-major order of its vertices, among the k nearest
+ * donors (k = 24, 48, 96, 192 in turn) that holds the hole, whose
+ * circumcircle holds no donor and lies within the k-th donor's distance;
+ * the nearest donor when no triangle holds the hole.  FP64, stored as
+ * float32; where cocircular donors leave several Delaunay triangles the
+ * first is taken (scipy's Qhull takes one by its own processing order).
+ * d_work: pivamd_infill_griddata_workspace(N, ny, nx) bytes.  *d_err bits:
+ * 1 an image with holes has no donor, 2 a hole's triangle reaches beyond 192
+ * donors, 4 the candidate search outgrew its buffer. */
+unsigned long pivamd_infill_griddata_workspace(int N, int ny, int nx);
+int pivamd_infill_griddata(const float *u, const float *d_excl, int N, int ny, int nx, float py, float px,
+                           float *out, void *d_work, unsigned long work_bytes, int *d_err, void *stream);
 int pivamd_fill_masked_nearest(const float *u, const float *d_mask, int N, int ny, int nx, float py,
                                float px, float *uo, int *d_err, void *stream);
 int pivamd_fill_linear(float *u, int planes, long plane_stride, int nf, const int *d_fill_idx, int nb,

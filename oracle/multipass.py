@@ -838,6 +838,8 @@ def run_multipass(A, B, grids, window_type="gaussian", finder=3, smoothing=True,
             u = infill_knn(u, sp_, exclude=m)
         elif method == "rbf_local":
             u = infill_rbf_local(u, sp_, exclude=m)
+        elif method == "griddata_linear" or (isinstance(method, dict) and method.get("method") == "griddata_linear"):
+            u = infill_griddata_linear(u, sp_, exclude=m)
         elif isinstance(method, dict) and method.get("method") == "rbf_local":
             u = infill_rbf_local(u, sp_, exclude=m, **method.get("parameters", {}))
         elif isinstance(method, dict) and method.get("method") == "knn":
@@ -852,4 +854,147 @@ def run_multipass(A, B, grids, window_type="gaussian", finder=3, smoothing=True,
             out.append({"ux": u[1], "uy": u[0], "nan_mask": nm, "nan_reason": reason, "peak_mag": pko,
                         "predictor": pred, "peak_choice": choice})
         prev_grid = g
+    return out
+
+
+def _griddata_scales(spacing):
+    """Axis scales for the triangulation: the pitches divided by their common
+    integer factor when both are integers (exact predicates), else the
+    pitches.  Scaling both axes alike changes neither the Delaunay triangles
+    nor the barycentric weights."""
+    py, px = float(spacing[0]), float(spacing[1])
+    if py == int(py) and px == int(px) and py > 0 and px > 0:
+        import math
+        g = math.gcd(int(py), int(px))
+        return py / g, px / g
+    return py, px
+
+
+def _triples(m):
+    """All index triples a < b < c of range(m), in lexicographic order."""
+    import itertools
+    return np.array(list(itertools.combinations(range(m), 3)), dtype=np.int64).reshape(-1, 3)
+
+
+def griddata_triangle(X, Y, covered_radius=None, chunk=8192):
+    """The triangle the griddata restatement uses for a hole at the origin,
+    among candidates (X, Y) in row-major order: the first (a, b, c) in
+    lexicographic order with nonzero area, the origin inside or on an edge,
+    no candidate strictly inside its circumcircle and, when covered_radius
+    is given, its circumcircle strictly within that distance of the origin.
+    Returns (a, b, c, area, oa, ob, oc), "uncovered" when the only triangles
+    found fail the last test, or None when no triangle holds the origin.
+    Triples are examined in lexicographic chunks, stopping at the first hit."""
+    m = X.size
+    if m < 3:
+        return None
+    t_all = _triples(m)
+    seen_empty = False
+    for s0 in range(0, t_all.shape[0], chunk):
+        t = t_all[s0:s0 + chunk]
+        a, b, c = t[:, 0], t[:, 1], t[:, 2]
+        ar = (X[b] - X[a]) * (Y[c] - Y[a]) - (Y[b] - Y[a]) * (X[c] - X[a])
+        oa = X[b] * Y[c] - Y[b] * X[c]
+        ob = X[c] * Y[a] - Y[c] * X[a]
+        oc = X[a] * Y[b] - Y[a] * X[b]
+        pos = (ar > 0) & (oa >= 0) & (ob >= 0) & (oc >= 0)
+        neg = (ar < 0) & (oa <= 0) & (ob <= 0) & (oc <= 0)
+        hold = np.flatnonzero(pos | neg)
+        if hold.size == 0:
+            continue
+        a, b, c, arh = a[hold], b[hold], c[hold], ar[hold]
+        adx, ady = X[a][:, None] - X[None, :], Y[a][:, None] - Y[None, :]
+        bdx, bdy = X[b][:, None] - X[None, :], Y[b][:, None] - Y[None, :]
+        cdx, cdy = X[c][:, None] - X[None, :], Y[c][:, None] - Y[None, :]
+        det = ((adx * adx + ady * ady) * (bdx * cdy - cdx * bdy)
+               + (bdx * bdx + bdy * bdy) * (cdx * ady - adx * cdy)
+               + (cdx * cdx + cdy * cdy) * (adx * bdy - bdx * ady))
+        det = det * np.sign(arh)[:, None]
+        empty = ~(det > 0).any(axis=1)      # a vertex gives det = 0 exactly
+        ok = empty.copy()
+        if covered_radius is not None:
+            bx, by = X[b] - X[a], Y[b] - Y[a]
+            cx, cy = X[c] - X[a], Y[c] - Y[a]
+            dd = 2.0 * (bx * cy - by * cx)
+            b2, c2 = bx * bx + by * by, cx * cx + cy * cy
+            ox = (cy * b2 - by * c2) / dd
+            oy = (bx * c2 - cx * b2) / dd
+            rho = np.sqrt(ox * ox + oy * oy)
+            gx, gy = ox + X[a], oy + Y[a]
+            ok &= np.sqrt(gx * gx + gy * gy) + rho < covered_radius
+        seen_empty |= bool(empty.any())
+        idx = np.flatnonzero(ok)
+        if idx.size:
+            j = hold[idx[0]]
+            return (t[j, 0], t[j, 1], t[j, 2], ar[j], oa[j], ob[j], oc[j])
+    return "uncovered" if seen_empty else None
+
+
+GRIDDATA_KS = (24, 48, 96, 192)
+
+
+def infill_griddata_linear(u, spacing, exclude=None, ks=GRIDDATA_KS):
+    """infill_griddata_linear (infilling.py:440-503) per image: NaN vectors
+    (not excluded) become the linear interpolation over a Delaunay
+    triangulation of the valid vectors in physical coordinates, and cells
+    outside the convex hull take the nearest valid vector.
+
+    The restatement, which the device follows step for step, for k = 24,
+    48, 96, 192 in turn until a triangle is found:
+      - candidates: the k nearest donors of the hole, ranked by (distance,
+        row-major index), then taken in row-major order;
+      - the triangle: griddata_triangle -- the first (a, b, c) holding the
+        hole whose circumcircle is empty of candidates and, unless every
+        donor is a candidate, lies strictly within the k-th candidate's
+        distance, so that it is empty of all donors and globally Delaunay;
+      - value: barycentric weights (ratios of orientation determinants, FP64)
+        applied to the vertices' components, stored as float32;
+      - no triangle holds the hole: the nearest candidate's value (the first
+        in (distance, index) order), as for cells outside the convex hull;
+      - triangles hold it but none is covered at k = 192: ValueError.
+    On a regular grid four donors around a hole are often cocircular: scipy's
+    Qhull then picks one of the equally Delaunay diagonals by its processing
+    order.  This rule picks the first in index order; both are Delaunay
+    interpolants, and they agree wherever the triangle is unique and on
+    linear fields."""
+    sy_, sx_ = _griddata_scales(spacing)
+    out = u.astype(np.float32).copy()
+    ny, nx = u.shape[2:]
+    rr, cc = np.mgrid[0:ny, 0:nx]
+    rr, cc = rr.ravel(), cc.ravel()
+    for i in range(u.shape[1]):
+        fy_all, fx_all = u[0, i].ravel(), u[1, i].ravel()
+        ex = np.zeros(ny * nx, bool) if exclude is None else np.asarray(exclude, bool).ravel()
+        fill = (np.isnan(fx_all) | np.isnan(fy_all)) & ~ex
+        if not fill.any():
+            continue
+        donor = ~fill & ~ex & np.isfinite(fx_all) & np.isfinite(fy_all)
+        didx = np.flatnonzero(donor)
+        if didx.size == 0:
+            raise ValueError("griddata_linear: no donor")
+        for q in np.flatnonzero(fill):
+            ry = (rr[didx] - rr[q]) * sy_
+            rx = (cc[didx] - cc[q]) * sx_
+            d = np.sqrt(ry * ry + rx * rx)
+            order = np.lexsort((didx, d))
+            for k in ks:
+                near = order[:k]
+                cand = np.sort(near)               # row-major order
+                tri = griddata_triangle(rx[cand], ry[cand], None if didx.size <= k else d[near[-1]])
+                if not isinstance(tri, str):
+                    break
+            if isinstance(tri, str):
+                raise ValueError(f"griddata_linear: a hole's Delaunay triangle reaches beyond its {ks[-1]} "
+                                 "nearest donors")
+            if tri is None:
+                j = didx[near[0]]
+                val = (float(fy_all[j]), float(fx_all[j]))
+            else:
+                a, b, c, ar, oa, ob, oc = tri
+                la, lb, lc = oa / ar, ob / ar, oc / ar
+                ia, ib, ic = didx[cand[a]], didx[cand[b]], didx[cand[c]]
+                val = (la * float(fy_all[ia]) + lb * float(fy_all[ib]) + lc * float(fy_all[ic]),
+                       la * float(fx_all[ia]) + lb * float(fx_all[ib]) + lc * float(fx_all[ic]))
+            out[0, i].ravel()[q] = np.float32(val[0])
+            out[1, i].ravel()[q] = np.float32(val[1])
     return out

@@ -881,3 +881,263 @@ extern "C" int pivamd_infill_rbf(const float *u, const float *d_excl, int N, int
                            out, d_err);
     return hipGetLastError() == hipSuccess ? 0 : PIVAMD_ERR_LAUNCH;
 }
+
+// ---------------------------------------------------------------------------
+// Linear griddata infill (infilling.py:440-503 infill_griddata_linear: scipy
+// griddata 'linear' over the valid vectors in physical coordinates, 'nearest'
+// outside their convex hull, only the holes replaced).  Each hole (a NaN
+// vector, not excluded) is filled from a Delaunay triangle of donors found
+// around it, one wave per hole, for k = 24, 48, 96, 192 in turn:
+//   - candidates: the k nearest donors (distance, then row-major index), in
+//     coordinates y = row * sy, x = col * sx relative to the hole (sy, sx:
+//     the pitches, divided by their common factor when both are integers, so
+//     the predicates below are exact), kept in row-major order;
+//   - the triangle: the first (a, b, c) in lexicographic order with nonzero
+//     area, the hole inside or on an edge, no candidate strictly inside the
+//     circumcircle and, unless every donor is a candidate, the circumcircle
+//     strictly within the k-th candidate's distance of the hole (so it is
+//     empty of every donor: a Delaunay triangle of the whole field);
+//   - value: barycentric weights (orientation ratios) in FP64, as float32;
+//   - no candidate triangle holds the hole: the nearest donor (outside the
+//     convex hull, scipy's nearest fallback);
+//   - triangles hold it but none is covered at k = 192: *err |= 2.
+// Where four donors are cocircular (common on a regular grid) scipy's Qhull
+// takes one of the equally Delaunay diagonals by its own processing order;
+// this takes the first in index order.  Both interpolate linear fields
+// exactly and agree wherever the Delaunay triangle is unique.
+// *err bits: 1 no donor in an image with holes; 2 as above; 4 the candidate
+// square outgrew its LDS buffer.
+// ---------------------------------------------------------------------------
+namespace pivamd {
+
+constexpr int kGdCand = 1024;    // donors a candidate square may hold
+constexpr int kGdMax = 192;      // largest k
+
+#pragma clang fp contract(off)
+
+__global__ void __launch_bounds__(64) griddata_fill_kernel(const float *u, const float *excl, int N, int ny, int nx,
+                                                           double sy, double sx, const int *n_donors,
+                                                           const int *hole_count, const int *holes, float *out,
+                                                           int *err)
+{
+    __shared__ double c_d[kGdCand];
+    __shared__ int c_q[kGdCand], c_r[kGdCand];
+    __shared__ double s_x[kGdMax], s_y[kGdMax];
+    __shared__ float s_f0[kGdMax], s_f1[kGdMax];
+    __shared__ int s_cnt;
+    __shared__ float s_near[2];
+    const int lane = (int)threadIdx.x;
+    const long long per = (long long)ny * nx, total = (long long)N * per;
+    const int nh = *hole_count;
+    const double step = sy < sx ? sy : sx;
+    // (distance, row-major index) order of candidates o and e
+    auto before = [&](int o, int e) { return c_d[o] < c_d[e] || (c_d[o] == c_d[e] && c_q[o] < c_q[e]); };
+    for (int h = blockIdx.x; h < nh; h += gridDim.x) {
+        const long long gi = holes[h];
+        const int n = (int)(gi / per), w = (int)(gi - (long long)n * per);
+        const float *f0 = u + (size_t)n * per, *f1 = u + (size_t)total + (size_t)n * per;
+        const int nd = n_donors[n];
+        if (nd < 1) {
+            if (lane == 0) atomicOr(err, 1);
+            continue;
+        }
+        const int r0 = w / nx, q0 = w % nx;
+        bool done = false;
+        float v0 = 0.0f, v1 = 0.0f;
+        for (int kk0 = 24; kk0 <= kGdMax && !done; kk0 *= 2) {
+            const int kk = kk0 < nd ? kk0 : nd;
+            // -- donors of a square of half-width R, grown until it surely
+            //    holds the kk nearest (nothing outside it is nearer than
+            //    (R + 1) * min pitch); their (distance, index) ranks
+            int R = 1;
+            while ((2 * R + 1) * (2 * R + 1) < 2 * kk) ++R;
+            int M = 0;
+            double dk = 0.0;
+            bool ok = false;
+            for (;;) {
+                const int ra = max(r0 - R, 0), rb = min(r0 + R, ny - 1);
+                const int ca = max(q0 - R, 0), cb = min(q0 + R, nx - 1);
+                const int wid = cb - ca + 1, cells = (rb - ra + 1) * wid;
+                if (lane == 0) s_cnt = 0;
+                __syncthreads();
+                for (int e = lane; e < cells; e += 64) {
+                    const int rr = ra + e / wid, cc = ca + e % wid;
+                    const int q = rr * nx + cc;
+                    const float a = f0[q], b = f1[q];
+                    if (!(excl && excl[q] != 0.0f) && isfinite(a) && isfinite(b)) {
+                        const int slot = atomicAdd(&s_cnt, 1);
+                        if (slot < kGdCand) {
+                            const double ey = (double)(rr - r0) * sy, ex = (double)(cc - q0) * sx;
+                            c_d[slot] = sqrt(ey * ey + ex * ex);
+                            c_q[slot] = q;
+                        }
+                    }
+                }
+                __syncthreads();
+                M = s_cnt;
+                if (M > kGdCand) break;
+                const bool whole = ra == 0 && ca == 0 && rb == ny - 1 && cb == nx - 1;
+                if (M >= kk) {
+                    double best = -1.0;
+                    for (int e = lane; e < M; e += 64) {
+                        int rank = 0;
+                        for (int o = 0; o < M; ++o) rank += before(o, e) ? 1 : 0;
+                        c_r[e] = rank;
+                        if (rank == kk - 1) best = c_d[e];
+                    }
+#pragma unroll
+                    for (int o = 32; o >= 1; o >>= 1) best = fmax(best, __shfl_xor(best, o, 64));
+                    dk = best;
+                    if (whole || dk < (double)(R + 1) * step) { ok = true; break; }
+                } else if (whole) {
+                    break;
+                }
+                R = R * 2;
+                __syncthreads();
+            }
+            if (!ok) {
+                if (lane == 0) atomicOr(err, 4);
+                break;
+            }
+            __syncthreads();
+            // -- the kk nearest in row-major order, and the nearest donor
+            for (int e = lane; e < M; e += 64) {
+                const int rank = c_r[e];
+                if (rank >= kk) continue;
+                int pos = 0;
+                for (int o = 0; o < M; ++o) pos += (c_r[o] < kk && c_q[o] < c_q[e]) ? 1 : 0;
+                const int q = c_q[e];
+                s_y[pos] = (double)(q / nx - r0) * sy;
+                s_x[pos] = (double)(q % nx - q0) * sx;
+                s_f0[pos] = f0[q];
+                s_f1[pos] = f1[q];
+                if (rank == 0) {
+                    s_near[0] = f0[q];
+                    s_near[1] = f1[q];
+                }
+            }
+            __syncthreads();
+            v0 = s_near[0];
+            v1 = s_near[1];
+            const int m = kk;
+            const bool need_cover = nd > kk;
+            // -- the first (a, b, c) in lexicographic order: a uniform, b
+            //    across the lanes, c in order; the wave keeps the smallest
+            //    (b, c) of the first a that has one
+            int found = -1, fa = -1;
+            bool lane_uncov = false;
+            for (int a = 0; a < m - 2 && fa < 0; ++a) {
+                found = -1;
+                for (int b = a + 1 + lane; b < m - 1 && found < 0; b += 64) {
+                    for (int c = b + 1; c < m; ++c) {
+                        const double ar = (s_x[b] - s_x[a]) * (s_y[c] - s_y[a]) - (s_y[b] - s_y[a]) * (s_x[c] - s_x[a]);
+                        if (ar == 0.0) continue;
+                        const double oa = s_x[b] * s_y[c] - s_y[b] * s_x[c];
+                        const double ob = s_x[c] * s_y[a] - s_y[c] * s_x[a];
+                        const double oc = s_x[a] * s_y[b] - s_y[a] * s_x[b];
+                        const bool hold = ar > 0.0 ? (oa >= 0.0 && ob >= 0.0 && oc >= 0.0)
+                                                   : (oa <= 0.0 && ob <= 0.0 && oc <= 0.0);
+                        if (!hold) continue;
+                        bool empty = true;
+                        for (int p = 0; p < m && empty; ++p) {
+                            const double adx = s_x[a] - s_x[p], ady = s_y[a] - s_y[p];
+                            const double bdx = s_x[b] - s_x[p], bdy = s_y[b] - s_y[p];
+                            const double cdx = s_x[c] - s_x[p], cdy = s_y[c] - s_y[p];
+                            double det = (adx * adx + ady * ady) * (bdx * cdy - cdx * bdy) +
+                                         (bdx * bdx + bdy * bdy) * (cdx * ady - adx * cdy) +
+                                         (cdx * cdx + cdy * cdy) * (adx * bdy - bdx * ady);
+                            if (ar < 0.0) det = -det;
+                            empty = !(det > 0.0);
+                        }
+                        if (!empty) continue;
+                        if (need_cover) {
+                            const double bx = s_x[b] - s_x[a], by = s_y[b] - s_y[a];
+                            const double cx = s_x[c] - s_x[a], cy = s_y[c] - s_y[a];
+                            const double dd = 2.0 * (bx * cy - by * cx);
+                            const double b2 = bx * bx + by * by, c2 = cx * cx + cy * cy;
+                            const double ox = (cy * b2 - by * c2) / dd;
+                            const double oy = (bx * c2 - cx * b2) / dd;
+                            const double rho = sqrt(ox * ox + oy * oy);
+                            const double gx = ox + s_x[a], gy = oy + s_y[a];
+                            if (!(sqrt(gx * gx + gy * gy) + rho < dk)) {
+                                lane_uncov = true;
+                                continue;
+                            }
+                        }
+                        found = b * m + c;
+                        break;
+                    }
+                }
+                int best = found < 0 ? 0x7fffffff : found;
+#pragma unroll
+                for (int o = 32; o >= 1; o >>= 1) best = min(best, __shfl_xor(best, o, 64));
+                if (best != 0x7fffffff) {
+                    fa = a;
+                    found = best;
+                }
+            }
+            const bool uncovered = __any(lane_uncov);
+            if (fa >= 0) {
+                const int a = fa, b = found / m, c = found % m;
+                const double ar = (s_x[b] - s_x[a]) * (s_y[c] - s_y[a]) - (s_y[b] - s_y[a]) * (s_x[c] - s_x[a]);
+                const double oa = s_x[b] * s_y[c] - s_y[b] * s_x[c];
+                const double ob = s_x[c] * s_y[a] - s_y[c] * s_x[a];
+                const double oc = s_x[a] * s_y[b] - s_y[a] * s_x[b];
+                const double la = oa / ar, lb = ob / ar, lc = oc / ar;
+                v0 = (float)(la * (double)s_f0[a] + lb * (double)s_f0[b] + lc * (double)s_f0[c]);
+                v1 = (float)(la * (double)s_f1[a] + lb * (double)s_f1[b] + lc * (double)s_f1[c]);
+                done = true;
+            } else if (!uncovered) {
+                done = true;            // outside the hull: the nearest donor
+            }
+            __syncthreads();
+        }
+        if (lane == 0) {
+            if (done) {
+                out[(size_t)n * per + w] = v0;
+                out[(size_t)total + (size_t)n * per + w] = v1;
+            } else {
+                atomicOr(err, 2);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+#pragma clang fp contract(on)
+
+}  // namespace pivamd
+
+extern "C" unsigned long pivamd_infill_griddata_workspace(int N, int ny, int nx)
+{
+    return pivamd_infill_rbf_workspace(N, ny, nx);
+}
+
+extern "C" int pivamd_infill_griddata(const float *u, const float *d_excl, int N, int ny, int nx, float py, float px,
+                                      float *out, void *d_work, unsigned long work_bytes, int *d_err, hipStream_t st)
+{
+    using namespace pivamd;
+    if (N < 0 || ny <= 0 || nx <= 0 || !u || !out || !d_err || !d_work) return PIVAMD_ERR_ARG;
+    if (!(py > 0.0f) || !(px > 0.0f)) return PIVAMD_ERR_ARG;
+    if (work_bytes < pivamd_infill_griddata_workspace(N, ny, nx)) return PIVAMD_ERR_ARG;
+    if (N == 0) return 0;
+    int *nd = (int *)d_work, *cnt = nd + N, *holes = cnt + 1;
+    const long long total = (long long)N * ny * nx;
+    if (total > 0x7fffffffLL) return PIVAMD_ERR_ARG;
+    // exact predicates for integer pitches: divide both by their common factor
+    double sy = py, sx = px;
+    if (py == (float)(long long)py && px == (float)(long long)px) {
+        long long a = (long long)py, b = (long long)px;
+        while (b) { const long long t = a % b; a = b; b = t; }
+        sy = (double)py / (double)a;
+        sx = (double)px / (double)a;
+    }
+    hipLaunchKernelGGL(rbf_zero_kernel, dim3(1), dim3(1), 0, st, cnt);
+    hipLaunchKernelGGL(knn_count_kernel, dim3(N), dim3(1024), 0, st, u, d_excl, N, ny * nx, nd);
+    hipLaunchKernelGGL(rbf_prep_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, u, d_excl, N,
+                       ny * nx, out, cnt, holes);
+    const unsigned grid = (unsigned)(total < 4096 ? total : 4096);
+    hipLaunchKernelGGL(griddata_fill_kernel, dim3(grid), dim3(64), 0, st, u, d_excl, N, ny, nx, sy, sx, nd, cnt,
+                       holes, out, d_err);
+    return hipGetLastError() == hipSuccess ? 0 : PIVAMD_ERR_LAUNCH;
+}

@@ -89,6 +89,11 @@ def core() -> ctypes.CDLL:
         L.pivamd_infill_knn.argtypes = [_vp, _vp, _i, _i, _i, ctypes.c_float, ctypes.c_float, _i, _i, _vp, _vp,
                                         ctypes.c_ulong, _vp, _vp]
         L.pivamd_infill_knn.restype = _i
+        L.pivamd_infill_griddata_workspace.argtypes = [_i, _i, _i]
+        L.pivamd_infill_griddata_workspace.restype = ctypes.c_ulong
+        L.pivamd_infill_griddata.argtypes = [_vp, _vp, _i, _i, _i, ctypes.c_float, ctypes.c_float, _vp, _vp,
+                                             ctypes.c_ulong, _vp, _vp]
+        L.pivamd_infill_griddata.restype = _i
         L.pivamd_infill_rbf_workspace.argtypes = [_i, _i, _i]
         L.pivamd_infill_rbf_workspace.restype = ctypes.c_ulong
         L.pivamd_infill_rbf.argtypes = [_vp, _vp, _i, _i, _i, ctypes.c_float, ctypes.c_float, _i, _i,

@@ -107,19 +107,19 @@ class _PassBuf:
 
 def _infill_cfg(cfg, final):
     """(method, parameters) of an infill option: a name, or the reference's
-    config form {"method": ..., "parameters": {...}} (infilling.py:698-790).
-    griddata_linear is not built on the device."""
+    config form {"method": ..., "parameters": {...}} (infilling.py:698-790)."""
     if isinstance(cfg, dict):
         method, params = str(cfg.get("method", "nearest")).lower(), dict(cfg.get("parameters", {}) or {})
         if final and not cfg.get("enabled", True):
             method = "none"
     else:
         method, params = str(cfg).lower(), {}
-    allowed = ("nearest", "harmonic", "biharmonic", "knn", "rbf_local") + (("none",) if final else ())
+    allowed = ("nearest", "harmonic", "biharmonic", "knn", "rbf_local", "griddata_linear") + \
+        (("none",) if final else ())
     if method not in allowed:
-        if method == "griddata_linear":
-            raise NotImplementedError(f"infill '{method}' is not on the device path")
         raise ValueError(f"infill method {method!r}: one of {allowed}")
+    if method == "griddata_linear" and str(params.get("method", "linear")) != "linear":
+        raise ValueError(f"griddata_linear method {params.get('method')!r}: only 'linear' is the reference's")
     if method == "knn":
         k = int(params.get("n_neighbors", 32))
         if not 1 <= k <= 128:
@@ -384,6 +384,8 @@ class MultipassPIV:
                 add("knn_work", (max(4, self.lib.pivamd_infill_knn_workspace(n)),), u8)
             if (self.final_infill if final else self.mid_infill) == "rbf_local":
                 add("rbf_work", (max(4, self.lib.pivamd_infill_rbf_workspace(n, ny, nx)),), u8)
+            if (self.final_infill if final else self.mid_infill) == "griddata_linear":
+                add("gd_work", (max(4, self.lib.pivamd_infill_griddata_workspace(n, ny, nx)),), u8)
             if (self.final_infill if final else self.mid_infill) in ("harmonic", "biharmonic"):
                 add("ls_work", (max(4, self.lib.pivamd_infill_ls_workspace(n, ny, nx)),), u8)
             add("work", (max(4, self.lib.pivamd_xcorr_workspace(n, ny, nx, g.win[0], g.win[1], P, 0)),), u8)
@@ -642,6 +644,13 @@ class MultipassPIV:
                                               kp["epsilon"], kp["smoothing"], _ptr(o["u3"]), _ptr(wk),
                                               wk.numel(), _ptr(o["err"]), st), "infill")
             u = o["u3"]
+        elif method == "griddata_linear":
+            o["err"].zero_()
+            wk = o["gd_work"]
+            _native.check(L.pivamd_infill_griddata(_ptr(u), _ptr(b.mask), n, ny, nx, float(g.spacing[0]),
+                                                   float(g.spacing[1]), _ptr(o["u3"]), _ptr(wk), wk.numel(),
+                                                   _ptr(o["err"]), st), "infill")
+            u = o["u3"]
         elif method in ("harmonic", "biharmonic"):
             o["err"].zero_()
             wk = o["ls_work"]
@@ -735,6 +744,12 @@ class MultipassPIV:
                     raise np.linalg.LinAlgError(f"pass {p}: rbf_local: Singular matrix.")
                 raise ValueError(f"pass {p}: rbf_local: the neighbour search exceeded its buffer "
                                  "(grid pitch too anisotropic)")
+            if (self.final_infill if p == last else self.mid_infill) == "griddata_linear" and e:
+                if e & 1:
+                    raise ValueError(f"pass {p}: Infilling method 'griddata_linear' cannot run: no valid donor")
+                raise ValueError(f"pass {p}: griddata_linear: a hole's Delaunay triangle reaches beyond its 192 "
+                                 "nearest donors" if e & 2 else
+                                 f"pass {p}: griddata_linear: the neighbour search exceeded its buffer")
             if e & 1:
                 raise ValueError(f"pass {p}: infilling has no donor points: 100% of vectors invalid")
             if e & 4:

@@ -1191,6 +1191,73 @@ def test_loop_with_knn_infill_matches_oracle_loop():
         assert np.sqrt(np.mean(d ** 2)) < 0.005, (p, d.max())
 
 
+@pytest.mark.parametrize("spacing", [(16.0, 16.0), (16.0, 8.0), (8.0, 12.0), (10.5, 10.5)])
+def test_griddata_infill_matches_oracle(spacing):
+    """The device griddata_linear infill equals the oracle's bit for bit:
+    scattered holes, a 7 x 11 block (candidate sets grown to 96 and 192),
+    holes on every border and a fully masked corner (outside the convex hull:
+    the nearest donor), an excluded block, three images, and a non-integer
+    pitch; an image without donors flags the error word."""
+    L = _lib()
+    rng = np.random.default_rng(71)
+    N, ny, nx = 3, 19, 26
+    u = rng.normal(size=(2, N, ny, nx)).astype(np.float32)
+    holes = rng.uniform(size=(N, ny, nx)) < 0.2
+    holes[1, 5:12, 3:14] = True
+    holes[2, 0:4, 0:5] = True
+    holes[2, :, nx - 1] = True
+    u[:, holes] = np.nan
+    ex = np.zeros((ny, nx), np.float32)
+    ex[2:4, 20:24] = 1.0
+    du, dex = _t(u), _t(ex)
+    wk = torch.empty(L.pivamd_infill_griddata_workspace(N, ny, nx), dtype=torch.uint8, device=dev)
+    err = torch.zeros(1, dtype=torch.int32, device=dev)
+    out = torch.empty_like(du)
+    assert L.pivamd_infill_griddata(du.data_ptr(), dex.data_ptr(), N, ny, nx, spacing[0], spacing[1], out.data_ptr(),
+                                    wk.data_ptr(), wk.numel(), err.data_ptr(), _st()) == 0
+    ref = M.infill_griddata_linear(u, spacing, exclude=ex.astype(bool))
+    torch.cuda.synchronize()
+    assert int(err.item()) == 0
+    np.testing.assert_array_equal(out.cpu().numpy(), ref)
+    v = np.full((2, 1, ny, nx), np.nan, np.float32)
+    dv = _t(v)
+    out = torch.empty_like(dv)
+    assert L.pivamd_infill_griddata(dv.data_ptr(), None, 1, ny, nx, 16.0, 16.0, out.data_ptr(), wk.data_ptr(),
+                                    wk.numel(), err.data_ptr(), _st()) == 0
+    torch.cuda.synchronize()
+    assert int(err.item()) & 1
+
+
+def test_loop_with_griddata_infill_matches_oracle_loop():
+    """The pass loop with griddata_linear infill on every pass: identical NaN
+    masks and reasons, RMS < 0.005 px against the oracle loop."""
+    shape = (192, 256)
+    sched = [(64, 64), (32, 32), (16, 16)]
+    A, B = [], []
+    for k in range(2):
+        rng = np.random.default_rng(990 + k)
+        x, y, inten = S._positions(rng, shape, 0.03)
+        dx, dy = S.lamb_oseen(x, y, shape, rc=40.0, peak=3.0, uniform=(1.5, 0.0))
+        A.append(S.render(shape, x, y, inten).numpy())
+        B.append(S.render(shape, x + dx, y + dy, inten).numpy())
+    A = np.stack(A).astype(np.float32)
+    B = np.stack(B).astype(np.float32)
+    mp = MultipassPIV(shape, sched, [50, 50, 50], warp_arithmetic="exact", mid_infill="griddata_linear",
+                      final_infill={"method": "griddata_linear", "parameters": {"method": "linear"}})
+    res = mp.run(_t(A), _t(B), keep_passes=True)
+    torch.cuda.synchronize()
+    mp.check_infill()
+    grids = build_pass_grids(shape, sched, [50, 50, 50])
+    ref = M.run_multipass(A, B, grids, keep_passes=True, mid_infill="griddata_linear",
+                          final_infill="griddata_linear")
+    for p, (g, r) in enumerate(zip(res, ref)):
+        gm = g.nan_mask.cpu().numpy().astype(bool)
+        np.testing.assert_array_equal(gm, r["nan_mask"])
+        np.testing.assert_array_equal(g.nan_reason.cpu().numpy(), r["nan_reason"])
+        d = np.hypot(g.ux.cpu().numpy() - r["ux"], g.uy.cpu().numpy() - r["uy"])
+        assert np.sqrt(np.mean(d ** 2)) < 0.005, (p, d.max())
+
+
 @pytest.mark.parametrize("kernel,eps,smoothing", [("thin_plate_spline", None, 0.0), ("cubic", None, 0.01),
                                                   ("quintic", None, 0.0), ("linear", None, 0.0),
                                                   ("multiquadric", 0.05, 0.0), ("gaussian", 0.05, 0.0),
