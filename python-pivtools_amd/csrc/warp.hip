@@ -13,8 +13,9 @@
 // contraction off, so every output pixel is bit-identical to it.  A second
 // bicubic form (mode 2) evaluates the same interpolants separably with fused
 // multiply-adds: rows of four taps are summed first and the 4x4 products
-// wy*wx are never formed, for about half the instructions; its results
-// differ from the reference's in the last bits only.
+// wy*wx are never formed, and B is sampled at the mirror image of A's rounded
+// position with A's weights reversed, for about half the instructions; its
+// results differ from the reference's in the last bits only.
 //
 // Mapping: a 256-lane workgroup covers a 64-column x 16-row tile of one
 // image (grid.z = image).  Each lane owns one column and four rows.  The
@@ -28,8 +29,8 @@
 // source window in LDS measured slower for them.  The FMA form (mode 2) by
 // default stages it: a 512-lane workgroup covers a 64 x 32 tile, converts
 // the source rectangles of A and B to float in LDS once, and each tap is
-// then an LDS read, which took the form from about 310 to about 215 VALU
-// instructions per pixel pair.
+// then an LDS read, which took the form from about 310 to about 190 VALU
+// instructions per pixel pair (with B's stencil mirrored from A's).
 #include <hip/hip_runtime.h>
 
 #include "pivamd_dev.h"
@@ -55,16 +56,17 @@ __device__ __forceinline__ void keys4(float d, float w[4])
 __device__ float pred_index(float coord, const float *ctrs, int nC)
 {
     if (nC == 1) return 0.0f;
-    if (coord <= ctrs[0]) return 0.0f;
-    if (coord >= ctrs[nC - 1]) return (float)(nC - 1);
-    // smallest seg in [0, nC-2] with ctrs[seg+1] >= coord
-    int lo = 0, hi = nC - 2;
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (ctrs[mid + 1] < coord) lo = mid + 1;
-        else hi = mid;
-    }
-    const int seg = lo;
+    const float c0 = ctrs[0], cl = ctrs[nC - 1];
+    if (coord <= c0) return 0.0f;
+    if (coord >= cl) return (float)(nC - 1);
+    // smallest seg in [0, nC-2] with ctrs[seg+1] >= coord: start from the
+    // segment a uniform grid would put it in (exact for the window grids, so
+    // two dependent loads instead of a binary search's log2 nC), then walk
+    // up and down to it; the walks end for any centres
+    const float guess = (coord - c0) / (cl - c0) * (float)(nC - 1);
+    int seg = (int)fminf(fmaxf(guess, 0.0f), (float)(nC - 2));   // NaN -> 0
+    while (seg < nC - 2 && ctrs[seg + 1] < coord) ++seg;
+    while (seg > 0 && ctrs[seg] >= coord) --seg;
     const float den = ctrs[seg + 1] - ctrs[seg];
     if (den < 1e-12f) return (float)seg;
     const float t = (coord - ctrs[seg]) / den;
@@ -149,15 +151,42 @@ __device__ __forceinline__ FrameBuf frame_buf(const void *frame, size_t bytes)
     return f;
 }
 
-template <typename TI>
-__device__ __forceinline__ float sample_bicubic_fast(const TI *img, float fy, float fx, int H, int W, int hm3,
-                                                     int wm3, const FrameBuf &fb)
-{
-    const float fyf = floorf(fy), fxf = floorf(fx);
-    const int iy = (int)fyf - 1, ix = (int)fxf - 1;
+// The FMA form's stencils of one output pixel (i, j) with half-shifts
+// (hy, hx): A's 4 x 4 taps start at (floor(i - hy) - 1, floor(j - hx) - 1)
+// with Keys weights wy, wx of the fractional parts.  B is sampled at the
+// mirror image of A's position about (i, j), 2 (i, j) - (i - hy, j - hx):
+// its taps start at (2i - floor(i - hy) - 2, 2j - floor(j - hx) - 2) and its
+// weights are A's in reverse order (the Keys kernel is even, so the weights
+// of the fraction 1 - d are those of d reversed, and at d = 0 both give the
+// tap at 2i - floor(i - hy)).  The mirror point differs from the rounded
+// i + hy by A's own position rounding, so B is as accurate as A, and its
+// weights cost nothing.
+struct FmaStencil {
+    int iyA, ixA, iyB, ixB;
     float wy[4], wx[4];
-    keys4_fma(fy - fyf, wy);
-    keys4_fma(fx - fxf, wx);
+};
+
+__device__ __forceinline__ FmaStencil fma_stencil(int i, int j, float hy, float hx)
+{
+    FmaStencil s;
+    const float fy = (float)i - hy, fx = (float)j - hx;
+    const float fyf = floorf(fy), fxf = floorf(fx);
+    keys4_fma(fy - fyf, s.wy);
+    keys4_fma(fx - fxf, s.wx);
+    s.iyA = (int)fyf - 1;
+    s.ixA = (int)fxf - 1;
+    s.iyB = 2 * i - (int)fyf - 2;
+    s.ixB = 2 * j - (int)fxf - 2;
+    return s;
+}
+
+// One FMA-form tap sum from global memory: rows iy..iy+3 with weights wy,
+// columns ix..ix+3 with weights wx, zero outside the image.
+template <typename TI>
+__device__ __forceinline__ float sample_bicubic_fast(const TI *img, int iy, int ix, const float (&wy)[4],
+                                                     const float (&wx)[4], int H, int W, int hm3, int wm3,
+                                                     const FrameBuf &fb)
+{
     float val = 0.0f;
     if (inside(iy, hm3) && inside(ix, wm3)) {
         if constexpr (sizeof(TI) == 2) {
@@ -630,13 +659,9 @@ __global__ void __launch_bounds__(STAGE ? 512 : 256, STAGE ? 8 : 1) warp_kernel(
         // one tap sum from the staged rectangle R (origin r0, c0; h rows),
         // the same terms in the same order as sample_bicubic_fast
         auto sample_staged = [&](const float *R, int r0, int c0, int h, const TI *src, const FrameBuf &fb,
-                                 float fy, float fxp) -> float {
-            const float fyf = floorf(fy), fxpf = floorf(fxp);
-            const int ry = (int)fyf - 1 - r0, rx = (int)fxpf - 1 - c0;
+                                 int iy, int ix, const float (&wys)[4], const float (&wxs)[4]) -> float {
+            const int ry = iy - r0, rx = ix - c0;
             if (ok && (unsigned)ry <= (unsigned)(h - 4) && (unsigned)rx <= (unsigned)(kRegP - 4)) {
-                float wys[4], wxs[4];
-                keys4_fma(fy - fyf, wys);
-                keys4_fma(fxp - fxpf, wxs);
                 // ry < kRegH and rx < kRegP here: a 24-bit multiply-add
                 const float *p = R + (__umul24((unsigned)ry, (unsigned)kRegP) + (unsigned)rx);
                 float val = 0.0f;
@@ -650,7 +675,7 @@ __global__ void __launch_bounds__(STAGE ? 512 : 256, STAGE ? 8 : 1) warp_kernel(
                 }
                 return val;
             }
-            return sample_bicubic_fast(src, fy, fxp, g.H, g.W, hm3, wm3, fb);
+            return sample_bicubic_fast(src, iy, ix, wys, wxs, g.H, g.W, hm3, wm3, fb);
         };
         // output planes as raw buffers based at this image: one 32-bit lane
         // offset, the row step a uniform offset
@@ -665,9 +690,11 @@ __global__ void __launch_bounds__(STAGE ? 512 : 256, STAGE ? 8 : 1) warp_kernel(
             const int i = y0 + ty + r * LY;
             if (i >= g.H) break;
             const int so = r * LY * g.W * (int)sizeof(float);
-            const float va = sample_staged(s_reg, r0a, c0a, ha, A, fbA, (float)i - hys[r], (float)j - hxs[r]);
-            const float vb = sample_staged(s_reg + kRegH * kRegP, r0b, c0b, hb, B, fbB, (float)i + hys[r],
-                                           (float)j + hxs[r]);
+            const FmaStencil st = fma_stencil(i, j, hys[r], hxs[r]);
+            const float wyr[4] = {st.wy[3], st.wy[2], st.wy[1], st.wy[0]};
+            const float wxr[4] = {st.wx[3], st.wx[2], st.wx[1], st.wx[0]};
+            const float va = sample_staged(s_reg, r0a, c0a, ha, A, fbA, st.iyA, st.ixA, st.wy, st.wx);
+            const float vb = sample_staged(s_reg + kRegH * kRegP, r0b, c0b, hb, B, fbB, st.iyB, st.ixB, wyr, wxr);
             __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, va), oa, vo, so, 0);
             __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, vb), ob, vo, so, 0);
         }
@@ -683,8 +710,11 @@ __global__ void __launch_bounds__(STAGE ? 512 : 256, STAGE ? 8 : 1) warp_kernel(
         half_shifts(ly, hy, hx);
         const size_t o = (size_t)img * plane + (size_t)i * g.W + j;
         if constexpr (FAST) {
-            g.outA[o] = sample_bicubic_fast(A, (float)i - hy, (float)j - hx, g.H, g.W, hm3, wm3, fbA);
-            g.outB[o] = sample_bicubic_fast(B, (float)i + hy, (float)j + hx, g.H, g.W, hm3, wm3, fbB);
+            const FmaStencil st = fma_stencil(i, j, hy, hx);
+            const float wyr[4] = {st.wy[3], st.wy[2], st.wy[1], st.wy[0]};
+            const float wxr[4] = {st.wx[3], st.wx[2], st.wx[1], st.wx[0]};
+            g.outA[o] = sample_bicubic_fast(A, st.iyA, st.ixA, st.wy, st.wx, g.H, g.W, hm3, wm3, fbA);
+            g.outB[o] = sample_bicubic_fast(B, st.iyB, st.ixB, wyr, wxr, g.H, g.W, hm3, wm3, fbB);
         } else if (g.mode == 0) {
             g.outA[o] = sample_bicubic(A, (float)i - hy, (float)j - hx, g.H, g.W, hm3, wm3);
             g.outB[o] = sample_bicubic(B, (float)i + hy, (float)j + hx, g.H, g.W, hm3, wm3);
