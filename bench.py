@@ -875,6 +875,7 @@ def main():
     if args.autotune and args.dft == "fft":
         from pivtools_amd import autotune
         tuned = autotune.tune_xcorr([w[0] for w in SCHEDULE], reps=10, log=log)
+        tuned["warp"] = autotune.tune_warp(SHAPE, n_pairs=16, reps=10, log=log)
     log("[bench] warmup")
     for _ in range(args.warmup):
         mp.run(A, B)

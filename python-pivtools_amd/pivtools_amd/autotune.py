@@ -238,3 +238,72 @@ def tune_xcorr(sizes, device=None, reps: int = 10, cache: str | None = None, inc
             json.dump(stored, f, indent=1)
         os.replace(tmp, cache)
     return table
+
+
+def get_warp_variant() -> int:
+    return int(_native.core().pivamd_warp_get_variant())
+
+
+def set_warp_variant(v: int) -> None:
+    """Select the FMA warp's form: 1 stages each 64 x 32 tile's source
+    rectangles in LDS (512-lane workgroups), 0 reads every tap from global
+    memory (256-lane workgroups on 64 x 16 tiles)."""
+    _native.check(_native.core().pivamd_warp_set_variant(int(v)), "warp variant")
+
+
+def tune_warp(shape=(2160, 2560), n_pairs: int = 16, device=None, reps: int = 10, log=None,
+              margin: float = 0.03) -> dict:
+    """Time the FMA warp's two tile shapes (64 x 16 tiles reading global
+    memory, 64 x 32 tiles staged in LDS) on a synthetic 16-bit batch with a
+    smooth per-image predictor on a 64-pixel grid, check that they give the
+    same frames, select the faster (the default unless the other is faster
+    by more than ``margin``) and return {"variant", "default", "ms"}."""
+    import torch
+
+    dev = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+    L = _native.core()
+    H, W = shape
+    g = torch.Generator(device="cpu")
+    g.manual_seed(11)
+    frames = torch.randint(0, 4096, (2, n_pairs, H, W), generator=g, dtype=torch.int16).to(dev)
+    cy = torch.arange(-64.0, H + 64.0, 64.0, device=dev)
+    cx = torch.arange(-64.0, W + 64.0, 64.0, device=dev)
+    yy, xx = torch.meshgrid(cy / H, cx / W, indexing="ij")
+    k = torch.arange(n_pairs, device=dev, dtype=torch.float32)[:, None, None]
+    pdy = (3.0 * torch.sin(6.0 * yy + k) + 1.0).contiguous()
+    pdx = (4.0 * torch.cos(5.0 * xx - k) - 2.0).contiguous()
+    outs = {v: (torch.empty((n_pairs, H, W), device=dev), torch.empty((n_pairs, H, W), device=dev)) for v in (0, 1)}
+    st = torch.cuda.current_stream(dev).cuda_stream
+    prev = get_warp_variant()
+    ms = {}
+    try:
+        for v in (0, 1):
+            set_warp_variant(v)
+            oa, ob = outs[v]
+
+            def run():
+                _native.check(L.pivamd_warp_u16(frames[0].data_ptr(), frames[1].data_ptr(), oa.data_ptr(),
+                                                ob.data_ptr(), pdy.data_ptr(), pdx.data_ptr(), n_pairs, H, W,
+                                                len(cy), len(cx), cy.data_ptr(), cx.data_ptr(), 2, 0, 0, st), "warp")
+            run()
+            ts = []
+            for _ in range(reps):
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+                run()
+                e1.record()
+                ts.append((e0, e1))
+            torch.cuda.synchronize(dev)
+            ms[v] = float(np.median([a.elapsed_time(b) for a, b in ts]))
+    finally:
+        set_warp_variant(prev)
+    same = all(torch.equal(a, b) for a, b in zip(outs[0], outs[1]))
+    dflt = 1
+    best = min(ms, key=ms.get) if same else dflt
+    if best != dflt and ms[best] > (1.0 - margin) * ms[dflt]:
+        best = dflt
+    set_warp_variant(best)
+    rec = {"variant": best, "default": dflt, "ms": {str(v): round(t, 4) for v, t in ms.items()}, "equal": same}
+    if log:
+        log(f"[autotune] warp {H}x{W} x {n_pairs}: {rec['ms']} ms -> variant {best}")
+    return rec

@@ -71,3 +71,14 @@ def test_tuner_selects_and_caches(tmp_path):
         for n in sizes:
             AT.set_variant(n, 0)
     torch.cuda.synchronize()
+
+
+def test_tune_warp_picks_a_tile_shape_with_equal_frames():
+    """The warp tuner times both FMA tile shapes, finds their frames equal,
+    and leaves one of them selected."""
+    from pivtools_amd import autotune
+    rec = autotune.tune_warp((300, 520), n_pairs=2, reps=2)
+    assert rec["equal"] is True
+    assert rec["variant"] in (0, 1) and set(rec["ms"]) == {"0", "1"}
+    assert autotune.get_warp_variant() == rec["variant"]
+    autotune.set_warp_variant(1)
