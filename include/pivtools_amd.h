@@ -494,7 +494,9 @@ int pivamd_kspace_debug_prep(const double *d_aa, const double *d_bb, const doubl
  * the background planes (NULL: none), optional plane-mean removal and
  * envelope divide (NULL: none), AB x ab_scale, division by
  * sqrt(max(AA(0) BB(0), 1e-12)), FP64 out), and displacement / stress
- * extraction from the k-space rows. */
+ * extraction from the k-space rows (peak height x peak_scale in FP64 before
+ * the clip to [0, 1]: sqrt(sum area / A area) for the single run type,
+ * single_pass_accumulator.py:1191-1210, else 1). */
 int pivamd_sum_images(const float *d_imgs, int n, unsigned long hw, double *d_sum, void *stream);
 int pivamd_mean_image(const double *d_sum, double n, unsigned long hw, float *d_out, void *stream);
 int pivamd_ensemble_finalize(const float *d_sAA, const float *d_sBB, const float *d_sAB, const float *d_bAA,
@@ -503,8 +505,13 @@ int pivamd_ensemble_finalize(const float *d_sAA, const float *d_sBB, const float
                              float ab_scale, int dc_zero, double *d_oAA, double *d_oBB, double *d_oAB,
                              void *stream);
 int pivamd_ensemble_extract(const double *d_gauss, const int *d_status, const unsigned char *d_mask,
-                            int n_windows, int corr_h, int corr_w, float *d_ux, float *d_uy, float *d_uu,
-                            float *d_vv, float *d_uv, float *d_peak, int *d_reason, void *stream);
+                            int n_windows, int corr_h, int corr_w, double peak_scale, float *d_ux, float *d_uy,
+                            float *d_uu, float *d_vv, float *d_uv, float *d_peak, int *d_reason, void *stream);
+/* Single run type image padding (window_utils.py:447-520, apply_single_mode_padding
+ * with pad_value 0): n frames of H x W (16-bit counts when src_u16, else
+ * float32) into zero-bordered float32 planes of Hp x Wp at (pad_top, pad_left). */
+int pivamd_pad_frames(const void *d_src, int src_u16, int n, int H, int W, int pad_top, int pad_left, int Hp,
+                      int Wp, float *d_dst, void *stream);
 
 /* Peak search + fit + detectability on n planes of ny x nx in device
  * memory (window-major); outputs peak_loc[3*n] (row, col, height) and

@@ -735,12 +735,30 @@ __global__ void __launch_bounds__(256) ens_finalize_kernel(FinArgs a)
     }
 }
 
+// Zero-bordered float32 copy of n frames (16-bit counts or float32) into
+// (Hp, Wp) planes at (top, left): the single run type's image padding
+// (window_utils.py:447-520, constant 0), fused with the count conversion.
+template <typename T>
+__global__ void pad_frames_kernel(const T *src, size_t n, int H, int W, int top, int left, int Hp, int Wp,
+                                  float *dst)
+{
+    const size_t plane = (size_t)Hp * Wp, total = n * plane;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+        const size_t f = i / plane;
+        const int rem = (int)(i - f * plane);
+        const int y = rem / Wp - top, x = rem % Wp - left;
+        float v = 0.0f;
+        if (y >= 0 && y < H && x >= 0 && x < W) v = (float)src[(f * H + y) * W + x];
+        dst[i] = v;
+    }
+}
+
 // Displacements and stresses from the k-space fit rows
 // (single_pass_accumulator.py:1067-1290 without the optional outlier
 // methods): status != 0 or the 3/4-window rule gives NaN; masked windows 0.
 __global__ void ens_extract_kernel(const double *gauss, const int *status, const unsigned char *mask, int nWin,
-                                   float cx1, float cy1, float maxdx, float maxdy, float *ux, float *uy,
-                                   float *uu, float *vv, float *uv, float *peak, int *reason)
+                                   float cx1, float cy1, float maxdx, float maxdy, double peakScale, float *ux,
+                                   float *uy, float *uu, float *vv, float *uv, float *peak, int *reason)
 {
     const int w = blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= nWin) return;
@@ -760,6 +778,9 @@ __global__ void ens_extract_kernel(const double *gauss, const int *status, const
     float gm = aA * aB;
     gm = sqrtf(gm > 1e-12f ? gm : 1e-12f);
     float ph = aAB / gm;
+    // single run type: sqrt(sum area / A area) applied in FP64 to the float32
+    // ratio, as single_pass_accumulator.py:1199-1205 (exact for a scale of 1)
+    ph = (float)((double)ph * peakScale);
     ph = ph < 0.0f ? 0.0f : (ph > 1.0f ? 1.0f : ph);
     const float sAx = safe(g[6], 1e6f), sAy = safe(g[7], 1e6f), sAxy = safe(g[8], 1e6f);
     float U = safe(g[9], 1e6f) - sAx, V = safe(g[10], 1e6f) - sAy, C = safe(g[11], 1e6f) - sAxy;
@@ -816,14 +837,34 @@ extern "C" int pivamd_ensemble_finalize(const float *d_sAA, const float *d_sBB, 
 }
 
 extern "C" int pivamd_ensemble_extract(const double *d_gauss, const int *d_status, const unsigned char *d_mask,
-                                       int n_windows, int corr_h, int corr_w, float *d_ux, float *d_uy,
-                                       float *d_uu, float *d_vv, float *d_uv, float *d_peak, int *d_reason,
-                                       hipStream_t st)
+                                       int n_windows, int corr_h, int corr_w, double peak_scale, float *d_ux,
+                                       float *d_uy, float *d_uu, float *d_vv, float *d_uv, float *d_peak,
+                                       int *d_reason, hipStream_t st)
 {
     if (n_windows <= 0) return PIVAMD_OK;
+    if (!(peak_scale > 0.0)) return PIVAMD_ERR_ARG;
     const float cx1 = corr_w / 2.0f + 1.0f, cy1 = corr_h / 2.0f + 1.0f;
     hipLaunchKernelGGL(ens_extract_kernel, dim3((n_windows + 255) / 256), dim3(256), 0, st, d_gauss, d_status,
-                       d_mask, n_windows, cx1, cy1, 0.75f * corr_w, 0.75f * corr_h, d_ux, d_uy, d_uu, d_vv, d_uv,
-                       d_peak, d_reason);
+                       d_mask, n_windows, cx1, cy1, 0.75f * corr_w, 0.75f * corr_h, peak_scale, d_ux, d_uy, d_uu,
+                       d_vv, d_uv, d_peak, d_reason);
+    return hipGetLastError() == hipSuccess ? PIVAMD_OK : PIVAMD_ERR_LAUNCH;
+}
+
+extern "C" int pivamd_pad_frames(const void *d_src, int src_u16, int n, int H, int W, int pad_top, int pad_left,
+                                 int Hp, int Wp, float *d_dst, hipStream_t st)
+{
+    if (n < 0 || H <= 0 || W <= 0 || pad_top < 0 || pad_left < 0 || Hp < H + pad_top || Wp < W + pad_left)
+        return PIVAMD_ERR_ARG;
+    if (n == 0) return PIVAMD_OK;
+    if (!d_src || !d_dst) return PIVAMD_ERR_ARG;
+    const size_t total = (size_t)n * Hp * Wp;
+    const size_t blocks = (total + 255) / 256;
+    const dim3 grid((unsigned)(blocks < 8192 ? blocks : 8192));
+    if (src_u16)
+        hipLaunchKernelGGL(pad_frames_kernel<unsigned short>, grid, dim3(256), 0, st,
+                           (const unsigned short *)d_src, (size_t)n, H, W, pad_top, pad_left, Hp, Wp, d_dst);
+    else
+        hipLaunchKernelGGL(pad_frames_kernel<float>, grid, dim3(256), 0, st, (const float *)d_src, (size_t)n, H,
+                           W, pad_top, pad_left, Hp, Wp, d_dst);
     return hipGetLastError() == hipSuccess ? PIVAMD_OK : PIVAMD_ERR_LAUNCH;
 }

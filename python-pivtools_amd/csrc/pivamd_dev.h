@@ -80,8 +80,10 @@ int pivamd_ensemble_finalize(const float *d_sAA, const float *d_sBB, const float
                              int n_windows, int corr_h, int corr_w, float n_images, float ab_scale, int dc_zero,
                              double *d_oAA, double *d_oBB, double *d_oAB, hipStream_t st);
 int pivamd_ensemble_extract(const double *d_gauss, const int *d_status, const unsigned char *d_mask, int n_windows,
-                            int corr_h, int corr_w, float *d_ux, float *d_uy, float *d_uu, float *d_vv, float *d_uv,
-                            float *d_peak, int *d_reason, hipStream_t st);
+                            int corr_h, int corr_w, double peak_scale, float *d_ux, float *d_uy, float *d_uu,
+                            float *d_vv, float *d_uv, float *d_peak, int *d_reason, hipStream_t st);
+int pivamd_pad_frames(const void *d_src, int src_u16, int n, int H, int W, int pad_top, int pad_left, int Hp, int Wp,
+                      float *d_dst, hipStream_t st);
 int pivamd_kspace_fft2_centered(const double *d_plane, int corr_h, int corr_w, double *d_re, double *d_im,
                                 void *d_work, size_t work_bytes, hipStream_t st);
 int pivamd_kspace_debug_prep(const double *d_aa, const double *d_bb, const double *d_ab,

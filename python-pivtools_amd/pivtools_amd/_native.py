@@ -141,14 +141,15 @@ def core() -> ctypes.CDLL:
         L.pivamd_mean_image.argtypes = [_vp, ctypes.c_double, ctypes.c_ulong, _vp, _vp]
         L.pivamd_ensemble_finalize.argtypes = [_vp] * 8 + [_i, _i, _i, ctypes.c_float, ctypes.c_float, _i,
                                                            _vp, _vp, _vp, _vp]
-        L.pivamd_ensemble_extract.argtypes = [_vp, _vp, _vp, _i, _i, _i] + [_vp] * 8
+        L.pivamd_ensemble_extract.argtypes = [_vp, _vp, _vp, _i, _i, _i, ctypes.c_double] + [_vp] * 8
+        L.pivamd_pad_frames.argtypes = [_vp, _i, _i, _i, _i, _i, _i, _i, _i, _vp, _vp]
         L.pivamd_ensemble_workspace.argtypes = [_i, _i, _i, _i]
         L.pivamd_ensemble_workspace.restype = ctypes.c_size_t
         L.pivamd_accumulate.argtypes = [_vp, _vp, _vp, _i, _i, _i, _vp, _vp, _i, _i, _vp, _vp, _vp, _vp, _i, _i,
                                         _i, _i, _i, _i, _i, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                         _vp, _vp, _vp, _vp, ctypes.c_size_t, _vp]
         for f in ("pivamd_sum_images", "pivamd_mean_image", "pivamd_ensemble_finalize", "pivamd_ensemble_extract",
-                  "pivamd_accumulate"):
+                  "pivamd_accumulate", "pivamd_pad_frames"):
             getattr(L, f).restype = _i
         L.pivamd_kspace_fit.restype = _i
         L.pivamd_kspace_fft2_centered.argtypes = [_vp, _i, _i, _vp, _vp, _vp, ctypes.c_size_t, _vp]

@@ -1,4 +1,4 @@
-"""Device-resident ensemble PIV pass (single pass, standard windows).
+"""Device-resident ensemble PIV pass (standard and 'single' run types).
 
 Follows the reference's streaming ensemble pass:
 
@@ -10,6 +10,18 @@ Follows the reference's streaming ensemble pass:
   plane-mean removal (``+dc_zero``) and envelope divide, normalisation by
   the auto peaks, the coloured noise-floor shape P (flat for ``flat``), the
   k-space LM fit, and displacement / Reynolds-stress extraction.
+
+The ``single`` run type (``cpu_ensemble.py:188-229,695-750,963-1010``,
+``single_pass_accumulator.py:388-506,700-795,855-880,1191-1210``) puts a
+small A window (``singlepix``: ones on the window block) inside a sum
+window (``bsingle``: all ones) on a grid spaced by the small window: the
+frames get a zero border so a sum window fits around every position (one
+device pass that also converts 16-bit counts), AB correlates wA with wB and
+both autos wB with wB at the sum size, cropped centrally to the fitting
+window; finalize divides AA/BB by the wB*wB envelope and AB by the wA*wB
+one (when the envelope divide is on), scales AB by sum area / A area, takes
+the coloured floor of the wB chain, and scales the peak height by
+sqrt(sum area / A area).
 
 Every step is a HIP kernel of libpivtools_amd.so on torch's current stream;
 only the floor shape P is host work (one small analytic model per pass).
@@ -34,9 +46,10 @@ import numpy as np
 import torch
 
 from . import _native, floor_psd
-from .window import build_pass_grids, compute_window_centers, window_weight
+from .window import build_pass_grids, window_centers_original, window_weight
 
 BG_METHODS = {"correlation", "window_mean", "correlation+window_mean", "correlation+dc_zero", "none"}
+RUN_TYPES = ("standard", "single")
 
 
 @dataclass
@@ -62,7 +75,9 @@ class EnsemblePIV:
     def __init__(self, image_shape, window_size=(32, 32), overlap=75, window_type="square",
                  bg_method="correlation", kspace_floor="coloured", kspace_shape="gaussian",
                  envelope_divide=False, per_pair_normalization=False, warp_kernel="cubic",
-                 mask=None, device=None):
+                 mask=None, device=None, run_type="standard", sum_window=(32, 32), fit_window=None):
+        if run_type not in RUN_TYPES:
+            raise ValueError(f"run type {run_type!r}: one of {RUN_TYPES}")
         if bg_method not in BG_METHODS:
             raise ValueError(f"background method {bg_method!r} not in {sorted(BG_METHODS)}")
         if kspace_floor not in ("coloured", "flat"):
@@ -70,14 +85,32 @@ class EnsemblePIV:
         if per_pair_normalization and bg_method != "window_mean":
             raise ValueError("per-pair normalisation requires the window_mean background method")
         L = _native.core()
-        if (int(window_size[0]) != int(window_size[1]) or
-                not L.pivamd_size_supported(int(window_size[0]), int(window_size[1]))):
-            raise ValueError(f"window {window_size} has no device ensemble kernel (square built sizes)")
+        self.single = run_type == "single"
+        self.run_type = run_type
+        self.win = (int(window_size[0]), int(window_size[1]))
+        if self.single:
+            self.comp = (int(sum_window[0]), int(sum_window[1]))
+            self.corr = self.comp if fit_window is None else (int(fit_window[0]), int(fit_window[1]))
+            if self.corr[0] > self.comp[0] or self.corr[1] > self.comp[1]:
+                raise ValueError(f"fitting window {self.corr} exceeds the sum window {self.comp}")
+            if not (L.pivamd_size_supported(self.corr[0], self.corr[0]) and
+                    L.pivamd_size_supported(self.corr[1], self.corr[1])):
+                raise ValueError(f"fitting window {self.corr}: each axis must be a built FFT size")
+        else:
+            if fit_window is not None:
+                raise ValueError("a fitting window applies to the single run type only")
+            self.comp = self.corr = self.win
+        if self.comp[0] != self.comp[1] or not L.pivamd_size_supported(self.comp[0], self.comp[1]):
+            raise ValueError(f"correlation size {self.comp} has no device ensemble kernel (square built sizes)")
         self.L = L
         self.dev = torch.device(device) if device is not None else torch.device("cuda")
         self.H, self.W = (int(image_shape[0]), int(image_shape[1]))
-        self.win = (int(window_size[0]), int(window_size[1]))
-        self.grid = compute_window_centers((self.H, self.W), self.win, overlap)
+        self.overlap = overlap
+        self.grid = window_centers_original((self.H, self.W), self.win, overlap, run_type, self.comp)
+        self.padding = self.grid.padding            # (top, bottom, left, right), zero in standard mode
+        pt, pb, pl, pr = self.padding
+        self.pad = (pt, pl)
+        self.Hp, self.Wp = self.H + pt + pb, self.W + pl + pr
         self.ny, self.nx = int(self.grid.n_win_y), int(self.grid.n_win_x)
         self.nwin = self.ny * self.nx
         self.bg_method = bg_method
@@ -91,22 +124,34 @@ class EnsemblePIV:
         self.envelope_divide = envelope_divide
         self.warp_kernel = warp_kernel
         d, f32 = self.dev, torch.float32
-        w = window_weight(self.win, window_type)
-        self.w_host = w
-        self.w = torch.as_tensor(w, device=d)
-        self.sum_w = float(np.float32(w.sum(dtype=np.float32)))
-        self.ctr_x = torch.as_tensor(self.grid.win_ctrs_x.astype(np.float32), device=d)
-        self.ctr_y = torch.as_tensor(self.grid.win_ctrs_y.astype(np.float32), device=d)
+        if self.single:
+            wa = window_weight(self.win, "singlepix", self.comp)
+            wb = window_weight(self.comp, "bsingle", self.comp)
+            area_sum, area_a = self.comp[0] * self.comp[1], self.win[0] * self.win[1]
+            self.ab_scale = float(np.float32(area_sum / area_a))
+            self.peak_scale = float(np.sqrt(area_sum / area_a))
+        else:
+            wa = wb = window_weight(self.win, window_type)
+            self.ab_scale = self.peak_scale = 1.0
+        self.wa_host, self.w_host = wa, wb          # w_host: the B / auto weight (floor chain)
+        self.wa = torch.as_tensor(wa, device=d)
+        self.w = self.wa if wb is wa else torch.as_tensor(wb, device=d)
+        self.sum_w = float(np.float32(wb.sum(dtype=np.float32)))
+        # the correlation takes padded coordinates (the padding is zero in standard mode)
+        self.ctr_x = torch.as_tensor((self.grid.win_ctrs_x + np.float32(pl)).astype(np.float32), device=d)
+        self.ctr_y = torch.as_tensor((self.grid.win_ctrs_y + np.float32(pt)).astype(np.float32), device=d)
         self.mask_host = (np.zeros(self.nwin, bool) if mask is None
                           else np.asarray(mask, bool).reshape(-1))
         self.mask_f = torch.as_tensor(self.mask_host.astype(np.float32), device=d)
         self.mask_u8 = torch.as_tensor(self.mask_host.astype(np.uint8), device=d)
-        P = self.win[0] * self.win[1]
+        P = self.corr[0] * self.corr[1]
         self.sums = [torch.zeros(self.nwin * P, dtype=f32, device=d) for _ in range(3)]   # AB, AA, BB
-        self.sum_a = torch.zeros(self.H * self.W, dtype=torch.float64, device=d)
-        self.sum_b = torch.zeros(self.H * self.W, dtype=torch.float64, device=d)
-        wb = int(L.pivamd_ensemble_workspace(self.ny, self.nx, self.win[0], self.win[1]))
-        self.work = torch.empty(max(wb, 8), dtype=torch.uint8, device=d)
+        # image sums at the padded size (the border stays zero, so the mean
+        # images come out already padded for the background correlation)
+        self.sum_a = torch.zeros(self.Hp * self.Wp, dtype=torch.float64, device=d)
+        self.sum_b = torch.zeros(self.Hp * self.Wp, dtype=torch.float64, device=d)
+        nbytes = int(L.pivamd_ensemble_workspace(self.ny, self.nx, self.comp[0], self.comp[1]))
+        self.work = torch.empty(max(nbytes, 8), dtype=torch.uint8, device=d)
         self._f32 = [None, None]
         self.n_images = 0
         self.predictor = None      # (2, ny, nx) [dy, dx] at the window centres (pass > 0)
@@ -126,26 +171,40 @@ class EnsemblePIV:
     def _stream(self):
         return torch.cuda.current_stream(self.dev).cuda_stream
 
+    def _buffer(self, slot: int, numel: int) -> torch.Tensor:
+        buf = self._f32[slot]
+        if buf is None or buf.numel() < numel:
+            buf = self._f32[slot] = torch.empty(numel, dtype=torch.float32, device=self.dev)
+        return buf[:numel]
+
     def _frames(self, x: torch.Tensor, slot: int) -> torch.Tensor:
-        if x.dtype == torch.float32:
-            return x.contiguous()
-        if x.dtype in (torch.int16, torch.uint16):
-            buf = self._f32[slot]
-            if buf is None or buf.numel() < x.numel():
-                buf = self._f32[slot] = torch.empty(x.numel(), dtype=torch.float32, device=self.dev)
-            out = buf[:x.numel()].view(x.shape)
-            _native.check(self.L.pivamd_u16_to_f32(x.data_ptr(), out.data_ptr(), x.numel(), self._stream()),
-                          "u16_to_f32")
+        """float32 (n, Hp, Wp) frames for the correlation: 16-bit counts are
+        converted, and in single mode the zero border is added in the same
+        device pass."""
+        if x.dtype not in (torch.float32, torch.int16, torch.uint16):
+            raise TypeError(f"unsupported image dtype {x.dtype}")
+        x = x.contiguous()
+        n = int(x.shape[0])
+        if self.single:
+            out = self._buffer(slot, n * self.Hp * self.Wp).view(n, self.Hp, self.Wp)
+            _native.check(self.L.pivamd_pad_frames(x.data_ptr(), int(x.dtype != torch.float32), n, self.H, self.W,
+                                                   self.pad[0], self.pad[1], self.Hp, self.Wp, out.data_ptr(),
+                                                   self._stream()), "pad_frames")
             return out
-        raise TypeError(f"unsupported image dtype {x.dtype}")
+        if x.dtype == torch.float32:
+            return x
+        out = self._buffer(slot, x.numel()).view(x.shape)
+        _native.check(self.L.pivamd_u16_to_f32(x.data_ptr(), out.data_ptr(), x.numel(), self._stream()),
+                      "u16_to_f32")
+        return out
 
     def _accumulate(self, a, b, n, outs, per_pair_norm):
         st = self._stream()
-        P = self.win
+        C, F = self.comp, self.corr
         _native.check(self.L.pivamd_accumulate(
-            a.data_ptr(), b.data_ptr(), self.mask_f.data_ptr(), self.H, self.W, n, self.ctr_x.data_ptr(),
-            self.ctr_y.data_ptr(), self.ny, self.nx, self.w.data_ptr(), self.w.data_ptr(), self.w.data_ptr(),
-            self.w.data_ptr(), P[0], P[1], P[0], P[1], int(self.mean_sub), int(per_pair_norm),
+            a.data_ptr(), b.data_ptr(), self.mask_f.data_ptr(), self.Hp, self.Wp, n, self.ctr_x.data_ptr(),
+            self.ctr_y.data_ptr(), self.ny, self.nx, self.wa.data_ptr(), self.w.data_ptr(), self.w.data_ptr(),
+            self.w.data_ptr(), C[0], C[1], F[0], F[1], int(self.mean_sub), int(per_pair_norm),
             1, self.sum_w, self.sum_w, self.sum_w, outs[0].data_ptr(), outs[1].data_ptr(), outs[2].data_ptr(),
             self.work.data_ptr(), self.work.numel(), st), "accumulate_triple")
 
@@ -157,7 +216,7 @@ class EnsemblePIV:
         a = self._frames(img_a, 0)
         b = self._frames(img_b, 1)
         st = self._stream()
-        hw = self.H * self.W
+        hw = self.Hp * self.Wp
         _native.check(self.L.pivamd_sum_images(a.data_ptr(), n, hw, self.sum_a.data_ptr(), st), "sum_images")
         _native.check(self.L.pivamd_sum_images(b.data_ptr(), n, hw, self.sum_b.data_ptr(), st), "sum_images")
         self._accumulate(a, b, n, self.sums, self.per_pair_norm)
@@ -169,7 +228,7 @@ class EnsemblePIV:
         every window sits at half-shift fraction 0 (no predictor, or an
         even-integer one), else one shape per window (n, h * w) from the
         fractions frac(pred / 2) of the predictor at the window centres."""
-        corr = self.win
+        corr = self.corr
         env_auto = floor_psd.pair_envelope(self.w_host, self.w_host, corr)
         env_for_floor = env_auto if self.envelope_divide else np.ones_like(env_auto)
         if self.predictor is None:
@@ -189,9 +248,9 @@ class EnsemblePIV:
         if self.n_images == 0:
             raise RuntimeError("no pairs accumulated")
         L, d, st = self.L, self.dev, self._stream()
-        h, w = self.win
+        h, w = self.corr
         P = h * w
-        hw = self.H * self.W
+        hw = self.Hp * self.Wp
         f32, f64 = torch.float32, torch.float64
         bg = None
         if self.bg_correlation:
@@ -203,17 +262,20 @@ class EnsemblePIV:
                           "mean_image")
             bg = [torch.zeros(self.nwin * P, dtype=f32, device=d) for _ in range(3)]
             self._accumulate(ma, mb, 1, bg, False)
-        env = None
+        env = env_ab = None
         if self.envelope_divide:
-            e = floor_psd.pair_envelope(self.w_host, self.w_host, self.win).astype(np.float32)
-            env = torch.as_tensor(e, device=d)
+            e = floor_psd.pair_envelope(self.w_host, self.w_host, self.corr).astype(np.float32)
+            env = env_ab = torch.as_tensor(e, device=d)
+            if self.single:
+                e = floor_psd.pair_envelope(self.wa_host, self.w_host, self.corr).astype(np.float32)
+                env_ab = torch.as_tensor(e, device=d)
         planes = [torch.empty(self.nwin * P, dtype=f64, device=d) for _ in range(3)]   # AA, BB, AB
         _native.check(L.pivamd_ensemble_finalize(
             self.sums[1].data_ptr(), self.sums[2].data_ptr(), self.sums[0].data_ptr(),
             None if bg is None else bg[1].data_ptr(), None if bg is None else bg[2].data_ptr(),
             None if bg is None else bg[0].data_ptr(), None if env is None else env.data_ptr(),
-            None if env is None else env.data_ptr(), self.nwin, h, w, float(np.float32(self.n_images)), 1.0,
-            int(self.dc_zero), planes[0].data_ptr(), planes[1].data_ptr(), planes[2].data_ptr(), st),
+            None if env_ab is None else env_ab.data_ptr(), self.nwin, h, w, float(np.float32(self.n_images)),
+            self.ab_scale, int(self.dc_zero), planes[0].data_ptr(), planes[1].data_ptr(), planes[2].data_ptr(), st),
             "ensemble_finalize")
         pw = None
         if self.floor == "coloured":
@@ -237,7 +299,8 @@ class EnsemblePIV:
         out = [torch.empty(n, dtype=f32, device=d) for _ in range(6)]
         reason = torch.empty(n, dtype=torch.int32, device=d)
         _native.check(L.pivamd_ensemble_extract(
-            gauss.data_ptr(), status.data_ptr(), self.mask_u8.data_ptr(), n, h, w, *[o.data_ptr() for o in out],
+            gauss.data_ptr(), status.data_ptr(), self.mask_u8.data_ptr(), n, h, w, self.peak_scale,
+            *[o.data_ptr() for o in out],
             reason.data_ptr(), st), "ensemble_extract")
         if self.predictor is not None:
             # the fit measured the residual of the warped pairs
@@ -252,8 +315,11 @@ class EnsemblePIV:
 class EnsembleMultipass:
     """Multi-pass ensemble PIV over pairs resident on the device.
 
-    ``window_sizes`` / ``overlaps`` give the standard-mode passes (the
-    'single' sum-window run type is not built).  Pass p > 0 warps every pair
+    ``window_sizes`` / ``overlaps`` give the passes and ``run_types`` (one
+    per pass, default all 'standard') their run type; 'single' passes share
+    ``sum_window`` / ``fit_window`` and keep their centres in original image
+    coordinates, so the predictor chain is the same for both.  Pass p > 0
+    warps every pair
     by the previous pass's field used as one shared predictor
     (``fused_symmetric_warp_batch(shared_predictor=1, round_shifts)``,
     ``cpu_ensemble.py:1445-1484``):
@@ -275,19 +341,25 @@ class EnsembleMultipass:
     def __init__(self, image_shape, window_sizes, overlaps, *, window_type="square", bg_method="correlation",
                  kspace_floor="coloured", kspace_shape="gaussian", envelope_divide=False,
                  predictor_smoothing=False, predictor_rounding=False, warp_interpolation="cubic",
-                 mid_infill="nearest", final_infill="biharmonic", device=None):
+                 mid_infill="nearest", final_infill="biharmonic", run_types=None, sum_window=(32, 32),
+                 fit_window=None, device=None):
         self.H, self.W = (int(image_shape[0]), int(image_shape[1]))
         self.window_sizes = [tuple(w) if np.ndim(w) else (w, w) for w in window_sizes]
         self.overlaps = list(overlaps)
         if len(self.overlaps) != len(self.window_sizes):
             raise ValueError("one overlap per pass")
+        self.run_types = ["standard"] * len(self.window_sizes) if run_types is None else list(run_types)
+        if len(self.run_types) != len(self.window_sizes):
+            raise ValueError("one run type per pass")
         if mid_infill not in ("nearest", "harmonic", "biharmonic"):
             raise ValueError(f"mid-pass infill {mid_infill!r}: nearest, harmonic or biharmonic")
         if final_infill not in ("nearest", "harmonic", "biharmonic", "none"):
             raise ValueError(f"final-pass infill {final_infill!r}")
         if warp_interpolation not in ("cubic", "lanczos"):
             raise ValueError(f"warp_interpolation {warp_interpolation!r}: cubic or lanczos")
-        self.grids = build_pass_grids((self.H, self.W), self.window_sizes, self.overlaps)
+        self.grids = build_pass_grids((self.H, self.W), self.window_sizes, self.overlaps, self.run_types,
+                                      sum_window)
+        self.sum_window, self.fit_window = sum_window, fit_window
         self.opts = dict(window_type=window_type, bg_method=bg_method, kspace_floor=kspace_floor,
                          kspace_shape=kspace_shape, envelope_divide=envelope_divide,
                          warp_kernel=warp_interpolation)
@@ -372,7 +444,10 @@ class EnsembleMultipass:
         results, field = [], None
         wa = wb = None
         for p, g in enumerate(self.grids):
-            ens = EnsemblePIV((self.H, self.W), g.win, self.overlaps[p], device=d, **self.opts)
+            single = self.run_types[p] == "single"
+            ens = EnsemblePIV((self.H, self.W), g.win, self.overlaps[p], device=d, run_type=self.run_types[p],
+                              sum_window=self.sum_window, fit_window=self.fit_window if single else None,
+                              **self.opts)
             pad = None
             if p > 0:
                 pad, win = self._predictor(p, field)

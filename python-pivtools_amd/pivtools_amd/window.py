@@ -5,6 +5,9 @@ arrays that cross the C ABI (window centres, weights, padded predictor grid)
 so the numbers handed to the kernels are the reference's numbers:
 
 * ``compute_window_centers``  pivtools_core/window_utils.py:142-273
+* ``compute_padding_for_single_mode`` / ``compute_window_centers_single_mode``
+                              pivtools_core/window_utils.py:79-139, 276-444
+                              (ensemble 'single' run type)
 * ``window_weight``           pivtools_cli/piv/piv_backend/base.py:52-135
 * ``vector_masks``            pivtools_core/image_handling/load_images.py:827-1000
                               (compute_vector_mask, standard mode)
@@ -31,6 +34,7 @@ class WindowCenters:
     n_win_y: int
     win_spacing_x: int
     win_spacing_y: int
+    padding: tuple = (0, 0, 0, 0)    # (top, bottom, left, right) of the single-mode zero border
 
 
 def compute_window_centers(image_shape, window_size, overlap, validate=True) -> WindowCenters:
@@ -55,11 +59,62 @@ def compute_window_centers(image_shape, window_size, overlap, validate=True) -> 
     return WindowCenters(cx, cy, nx, ny, sx, sy)
 
 
-def window_weight(wsize, window_type: str) -> np.ndarray:
-    """2D window weight (float32), the reference's _window_weight_fun for the
-    types an instantaneous or standard ensemble pass uses."""
+def compute_padding_for_single_mode(window_size, sum_window) -> tuple:
+    """Zero border (top, bottom, left, right) that lets a sum window sit
+    around every small-window position: ceil / floor halves of sum - window."""
+    (wh, ww), (sh, sw) = window_size, sum_window
+    if sh < wh or sw < ww:
+        raise ValueError(f"SumWindow {tuple(sum_window)} must be >= window_size {tuple(window_size)} for single mode")
+    dy, dx = sh - wh, sw - ww
+    return (dy - dy // 2, dy // 2, dx - dx // 2, dx // 2)
+
+
+def compute_window_centers_single_mode(image_shape, window_size, sum_window, overlap,
+                                       validate=True) -> WindowCenters:
+    """Single-mode grid in PADDED image coordinates: spacing from the small
+    window, first centre at pad + (w-1)/2, last at pad + size - (w+1)/2, y
+    anchored to the bottom; every sum window must fit the padded image."""
+    H, W = image_shape
+    wh, ww = window_size
+    sh, sw = sum_window
+    if validate:
+        if sh < wh or sw < ww:
+            raise ValueError(f"SumWindow {tuple(sum_window)} must be >= window_size {tuple(window_size)}")
+        if wh > H or ww > W:
+            raise ValueError(f"Window size {window_size} exceeds image dimensions {image_shape}")
+        if overlap < 0 or overlap >= 100:
+            raise ValueError(f"Overlap must be in range [0, 100), got {overlap}")
+    pt, pb, pl, pr = compute_padding_for_single_mode(window_size, sum_window)
+    sx = max(1, round((1 - overlap / 100) * ww))
+    sy = max(1, round((1 - overlap / 100) * wh))
+    fx, lx = pl + (ww - 1) / 2.0, pl + W - (ww + 1) / 2.0
+    fy, ly = pt + (wh - 1) / 2.0, pt + H - (wh + 1) / 2.0
+    nx = max(1, int(np.floor((lx - fx) / sx)) + 1)
+    ny = max(1, int(np.floor((ly - fy) / sy)) + 1)
+    cx = np.linspace(fx, fx + sx * (nx - 1), nx, dtype=np.float32)
+    cy = np.linspace(ly - sy * (ny - 1), ly, ny, dtype=np.float32)
+    if cx[-1] + (sw - 1) / 2.0 >= W + pl + pr or cy[-1] + (sh - 1) / 2.0 >= H + pt + pb:
+        raise ValueError(f"single-mode window centres exceed the padded image (sum window {tuple(sum_window)})")
+    return WindowCenters(cx, cy, nx, ny, sx, sy, (pt, pb, pl, pr))
+
+
+def window_weight(wsize, window_type: str, sum_window=None) -> np.ndarray:
+    """2D window weight (float32), the reference's _window_weight_fun: the
+    types an instantaneous or standard ensemble pass uses, and the single-mode
+    pair ('singlepix': ones on the wsize block at ceil((sum - wsize)/2) of a
+    sum-window array; 'bsingle': ones over the sum window)."""
     m, n = wsize
     t = window_type.lower()
+    if t in ("singlepix", "bsingle"):
+        if sum_window is None:
+            raise ValueError(f"SumWindow must be provided for {window_type!r}")
+        sh, sw = int(sum_window[0]), int(sum_window[1])
+        if t == "bsingle":
+            return np.ones((sh, sw), dtype=np.float32)
+        w = np.zeros((sh, sw), dtype=np.float32)
+        r0, c0 = -((m - sh) // 2), -((n - sw) // 2)     # ceil((sum - size) / 2)
+        w[r0:r0 + m, c0:c0 + n] = 1.0
+        return w
     if t == "square":
         return np.ones((m, n), dtype=np.float32)
     if t in ("gaussian", "gaussiantop"):
@@ -129,11 +184,28 @@ def _pad_axis(ctrs, spacing, size):
     return np.concatenate([pre, ctrs, post[::-1]]), len(pre), len(post)
 
 
-def build_pass_grids(image_shape, window_sizes, overlaps) -> list[PassGrid]:
+def window_centers_original(image_shape, window_size, overlap, run_type="standard",
+                            sum_window=None) -> WindowCenters:
+    """Window centres in original image coordinates for either run type: a
+    single-mode grid has its padding subtracted (cpu_ensemble.py:695-750), so
+    every pass shares one coordinate system; ``padding`` is kept for the
+    correlation call, which adds it back."""
+    if run_type == "standard":
+        return compute_window_centers(image_shape, window_size, overlap)
+    if run_type != "single":
+        raise ValueError(f"run type {run_type!r}: standard or single")
+    wc = compute_window_centers_single_mode(image_shape, window_size, sum_window, overlap)
+    pt, _, pl, _ = wc.padding
+    return WindowCenters(wc.win_ctrs_x - np.float32(pl), wc.win_ctrs_y - np.float32(pt), wc.n_win_x, wc.n_win_y,
+                         wc.win_spacing_x, wc.win_spacing_y, wc.padding)
+
+
+def build_pass_grids(image_shape, window_sizes, overlaps, run_types=None, sum_window=None) -> list[PassGrid]:
     H, W = image_shape
     grids: list[PassGrid] = []
     for p, win in enumerate(window_sizes):
-        wc = compute_window_centers(image_shape, win, overlaps[p])
+        rt = "standard" if run_types is None else run_types[p]
+        wc = window_centers_original(image_shape, win, overlaps[p], rt, sum_window)
         xa, prex, postx = _pad_axis(wc.win_ctrs_x, wc.win_spacing_x, W)
         ya, prey, posty = _pad_axis(wc.win_ctrs_y, wc.win_spacing_y, H)
         if p == 0:
