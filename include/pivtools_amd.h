@@ -342,6 +342,21 @@ int pivamd_davis_tokens(const unsigned char *data, unsigned long n, unsigned lon
                         unsigned short *out, unsigned long nvals, long long *prev_io, unsigned long *written);
 long long pivamd_lz4_block(const unsigned char *src, unsigned long n, unsigned char *dst, unsigned long cap);
 
+/* TIFF frames (generic_readers.py:81-122, read_tiff through tifffile, and its
+ * _luma, :33-51).  Host memory: pivamd_tiff_lzw and pivamd_packbits decode one
+ * strip or tile (compression 5, 32773) into dst and return its size (LZW: -1
+ * for a malformed or old-style stream).  Device: pivamd_gray_from_samples
+ * turns n frames of H x W pixels of S samples (interleaved, or planar S x H x
+ * W) into float32 frames: the sample (S 1), OpenCV's BT.601 RGB2GRAY of the
+ * first three (S 3 or 4; 8/16-bit (9798 R + 19235 G + 3735 B + 2^14) >> 15,
+ * float 0.299 R + 0.587 G + 0.114 B), or the samples' mean (other S).
+ * sample_type: 0 u8, 1 u16, 2 i8, 3 i16, 4 u32, 5 i32, 6 f32; 1 is returned
+ * for luma of another type. */
+long long pivamd_tiff_lzw(const unsigned char *src, unsigned long n, unsigned char *dst, unsigned long cap);
+long long pivamd_packbits(const unsigned char *src, unsigned long n, unsigned char *dst, unsigned long cap);
+int pivamd_gray_from_samples(const void *d_src, int n_frames, int H, int W, int S, int planar, int sample_type,
+                             float *d_out, void *stream);
+
 /* Multipass glue (device buffers, planar fields [dy plane, dx plane]):
  * uint16 ingest, displacement gate + predictor add-back, normalised-median
  * test, nearest infill, edge padding, one axis of the Gaussian predictor

@@ -259,9 +259,100 @@ __host__ __device__ inline size_t davis_payload_bytes(int enc, size_t n)
     }
 }
 
+// ---------------------------------------------------------------------------
+// TIFF samples to float32 gray frames (generic_readers.py:33-51, 81-122): the
+// host decompresses the strips or tiles (and undoes a horizontal predictor);
+// here every pixel of n frames of S samples (interleaved, or planar S x H x W
+// when PLANAR) becomes one float32 value:
+//   S == 1        the sample;
+//   S == 3 or 4   BT.601 luma of the first three samples (R, G, B; a fourth
+//                 is alpha and ignored), OpenCV's RGB2GRAY rule: for 8- and
+//                 16-bit samples (R 9798 + G 19235 + B 3735 + 2^14) >> 15, for
+//                 float samples R 0.299f + G 0.587f + B 0.114f in that order;
+//   other S       the mean of the samples (numpy's mean: a double sum for
+//                 integer samples, a float sum for float ones).
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ float gray_luma(T r, T g, T b)
+{
+    if constexpr (sizeof(T) <= 2 && T(-1) > T(0)) {
+        const uint32_t y = (uint32_t)r * 9798u + (uint32_t)g * 19235u + (uint32_t)b * 3735u + 16384u;
+        return (float)(y >> 15);
+    } else {
+        // three rounded products and two rounded sums, as OpenCV's scalar loop
+#pragma clang fp contract(off)
+        const float pr = (float)r * 0.299f, pg = (float)g * 0.587f, pb = (float)b * 0.114f;
+        return (pr + pg) + pb;
+    }
+}
+
+template <typename T, bool PLANAR>
+__global__ void __launch_bounds__(256) gray_kernel(const T *__restrict__ src, size_t npx, int S, int luma,
+                                                   float *__restrict__ out)
+{
+    const size_t f = blockIdx.y;
+    const T *fr = src + f * npx * S;
+    float *dst = out + f * npx;
+    const size_t step = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < npx; i += step) {
+        auto at = [&](int k) -> T { return PLANAR ? fr[(size_t)k * npx + i] : fr[i * S + k]; };
+        float v;
+        if (S == 1) {
+            v = (float)at(0);
+        } else if (luma) {
+            v = gray_luma<T>(at(0), at(1), at(2));
+        } else if constexpr (sizeof(T) == 4 && T(0.5) != T(0)) {
+            float a = at(0);
+            for (int k = 1; k < S; ++k) a += at(k);
+            v = a / (float)S;
+        } else {
+            double a = 0.0;
+            for (int k = 0; k < S; ++k) a += (double)at(k);
+            v = (float)(a / (double)S);
+        }
+        dst[i] = v;
+    }
+}
+
+template <typename T>
+static int launch_gray(const void *src, int n, size_t npx, int S, int planar, int luma, float *out, hipStream_t st)
+{
+    const size_t blocks = (npx + 255) / 256;
+    const dim3 grid((unsigned)(blocks < 4096 ? blocks : 4096), (unsigned)n);
+    if (planar)
+        hipLaunchKernelGGL((gray_kernel<T, true>), grid, dim3(256), 0, st, (const T *)src, npx, S, luma, out);
+    else
+        hipLaunchKernelGGL((gray_kernel<T, false>), grid, dim3(256), 0, st, (const T *)src, npx, S, luma, out);
+    return hipGetLastError() == hipSuccess ? PIVAMD_OK : PIVAMD_ERR_LAUNCH;
+}
+
 }  // namespace pivamd
 
 using namespace pivamd;
+
+// sample type codes: 0 uint8, 1 uint16, 2 int8, 3 int16, 4 uint32, 5 int32, 6 float32.
+// Luma (S of 3 or 4) is defined for uint8, uint16 and float32, as OpenCV's
+// conversion is; other types with S of 3 or 4 are refused (1).
+extern "C" int pivamd_gray_from_samples(const void *d_src, int n_frames, int H, int W, int S, int planar,
+                                        int sample_type, float *d_out, hipStream_t st)
+{
+    if (n_frames < 0 || H <= 0 || W <= 0 || S < 1 || n_frames > 65535) return PIVAMD_ERR_ARG;
+    if (n_frames == 0) return PIVAMD_OK;
+    if (!d_src || !d_out) return PIVAMD_ERR_ARG;
+    const int luma = (S == 3 || S == 4) ? 1 : 0;
+    if (luma && sample_type != 0 && sample_type != 1 && sample_type != 6) return PIVAMD_ERR_ARG;
+    const size_t npx = (size_t)H * W;
+    switch (sample_type) {
+    case 0: return launch_gray<uint8_t>(d_src, n_frames, npx, S, planar, luma, d_out, st);
+    case 1: return launch_gray<uint16_t>(d_src, n_frames, npx, S, planar, luma, d_out, st);
+    case 2: return launch_gray<int8_t>(d_src, n_frames, npx, S, planar, luma, d_out, st);
+    case 3: return launch_gray<int16_t>(d_src, n_frames, npx, S, planar, luma, d_out, st);
+    case 4: return launch_gray<uint32_t>(d_src, n_frames, npx, S, planar, luma, d_out, st);
+    case 5: return launch_gray<int32_t>(d_src, n_frames, npx, S, planar, luma, d_out, st);
+    case 6: return launch_gray<float>(d_src, n_frames, npx, S, planar, luma, d_out, st);
+    default: return PIVAMD_ERR_ARG;
+    }
+}
 
 #define PIVAMD_LAUNCHED() (hipGetLastError() == hipSuccess ? 0 : PIVAMD_ERR_LAUNCH)
 

@@ -1,4 +1,4 @@
-// readers.cpp -- the serial decompressors of the DaVis containers, on the host.
+// readers.cpp -- the serial decompressors of the camera containers, on the host.
 //
 // The camera containers the reference reads (pivtools_core/image_handling/
 // readers/) store a frame either packed (decoded on the device by
@@ -22,6 +22,16 @@
 //                       literals, a little-endian 16-bit match offset and
 //                       match length 4 + nibble + 255-run extension; the last
 //                       sequence carries literals only).
+//   TIFF LZW            compression 5 (TIFF 6.0 section 13): codes MSB-first,
+//                       9 to 12 bits, 256 clear and 257 end of information,
+//                       new codes from 258, the width growing one code early
+//                       (at 511, 1023 and 2047 entries).
+//   PackBits            compression 32773 (TIFF 6.0 section 9): a header byte
+//                       n copies n + 1 literal bytes (0..127), repeats the next
+//                       byte 1 - n times (-127..-1), or is skipped (-128).
+//
+// The TIFF strips are read by pivtools_amd/tiff.py, as the reference reads
+// them through tifffile (generic_readers.py:81-122).
 #include <stddef.h>
 #include <stdint.h>
 #include <string.h>
@@ -130,6 +140,116 @@ extern "C" long long pivamd_lz4_block(const uint8_t *src, size_t n, uint8_t *dst
         const uint8_t *m = dst + op - offset;
         for (size_t k = 0; k < mlen; ++k) dst[op + k] = m[k];
         op += mlen;
+    }
+    return (long long)op;
+}
+
+// Decode one TIFF LZW strip or tile of n bytes into dst (capacity cap; the
+// block's expected size).  Returns the bytes written (a stream that ends
+// before its end-of-information code keeps what it decoded, as libtiff
+// does), or -1 for a code that is not yet in the table or an old-style
+// (LSB-first) stream.
+extern "C" long long pivamd_tiff_lzw(const uint8_t *src, size_t n, uint8_t *dst, size_t cap)
+{
+    if (n >= 2 && src[0] == 0 && (src[1] & 1)) return -1;     // old-style LZW, bit-reversed codes
+    // table: each code is a prefix code plus a last byte, with its length and
+    // first byte, so a string is written back to front without a stack
+    static thread_local uint16_t prefix[4096];
+    static thread_local uint8_t suffix[4096], first[4096];
+    static thread_local uint16_t length[4096];
+    for (int c = 0; c < 256; ++c) {
+        prefix[c] = 0xFFFF;
+        suffix[c] = (uint8_t)c;
+        first[c] = (uint8_t)c;
+        length[c] = 1;
+    }
+    int width = 9, next = 258, prev = -1;
+    size_t op = 0;
+    uint64_t acc = 0;
+    int have = 0;
+    size_t ip = 0;
+    for (;;) {
+        while (have < width && ip < n) {
+            acc = (acc << 8) | src[ip++];
+            have += 8;
+        }
+        if (have < width) break;                       // input ended without EOI
+        const int code = (int)((acc >> (have - width)) & ((1u << width) - 1u));
+        have -= width;
+        if (code == 257) break;
+        if (code == 256) {
+            width = 9;
+            next = 258;
+            prev = -1;
+            continue;
+        }
+        int out_code;
+        uint8_t head;
+        if (code < next) {
+            if (code >= 256 && code < 258) return -1;
+            out_code = code;
+            head = first[code];
+        } else if (code == next && prev >= 0) {
+            out_code = -1;                            // KwKwK: prev's string plus its first byte
+            head = first[prev];
+        } else {
+            return -1;
+        }
+        // add prev + head as a new code
+        if (prev >= 0 && next < 4096) {
+            prefix[next] = (uint16_t)prev;
+            suffix[next] = head;
+            first[next] = first[prev];
+            length[next] = (uint16_t)(length[prev] + 1);
+            ++next;
+        }
+        const int wcode = out_code >= 0 ? out_code : next - 1;
+        const size_t len = length[wcode];
+        if (op + len > cap) {
+            // write what fits of the string and stop: the block is full
+            size_t keep = cap - op;
+            int c = wcode;
+            for (size_t k = len; k > keep; --k) c = prefix[c];
+            for (size_t k = keep; k > 0; --k) {
+                dst[op + k - 1] = suffix[c];
+                c = prefix[c];
+            }
+            op = cap;
+            break;
+        }
+        int c = wcode;
+        for (size_t k = len; k > 0; --k) {
+            dst[op + k - 1] = suffix[c];
+            c = prefix[c];
+        }
+        op += len;
+        prev = wcode;
+        if (next + 1 >= (1 << width) && width < 12) ++width;
+    }
+    return (long long)op;
+}
+
+// Decode one PackBits strip or tile into dst (capacity cap).  Returns the
+// bytes written; a run that passes cap or the input is cut at it.
+extern "C" long long pivamd_packbits(const uint8_t *src, size_t n, uint8_t *dst, size_t cap)
+{
+    size_t ip = 0, op = 0;
+    while (ip < n && op < cap) {
+        const int h = (int8_t)src[ip++];
+        if (h >= 0) {
+            size_t cnt = (size_t)h + 1;
+            if (cnt > n - ip) cnt = n - ip;
+            if (cnt > cap - op) cnt = cap - op;
+            memcpy(dst + op, src + ip, cnt);
+            ip += cnt;
+            op += cnt;
+        } else if (h != -128) {
+            if (ip >= n) break;
+            size_t cnt = (size_t)(1 - h);
+            if (cnt > cap - op) cnt = cap - op;
+            memset(dst + op, src[ip++], cnt);
+            op += cnt;
+        }
     }
     return (long long)op;
 }

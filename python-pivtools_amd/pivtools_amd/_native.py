@@ -119,6 +119,9 @@ def core() -> ctypes.CDLL:
         L.pivamd_davis_tokens.restype = _i
         L.pivamd_lz4_block.argtypes = [_vp, _sz, _vp, _sz]
         L.pivamd_lz4_block.restype = ctypes.c_longlong
+        for f in ("pivamd_tiff_lzw", "pivamd_packbits"):
+            getattr(L, f).argtypes = [_vp, _sz, _vp, _sz]
+            getattr(L, f).restype = ctypes.c_longlong
         L.pivamd_warp_set_variant.argtypes = [_i]
         L.pivamd_warp_set_variant.restype = _i
         L.pivamd_warp_get_variant.argtypes = []
@@ -194,6 +197,7 @@ GLUE_SIGNATURES: dict = {
     "pivamd_unpack_mono12p": [_vp, _sz, _i, _i, _i, _vp, _vp, _vp],
     "pivamd_bayer_rg8_superpixel": [_vp, _sz, _i, _i, _i, _vp, _vp, _vp],
     "pivamd_davis_decode": [_vp, _sz, _i, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _vp],
+    "pivamd_gray_from_samples": [_vp, _i, _i, _i, _i, _i, _i, _vp, _vp],
 }
 
 
