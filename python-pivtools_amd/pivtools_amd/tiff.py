@@ -140,11 +140,13 @@ def _read_tags(buf: bytes, off: int, bo: str, big: bool, name: str) -> Tuple[dic
     return tags, nxt
 
 
-def parse_tiff(path) -> Tuple[int, TiffPage]:
-    """(number of pages, first page) of a TIFF file (classic or BigTIFF)."""
+def parse_tiff(path, buf: Optional[bytes] = None) -> Tuple[int, TiffPage]:
+    """(number of pages, first page) of a TIFF file (classic or BigTIFF);
+    buf: the file's bytes when the caller has read them."""
     path = str(path)
     name = Path(path).name
-    buf = Path(path).read_bytes()
+    if buf is None:
+        buf = Path(path).read_bytes()
     if len(buf) < 8 or buf[:2] not in (b"II", b"MM"):
         raise ValueError(f"{name} does not start with a TIFF byte-order mark")
     bo = "<" if buf[:2] == b"II" else ">"
@@ -194,6 +196,10 @@ def parse_tiff(path) -> Tuple[int, TiffPage]:
     if offs is None or cnts is None or len(offs) != len(cnts):
         raise ValueError(f"{name}: missing or mismatched {'tile' if tiled else 'strip'} offsets and byte counts")
     height = one(_T_LENGTH)
+    if tiled and (one(_T_TILE_L) <= 0 or one(_T_TILE_W) <= 0):
+        raise ValueError(f"{name}: tile size {one(_T_TILE_L)} x {one(_T_TILE_W)}")
+    if not tiled and one(_T_ROWS_PER_STRIP, height) <= 0:
+        raise ValueError(f"{name}: RowsPerStrip {one(_T_ROWS_PER_STRIP, height)}")
     page = TiffPage(path=path, byteorder=bo, width=one(_T_WIDTH), height=height, depth=one(_T_IMAGE_DEPTH, 1),
                     spp=spp, bits=int(bits[0]), sample_format=int(fmt[0]), planar=one(_T_PLANAR, 1),
                     compression=one(_T_COMPRESSION, 1), predictor=one(_T_PREDICTOR, 1),
@@ -258,7 +264,8 @@ def tiff_samples(path) -> Tuple[np.ndarray, str, int]:
     """The first page's samples in native byte order, (H, W), (H, W, S) or
     (S, H, W) as its axes say, with the axes and the device sample code.
     Refuses what ``read_tiff`` refuses."""
-    pages, page = parse_tiff(path)
+    buf = Path(str(path)).read_bytes()
+    pages, page = parse_tiff(path, buf)
     _check_page(pages, page)
     name = Path(page.path).name
     typ, code = page.sample_type
@@ -267,7 +274,6 @@ def tiff_samples(path) -> Tuple[np.ndarray, str, int]:
     sp = S if page.planar == 1 else 1          # samples per stored pixel
     planes = 1 if page.planar == 1 else S
     out = np.empty((planes, H, W, sp), dtype=np.dtype(typ))
-    buf = Path(page.path).read_bytes()
 
     def block(k: int, rows: int, cols: int) -> np.ndarray:
         o, n = page.offsets[k], page.counts[k]
