@@ -354,6 +354,12 @@ long long pivamd_lz4_block(const unsigned char *src, unsigned long n, unsigned c
  * for luma of another type. */
 long long pivamd_tiff_lzw(const unsigned char *src, unsigned long n, unsigned char *dst, unsigned long cap);
 long long pivamd_packbits(const unsigned char *src, unsigned long n, unsigned char *dst, unsigned long cap);
+/* PNG frames (generic_readers.py:125-176, read_png_jpeg for PNG): undo the
+ * scanline filters of rows rows of rowbytes bytes (src: filter byte + row
+ * each), bpp bytes per pixel; 0, or -1 for a filter type above 4.  The gray
+ * conversion is pivamd_gray_from_samples, as for TIFF. */
+int pivamd_png_unfilter(const unsigned char *src, unsigned long rows, unsigned long rowbytes, int bpp,
+                        unsigned char *dst);
 int pivamd_gray_from_samples(const void *d_src, int n_frames, int H, int W, int S, int planar, int sample_type,
                              float *d_out, void *stream);
 

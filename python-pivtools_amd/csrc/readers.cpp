@@ -30,6 +30,14 @@
 //                       n copies n + 1 literal bytes (0..127), repeats the next
 //                       byte 1 - n times (-127..-1), or is skipped (-128).
 //
+//   PNG scanlines       (PNG specification, filter method 0): each row of the
+//                       inflated stream is a filter-type byte and the row's
+//                       bytes, filtered against the reconstructed left byte a
+//                       (bpp bytes back), up byte b and up-left byte c: 0 none,
+//                       1 Sub x + a, 2 Up x + b, 3 Average x + floor((a + b)
+//                       / 2), 4 Paeth x + the one of a, b, c nearest
+//                       p = a + b - c (ties to a, then b), all modulo 256.
+//
 // The TIFF strips are read by pivtools_amd/tiff.py, as the reference reads
 // them through tifffile (generic_readers.py:81-122).
 #include <stddef.h>
@@ -252,4 +260,47 @@ extern "C" long long pivamd_packbits(const uint8_t *src, size_t n, uint8_t *dst,
         }
     }
     return (long long)op;
+}
+
+// Undo the PNG row filters of rows scanlines of rowbytes bytes each (src
+// holds each row's filter byte then its bytes; dst the reconstructed rows).
+// bpp: bytes per complete pixel, at least 1.  Returns 0, or -1 for a filter
+// type above 4 (a damaged stream).
+extern "C" int pivamd_png_unfilter(const uint8_t *src, size_t rows, size_t rowbytes, int bpp, uint8_t *dst)
+{
+    if (bpp < 1) return -1;
+    const size_t b = (size_t)bpp;
+    for (size_t r = 0; r < rows; ++r) {
+        const uint8_t *in = src + r * (rowbytes + 1);
+        const int type = in[0];
+        ++in;
+        uint8_t *out = dst + r * rowbytes;
+        const uint8_t *up = r ? out - rowbytes : nullptr;
+        switch (type) {
+        case 0: memcpy(out, in, rowbytes); break;
+        case 1:
+            for (size_t i = 0; i < rowbytes; ++i) out[i] = (uint8_t)(in[i] + (i >= b ? out[i - b] : 0));
+            break;
+        case 2:
+            for (size_t i = 0; i < rowbytes; ++i) out[i] = (uint8_t)(in[i] + (up ? up[i] : 0));
+            break;
+        case 3:
+            for (size_t i = 0; i < rowbytes; ++i) {
+                const unsigned a = i >= b ? out[i - b] : 0, u = up ? up[i] : 0;
+                out[i] = (uint8_t)(in[i] + ((a + u) >> 1));
+            }
+            break;
+        case 4:
+            for (size_t i = 0; i < rowbytes; ++i) {
+                const int a = i >= b ? out[i - b] : 0, u = up ? up[i] : 0, c = (up && i >= b) ? up[i - b] : 0;
+                const int p = a + u - c;
+                const int pa = p > a ? p - a : a - p, pb = p > u ? p - u : u - p, pc = p > c ? p - c : c - p;
+                const int pred = (pa <= pb && pa <= pc) ? a : (pb <= pc ? u : c);
+                out[i] = (uint8_t)(in[i] + pred);
+            }
+            break;
+        default: return -1;
+        }
+    }
+    return 0;
 }

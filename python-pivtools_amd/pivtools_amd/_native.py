@@ -122,6 +122,8 @@ def core() -> ctypes.CDLL:
         for f in ("pivamd_tiff_lzw", "pivamd_packbits"):
             getattr(L, f).argtypes = [_vp, _sz, _vp, _sz]
             getattr(L, f).restype = ctypes.c_longlong
+        L.pivamd_png_unfilter.argtypes = [_vp, _sz, _sz, _i, _vp]
+        L.pivamd_png_unfilter.restype = _i
         L.pivamd_warp_set_variant.argtypes = [_i]
         L.pivamd_warp_set_variant.restype = _i
         L.pivamd_warp_get_variant.argtypes = []
