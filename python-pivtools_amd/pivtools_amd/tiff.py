@@ -177,17 +177,17 @@ def parse_tiff(path, buf: Optional[bytes] = None) -> Tuple[int, TiffPage]:
 
     def one(tag, default=None):
         v = tags0.get(tag)
-        if v is None:
+        if not v:
             if default is None:
                 raise ValueError(f"{name}: required TIFF tag {tag} missing")
             return default
         return int(v[0])
 
     spp = one(_T_SPP, 1)
-    bits = tags0.get(_T_BITS, (1,))
+    bits = tags0.get(_T_BITS) or (1,)
     if len(set(bits)) != 1:
         raise ValueError(f"{name} has samples of different bit depths {tuple(bits)}")
-    fmt = tags0.get(_T_SAMPLE_FORMAT, (1,))
+    fmt = tags0.get(_T_SAMPLE_FORMAT) or (1,)
     if len(set(fmt)) != 1:
         raise ValueError(f"{name} has samples of different formats {tuple(fmt)}")
     tiled = _T_TILE_OFFSETS in tags0
