@@ -1200,6 +1200,51 @@ __device__ __forceinline__ void xc_window_half1(const XcorrArgs &g, float (&zr)[
     plane_epilogue<N, T>(g, S, PX, valid, gw, n, w, enorm, t, r0, r1, r2, r3, r4);
 }
 
+// The K weighted samples of one lane's column of a window: elements off,
+// off + step, ... of the image stack, times the weights w[0], w[wstep], ...
+// As raw buffer loads (one resource over the whole stack, uniform for the
+// workgroup; the lane's byte offset in a VGPR; the row step a uniform
+// scalar offset) no load needs its own 64-bit address arithmetic.  Stacks of
+// 2^32 bytes or more take plain loads.
+template <int K, typename TI>
+__device__ __forceinline__ void load_window(const TI *sa_, const TI *sb_, size_t elems, size_t off, size_t step,
+                                            const float *wa, const float *wb, int wstep, float (&zr)[K],
+                                            float (&zi)[K], float &sa, float &sb)
+{
+    auto use = [&](int k, float a, float b) {
+        a = a * wa[k * wstep];
+        b = b * wb[k * wstep];
+        zr[k] = a;
+        zi[k] = b;
+        sa += a;
+        sb += b;
+    };
+    const size_t bytes = elems * sizeof(TI);
+    if (bytes < ((size_t)1 << 32)) {
+        const __amdgpu_buffer_rsrc_t ra =
+            __builtin_amdgcn_make_buffer_rsrc((void *)sa_, (short)0, (int)(uint32_t)bytes, 0x00020000);
+        const __amdgpu_buffer_rsrc_t rb =
+            __builtin_amdgcn_make_buffer_rsrc((void *)sb_, (short)0, (int)(uint32_t)bytes, 0x00020000);
+        const uint32_t vo = (uint32_t)(off * sizeof(TI)), so = (uint32_t)(step * sizeof(TI));
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            float a, b;
+            if constexpr (sizeof(TI) == 2) {
+                a = (float)__builtin_amdgcn_raw_buffer_load_b16(ra, vo, (uint32_t)k * so, 0);
+                b = (float)__builtin_amdgcn_raw_buffer_load_b16(rb, vo, (uint32_t)k * so, 0);
+            } else {
+                a = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(ra, vo, (uint32_t)k * so, 0));
+                b = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rb, vo, (uint32_t)k * so, 0));
+            }
+            use(k, a, b);
+        }
+    } else {
+        const TI *pa = sa_ + off, *pb = sb_ + off;
+#pragma unroll
+        for (int k = 0; k < K; ++k) use(k, (float)pa[(size_t)k * step], (float)pb[(size_t)k * step]);
+    }
+}
+
 // Window geometry of global window gw (image n, window w, top-left corner);
 // false when masked, out of the image or past the end.
 template <int N>
@@ -1257,22 +1302,12 @@ __device__ __forceinline__ void xcorr_body(const XcorrArgs &g, const int b)
     float zr[N1], zi[N1];
     float sa = 0.0f, sb = 0.0f;
     if (valid) {
-        const size_t img = (size_t)n * g.H * g.W + (size_t)(row0 + q) * g.W + col0 + l;
+        const size_t plane = (size_t)g.H * g.W, elems = (size_t)(g.total / ((long long)g.nWy * g.nWx)) * plane;
+        const size_t off = (size_t)n * plane + (size_t)(row0 + q) * g.W + col0 + l, step = (size_t)R * g.W;
         const float *wa = g.wA + q * N + l, *wb = g.wB + q * N + l;
         // 16-bit counts convert exactly, so both inputs give the same bits
-        auto load = [&](const auto *pa, const auto *pb) {
-#pragma unroll
-            for (int k = 0; k < N1; ++k) {
-                const float a = (float)pa[(size_t)k * R * g.W] * wa[k * R * N];
-                const float bb = (float)pb[(size_t)k * R * g.W] * wb[k * R * N];
-                zr[k] = a;
-                zi[k] = bb;
-                sa += a;
-                sb += bb;
-            }
-        };
-        if (g.imgA16) load(g.imgA16 + img, g.imgB16 + img);
-        else load(g.imgA + img, g.imgB + img);
+        if (g.imgA16) load_window<N1>(g.imgA16, g.imgB16, elems, off, step, wa, wb, R * N, zr, zi, sa, sb);
+        else load_window<N1>(g.imgA, g.imgB, elems, off, step, wa, wb, R * N, zr, zi, sa, sb);
     } else {
 #pragma unroll
         for (int k = 0; k < N1; ++k) { zr[k] = 0.0f; zi[k] = 0.0f; }
@@ -1320,21 +1355,12 @@ __device__ __forceinline__ void xcorr_half_body(const XcorrArgs &g, const int b)
     float zr[N1], zi[N1];
     float sa = 0.0f, sb = 0.0f;
     if (valid) {
-        const size_t img = (size_t)n * g.H * g.W + (size_t)(row0 + q) * g.W + col0 + l;
+        const size_t plane = (size_t)g.H * g.W, elems = (size_t)(g.total / ((long long)g.nWy * g.nWx)) * plane;
+        const size_t off = (size_t)n * plane + (size_t)(row0 + q) * g.W + col0 + l, step = (size_t)R * g.W;
         const float *wa = g.wA + q * N + l, *wb = g.wB + q * N + l;
-        auto load = [&](const auto *pa, const auto *pb) {
-#pragma unroll
-            for (int k = 0; k < N1; ++k) {
-                const float a = (float)pa[(size_t)k * R * g.W] * wa[k * R * N];
-                const float bb = (float)pb[(size_t)k * R * g.W] * wb[k * R * N];
-                zr[k] = a;
-                zi[k] = bb;
-                sa += a;
-                sb += bb;
-            }
-        };
-        if (g.imgA16) load(g.imgA16 + img, g.imgB16 + img);
-        else load(g.imgA + img, g.imgB + img);
+        // 16-bit counts convert exactly, so both inputs give the same bits
+        if (g.imgA16) load_window<N1>(g.imgA16, g.imgB16, elems, off, step, wa, wb, R * N, zr, zi, sa, sb);
+        else load_window<N1>(g.imgA, g.imgB, elems, off, step, wa, wb, R * N, zr, zi, sa, sb);
     } else {
 #pragma unroll
         for (int k = 0; k < N1; ++k) { zr[k] = 0.0f; zi[k] = 0.0f; }
@@ -1377,21 +1403,11 @@ __device__ __forceinline__ void xcorr_half1_body(const XcorrArgs &g, const int b
     float zr[N], zi[N];
     float sa = 0.0f, sb = 0.0f;
     if (valid) {
-        const size_t img = (size_t)n * g.H * g.W + (size_t)row0 * g.W + col0 + l;
+        const size_t plane = (size_t)g.H * g.W, elems = (size_t)(g.total / ((long long)g.nWy * g.nWx)) * plane;
+        const size_t off = (size_t)n * plane + (size_t)row0 * g.W + col0 + l, step = (size_t)g.W;
         const float *wa = g.wA + l, *wb = g.wB + l;
-        auto load = [&](const auto *pa, const auto *pb) {
-#pragma unroll
-            for (int k = 0; k < N; ++k) {
-                const float a = (float)pa[(size_t)k * g.W] * wa[k * N];
-                const float bb = (float)pb[(size_t)k * g.W] * wb[k * N];
-                zr[k] = a;
-                zi[k] = bb;
-                sa += a;
-                sb += bb;
-            }
-        };
-        if (g.imgA16) load(g.imgA16 + img, g.imgB16 + img);
-        else load(g.imgA + img, g.imgB + img);
+        if (g.imgA16) load_window<N>(g.imgA16, g.imgB16, elems, off, step, wa, wb, N, zr, zi, sa, sb);
+        else load_window<N>(g.imgA, g.imgB, elems, off, step, wa, wb, N, zr, zi, sa, sb);
     } else {
 #pragma unroll
         for (int k = 0; k < N; ++k) { zr[k] = 0.0f; zi[k] = 0.0f; }
@@ -2636,7 +2652,7 @@ int launch_ragged_main<PIVAMD_XCORR_SIZE>(RaggedArgs &ra, int variant, hipStream
     case 8: return ragged_fft<N, 8>(ra, st);
     case 0:
         if constexpr (N == 48 || N == 64 || N == 128) return ragged_half<N>(ra, st);
-        else if constexpr (N == 32) return ragged_half1<N>(ra, st);
+        else if constexpr (N == 16 || N == 32) return ragged_half1<N>(ra, st);
         else return ragged_fft<N, xc_radix(N)>(ra, st);
     default: return PIVAMD_ERR_ARG;    // the MFMA and persistent forms have no ragged launch
     }
@@ -2664,7 +2680,7 @@ int launch_main<PIVAMD_XCORR_SIZE>(XcorrArgs g, hipStream_t st)
     case 8: return launch_fft<N, 8>(g, st);
     case 0:
         if constexpr (N == 48 || N == 64 || N == 128) return launch_half<N, 4>(g, st);   // more windows per CU
-        else if constexpr (N == 32) return launch_half1<N>(g, st);
+        else if constexpr (N == 16 || N == 32) return launch_half1<N>(g, st);
         else return launch_fft<N, xc_radix(N)>(g, st);
     default: return PIVAMD_ERR_ARG;
     }
@@ -2821,7 +2837,8 @@ extern "C" int pivamd_xcorr_variants(int N, int *out, int max_out)
 extern "C" int pivamd_xcorr_default_variant(int N)
 {
     if (!pivamd_size_supported(N, N)) return -1;
-    return (N == 48 || N == 64 || N == 128) ? kVariantHalf + 4 : N == 32 ? kVariantHalf + 1 : xc_radix(N);
+    return (N == 48 || N == 64 || N == 128) ? kVariantHalf + 4 : (N == 16 || N == 32) ? kVariantHalf + 1
+                                                                                        : xc_radix(N);
 }
 
 extern "C" int pivamd_xcorr_set_variant(int N, int v)

@@ -12,7 +12,7 @@ from pivtools_amd import autotune as AT
 
 BUILT = {8: [1, 65], 12: [1, 65], 16: [1, 2, 16, 65, 96, 97, 98], 24: [1, 2, 65], 32: [1, 2, 4, 16, 65],
          48: [2, 4, 16, 68], 64: [2, 4, 16, 36, 68], 96: [4, 36, 68], 128: [4, 8, 36, 68]}
-DEFAULT = {8: 1, 12: 1, 16: 1, 24: 1, 32: 65, 48: 68, 64: 68, 96: 4, 128: 68}
+DEFAULT = {8: 1, 12: 1, 16: 65, 24: 1, 32: 65, 48: 68, 64: 68, 96: 4, 128: 68}
 
 
 @pytest.mark.parametrize("n", sorted(BUILT))
