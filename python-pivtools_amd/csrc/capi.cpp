@@ -179,10 +179,15 @@ unsigned char bulkxcorr2d(const float *fImageA, const float *fImageB, const floa
     for (int k = 0; k < 6; ++k)
         if (!dO[k]) return kErrNomem;
     float *hO[6] = {fPkLocX, fPkLocY, fPkHeight, fSx, fSy, fSxy};
+    // one weight for A and B (the instantaneous path's call) goes up once and
+    // reaches the kernel as one pointer, which then reads it once per sample
+    const bool same_w = fWindowWeightA == fWindowWeightB ||
+                        std::memcmp(fWindowWeightA, fWindowWeightB, numel * sizeof(float)) == 0;
+    if (same_w) dWb = dWa;
     bool ok = h2d(dA, fImageA, img) && h2d(dB, fImageB, img) && h2d(dM, fMask, nWin * sizeof(float)) &&
               h2d(dCx, fWinCtrsX, nWx * sizeof(float)) && h2d(dCy, fWinCtrsY, nWy * sizeof(float)) &&
               h2d(dWa, fWindowWeightA, numel * sizeof(float)) &&
-              h2d(dWb, fWindowWeightB, numel * sizeof(float));
+              (same_w || h2d(dWb, fWindowWeightB, numel * sizeof(float)));
     // the library leaves masked / out-of-bounds windows untouched: start
     // from the caller's contents
     if (!bEnsemble) {

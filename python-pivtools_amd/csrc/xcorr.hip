@@ -1205,19 +1205,33 @@ __device__ __forceinline__ void xc_window_half1(const XcorrArgs &g, float (&zr)[
 // As raw buffer loads (one resource over the whole stack, uniform for the
 // workgroup; the lane's byte offset in a VGPR; the row step a uniform
 // scalar offset) no load needs its own 64-bit address arithmetic.  Stacks of
-// 2^32 bytes or more take plain loads.
+// 2^32 bytes or more take plain loads.  same_w (uniform: the caller passed
+// one weight array as A and B, as the instantaneous path does) reads each
+// weight once.
 template <int K, typename TI>
 __device__ __forceinline__ void load_window(const TI *sa_, const TI *sb_, size_t elems, size_t off, size_t step,
-                                            const float *wa, const float *wb, int wstep, float (&zr)[K],
-                                            float (&zi)[K], float &sa, float &sb)
+                                            const float *wa, const float *wb, int wstep, bool same_w,
+                                            float (&zr)[K], float (&zi)[K], float &sa, float &sb)
 {
-    auto use = [&](int k, float a, float b) {
-        a = a * wa[k * wstep];
-        b = b * wb[k * wstep];
-        zr[k] = a;
-        zi[k] = b;
-        sa += a;
-        sb += b;
+    // each sample is weighted as soon as it arrives (few live registers)
+    auto run = [&](auto same, auto load) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            float a, b;
+            load(k, a, b);
+            const float w1 = wa[k * wstep];
+            const float w2 = decltype(same)::value ? w1 : wb[k * wstep];
+            a = a * w1;
+            b = b * w2;
+            zr[k] = a;
+            zi[k] = b;
+            sa += a;
+            sb += b;
+        }
+    };
+    auto go = [&](auto load) {
+        if (same_w) run(std::true_type{}, load);
+        else run(std::false_type{}, load);
     };
     const size_t bytes = elems * sizeof(TI);
     if (bytes < ((size_t)1 << 32)) {
@@ -1226,9 +1240,7 @@ __device__ __forceinline__ void load_window(const TI *sa_, const TI *sb_, size_t
         const __amdgpu_buffer_rsrc_t rb =
             __builtin_amdgcn_make_buffer_rsrc((void *)sb_, (short)0, (int)(uint32_t)bytes, 0x00020000);
         const uint32_t vo = (uint32_t)(off * sizeof(TI)), so = (uint32_t)(step * sizeof(TI));
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            float a, b;
+        go([&](int k, float &a, float &b) {
             if constexpr (sizeof(TI) == 2) {
                 a = (float)__builtin_amdgcn_raw_buffer_load_b16(ra, vo, (uint32_t)k * so, 0);
                 b = (float)__builtin_amdgcn_raw_buffer_load_b16(rb, vo, (uint32_t)k * so, 0);
@@ -1236,12 +1248,13 @@ __device__ __forceinline__ void load_window(const TI *sa_, const TI *sb_, size_t
                 a = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(ra, vo, (uint32_t)k * so, 0));
                 b = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rb, vo, (uint32_t)k * so, 0));
             }
-            use(k, a, b);
-        }
+        });
     } else {
         const TI *pa = sa_ + off, *pb = sb_ + off;
-#pragma unroll
-        for (int k = 0; k < K; ++k) use(k, (float)pa[(size_t)k * step], (float)pb[(size_t)k * step]);
+        go([&](int k, float &a, float &b) {
+            a = (float)pa[(size_t)k * step];
+            b = (float)pb[(size_t)k * step];
+        });
     }
 }
 
@@ -1306,8 +1319,8 @@ __device__ __forceinline__ void xcorr_body(const XcorrArgs &g, const int b)
         const size_t off = (size_t)n * plane + (size_t)(row0 + q) * g.W + col0 + l, step = (size_t)R * g.W;
         const float *wa = g.wA + q * N + l, *wb = g.wB + q * N + l;
         // 16-bit counts convert exactly, so both inputs give the same bits
-        if (g.imgA16) load_window<N1>(g.imgA16, g.imgB16, elems, off, step, wa, wb, R * N, zr, zi, sa, sb);
-        else load_window<N1>(g.imgA, g.imgB, elems, off, step, wa, wb, R * N, zr, zi, sa, sb);
+        if (g.imgA16) load_window<N1>(g.imgA16, g.imgB16, elems, off, step, wa, wb, R * N, g.wA == g.wB, zr, zi, sa, sb);
+        else load_window<N1>(g.imgA, g.imgB, elems, off, step, wa, wb, R * N, g.wA == g.wB, zr, zi, sa, sb);
     } else {
 #pragma unroll
         for (int k = 0; k < N1; ++k) { zr[k] = 0.0f; zi[k] = 0.0f; }
@@ -1359,8 +1372,8 @@ __device__ __forceinline__ void xcorr_half_body(const XcorrArgs &g, const int b)
         const size_t off = (size_t)n * plane + (size_t)(row0 + q) * g.W + col0 + l, step = (size_t)R * g.W;
         const float *wa = g.wA + q * N + l, *wb = g.wB + q * N + l;
         // 16-bit counts convert exactly, so both inputs give the same bits
-        if (g.imgA16) load_window<N1>(g.imgA16, g.imgB16, elems, off, step, wa, wb, R * N, zr, zi, sa, sb);
-        else load_window<N1>(g.imgA, g.imgB, elems, off, step, wa, wb, R * N, zr, zi, sa, sb);
+        if (g.imgA16) load_window<N1>(g.imgA16, g.imgB16, elems, off, step, wa, wb, R * N, g.wA == g.wB, zr, zi, sa, sb);
+        else load_window<N1>(g.imgA, g.imgB, elems, off, step, wa, wb, R * N, g.wA == g.wB, zr, zi, sa, sb);
     } else {
 #pragma unroll
         for (int k = 0; k < N1; ++k) { zr[k] = 0.0f; zi[k] = 0.0f; }
@@ -1406,8 +1419,8 @@ __device__ __forceinline__ void xcorr_half1_body(const XcorrArgs &g, const int b
         const size_t plane = (size_t)g.H * g.W, elems = (size_t)(g.total / ((long long)g.nWy * g.nWx)) * plane;
         const size_t off = (size_t)n * plane + (size_t)row0 * g.W + col0 + l, step = (size_t)g.W;
         const float *wa = g.wA + l, *wb = g.wB + l;
-        if (g.imgA16) load_window<N>(g.imgA16, g.imgB16, elems, off, step, wa, wb, N, zr, zi, sa, sb);
-        else load_window<N>(g.imgA, g.imgB, elems, off, step, wa, wb, N, zr, zi, sa, sb);
+        if (g.imgA16) load_window<N>(g.imgA16, g.imgB16, elems, off, step, wa, wb, N, g.wA == g.wB, zr, zi, sa, sb);
+        else load_window<N>(g.imgA, g.imgB, elems, off, step, wa, wb, N, g.wA == g.wB, zr, zi, sa, sb);
     } else {
 #pragma unroll
         for (int k = 0; k < N; ++k) { zr[k] = 0.0f; zi[k] = 0.0f; }
