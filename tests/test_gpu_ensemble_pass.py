@@ -327,3 +327,48 @@ def test_multipass_with_a_single_pass_matches_oracle():
         else:
             assert np.median(d) < 1e-4 and np.mean(d < 2e-3) > 0.95, (np.median(d), np.mean(d < 2e-3))
     assert np.nanmedian(np.abs(res[1].ux.cpu().numpy())) > 0.5
+
+
+def test_single_run_type_with_mask_and_two_single_passes():
+    """Masked windows of a single-mode pass are zero with reason -1 and the
+    rest equal the oracle chain with the same mask; two single passes
+    (16 then 8 in one 32 sum window) chain like standard ones."""
+    from pivtools_amd.ensemble import EnsembleMultipass
+    from pivtools_amd.window import compute_window_centers_single_mode
+
+    shape, m, S_ = (96, 112), (8, 8), (32, 32)
+    A, B = _pairs(6, shape, seed0=1900, log_scale=1.0)
+    wp = compute_window_centers_single_mode(shape, m, S_, 50)
+    mask = np.zeros(wp.n_win_y * wp.n_win_x, bool)
+    mask[::7] = True
+    print("single run type with a mask", flush=True)
+    ens = EnsemblePIV(shape, m, 50, kspace_floor="flat", run_type="single", sum_window=S_, mask=mask)
+    ens.accumulate(torch.as_tensor(A, device=dev), torch.as_tensor(B, device=dev))
+    res = ens.finalize()
+    torch.cuda.synchronize()
+    wa = window_weight(m, "singlepix", S_)
+    wb = window_weight(S_, "bsingle", S_)
+    area = S_[0] * S_[1] / (m[0] * m[1])
+    ref = EO.ensemble_pass(EO.pad_single(A, wp.padding), EO.pad_single(B, wp.padding), wp.win_ctrs_x,
+                           wp.win_ctrs_y, S_, wb, mask=mask, weight_a=wa, ab_scale=area, peak_scale=np.sqrt(area))
+    reason = res.nan_reason.cpu().numpy().ravel()
+    assert np.all(reason[mask] == -1) and np.all(res.ux.cpu().numpy().ravel()[mask] == 0.0)
+    for dp, rp in zip(res.planes, ref["planes"]):
+        assert _rel(dp.cpu().numpy()[~mask], rp[~mask]) < 1e-5
+    np.testing.assert_array_equal(reason[mask], ref["nan_reason"].ravel()[mask])
+
+    print("multipass single -> single", flush=True)
+    mp = EnsembleMultipass(shape, [(16, 16), (8, 8)], [50, 50], run_types=["single", "single"], sum_window=S_,
+                           mid_infill="nearest", final_infill="none")
+    out = mp.run(torch.as_tensor(A, device=dev), torch.as_tensor(B, device=dev), batch=3)
+    torch.cuda.synchronize()
+    oref = EO.ensemble_multipass(A, B, mp.grids, floor_fn=_floor_fn(mp), mid_infill="nearest",
+                                 final_infill="none", run_types=["single", "single"], sum_window=S_)
+    for p, (r, o) in enumerate(zip(out, oref)):
+        st, ost = r.status.cpu().numpy(), o["fit"]["status"]
+        assert np.mean(st != ost) < 0.02, p
+        ux, uy = r.ux.cpu().numpy(), r.uy.cpu().numpy()
+        both = np.isfinite(ux) & np.isfinite(o["ux"]) & (st.reshape(ux.shape) == ost.reshape(ux.shape))
+        assert both.mean() > 0.8
+        d = np.hypot(ux[both] - o["ux"][both], uy[both] - o["uy"][both])
+        assert np.median(d) < 1e-4 and np.mean(d < 2e-3) > 0.95, (p, np.median(d), np.mean(d < 2e-3))
