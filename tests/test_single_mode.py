@@ -215,3 +215,13 @@ def test_oracle_single_chain_recovers_a_uniform_shift(fit):
     assert r["ux"].shape == (wc.n_win_y, wc.n_win_x)
     ph = r["peakheight"][ok]
     assert np.all((ph >= 0) & (ph <= 1)) and np.median(ph) > 0.3
+
+
+def test_pad_frames_refuses_bad_geometry_before_any_launch():
+    from pivtools_amd import _native
+    L = _native.core()
+    p = 16
+    assert L.pivamd_pad_frames(p, 0, 1, 8, 8, 2, 2, 9, 12, p, None) == 1     # Hp < H + top
+    assert L.pivamd_pad_frames(p, 1, 1, 8, 8, -1, 2, 12, 12, p, None) == 1   # negative border
+    assert L.pivamd_pad_frames(None, 0, 1, 8, 8, 2, 2, 12, 12, p, None) == 1
+    assert L.pivamd_pad_frames(p, 0, 0, 8, 8, 2, 2, 12, 12, p, None) == 0    # no frames

@@ -299,3 +299,17 @@ def test_lzw_refuses_old_style_and_bad_codes():
     bits += "0" * (-len(bits) % 8)
     bad = np.frombuffer(bytes(int(bits[i:i + 8], 2) for i in range(0, len(bits), 8)), np.uint8)
     assert L.pivamd_tiff_lzw(bad.ctypes.data, bad.size, dst.ctypes.data, 16) == -1
+
+
+def test_gray_conversion_refuses_bad_arguments_before_any_launch():
+    """Argument checks of pivamd_gray_from_samples return before touching a
+    device, so they run here: luma of a type OpenCV does not convert, and
+    empty or oversized batches."""
+    L = _native.core()
+    p = 16                                              # any non-null address; nothing is read
+    assert L.pivamd_gray_from_samples(p, 1, 4, 4, 3, 0, 3, p, None) == 1       # int16 RGB
+    assert L.pivamd_gray_from_samples(p, 1, 4, 4, 4, 1, 5, p, None) == 1       # int32 RGBA
+    assert L.pivamd_gray_from_samples(p, 1, 0, 4, 1, 0, 0, p, None) == 1       # empty frame
+    assert L.pivamd_gray_from_samples(p, 70000, 4, 4, 1, 0, 0, p, None) == 1   # past the grid's y limit
+    assert L.pivamd_gray_from_samples(p, 1, 4, 4, 1, 0, 9, p, None) == 1       # unknown sample type
+    assert L.pivamd_gray_from_samples(p, 0, 4, 4, 1, 0, 0, p, None) == 0       # nothing to do
