@@ -186,10 +186,11 @@ def f_ens(n):
     return 10 * n * n * math.log2(n) + 12 * n * n + 18 * n * (n // 2 + 1)
 
 
-def run_ensemble(args):
-    """Ensemble workload: one 2000-pair pass at 1024 x 1024, 32 x 32 windows
-    at 75 % overlap, correlation background, coloured floor, k-space fit."""
-    import numpy as np
+def ensemble_measure(args):
+    """One timed ens_pairs-pair ensemble pass (1024 x 1024, 32 x 32 windows
+    at 75 % overlap, correlation background, coloured floor, k-space fit)
+    after a warmup pass: (ens, result, wall s, accumulate ms, finalize ms,
+    distinct pairs, device pools)."""
     import torch
     from pivtools_amd import synthetic as S
     from pivtools_amd.ensemble import EnsemblePIV
@@ -238,15 +239,39 @@ def run_ensemble(args):
     wall = time.perf_counter() - t0
     acc_ms = sum(a.elapsed_time(b) for a, b, _ in ev)
     fin_ms = fin[0].elapsed_time(fin[1])
-    value = args.ens_pairs / wall
-    nwin = ens.nwin
-    flops = f_ens(ENS_WIN[0]) * nwin * args.ens_pairs
-    achieved = flops / (acc_ms * 1e-3) / 1e12
+    return ens, res, wall, acc_ms, fin_ms, nd, (pool_a, pool_b)
+
+
+def ensemble_fit_stats(res):
+    import numpy as np
     ok = res.nan_reason.cpu().numpy() == 0
     stats = {"valid_frac": round(float(ok.mean()), 4)}
     if ok.any():
         for k in ("UU", "VV", "UV"):
             stats[f"{k}_median_px2"] = round(float(np.median(getattr(res, k).cpu().numpy()[ok])), 4)
+    return stats
+
+
+def ensemble_leg(args):
+    """The ensemble workload's pass, reported inside the default line."""
+    ens, res, wall, acc_ms, fin_ms, nd, _ = ensemble_measure(args)
+    achieved = f_ens(ENS_WIN[0]) * ens.nwin * args.ens_pairs / (acc_ms * 1e-3) / 1e12
+    return {"workload": "ensemble_1024_w32_o75", "pairs": args.ens_pairs, "pairs_per_s": round(args.ens_pairs / wall, 3),
+            "ms": round(wall * 1e3, 3), "accumulate_ms": round(acc_ms, 3), "finalize_ms": round(fin_ms, 3),
+            "accumulate_tflops": round(achieved, 3), "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+            "fit": ensemble_fit_stats(res)}
+
+
+def run_ensemble(args):
+    """Ensemble workload: one 2000-pair pass at 1024 x 1024, 32 x 32 windows
+    at 75 % overlap, correlation background, coloured floor, k-space fit."""
+    ens, res, wall, acc_ms, fin_ms, nd, (pool_a, pool_b) = ensemble_measure(args)
+    batch = args.pairs
+    value = args.ens_pairs / wall
+    nwin = ens.nwin
+    flops = f_ens(ENS_WIN[0]) * nwin * args.ens_pairs
+    achieved = flops / (acc_ms * 1e-3) / 1e12
+    stats = ensemble_fit_stats(res)
     cpu = None
     if args.cpu_budget > 0:
         cpu = ensemble_cpu_baseline(pool_a, pool_b, ens, res, args.ens_pairs)
@@ -836,6 +861,8 @@ def main():
                     help="multipass workload: offered pairs/s of the serving leg")
     ap.add_argument("--serve-hbm-gb", type=float, default=64.0)
     ap.add_argument("--ens-pairs", type=int, default=2000)
+    ap.add_argument("--ensemble-leg-pairs", type=int, default=2000,
+                    help="multipass workload: pairs of the ensemble pass reported beside value (0 skips it)")
     ap.add_argument("--ens-distinct", type=int, default=64)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one plain 256 MiB device copy (byte-counter calibration for profiles)")
@@ -960,6 +987,13 @@ def main():
     if args.host_fed_steps > 0:
         fed = {f: host_fed(mp, A, B, args.host_fed_steps, f) for f in ("u16", "mono12p")}
         log(f"[bench] host-fed (PCIe-inclusive): {fed}")
+    ens_leg = None
+    if args.ensemble_leg_pairs > 0:
+        log(f"[bench] ensemble leg: one {args.ensemble_leg_pairs}-pair pass")
+        ens_args = argparse.Namespace(**vars(args))
+        ens_args.ens_pairs = args.ensemble_leg_pairs
+        ens_leg = ensemble_leg(ens_args)
+        log(f"[bench] ensemble leg: {ens_leg}")
     cpu = None
     acc = {}
     if args.cpu_budget > 0:
@@ -1000,6 +1034,7 @@ def main():
         "one_pair_fp32": one,
         "serving": serve,
         "host_fed": fed,
+        "ensemble": ens_leg,
         "autotune": ({str(n): r for n, r in tuned.items()} if tuned else None),
     }
     print(json.dumps(line), flush=True)
