@@ -502,15 +502,11 @@ def mixed_accuracy(pairs, finder, log, forms=("fft", "split")):
     return out
 
 
-def run_mixed(args):
-    """Mixed workload: pairs of 1024 x 1024 and 2048 x 2048 uint16 frames
-    (the multipass physics), ragged windows 64 -> 48 -> 32 -> 16 at 50 %
-    overlap, timed with the FP32 FFT (autotuned variants) and with the
-    split-fp16 MFMA DFT, plus FP64 accuracy of both forms."""
-    import numpy as np
+def mixed_groups(args):
+    """The mixed workload's frames: mixed_pairs uint16 pairs split over the
+    image sizes (the multipass physics, seeds 3000+)."""
     import torch
     from pivtools_amd import synthetic as S
-    from pivtools_amd.multipass import MultipassPIV
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
@@ -526,6 +522,75 @@ def run_mixed(args):
             B[k].copy_(b)
         groups.append((shape, A, B))
     torch.cuda.synchronize()
+    return groups
+
+
+def mixed_time(groups, args, dft):
+    """warmup + steps multipasses of every group with one correlation form:
+    (leg dict, last results per group)."""
+    import numpy as np
+    import torch
+    from pivtools_amd.multipass import MultipassPIV
+
+    mps = [MultipassPIV(shape, MIXED_SCHEDULE, [50] * 4, peak_finder=args.finder, dft=dft,
+                        mid_infill=args.infill, final_infill=args.infill, warp_arithmetic=args.warp)
+           for shape, _, _ in groups]
+    log(f"[bench] {dft}: warmup")
+    for _ in range(args.warmup):
+        for mp, (_, A, B) in zip(mps, groups):
+            mp.run(A, B)
+    torch.cuda.synchronize()
+    for mp in mps:
+        mp.check_infill()
+    log(f"[bench] {dft}: timing {args.steps} steps")
+    batch_ev, pass_ev = [], [[] for _ in mps]
+    finals = [None] * len(mps)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        for gi, (mp, (_, A, B)) in enumerate(zip(mps, groups)):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            res = mp.run(A, B, events=pass_ev[gi])
+            e1.record()
+            batch_ev.append((gi, e0, e1))
+            finals[gi] = res[-1]
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    lat = []
+    for gi, e0, e1 in batch_ev:
+        lat += [e0.elapsed_time(e1)] * groups[gi][1].shape[0]
+    passes = []
+    for gi, mp in enumerate(mps):
+        per_pass = {}
+        for p, (a, b) in pass_ev[gi]:
+            per_pass.setdefault(p, []).append(a.elapsed_time(b))
+        for p, g in enumerate(mp.grids):
+            ms = float(np.mean(per_pass[p]))
+            nwin = groups[gi][1].shape[0] * g.n_win[0] * g.n_win[1]
+            passes.append({"image": f"{groups[gi][0][0]}x{groups[gi][0][1]}", "window": g.win[0],
+                           "windows": nwin, "ms": round(ms, 4),
+                           "tflops": round(f_corr(g.win[0]) * nwin / (ms * 1e-3) / 1e12, 3)})
+    total = sum(int(A.shape[0]) for _, A, _ in groups) * args.steps
+    leg = {"pairs_per_s": round(total / wall, 3), "ms_per_step": round(wall * 1e3 / args.steps, 3),
+           "latency_ms_p50": round(float(np.percentile(lat, 50)), 3),
+           "latency_ms_p99": round(float(np.percentile(lat, 99)), 3), "passes": passes}
+    log(f"[bench] {dft}: {leg['pairs_per_s']} pairs/s, {leg['ms_per_step']} ms/step")
+    del mps
+    torch.cuda.empty_cache()
+    return leg, finals
+
+
+def run_mixed(args):
+    """Mixed workload: pairs of 1024 x 1024 and 2048 x 2048 uint16 frames
+    (the multipass physics), ragged windows 64 -> 48 -> 32 -> 16 at 50 %
+    overlap, timed with the FP32 FFT (autotuned variants) and with the
+    split-fp16 MFMA DFT, plus FP64 accuracy of both forms."""
+    import numpy as np
+    import torch
+    from pivtools_amd.multipass import MultipassPIV
+
+    groups = mixed_groups(args)
     tuned = None
     if args.autotune:
         from pivtools_amd import autotune
@@ -536,52 +601,7 @@ def run_mixed(args):
     if "fft" not in args.mixed_dft:
         args.mixed_dft = ["fft"] + list(args.mixed_dft)
     for dft in args.mixed_dft:
-        mps = [MultipassPIV(shape, MIXED_SCHEDULE, [50] * 4, peak_finder=args.finder, dft=dft,
-                            mid_infill=args.infill, final_infill=args.infill, warp_arithmetic=args.warp)
-               for shape, _, _ in groups]
-        log(f"[bench] {dft}: warmup")
-        for _ in range(args.warmup):
-            for mp, (_, A, B) in zip(mps, groups):
-                mp.run(A, B)
-        torch.cuda.synchronize()
-        for mp in mps:
-            mp.check_infill()
-        log(f"[bench] {dft}: timing {args.steps} steps")
-        batch_ev, pass_ev = [], [[] for _ in mps]
-        finals[dft] = [None] * len(mps)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            for gi, (mp, (_, A, B)) in enumerate(zip(mps, groups)):
-                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-                e0.record()
-                res = mp.run(A, B, events=pass_ev[gi])
-                e1.record()
-                batch_ev.append((gi, e0, e1))
-                finals[dft][gi] = res[-1]
-        torch.cuda.synchronize()
-        wall = time.perf_counter() - t0
-        lat = []
-        for gi, e0, e1 in batch_ev:
-            lat += [e0.elapsed_time(e1)] * groups[gi][1].shape[0]
-        passes = []
-        for gi, mp in enumerate(mps):
-            per_pass = {}
-            for p, (a, b) in pass_ev[gi]:
-                per_pass.setdefault(p, []).append(a.elapsed_time(b))
-            for p, g in enumerate(mp.grids):
-                ms = float(np.mean(per_pass[p]))
-                nwin = groups[gi][1].shape[0] * g.n_win[0] * g.n_win[1]
-                passes.append({"image": f"{groups[gi][0][0]}x{groups[gi][0][1]}", "window": g.win[0],
-                               "windows": nwin, "ms": round(ms, 4),
-                               "tflops": round(f_corr(g.win[0]) * nwin / (ms * 1e-3) / 1e12, 3)})
-        total = args.mixed_pairs * args.steps
-        legs[dft] = {"pairs_per_s": round(total / wall, 3), "ms_per_step": round(wall * 1e3 / args.steps, 3),
-                     "latency_ms_p50": round(float(np.percentile(lat, 50)), 3),
-                     "latency_ms_p99": round(float(np.percentile(lat, 99)), 3), "passes": passes}
-        log(f"[bench] {dft}: {legs[dft]['pairs_per_s']} pairs/s, {legs[dft]['ms_per_step']} ms/step")
-        del mps
-        torch.cuda.empty_cache()
+        legs[dft], finals[dft] = mixed_time(groups, args, dft)
 
     # both image sizes through one ragged correlation launch per pass
     from pivtools_amd.multipass import run_ragged
@@ -861,6 +881,8 @@ def main():
                     help="multipass workload: offered pairs/s of the serving leg")
     ap.add_argument("--serve-hbm-gb", type=float, default=64.0)
     ap.add_argument("--ens-pairs", type=int, default=2000)
+    ap.add_argument("--mixed-leg-steps", type=int, default=3,
+                    help="multipass workload: timed steps of the mixed-size leg reported beside value (0 skips it)")
     ap.add_argument("--ensemble-leg-pairs", type=int, default=2000,
                     help="multipass workload: pairs of the ensemble pass reported beside value (0 skips it)")
     ap.add_argument("--ens-distinct", type=int, default=64)
@@ -994,6 +1016,18 @@ def main():
         ens_args.ens_pairs = args.ensemble_leg_pairs
         ens_leg = ensemble_leg(ens_args)
         log(f"[bench] ensemble leg: {ens_leg}")
+    mixed_leg = None
+    if args.mixed_leg_steps > 0:
+        log(f"[bench] mixed leg: {args.mixed_pairs} pairs over {MIXED_SHAPES}, windows "
+            f"{[w[0] for w in MIXED_SCHEDULE]}, FP32 FFT")
+        mx_args = argparse.Namespace(**vars(args))
+        mx_args.steps, mx_args.warmup = args.mixed_leg_steps, 1
+        groups = mixed_groups(mx_args)
+        mixed_leg, _ = mixed_time(groups, mx_args, "fft")
+        mixed_leg = {"workload": "mixed_1024_2048_u16_w64-16", "pairs_per_step": args.mixed_pairs,
+                     "steps": args.mixed_leg_steps, **mixed_leg}
+        del groups
+        torch.cuda.empty_cache()
     cpu = None
     acc = {}
     if args.cpu_budget > 0:
@@ -1035,6 +1069,7 @@ def main():
         "serving": serve,
         "host_fed": fed,
         "ensemble": ens_leg,
+        "mixed": mixed_leg,
         "autotune": ({str(n): r for n, r in tuned.items()} if tuned else None),
     }
     print(json.dumps(line), flush=True)
