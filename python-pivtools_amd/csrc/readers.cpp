@@ -194,7 +194,6 @@ extern "C" long long pivamd_tiff_lzw(const uint8_t *src, size_t n, uint8_t *dst,
         int out_code;
         uint8_t head;
         if (code < next) {
-            if (code >= 256 && code < 258) return -1;
             out_code = code;
             head = first[code];
         } else if (code == next && prev >= 0) {
