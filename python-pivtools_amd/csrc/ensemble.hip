@@ -855,6 +855,7 @@ extern "C" int pivamd_pad_frames(const void *d_src, int src_u16, int n, int H, i
 {
     if (n < 0 || H <= 0 || W <= 0 || pad_top < 0 || pad_left < 0 || Hp < H + pad_top || Wp < W + pad_left)
         return PIVAMD_ERR_ARG;
+    if ((size_t)Hp * Wp >= ((size_t)1 << 31)) return PIVAMD_ERR_SIZE;   // in-frame index is 32-bit
     if (n == 0) return PIVAMD_OK;
     if (!d_src || !d_dst) return PIVAMD_ERR_ARG;
     const size_t total = (size_t)n * Hp * Wp;
