@@ -530,7 +530,8 @@ int pivamd_ensemble_extract(const double *d_gauss, const int *d_status, const un
                             float *d_uu, float *d_vv, float *d_uv, float *d_peak, int *d_reason, void *stream);
 /* Single run type image padding (window_utils.py:447-520, apply_single_mode_padding
  * with pad_value 0): n frames of H x W (16-bit counts when src_u16, else
- * float32) into zero-bordered float32 planes of Hp x Wp at (pad_top, pad_left). */
+ * float32) into zero-bordered float32 planes of Hp x Wp at (pad_top, pad_left);
+ * 2 (size) for padded frames of 2^31 pixels or more. */
 int pivamd_pad_frames(const void *d_src, int src_u16, int n, int H, int W, int pad_top, int pad_left, int Hp,
                       int Wp, float *d_dst, void *stream);
 
