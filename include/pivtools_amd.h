@@ -360,6 +360,15 @@ long long pivamd_packbits(const unsigned char *src, unsigned long n, unsigned ch
  * conversion is pivamd_gray_from_samples, as for TIFF. */
 int pivamd_png_unfilter(const unsigned char *src, unsigned long rows, unsigned long rowbytes, int bpp,
                         unsigned char *dst);
+/* JPEG frames (generic_readers.py:125-176, read_png_jpeg for JPEG, which
+ * decodes with libjpeg-turbo's defaults): baseline and extended sequential
+ * Huffman 8-bit files, host memory.  pivamd_jpeg_info gives width, height and
+ * channels (1 gray, 3 RGB) in info[0..2]; pivamd_jpeg_decode writes height x
+ * width x channels bytes.  Status 0, or 1 malformed, 2 progressive, 3
+ * arithmetic coding, 4 lossless / hierarchical, 5 12-bit, 6 two or four
+ * components, 7 unsupported sampling factors, 8 out too small. */
+int pivamd_jpeg_info(const unsigned char *data, unsigned long n, int *info);
+int pivamd_jpeg_decode(const unsigned char *data, unsigned long n, unsigned char *out, unsigned long cap);
 int pivamd_gray_from_samples(const void *d_src, int n_frames, int H, int W, int S, int planar, int sample_type,
                              float *d_out, void *stream);
 

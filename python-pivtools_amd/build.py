@@ -36,7 +36,8 @@ XCORR_SIZES = (8, 12, 16, 24, 32, 48, 64, 96, 128)
 KERNEL_TUS = [("warp", "warp.hip", [])]
 KERNEL_TUS += [("xcorr_common", "xcorr.hip", ["-DPIVAMD_XCORR_COMMON"])]
 KERNEL_TUS += [(f"xcorr_{n}", "xcorr.hip", [f"-DPIVAMD_XCORR_SIZE={n}"]) for n in XCORR_SIZES]
-EXTRA_TUS = ["ensemble.hip", "glue.hip", "infill.hip", "ingest.hip", "kspace.hip", "outliers.hip", "readers.cpp"]
+EXTRA_TUS = ["ensemble.hip", "glue.hip", "infill.hip", "ingest.hip", "kspace.hip", "outliers.hip", "readers.cpp",
+             "jpeg.cpp"]
 
 
 def _run_parallel(jobs):

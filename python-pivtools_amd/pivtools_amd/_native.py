@@ -124,6 +124,10 @@ def core() -> ctypes.CDLL:
             getattr(L, f).restype = ctypes.c_longlong
         L.pivamd_png_unfilter.argtypes = [_vp, _sz, _sz, _i, _vp]
         L.pivamd_png_unfilter.restype = _i
+        L.pivamd_jpeg_info.argtypes = [_vp, _sz, _vp]
+        L.pivamd_jpeg_info.restype = _i
+        L.pivamd_jpeg_decode.argtypes = [_vp, _sz, _vp, _sz]
+        L.pivamd_jpeg_decode.restype = _i
         L.pivamd_warp_set_variant.argtypes = [_i]
         L.pivamd_warp_set_variant.restype = _i
         L.pivamd_warp_get_variant.argtypes = []
