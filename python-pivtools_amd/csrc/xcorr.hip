@@ -269,6 +269,18 @@ struct Cand {
         p2 = n2;
         nan |= o.nan;
     }
+    // the partner given by a DPP lane pattern within a row of 16 lanes
+    template <int CTRL>
+    __device__ Cand dpp() const
+    {
+        Cand o;
+        o.v = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, true));
+        o.c = __builtin_amdgcn_mov_dpp(c, CTRL, 0xF, 0xF, true);
+        o.p1 = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(p1), CTRL, 0xF, 0xF, true));
+        o.p2 = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(p2), CTRL, 0xF, 0xF, true));
+        o.nan = __builtin_amdgcn_mov_dpp(nan, CTRL, 0xF, 0xF, true);
+        return o;
+    }
     __device__ Cand shfl(int m, int w) const
     {
         Cand o;
@@ -573,7 +585,42 @@ __device__ __forceinline__ void plane_epilogue(const XcorrArgs &g, const float *
     }
     constexpr int CW = N / 2, NC = (N / 2) * (N / 2);
     Cand cd = {0.0f, 0x7fffffff, 0.0f, 0.0f, 0};
-    if (valid) {
+    constexpr int RPL = NC / T;
+    if constexpr (T % CW == 0 && NC % T == 0 && RPL >= 1) {
+        if (valid) {
+            // Lane t scans RPL consecutive rows of one column.  The 3x3 maxima
+            // come from row-wise 3-maxima of RPL + 2 rows (3 (RPL + 2) reads in
+            // place of 9 RPL); including the centre changes nothing, since the
+            // test is "no neighbour above v" and fmaxf skips NaN as before.  The
+            // visiting order stays increasing c within the lane.
+            const int jc = t % CW, ib = (t / CW) * RPL;
+            const float *col = P + (N / 4 + ib - 1) * PX + N / 4 + jc;
+            float hm[RPL + 2], cen[RPL];
+#pragma unroll
+            for (int r = 0; r < RPL + 2; ++r) {
+                const float *p = col + r * PX;
+                const float b = p[0];
+                hm[r] = fmaxf(fmaxf(p[-1], b), p[1]);
+                if (r >= 1 && r <= RPL) cen[r - 1] = b;
+            }
+#pragma unroll
+            for (int k = 0; k < RPL; ++k) {
+                const float v = cen[k];
+                const float m = fmaxf(fmaxf(hm[k], hm[k + 1]), hm[k + 2]);
+                const int c = (ib + k) * CW + jc;
+                const bool vnan = v != v;
+                cd.nan = cd.nan || vnan;
+                const bool cand = !vnan && v > 0.0f && !(m > v);
+                const bool best = cand && v > cd.v;
+                cd.v = best ? v : cd.v;
+                cd.c = best ? c : cd.c;
+                const bool g1 = cand && v > cd.p1;
+                const bool g2 = cand && !g1 && v > cd.p2;
+                cd.p2 = g1 ? cd.p1 : (g2 ? v : cd.p2);
+                cd.p1 = g1 ? v : cd.p1;
+            }
+        }
+    } else if (valid) {
         // Branch-free and unrolled, so the nine LDS reads of every candidate
         // issue together.  A neighbour counts only if it is > v; fmaxf skips
         // NaN neighbours, which a > test never counts either.  Candidates are
@@ -603,11 +650,28 @@ __device__ __forceinline__ void plane_epilogue(const XcorrArgs &g, const float *
         }
     }
     if constexpr (64 % T == 0) {
+        // Within a row of 16 lanes the partners come by DPP (quad xor 1 and 2,
+        // then the half-row and row mirrors, each pairing two halves already
+        // reduced); only wider windows take LDS-routed shuffles.  The merge is
+        // order-free (larger value, then smaller index; top two of the multiset),
+        // so any pairing that covers the window gives the same summary.
 #pragma unroll
-        for (int o = 1; o < T; o <<= 1) cd.merge(cd.shfl(o, T));
+        for (int o = 1; o < T; o <<= 1) {
+            if (o == 1) cd.merge(cd.template dpp<0xB1>());
+            else if (o == 2) cd.merge(cd.template dpp<0x4E>());
+            else if (o == 4) cd.merge(cd.template dpp<0x141>());
+            else if (o == 8) cd.merge(cd.template dpp<0x140>());
+            else cd.merge(cd.shfl(o, T));
+        }
     } else if constexpr (T % 64 == 0) {
 #pragma unroll
-        for (int o = 1; o < 64; o <<= 1) cd.merge(cd.shfl(o, 64));
+        for (int o = 1; o < 64; o <<= 1) {
+            if (o == 1) cd.merge(cd.template dpp<0xB1>());
+            else if (o == 2) cd.merge(cd.template dpp<0x4E>());
+            else if (o == 4) cd.merge(cd.template dpp<0x141>());
+            else if (o == 8) cd.merge(cd.template dpp<0x140>());
+            else cd.merge(cd.shfl(o, 64));
+        }
         if ((t & 63) == 0) {
             r0[t >> 6] = cd.v;
             r1[t >> 6] = __int_as_float(cd.c);
