@@ -34,10 +34,10 @@ HBM_TBS = 8.0
 # 2160x2560 workload, from rocprofv3 FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0)
 # counters calibrated on a copy kernel of known size; recorded in
 # profiles/hbm_counters_multipass_2160x2560_u16.md.
-MEASURED_TRAFFIC_64_PAIRS = {128: 1.432e9 + 0.465e9 + 0.010e9 + 0.002e9,   # reads the 16-bit frames; writes are register spills
-                             64: 2.811e9 + 0.044e9 + 0.043e9 + 0.008e9,
-                             32: 2.832e9 + 0.180e9 + 0.175e9 + 0.033e9,
-                             16: 2.831e9 + 0.725e9 + 0.703e9 + 0.132e9}
+MEASURED_TRAFFIC_64_PAIRS = {128: 1.430e9 + 0.449e9 + 0.003e9 + 0.002e9,   # reads the 16-bit frames; writes are register spills
+                             64: 2.811e9 + 0.012e9 + 0.011e9 + 0.008e9,
+                             32: 2.832e9 + 0.049e9 + 0.044e9 + 0.033e9,
+                             16: 2.832e9 + 0.198e9 + 0.176e9 + 0.132e9}
 
 
 def log(*a):

@@ -521,6 +521,17 @@ __device__ void search_and_fit(float *P, int NY, int NX, int PX, int pwx, const 
 // Per-window peak record written by xcorr_kernel and consumed by fit_kernel.
 constexpr int kRec = 32;
 constexpr int kRecH = 25, kRecI0 = 26, kRecJ0 = 27, kRecEnorm = 28, kRecFlag = 29;
+// The 3-point fits read only the cross through the peak, so with one peak and
+// a 3-point finder the record is 8 words at an 8-word stride: the weighted
+// plane at (-1, 0), (0, -1), (0, 0), (0, 1), (1, 0) from the peak, the height,
+// the energy norm, and the peak position as the integer i0 * 256 + j0 (-1: no
+// fit).  A quarter of the bytes through HBM, written and read.
+constexpr int kRecC = 8;
+constexpr int kRecCH = 5, kRecCEnorm = 6, kRecCPos = 7;
+__host__ __device__ constexpr bool rec_compact(int finder, int nPeaks)
+{
+    return nPeaks == 1 && !(finder >= 4 && finder <= 6);
+}
 
 // Lane-order reduction of the per-lane candidate summaries (largest 3x3
 // local maximum, first in row-major order on ties; top-2 for the
@@ -701,9 +712,13 @@ __device__ __forceinline__ void plane_epilogue(const XcorrArgs &g, const float *
         }
     }
     if (gw >= g.total) return;
-    float *rec = g.rec + (size_t)gw * kRec;
+    const bool compact = rec_compact(g.finder, g.nPeaks);    // uniform: kernel arguments
+    float *rec = g.rec + (size_t)gw * (compact ? kRecC : kRec);
     if (!valid) {
-        if (t == 0) rec[kRecFlag] = 0.0f;
+        if (t == 0) {
+            if (compact) rec[kRecCPos] = __int_as_float(-1);
+            else rec[kRecFlag] = 0.0f;
+        }
         return;
     }
     float h = cd.v, det = (cd.p2 > 0.0f) ? cd.p1 / cd.p2 : 0.0f;
@@ -724,12 +739,29 @@ __device__ __forceinline__ void plane_epilogue(const XcorrArgs &g, const float *
             g.pkY[o] = __builtin_nanf("") - N / 2.0f;
             g.sx[o] = 0.0f; g.sy[o] = 0.0f; g.sxy[o] = 0.0f;
             g.pkH[o] = 0.0f * enorm;
-            rec[kRecFlag] = 0.0f;
+            if (compact) rec[kRecCPos] = __int_as_float(-1);
+            else rec[kRecFlag] = 0.0f;
+        }
+        return;
+    }
+    const int nl = cd.nan ? 1 : T;
+    if (compact) {
+        // the cross through the peak times the plane weight (5x5 patch
+        // entries 7, 11, 12, 13, 17), spread over the window's lanes
+        for (int e = t; e < 5; e += nl) {
+            const int r = i0 + (e == 0 ? -1 : (e == 4 ? 1 : 0)), c = j0 + (e >= 1 && e <= 3 ? e - 2 : 0);
+            const float v = P[r * PX + c];
+            rec[e] = g.pw ? v * g.pw[r * N + c] : v;
+        }
+        if (t == 0) {
+            g.ratio[o] = det;
+            rec[kRecCH] = h;
+            rec[kRecCEnorm] = enorm;
+            rec[kRecCPos] = __int_as_float(i0 * 256 + j0);
         }
         return;
     }
     // the 5x5 patch times the plane weight, spread over the window's lanes
-    const int nl = cd.nan ? 1 : T;
     for (int e = t; e < PK * PK; e += nl) {
         const int r = i0 + e / PK - 2, c = j0 + e % PK - 2;
         const float v = P[r * PX + c];
@@ -1660,6 +1692,21 @@ __global__ void __launch_bounds__(256) fit_kernel(const float *rec_all, long lon
 {
     const long long gw = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (gw >= total) return;
+    if (rec_compact(finder, 1)) {
+        const float *rc = rec_all + (size_t)gw * kRecC;
+        const int pos = __float_as_int(rc[kRecCPos]);
+        if (pos < 0) return;
+        float sub[PK * PK];    // threept reads entries 7, 11, 12, 13 and 17 only
+        sub[7] = rc[0]; sub[11] = rc[1]; sub[12] = rc[2]; sub[13] = rc[3]; sub[17] = rc[4];
+        float pk[2], A, s0, s1;
+        threept(sub, pk, &A, &s0, &s1);
+        const size_t o = (size_t)gw;
+        pkX[o] = (pk[1] + (float)(pos & 255)) - winX / 2.0f;
+        pkY[o] = (pk[0] + (float)(pos >> 8)) - winY / 2.0f;
+        sx[o] = s0; sy[o] = s1; sxy[o] = 0.0f;
+        pkH[o] = rc[kRecCH] * rc[kRecCEnorm];
+        return;
+    }
     const float *rec = rec_all + (size_t)gw * kRec;
     if (rec[kRecFlag] != 1.0f) return;
     float sub[PK * PK];
