@@ -211,7 +211,12 @@ __device__ __forceinline__ void sort_net(float (&w)[K])
 // for an even one, NaN when nothing is left).  NaNs become +inf and sort
 // last; the n values in front are the finite ones (and any real +inf, equal
 // to the stand-ins), so the middle of those is the median.
-template <int K>
+// ROOT: v holds squares of non-negative values; the median of their square
+// roots is returned.  sqrtf is monotone (non-decreasing, NaN to NaN), so the
+// order statistics of the roots are the roots of the order statistics: only
+// the one or two selected values need a root, the same bits as taking every
+// root first.
+template <int K, bool ROOT = false>
 __device__ __forceinline__ float nanmedian_reg(const float (&v)[K])
 {
     float w[K];
@@ -230,6 +235,10 @@ __device__ __forceinline__ float nanmedian_reg(const float (&v)[K])
     for (int i = 1; i < K; ++i) {
         lo = i == klo ? w[i] : lo;
         hi = i == khi ? w[i] : hi;
+    }
+    if constexpr (ROOT) {
+        lo = sqrtf(lo);
+        hi = sqrtf(hi);
     }
     return (n & 1) ? lo : 0.5f * (lo + hi);
 }
@@ -264,13 +273,13 @@ __device__ __forceinline__ bool median_outlier(const float *uy, const float *ux,
     const float x0 = ux[r * nx + c], y0 = uy[r * nx + c];
     const float ddx = x0 - mu, ddy = y0 - mv;
     const float r0 = sqrtf(ddx * ddx + ddy * ddy);
-    float ri[K];
+    float ri2[K];    // squared residual norms; the median takes the roots
 #pragma unroll
     for (int i = 0; i < K; ++i) {
         const float ex = vx[i] - mu, ey = vy[i] - mv;
-        ri[i] = sqrtf(ex * ex + ey * ey);
+        ri2[i] = ex * ex + ey * ey;
     }
-    const float rm = nanmedian_reg<K>(ri);
+    const float rm = nanmedian_reg<K, true>(ri2);
     const float nr = r0 / (rm + eps);
     const bool exceeded = (nr > thr) || !isfinite(nr);
     int present = 0;
